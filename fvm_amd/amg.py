@@ -1,0 +1,255 @@
+"""Python mirror of the reference ``AMG`` / ``LinearSolver`` (and a flexible GMRES).
+
+Attribute names, enum values and defaults follow
+src/fvm/src/modules/fvmbase/AMG.h:27-110, AMG.cpp:15-36 and LinearSolver.h:15-35,
+so driver scripts written against ``fvmbaseExt.AMG()`` read the same:
+
+    solver = AMG()
+    solver.relativeTolerance = 1e-9
+    solver.nMaxIterations = 200000
+    rnorm0 = solver.solve(ls)        # ls.delta receives the correction
+    solver.cleanup()
+
+Every call goes through the C ABI of libfvm_amd.so; errors raise ``FvmError``
+(the reference raises ``CException``, seen from Python as RuntimeError).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import AmgParams, FgmresParams, check, lib
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class AMG:
+    # AMG::CycleType / AMG::SmootherType (AMG.h:31-42)
+    V_CYCLE, W_CYCLE, F_CYCLE = 0, 1, 2
+    GAUSS_SEIDEL, JACOBI = 0, 1
+    # extensions
+    GS_EXACT, GS_MULTICOLOR = 0, 1
+    PREC_FP64, PREC_MIXED = 0, 1
+
+    _FIELDS = [f for f, _ in AmgParams._fields_]
+
+    def __init__(self, device=0):
+        L = lib()
+        h = ctypes.c_void_p()
+        check(L.fvm_amg_create(ctypes.byref(h), int(device)))
+        self._h = h
+        p = AmgParams()
+        check(L.fvm_amg_default_params(ctypes.byref(p)))
+        object.__setattr__(self, "_p", p)
+        self._finest = None
+        self._print_file = None
+
+    # parameter attributes map onto the C struct
+    def __getattr__(self, name):
+        if name in AMG._FIELDS:
+            return getattr(self._p, name)
+        raise AttributeError(name)
+
+    def __setattr__(self, name, value):
+        if name in AMG._FIELDS:
+            setattr(self._p, name, value)
+        else:
+            object.__setattr__(self, name, value)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().fvm_amg_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def _push_params(self):
+        check(lib().fvm_amg_set_params(self._h, ctypes.byref(self._p)))
+
+    # ------------------------------------------------------------------ setup
+    def setup(self, ls):
+        """Build the hierarchy for ``ls`` (the first-call branch of AMG::solve)."""
+        self.isSymmetric = int(ls.isSymmetric)
+        self._push_params()
+        check(lib().fvm_amg_setup(self._h, ls.n_self, ls.n_total, _ptr(ls.row), _ptr(ls.col),
+                                  _ptr(ls.diag), _ptr(ls.off), _ptr(ls.is_boundary)))
+        self._finest = ls
+
+    def setup_device(self, n_self, n_total, row, col, diag, off, is_boundary=None, isSymmetric=False):
+        """Build from device tensors (torch, on this handle's device)."""
+        self.isSymmetric = int(isSymmetric)
+        self._push_params()
+        dp = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+        check(lib().fvm_amg_setup_device(self._h, int(n_self), int(n_total), dp(row), dp(col), dp(diag),
+                                         dp(off), dp(is_boundary)))
+        self._finest = "device"
+
+    # ------------------------------------------------------------------ LinearSolver API
+    def solve(self, ls):
+        """AMG::solve (AMG.cpp:219-282). Returns the initial residual 1-norm."""
+        if self._finest is not ls:
+            self.setup(ls)
+        self._push_params()
+        r0 = ctypes.c_double()
+        rf = ctypes.c_double()
+        cyc = ctypes.c_int()
+        check(lib().fvm_amg_solve(self._h, _ptr(ls.b), _ptr(ls.delta), ctypes.byref(r0), ctypes.byref(rf),
+                                  ctypes.byref(cyc)))
+        self.lastCycles = cyc.value
+        self.lastResidual = rf.value
+        self._emit_history()
+        return r0.value
+
+    def solve_device(self, b, x):
+        """Solve on device tensors (delta form); returns (r0, r_final, cycles)."""
+        self._push_params()
+        r0 = ctypes.c_double()
+        rf = ctypes.c_double()
+        cyc = ctypes.c_int()
+        check(lib().fvm_amg_solve_device(self._h, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                         ctypes.byref(r0), ctypes.byref(rf), ctypes.byref(cyc)))
+        return r0.value, rf.value, cyc.value
+
+    def smooth(self, ls):
+        """AMG::smooth (AMG.cpp:285-298): one cycle on ls.delta."""
+        if self._finest is not ls:
+            self.setup(ls)
+        self._push_params()
+        check(lib().fvm_amg_smooth(self._h, _ptr(ls.b), _ptr(ls.delta)))
+
+    def cleanup(self):
+        """AMG::cleanup (AMG.cpp:212-217)."""
+        check(lib().fvm_amg_cleanup(self._h))
+        self._finest = None
+
+    def getTotalIterations(self):
+        return int(lib().fvm_amg_total_iterations(self._h))
+
+    def setMergeLevelSize(self, size):
+        """Merging coarse levels onto one MPI rank (AMG.h:53-65) is meaningless on a
+        single device; the value is recorded and has no effect."""
+        self.mergeLevelSize = int(size)
+
+    def redirectPrintToFile(self, fname):
+        """AMG.cpp:323-330: convergence lines go to ``fname`` instead of stdout."""
+        self._print_file = open(fname, "w")
+
+    def redirectPrintToScreen(self):
+        if self._print_file:
+            self._print_file.close()
+        self._print_file = None
+
+    # ------------------------------------------------------------------ introspection
+    def history(self, cap=1 << 20):
+        buf = np.zeros(cap)
+        k = check(lib().fvm_amg_history(self._h, _ptr(buf), cap))
+        return buf[:k].copy()
+
+    def num_levels(self):
+        return lib().fvm_amg_num_levels(self._h)
+
+    def level(self, l):
+        n = ctypes.c_int()
+        nnz = ctypes.c_int()
+        check(lib().fvm_amg_level_info(self._h, l, ctypes.byref(n), ctypes.byref(nnz)))
+        row = np.zeros(n.value + 1, np.int32)
+        col = np.zeros(max(nnz.value, 1), np.int32)
+        diag = np.zeros(max(n.value, 1))
+        off = np.zeros(max(nnz.value, 1))
+        ci = np.zeros(max(n.value, 1), np.int32)
+        check(lib().fvm_amg_level_get(self._h, l, _ptr(row), _ptr(col), _ptr(diag), _ptr(off), _ptr(ci)))
+        return dict(n=n.value, nnz=nnz.value, row=row, col=col[:nnz.value], diag=diag[:n.value],
+                    off=off[:nnz.value], ci=ci[:n.value])
+
+    def setup_stats(self):
+        t = ctypes.c_double()
+        r = ctypes.c_int()
+        check(lib().fvm_amg_setup_stats(self._h, ctypes.byref(t), ctypes.byref(r)))
+        return dict(setup_seconds=t.value, finest_rounds=r.value)
+
+    def _emit_history(self, name="test"):
+        """Print 'i: [name : norm]' lines like AMG.cpp:243-272 when verbosity > 0."""
+        if self.verbosity <= 0 or self._print_file is None:
+            return
+        h = self.history()
+        for i, v in enumerate(h):
+            self._print_file.write("%d: [%s : %s]\n" % (i, name, _fmt6(v)))
+        self._print_file.flush()
+
+
+def _fmt6(v):
+    """C++ ostream default formatting (precision 6, %g-like)."""
+    return "%g" % v
+
+
+class FGMRES:
+    """FP64 flexible GMRES(restart), right-preconditioned by an AMG handle's cycle.
+
+    Plays the role of the reference's Krylov wrappers around ``AMG::smooth``
+    (BCGStab.cpp:27-171): ``preconditioner`` is an AMG whose hierarchy is built.
+    """
+
+    _FIELDS = [f for f, _ in FgmresParams._fields_]
+
+    def __init__(self, preconditioner):
+        L = lib()
+        self.preconditioner = preconditioner
+        h = ctypes.c_void_p()
+        check(L.fvm_fgmres_create(ctypes.byref(h), preconditioner._h))
+        self._h = h
+        p = FgmresParams()
+        check(L.fvm_fgmres_default_params(ctypes.byref(p)))
+        self._p = p
+
+    def __getattr__(self, name):
+        if name in FGMRES._FIELDS:
+            return getattr(self.__dict__["_p"], name)
+        raise AttributeError(name)
+
+    def __setattr__(self, name, value):
+        if name in FGMRES._FIELDS:
+            setattr(self._p, name, value)
+        else:
+            object.__setattr__(self, name, value)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().fvm_fgmres_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def solve(self, ls):
+        pc = self.preconditioner
+        if pc._finest is not ls:
+            pc.setup(ls)
+        pc._push_params()
+        check(lib().fvm_fgmres_set_params(self._h, ctypes.byref(self._p)))
+        r0 = ctypes.c_double()
+        rf = ctypes.c_double()
+        it = ctypes.c_int()
+        check(lib().fvm_fgmres_solve(self._h, _ptr(ls.b), _ptr(ls.delta), ctypes.byref(r0), ctypes.byref(rf),
+                                     ctypes.byref(it)))
+        self.lastIterations = it.value
+        self.lastResidual = rf.value
+        return r0.value
+
+    def solve_device(self, b, x):
+        pc = self.preconditioner
+        pc._push_params()
+        check(lib().fvm_fgmres_set_params(self._h, ctypes.byref(self._p)))
+        r0 = ctypes.c_double()
+        rf = ctypes.c_double()
+        it = ctypes.c_int()
+        check(lib().fvm_fgmres_solve_device(self._h, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                            ctypes.byref(r0), ctypes.byref(rf), ctypes.byref(it)))
+        return r0.value, rf.value, it.value
+
+    def history(self, cap=1 << 16):
+        buf = np.zeros(cap)
+        k = check(lib().fvm_fgmres_history(self._h, _ptr(buf), cap))
+        return buf[:k].copy()

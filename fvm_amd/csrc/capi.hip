@@ -1,0 +1,427 @@
+// C ABI of fvm_amd (declared in include/fvm_amd.h). No exception crosses it:
+// every entry point returns FVM_OK or a negative FVM_E_* code and records the
+// message for fvm_last_error().
+#include "amg.hpp"
+
+#include <algorithm>
+#include <chrono>
+
+namespace fvm {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+struct KrylovWork;
+void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const double *d_b, double *d_x,
+                  double *rnorm0, double *rnorm_final, int *iterations, std::vector<double> &hist);
+
+}  // namespace fvm
+
+using fvm::DArr;
+
+#define FVM_API_BEGIN try {
+#define FVM_API_END                                                    \
+    }                                                                  \
+    catch (const fvm::Error &e) {                                      \
+        fvm::set_last_error(e.what());                                 \
+        return e.code;                                                 \
+    }                                                                  \
+    catch (const std::exception &e) {                                  \
+        fvm::set_last_error(e.what());                                 \
+        return FVM_E_INVALID;                                          \
+    }                                                                  \
+    catch (...) {                                                      \
+        fvm::set_last_error("unknown error");                          \
+        return FVM_E_INVALID;                                          \
+    }
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        FVM_HIP(hipSetDevice(dev));
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+void check_params(const fvm_amg_params *p) {
+    FVM_CHECK(p, FVM_E_INVALID, "null parameter struct");
+    FVM_CHECK(p->maxCoarseLevels >= 0, FVM_E_INVALID, "maxCoarseLevels must be >= 0");
+    FVM_CHECK(p->nPreSweeps >= 0 && p->nPostSweeps >= 0, FVM_E_INVALID, "sweep counts must be >= 0");
+    FVM_CHECK(p->cycleType >= FVM_V_CYCLE && p->cycleType <= FVM_F_CYCLE, FVM_E_INVALID, "bad cycleType");
+    FVM_CHECK(p->smootherType == FVM_GAUSS_SEIDEL || p->smootherType == FVM_JACOBI, FVM_E_INVALID,
+              "bad smootherType");
+    FVM_CHECK(p->gsOrdering == FVM_GS_EXACT || p->gsOrdering == FVM_GS_MULTICOLOR, FVM_E_INVALID,
+              "bad gsOrdering");
+    FVM_CHECK(p->precision == FVM_PREC_FP64 || p->precision == FVM_PREC_MIXED, FVM_E_INVALID,
+              "bad precision");
+    FVM_CHECK(p->coarseGroupSize >= 1, FVM_E_INVALID, "coarseGroupSize must be >= 1");
+}
+
+}  // namespace
+
+struct fvm_fgmres {
+    fvm_amg *pc = nullptr;
+    fvm_fgmres_params prm{};
+    fvm::KrylovWork *work = nullptr;
+    std::vector<double> hist;
+};
+
+namespace fvm {
+struct KrylovWork;
+KrylovWork *new_krylov_work();
+void delete_krylov_work(KrylovWork *w);
+}  // namespace fvm
+
+extern "C" {
+
+int fvm_version(void) { return FVM_AMD_VERSION; }
+
+const char *fvm_last_error(void) { return fvm::g_last_error.c_str(); }
+
+int fvm_amg_default_params(fvm_amg_params *p) {
+    if (!p) return FVM_E_INVALID;
+    // AMG.cpp:15-36 and LinearSolver.h:19-24
+    p->maxCoarseLevels = 30;
+    p->nPreSweeps = 0;
+    p->nPostSweeps = 1;
+    p->coarseGroupSize = 2;
+    p->weightRatioThreshold = 0.65;
+    p->cycleType = FVM_V_CYCLE;
+    p->smootherType = FVM_GAUSS_SEIDEL;
+    p->scaleCorrections = 1;
+    p->nMaxIterations = 100;
+    p->verbosity = 2;
+    p->relativeTolerance = 1e-8;
+    p->absoluteTolerance = 1e-50;
+    p->isSymmetric = 0;
+    p->parallelStopRule = 1;
+    p->gsOrdering = FVM_GS_EXACT;
+    p->precision = FVM_PREC_FP64;
+    return FVM_OK;
+}
+
+int fvm_amg_create(fvm_amg **out, int device) {
+    FVM_API_BEGIN
+    FVM_CHECK(out, FVM_E_INVALID, "null output handle");
+    int ndev = 0;
+    FVM_HIP(hipGetDeviceCount(&ndev));
+    FVM_CHECK(device >= 0 && device < ndev, FVM_E_INVALID, "device ordinal out of range");
+    DeviceGuard g(device);
+    fvm_amg *h = new fvm_amg();
+    h->device = device;
+    fvm_amg_default_params(&h->prm);
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete h;
+        throw fvm::Error(FVM_E_HIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
+    }
+    *out = h;
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_amg_destroy(fvm_amg *h) {
+    FVM_API_BEGIN
+    if (!h) return FVM_OK;
+    {
+        DeviceGuard g(h->device);
+        (void)hipStreamSynchronize(h->stream);
+        h->lev.clear();
+        h->ws = fvm::Workspace();
+        h->dense_inv.release();
+        h->h_stage_b.release();
+        h->h_stage_x.release();
+        (void)hipStreamDestroy(h->stream);
+    }
+    delete h;
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_amg_set_params(fvm_amg *h, const fvm_amg_params *p) {
+    FVM_API_BEGIN
+    FVM_CHECK(h, FVM_E_INVALID, "null handle");
+    check_params(p);
+    const bool rebuild = h->ready && (p->coarseGroupSize != h->prm.coarseGroupSize ||
+                                      p->weightRatioThreshold != h->prm.weightRatioThreshold ||
+                                      p->maxCoarseLevels != h->prm.maxCoarseLevels ||
+                                      p->parallelStopRule != h->prm.parallelStopRule);
+    h->prm = *p;
+    if (rebuild) {
+        // hierarchy parameters changed: the next solve needs a new setup (AMG rebuilds per solve)
+        DeviceGuard g(h->device);
+        h->lev.clear();
+        h->ready = false;
+    }
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_amg_get_params(const fvm_amg *h, fvm_amg_params *p) {
+    if (!h || !p) return FVM_E_INVALID;
+    *p = h->prm;
+    return FVM_OK;
+}
+
+int fvm_amg_setup_device(fvm_amg *h, int n_self, int n_total, const int32_t *d_row, const int32_t *d_col,
+                         const double *d_diag, const double *d_off, const uint8_t *d_is_boundary) {
+    FVM_API_BEGIN
+    FVM_CHECK(h, FVM_E_INVALID, "null handle");
+    FVM_CHECK(n_self >= 0 && n_total >= n_self, FVM_E_INVALID, "need 0 <= n_self <= n_total");
+    FVM_CHECK(d_row && (n_total == 0 || (d_diag)), FVM_E_INVALID, "null matrix array");
+    DeviceGuard g(h->device);
+    fvm::build_hierarchy(h, n_self, n_total, d_row, d_col, d_diag, d_off, d_is_boundary);
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_amg_setup(fvm_amg *h, int n_self, int n_total, const int32_t *row, const int32_t *col,
+                  const double *diag, const double *off, const uint8_t *is_boundary) {
+    FVM_API_BEGIN
+    FVM_CHECK(h, FVM_E_INVALID, "null handle");
+    FVM_CHECK(n_self >= 0 && n_total >= n_self, FVM_E_INVALID, "need 0 <= n_self <= n_total");
+    FVM_CHECK(row && (n_total == 0 || diag), FVM_E_INVALID, "null matrix array");
+    const long long nnz = n_total > 0 ? row[n_total] : 0;
+    FVM_CHECK(nnz >= 0 && row[0] == 0, FVM_E_INVALID, "row pointers must start at 0");
+    FVM_CHECK(nnz == 0 || (col && off), FVM_E_INVALID, "null col/off arrays");
+    DeviceGuard g(h->device);
+    DArr<int> d_row((size_t)n_total + 1), d_col(std::max<long long>(nnz, 1));
+    DArr<double> d_diag(std::max(n_total, 1)), d_off(std::max<long long>(nnz, 1));
+    DArr<uint8_t> d_bnd;
+    hipStream_t s = h->stream;
+    FVM_HIP(hipMemcpyAsync(d_row.p, row, sizeof(int) * ((size_t)n_total + 1), hipMemcpyHostToDevice, s));
+    if (nnz > 0) {
+        FVM_HIP(hipMemcpyAsync(d_col.p, col, sizeof(int) * nnz, hipMemcpyHostToDevice, s));
+        FVM_HIP(hipMemcpyAsync(d_off.p, off, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+    }
+    if (n_total > 0)
+        FVM_HIP(hipMemcpyAsync(d_diag.p, diag, sizeof(double) * n_total, hipMemcpyHostToDevice, s));
+    if (is_boundary && n_total > 0) {
+        d_bnd.alloc(n_total);
+        FVM_HIP(hipMemcpyAsync(d_bnd.p, is_boundary, n_total, hipMemcpyHostToDevice, s));
+    }
+    FVM_HIP(hipStreamSynchronize(s));
+    fvm::build_hierarchy(h, n_self, n_total, d_row.p, d_col.p, d_diag.p, d_off.p, d_bnd.p);
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_amg_solve_device(fvm_amg *h, const double *d_b, double *d_x, double *rnorm0, double *rnorm_final,
+                         int *cycles) {
+    FVM_API_BEGIN
+    FVM_CHECK(h, FVM_E_INVALID, "null handle");
+    FVM_CHECK(h->ready, FVM_E_NOT_SETUP, "solve called before setup");
+    FVM_CHECK(h->n_self == 0 || (d_b && d_x), FVM_E_INVALID, "null vector");
+    DeviceGuard g(h->device);
+    fvm::solve_device(h, d_b, d_x, rnorm0, rnorm_final, cycles);
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_amg_solve(fvm_amg *h, const double *b, double *x, double *rnorm0, double *rnorm_final, int *cycles) {
+    FVM_API_BEGIN
+    FVM_CHECK(h, FVM_E_INVALID, "null handle");
+    FVM_CHECK(h->ready, FVM_E_NOT_SETUP, "solve called before setup");
+    FVM_CHECK(h->n_total == 0 || (b && x), FVM_E_INVALID, "null vector");
+    DeviceGuard g(h->device);
+    const size_t n = std::max(h->n_total, 1);
+    h->h_stage_b.ensure(n);
+    h->h_stage_x.ensure(n);
+    hipStream_t s = h->stream;
+    if (h->n_total > 0) {
+        FVM_HIP(hipMemcpyAsync(h->h_stage_b.p, b, sizeof(double) * h->n_total, hipMemcpyHostToDevice, s));
+        FVM_HIP(hipMemcpyAsync(h->h_stage_x.p, x, sizeof(double) * h->n_total, hipMemcpyHostToDevice, s));
+    }
+    fvm::solve_device(h, h->h_stage_b.p, h->h_stage_x.p, rnorm0, rnorm_final, cycles);
+    if (h->n_self > 0) {
+        FVM_HIP(hipMemcpyAsync(x, h->h_stage_x.p, sizeof(double) * h->n_self, hipMemcpyDeviceToHost, s));
+        FVM_HIP(hipStreamSynchronize(s));
+    }
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_amg_smooth_device(fvm_amg *h, const double *d_b, double *d_x) {
+    FVM_API_BEGIN
+    FVM_CHECK(h, FVM_E_INVALID, "null handle");
+    FVM_CHECK(h->ready, FVM_E_NOT_SETUP, "smooth called before setup");
+    DeviceGuard g(h->device);
+    fvm::smooth_device(h, d_b, d_x);
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_amg_smooth(fvm_amg *h, const double *b, double *x) {
+    FVM_API_BEGIN
+    FVM_CHECK(h, FVM_E_INVALID, "null handle");
+    FVM_CHECK(h->ready, FVM_E_NOT_SETUP, "smooth called before setup");
+    FVM_CHECK(h->n_total == 0 || (b && x), FVM_E_INVALID, "null vector");
+    DeviceGuard g(h->device);
+    const size_t n = std::max(h->n_total, 1);
+    h->h_stage_b.ensure(n);
+    h->h_stage_x.ensure(n);
+    hipStream_t s = h->stream;
+    if (h->n_total > 0) {
+        FVM_HIP(hipMemcpyAsync(h->h_stage_b.p, b, sizeof(double) * h->n_total, hipMemcpyHostToDevice, s));
+        FVM_HIP(hipMemcpyAsync(h->h_stage_x.p, x, sizeof(double) * h->n_total, hipMemcpyHostToDevice, s));
+    }
+    fvm::smooth_device(h, h->h_stage_b.p, h->h_stage_x.p);
+    if (h->n_self > 0) {
+        FVM_HIP(hipMemcpyAsync(x, h->h_stage_x.p, sizeof(double) * h->n_self, hipMemcpyDeviceToHost, s));
+        FVM_HIP(hipStreamSynchronize(s));
+    }
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_amg_cleanup(fvm_amg *h) {
+    FVM_API_BEGIN
+    FVM_CHECK(h, FVM_E_INVALID, "null handle");
+    DeviceGuard g(h->device);
+    FVM_HIP(hipStreamSynchronize(h->stream));
+    h->lev.clear();
+    h->dense_inv.release();
+    h->direct_level = -1;
+    h->ready = false;
+    return FVM_OK;
+    FVM_API_END
+}
+
+long long fvm_amg_total_iterations(const fvm_amg *h) { return h ? h->total_iterations : 0; }
+
+int fvm_amg_history(const fvm_amg *h, double *hist, int cap) {
+    if (!h || (cap > 0 && !hist)) return FVM_E_INVALID;
+    const int k = std::min<int>(cap, (int)h->hist.size());
+    for (int i = 0; i < k; i++) hist[i] = h->hist[i];
+    return k;
+}
+
+int fvm_amg_num_levels(const fvm_amg *h) { return (h && h->ready) ? (int)h->lev.size() : 0; }
+
+int fvm_amg_level_info(const fvm_amg *h, int level, int *n_rows, int *nnz) {
+    if (!h || !h->ready || level < 0 || level >= (int)h->lev.size()) return FVM_E_INVALID;
+    if (n_rows) *n_rows = h->lev[level].n;
+    if (nnz) *nnz = (int)h->lev[level].nnz;
+    return FVM_OK;
+}
+
+int fvm_amg_level_get(const fvm_amg *h, int level, int32_t *row, int32_t *col, double *diag, double *off,
+                      int32_t *ci) {
+    FVM_API_BEGIN
+    FVM_CHECK(h && h->ready && level >= 0 && level < (int)h->lev.size(), FVM_E_INVALID, "bad level");
+    DeviceGuard g(h->device);
+    const fvm::Level &L = h->lev[level];
+    hipStream_t s = h->stream;
+    if (row) FVM_HIP(hipMemcpyAsync(row, L.row.p, sizeof(int) * ((size_t)L.n + 1), hipMemcpyDeviceToHost, s));
+    if (col && L.nnz) FVM_HIP(hipMemcpyAsync(col, L.col.p, sizeof(int) * L.nnz, hipMemcpyDeviceToHost, s));
+    if (diag && L.n) FVM_HIP(hipMemcpyAsync(diag, L.diag.p, sizeof(double) * L.n, hipMemcpyDeviceToHost, s));
+    if (off && L.nnz) FVM_HIP(hipMemcpyAsync(off, L.off.p, sizeof(double) * L.nnz, hipMemcpyDeviceToHost, s));
+    if (ci) {
+        if (L.ci.p && level + 1 < (int)h->lev.size())
+            FVM_HIP(hipMemcpyAsync(ci, L.ci.p, sizeof(int) * L.n, hipMemcpyDeviceToHost, s));
+        else
+            for (int i = 0; i < L.n; i++) ci[i] = -1;
+    }
+    FVM_HIP(hipStreamSynchronize(s));
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_amg_setup_stats(const fvm_amg *h, double *setup_seconds, int *finest_rounds) {
+    if (!h) return FVM_E_INVALID;
+    if (setup_seconds) *setup_seconds = h->setup_seconds;
+    if (finest_rounds) *finest_rounds = h->finest_rounds;
+    return FVM_OK;
+}
+
+// ----------------------------------------------------------------------- FGMRES
+int fvm_fgmres_default_params(fvm_fgmres_params *p) {
+    if (!p) return FVM_E_INVALID;
+    p->restart = 30;
+    p->nMaxIterations = 1000;
+    p->verbosity = 0;
+    p->relativeTolerance = 1e-8;
+    p->absoluteTolerance = 1e-50;
+    return FVM_OK;
+}
+
+int fvm_fgmres_create(fvm_fgmres **out, fvm_amg *pc) {
+    FVM_API_BEGIN
+    FVM_CHECK(out && pc, FVM_E_INVALID, "null argument");
+    fvm_fgmres *s = new fvm_fgmres();
+    s->pc = pc;
+    fvm_fgmres_default_params(&s->prm);
+    s->work = fvm::new_krylov_work();
+    *out = s;
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_fgmres_destroy(fvm_fgmres *s) {
+    FVM_API_BEGIN
+    if (!s) return FVM_OK;
+    if (s->pc) {
+        DeviceGuard g(s->pc->device);
+        fvm::delete_krylov_work(s->work);
+    }
+    delete s;
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_fgmres_set_params(fvm_fgmres *s, const fvm_fgmres_params *p) {
+    if (!s || !p || p->restart < 1 || p->nMaxIterations < 0) return FVM_E_INVALID;
+    s->prm = *p;
+    return FVM_OK;
+}
+
+int fvm_fgmres_solve_device(fvm_fgmres *s, const double *d_b, double *d_x, double *rnorm0, double *rnorm_final,
+                            int *iterations) {
+    FVM_API_BEGIN
+    FVM_CHECK(s && s->pc, FVM_E_INVALID, "null solver");
+    FVM_CHECK(s->pc->ready, FVM_E_NOT_SETUP, "preconditioner not set up");
+    DeviceGuard g(s->pc->device);
+    fvm::fgmres_solve(s->pc, *s->work, s->prm, d_b, d_x, rnorm0, rnorm_final, iterations, s->hist);
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_fgmres_solve(fvm_fgmres *s, const double *b, double *x, double *rnorm0, double *rnorm_final,
+                     int *iterations) {
+    FVM_API_BEGIN
+    FVM_CHECK(s && s->pc, FVM_E_INVALID, "null solver");
+    fvm_amg *h = s->pc;
+    FVM_CHECK(h->ready, FVM_E_NOT_SETUP, "preconditioner not set up");
+    DeviceGuard g(h->device);
+    const size_t n = std::max(h->n_total, 1);
+    h->h_stage_b.ensure(n);
+    h->h_stage_x.ensure(n);
+    hipStream_t st = h->stream;
+    if (h->n_total > 0) {
+        FVM_HIP(hipMemcpyAsync(h->h_stage_b.p, b, sizeof(double) * h->n_total, hipMemcpyHostToDevice, st));
+        FVM_HIP(hipMemcpyAsync(h->h_stage_x.p, x, sizeof(double) * h->n_total, hipMemcpyHostToDevice, st));
+    }
+    fvm::fgmres_solve(h, *s->work, s->prm, h->h_stage_b.p, h->h_stage_x.p, rnorm0, rnorm_final, iterations,
+                      s->hist);
+    if (h->n_self > 0) {
+        FVM_HIP(hipMemcpyAsync(x, h->h_stage_x.p, sizeof(double) * h->n_self, hipMemcpyDeviceToHost, st));
+        FVM_HIP(hipStreamSynchronize(st));
+    }
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_fgmres_history(const fvm_fgmres *s, double *hist, int cap) {
+    if (!s || (cap > 0 && !hist)) return FVM_E_INVALID;
+    const int k = std::min<int>(cap, (int)s->hist.size());
+    for (int i = 0; i < k; i++) hist[i] = s->hist[i];
+    return k;
+}
+
+}  // extern "C"

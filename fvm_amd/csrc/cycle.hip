@@ -1,0 +1,876 @@
+// AMG cycle and solve loop on the GPU.
+//
+// Reference semantics (src/fvm/src/modules/fvmbase/):
+//   AMG::solve   AMG.cpp:219-282  (residual, 1-norm, stop rule, returns r0)
+//   AMG::cycle   AMG.cpp:70-147   (V/W/F recursion, restriction, correction)
+//   doSweeps     AMG.cpp:43-68    (symmetric GS = forward + reverse, or 2 Jacobi)
+//   residual     CRMatrix.h:407-426, GS CRMatrix.h:303-346, Jacobi :353-374
+//   inject       Array.h:427-436, correct Array.h:438-467
+//
+// Every row keeps the reference's accumulation order (b + a_ii x_i, then the
+// off-diagonals in CSR order) and the library is built without FP contraction,
+// so the exact layout reproduces the reference bit for bit; the coloured
+// layout changes only the Gauss-Seidel row order.
+//
+// Two provably-equal shortcuts of the reference are taken: a coarse level
+// entered with delta = 0 and no pre-sweeps restricts b itself (its residual is
+// b + A*0), and the finest residual of the convergence check is reused as the
+// residual of the next cycle's restriction (delta has not changed in between).
+#include "amg.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace fvm {
+
+namespace {
+
+constexpr int BS = 256;
+constexpr int NORM_BLOCKS = 1024;
+
+__device__ __forceinline__ double ldv(const double *p, size_t i) { return p[i]; }
+__device__ __forceinline__ float ldv(const float *p, size_t i) { return p[i]; }
+__device__ __forceinline__ float ldv(const uint16_t *p, size_t i) {
+    return __uint_as_float(((unsigned)p[i]) << 16);
+}
+
+// ------------------------------------------------------------------ CSR (reference order)
+template <class V, class D, class X>
+__global__ void k_csr_residual(int n, const int *__restrict__ row, const int *__restrict__ col,
+                               const V *__restrict__ off, const D *__restrict__ diag,
+                               const X *__restrict__ x, const X *__restrict__ b, X *__restrict__ r) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        X s = b[i] + (X)diag[i] * x[i];
+        for (int p = row[i]; p < row[i + 1]; p++) s += (X)ldv(off, p) * x[col[p]];
+        r[i] = s;
+    }
+}
+
+template <class V, class D, class X>
+__global__ void k_csr_gs_level(const int *__restrict__ ptr, int lvl, const int *__restrict__ rows,
+                               const int *__restrict__ row, const int *__restrict__ col,
+                               const V *__restrict__ off, const D *__restrict__ diag, X *x,
+                               const X *__restrict__ b) {
+    const int beg = ptr[lvl], end = ptr[lvl + 1];
+    for (int k = beg + blockIdx.x * blockDim.x + threadIdx.x; k < end; k += gridDim.x * blockDim.x) {
+        const int i = rows[k];
+        X s = b[i];
+        for (int p = row[i]; p < row[i + 1]; p++) s += (X)ldv(off, p) * x[col[p]];
+        x[i] = -s / (X)diag[i];
+    }
+}
+
+template <class V, class D, class X>
+__global__ void k_csr_jacobi(int n, const int *__restrict__ row, const int *__restrict__ col,
+                             const V *__restrict__ off, const D *__restrict__ diag,
+                             const X *__restrict__ xo, const X *__restrict__ b, X *__restrict__ xn) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        X s = b[i];
+        for (int p = row[i]; p < row[i + 1]; p++) s += (X)ldv(off, p) * xo[col[p]];
+        xn[i] = -s / (X)diag[i];
+    }
+}
+
+// ------------------------------------------------------------------ SELL-64 (coloured order)
+template <class V, class D, class X>
+__global__ void __launch_bounds__(256)
+k_sell_residual(int n, const int *__restrict__ soff, const int *__restrict__ scol,
+                const V *__restrict__ sval, const D *__restrict__ diag, const X *__restrict__ x,
+                const X *__restrict__ b, X *__restrict__ r) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int s = p >> 6, lane = p & 63;
+    const int o = soff[s];
+    const int w = (soff[s + 1] - o) >> 6;
+    X acc = b[p] + (X)diag[p] * x[p];
+    for (int k = 0; k < w; k++) {
+        const int e = o + k * 64 + lane;
+        acc += (X)ldv(sval, e) * x[scol[e]];
+    }
+    r[p] = acc;
+}
+
+template <class V, class D, class X>
+__global__ void __launch_bounds__(256)
+k_sell_gs(int beg, int end, const int *__restrict__ soff, const int *__restrict__ scol,
+          const V *__restrict__ sval, const D *__restrict__ diag, X *x, const X *__restrict__ b) {
+    const int p = beg + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= end) return;
+    const int s = p >> 6, lane = p & 63;
+    const int o = soff[s];
+    const int w = (soff[s + 1] - o) >> 6;
+    X acc = b[p];
+    for (int k = 0; k < w; k++) {
+        const int e = o + k * 64 + lane;
+        acc += (X)ldv(sval, e) * x[scol[e]];
+    }
+    x[p] = -acc / (X)diag[p];
+}
+
+template <class V, class D, class X>
+__global__ void __launch_bounds__(256)
+k_sell_jacobi(int n, const int *__restrict__ soff, const int *__restrict__ scol,
+              const V *__restrict__ sval, const D *__restrict__ diag, const X *__restrict__ xo,
+              const X *__restrict__ b, X *__restrict__ xn) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int s = p >> 6, lane = p & 63;
+    const int o = soff[s];
+    const int w = (soff[s + 1] - o) >> 6;
+    X acc = b[p];
+    for (int k = 0; k < w; k++) {
+        const int e = o + k * 64 + lane;
+        acc += (X)ldv(sval, e) * xo[scol[e]];
+    }
+    xn[p] = -acc / (X)diag[p];
+}
+
+// ------------------------------------------------------------------ transfer operators
+// b_c[I] = sum of r over the fine rows of I in ascending order (inject, Array.h:427-436)
+template <class X>
+__global__ void k_restrict(int nc, const int *__restrict__ c2f_row, const int *__restrict__ c2f_col,
+                           const X *__restrict__ r, X *__restrict__ bc) {
+    for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < nc; I += gridDim.x * blockDim.x) {
+        X s = 0;
+        for (int q = c2f_row[I]; q < c2f_row[I + 1]; q++) s += r[c2f_col[q]];
+        bc[I] = s;
+    }
+}
+
+// x_i += X_c[ci_i] (correct, Array.h:438-467); set != 0: x was zero, write X_c[ci_i]
+template <class X>
+__global__ void k_prolong(int n, const int *__restrict__ ci, const X *__restrict__ xc, X *x, int set,
+                          const double *scale) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int I = ci[i];
+        if (scale) {
+            const X sc = (X)*scale;
+            if (I >= 0) x[i] = set ? (X)0 + xc[I] * sc : x[i] + xc[I] * sc;
+            else if (set) x[i] = 0;
+        } else {
+            if (I >= 0) x[i] = set ? (X)0 + xc[I] : x[i] + xc[I];
+            else if (set) x[i] = 0;
+        }
+    }
+}
+
+template <class X>
+__global__ void k_fill(long long n, X *x, X v) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        x[i] = v;
+}
+
+// ------------------------------------------------------------------ reductions
+// Deterministic: fixed row->thread assignment and fixed reduction trees.
+template <int B>
+__device__ double block_sum(double v) {
+    __shared__ double sh[B / 64];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < B / 64; k++) t += sh[k];
+    __syncthreads();
+    return t;
+}
+
+template <class X>
+__global__ void __launch_bounds__(BS) k_norm1_partial(int n, const X *__restrict__ r, double *part) {
+    double s = 0;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) s += fabs((double)r[i]);
+    s = block_sum<BS>(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+template <class X>
+__global__ void __launch_bounds__(BS) k_dot_partial(int n, const X *__restrict__ a,
+                                                    const X *__restrict__ b, double *part) {
+    double s = 0;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) s += (double)a[i] * (double)b[i];
+    s = block_sum<BS>(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// x^T A x over the reference-order CSR (quadProduct, CRMatrix.h:273-296)
+template <class V, class D, class X>
+__global__ void __launch_bounds__(BS) k_quad_partial(int n, const int *row, const int *col, const V *off,
+                                                     const D *diag, const X *x, double *part) {
+    double s = 0;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        X sn = (X)diag[i] * x[i];
+        for (int p = row[i]; p < row[i + 1]; p++) sn += (X)ldv(off, p) * x[col[p]];
+        s += (double)(sn * x[i]);
+    }
+    s = block_sum<BS>(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+template <class V, class D, class X>
+__global__ void __launch_bounds__(BS) k_quad_partial_sell(int n, const int *soff, const int *scol,
+                                                          const V *sval, const D *diag, const X *x,
+                                                          double *part) {
+    double s = 0;
+    for (int p = blockIdx.x * BS + threadIdx.x; p < n; p += gridDim.x * BS) {
+        const int sl = p >> 6, lane = p & 63;
+        const int o = soff[sl];
+        const int w = (soff[sl + 1] - o) >> 6;
+        X sn = (X)diag[p] * x[p];
+        for (int k = 0; k < w; k++) {
+            const int e = o + k * 64 + lane;
+            sn += (X)ldv(sval, e) * x[scol[e]];
+        }
+        s += (double)(sn * x[p]);
+    }
+    s = block_sum<BS>(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(BS) k_final_sum(int m, const double *part, double *out) {
+    double s = 0;
+    for (int i = threadIdx.x; i < m; i += BS) s += part[i];
+    s = block_sum<BS>(s);
+    if (threadIdx.x == 0) *out = s;
+}
+
+// scale = -(x.b) / (x^T A x) with safeDivide, then limit(1,1): AMG.cpp:124-133,
+// MultiFieldReduction.cpp:107-118, NumType.h:159-167
+__global__ void k_scale(const double *xb, const double *xax, double *scale) {
+    double s = -(*xb);
+    if (*xax != 0) s /= *xax;
+    if (s < 1.0) s = 1.0;
+    else if (s > 1.0) s = 1.0;
+    *scale = s;
+}
+
+// ------------------------------------------------------------------ precision / layout moves
+__global__ void k_d2f(int n, const double *a, float *b) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = (float)a[i];
+}
+__global__ void k_d2f_neg(int n, const double *a, float *b) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = (float)(-a[i]);
+}
+__global__ void k_add_f2d(int n, const float *e, double *x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] += (double)e[i];
+}
+__global__ void k_gather(int n, const int *perm, const double *src, double *dst) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
+        dst[p] = src[perm ? perm[p] : p];
+}
+__global__ void k_scatter(int n, const int *perm, const double *src, double *dst) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
+        dst[perm ? perm[p] : p] = src[p];
+}
+
+// x = -A^{-1} b on the direct level (dense FP64 inverse in reference order)
+template <class X>
+__global__ void k_dense_apply(int n, const double *inv, const int *perm, const int *iperm, const X *b,
+                              X *x) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const int i = perm ? perm[p] : p;
+        double s = 0;
+        for (int j = 0; j < n; j++) s += inv[(size_t)i * n + j] * (double)b[iperm ? iperm[j] : j];
+        x[p] = (X)(-s);
+    }
+}
+
+}  // namespace
+
+// ====================================================================== host side
+
+namespace {
+
+enum VT { V64 = 0, V32 = 1, V16 = 2 };
+
+struct Run {
+    fvm_amg *h;
+    bool fast;
+    bool mixed;
+    int last;       // last level of the recursion
+    bool r0_valid;  // level-0 residual holds b + A x for the current x
+};
+
+inline int value_type(const Run &R, int l) {
+    if (!R.mixed) return V64;
+    return l == 0 ? V32 : V16;
+}
+
+template <class X> X *vb(Level &L);
+template <class X> X *vx(Level &L);
+template <class X> X *vr(Level &L);
+template <class X> DArr<X> &vx_arr(Level &L);
+template <class X> DArr<X> &vt_arr(Level &L);
+template <> double *vb<double>(Level &L) { return L.b.p; }
+template <> double *vx<double>(Level &L) { return L.x.p; }
+template <> double *vr<double>(Level &L) { return L.r.p; }
+template <> DArr<double> &vx_arr<double>(Level &L) { return L.x; }
+template <> DArr<double> &vt_arr<double>(Level &L) { return L.t; }
+template <> float *vb<float>(Level &L) { return L.b32.p; }
+template <> float *vx<float>(Level &L) { return L.x32.p; }
+template <> float *vr<float>(Level &L) { return L.r32.p; }
+template <> DArr<float> &vx_arr<float>(Level &L) { return L.x32; }
+template <> DArr<float> &vt_arr<float>(Level &L) { return L.t32; }
+
+// Dispatch a kernel over the (value, diagonal) types of level l.
+#define FVM_DISPATCH_VALS(R, l, L, X, FAST_CALL, CSR_CALL)                                        \
+    do {                                                                                          \
+        const int vt_ = value_type(R, l);                                                         \
+        if (R.fast) {                                                                             \
+            if (vt_ == V64) { auto *val = L.sell_val.p; auto *dg = L.pdiag.p; FAST_CALL; }        \
+            else if (vt_ == V32) { auto *val = L.sell_val32.p; auto *dg = L.pdiag32.p; FAST_CALL; } \
+            else { auto *val = L.sell_bf16.p; auto *dg = L.pdiag32.p; FAST_CALL; }                \
+        } else {                                                                                  \
+            if (vt_ == V64) { auto *val = L.off.p; auto *dg = L.diag.p; CSR_CALL; }               \
+            else if (vt_ == V32) { auto *val = L.off32.p; auto *dg = L.diag32.p; CSR_CALL; }      \
+            else { auto *val = L.off16.p; auto *dg = L.diag32.p; CSR_CALL; }                      \
+        }                                                                                         \
+    } while (0)
+
+template <class X>
+void residual(Run &R, int l, const X *x, const X *b, X *r) {
+    Level &L = R.h->lev[l];
+    hipStream_t s = R.h->stream;
+    const int n = L.n;
+    if (n == 0) return;
+    FVM_DISPATCH_VALS(R, l, L, X,
+        (k_sell_residual<<<grid_for(n, BS), BS, 0, s>>>(n, L.sell_off.p, L.sell_col.p, val, dg, x, b, r)),
+        (k_csr_residual<<<grid_for(n, BS), BS, 0, s>>>(n, L.row.p, L.col.p, val, dg, x, b, r)));
+}
+
+template <class X>
+void gs_sweep(Run &R, int l, X *x, const X *b) {
+    Level &L = R.h->lev[l];
+    hipStream_t s = R.h->stream;
+    if (L.n == 0) return;
+    if (R.fast) {
+        for (int dir = 0; dir < 2; dir++) {
+            for (int k = 0; k < L.ncolor; k++) {
+                const int c = dir ? L.ncolor - 1 - k : k;
+                const int beg = L.color_ptr[c], end = L.color_ptr[c + 1];
+                if (end <= beg) continue;
+                FVM_DISPATCH_VALS(R, l, L, X,
+                    (k_sell_gs<<<grid_for(end - beg, BS), BS, 0, s>>>(beg, end, L.sell_off.p, L.sell_col.p, val, dg, x, b)),
+                    (void)0);
+            }
+        }
+    } else {
+        for (int dir = 0; dir < 2; dir++) {
+            const int nlev = dir ? L.bwd_nlev : L.fwd_nlev;
+            const int maxw = dir ? L.bwd_maxw : L.fwd_maxw;
+            const int *ptr = dir ? L.bwd_ptr.p : L.fwd_ptr.p;
+            const int *rows = dir ? L.bwd_rows.p : L.fwd_rows.p;
+            const unsigned g = grid_for(maxw, BS);
+            for (int k = 0; k < nlev; k++) {
+                FVM_DISPATCH_VALS(R, l, L, X, (void)0,
+                    (k_csr_gs_level<<<g, BS, 0, s>>>(ptr, k, rows, L.row.p, L.col.p, val, dg, x, b)));
+            }
+        }
+    }
+}
+
+template <class X>
+void jacobi_pass(Run &R, int l) {
+    Level &L = R.h->lev[l];
+    hipStream_t s = R.h->stream;
+    const int n = L.n;
+    if (n == 0) return;
+    DArr<X> &xa = vx_arr<X>(L);
+    DArr<X> &ta = vt_arr<X>(L);
+    const X *b = vb<X>(L);
+    FVM_DISPATCH_VALS(R, l, L, X,
+        (k_sell_jacobi<<<grid_for(n, BS), BS, 0, s>>>(n, L.sell_off.p, L.sell_col.p, val, dg, xa.p, b, ta.p)),
+        (k_csr_jacobi<<<grid_for(n, BS), BS, 0, s>>>(n, L.row.p, L.col.p, val, dg, xa.p, b, ta.p)));
+    std::swap(xa.p, ta.p);   // xnew becomes x (MultiFieldMatrix.cpp:203-215 copies it back)
+}
+
+template <class X>
+void sweeps(Run &R, int l, int ns) {
+    Level &L = R.h->lev[l];
+    for (int k = 0; k < ns; k++) {
+        if (R.h->prm.smootherType == FVM_GAUSS_SEIDEL) {
+            gs_sweep<X>(R, l, vx<X>(L), vb<X>(L));
+        } else {
+            jacobi_pass<X>(R, l);
+            jacobi_pass<X>(R, l);
+        }
+    }
+}
+
+template <class X>
+void fill0(Run &R, int l) {
+    Level &L = R.h->lev[l];
+    if (L.n) k_fill<X><<<grid_for(L.n, BS, 4096), BS, 0, R.h->stream>>>(L.n, vx<X>(L), (X)0);
+}
+
+double *scratch_scalar(fvm_amg *h, int k) { return h->ws.scal.p + k; }
+
+void reduce_final(fvm_amg *h, int nparts, double *out) {
+    k_final_sum<<<1, BS, 0, h->stream>>>(nparts, h->ws.partials.p, out);
+}
+
+template <class X>
+void symmetric_scale(Run &R, int l, double *scale_out) {
+    // coarse level l+1: scale = -(x.b)/(x^T A x) limited to [1,1]
+    fvm_amg *h = R.h;
+    Level &C = h->lev[l + 1];
+    hipStream_t s = h->stream;
+    const int n = C.n;
+    const unsigned g = grid_for(n, BS, NORM_BLOCKS);
+    k_dot_partial<X><<<g, BS, 0, s>>>(n, vx<X>(C), vb<X>(C), h->ws.partials.p);
+    reduce_final(h, g, scratch_scalar(h, 8));
+    FVM_DISPATCH_VALS(R, l + 1, C, X,
+        (k_quad_partial_sell<<<g, BS, 0, s>>>(n, C.sell_off.p, C.sell_col.p, val, dg, vx<X>(C), h->ws.partials.p)),
+        (k_quad_partial<<<g, BS, 0, s>>>(n, C.row.p, C.col.p, val, dg, vx<X>(C), h->ws.partials.p)));
+    reduce_final(h, g, scratch_scalar(h, 9));
+    k_scale<<<1, 1, 0, s>>>(scratch_scalar(h, 8), scratch_scalar(h, 9), scale_out);
+}
+
+template <class X>
+void cycle(Run &R, int type, int l, bool x_zero) {
+    fvm_amg *h = R.h;
+    Level &L = h->lev[l];
+    hipStream_t s = h->stream;
+    if (R.mixed && l == h->direct_level) {
+        const bool perm = R.fast;
+        k_dense_apply<X><<<grid_for(L.n, 64), 64, 0, s>>>(L.n, h->dense_inv.p, perm ? L.perm.p : nullptr,
+                                                          perm ? L.iperm.p : nullptr, vb<X>(L), vx<X>(L));
+        return;
+    }
+    if (h->prm.nPreSweeps > 0) {
+        if (x_zero) fill0<X>(R, l);
+        sweeps<X>(R, l, h->prm.nPreSweeps);
+        x_zero = false;
+        if (l == 0) R.r0_valid = false;   // delta changed: the stored residual is stale
+    }
+    if (l < R.last) {
+        Level &C = h->lev[l + 1];
+        const X *rr;
+        if (x_zero) {
+            rr = vb<X>(L);
+        } else if (l == 0 && R.r0_valid) {
+            rr = vr<X>(L);
+        } else {
+            residual<X>(R, l, vx<X>(L), vb<X>(L), vr<X>(L));
+            rr = vr<X>(L);
+        }
+        if (l == 0) R.r0_valid = false;
+        const int *c2r = R.fast ? L.pc2f_row.p : L.c2f_row.p;
+        const int *c2c = R.fast ? L.pc2f_col.p : L.c2f_col.p;
+        if (C.n) k_restrict<X><<<grid_for(C.n, BS), BS, 0, s>>>(C.n, c2r, c2c, rr, vb<X>(C));
+        cycle<X>(R, type, l + 1, true);
+        if (type == FVM_W_CYCLE) cycle<X>(R, FVM_W_CYCLE, l + 1, false);
+        else if (type == FVM_F_CYCLE) cycle<X>(R, FVM_V_CYCLE, l + 1, false);
+        const double *scale = nullptr;
+        if (h->prm.isSymmetric && h->prm.scaleCorrections) {
+            symmetric_scale<X>(R, l, scratch_scalar(h, 10));
+            scale = scratch_scalar(h, 10);
+        }
+        const int *ci = R.fast ? L.pci.p : L.ci.p;
+        if (L.n) k_prolong<X><<<grid_for(L.n, BS), BS, 0, s>>>(L.n, ci, vx<X>(C), vx<X>(L), x_zero ? 1 : 0, scale);
+        x_zero = false;
+    }
+    if (h->prm.nPostSweeps > 0) {
+        if (x_zero) fill0<X>(R, l);
+        sweeps<X>(R, l, h->prm.nPostSweeps);
+    } else if (x_zero) {
+        fill0<X>(R, l);
+    }
+    FVM_HIP(hipGetLastError());
+}
+
+Run make_run(fvm_amg *h) {
+    FVM_CHECK(h->ready, FVM_E_NOT_SETUP, "solve/smooth called before setup");
+    Run R;
+    R.h = h;
+    R.fast = (h->prm.gsOrdering == FVM_GS_MULTICOLOR);
+    R.mixed = (h->prm.precision == FVM_PREC_MIXED);
+    R.r0_valid = false;
+    if (R.fast) prepare_fast(h);
+    else if (h->prm.smootherType == FVM_GAUSS_SEIDEL)
+        for (auto &L : h->lev) prepare_exact_schedule(h, L);
+    if (R.mixed) {
+        prepare_mixed(h, R.fast);
+        if (h->direct_level < 0) prepare_direct(h);
+    }
+    R.last = (int)h->lev.size() - 1;
+    if (R.mixed && h->direct_level >= 0) R.last = h->direct_level;
+    return R;
+}
+
+// 1-norm of the finest FP64 residual after r = b + A x; returns the norm (host).
+double finest_residual_norm(Run &R) {
+    fvm_amg *h = R.h;
+    Level &F = h->lev[0];
+    hipStream_t s = h->stream;
+    Run R64 = R;
+    R64.mixed = false;
+    residual<double>(R64, 0, F.x.p, F.b.p, F.r.p);
+    const unsigned g = grid_for(F.n, BS, NORM_BLOCKS);
+    k_norm1_partial<double><<<g, BS, 0, s>>>(F.n, F.r.p, h->ws.partials.p);
+    reduce_final(h, g, scratch_scalar(h, 0));
+    FVM_HIP(hipGetLastError());
+    R.r0_valid = true;
+    return read_scalar(scratch_scalar(h, 0), s);
+}
+
+// one cycle on the finest level; in mixed mode an FP32 cycle on the FP64 residual
+void one_cycle(Run &R) {
+    fvm_amg *h = R.h;
+    Level &F = h->lev[0];
+    hipStream_t s = h->stream;
+    if (!R.mixed) {
+        cycle<double>(R, h->prm.cycleType, 0, false);
+        return;
+    }
+    if (!R.r0_valid) {
+        Run R64 = R;
+        R64.mixed = false;
+        residual<double>(R64, 0, F.x.p, F.b.p, F.r.p);
+    }
+    k_d2f<<<grid_for(F.n, BS), BS, 0, s>>>(F.n, F.r.p, F.b32.p);
+    R.r0_valid = false;
+    cycle<float>(R, h->prm.cycleType, 0, true);
+    k_add_f2d<<<grid_for(F.n, BS), BS, 0, s>>>(F.n, F.x32.p, F.x.p);
+    FVM_HIP(hipGetLastError());
+}
+
+void load_finest(Run &R, const double *d_b, const double *d_x) {
+    fvm_amg *h = R.h;
+    Level &F = h->lev[0];
+    const int *perm = R.fast ? F.perm.p : nullptr;
+    if (F.n == 0) return;
+    k_gather<<<grid_for(F.n, BS), BS, 0, h->stream>>>(F.n, perm, d_b, F.b.p);
+    k_gather<<<grid_for(F.n, BS), BS, 0, h->stream>>>(F.n, perm, d_x, F.x.p);
+}
+
+void store_finest(Run &R, double *d_x) {
+    fvm_amg *h = R.h;
+    Level &F = h->lev[0];
+    const int *perm = R.fast ? F.perm.p : nullptr;
+    if (F.n == 0) return;
+    k_scatter<<<grid_for(F.n, BS), BS, 0, h->stream>>>(F.n, perm, F.x.p, d_x);
+}
+
+}  // namespace
+
+void solve_device(fvm_amg *h, const double *d_b, double *d_x, double *rnorm0, double *rnorm_final,
+                  int *cycles) {
+    Run R = make_run(h);
+    load_finest(R, d_b, d_x);
+    const double r0 = finest_residual_norm(R);
+    h->hist.assign(1, r0);
+    if (h->prm.verbosity > 0) printf("0: %g\n", r0);
+    double rn = r0;
+    int it = 0;
+    if (!(r0 < h->prm.absoluteTolerance)) {
+        for (int i = 1; i < h->prm.nMaxIterations; i++) {
+            h->total_iterations++;
+            one_cycle(R);
+            rn = finest_residual_norm(R);
+            double ratio = rn;
+            if (r0 != 0) ratio /= r0;
+            it = i;
+            h->hist.push_back(rn);
+            if (h->prm.verbosity > 0) printf("%d: %g\n", i, rn);
+            if (rn < h->prm.absoluteTolerance || ratio < h->prm.relativeTolerance) break;
+        }
+    }
+    store_finest(R, d_x);
+    FVM_HIP(hipStreamSynchronize(h->stream));
+    if (rnorm0) *rnorm0 = r0;
+    if (rnorm_final) *rnorm_final = rn;
+    if (cycles) *cycles = it;
+}
+
+void smooth_device(fvm_amg *h, const double *d_b, double *d_x) {
+    Run R = make_run(h);
+    load_finest(R, d_b, d_x);
+    one_cycle(R);
+    store_finest(R, d_x);
+    FVM_HIP(hipStreamSynchronize(h->stream));
+}
+
+
+// ===========================================================================
+// FP64 flexible GMRES(m), right-preconditioned by one AMG cycle (FP64 or mixed).
+// Works on finest-level vectors in the active layout of the AMG handle.
+// Textbook algorithm: Saad, "Iterative Methods for Sparse Linear Systems",
+// 2nd ed., Alg. 9.6 (FGMRES), with classical Gram-Schmidt applied twice.
+// ===========================================================================
+namespace {
+
+template <class V, class D>
+__global__ void __launch_bounds__(256)
+k_sell_spmv(int n, const int *__restrict__ soff, const int *__restrict__ scol, const V *__restrict__ sval,
+            const D *__restrict__ diag, const double *__restrict__ x, double *__restrict__ y) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int s = p >> 6, lane = p & 63;
+    const int o = soff[s];
+    const int w = (soff[s + 1] - o) >> 6;
+    double acc = (double)diag[p] * x[p];
+    for (int k = 0; k < w; k++) {
+        const int e = o + k * 64 + lane;
+        acc += (double)ldv(sval, e) * x[scol[e]];
+    }
+    y[p] = acc;
+}
+
+__global__ void k_csr_spmv(int n, const int *__restrict__ row, const int *__restrict__ col,
+                           const double *__restrict__ off, const double *__restrict__ diag,
+                           const double *__restrict__ x, double *__restrict__ y) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double s = diag[i] * x[i];
+        for (int p = row[i]; p < row[i + 1]; p++) s += off[p] * x[col[p]];
+        y[i] = s;
+    }
+}
+
+// partial dots of up to 8 basis vectors V_j (stride n) with w
+__global__ void __launch_bounds__(BS) k_multidot8(int n, const double *__restrict__ V, int j0, int k,
+                                                  const double *__restrict__ w, double *part, int nblk) {
+    double acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) acc[q] = 0;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        const double wi = w[i];
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (q < k) acc[q] += V[(size_t)(j0 + q) * n + i] * wi;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        if (q < k) {
+            const double t = block_sum<BS>(acc[q]);
+            if (threadIdx.x == 0) part[(size_t)(j0 + q) * nblk + blockIdx.x] = t;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(BS) k_multi_final(int nblk, const double *part, double *out) {
+    const int j = blockIdx.x;
+    double s = 0;
+    for (int i = threadIdx.x; i < nblk; i += BS) s += part[(size_t)j * nblk + i];
+    s = block_sum<BS>(s);
+    if (threadIdx.x == 0) out[j] = s;
+}
+
+// w += sign * sum_j c_j V_j
+__global__ void k_multi_axpy(int n, const double *__restrict__ V, int k, const double *__restrict__ c,
+                             double sign, double *w) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double s = 0;
+        for (int j = 0; j < k; j++) s += c[j] * V[(size_t)j * n + i];
+        w[i] += sign * s;
+    }
+}
+
+__global__ void k_scale_into(int n, const double *a, double alpha, double *b) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = alpha * a[i];
+}
+
+__global__ void k_neg_into(int n, const double *a, double *b) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = -a[i];
+}
+
+__global__ void __launch_bounds__(BS) k_norm2sq_partial(int n, const double *__restrict__ r, double *part) {
+    double s = 0;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) s += r[i] * r[i];
+    s = block_sum<BS>(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+}  // namespace
+
+struct KrylovWork {
+    int n = 0, m = 0;
+    DArr<double> V, Z, w, b, x, r, part, dev_h;
+};
+
+namespace {
+
+void spmv64(Run &R, const double *x, double *y) {
+    fvm_amg *h = R.h;
+    Level &F = h->lev[0];
+    const int n = F.n;
+    if (n == 0) return;
+    if (R.fast)
+        k_sell_spmv<<<grid_for(n, BS), BS, 0, h->stream>>>(n, F.sell_off.p, F.sell_col.p, F.sell_val.p,
+                                                           F.pdiag.p, x, y);
+    else
+        k_csr_spmv<<<grid_for(n, BS), BS, 0, h->stream>>>(n, F.row.p, F.col.p, F.off.p, F.diag.p, x, y);
+}
+
+// z = M(v) ~= A^{-1} v : one cycle from zero on A z + (-v) = 0
+void precond(Run &R, const double *v, double *z) {
+    fvm_amg *h = R.h;
+    Level &F = h->lev[0];
+    hipStream_t s = h->stream;
+    const int n = F.n;
+    R.r0_valid = false;
+    if (!R.mixed) {
+        k_neg_into<<<grid_for(n, BS), BS, 0, s>>>(n, v, F.b.p);
+        cycle<double>(R, h->prm.cycleType, 0, true);
+        FVM_HIP(hipMemcpyAsync(z, F.x.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    } else {
+        k_d2f_neg<<<grid_for(n, BS), BS, 0, s>>>(n, v, F.b32.p);
+        cycle<float>(R, h->prm.cycleType, 0, true);
+        k_fill<double><<<grid_for(n, BS, 4096), BS, 0, s>>>(n, z, 0.0);
+        k_add_f2d<<<grid_for(n, BS), BS, 0, s>>>(n, F.x32.p, z);
+    }
+    FVM_HIP(hipGetLastError());
+}
+
+double reduce_to_host(fvm_amg *h, unsigned g, int slot) {
+    k_final_sum<<<1, BS, 0, h->stream>>>(g, h->ws.partials.p, scratch_scalar(h, slot));
+    return read_scalar(scratch_scalar(h, slot), h->stream);
+}
+
+double norm1_of(Run &R, const double *r, int n) {
+    fvm_amg *h = R.h;
+    const unsigned g = grid_for(n, BS, NORM_BLOCKS);
+    k_norm1_partial<double><<<g, BS, 0, h->stream>>>(n, r, h->ws.partials.p);
+    return reduce_to_host(h, g, 1);
+}
+
+double norm2_of(Run &R, const double *r, int n) {
+    fvm_amg *h = R.h;
+    const unsigned g = grid_for(n, BS, NORM_BLOCKS);
+    k_norm2sq_partial<<<g, BS, 0, h->stream>>>(n, r, h->ws.partials.p);
+    return std::sqrt(reduce_to_host(h, g, 2));
+}
+
+// h_out[0..k) = V_{0..k}^T w
+void multidot(Run &R, KrylovWork &W, int k, const double *w, std::vector<double> &h_out) {
+    fvm_amg *h = R.h;
+    hipStream_t s = h->stream;
+    const int n = W.n;
+    const unsigned g = grid_for(n, BS, 512);
+    for (int j0 = 0; j0 < k; j0 += 8)
+        k_multidot8<<<g, BS, 0, s>>>(n, W.V.p, j0, std::min(8, k - j0), w, W.part.p, (int)g);
+    k_multi_final<<<k, BS, 0, s>>>((int)g, W.part.p, W.dev_h.p);
+    FVM_HIP(hipGetLastError());
+    h_out.resize(k);
+    FVM_HIP(hipMemcpyAsync(h_out.data(), W.dev_h.p, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    FVM_HIP(hipStreamSynchronize(s));
+}
+
+void multi_axpy(Run &R, KrylovWork &W, const double *basis, int k, const std::vector<double> &c, double sign,
+                double *w) {
+    fvm_amg *h = R.h;
+    hipStream_t s = h->stream;
+    FVM_HIP(hipMemcpyAsync(W.dev_h.p, c.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+    k_multi_axpy<<<grid_for(W.n, BS), BS, 0, s>>>(W.n, basis, k, W.dev_h.p, sign, w);
+    FVM_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const double *d_b, double *d_x,
+                  double *rnorm0, double *rnorm_final, int *iterations, std::vector<double> &hist) {
+    Run R = make_run(h);
+    Level &F = h->lev[0];
+    hipStream_t s = h->stream;
+    const int n = F.n;
+    const int m = std::max(1, p.restart);
+    if (W.n != n || W.m != m) {
+        W.n = n; W.m = m;
+        W.V.alloc((size_t)(m + 1) * std::max(n, 1));
+        W.Z.alloc((size_t)m * std::max(n, 1));
+        W.w.alloc(std::max(n, 1)); W.b.alloc(std::max(n, 1)); W.x.alloc(std::max(n, 1));
+        W.r.alloc(std::max(n, 1));
+        W.part.alloc((size_t)(m + 1) * 512);
+        W.dev_h.alloc(m + 1);
+    }
+    const int *perm = R.fast ? F.perm.p : nullptr;
+    k_gather<<<grid_for(n, BS), BS, 0, s>>>(n, perm, d_b, W.b.p);
+    k_gather<<<grid_for(n, BS), BS, 0, s>>>(n, perm, d_x, W.x.p);
+    auto residual_into_r = [&]() {
+        // r = b + A x, with the residual kernel of the FP64 finest level
+        Run R64 = R;
+        R64.mixed = false;
+        residual<double>(R64, 0, W.x.p, W.b.p, W.r.p);
+    };
+    residual_into_r();
+    const double r0 = norm1_of(R, W.r.p, n);
+    const double r0_2 = norm2_of(R, W.r.p, n);
+    hist.assign(1, r0);
+    double rn = r0;
+    int total = 0;
+    if (!(r0 < p.absoluteTolerance) && n > 0) {
+        std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hc, hc2, y;
+        for (;;) {
+            const double beta = norm2_of(R, W.r.p, n);
+            if (!(beta > 0)) break;
+            k_scale_into<<<grid_for(n, BS), BS, 0, s>>>(n, W.r.p, -1.0 / beta, W.V.p);
+            std::fill(g.begin(), g.end(), 0.0);
+            g[0] = beta;
+            int jend = 0;
+            for (int j = 0; j < m; j++) {
+                double *vj = W.V.p + (size_t)j * n;
+                double *zj = W.Z.p + (size_t)j * n;
+                precond(R, vj, zj);
+                spmv64(R, zj, W.w.p);
+                multidot(R, W, j + 1, W.w.p, hc);
+                multi_axpy(R, W, W.V.p, j + 1, hc, -1.0, W.w.p);
+                multidot(R, W, j + 1, W.w.p, hc2);
+                multi_axpy(R, W, W.V.p, j + 1, hc2, -1.0, W.w.p);
+                for (int q = 0; q <= j; q++) hc[q] += hc2[q];
+                const double hn = norm2_of(R, W.w.p, n);
+                for (int q = 0; q <= j; q++) H[(size_t)q * m + j] = hc[q];
+                H[(size_t)(j + 1) * m + j] = hn;
+                if (hn > 0)
+                    k_scale_into<<<grid_for(n, BS), BS, 0, s>>>(n, W.w.p, 1.0 / hn, W.V.p + (size_t)(j + 1) * n);
+                for (int q = 0; q < j; q++) {
+                    const double a = H[(size_t)q * m + j], b = H[(size_t)(q + 1) * m + j];
+                    H[(size_t)q * m + j] = cs[q] * a + sn[q] * b;
+                    H[(size_t)(q + 1) * m + j] = -sn[q] * a + cs[q] * b;
+                }
+                const double a = H[(size_t)j * m + j], b = H[(size_t)(j + 1) * m + j];
+                const double den = std::hypot(a, b);
+                cs[j] = den > 0 ? a / den : 1.0;
+                sn[j] = den > 0 ? b / den : 0.0;
+                H[(size_t)j * m + j] = den;
+                H[(size_t)(j + 1) * m + j] = 0.0;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] = cs[j] * g[j];
+                total++;
+                jend = j + 1;
+                const double est = std::fabs(g[j + 1]);
+                if (p.verbosity > 1) printf("fgmres %d: est2 %g\n", total, est);
+                if (est <= p.relativeTolerance * r0_2 || est < p.absoluteTolerance || !(hn > 0) ||
+                    total >= p.nMaxIterations)
+                    break;
+            }
+            y.assign(jend, 0.0);
+            for (int q = jend - 1; q >= 0; q--) {
+                double t = g[q];
+                for (int c = q + 1; c < jend; c++) t -= H[(size_t)q * m + c] * y[c];
+                y[q] = t / H[(size_t)q * m + q];
+            }
+            multi_axpy(R, W, W.Z.p, jend, y, 1.0, W.x.p);
+            residual_into_r();
+            rn = norm1_of(R, W.r.p, n);
+            hist.push_back(rn);
+            if (p.verbosity > 0) printf("%d: %g\n", total, rn);
+            double ratio = rn;
+            if (r0 != 0) ratio /= r0;
+            if (rn < p.absoluteTolerance || ratio < p.relativeTolerance) break;
+            if (total >= p.nMaxIterations) break;
+        }
+    }
+    k_scatter<<<grid_for(n, BS), BS, 0, s>>>(n, perm, W.x.p, d_x);
+    FVM_HIP(hipStreamSynchronize(s));
+    if (rnorm0) *rnorm0 = r0;
+    if (rnorm_final) *rnorm_final = rn;
+    if (iterations) *iterations = total;
+}
+
+}  // namespace fvm
+
+namespace fvm {
+KrylovWork *new_krylov_work() { return new KrylovWork(); }
+void delete_krylov_work(KrylovWork *w) { delete w; }
+}  // namespace fvm

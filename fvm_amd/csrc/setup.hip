@@ -1,0 +1,1015 @@
+// AMG hierarchy construction on the GPU.
+//
+// Reference semantics (src/fvm/src/modules/fvmbase/):
+//   coarsening            CRMatrix.h:468-586   (greedy pairwise agglomeration)
+//   coarse-to-fine map    MultiFieldMatrix.cpp:627-667
+//   coarse connectivity   CRMatrix.h:597-691   (first-appearance column order)
+//   Galerkin operator     CRMatrix.h:699-758   (fixed summation order)
+//   level loop/stop rule  AMG.cpp:149-210
+//
+// The greedy agglomeration is sequential in the reference (row nr's decision
+// reads the state left by rows < nr). Here it runs in rounds: a row decides
+// as soon as every row that could still change its inputs has decided, which
+// yields exactly the sequential result (see k_coarsen_round).
+#include "amg.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+
+namespace fvm {
+
+namespace {
+
+constexpr int BS = 256;
+constexpr int MEMW = 5;  // members per group: at most groupSize + 3 for groupSize 2
+
+__device__ __forceinline__ bool isb(const uint8_t *bnd, int i) { return bnd && bnd[i]; }
+
+// ------------------------------------------------------------------ input checks
+__global__ void k_validate(int n_self, int n_total, long long nnz, const int *row, const int *col,
+                           const double *diag, int *flag) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_total; i += gridDim.x * blockDim.x) {
+        const int b = row[i], e = row[i + 1];
+        if (b < 0 || e < b || (long long)e > nnz) { atomicOr(flag, 1); continue; }
+        if (i >= n_self) continue;
+        for (int p = b; p < e; p++) {
+            const int c = col[p];
+            if (c < 0 || c >= n_total || c == i) atomicOr(flag, 2);
+        }
+        const double d = diag[i];
+        if (!(d != 0.0) || !isfinite(d)) atomicOr(flag, 4);
+    }
+}
+
+__global__ void k_compact_count(int n_self, const int *row, const int *col, int *cnt) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_self; i += gridDim.x * blockDim.x) {
+        int k = 0;
+        for (int p = row[i]; p < row[i + 1]; p++) k += (col[p] < n_self);
+        cnt[i] = k;
+    }
+}
+
+__global__ void k_compact_fill(int n_self, const int *row, const int *col, const double *off,
+                               const int *nrow, int *ncol, double *noff) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_self; i += gridDim.x * blockDim.x) {
+        int q = nrow[i];
+        for (int p = row[i]; p < row[i + 1]; p++) {
+            const int c = col[p];
+            if (c < n_self) { ncol[q] = c; noff[q] = off[p]; q++; }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ coarsening
+__global__ void k_coarsen_init(int n, const uint8_t *bnd, int *ldr, int *dround, int *memr,
+                               int *mcnt, int *stamp, int *remaining) {
+    int local = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const bool b = isb(bnd, i);
+        ldr[i] = -1;
+        dround[i] = b ? -1 : INT_MAX;
+        mcnt[i] = 0;
+        stamp[i] = 0;
+        for (int k = 0; k < MEMW; k++) memr[(size_t)i * MEMW + k] = INT_MAX;
+        local += b ? 0 : 1;
+    }
+    if (local) atomicAdd(remaining, local);
+}
+
+__device__ __forceinline__ void push_cand(int x, int t, const uint8_t *bnd, const int *dround,
+                                          int *stamp, int *next, int *next_cnt) {
+    if (isb(bnd, x)) return;
+    if (dround[x] <= t) return;               // already decided
+    if (stamp[x] >= t + 1) return;            // already queued for round t+1
+    if (atomicMax(&stamp[x], t + 1) < t + 1) {
+        const int k = atomicAdd(next_cnt, 1);
+        next[k] = x;
+    }
+}
+
+// Rows whose readiness may depend on d having decided (see k_coarsen_round):
+// rows > d in row(d), in rowT(d), and in rowT(c) for c in row(d).
+__device__ void notify(int d, int t, const int *row, const int *col, const int *rowT,
+                       const int *colT, int sym, const uint8_t *bnd, const int *dround, int *stamp,
+                       int *next, int *next_cnt) {
+    for (int p = row[d]; p < row[d + 1]; p++) {
+        const int c = col[p];
+        if (c > d) push_cand(c, t, bnd, dround, stamp, next, next_cnt);
+        for (int q = rowT[c]; q < rowT[c + 1]; q++) {
+            const int c2 = colT[q];
+            if (c2 > d) push_cand(c2, t, bnd, dround, stamp, next, next_cnt);
+        }
+    }
+    if (!sym)
+        for (int q = rowT[d]; q < rowT[d + 1]; q++) {
+            const int c = colT[q];
+            if (c > d) push_cand(c, t, bnd, dround, stamp, next, next_cnt);
+        }
+}
+
+// One round of the exact parallel form of CRMatrix::createCoarsening for
+// groupSize 2 (CRMatrix.h:485-583).
+//
+// State: dround[i] = round in which row i became grouped (INT_MAX: not yet,
+// -1: boundary row); ldr[i] = the seed row ("leader") of i's group; per leader
+// the member list mem/memr (member row, round it joined). Only state stamped
+// with a round < t is read as decided, so every decision of round t is a
+// function of the state after round t-1.
+//
+// In the sequential order, when row r comes up every non-boundary row < r is
+// grouped, and a row c > r is grouped iff an earlier seed s < r with c in
+// row(s) paired it. rowT is the transpose pattern: rowT(c) = {s : c in row(s)}.
+// Row r may decide in round t when
+//   R1  every s < r in rowT(r) has decided (s could still pair r);
+//   R1a every c < r in row(r) has decided (r needs c's group);
+//   R2  for every undecided c > r in row(r), every s < r in rowT(c) has decided
+//       (s could still pair c before r's turn);
+//   R3  (only when r would join the group g of its strongest grouped neighbour)
+//       every s < r in rowT(m) of a member m of g has decided (s could still
+//       join g and change g's count).
+// Two rows that are both ready in one round never write state the other reads,
+// so the result equals the sequential one (checked against the oracle on every
+// level, on symmetric and non-symmetric patterns).
+__global__ void k_coarsen_round(int t, int all_rows, int n, const int *row, const int *col,
+                                const int *rowT, const int *colT, int sym, const double *off,
+                                const double *diag, const uint8_t *bnd, double theta, int *ldr,
+                                int *dround, int *mem, int *memr, int *mcnt, int *stamp,
+                                const int *cand, const int *cand_cnt, int *next, int *next_cnt,
+                                int *zero_cnt, int *remaining) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *zero_cnt = 0;
+    const int ncand = all_rows ? n : *cand_cnt;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < ncand;
+         idx += gridDim.x * blockDim.x) {
+        const int r = all_rows ? idx : cand[idx];
+        if (dround[r] < t) continue;
+        bool ready = true;
+        // R1
+        if (!sym)
+            for (int q = rowT[r]; q < rowT[r + 1]; q++) {
+                const int s = colT[q];
+                if (s < r && !isb(bnd, s) && !(dround[s] < t)) { ready = false; break; }
+            }
+        // R1a / R2
+        for (int p = row[r]; p < row[r + 1] && ready; p++) {
+            const int c = col[p];
+            if (isb(bnd, c)) continue;
+            if (c < r) {
+                if (!(dround[c] < t)) ready = false;
+            } else if (!(dround[c] < t)) {
+                for (int q = rowT[c]; q < rowT[c + 1]; q++) {
+                    const int s = colT[q];
+                    if (s < r && !isb(bnd, s) && !(dround[s] < t)) { ready = false; break; }
+                }
+            }
+        }
+        if (!ready) continue;
+        // strongest ungrouped / grouped neighbours; weight uses the seed's diagonal
+        const double d0 = fabs(diag[r]);
+        int cu = -1, cg = -1;
+        double mu = 0, mg = 0;
+        for (int p = row[r]; p < row[r + 1]; p++) {
+            const int c = col[p];
+            if (isb(bnd, c)) continue;
+            const double d1 = fabs(diag[c]);
+            const double w = fabs(fabs(off[p]) / (d0 < d1 ? d1 : d0));
+            if (!(dround[c] < t)) {
+                if (cu == -1 || w > mu) { cu = c; mu = w; }
+            } else {
+                if (cg == -1 || w > mg) { cg = c; mg = w; }
+            }
+        }
+        int action;  // 0 pair, 1 new singleton group, 2 join cg's group
+        int g = -1;
+        if (cu != -1 && (cg == -1 || mu > theta * mg)) {
+            action = 0;
+        } else if (cg == -1) {
+            action = 1;
+        } else {
+            g = ldr[cg];
+            int count = 0;
+            bool blocked = false;
+            for (int k = 0; k < MEMW && !blocked; k++) {
+                if (!(memr[(size_t)g * MEMW + k] < t)) continue;
+                count++;
+                const int m = mem[(size_t)g * MEMW + k];
+                for (int q = rowT[m]; q < rowT[m + 1]; q++) {
+                    const int s = colT[q];
+                    if (s < r && !isb(bnd, s) && !(dround[s] < t)) { blocked = true; break; }
+                }
+            }
+            if (blocked) {  // R3: retry next round
+                if (atomicMax(&stamp[r], t + 1) < t + 1) {
+                    const int k = atomicAdd(next_cnt, 1);
+                    next[k] = r;
+                }
+                continue;
+            }
+            action = (count > 2 + 2) ? 1 : 2;   // coarseCount > groupSize+2
+        }
+        if (action == 0) {
+            ldr[r] = r; ldr[cu] = r;
+            mem[(size_t)r * MEMW] = r; memr[(size_t)r * MEMW] = t;
+            mem[(size_t)r * MEMW + 1] = cu; memr[(size_t)r * MEMW + 1] = t;
+            mcnt[r] = 2;
+            dround[cu] = t; dround[r] = t;
+            atomicSub(remaining, 2);
+        } else if (action == 1) {
+            ldr[r] = r;
+            mem[(size_t)r * MEMW] = r; memr[(size_t)r * MEMW] = t;
+            mcnt[r] = 1;
+            dround[r] = t;
+            atomicSub(remaining, 1);
+        } else {
+            const int k = mcnt[g];
+            mem[(size_t)g * MEMW + k] = r; memr[(size_t)g * MEMW + k] = t;
+            mcnt[g] = k + 1;
+            ldr[r] = g;
+            dround[r] = t;
+            atomicSub(remaining, 1);
+        }
+        notify(r, t, row, col, rowT, colT, sym, bnd, dround, stamp, next, next_cnt);
+        if (action == 0) notify(cu, t, row, col, rowT, colT, sym, bnd, dround, stamp, next, next_cnt);
+    }
+}
+
+// transpose pattern (order inside a row is irrelevant to its users)
+__global__ void k_tr_count(int n, const int *row, const int *col, int *cnt) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        for (int p = row[i]; p < row[i + 1]; p++) atomicAdd(&cnt[col[p]], 1);
+}
+
+__global__ void k_tr_fill(int n, const int *row, const int *col, const int *rowT, int *fill, int *colT) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        for (int p = row[i]; p < row[i + 1]; p++) {
+            const int c = col[p];
+            colT[rowT[c] + atomicAdd(&fill[c], 1)] = i;
+        }
+}
+
+// flag != 0 if some entry (i,c) has no mirror (c,i)
+__global__ void k_sym_check(int n, const int *row, const int *col, int *flag) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        for (int p = row[i]; p < row[i + 1]; p++) {
+            const int c = col[p];
+            bool found = false;
+            for (int q = row[c]; q < row[c + 1]; q++)
+                if (col[q] == i) { found = true; break; }
+            if (!found) { atomicOr(flag, 1); return; }
+        }
+}
+
+__global__ void k_leader_flags(int n, const int *ldr, const uint8_t *bnd, int *flag) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        flag[i] = (!isb(bnd, i) && ldr[i] == i) ? 1 : 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) flag[n] = 0;
+}
+
+__global__ void k_assign_ci(int n, const int *ldr, const uint8_t *bnd, const int *gid, int *ci) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        ci[i] = (isb(bnd, i) || ldr[i] < 0) ? -1 : gid[ldr[i]];
+}
+
+// ------------------------------------------------------------------ coarse-to-fine
+__global__ void k_c2f_count(int n, const int *ci, int *cnt) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (ci[i] >= 0) atomicAdd(&cnt[ci[i]], 1);
+}
+
+__global__ void k_c2f_fill(int n, const int *ci, const int *c2f_row, int *fill, int *c2f_col) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int I = ci[i];
+        if (I >= 0) c2f_col[c2f_row[I] + atomicAdd(&fill[I], 1)] = i;
+    }
+}
+
+__global__ void k_segsort(int nseg, const int *ptr, int *v) {
+    for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < nseg; I += gridDim.x * blockDim.x) {
+        const int b = ptr[I], e = ptr[I + 1];
+        for (int a = b + 1; a < e; a++) {
+            const int key = v[a];
+            int j = a - 1;
+            while (j >= b && v[j] > key) { v[j + 1] = v[j]; j--; }
+            v[j + 1] = key;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ coarse operator
+__global__ void k_cc_ub(int nc, const int *c2f_row, const int *c2f_col, const int *row, int *ub) {
+    for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < nc; I += gridDim.x * blockDim.x) {
+        int s = 0;
+        for (int q = c2f_row[I]; q < c2f_row[I + 1]; q++) {
+            const int i = c2f_col[q];
+            s += row[i + 1] - row[i];
+        }
+        ub[I] = s;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ub[nc] = 0;
+}
+
+// Distinct coarse neighbours J != I of coarse row I in order of first
+// appearance (fine rows ascending, then CSR order): CRMatrix.h:616-640.
+__global__ void k_cc_list(int nc, const int *c2f_row, const int *c2f_col, const int *row,
+                          const int *col, const int *ci, const int *uboff, int *scratch, int *cnt) {
+    for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < nc; I += gridDim.x * blockDim.x) {
+        int *lst = scratch + uboff[I];
+        int k = 0;
+        for (int q = c2f_row[I]; q < c2f_row[I + 1]; q++) {
+            const int i = c2f_col[q];
+            for (int p = row[i]; p < row[i + 1]; p++) {
+                const int J = ci[col[p]];
+                if (J < 0 || J == I) continue;
+                bool seen = false;
+                for (int e = 0; e < k; e++)
+                    if (lst[e] == J) { seen = true; break; }
+                if (!seen) lst[k++] = J;
+            }
+        }
+        cnt[I] = k;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[nc] = 0;
+}
+
+// Galerkin values with the reference's summation order (CRMatrix.h:722-755).
+__global__ void k_cc_galerkin(int nc, const int *c2f_row, const int *c2f_col, const int *row,
+                              const int *col, const double *diag, const double *off, const int *ci,
+                              const int *uboff, const int *scratch, const int *crow, int *ccol,
+                              double *cdiag, double *coff) {
+    for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < nc; I += gridDim.x * blockDim.x) {
+        const int base = crow[I];
+        const int k = crow[I + 1] - base;
+        const int *lst = scratch + uboff[I];
+        for (int e = 0; e < k; e++) { ccol[base + e] = lst[e]; coff[base + e] = 0.0; }
+        double d = 0.0;
+        for (int q = c2f_row[I]; q < c2f_row[I + 1]; q++) {
+            const int i = c2f_col[q];
+            d += diag[i];
+            for (int p = row[i]; p < row[i + 1]; p++) {
+                const int J = ci[col[p]];
+                if (J < 0) continue;
+                if (J != I) {
+                    int e = 0;
+                    while (lst[e] != J) e++;
+                    coff[base + e] += off[p];
+                } else {
+                    d += off[p];
+                }
+            }
+        }
+        cdiag[I] = d;
+    }
+}
+
+// ------------------------------------------------------------------ GS level schedule
+// Rows i < j are ordered by the sweep whenever j is in row(i) or i is in row(j):
+// the later row reads the earlier one's new value, or the earlier row must read
+// the later one's old value. Forward sweep (dir 0): edges go from lower to
+// higher index; reverse sweep (dir 1): from higher to lower. With a
+// non-symmetric pattern both row and rowT are walked (an edge present in both
+// is counted twice on both sides, which keeps the in-degree bookkeeping exact).
+__global__ void k_sched_init(int n, int dir, const int *row, const int *col, const int *rowT,
+                             const int *colT, int *indeg, int *rows, int *tail) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int k = 0;
+        for (int p = row[i]; p < row[i + 1]; p++) k += dir ? (col[p] > i) : (col[p] < i);
+        if (rowT)
+            for (int p = rowT[i]; p < rowT[i + 1]; p++) k += dir ? (colT[p] > i) : (colT[p] < i);
+        indeg[i] = k;
+        if (k == 0) rows[atomicAdd(tail, 1)] = i;
+    }
+}
+
+__global__ void k_sched_round(int lvl, int dir, const int *row, const int *col, const int *rowT,
+                              const int *colT, int *indeg, int *rows, int *ptr, int *tail,
+                              unsigned *done_blocks) {
+    const int b = ptr[lvl], e = ptr[lvl + 1];
+    for (int idx = b + blockIdx.x * blockDim.x + threadIdx.x; idx < e;
+         idx += gridDim.x * blockDim.x) {
+        const int i = rows[idx];
+        for (int p = row[i]; p < row[i + 1]; p++) {
+            const int c = col[p];
+            if (dir ? (c < i) : (c > i))
+                if (atomicSub(&indeg[c], 1) == 1) rows[atomicAdd(tail, 1)] = c;
+        }
+        if (rowT)
+            for (int p = rowT[i]; p < rowT[i + 1]; p++) {
+                const int c = colT[p];
+                if (dir ? (c < i) : (c > i))
+                    if (atomicSub(&indeg[c], 1) == 1) rows[atomicAdd(tail, 1)] = c;
+            }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned prev = atomicAdd(done_blocks, 1u);
+        if (prev == gridDim.x - 1) {
+            ptr[lvl + 2] = atomicAdd(tail, 0);
+            *done_blocks = 0u;
+        }
+    }
+}
+
+__global__ void k_set_ptr01(int *ptr, const int *tail) {
+    ptr[0] = 0;
+    ptr[1] = *tail;
+}
+
+// ------------------------------------------------------------------ colouring
+__device__ __forceinline__ unsigned hash32(unsigned x) {
+    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+    return x;
+}
+
+// Jones-Plassmann rounds with first-fit colours; a row is coloured in round t
+// if its (hash, index) priority beats every neighbour not coloured before t.
+__global__ void k_color_round(int t, int n, const int *row, const int *col, const int *rowT,
+                              const int *colT, int *color, int *cround, int *left) {
+    int done = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (cround[i] < t) continue;
+        const unsigned pi = hash32((unsigned)i);
+        bool best = true;
+        unsigned long long used[4] = {0, 0, 0, 0};
+        for (int pass = 0; pass < 2 && best; pass++) {
+            const int *rp = pass ? rowT : row;
+            const int *cp = pass ? colT : col;
+            if (!rp) break;
+            for (int p = rp[i]; p < rp[i + 1]; p++) {
+                const int j = cp[p];
+                if (cround[j] < t) {
+                    const int cj = color[j];
+                    if (cj < 256) used[cj >> 6] |= 1ull << (cj & 63);
+                } else {
+                    const unsigned pj = hash32((unsigned)j);
+                    if (pj > pi || (pj == pi && j > i)) { best = false; break; }
+                }
+            }
+        }
+        if (!best) continue;
+        int c = 0;
+        while (c < 256 && ((used[c >> 6] >> (c & 63)) & 1ull)) c++;
+        color[i] = c;
+        cround[i] = t;
+        done++;
+    }
+    if (done) atomicSub(left, done);
+}
+
+__global__ void k_fill_int(int n, int *v, int val) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] = val;
+}
+
+__global__ void k_iota(int n, int *v) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] = i;
+}
+
+__global__ void k_invert_perm(int n, const int *perm, int *iperm) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
+        iperm[perm[p]] = p;
+}
+
+__global__ void k_color_hist(int n, const int *color, int *hist) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        atomicAdd(&hist[color[i]], 1);
+}
+
+// ------------------------------------------------------------------ SELL-64
+__global__ void k_sell_width(int n, int nslice, const int *perm, const int *row, int *w64) {
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nslice; s += gridDim.x * blockDim.x) {
+        int w = 0;
+        for (int l = 0; l < 64; l++) {
+            const int p = s * 64 + l;
+            if (p >= n) break;
+            const int i = perm[p];
+            w = max(w, row[i + 1] - row[i]);
+        }
+        w64[s] = w * 64;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) w64[nslice] = 0;
+}
+
+__global__ void k_sell_fill(int n, const int *perm, const int *iperm, const int *row,
+                            const int *col, const double *off, const double *diag,
+                            const int *soff, int *scol, double *sval, double *pdiag) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const int s = p >> 6, lane = p & 63;
+        const int w = (soff[s + 1] - soff[s]) >> 6;
+        const int i = perm[p];
+        const int b = row[i], len = row[i + 1] - b;
+        for (int k = 0; k < w; k++) {
+            const size_t e = (size_t)soff[s] + (size_t)k * 64 + lane;
+            if (k < len) { scol[e] = iperm[col[b + k]]; sval[e] = off[b + k]; }
+            else { scol[e] = p; sval[e] = 0.0; }
+        }
+        pdiag[p] = diag[i];
+    }
+}
+
+// rows of the last slice beyond n: make their padding entries harmless
+__global__ void k_sell_tail(int n, int nslice, const int *soff, int *scol, double *sval) {
+    const int s = nslice - 1;
+    const int w = (soff[s + 1] - soff[s]) >> 6;
+    for (int lane = threadIdx.x; lane < 64; lane += blockDim.x) {
+        const int p = s * 64 + lane;
+        if (p < n) continue;
+        for (int k = 0; k < w; k++) {
+            const size_t e = (size_t)soff[s] + (size_t)k * 64 + lane;
+            scol[e] = 0; sval[e] = 0.0;
+        }
+    }
+}
+
+__global__ void k_perm_ci(int n, const int *perm, const int *ci, const int *iperm_next, int *pci) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const int I = ci[perm[p]];
+        pci[p] = I >= 0 ? iperm_next[I] : -1;
+    }
+}
+
+// ------------------------------------------------------------------ mixed precision
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    const unsigned lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return (uint16_t)(u >> 16);
+}
+
+__global__ void k_to_f32(long long n, const double *a, float *b) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
+__global__ void k_to_bf16(long long n, const double *a, uint16_t *b) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        b[i] = f32_to_bf16_rne((float)a[i]);
+}
+
+// ------------------------------------------------------------------ dense inverse
+// Gauss-Jordan with partial pivoting of a dense n x n FP64 matrix in one block.
+__global__ void k_dense_from_csr(int n, const int *row, const int *col, const double *diag,
+                                 const double *off, double *A) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        for (int j = 0; j < n; j++) A[(size_t)i * n + j] = 0.0;
+        A[(size_t)i * n + i] = diag[i];
+        for (int p = row[i]; p < row[i + 1]; p++) A[(size_t)i * n + col[p]] += off[p];
+    }
+}
+
+__global__ void k_gauss_jordan(int n, double *A, double *Inv, int *status) {
+    __shared__ int piv;
+    __shared__ double pval;
+    for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x)
+        Inv[idx] = ((idx / n) == (idx % n)) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int k = 0; k < n; k++) {
+        if (threadIdx.x == 0) {
+            int best = k;
+            double bv = fabs(A[(size_t)k * n + k]);
+            for (int i = k + 1; i < n; i++) {
+                const double v = fabs(A[(size_t)i * n + k]);
+                if (v > bv) { bv = v; best = i; }
+            }
+            piv = best;
+            pval = bv;
+        }
+        __syncthreads();
+        if (!(pval > 0.0) || !isfinite(pval)) {
+            if (threadIdx.x == 0) *status = 1;
+            return;
+        }
+        if (piv != k) {
+            for (int j = threadIdx.x; j < n; j += blockDim.x) {
+                double a = A[(size_t)k * n + j]; A[(size_t)k * n + j] = A[(size_t)piv * n + j]; A[(size_t)piv * n + j] = a;
+                double b = Inv[(size_t)k * n + j]; Inv[(size_t)k * n + j] = Inv[(size_t)piv * n + j]; Inv[(size_t)piv * n + j] = b;
+            }
+        }
+        __syncthreads();
+        const double inv_p = 1.0 / A[(size_t)k * n + k];
+        __syncthreads();
+        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+            A[(size_t)k * n + j] *= inv_p;
+            Inv[(size_t)k * n + j] *= inv_p;
+        }
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
+            const int i = idx / n, j = idx % n;
+            if (i == k) continue;
+            const double f = A[(size_t)i * n + k];
+            if (j != k) A[(size_t)i * n + j] -= f * A[(size_t)k * n + j];
+            Inv[(size_t)i * n + j] -= f * Inv[(size_t)k * n + j];
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += blockDim.x)
+            if (i != k) A[(size_t)i * n + k] = 0.0;
+        __syncthreads();
+    }
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Structural symmetry test and, for non-symmetric patterns, the transpose pattern.
+void ensure_transpose(fvm_amg *h, Level &L) {
+    if (L.sym >= 0) return;
+    hipStream_t s = h->stream;
+    const int n = L.n;
+    if (n == 0) { L.sym = 1; return; }
+    DArr<int> sflag(1);
+    FVM_HIP(hipMemsetAsync(sflag.p, 0, sizeof(int), s));
+    k_sym_check<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.row.p, L.col.p, sflag.p);
+    L.sym = read_scalar(sflag.p, s) == 0 ? 1 : 0;
+    if (L.sym) return;
+    DArr<int> cnt((size_t)n + 1), fill((size_t)n + 1);
+    FVM_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes(), s));
+    FVM_HIP(hipMemsetAsync(fill.p, 0, fill.bytes(), s));
+    k_tr_count<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.row.p, L.col.p, cnt.p);
+    L.rowT.alloc((size_t)n + 1);
+    h->ws.scan.exclusive(cnt.p, L.rowT.p, n + 1, s);
+    L.colT.alloc(std::max<long long>(L.nnz, 1));
+    k_tr_fill<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.row.p, L.col.p, L.rowT.p, fill.p, L.colT.p);
+    FVM_HIP(hipGetLastError());
+}
+
+// Agglomerate level L (reference order); fills L.ci, L.nc and returns the rounds used.
+int coarsen_level(fvm_amg *h, Level &L, double theta) {
+    hipStream_t s = h->stream;
+    const int n = L.n;
+    ensure_transpose(h, L);
+    const int sym = L.sym;
+    const int *rowT = sym ? L.row.p : L.rowT.p;
+    const int *colT = sym ? L.col.p : L.colT.p;
+    DArr<int> ldr(n), dround(n), mem((size_t)n * MEMW), memr((size_t)n * MEMW), mcnt(n),
+        stamp(n), lists(2 * (size_t)n + 2), ctr(8);
+    FVM_HIP(hipMemsetAsync(ctr.p, 0, ctr.bytes(), s));
+    int *cnt = ctr.p;            // cnt[0..2] rotating candidate counters
+    int *remaining = ctr.p + 3;
+    const uint8_t *bnd = L.bnd.p;
+    k_coarsen_init<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, bnd, ldr.p, dround.p, memr.p, mcnt.p,
+                                                       stamp.p, remaining);
+    FVM_HIP(hipGetLastError());
+    const unsigned g_all = grid_for(n, BS);
+    const unsigned g_round = grid_for(n, BS, 2048);
+    int t = 1;
+    const int batch = 16;
+    const long long cap = 4LL * n + 64;
+    for (;;) {
+        for (int k = 0; k < batch; k++, t++) {
+            const bool first = (t == 1);
+            k_coarsen_round<<<first ? g_all : g_round, BS, 0, s>>>(
+                t, first ? 1 : 0, n, L.row.p, L.col.p, rowT, colT, sym, L.off.p, L.diag.p, bnd,
+                theta, ldr.p, dround.p, mem.p, memr.p, mcnt.p, stamp.p,
+                lists.p + (size_t)(t & 1) * n, cnt + (t % 3), lists.p + (size_t)((t + 1) & 1) * n,
+                cnt + ((t + 1) % 3), cnt + ((t + 2) % 3), remaining);
+        }
+        FVM_HIP(hipGetLastError());
+        const int rem = read_scalar(remaining, s);
+        if (rem == 0) break;
+        FVM_CHECK(t < cap, FVM_E_NUMERIC, "coarsening did not terminate");
+    }
+    // leaders -> contiguous ids in seed order
+    DArr<int> flag((size_t)n + 1), gid((size_t)n + 1);
+    k_leader_flags<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, ldr.p, bnd, flag.p);
+    h->ws.scan.exclusive(flag.p, gid.p, n + 1, s);
+    L.ci.alloc(n);
+    k_assign_ci<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, ldr.p, bnd, gid.p, L.ci.p);
+    FVM_HIP(hipGetLastError());
+    L.nc = read_scalar(gid.p + n, s);
+    return t - 1;
+}
+
+void coarse_to_fine(fvm_amg *h, Level &L) {
+    hipStream_t s = h->stream;
+    const int n = L.n, nc = L.nc;
+    DArr<int> cnt((size_t)nc + 1), fill((size_t)nc + 1);
+    FVM_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes(), s));
+    FVM_HIP(hipMemsetAsync(fill.p, 0, fill.bytes(), s));
+    k_c2f_count<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.ci.p, cnt.p);
+    L.c2f_row.alloc((size_t)nc + 1);
+    h->ws.scan.exclusive(cnt.p, L.c2f_row.p, nc + 1, s);
+    L.c2f_col.alloc(std::max(n, 1));
+    k_c2f_fill<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.ci.p, L.c2f_row.p, fill.p, L.c2f_col.p);
+    k_segsort<<<grid_for(nc, BS, 4096), BS, 0, s>>>(nc, L.c2f_row.p, L.c2f_col.p);
+    FVM_HIP(hipGetLastError());
+}
+
+void coarse_operator(fvm_amg *h, Level &F, Level &C) {
+    hipStream_t s = h->stream;
+    const int nc = F.nc;
+    DArr<int> ub((size_t)nc + 1), uboff((size_t)nc + 1), cnt((size_t)nc + 1);
+    k_cc_ub<<<grid_for(nc, BS, 4096), BS, 0, s>>>(nc, F.c2f_row.p, F.c2f_col.p, F.row.p, ub.p);
+    h->ws.scan.exclusive(ub.p, uboff.p, nc + 1, s);
+    DArr<int> scratch(std::max<long long>(F.nnz, 1));
+    k_cc_list<<<grid_for(nc, BS, 4096), BS, 0, s>>>(nc, F.c2f_row.p, F.c2f_col.p, F.row.p, F.col.p,
+                                                    F.ci.p, uboff.p, scratch.p, cnt.p);
+    C.n = nc;
+    C.row.alloc((size_t)nc + 1);
+    h->ws.scan.exclusive(cnt.p, C.row.p, nc + 1, s);
+    C.nnz = read_scalar(C.row.p + nc, s);
+    C.col.alloc(std::max<long long>(C.nnz, 1));
+    C.off.alloc(std::max<long long>(C.nnz, 1));
+    C.diag.alloc(std::max(nc, 1));
+    k_cc_galerkin<<<grid_for(nc, BS, 4096), BS, 0, s>>>(
+        nc, F.c2f_row.p, F.c2f_col.p, F.row.p, F.col.p, F.diag.p, F.off.p, F.ci.p, uboff.p,
+        scratch.p, C.row.p, C.col.p, C.diag.p, C.off.p);
+    FVM_HIP(hipGetLastError());
+}
+
+void alloc_vectors(Level &L) {
+    const size_t n = std::max(L.n, 1);
+    L.b.alloc(n); L.x.alloc(n); L.r.alloc(n); L.t.alloc(n);
+}
+
+}  // namespace
+
+// Build the reference hierarchy (AMG::createCoarseLevels, AMG.cpp:149-210) from a
+// device CSR of the finest level; ghost rows/columns are dropped first.
+void build_hierarchy(fvm_amg *h, int n_self, int n_total, const int *d_row, const int *d_col,
+                     const double *d_diag, const double *d_off, const uint8_t *d_bnd) {
+    const double t0 = now_s();
+    hipStream_t s = h->stream;
+    FVM_CHECK(n_self >= 0 && n_total >= n_self, FVM_E_INVALID, "need 0 <= n_self <= n_total");
+    FVM_CHECK(h->prm.coarseGroupSize == 2 || n_self == 0, FVM_E_INVALID,
+              "the GPU agglomeration supports coarseGroupSize == 2 only");
+    h->lev.clear();
+    h->ready = false;
+    h->direct_level = -1;
+    h->n_self = n_self;
+    h->n_total = n_total;
+    const long long nnz_in = n_total > 0 ? (long long)read_scalar(d_row + n_total, s) : 0;
+    {
+        DArr<int> flag(1);
+        FVM_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+        if (n_total > 0) {
+            k_validate<<<grid_for(n_total, BS, 4096), BS, 0, s>>>(n_self, n_total, nnz_in, d_row,
+                                                                  d_col, d_diag, flag.p);
+            FVM_HIP(hipGetLastError());
+        }
+        const int f = read_scalar(flag.p, s);
+        FVM_CHECK(!(f & 1), FVM_E_INVALID, "row pointers are not a valid CSR");
+        FVM_CHECK(!(f & 2), FVM_E_INVALID, "column index out of range or on the diagonal");
+        FVM_CHECK(!(f & 4), FVM_E_NUMERIC, "zero or non-finite diagonal on a self row");
+    }
+    h->lev.emplace_back();
+    Level &F = h->lev[0];
+    F.n = n_self;
+    {
+        DArr<int> cnt((size_t)n_self + 1);
+        FVM_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes(), s));
+        if (n_self > 0)
+            k_compact_count<<<grid_for(n_self, BS, 4096), BS, 0, s>>>(n_self, d_row, d_col, cnt.p);
+        F.row.alloc((size_t)n_self + 1);
+        h->ws.scan.exclusive(cnt.p, F.row.p, n_self + 1, s);
+        F.nnz = read_scalar(F.row.p + n_self, s);
+        F.col.alloc(std::max<long long>(F.nnz, 1));
+        F.off.alloc(std::max<long long>(F.nnz, 1));
+        F.diag.alloc(std::max(n_self, 1));
+        if (n_self > 0) {
+            k_compact_fill<<<grid_for(n_self, BS, 4096), BS, 0, s>>>(n_self, d_row, d_col, d_off,
+                                                                     F.row.p, F.col.p, F.off.p);
+            FVM_HIP(hipMemcpyAsync(F.diag.p, d_diag, sizeof(double) * n_self,
+                                   hipMemcpyDeviceToDevice, s));
+        }
+        if (d_bnd && n_self > 0) {
+            F.bnd.alloc(n_self);
+            FVM_HIP(hipMemcpyAsync(F.bnd.p, d_bnd, n_self, hipMemcpyDeviceToDevice, s));
+        }
+        FVM_HIP(hipGetLastError());
+    }
+    int fine_size = n_total;  // getLocalSize() of the finest level counts ghosts
+    for (int k = 0; k < h->prm.maxCoarseLevels && n_self > 0; k++) {
+        Level &A = h->lev.back();
+        const int rounds = coarsen_level(h, A, h->prm.weightRatioThreshold);
+        if (k == 0) h->finest_rounds = rounds;
+        const int nc = A.nc;
+        if (nc == fine_size) { A.ci.release(); A.nc = 0; break; }       // AMG.cpp:163-168
+        if (!h->prm.parallelStopRule && nc <= 3) { A.ci.release(); A.nc = 0; break; }
+        coarse_to_fine(h, A);
+        Level C;
+        coarse_operator(h, A, C);
+        h->lev.push_back(std::move(C));
+        if (h->prm.verbosity > 1)
+            printf("Created coarse level %d of size %d\n", k, nc);
+        fine_size = nc;
+        if (h->prm.parallelStopRule && nc <= 3) break;                  // AMG.cpp:180-181
+    }
+    for (auto &L : h->lev) alloc_vectors(L);
+    h->ws.partials.ensure(4096);
+    h->ws.scal.ensure(64);
+    FVM_HIP(hipStreamSynchronize(s));
+    h->ready = true;
+    // mode-specific preparation
+    if (h->prm.smootherType == FVM_GAUSS_SEIDEL && h->prm.gsOrdering == FVM_GS_EXACT)
+        for (auto &L : h->lev) prepare_exact_schedule(h, L);
+    if (h->prm.gsOrdering == FVM_GS_MULTICOLOR) prepare_fast(h);
+    if (h->prm.precision == FVM_PREC_MIXED) {
+        prepare_mixed(h, h->prm.gsOrdering == FVM_GS_MULTICOLOR);
+        prepare_direct(h);
+    }
+    FVM_HIP(hipStreamSynchronize(s));
+    h->setup_seconds = now_s() - t0;
+}
+
+void prepare_exact_schedule(fvm_amg *h, Level &L) {
+    if (L.sched_ready) return;
+    hipStream_t s = h->stream;
+    const int n = L.n;
+    if (n == 0) { L.sched_ready = true; return; }
+    ensure_transpose(h, L);
+    for (int dir = 0; dir < 2; dir++) {
+        DArr<int> indeg(n), ctr(2);
+        DArr<int> &rows = dir ? L.bwd_rows : L.fwd_rows;
+        DArr<int> &ptr = dir ? L.bwd_ptr : L.fwd_ptr;
+        rows.alloc(n);
+        const int batch = 32;
+        ptr.alloc((size_t)n + batch + 4);
+        FVM_HIP(hipMemsetAsync(ctr.p, 0, ctr.bytes(), s));
+        int *tail = ctr.p;
+        unsigned *done_blocks = (unsigned *)(ctr.p + 1);
+        const int *rT = L.sym ? nullptr : L.rowT.p;
+        const int *cT = L.sym ? nullptr : L.colT.p;
+        k_sched_init<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, dir, L.row.p, L.col.p, rT, cT, indeg.p,
+                                                          rows.p, tail);
+        k_set_ptr01<<<1, 1, 0, s>>>(ptr.p, tail);
+        FVM_HIP(hipGetLastError());
+        int lvl = 0;
+        const unsigned g = grid_for(n, BS, 1024);
+        for (;;) {
+            for (int k = 0; k < batch; k++, lvl++) {
+                FVM_CHECK((size_t)lvl + 2 < ptr.n, FVM_E_NUMERIC, "GS schedule overflow");
+                k_sched_round<<<g, BS, 0, s>>>(lvl, dir, L.row.p, L.col.p, rT, cT, indeg.p, rows.p,
+                                               ptr.p, tail, done_blocks);
+            }
+            FVM_HIP(hipGetLastError());
+            if (read_scalar(tail, s) >= n) break;
+            FVM_CHECK(lvl <= n + 1, FVM_E_NUMERIC, "GS schedule did not terminate");
+            // grow the pointer array if the schedule is deep
+            if ((size_t)lvl + batch + 3 >= ptr.n) {
+                DArr<int> np(ptr.n * 2);
+                FVM_HIP(hipMemcpyAsync(np.p, ptr.p, ptr.bytes(), hipMemcpyDeviceToDevice, s));
+                ptr = std::move(np);
+            }
+        }
+        std::vector<int> hp(lvl + 2);
+        FVM_HIP(hipMemcpyAsync(hp.data(), ptr.p, sizeof(int) * (lvl + 2), hipMemcpyDeviceToHost, s));
+        FVM_HIP(hipStreamSynchronize(s));
+        int nl = 0, maxw = 0;
+        while (nl + 1 < (int)hp.size() && hp[nl] < n) { maxw = std::max(maxw, hp[nl + 1] - hp[nl]); nl++; }
+        hp.resize(nl + 1);
+        if (dir) { L.bwd_nlev = nl; L.bwd_maxw = maxw; L.h_bwd_ptr = hp; }
+        else { L.fwd_nlev = nl; L.fwd_maxw = maxw; L.h_fwd_ptr = hp; }
+    }
+    L.sched_ready = true;
+}
+
+// Graph colouring and coloured SELL-64 layout of every level.
+void prepare_fast(fvm_amg *h) {
+    hipStream_t s = h->stream;
+    const int nl = (int)h->lev.size();
+    bool all = true;
+    for (auto &L : h->lev) all = all && L.fast_ready;
+    if (all) return;
+    for (int l = 0; l < nl; l++) {
+        Level &L = h->lev[l];
+        const int n = L.n;
+        if (n == 0) continue;
+        ensure_transpose(h, L);
+        const int *rT = L.sym ? nullptr : L.rowT.p;
+        const int *cT = L.sym ? nullptr : L.colT.p;
+        DArr<int> color(n), cround(n), left(1);
+        FVM_HIP(hipMemsetAsync(color.p, 0, color.bytes(), s));
+        k_fill_int<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, cround.p, INT_MAX);
+        k_fill_int<<<1, 1, 0, s>>>(1, left.p, n);
+        int t = 1;
+        for (;;) {
+            for (int k = 0; k < 8; k++, t++)
+                k_color_round<<<grid_for(n, BS, 4096), BS, 0, s>>>(t, n, L.row.p, L.col.p, rT, cT,
+                                                                   color.p, cround.p, left.p);
+            FVM_HIP(hipGetLastError());
+            if (read_scalar(left.p, s) == 0) break;
+            FVM_CHECK(t < 4 * n + 64, FVM_E_NUMERIC, "colouring did not terminate");
+        }
+        // stable sort of rows by colour -> perm
+        DArr<int> idx(n), keys_out(n);
+        L.perm.alloc(n);
+        L.iperm.alloc(n);
+        k_iota<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, idx.p);
+        size_t bytes = 0;
+        FVM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, color.p, keys_out.p, idx.p,
+                                                   L.perm.p, n, 0, 9, s));
+        h->ws.sort_tmp.ensure(bytes + 16);
+        FVM_HIP(hipcub::DeviceRadixSort::SortPairs(h->ws.sort_tmp.p, bytes, color.p, keys_out.p,
+                                                   idx.p, L.perm.p, n, 0, 9, s));
+        k_invert_perm<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.perm.p, L.iperm.p);
+        DArr<int> hist(257);
+        FVM_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), s));
+        k_color_hist<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, color.p, hist.p);
+        std::vector<int> hh(257);
+        FVM_HIP(hipMemcpyAsync(hh.data(), hist.p, hist.bytes(), hipMemcpyDeviceToHost, s));
+        FVM_HIP(hipStreamSynchronize(s));
+        L.color_ptr.assign(1, 0);
+        for (int c = 0; c < 257 && L.color_ptr.back() < n; c++) L.color_ptr.push_back(L.color_ptr.back() + hh[c]);
+        L.ncolor = (int)L.color_ptr.size() - 1;
+        // SELL-64
+        const int nslice = (n + 63) / 64;
+        DArr<int> w64((size_t)nslice + 1);
+        k_sell_width<<<grid_for(nslice, BS, 4096), BS, 0, s>>>(n, nslice, L.perm.p, L.row.p, w64.p);
+        L.sell_off.alloc((size_t)nslice + 1);
+        h->ws.scan.exclusive(w64.p, L.sell_off.p, nslice + 1, s);
+        const int total = read_scalar(L.sell_off.p + nslice, s);
+        L.sell_col.alloc(std::max(total, 1));
+        L.sell_val.alloc(std::max(total, 1));
+        L.pdiag.alloc(n);
+        k_sell_fill<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.perm.p, L.iperm.p, L.row.p, L.col.p,
+                                                         L.off.p, L.diag.p, L.sell_off.p,
+                                                         L.sell_col.p, L.sell_val.p, L.pdiag.p);
+        k_sell_tail<<<1, 64, 0, s>>>(n, nslice, L.sell_off.p, L.sell_col.p, L.sell_val.p);
+        FVM_HIP(hipGetLastError());
+    }
+    for (int l = 0; l + 1 < nl; l++) {
+        Level &L = h->lev[l];
+        Level &C = h->lev[l + 1];
+        const int n = L.n, nc = C.n;
+        L.pci.alloc(std::max(n, 1));
+        k_perm_ci<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.perm.p, L.ci.p, C.iperm.p, L.pci.p);
+        DArr<int> cnt((size_t)nc + 1), fill((size_t)nc + 1);
+        FVM_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes(), s));
+        FVM_HIP(hipMemsetAsync(fill.p, 0, fill.bytes(), s));
+        k_c2f_count<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.pci.p, cnt.p);
+        L.pc2f_row.alloc((size_t)nc + 1);
+        h->ws.scan.exclusive(cnt.p, L.pc2f_row.p, nc + 1, s);
+        L.pc2f_col.alloc(std::max(n, 1));
+        k_c2f_fill<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.pci.p, L.pc2f_row.p, fill.p,
+                                                        L.pc2f_col.p);
+        k_segsort<<<grid_for(nc, BS, 4096), BS, 0, s>>>(nc, L.pc2f_row.p, L.pc2f_col.p);
+        FVM_HIP(hipGetLastError());
+    }
+    for (auto &L : h->lev) L.fast_ready = true;
+    FVM_HIP(hipStreamSynchronize(s));
+}
+
+// FP32 values on the finest level, BF16 off-diagonals (FP32 diagonal) on coarse levels.
+void prepare_mixed(fvm_amg *h, bool fast) {
+    hipStream_t s = h->stream;
+    for (size_t l = 0; l < h->lev.size(); l++) {
+        Level &L = h->lev[l];
+        const int n = std::max(L.n, 1);
+        if (!L.b32.p) { L.b32.alloc(n); L.x32.alloc(n); L.r32.alloc(n); L.t32.alloc(n); }
+        if (!fast && !L.mixed_ready_exact) {
+            L.diag32.alloc(n);
+            k_to_f32<<<grid_for(L.n, BS, 4096), BS, 0, s>>>(L.n, L.diag.p, L.diag32.p);
+            if (l == 0) {
+                L.off32.alloc(std::max<long long>(L.nnz, 1));
+                k_to_f32<<<grid_for(L.nnz, BS, 4096), BS, 0, s>>>(L.nnz, L.off.p, L.off32.p);
+            } else {
+                L.off16.alloc(std::max<long long>(L.nnz, 1));
+                k_to_bf16<<<grid_for(L.nnz, BS, 4096), BS, 0, s>>>(L.nnz, L.off.p, L.off16.p);
+            }
+            L.mixed_ready_exact = true;
+        }
+        if (fast && !L.mixed_ready_fast && L.n > 0) {
+            const int nslice = (L.n + 63) / 64;
+            const int total = read_scalar(L.sell_off.p + nslice, s);
+            L.pdiag32.alloc(n);
+            k_to_f32<<<grid_for(L.n, BS, 4096), BS, 0, s>>>(L.n, L.pdiag.p, L.pdiag32.p);
+            if (l == 0) {
+                L.sell_val32.alloc(std::max(total, 1));
+                k_to_f32<<<grid_for(total, BS, 4096), BS, 0, s>>>(total, L.sell_val.p, L.sell_val32.p);
+            } else {
+                L.sell_bf16.alloc(std::max(total, 1));
+                k_to_bf16<<<grid_for(total, BS, 4096), BS, 0, s>>>(total, L.sell_val.p, L.sell_bf16.p);
+            }
+            L.mixed_ready_fast = true;
+        }
+        FVM_HIP(hipGetLastError());
+    }
+    FVM_HIP(hipStreamSynchronize(s));
+}
+
+// FP64 inverse of the first coarse level with at most 256 rows (coarsest solve of the
+// mixed-precision cycle). Falls back to smoothing if that matrix is singular.
+void prepare_direct(fvm_amg *h) {
+    hipStream_t s = h->stream;
+    h->direct_level = -1;
+    int lv = -1;
+    for (size_t l = 1; l < h->lev.size(); l++)
+        if (h->lev[l].n <= 256 && h->lev[l].n > 0) { lv = (int)l; break; }
+    if (lv < 0) return;
+    Level &L = h->lev[lv];
+    const int n = L.n;
+    DArr<double> A((size_t)n * n);
+    h->dense_inv.alloc((size_t)n * n);
+    DArr<int> st(1);
+    FVM_HIP(hipMemsetAsync(st.p, 0, sizeof(int), s));
+    k_dense_from_csr<<<1, 256, 0, s>>>(n, L.row.p, L.col.p, L.diag.p, L.off.p, A.p);
+    k_gauss_jordan<<<1, 1024, 0, s>>>(n, A.p, h->dense_inv.p, st.p);
+    FVM_HIP(hipGetLastError());
+    if (read_scalar(st.p, s) == 0) h->direct_level = lv;
+}
+
+}  // namespace fvm

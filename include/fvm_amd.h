@@ -1,0 +1,191 @@
+/*
+ * fvm_amd -- MI355X (gfx950) algebraic-multigrid linear solver for finite-volume
+ * CRMatrix systems. C ABI: plain pointers and sizes, int status codes, no C++
+ * types and no exceptions across the boundary.
+ *
+ * Reference interfaces replaced (paths relative to src/fvm/src/modules/fvmbase/
+ * of c-PRIMED/fvm):
+ *   class LinearSolver  LinearSolver.h:15-35   (solve / smooth / cleanup, tolerances)
+ *   class AMG           AMG.h:27-110, AMG.cpp:15-337 (parameters, cycle, hierarchy)
+ *   SWIG AMG binding    AMG.i:1-34
+ *   class BCGStab       BCGStab.h / BCGStab.cpp:27-171 (Krylov loop around AMG::smooth)
+ *
+ * Conventions (same as the reference):
+ *   - The system is stored as CSR with the diagonal kept apart: row[n_total+1],
+ *     col[nnz] (never contains the row itself), off[nnz], diag[n_total]
+ *     (CRMatrix.h:73-84).
+ *   - Rows [0, n_self) are unknowns; rows [n_self, n_total) are ghost rows that
+ *     the solver never updates (StorageSite self/ghost counts).
+ *   - Delta form: the residual is r = b + A x and the solver drives it to zero,
+ *     i.e. it solves A x = -b (CRMatrix.h:407-426).
+ *   - Convergence uses the 1-norm of r over self rows; solve() returns the
+ *     initial norm r0 through *rnorm0 (AMG.cpp:235,281).
+ *
+ * Every entry point returns FVM_OK (0) on success or a negative FVM_E_* code;
+ * fvm_last_error() returns a thread-local message for the last failure.
+ * A handle is not thread-safe; it owns one HIP stream and all device memory of
+ * its hierarchy until fvm_amg_cleanup() or fvm_amg_destroy(). Host pointers
+ * passed in are only read (or written) during the call.
+ */
+#ifndef FVM_AMD_H
+#define FVM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FVM_AMD_VERSION 10000 /* 1.0.0 */
+
+enum {
+    FVM_OK = 0,
+    FVM_E_INVALID = -1,     /* bad argument or unsupported parameter value */
+    FVM_E_HIP = -2,         /* a HIP runtime call failed */
+    FVM_E_NOT_SETUP = -3,   /* solve/smooth before setup */
+    FVM_E_NUMERIC = -4,     /* zero / non-finite diagonal, breakdown */
+    FVM_E_NOMEM = -5
+};
+
+/* AMG::CycleType, AMG.h:31-36 */
+enum { FVM_V_CYCLE = 0, FVM_W_CYCLE = 1, FVM_F_CYCLE = 2 };
+/* AMG::SmootherType, AMG.h:38-42 */
+enum { FVM_GAUSS_SEIDEL = 0, FVM_JACOBI = 1 };
+/* Gauss-Seidel row order (no counterpart in the reference, which is sequential) */
+enum {
+    FVM_GS_EXACT = 0,      /* reference row order, level-scheduled: bit-identical to the reference */
+    FVM_GS_MULTICOLOR = 1  /* graph-coloured order: a different (parallel) Gauss-Seidel */
+};
+/* arithmetic of the V-cycle (no counterpart in the reference, which is FP64 only) */
+enum {
+    FVM_PREC_FP64 = 0,     /* FP64 values and vectors on every level */
+    FVM_PREC_MIXED = 1     /* FP32 vectors, FP32 finest values, BF16 coarse off-diagonals;
+                              the outer residual stays FP64 (iterative refinement) */
+};
+
+/*
+ * Parameters. Names, meanings and defaults are those of LinearSolver
+ * (LinearSolver.h:19-24) and AMG (AMG.cpp:15-36); the fields after
+ * scaleCorrections are extensions.
+ */
+typedef struct fvm_amg_params {
+    int maxCoarseLevels;          /* 30 */
+    int nPreSweeps;               /* 0 */
+    int nPostSweeps;              /* 1 */
+    int coarseGroupSize;          /* 2 (the only value the GPU coarsening accepts) */
+    double weightRatioThreshold;  /* 0.65 */
+    int cycleType;                /* FVM_V_CYCLE */
+    int smootherType;             /* FVM_GAUSS_SEIDEL */
+    int scaleCorrections;         /* 1; active only when isSymmetric (AMG.cpp:119) */
+    int nMaxIterations;           /* 100: at most nMaxIterations-1 cycles (AMG.cpp:250) */
+    int verbosity;                /* 2 in the reference; >0 prints "i: norm" lines */
+    double relativeTolerance;     /* 1e-8 */
+    double absoluteTolerance;     /* 1e-50 */
+    /* extensions */
+    int isSymmetric;              /* LinearSystem::isSymmetric (MMReader sets it for symmetric files) */
+    int parallelStopRule;         /* 1: keep a coarse level of size <= 3 then stop (FVM_PARALLEL
+                                     build, AMG.cpp:170-181); 0: serial build rule (AMG.cpp:199-204) */
+    int gsOrdering;               /* FVM_GS_EXACT */
+    int precision;                /* FVM_PREC_FP64 */
+} fvm_amg_params;
+
+typedef struct fvm_amg fvm_amg;
+
+/* Version as MAJOR*10000 + MINOR*100 + PATCH. */
+int fvm_version(void);
+/* Thread-local text of the last error ("" if none). */
+const char *fvm_last_error(void);
+
+/* Fill *p with the reference defaults. */
+int fvm_amg_default_params(fvm_amg_params *p);
+
+/* AMG::AMG() (AMG.cpp:15-36). device = HIP device ordinal. */
+int fvm_amg_create(fvm_amg **out, int device);
+/* AMG::~AMG() */
+int fvm_amg_destroy(fvm_amg *h);
+int fvm_amg_set_params(fvm_amg *h, const fvm_amg_params *p);
+int fvm_amg_get_params(const fvm_amg *h, fvm_amg_params *p);
+
+/*
+ * Build the hierarchy (AMG::createCoarseLevels, AMG.cpp:149-210) from a host CSR
+ * matrix. is_boundary may be NULL (no boundary rows). Replaces the first-call
+ * branch of AMG::solve / AMG::smooth (AMG.cpp:222-226).
+ */
+int fvm_amg_setup(fvm_amg *h, int n_self, int n_total, const int32_t *row, const int32_t *col,
+                  const double *diag, const double *off, const uint8_t *is_boundary);
+/* Same, with all arrays already in device memory of this handle's device. */
+int fvm_amg_setup_device(fvm_amg *h, int n_self, int n_total, const int32_t *d_row,
+                         const int32_t *d_col, const double *d_diag, const double *d_off,
+                         const uint8_t *d_is_boundary);
+
+/*
+ * AMG::solve (AMG.cpp:219-282): b and x have n_total entries (host memory);
+ * x holds the initial delta and receives the result on self rows (ghost
+ * entries are left unchanged). *rnorm0 receives the initial 1-norm (the
+ * reference's return value), *rnorm_final the last computed one, *cycles the
+ * number of cycles run. Non-convergence is not an error.
+ */
+int fvm_amg_solve(fvm_amg *h, const double *b, double *x, double *rnorm0, double *rnorm_final,
+                  int *cycles);
+/* Same with device pointers; runs on the handle's stream and synchronizes it. */
+int fvm_amg_solve_device(fvm_amg *h, const double *d_b, double *d_x, double *rnorm0,
+                         double *rnorm_final, int *cycles);
+/* AMG::smooth (AMG.cpp:285-298): exactly one cycle from the given x. */
+int fvm_amg_smooth(fvm_amg *h, const double *b, double *x);
+int fvm_amg_smooth_device(fvm_amg *h, const double *d_b, double *d_x);
+/* AMG::cleanup (AMG.cpp:212-217): drop the hierarchy. */
+int fvm_amg_cleanup(fvm_amg *h);
+/* AMG::getTotalIterations (AMG.h:69) */
+long long fvm_amg_total_iterations(const fvm_amg *h);
+/* Residual 1-norms of the last solve: hist[0] = r0, hist[i] after cycle i.
+ * Returns the number of entries written (<= cap). */
+int fvm_amg_history(const fvm_amg *h, double *hist, int cap);
+
+/* Hierarchy introspection (the reference prints level sizes at verbosity > 1,
+ * AMG.cpp:176-178). Level 0 is the finest (self rows only, ghost columns removed). */
+int fvm_amg_num_levels(const fvm_amg *h);
+int fvm_amg_level_info(const fvm_amg *h, int level, int *n_rows, int *nnz);
+/* Copy a level to host arrays (any may be NULL): CSR in reference order and the
+ * fine-to-coarse map ci[n_rows] (-1 for rows that join no group; all -1 on the
+ * coarsest level). */
+int fvm_amg_level_get(const fvm_amg *h, int level, int32_t *row, int32_t *col, double *diag,
+                      double *off, int32_t *ci);
+/* Setup statistics: seconds spent in setup, number of coarsening rounds of the finest level. */
+int fvm_amg_setup_stats(const fvm_amg *h, double *setup_seconds, int *finest_rounds);
+
+/* ---------------------------------------------------------------------------
+ * FP64 flexible GMRES(m) with the AMG handle's cycle as right preconditioner.
+ * No reference counterpart for FGMRES itself; it takes the place of the
+ * reference's Krylov loops around AMG::smooth (BCGStab.cpp:27-171) and keeps
+ * LinearSolver's fields and stop rule (1-norm of the FP64 true residual
+ * r = b + A x relative to r0).
+ * ------------------------------------------------------------------------- */
+typedef struct fvm_fgmres_params {
+    int restart;                  /* 30 */
+    int nMaxIterations;           /* 1000 (total inner iterations) */
+    int verbosity;                /* 0 */
+    double relativeTolerance;     /* 1e-8 */
+    double absoluteTolerance;     /* 1e-50 */
+} fvm_fgmres_params;
+
+typedef struct fvm_fgmres fvm_fgmres;
+
+int fvm_fgmres_default_params(fvm_fgmres_params *p);
+/* The preconditioner handle must have been set up; it is borrowed, not owned. */
+int fvm_fgmres_create(fvm_fgmres **out, fvm_amg *preconditioner);
+int fvm_fgmres_destroy(fvm_fgmres *s);
+int fvm_fgmres_set_params(fvm_fgmres *s, const fvm_fgmres_params *p);
+/* Solve A x = -b (delta form) on device pointers of n_self... n_total entries. */
+int fvm_fgmres_solve_device(fvm_fgmres *s, const double *d_b, double *d_x, double *rnorm0,
+                            double *rnorm_final, int *iterations);
+int fvm_fgmres_solve(fvm_fgmres *s, const double *b, double *x, double *rnorm0,
+                     double *rnorm_final, int *iterations);
+/* True-residual 1-norms of the last solve: hist[0] = r0, then one per restart
+ * cycle. Returns the number of entries written (<= cap). */
+int fvm_fgmres_history(const fvm_fgmres *s, double *hist, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FVM_AMD_H */

@@ -1,0 +1,287 @@
+"""GPU parity of the HIP AMG (through the C ABI) against the CPU oracle.
+
+Bar: coarsening maps, coarse matrices and (in the exact GS layout) the
+correction delta are bit-identical to the oracle; residual 1-norms agree to
+1e-12 relative (the GPU sums the norm in a fixed parallel order, the reference
+sequentially) and cycle counts are equal. The coloured Gauss-Seidel layout and
+the mixed-precision cycle are different smoothers: they must reach the same
+tolerance and agree with the oracle's converged solution within 1e-9 relative
+(FP64) or the stated tolerance (mixed).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import matgen
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _fvm():
+    import fvm_amd
+    return fvm_amd
+
+
+def ls_from(m, sym=False, is_bnd=None):
+    f = _fvm()
+    return f.LinearSystem(m["n_self"], m["n_total"], m["row"], m["col"], m["diag"], m["off"], m["b"],
+                          is_boundary=is_bnd, isSymmetric=sym)
+
+
+def oracle_for(m, is_bnd=None, **kw):
+    a = O.OracleAMG(**kw)
+    a.setup(m["n_self"], m["n_total"], m["row"], m["col"], m["diag"], m["off"], is_bnd)
+    return a
+
+
+def gpu_amg(**kw):
+    f = _fvm()
+    a = f.AMG()
+    a.verbosity = 0
+    for k, v in kw.items():
+        setattr(a, k, v)
+    return a
+
+
+def compact_level0(m):
+    """Reference-order CSR of the self rows with ghost columns removed."""
+    n = m["n_self"]
+    row = [0]; col = []; off = []
+    for i in range(n):
+        for p in range(m["row"][i], m["row"][i + 1]):
+            if m["col"][p] < n:
+                col.append(m["col"][p]); off.append(m["off"][p])
+        row.append(len(col))
+    return np.array(row), np.array(col), np.array(off)
+
+
+def check_hierarchy(ga, oa, m):
+    assert ga.num_levels() == oa.num_levels()
+    for l in range(oa.num_levels()):
+        g = ga.level(l)
+        o = oa.level(l)
+        if l == 0:
+            row, col, off = compact_level0(m)
+            assert np.array_equal(g["row"], row)
+            assert np.array_equal(g["col"], col)
+            assert np.array_equal(g["off"], off)
+            assert np.array_equal(g["ci"], o["ci"][: m["n_self"]])
+        else:
+            assert g["n"] == o["n_self"]
+            assert np.array_equal(g["row"], o["row"]), l
+            assert np.array_equal(g["col"], o["col"]), l
+            assert np.array_equal(g["diag"], o["diag"]), l      # bit-identical Galerkin sums
+            assert np.array_equal(g["off"], o["off"]), l
+            assert np.array_equal(g["ci"], o["ci"]), l
+
+
+def run_pair(m, gkw=None, okw=None, sym=False, is_bnd=None):
+    gkw = dict(gkw or {}); okw = dict(okw or {})
+    oa = oracle_for(m, is_bnd=is_bnd, is_symmetric=int(sym), **okw)
+    ores = oa.solve(m["b"])
+    ga = gpu_amg(**gkw)
+    ls = ls_from(m, sym=sym, is_bnd=is_bnd)
+    r0 = ga.solve(ls)
+    return ga, oa, ls, r0, ores
+
+
+def test_kat226_exact():
+    m = O.load_mm(os.path.join(G, "MatrixMarket226.dat"), os.path.join(G, "rhs226.dat"))
+    ga, oa, ls, r0, ores = run_pair(m)
+    assert [ga.level(l)["n"] for l in range(1, ga.num_levels())] == [108, 48, 20, 8, 3]
+    check_hierarchy(ga, oa, m)
+    np.testing.assert_allclose(r0, ores["r0"], rtol=1e-12)   # parallel vs sequential 1-norm sum
+    assert ga.lastCycles == ores["cycles"] == 40
+    np.testing.assert_allclose(ga.history(), ores["history"], rtol=1e-12)
+    assert np.array_equal(ls.delta, ores["x"])
+    assert "%.6g" % ga.lastResidual == "5.32223e-05"
+
+
+def test_kat_cav32_exact():
+    m = O.load_mm(os.path.join(G, "cav32_matrix_mesh0.mat"), os.path.join(G, "cav32_matrix.rhs"))
+    kw = dict(relativeTolerance=1e-9, maxCoarseLevels=20, nMaxIterations=200000)
+    okw = dict(relative_tolerance=1e-9, max_coarse_levels=20, n_max_iterations=200000)
+    ga, oa, ls, r0, ores = run_pair(m, kw, okw)
+    check_hierarchy(ga, oa, m)
+    assert r0 == 63200.0
+    assert ga.lastCycles == 56 == ores["cycles"]
+    assert np.array_equal(ls.delta, ores["x"])
+    assert "%.6g" % ga.lastResidual == "5.75812e-05"
+
+
+@pytest.mark.parametrize("N,seed,random_coeffs,ghosts", [(12, 0, True, True), (16, 1, True, False),
+                                                          (10, 2, False, True), (20, 3, True, True)])
+def test_grid_exact(N, seed, random_coeffs, ghosts):
+    m = matgen.grid3d(N, seed=seed, random_coeffs=random_coeffs, ghosts=ghosts)
+    kw = dict(nMaxIterations=60)
+    ga, oa, ls, r0, ores = run_pair(m, kw, dict(n_max_iterations=60))
+    check_hierarchy(ga, oa, m)
+    assert ga.lastCycles == ores["cycles"]
+    assert np.array_equal(ls.delta[: m["n_self"]], ores["x"][: m["n_self"]])
+    assert np.array_equal(ls.delta[m["n_self"]:], np.zeros(m["n_total"] - m["n_self"]))  # ghosts untouched
+    np.testing.assert_allclose(ga.history(), ores["history"], rtol=1e-12)
+
+
+def test_boundary_rows_exact():
+    m = matgen.grid3d(10, seed=5)
+    isb = np.zeros(m["n_total"], np.uint8)
+    isb[[3, 50, 51, 400]] = 1
+    ga, oa, ls, r0, ores = run_pair(m, dict(nMaxIterations=40), dict(n_max_iterations=40), is_bnd=isb)
+    check_hierarchy(ga, oa, m)
+    assert np.array_equal(ls.delta, ores["x"])
+
+
+@pytest.mark.parametrize("cycle", [1, 2])
+def test_w_and_f_cycles_exact(cycle):
+    m = matgen.grid3d(10, seed=6)
+    ga, oa, ls, r0, ores = run_pair(m, dict(cycleType=cycle, nMaxIterations=30),
+                                    dict(cycle_type=cycle, n_max_iterations=30))
+    assert ga.lastCycles == ores["cycles"]
+    assert np.array_equal(ls.delta, ores["x"])
+
+
+def test_jacobi_exact_both_layouts():
+    m = matgen.grid3d(10, seed=7)
+    okw = dict(smoother_type=1, n_max_iterations=40)
+    for order in (0, 1):
+        ga, oa, ls, r0, ores = run_pair(m, dict(smootherType=1, gsOrdering=order, nMaxIterations=40), okw)
+        assert ga.lastCycles == ores["cycles"]
+        if order == 0:
+            assert np.array_equal(ls.delta, ores["x"])
+        else:   # same arithmetic per row, different reduction order of the coarse sums only
+            np.testing.assert_allclose(ls.delta, ores["x"], rtol=1e-12, atol=1e-14)
+
+
+def test_pre_sweeps_and_symmetric_scaling_exact():
+    m = matgen.grid3d(9, seed=8, ghosts=False)
+    ga, oa, ls, r0, ores = run_pair(m, dict(nPreSweeps=1, nPostSweeps=2, nMaxIterations=25),
+                                    dict(n_pre_sweeps=1, n_post_sweeps=2, n_max_iterations=25), sym=True)
+    assert ga.lastCycles == ores["cycles"]
+    assert np.array_equal(ls.delta, ores["x"])
+
+
+def test_smooth_is_one_cycle():
+    m = matgen.grid3d(8, seed=9)
+    oa = oracle_for(m)
+    x0 = np.zeros(m["n_total"])
+    xo = oa.smooth(m["b"], x0)
+    ga = gpu_amg()
+    ls = ls_from(m)
+    ga.smooth(ls)
+    assert np.array_equal(ls.delta, xo)
+
+
+def _solution(m):
+    oa = oracle_for(m, relative_tolerance=1e-14, n_max_iterations=5000)
+    return oa.solve(m["b"])["x"][: m["n_self"]]
+
+
+def test_multicolor_reaches_tolerance_and_solution():
+    m = matgen.grid3d(16, seed=10, contrast=1.5)
+    xs = _solution(m)
+    ga = gpu_amg(gsOrdering=1, relativeTolerance=1e-14, nMaxIterations=3000)
+    ls = ls_from(m)
+    ga.solve(ls)
+    h = ga.history()
+    assert h[-1] / h[0] < 1e-13
+    np.testing.assert_allclose(ls.delta[: m["n_self"]], xs, rtol=1e-9, atol=1e-9 * np.abs(xs).max())
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_mixed_precision_refinement(order):
+    m = matgen.grid3d(16, seed=11, contrast=1.5)
+    xs = _solution(m)
+    ga = gpu_amg(precision=1, gsOrdering=order, relativeTolerance=1e-13, nMaxIterations=3000)
+    ls = ls_from(m)
+    ga.solve(ls)
+    h = ga.history()
+    assert h[-1] / h[0] < 1e-13
+    np.testing.assert_allclose(ls.delta[: m["n_self"]], xs, rtol=0, atol=1e-9 * np.abs(xs).max())
+
+
+@pytest.mark.parametrize("prec,order", [(0, 0), (0, 1), (1, 1)])
+def test_fgmres(prec, order):
+    f = _fvm()
+    m = matgen.grid3d(16, seed=12, contrast=2.0)
+    xs = _solution(m)
+    pc = gpu_amg(precision=prec, gsOrdering=order)
+    kr = f.FGMRES(pc)
+    kr.relativeTolerance = 1e-13
+    ls = ls_from(m)
+    r0 = kr.solve(ls)
+    h = kr.history()
+    assert h[0] == r0 and h[-1] / r0 < 1e-13
+    r = ls.residual()
+    assert np.abs(r).sum() / r0 < 1e-12
+    np.testing.assert_allclose(ls.delta[: m["n_self"]], xs, rtol=0, atol=1e-9 * np.abs(xs).max())
+    # fewer preconditioner applications than plain AMG cycles
+    pa = gpu_amg(precision=0, gsOrdering=order, relativeTolerance=1e-13, nMaxIterations=5000)
+    ls2 = ls_from(m)
+    pa.solve(ls2)
+    assert kr.lastIterations < pa.lastCycles
+
+
+def test_errors_are_reported():
+    f = _fvm()
+    from fvm_amd._lib import FvmError, FVM_E_NUMERIC, FVM_E_INVALID, FVM_E_NOT_SETUP
+    m = matgen.grid3d(4, seed=13)
+    bad = dict(m); bad["diag"] = m["diag"].copy(); bad["diag"][5] = 0.0
+    a = gpu_amg()
+    with pytest.raises(FvmError) as e:
+        a.solve(ls_from(bad))
+    assert e.value.code == FVM_E_NUMERIC
+    bad = dict(m); bad["col"] = m["col"].copy(); bad["col"][3] = m["n_total"] + 7
+    with pytest.raises(FvmError) as e:
+        gpu_amg().solve(ls_from(bad))
+    assert e.value.code == FVM_E_INVALID
+    a = gpu_amg(coarseGroupSize=3)
+    with pytest.raises(FvmError) as e:
+        a.solve(ls_from(m))
+    assert e.value.code == FVM_E_INVALID
+    b = np.zeros(m["n_total"]); x = np.zeros(m["n_total"])
+    import ctypes
+    from fvm_amd._lib import lib
+    a = gpu_amg()
+    rc = lib().fvm_amg_solve(a._h, ctypes.c_void_p(b.ctypes.data), ctypes.c_void_p(x.ctypes.data), None, None, None)
+    assert rc == FVM_E_NOT_SETUP
+
+
+def test_small_and_degenerate_systems():
+    f = _fvm()
+    # 1x1 system
+    ls = f.LinearSystem(1, 1, np.array([0, 0]), np.zeros(0, np.int32), np.array([-2.0]), np.zeros(0), np.array([4.0]))
+    a = gpu_amg()
+    r0 = a.solve(ls)
+    assert r0 == 4.0 and ls.delta[0] == 2.0
+    # zero right-hand side: r0 = 0 < absTol fails (0 < 1e-50 is true) -> no cycles
+    m = matgen.grid3d(5, seed=14)
+    m0 = dict(m); m0["b"] = np.zeros(m["n_total"])
+    a = gpu_amg()
+    ls = ls_from(m0)
+    assert a.solve(ls) == 0.0 and a.lastCycles == 0
+    # iteration cap: nMaxIterations-1 cycles
+    a = gpu_amg(nMaxIterations=7, relativeTolerance=1e-30)
+    a.solve(ls_from(m))
+    assert a.lastCycles == 6 and a.getTotalIterations() == 6
+
+
+def test_device_entry_points_match_host():
+    import torch
+    f = _fvm()
+    m = matgen.grid3d(12, seed=15)
+    a = gpu_amg()
+    ls = ls_from(m)
+    a.solve(ls)
+    d = torch.device("cuda:0")
+    t = lambda v, dt: torch.from_numpy(np.ascontiguousarray(v)).to(device=d, dtype=dt)
+    b = gpu_amg()
+    b.setup_device(m["n_self"], m["n_total"], t(m["row"], torch.int32), t(m["col"], torch.int32),
+                   t(m["diag"], torch.float64), t(m["off"], torch.float64))
+    x = torch.zeros(m["n_total"], dtype=torch.float64, device=d)
+    r0, rf, cyc = b.solve_device(t(m["b"], torch.float64), x)
+    torch.cuda.synchronize()
+    assert cyc == a.lastCycles
+    assert np.array_equal(x.cpu().numpy(), ls.delta)
