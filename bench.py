@@ -1,0 +1,249 @@
+#!/usr/bin/env python3
+"""Benchmark: time to a 1e-8 relative residual on the 8M-cell heterogeneous
+diffusion system, and HBM GB/s of the finest-level SpMV / smoother.
+
+Workload (one GPU): 200^3 hexahedral mesh of the unit cube, interior nodes
+jittered by U[-0.2h, 0.2h], conductivity exp(2.5 z) constant in 8^3-cell
+blocks, source U[0,1], T=1 at x=0 and T=0 at x=1, zero flux elsewhere,
+assembled on the GPU (thermal finite-volume discretization). One step = build
+the AMG hierarchy + solve to ||r||_1 / ||r0||_1 < 1e-8 (FP64 true residual)
+with FP64 FGMRES(30) preconditioned by an FP32 V-cycle (BF16 coarse
+off-diagonals, FP64 dense coarsest solve). Reported beside it: the plain FP64
+GPU AMG (coloured Gauss-Seidel) on the same system, the cycle count of the
+reference-order AMG, and the CPU oracle timed on this host.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+SEED = 20261202
+METRIC = "time-to-1e-8 rel. residual, 8M-cell diffusion; SpMV/smoother HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(N, torch):
+    from fvm_amd.thermal import Mesh, assemble
+    t0 = time.perf_counter()
+    mesh = Mesh.hex(N, 0.2, SEED)
+    nt = mesh.n_total
+    k = torch.empty(nt, dtype=torch.float64, device="cuda:0")
+    s = torch.empty(nt, dtype=torch.float64, device="cuda:0")
+    x = torch.zeros(nt, dtype=torch.float64, device="cuda:0")
+    mesh.lognormal_field(k, 2.5, 8, SEED + 1)
+    mesh.uniform_field(s, 0.0, 1.0, SEED + 2)
+    bc_t = [0, 0, 0, 1, 1, 1, 1]
+    bc_v = [0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0]
+    S = assemble(mesh, k, s, x, bc_t, bc_v)
+    torch.cuda.synchronize()
+    return mesh, S, time.perf_counter() - t0
+
+
+def make_amg(gs_order, precision, maxit=100000):
+    import fvm_amd
+    a = fvm_amd.AMG(device=0)
+    a.verbosity = 0
+    a.gsOrdering = gs_order
+    a.precision = precision
+    a.relativeTolerance = 1e-8
+    a.nMaxIterations = maxit
+    return a
+
+
+def setup_on(amg, S):
+    from fvm_amd._lib import check, lib
+    amg._push_params()
+    check(lib().fvm_amg_setup_system(amg._h, S._h))
+
+
+def level_sizes(amg):
+    return [(amg.level(l)["n"], amg.level(l)["nnz"]) for l in range(amg.num_levels())]
+
+
+def kernel_stats(amg, kind):
+    from fvm_amd._lib import lib
+    n = ctypes.c_longlong()
+    ms = ctypes.c_double()
+    lib().fvm_amg_kernel_timing(amg._h, kind, ctypes.byref(n), ctypes.byref(ms))
+    return n.value, ms.value
+
+
+def cpu_baseline(S, cycles_needed):
+    """Oracle (C restatement of the reference AMG, single thread) on the same
+    system: time the hierarchy build and 3 cycles, extrapolate to the number of
+    cycles the reference-order AMG needs for 1e-8."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    h = S.to_host()
+    a = O.OracleAMG(n_max_iterations=4, relative_tolerance=1e-30)
+    t0 = time.perf_counter()
+    a.setup(h["n_self"], h["n_total"], h["row"], h["col"], h["diag"], h["off"], h["is_boundary"])
+    t1 = time.perf_counter()
+    res = a.solve(h["b"])
+    t2 = time.perf_counter()
+    per_cycle = (t2 - t1) / max(res["cycles"], 1)
+    return dict(value=(t1 - t0) + cycles_needed * per_cycle, unit="s", cores=1, kind="port",
+                sample="oracle setup (%.2f s) + %d V-cycles (%.3f s/cycle) on the same 8M-cell system, "
+                       "extrapolated to the %d cycles of the reference-order AMG"
+                       % (t1 - t0, res["cycles"], per_cycle, cycles_needed),
+                setup_s=t1 - t0, per_cycle_s=per_cycle, host_threads=os.cpu_count())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=200, help="cells per direction (200: the 8M-cell workload)")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-exact", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the row-sharded design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import fvm_amd
+    from fvm_amd._lib import lib
+
+    N = args.n
+    log("building %d^3 workload" % N)
+    mesh, S, t_asm = build_workload(N, torch)
+    n_total = S.n_total
+    b = S.device_pointers()["b"]          # right-hand side stays in the system's device memory
+    log("assembled: n=%d n_total=%d nnz=%d in %.2f s" % (S.n_self, n_total, S.nnz, t_asm))
+
+    # ---------------- headline: mixed-precision AMG + FP64 FGMRES, setup every solve
+    amg = make_amg(1, 1)
+    fg = fvm_amd.FGMRES(amg)
+    fg.restart = 30
+    fg.relativeTolerance = 1e-8
+    fg.nMaxIterations = 2000
+
+    def step():
+        x = torch.zeros(n_total, dtype=torch.float64, device="cuda:0")
+        setup_on(amg, S)
+        r0, rf, it = fg.solve_device(b, x)
+        return x, r0, rf, it
+
+    for w in range(args.warmup):
+        x, r0, rf, it = step()
+        log("warmup %d: iterations %d ratio %.3e" % (w, it, rf / r0))
+    lib().fvm_amg_set_kernel_timing(amg._h, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        x, r0, rf, it = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = 1e3 * (t1 - t0) / args.steps
+    st = amg.setup_stats()
+    levels = level_sizes(amg)
+    n0, nnz0 = levels[0]
+    kr = kernel_stats(amg, 0)
+    ks = kernel_stats(amg, 1)
+    lib().fvm_amg_set_kernel_timing(amg._h, 0)
+    log("mixed: %.1f ms/solve (setup %.1f ms), %d FGMRES iterations, ratio %.3e"
+        % (ms_per_step, 1e3 * st["setup_seconds"], it, rf / r0))
+    x_mixed = x
+
+    # ---------------- plain FP64 GPU AMG (coloured GS), same system
+    plain = make_amg(1, 0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    setup_on(plain, S)
+    xp = torch.zeros(n_total, dtype=torch.float64, device="cuda:0")
+    lib().fvm_amg_set_kernel_timing(plain._h, 1)
+    r0p, rfp, cyc_plain = plain.solve_device(b, xp)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    kp_r = kernel_stats(plain, 0)
+    kp_s = kernel_stats(plain, 1)
+    lib().fvm_amg_set_kernel_timing(plain._h, 0)
+    log("plain FP64 AMG: %.3f s, %d cycles, ratio %.3e" % (t_plain, cyc_plain, rfp / r0p))
+    err = (x_mixed[: S.n_self] - xp[: S.n_self]).abs().max().item() / xp[: S.n_self].abs().max().item()
+
+    # ---------------- reference-order AMG: cycle count (the reference's own number)
+    cyc_exact = None
+    t_exact = None
+    if not args.skip_exact:
+        ex = make_amg(0, 0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        setup_on(ex, S)
+        xe = torch.zeros(n_total, dtype=torch.float64, device="cuda:0")
+        r0e, rfe, cyc_exact = ex.solve_device(b, xe)
+        torch.cuda.synchronize()
+        t_exact = time.perf_counter() - t0
+        log("exact-order FP64 AMG: %.3f s, %d cycles" % (t_exact, cyc_exact))
+        del ex
+
+    # ---------------- roofline of the finest-level kernels (algorithmic bytes)
+    bytes_sweep32 = 20 * n0 + 8 * nnz0          # one FP32 GS sweep: diag,b,x r/w,rowptr + col,val
+    bytes_resid64 = 36 * n0 + 12 * nnz0         # FP64 residual / SpMV
+    kernels = {}
+    if kr[0]:
+        kernels["finest_fp64_spmv_residual"] = dict(launches=kr[0], avg_us=1e3 * kr[1] / kr[0],
+                                                    bytes=bytes_resid64,
+                                                    gbs=bytes_resid64 / (kr[1] / kr[0] * 1e-3) / 1e9)
+    if ks[0]:
+        kernels["finest_fp32_gs_sweep"] = dict(launches=ks[0], avg_us=1e3 * ks[1] / ks[0], bytes=bytes_sweep32,
+                                               gbs=bytes_sweep32 / (ks[1] / ks[0] * 1e-3) / 1e9)
+    if kp_s[0]:
+        b64 = 36 * n0 + 12 * nnz0
+        kernels["plain_finest_fp64_gs_sweep"] = dict(launches=kp_s[0], avg_us=1e3 * kp_s[1] / kp_s[0], bytes=b64,
+                                                     gbs=b64 / (kp_s[1] / kp_s[0] * 1e-3) / 1e9)
+    # dominant kernel of the headline solve: the one with the larger total time
+    dom = max([("finest_fp64_spmv_residual", kr[1]), ("finest_fp32_gs_sweep", ks[1])], key=lambda t: t[1])[0]
+    kd = kernels[dom]
+    roofline = dict(bound="hbm", kernel=dom, achieved=round(kd["gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(kd["gbs"] / HBM_PEAK_GBS, 4), traffic=None,
+                    algorithmic_bytes=kd["bytes"], avg_us=round(kd["avg_us"], 2))
+
+    cpu = None
+    if not args.skip_cpu:
+        log("cpu baseline (oracle, 1 thread)")
+        cpu = cpu_baseline(S, cyc_exact if cyc_exact else cyc_plain)
+        log("cpu: %.1f s estimated time-to-1e-8" % cpu["value"])
+
+    out = {
+        "metric": METRIC, "value": round(ms_per_step / 1e3, 6), "unit": "s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 outer residual/Krylov, f32 V-cycle, bf16 coarse off-diagonals",
+        "data": "synthetic (seeded jittered mesh and fields generated on the GPU)",
+        "config": {"workload": "%d^3 jittered hex (n=%d, nnz=%d), lognormal k sigma=2.5 in 8^3 blocks, "
+                               "source U[0,1], Dirichlet x0/x1, Neumann y/z; FP64 FGMRES(30) + mixed AMG "
+                               "V(0,1), setup included, to 1e-8 (1-norm, FP64 true residual)"
+                               % (N, S.n_self, nnz0),
+                   "n_cells": S.n_self, "levels": len(levels)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "setup_ms": round(1e3 * st["setup_seconds"], 2),
+        "fgmres_iterations": it,
+        "final_ratio": rf / r0,
+        "plain_fp64_gpu_amg_s": round(t_plain, 4),
+        "plain_fp64_cycles": cyc_plain,
+        "speedup_vs_plain_fp64_gpu": round(t_plain / (ms_per_step / 1e3), 3),
+        "exact_order_gpu_amg_s": None if t_exact is None else round(t_exact, 4),
+        "exact_order_cycles": cyc_exact,
+        "rel_error_vs_fp64_amg": err,
+        "assembly_s": round(t_asm, 4),
+        "kernels": kernels,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

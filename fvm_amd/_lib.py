@@ -60,6 +60,8 @@ SIGNATURES = {
     "fvm_amg_level_info": (_i, [_p, _i, _p, _p]),
     "fvm_amg_level_get": (_i, [_p, _i, _p, _p, _p, _p, _p]),
     "fvm_amg_setup_stats": (_i, [_p, _p, _p]),
+    "fvm_amg_set_kernel_timing": (_i, [_p, _i]),
+    "fvm_amg_kernel_timing": (_i, [_p, _i, _p, _p]),
     "fvm_fgmres_default_params": (_i, [ctypes.POINTER(FgmresParams)]),
     "fvm_fgmres_create": (_i, [ctypes.POINTER(_p), _p]),
     "fvm_fgmres_destroy": (_i, [_p]),
@@ -67,6 +69,20 @@ SIGNATURES = {
     "fvm_fgmres_solve_device": (_i, [_p, _p, _p, _p, _p, _p]),
     "fvm_fgmres_solve": (_i, [_p, _p, _p, _p, _p, _p]),
     "fvm_fgmres_history": (_i, [_p, _p, _i]),
+    "fvm_mesh_create_hex": (_i, [ctypes.POINTER(_p), _i, _i, _d, ctypes.c_uint64]),
+    "fvm_mesh_create": (_i, [ctypes.POINTER(_p), _i, _i, _p, _i, _p, _p, _i, _i, _i, _p]),
+    "fvm_mesh_destroy": (_i, [_p]),
+    "fvm_mesh_info": (_i, [_p, _p, _p, _p, _p, _p]),
+    "fvm_mesh_get": (_i, [_p, _p, _p, _p, _p]),
+    "fvm_mesh_metrics": (_i, [_p, _p, _p, _p, _p, _p]),
+    "fvm_mesh_lognormal_field": (_i, [_p, _d, _i, ctypes.c_uint64, _p]),
+    "fvm_mesh_uniform_field": (_i, [_p, _d, _d, ctypes.c_uint64, _p]),
+    "fvm_thermal_assemble": (_i, [_p, _p, _p, _p, _p, _p, ctypes.POINTER(_p)]),
+    "fvm_system_info": (_i, [_p, _p, _p, _p]),
+    "fvm_system_device": (_i, [_p, _p, _p, _p, _p, _p, _p]),
+    "fvm_system_get": (_i, [_p, _p, _p, _p, _p, _p, _p]),
+    "fvm_system_destroy": (_i, [_p]),
+    "fvm_amg_setup_system": (_i, [_p, _p]),
 }
 
 _lib = None

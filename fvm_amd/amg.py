@@ -25,6 +25,11 @@ def _ptr(a):
     return None if a is None else ctypes.c_void_p(a.ctypes.data)
 
 
+def _dptr(t):
+    """Device address of a tensor-like object (``data_ptr()``) or a raw integer address."""
+    return ctypes.c_void_p(t if isinstance(t, int) else t.data_ptr())
+
+
 class AMG:
     # AMG::CycleType / AMG::SmootherType (AMG.h:31-42)
     V_CYCLE, W_CYCLE, F_CYCLE = 0, 1, 2
@@ -109,8 +114,8 @@ class AMG:
         r0 = ctypes.c_double()
         rf = ctypes.c_double()
         cyc = ctypes.c_int()
-        check(lib().fvm_amg_solve_device(self._h, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
-                                         ctypes.byref(r0), ctypes.byref(rf), ctypes.byref(cyc)))
+        check(lib().fvm_amg_solve_device(self._h, _dptr(b), _dptr(x), ctypes.byref(r0), ctypes.byref(rf),
+                                         ctypes.byref(cyc)))
         return r0.value, rf.value, cyc.value
 
     def smooth(self, ls):
@@ -245,8 +250,8 @@ class FGMRES:
         r0 = ctypes.c_double()
         rf = ctypes.c_double()
         it = ctypes.c_int()
-        check(lib().fvm_fgmres_solve_device(self._h, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
-                                            ctypes.byref(r0), ctypes.byref(rf), ctypes.byref(it)))
+        check(lib().fvm_fgmres_solve_device(self._h, _dptr(b), _dptr(x), ctypes.byref(r0), ctypes.byref(rf),
+                                            ctypes.byref(it)))
         return r0.value, rf.value, it.value
 
     def history(self, cap=1 << 16):

@@ -83,10 +83,21 @@ struct fvm_amg {
     // mixed-precision coarsest dense inverse (FP64)
     int direct_level = -1;
     fvm::DArr<double> dense_inv;
+    // live timing of finest-level kernels with HIP events on the handle's stream
+    bool ktime = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_open;
+    long long k_launches[4] = {0, 0, 0, 0};
+    double k_ms[4] = {0, 0, 0, 0};
     // double-buffer used by pointer-swapped Jacobi
 };
 
 namespace fvm {
+// kernel timing kinds
+enum { KT_RESID64 = 0, KT_SWEEP = 1 };
+hipEvent_t timing_begin(fvm_amg *h);
+void timing_end(fvm_amg *h, int kind, hipEvent_t start);
+void timing_harvest(fvm_amg *h);
 // setup.hip
 void build_hierarchy(fvm_amg *h, int n_self, int n_total, const int *d_row, const int *d_col,
                      const double *d_diag, const double *d_off, const uint8_t *d_bnd);

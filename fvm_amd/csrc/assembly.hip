@@ -1,0 +1,825 @@
+// Finite-volume assembly of the thermal (diffusion) equation on the GPU.
+//
+// Reference semantics (src/fvm/src/modules/fvmbase/):
+//   MeshMetricsCalculator_impl.h:59-236 (face / cell centroids), :239-305
+//     (quad face areas), :385 (|A|), :393-462 (cell volumes)
+//   CRConnectivity.cpp:167-270 (cellCells = cellFaces x faceCells, first appearance)
+//   CRMatrix.h:117-160 + CRConnectivity.cpp:729-792 (face -> CSR positions)
+//   DiffusionDiscretization.h:19-27, :153-230; SourceDiscretization.h:40-60
+//   GenericBCS.h:77-157 (Dirichlet / Neumann); CRMatrix.h:899-944, :1064-1085
+//     (LinearSystem::initSolve eliminates boundary rows)
+//
+// The reference runs sequential face loops that scatter into cells. Here every
+// per-face quantity is computed once by a face-parallel kernel and every cell
+// gathers its faces in ascending face order, which is the order in which the
+// face loop touched that cell; with no FP contraction each cell's sums are
+// therefore bit-identical to the reference's.
+#include "amg.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace fvm {
+namespace {
+
+constexpr int BS = 256;
+constexpr int MAXG = 64;   // face groups
+
+struct Groups {
+    int n;
+    int ptr[MAXG + 1];
+    int type[MAXG];
+    double value[MAXG];
+};
+
+__device__ __forceinline__ int group_of(const Groups &G, int f) {
+    int g = 0;
+    while (g + 1 < G.n && f >= G.ptr[g + 1]) g++;
+    return g;
+}
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+    x += 0x9e3779b97f4a7c15ULL;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+// uniform in [0,1): 53 random bits
+__device__ __forceinline__ double u01(unsigned long long seed, unsigned long long idx, unsigned stream) {
+    const unsigned long long h = mix64(mix64(seed ^ (0xD1B54A32D192ED03ULL * (stream + 1))) ^ idx);
+    return (double)(h >> 11) * 0x1.0p-53;
+}
+
+// ------------------------------------------------------------------ jittered hex mesh
+__device__ __forceinline__ int nid(int N, int i, int j, int k) { return i + (N + 1) * (j + (N + 1) * k); }
+__device__ __forceinline__ int cid(int N, int i, int j, int k) { return i + N * (j + N * k); }
+
+__global__ void k_hex_nodes(int N, double jitter, unsigned long long seed, double *xyz) {
+    const long long nn = (long long)(N + 1) * (N + 1) * (N + 1);
+    const double h = 1.0 / N;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < nn;
+         id += (long long)gridDim.x * blockDim.x) {
+        int idx[3];
+        idx[0] = (int)(id % (N + 1));
+        idx[1] = (int)((id / (N + 1)) % (N + 1));
+        idx[2] = (int)(id / ((long long)(N + 1) * (N + 1)));
+        for (int a = 0; a < 3; a++) {
+            double v = (double)idx[a] * h;
+            if (idx[a] > 0 && idx[a] < N)       // walls stay planar: tangential jitter only
+                v += (2.0 * u01(seed, (unsigned long long)id, a) - 1.0) * jitter * h;
+            xyz[3 * id + a] = v;
+        }
+    }
+}
+
+// Faces: interior x-, y-, z-faces (lexicographic, x fastest), then boundary
+// groups x0, x1, y0, y1, z0, z1 (N^2 faces each). Face nodes are ordered so
+// that 0.5 (x2-x0) x (x3-x1) points from c0 to c1 (outward on the boundary).
+__global__ void k_hex_faces(int N, int *fnodes, int *fcells) {
+    const long long Fx = (long long)(N - 1) * N * N;
+    const long long Fint = 3 * Fx;
+    const long long nf = Fint + 6LL * N * N;
+    const int n = N * N * N;
+    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < nf;
+         f += (long long)gridDim.x * blockDim.x) {
+        int c0, c1, q[4];
+        if (f < Fint) {
+            const int dir = (int)(f / Fx);
+            const long long r = f - dir * Fx;
+            if (dir == 0) {
+                const int i = (int)(r % (N - 1)), j = (int)((r / (N - 1)) % N), k = (int)(r / ((long long)(N - 1) * N));
+                c0 = cid(N, i, j, k); c1 = cid(N, i + 1, j, k);
+                q[0] = nid(N, i + 1, j, k); q[1] = nid(N, i + 1, j + 1, k);
+                q[2] = nid(N, i + 1, j + 1, k + 1); q[3] = nid(N, i + 1, j, k + 1);
+            } else if (dir == 1) {
+                const int i = (int)(r % N), j = (int)((r / N) % (N - 1)), k = (int)(r / ((long long)N * (N - 1)));
+                c0 = cid(N, i, j, k); c1 = cid(N, i, j + 1, k);
+                q[0] = nid(N, i, j + 1, k); q[1] = nid(N, i, j + 1, k + 1);
+                q[2] = nid(N, i + 1, j + 1, k + 1); q[3] = nid(N, i + 1, j + 1, k);
+            } else {
+                const int i = (int)(r % N), j = (int)((r / N) % N), k = (int)(r / ((long long)N * N));
+                c0 = cid(N, i, j, k); c1 = cid(N, i, j, k + 1);
+                q[0] = nid(N, i, j, k + 1); q[1] = nid(N, i + 1, j, k + 1);
+                q[2] = nid(N, i + 1, j + 1, k + 1); q[3] = nid(N, i, j + 1, k + 1);
+            }
+        } else {
+            const long long bf = f - Fint;
+            const int grp = (int)(bf / ((long long)N * N));
+            const int b = (int)(bf % ((long long)N * N));
+            const int a0 = b % N, a1 = b / N;
+            c1 = n + (int)bf;
+            switch (grp) {
+            case 0:  // x = 0, outward -x
+                c0 = cid(N, 0, a0, a1);
+                q[0] = nid(N, 0, a0, a1); q[1] = nid(N, 0, a0, a1 + 1);
+                q[2] = nid(N, 0, a0 + 1, a1 + 1); q[3] = nid(N, 0, a0 + 1, a1);
+                break;
+            case 1:  // x = 1, +x
+                c0 = cid(N, N - 1, a0, a1);
+                q[0] = nid(N, N, a0, a1); q[1] = nid(N, N, a0 + 1, a1);
+                q[2] = nid(N, N, a0 + 1, a1 + 1); q[3] = nid(N, N, a0, a1 + 1);
+                break;
+            case 2:  // y = 0, -y
+                c0 = cid(N, a0, 0, a1);
+                q[0] = nid(N, a0, 0, a1); q[1] = nid(N, a0 + 1, 0, a1);
+                q[2] = nid(N, a0 + 1, 0, a1 + 1); q[3] = nid(N, a0, 0, a1 + 1);
+                break;
+            case 3:  // y = 1, +y
+                c0 = cid(N, a0, N - 1, a1);
+                q[0] = nid(N, a0, N, a1); q[1] = nid(N, a0, N, a1 + 1);
+                q[2] = nid(N, a0 + 1, N, a1 + 1); q[3] = nid(N, a0 + 1, N, a1);
+                break;
+            case 4:  // z = 0, -z
+                c0 = cid(N, a0, a1, 0);
+                q[0] = nid(N, a0, a1, 0); q[1] = nid(N, a0, a1 + 1, 0);
+                q[2] = nid(N, a0 + 1, a1 + 1, 0); q[3] = nid(N, a0 + 1, a1, 0);
+                break;
+            default:  // z = 1, +z
+                c0 = cid(N, a0, a1, N - 1);
+                q[0] = nid(N, a0, a1, N); q[1] = nid(N, a0 + 1, a1, N);
+                q[2] = nid(N, a0 + 1, a1 + 1, N); q[3] = nid(N, a0, a1 + 1, N);
+                break;
+            }
+        }
+        fcells[2 * f] = c0;
+        fcells[2 * f + 1] = c1;
+        for (int t = 0; t < 4; t++) fnodes[4 * f + t] = q[t];
+    }
+}
+
+// ------------------------------------------------------------------ connectivity
+__global__ void k_cf_count(int nf, const int *fcells, int *cnt) {
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
+        atomicAdd(&cnt[fcells[2 * f]], 1);
+        atomicAdd(&cnt[fcells[2 * f + 1]], 1);
+    }
+}
+__global__ void k_cf_fill(int nf, const int *fcells, const int *row, int *fill, int *col) {
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
+        const int a = fcells[2 * f], b = fcells[2 * f + 1];
+        col[row[a] + atomicAdd(&fill[a], 1)] = f;
+        col[row[b] + atomicAdd(&fill[b], 1)] = f;
+    }
+}
+__global__ void k_sort_segments(int nseg, const int *ptr, int *v) {
+    for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < nseg; I += gridDim.x * blockDim.x) {
+        const int b = ptr[I], e = ptr[I + 1];
+        for (int a = b + 1; a < e; a++) {
+            const int key = v[a];
+            int j = a - 1;
+            while (j >= b && v[j] > key) { v[j + 1] = v[j]; j--; }
+            v[j + 1] = key;
+        }
+    }
+}
+// cellCells row lengths (pass 0) / entries and face positions (pass 1)
+__global__ void k_cc_build(int nct, const int *cf_row, const int *cf_col, const int *fcells,
+                           const int *cc_row, int *cnt, int *cc_col, int *pos01, int *pos10, int pass) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nct; c += gridDim.x * blockDim.x) {
+        const int base = pass ? cc_row[c] : 0;
+        int k = 0;
+        // pass 0 has no storage: count distinct neighbours by rescanning
+        for (int q = cf_row[c]; q < cf_row[c + 1]; q++) {
+            const int f = cf_col[q];
+            for (int s = 0; s < 2; s++) {
+                const int o = fcells[2 * f + s];
+                if (o == c) continue;
+                int pos = -1;
+                if (pass) {
+                    for (int e = 0; e < k; e++)
+                        if (cc_col[base + e] == o) { pos = e; break; }
+                } else {
+                    // seen before iff an earlier face of c (or an earlier slot of f) has o
+                    for (int q2 = cf_row[c]; q2 <= q && pos < 0; q2++) {
+                        const int f2 = cf_col[q2];
+                        for (int s2 = 0; s2 < 2; s2++) {
+                            if (q2 == q && s2 >= s) break;
+                            if (fcells[2 * f2 + s2] == o) { pos = 0; break; }
+                        }
+                    }
+                }
+                if (pos < 0) {
+                    if (pass) cc_col[base + k] = o;
+                    pos = k++;
+                }
+                if (pass) {
+                    if (s == 1) pos01[f] = base + pos;    // c is c0, neighbour c1
+                    else pos10[f] = base + pos;          // c is c1, neighbour c0
+                }
+            }
+        }
+        if (!pass) cnt[c] = k;
+    }
+}
+
+// ------------------------------------------------------------------ metrics
+__global__ void k_face_metrics(int nf, const double *xyz, const int *fnodes, double *fA, double *fAm,
+                               double *fc) {
+    const double twoThirds = 2. / 3.;
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
+        const int *q = fnodes + 4 * f;
+        double x[4][3];
+        for (int t = 0; t < 4; t++)
+            for (int a = 0; a < 3; a++) x[t][a] = xyz[3 * q[t] + a];
+        double d20[3], d31[3], A[3];
+        for (int a = 0; a < 3; a++) { d20[a] = x[2][a] - x[0][a]; d31[a] = x[3][a] - x[1][a]; }
+        A[0] = 0.5 * (d20[1] * d31[2] - d20[2] * d31[1]);
+        A[1] = 0.5 * (d20[2] * d31[0] - d20[0] * d31[2]);
+        A[2] = 0.5 * (d20[0] * d31[1] - d20[1] * d31[0]);
+        const double Am = sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]);
+        double p[3];
+        for (int a = 0; a < 3; a++) p[a] = x[0][a];
+        for (int t = 1; t < 4; t++)
+            for (int a = 0; a < 3; a++) p[a] += x[t][a];
+        for (int a = 0; a < 3; a++) p[a] /= 4.0;
+        double en[3];
+        for (int a = 0; a < 3; a++) en[a] = A[a] / Am;
+        double denom = 0.0, cfc[3] = {0.0, 0.0, 0.0};
+        for (int t = 0; t < 4; t++) {
+            const double *u = x[t], *v = x[(t + 1) & 3];
+            double r0[3], r1[3], tri[3], xm[3];
+            for (int a = 0; a < 3; a++) { r0[a] = u[a] - p[a]; r1[a] = v[a] - p[a]; }
+            tri[0] = 0.5 * (r0[1] * r1[2] - r0[2] * r1[1]);
+            tri[1] = 0.5 * (r0[2] * r1[0] - r0[0] * r1[2]);
+            tri[2] = 0.5 * (r0[0] * r1[1] - r0[1] * r1[0]);
+            const double triP = tri[0] * en[0] + tri[1] * en[1] + tri[2] * en[2];
+            for (int a = 0; a < 3; a++) xm[a] = 0.5 * (u[a] + v[a]);
+            for (int a = 0; a < 3; a++) cfc[a] += (twoThirds * (xm[a] - p[a])) * triP;
+            denom += triP;
+        }
+        for (int a = 0; a < 3; a++) cfc[a] /= denom;
+        for (int a = 0; a < 3; a++) {
+            fA[3 * f + a] = A[a];
+            fc[3 * f + a] = p[a] + cfc[a];
+        }
+        fAm[f] = Am;
+    }
+}
+
+__global__ void k_cell_centroids(int n_cells, int nct, const int *cf_row, const int *cf_col,
+                                 const double *fAm, const double *fc, double *cc) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nct; c += gridDim.x * blockDim.x) {
+        if (c >= n_cells) {   // ghost: centroid of its boundary face
+            const int f = cf_col[cf_row[c]];
+            for (int a = 0; a < 3; a++) cc[3 * c + a] = fc[3 * f + a];
+            continue;
+        }
+        double s[3] = {0.0, 0.0, 0.0}, w = 0.0;
+        for (int q = cf_row[c]; q < cf_row[c + 1]; q++) {
+            const int f = cf_col[q];
+            for (int a = 0; a < 3; a++) s[a] += fc[3 * f + a] * fAm[f];
+            w += fAm[f];
+        }
+        for (int a = 0; a < 3; a++) cc[3 * c + a] = s[a] / w;
+    }
+}
+
+__global__ void k_cell_volumes(int n_cells, const int *cf_row, const int *cf_col, const int *fcells,
+                               const double *fA, const double *fc, const double *cc, double *vol) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n_cells; c += gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int q = cf_row[c]; q < cf_row[c + 1]; q++) {
+            const int f = cf_col[q];
+            double d[3];
+            for (int a = 0; a < 3; a++) d[a] = fc[3 * f + a] - cc[3 * c + a];
+            const double t = (d[0] * fA[3 * f] + d[1] * fA[3 * f + 1] + d[2] * fA[3 * f + 2]) / 3.0;
+            if (fcells[2 * f] == c) v += t;
+            else v -= t;
+        }
+        vol[c] = v;
+    }
+}
+
+__global__ void k_ghost_volumes(int n_cells, int nct, const int *cf_row, const int *cf_col,
+                                const int *fcells, double *vol) {
+    for (int g = n_cells + blockIdx.x * blockDim.x + threadIdx.x; g < nct; g += gridDim.x * blockDim.x)
+        vol[g] = vol[fcells[2 * cf_col[cf_row[g]]]];
+}
+
+// ------------------------------------------------------------------ thermal assembly
+__device__ __forceinline__ double harmonic(double x0, double x1) {
+    const double sum = x0 + x1;
+    if (x0 + x1 != 0.0) return 2.0 * x0 * x1 / sum;
+    return sum;
+}
+
+// per face: diffusion coefficient and flux (DiffusionDiscretization.h:172-212)
+__global__ void k_face_diffusion(int nf, const int *fcells, const double *fA, const double *fAm,
+                                 const double *cc, const double *vol, const double *kc,
+                                 const double *x, double *dcoef, double *dflux) {
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
+        const int c0 = fcells[2 * f], c1 = fcells[2 * f + 1];
+        const double vol0 = vol[c0], vol1 = vol[c1];
+        double ds[3];
+        for (int a = 0; a < 3; a++) ds[a] = cc[3 * c1 + a] - cc[3 * c0 + a];
+        double kf;
+        if (vol0 == 0.) kf = kc[c1];
+        else if (vol1 == 0.) kf = kc[c0];
+        else kf = harmonic(kc[c0], kc[c1]);
+        const double *A = fA + 3 * f;
+        const double diffMetric = fAm[f] * fAm[f] / (A[0] * ds[0] + A[1] * ds[1] + A[2] * ds[2]);
+        const double diffCoeff = kf * diffMetric;
+        double sec[3], gradF[3];
+        for (int a = 0; a < 3; a++) sec[a] = kf * (A[a] - ds[a] * diffMetric);
+        for (int a = 0; a < 3; a++) gradF[a] = (0.0 * vol0 + 0.0 * vol1) / (vol0 + vol1);
+        double dsec = 0.0;
+        for (int a = 0; a < 3; a++) dsec += gradF[a] * sec[a];
+        dcoef[f] = diffCoeff;
+        dflux[f] = diffCoeff * (x[c1] - x[c0]) + dsec;
+    }
+}
+
+// per cell: diffusion sums in face order, then source (interior cells)
+__global__ void k_cell_assemble(int n_cells, int nct, const int *cf_row, const int *cf_col,
+                                const int *fcells, const int *pos01, const int *pos10,
+                                const int *row, const double *dcoef, const double *dflux,
+                                const double *vol, const double *src, double *diag, double *off,
+                                double *b) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nct; c += gridDim.x * blockDim.x) {
+        for (int p = row[c]; p < row[c + 1]; p++) off[p] = 0.0;
+        double d = 0.0, r = 0.0;
+        for (int q = cf_row[c]; q < cf_row[c + 1]; q++) {
+            const int f = cf_col[q];
+            const bool first = (fcells[2 * f] == c);
+            off[first ? pos01[f] : pos10[f]] += dcoef[f];
+            d -= dcoef[f];
+            if (first) r += dflux[f];
+            else r -= dflux[f];
+        }
+        if (c < n_cells) r += vol[c] * src[c];
+        diag[c] = d;
+        b[c] = r;
+    }
+}
+
+// GenericBCS, interior side: Dirichlet adds coeff01*(Tb - x_ghost) to b[c0]
+// and drops the coefficient (GenericBCS.h:77-100), faces in order.
+__global__ void k_bc_interior(int n_cells, const int *cf_row, const int *cf_col, const int *fcells,
+                             const int *pos01, Groups G, const double *x, double *off, double *b) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n_cells; c += gridDim.x * blockDim.x) {
+        for (int q = cf_row[c]; q < cf_row[c + 1]; q++) {
+            const int f = cf_col[q];
+            if (f < G.ptr[1]) continue;
+            const int g = group_of(G, f);
+            if (G.type[g] != FVM_BC_DIRICHLET) continue;
+            const double dX = G.value[g] - x[fcells[2 * f + 1]];
+            b[c] += off[pos01[f]] * dX;
+            off[pos01[f]] = 0.0;
+        }
+    }
+}
+
+// GenericBCS, ghost side (GenericBCS.h:100-110 Dirichlet, :129-150 Neumann)
+__global__ void k_bc_ghost(int n_cells, int nct, const int *cf_row, const int *cf_col,
+                           const int *pos10, const double *fAm, Groups G, double *diag, double *off,
+                           double *b, uint8_t *isb) {
+    for (int gc = n_cells + blockIdx.x * blockDim.x + threadIdx.x; gc < nct; gc += gridDim.x * blockDim.x) {
+        const int f = cf_col[cf_row[gc]];
+        const int g = group_of(G, f);
+        isb[gc] = 0;
+        if (G.type[g] == FVM_BC_DIRICHLET) {
+            off[pos10[f]] = 0.0;
+            b[gc] = 0.0;
+            diag[gc] = -1.0;
+        } else {
+            const double fluxB = -b[gc];
+            b[gc] = G.value[g] * fAm[f] - fluxB;
+            isb[gc] = 1;
+        }
+    }
+}
+
+// LinearSystem::initSolve: eliminate the marked ghost rows (one neighbour each)
+// into their interior cell, ghosts in ascending order (CRMatrix.h:899-944).
+__global__ void k_eliminate(int n_cells, const int *cf_row, const int *cf_col, const int *fcells,
+                            const int *pos01, const int *pos10, const uint8_t *isb, double *diag,
+                            double *off, double *b) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n_cells; c += gridDim.x * blockDim.x) {
+        for (int q = cf_row[c]; q < cf_row[c + 1]; q++) {
+            const int f = cf_col[q];
+            const int j = fcells[2 * f + 1];
+            if (j < n_cells || fcells[2 * f] != c || !isb[j]) continue;
+            const double ajj = diag[j];
+            const double aij = off[pos01[f]];
+            const double ajk = off[pos10[f]];
+            diag[c] -= aij * (ajk / ajj);
+            b[c] -= aij * (b[j] / ajj);
+            off[pos01[f]] = 0.0;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ fields
+__global__ void k_lognormal(int N, int n_cells, int block, double sigma, unsigned long long seed,
+                            double *k) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n_cells; c += gridDim.x * blockDim.x) {
+        unsigned long long key = (unsigned long long)c;
+        if (block > 1 && N > 0) {
+            const int i = c % N, j = (c / N) % N, kk = c / (N * N);
+            const int nb = (N + block - 1) / block;
+            key = (unsigned long long)(i / block) + (unsigned long long)nb * ((j / block) + (unsigned long long)nb * (kk / block));
+        }
+        const double u1 = 1.0 - u01(seed, key, 10);   // (0,1]
+        const double u2 = u01(seed, key, 11);
+        const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+        k[c] = exp(sigma * z);
+    }
+}
+
+__global__ void k_uniform(int n, double lo, double hi, unsigned long long seed, double *v) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+        v[c] = lo + (hi - lo) * u01(seed, (unsigned long long)c, 12);
+}
+
+__global__ void k_ghost_copy(int n_cells, int nct, const int *cf_row, const int *cf_col,
+                             const int *fcells, double *v) {
+    for (int g = n_cells + blockIdx.x * blockDim.x + threadIdx.x; g < nct; g += gridDim.x * blockDim.x)
+        v[g] = v[fcells[2 * cf_col[cf_row[g]]]];
+}
+
+}  // namespace
+}  // namespace fvm
+
+using fvm::DArr;
+
+struct fvm_mesh {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int hexN = 0;
+    int n_cells = 0, n_ghost = 0, n_faces = 0, n_nodes = 0, n_groups = 0;
+    std::vector<int> gptr;
+    DArr<double> nodes;
+    DArr<int> fnodes, fcells;
+    // derived
+    bool conn = false, metrics = false;
+    DArr<int> cf_row, cf_col, cc_row, cc_col, pos01, pos10;
+    long long nnz = 0;
+    DArr<double> fA, fAm, fc, cc, vol;
+};
+
+struct fvm_system {
+    int device = 0;
+    int n_self = 0, n_total = 0;
+    long long nnz = 0;
+    DArr<int> row, col;
+    DArr<double> diag, off, b;
+    DArr<uint8_t> isb;
+};
+
+namespace fvm {
+
+namespace {
+struct Dev {
+    int prev = -1;
+    explicit Dev(int d) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        FVM_HIP(hipSetDevice(d));
+    }
+    ~Dev() { if (prev >= 0) (void)hipSetDevice(prev); }
+};
+
+void mesh_connectivity(fvm_mesh *m) {
+    if (m->conn) return;
+    hipStream_t s = m->stream;
+    const int nct = m->n_cells + m->n_ghost, nf = m->n_faces;
+    Scanner sc;
+    DArr<int> cnt((size_t)nct + 1), fill((size_t)nct + 1);
+    FVM_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes(), s));
+    FVM_HIP(hipMemsetAsync(fill.p, 0, fill.bytes(), s));
+    k_cf_count<<<grid_for(nf, BS, 4096), BS, 0, s>>>(nf, m->fcells.p, cnt.p);
+    m->cf_row.alloc((size_t)nct + 1);
+    sc.exclusive(cnt.p, m->cf_row.p, nct + 1, s);
+    m->cf_col.alloc(2 * (size_t)std::max(nf, 1));
+    k_cf_fill<<<grid_for(nf, BS, 4096), BS, 0, s>>>(nf, m->fcells.p, m->cf_row.p, fill.p, m->cf_col.p);
+    k_sort_segments<<<grid_for(nct, BS, 4096), BS, 0, s>>>(nct, m->cf_row.p, m->cf_col.p);
+    FVM_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes(), s));
+    k_cc_build<<<grid_for(nct, BS, 4096), BS, 0, s>>>(nct, m->cf_row.p, m->cf_col.p, m->fcells.p, nullptr,
+                                                      cnt.p, nullptr, nullptr, nullptr, 0);
+    m->cc_row.alloc((size_t)nct + 1);
+    sc.exclusive(cnt.p, m->cc_row.p, nct + 1, s);
+    m->nnz = read_scalar(m->cc_row.p + nct, s);
+    m->cc_col.alloc(std::max<long long>(m->nnz, 1));
+    m->pos01.alloc(std::max(nf, 1));
+    m->pos10.alloc(std::max(nf, 1));
+    k_cc_build<<<grid_for(nct, BS, 4096), BS, 0, s>>>(nct, m->cf_row.p, m->cf_col.p, m->fcells.p,
+                                                      m->cc_row.p, nullptr, m->cc_col.p, m->pos01.p,
+                                                      m->pos10.p, 1);
+    FVM_HIP(hipGetLastError());
+    FVM_HIP(hipStreamSynchronize(s));
+    m->conn = true;
+}
+
+void mesh_metrics(fvm_mesh *m) {
+    mesh_connectivity(m);
+    if (m->metrics) return;
+    hipStream_t s = m->stream;
+    const int nct = m->n_cells + m->n_ghost, nf = m->n_faces;
+    m->fA.alloc(3 * (size_t)nf); m->fAm.alloc(nf); m->fc.alloc(3 * (size_t)nf);
+    m->cc.alloc(3 * (size_t)nct); m->vol.alloc(nct);
+    k_face_metrics<<<grid_for(nf, BS), BS, 0, s>>>(nf, m->nodes.p, m->fnodes.p, m->fA.p, m->fAm.p, m->fc.p);
+    k_cell_centroids<<<grid_for(nct, BS), BS, 0, s>>>(m->n_cells, nct, m->cf_row.p, m->cf_col.p, m->fAm.p,
+                                                      m->fc.p, m->cc.p);
+    k_cell_volumes<<<grid_for(m->n_cells, BS), BS, 0, s>>>(m->n_cells, m->cf_row.p, m->cf_col.p,
+                                                           m->fcells.p, m->fA.p, m->fc.p, m->cc.p, m->vol.p);
+    if (m->n_ghost)
+        k_ghost_volumes<<<grid_for(m->n_ghost, BS), BS, 0, s>>>(m->n_cells, nct, m->cf_row.p, m->cf_col.p,
+                                                                m->fcells.p, m->vol.p);
+    FVM_HIP(hipGetLastError());
+    FVM_HIP(hipStreamSynchronize(s));
+    m->metrics = true;
+}
+
+Groups make_groups(const fvm_mesh *m, const int32_t *type, const double *value) {
+    Groups G{};
+    G.n = m->n_groups;
+    for (int g = 0; g <= m->n_groups; g++) G.ptr[g] = m->gptr[g];
+    for (int g = 0; g < m->n_groups; g++) {
+        G.type[g] = g == 0 ? -1 : type[g];
+        G.value[g] = g == 0 ? 0.0 : value[g];
+        if (g > 0)
+            FVM_CHECK(type[g] == FVM_BC_DIRICHLET || type[g] == FVM_BC_NEUMANN, FVM_E_INVALID, "bad bc type");
+    }
+    return G;
+}
+
+}  // namespace
+}  // namespace fvm
+
+#define FVM_API_BEGIN try {
+#define FVM_API_END                                  \
+    }                                                \
+    catch (const fvm::Error &e) {                    \
+        fvm::set_last_error(e.what());               \
+        return e.code;                               \
+    }                                                \
+    catch (const std::exception &e) {                \
+        fvm::set_last_error(e.what());               \
+        return FVM_E_INVALID;                        \
+    }
+
+extern "C" {
+
+int fvm_mesh_create_hex(fvm_mesh **out, int device, int n, double jitter, uint64_t seed) {
+    FVM_API_BEGIN
+    FVM_CHECK(out && n >= 2 && n <= 1024 && jitter >= 0.0 && jitter < 0.5, FVM_E_INVALID,
+              "need 2 <= n <= 1024 and 0 <= jitter < 0.5");
+    fvm::Dev d(device);
+    fvm_mesh *m = new fvm_mesh();
+    try {
+        m->device = device;
+        FVM_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+        m->hexN = n;
+        m->n_cells = n * n * n;
+        m->n_ghost = 6 * n * n;
+        const long long Fint = 3LL * (n - 1) * n * n;
+        m->n_faces = (int)(Fint + m->n_ghost);
+        m->n_nodes = (n + 1) * (n + 1) * (n + 1);
+        m->n_groups = 7;
+        m->gptr.resize(8);
+        m->gptr[0] = 0;
+        m->gptr[1] = (int)Fint;
+        for (int g = 0; g < 6; g++) m->gptr[g + 2] = (int)(Fint + (long long)(g + 1) * n * n);
+        m->nodes.alloc(3 * (size_t)m->n_nodes);
+        m->fnodes.alloc(4 * (size_t)m->n_faces);
+        m->fcells.alloc(2 * (size_t)m->n_faces);
+        fvm::k_hex_nodes<<<fvm::grid_for(m->n_nodes, 256), 256, 0, m->stream>>>(n, jitter, seed, m->nodes.p);
+        fvm::k_hex_faces<<<fvm::grid_for(m->n_faces, 256), 256, 0, m->stream>>>(n, m->fnodes.p, m->fcells.p);
+        FVM_HIP(hipGetLastError());
+        FVM_HIP(hipStreamSynchronize(m->stream));
+    } catch (...) {
+        if (m->stream) (void)hipStreamDestroy(m->stream);
+        delete m;
+        throw;
+    }
+    *out = m;
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_mesh_create(fvm_mesh **out, int device, int n_nodes, const double *nodes, int n_faces,
+                    const int32_t *face_nodes, const int32_t *face_cells, int n_cells, int n_ghost,
+                    int n_groups, const int32_t *group_ptr) {
+    FVM_API_BEGIN
+    FVM_CHECK(out && nodes && face_nodes && face_cells && group_ptr, FVM_E_INVALID, "null argument");
+    FVM_CHECK(n_groups >= 1 && n_groups <= 64 && group_ptr[0] == 0 && group_ptr[n_groups] == n_faces,
+              FVM_E_INVALID, "bad face groups");
+    FVM_CHECK(group_ptr[1] == n_faces - n_ghost, FVM_E_INVALID, "one ghost cell per boundary face");
+    const int nct = n_cells + n_ghost;
+    for (int f = 0; f < n_faces; f++) {
+        const int c0 = face_cells[2 * f], c1 = face_cells[2 * f + 1];
+        FVM_CHECK(c0 >= 0 && c0 < n_cells && c1 >= 0 && c1 < nct && c0 != c1, FVM_E_INVALID, "bad face cells");
+        if (f >= group_ptr[1]) FVM_CHECK(c1 == n_cells + (f - group_ptr[1]), FVM_E_INVALID, "ghost numbering");
+        for (int t = 0; t < 4; t++)
+            FVM_CHECK(face_nodes[4 * f + t] >= 0 && face_nodes[4 * f + t] < n_nodes, FVM_E_INVALID, "bad face node");
+    }
+    fvm::Dev d(device);
+    fvm_mesh *m = new fvm_mesh();
+    try {
+        m->device = device;
+        FVM_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+        m->n_cells = n_cells; m->n_ghost = n_ghost; m->n_faces = n_faces; m->n_nodes = n_nodes;
+        m->n_groups = n_groups;
+        m->gptr.assign(group_ptr, group_ptr + n_groups + 1);
+        m->nodes.alloc(3 * (size_t)n_nodes);
+        m->fnodes.alloc(4 * (size_t)n_faces);
+        m->fcells.alloc(2 * (size_t)n_faces);
+        FVM_HIP(hipMemcpy(m->nodes.p, nodes, m->nodes.bytes(), hipMemcpyHostToDevice));
+        FVM_HIP(hipMemcpy(m->fnodes.p, face_nodes, m->fnodes.bytes(), hipMemcpyHostToDevice));
+        FVM_HIP(hipMemcpy(m->fcells.p, face_cells, m->fcells.bytes(), hipMemcpyHostToDevice));
+    } catch (...) {
+        if (m->stream) (void)hipStreamDestroy(m->stream);
+        delete m;
+        throw;
+    }
+    *out = m;
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_mesh_destroy(fvm_mesh *m) {
+    FVM_API_BEGIN
+    if (!m) return FVM_OK;
+    {
+        fvm::Dev d(m->device);
+        (void)hipStreamSynchronize(m->stream);
+        m->nodes.release(); m->fnodes.release(); m->fcells.release();
+        m->cf_row.release(); m->cf_col.release(); m->cc_row.release(); m->cc_col.release();
+        m->pos01.release(); m->pos10.release();
+        m->fA.release(); m->fAm.release(); m->fc.release(); m->cc.release(); m->vol.release();
+        (void)hipStreamDestroy(m->stream);
+    }
+    delete m;
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_mesh_info(const fvm_mesh *m, int *n_cells, int *n_ghost, int *n_faces, int *n_nodes, int *n_groups) {
+    if (!m) return FVM_E_INVALID;
+    if (n_cells) *n_cells = m->n_cells;
+    if (n_ghost) *n_ghost = m->n_ghost;
+    if (n_faces) *n_faces = m->n_faces;
+    if (n_nodes) *n_nodes = m->n_nodes;
+    if (n_groups) *n_groups = m->n_groups;
+    return FVM_OK;
+}
+
+int fvm_mesh_get(const fvm_mesh *m, double *nodes, int32_t *face_nodes, int32_t *face_cells, int32_t *group_ptr) {
+    FVM_API_BEGIN
+    FVM_CHECK(m, FVM_E_INVALID, "null mesh");
+    fvm::Dev d(m->device);
+    if (nodes) FVM_HIP(hipMemcpy(nodes, m->nodes.p, m->nodes.bytes(), hipMemcpyDeviceToHost));
+    if (face_nodes) FVM_HIP(hipMemcpy(face_nodes, m->fnodes.p, m->fnodes.bytes(), hipMemcpyDeviceToHost));
+    if (face_cells) FVM_HIP(hipMemcpy(face_cells, m->fcells.p, m->fcells.bytes(), hipMemcpyDeviceToHost));
+    if (group_ptr) for (int g = 0; g <= m->n_groups; g++) group_ptr[g] = m->gptr[g];
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_mesh_metrics(fvm_mesh *m, double *fA, double *fAm, double *fc, double *cc, double *vol) {
+    FVM_API_BEGIN
+    FVM_CHECK(m, FVM_E_INVALID, "null mesh");
+    fvm::Dev d(m->device);
+    fvm::mesh_metrics(m);
+    if (fA) FVM_HIP(hipMemcpy(fA, m->fA.p, m->fA.bytes(), hipMemcpyDeviceToHost));
+    if (fAm) FVM_HIP(hipMemcpy(fAm, m->fAm.p, m->fAm.bytes(), hipMemcpyDeviceToHost));
+    if (fc) FVM_HIP(hipMemcpy(fc, m->fc.p, m->fc.bytes(), hipMemcpyDeviceToHost));
+    if (cc) FVM_HIP(hipMemcpy(cc, m->cc.p, m->cc.bytes(), hipMemcpyDeviceToHost));
+    if (vol) FVM_HIP(hipMemcpy(vol, m->vol.p, m->vol.bytes(), hipMemcpyDeviceToHost));
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_mesh_lognormal_field(fvm_mesh *m, double sigma, int block, uint64_t seed, double *d_out) {
+    FVM_API_BEGIN
+    FVM_CHECK(m && d_out && block >= 1, FVM_E_INVALID, "bad argument");
+    FVM_CHECK(block == 1 || m->hexN > 0, FVM_E_INVALID, "block fields need a hex mesh");
+    fvm::Dev d(m->device);
+    fvm::mesh_connectivity(m);
+    const int nct = m->n_cells + m->n_ghost;
+    fvm::k_lognormal<<<fvm::grid_for(m->n_cells, 256), 256, 0, m->stream>>>(m->hexN, m->n_cells, block, sigma,
+                                                                            seed, d_out);
+    if (m->n_ghost)
+        fvm::k_ghost_copy<<<fvm::grid_for(m->n_ghost, 256), 256, 0, m->stream>>>(m->n_cells, nct, m->cf_row.p,
+                                                                                 m->cf_col.p, m->fcells.p, d_out);
+    FVM_HIP(hipGetLastError());
+    FVM_HIP(hipStreamSynchronize(m->stream));
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_mesh_uniform_field(fvm_mesh *m, double lo, double hi, uint64_t seed, double *d_out) {
+    FVM_API_BEGIN
+    FVM_CHECK(m && d_out, FVM_E_INVALID, "bad argument");
+    fvm::Dev d(m->device);
+    const int nct = m->n_cells + m->n_ghost;
+    fvm::k_uniform<<<fvm::grid_for(nct, 256), 256, 0, m->stream>>>(nct, lo, hi, seed, d_out);
+    FVM_HIP(hipGetLastError());
+    FVM_HIP(hipStreamSynchronize(m->stream));
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_thermal_assemble(fvm_mesh *m, const double *d_k, const double *d_source, const double *d_x,
+                         const int32_t *bc_type, const double *bc_value, fvm_system **out) {
+    FVM_API_BEGIN
+    FVM_CHECK(m && d_k && d_source && d_x && bc_type && bc_value && out, FVM_E_INVALID, "null argument");
+    fvm::Dev d(m->device);
+    fvm::mesh_metrics(m);
+    const fvm::Groups G = fvm::make_groups(m, bc_type, bc_value);
+    hipStream_t s = m->stream;
+    const int nc = m->n_cells, nct = m->n_cells + m->n_ghost, nf = m->n_faces;
+    fvm_system *S = new fvm_system();
+    try {
+        S->device = m->device;
+        S->n_self = nc;
+        S->n_total = nct;
+        S->nnz = m->nnz;
+        S->row.alloc((size_t)nct + 1);
+        S->col.alloc(std::max<long long>(m->nnz, 1));
+        S->off.alloc(std::max<long long>(m->nnz, 1));
+        S->diag.alloc(nct); S->b.alloc(nct); S->isb.alloc(nct);
+        FVM_HIP(hipMemcpyAsync(S->row.p, m->cc_row.p, S->row.bytes(), hipMemcpyDeviceToDevice, s));
+        FVM_HIP(hipMemcpyAsync(S->col.p, m->cc_col.p, sizeof(int) * m->nnz, hipMemcpyDeviceToDevice, s));
+        DArr<double> dcoef(std::max(nf, 1)), dflux(std::max(nf, 1));
+        fvm::k_face_diffusion<<<fvm::grid_for(nf, 256), 256, 0, s>>>(nf, m->fcells.p, m->fA.p, m->fAm.p, m->cc.p,
+                                                                     m->vol.p, d_k, d_x, dcoef.p, dflux.p);
+        fvm::k_cell_assemble<<<fvm::grid_for(nct, 256), 256, 0, s>>>(
+            nc, nct, m->cf_row.p, m->cf_col.p, m->fcells.p, m->pos01.p, m->pos10.p, m->cc_row.p, dcoef.p,
+            dflux.p, m->vol.p, d_source, S->diag.p, S->off.p, S->b.p);
+        fvm::k_bc_interior<<<fvm::grid_for(nc, 256), 256, 0, s>>>(nc, m->cf_row.p, m->cf_col.p, m->fcells.p,
+                                                                  m->pos01.p, G, d_x, S->off.p, S->b.p);
+        FVM_HIP(hipMemsetAsync(S->isb.p, 0, nct, s));
+        if (m->n_ghost)
+            fvm::k_bc_ghost<<<fvm::grid_for(m->n_ghost, 256), 256, 0, s>>>(nc, nct, m->cf_row.p, m->cf_col.p,
+                                                                           m->pos10.p, m->fAm.p, G, S->diag.p,
+                                                                           S->off.p, S->b.p, S->isb.p);
+        fvm::k_eliminate<<<fvm::grid_for(nc, 256), 256, 0, s>>>(nc, m->cf_row.p, m->cf_col.p, m->fcells.p,
+                                                                m->pos01.p, m->pos10.p, S->isb.p, S->diag.p,
+                                                                S->off.p, S->b.p);
+        FVM_HIP(hipGetLastError());
+        FVM_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        delete S;
+        throw;
+    }
+    *out = S;
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_system_info(const fvm_system *S, int *n_self, int *n_total, long long *nnz) {
+    if (!S) return FVM_E_INVALID;
+    if (n_self) *n_self = S->n_self;
+    if (n_total) *n_total = S->n_total;
+    if (nnz) *nnz = S->nnz;
+    return FVM_OK;
+}
+
+int fvm_system_device(const fvm_system *S, const int32_t **row, const int32_t **col, const double **diag,
+                      const double **off, const double **b, const uint8_t **is_boundary) {
+    if (!S) return FVM_E_INVALID;
+    if (row) *row = S->row.p;
+    if (col) *col = S->col.p;
+    if (diag) *diag = S->diag.p;
+    if (off) *off = S->off.p;
+    if (b) *b = S->b.p;
+    if (is_boundary) *is_boundary = S->isb.p;
+    return FVM_OK;
+}
+
+int fvm_system_get(const fvm_system *S, int32_t *row, int32_t *col, double *diag, double *off, double *b,
+                   uint8_t *is_boundary) {
+    FVM_API_BEGIN
+    FVM_CHECK(S, FVM_E_INVALID, "null system");
+    fvm::Dev d(S->device);
+    if (row) FVM_HIP(hipMemcpy(row, S->row.p, S->row.bytes(), hipMemcpyDeviceToHost));
+    if (col && S->nnz) FVM_HIP(hipMemcpy(col, S->col.p, sizeof(int) * S->nnz, hipMemcpyDeviceToHost));
+    if (diag) FVM_HIP(hipMemcpy(diag, S->diag.p, S->diag.bytes(), hipMemcpyDeviceToHost));
+    if (off && S->nnz) FVM_HIP(hipMemcpy(off, S->off.p, sizeof(double) * S->nnz, hipMemcpyDeviceToHost));
+    if (b) FVM_HIP(hipMemcpy(b, S->b.p, S->b.bytes(), hipMemcpyDeviceToHost));
+    if (is_boundary) FVM_HIP(hipMemcpy(is_boundary, S->isb.p, S->isb.bytes(), hipMemcpyDeviceToHost));
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_system_destroy(fvm_system *S) {
+    FVM_API_BEGIN
+    if (!S) return FVM_OK;
+    {
+        fvm::Dev d(S->device);
+        S->row.release(); S->col.release(); S->diag.release(); S->off.release(); S->b.release(); S->isb.release();
+    }
+    delete S;
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_amg_setup_system(fvm_amg *h, const fvm_system *S) {
+    FVM_API_BEGIN
+    FVM_CHECK(h && S, FVM_E_INVALID, "null argument");
+    FVM_CHECK(h->device == S->device, FVM_E_INVALID, "system and solver on different devices");
+    return fvm_amg_setup_device(h, S->n_self, S->n_total, S->row.p, S->col.p, S->diag.p, S->off.p, S->isb.p);
+    FVM_API_END
+}
+
+}  // extern "C"

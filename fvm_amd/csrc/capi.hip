@@ -137,6 +137,9 @@ int fvm_amg_destroy(fvm_amg *h) {
         h->dense_inv.release();
         h->h_stage_b.release();
         h->h_stage_x.release();
+        fvm::timing_harvest(h);
+        for (auto e : h->ev_pool) (void)hipEventDestroy(e);
+        h->ev_pool.clear();
         (void)hipStreamDestroy(h->stream);
     }
     delete h;
@@ -331,6 +334,25 @@ int fvm_amg_level_get(const fvm_amg *h, int level, int32_t *row, int32_t *col, d
     FVM_HIP(hipStreamSynchronize(s));
     return FVM_OK;
     FVM_API_END
+}
+
+int fvm_amg_set_kernel_timing(fvm_amg *h, int enable) {
+    FVM_API_BEGIN
+    FVM_CHECK(h, FVM_E_INVALID, "null handle");
+    DeviceGuard g(h->device);
+    FVM_HIP(hipStreamSynchronize(h->stream));
+    fvm::timing_harvest(h);
+    h->ktime = enable != 0;
+    for (int k = 0; k < 4; k++) { h->k_launches[k] = 0; h->k_ms[k] = 0.0; }
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_amg_kernel_timing(const fvm_amg *h, int kind, long long *launches, double *total_ms) {
+    if (!h || kind < 0 || kind > 3) return FVM_E_INVALID;
+    if (launches) *launches = h->k_launches[kind];
+    if (total_ms) *total_ms = h->k_ms[kind];
+    return FVM_OK;
 }
 
 int fvm_amg_setup_stats(const fvm_amg *h, double *setup_seconds, int *finest_rounds) {

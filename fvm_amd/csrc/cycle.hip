@@ -23,6 +23,45 @@
 
 namespace fvm {
 
+hipEvent_t timing_begin(fvm_amg *h) {
+    if (!h->ktime) return nullptr;
+    hipEvent_t e;
+    if (h->ev_pool.empty()) {
+        FVM_HIP(hipEventCreate(&e));
+    } else {
+        e = h->ev_pool.back();
+        h->ev_pool.pop_back();
+    }
+    FVM_HIP(hipEventRecord(e, h->stream));
+    return e;
+}
+
+void timing_end(fvm_amg *h, int kind, hipEvent_t start) {
+    if (!h->ktime || !start) return;
+    hipEvent_t e;
+    if (h->ev_pool.empty()) {
+        FVM_HIP(hipEventCreate(&e));
+    } else {
+        e = h->ev_pool.back();
+        h->ev_pool.pop_back();
+    }
+    FVM_HIP(hipEventRecord(e, h->stream));
+    h->ev_open.push_back({kind, {start, e}});
+}
+
+// after the stream is synchronized: accumulate and recycle the events
+void timing_harvest(fvm_amg *h) {
+    for (auto &it : h->ev_open) {
+        float ms = 0.f;
+        FVM_HIP(hipEventElapsedTime(&ms, it.second.first, it.second.second));
+        h->k_launches[it.first]++;
+        h->k_ms[it.first] += ms;
+        h->ev_pool.push_back(it.second.first);
+        h->ev_pool.push_back(it.second.second);
+    }
+    h->ev_open.clear();
+}
+
 namespace {
 
 constexpr int BS = 256;
@@ -334,9 +373,11 @@ void residual(Run &R, int l, const X *x, const X *b, X *r) {
     hipStream_t s = R.h->stream;
     const int n = L.n;
     if (n == 0) return;
+    hipEvent_t t0 = (l == 0 && value_type(R, l) == V64) ? timing_begin(R.h) : nullptr;
     FVM_DISPATCH_VALS(R, l, L, X,
         (k_sell_residual<<<grid_for(n, BS), BS, 0, s>>>(n, L.sell_off.p, L.sell_col.p, val, dg, x, b, r)),
         (k_csr_residual<<<grid_for(n, BS), BS, 0, s>>>(n, L.row.p, L.col.p, val, dg, x, b, r)));
+    if (t0) timing_end(R.h, KT_RESID64, t0);
 }
 
 template <class X>
@@ -346,6 +387,7 @@ void gs_sweep(Run &R, int l, X *x, const X *b) {
     if (L.n == 0) return;
     if (R.fast) {
         for (int dir = 0; dir < 2; dir++) {
+            hipEvent_t t0 = (l == 0) ? timing_begin(R.h) : nullptr;
             for (int k = 0; k < L.ncolor; k++) {
                 const int c = dir ? L.ncolor - 1 - k : k;
                 const int beg = L.color_ptr[c], end = L.color_ptr[c + 1];
@@ -354,6 +396,7 @@ void gs_sweep(Run &R, int l, X *x, const X *b) {
                     (k_sell_gs<<<grid_for(end - beg, BS), BS, 0, s>>>(beg, end, L.sell_off.p, L.sell_col.p, val, dg, x, b)),
                     (void)0);
             }
+            if (t0) timing_end(R.h, KT_SWEEP, t0);
         }
     } else {
         for (int dir = 0; dir < 2; dir++) {
@@ -579,6 +622,7 @@ void solve_device(fvm_amg *h, const double *d_b, double *d_x, double *rnorm0, do
     }
     store_finest(R, d_x);
     FVM_HIP(hipStreamSynchronize(h->stream));
+    timing_harvest(h);
     if (rnorm0) *rnorm0 = r0;
     if (rnorm_final) *rnorm_final = rn;
     if (cycles) *cycles = it;
@@ -696,6 +740,11 @@ void spmv64(Run &R, const double *x, double *y) {
     Level &F = h->lev[0];
     const int n = F.n;
     if (n == 0) return;
+    struct T {
+        fvm_amg *h; hipEvent_t e;
+        T(fvm_amg *hh) : h(hh), e(timing_begin(hh)) {}
+        ~T() { if (e) timing_end(h, KT_RESID64, e); }
+    } timer(h);
     if (R.fast)
         k_sell_spmv<<<grid_for(n, BS), BS, 0, h->stream>>>(n, F.sell_off.p, F.sell_col.p, F.sell_val.p,
                                                            F.pdiag.p, x, y);
@@ -863,6 +912,7 @@ void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const d
     }
     k_scatter<<<grid_for(n, BS), BS, 0, s>>>(n, perm, W.x.p, d_x);
     FVM_HIP(hipStreamSynchronize(s));
+    timing_harvest(h);
     if (rnorm0) *rnorm0 = r0;
     if (rnorm_final) *rnorm_final = rn;
     if (iterations) *iterations = total;

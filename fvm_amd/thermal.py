@@ -1,0 +1,140 @@
+"""Device mesh and thermal finite-volume assembly (ThermalModel linearization).
+
+Mirrors the steps of ThermalModel<T>::Impl::linearize + LinearSystem::initSolve
+(src/fvm/src/modules/fvmbase/ThermalModel_impl.h:236-398, LinearSystem.cpp:40-64):
+mesh metrics, DiffusionDiscretization, SourceDiscretization, Dirichlet
+('SpecifiedTemperature') and Neumann ('SpecifiedHeatFlux') boundary conditions
+and boundary-row elimination, all on the GPU through libfvm_amd.so.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import check, lib
+from .linear_system import LinearSystem
+
+BC_DIRICHLET = 0
+BC_NEUMANN = 1
+
+
+def _vp(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class Mesh:
+    """Quad-face mesh in device memory (interior cells + one ghost per boundary face)."""
+
+    def __init__(self, handle, device=0):
+        self._h = handle
+        self.device = device
+        nc, ng, nf, nn, ngr = (ctypes.c_int() for _ in range(5))
+        check(lib().fvm_mesh_info(self._h, ctypes.byref(nc), ctypes.byref(ng), ctypes.byref(nf),
+                                  ctypes.byref(nn), ctypes.byref(ngr)))
+        self.n_cells, self.n_ghost, self.n_faces, self.n_nodes, self.n_groups = (
+            nc.value, ng.value, nf.value, nn.value, ngr.value)
+
+    @classmethod
+    def hex(cls, n, jitter=0.0, seed=0, device=0):
+        h = ctypes.c_void_p()
+        check(lib().fvm_mesh_create_hex(ctypes.byref(h), int(device), int(n), float(jitter), int(seed)))
+        return cls(h, device)
+
+    @classmethod
+    def from_arrays(cls, nodes, face_nodes, face_cells, n_cells, n_ghost, group_ptr, device=0):
+        nodes = np.ascontiguousarray(nodes, np.float64).reshape(-1)
+        face_nodes = np.ascontiguousarray(face_nodes, np.int32).reshape(-1)
+        face_cells = np.ascontiguousarray(face_cells, np.int32).reshape(-1)
+        group_ptr = np.ascontiguousarray(group_ptr, np.int32)
+        h = ctypes.c_void_p()
+        check(lib().fvm_mesh_create(ctypes.byref(h), int(device), nodes.size // 3, _vp(nodes),
+                                    face_cells.size // 2, _vp(face_nodes), _vp(face_cells), int(n_cells),
+                                    int(n_ghost), group_ptr.size - 1, _vp(group_ptr)))
+        return cls(h, device)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().fvm_mesh_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def n_total(self):
+        return self.n_cells + self.n_ghost
+
+    def arrays(self):
+        nodes = np.zeros(3 * self.n_nodes)
+        fn = np.zeros(4 * self.n_faces, np.int32)
+        fc = np.zeros(2 * self.n_faces, np.int32)
+        gp = np.zeros(self.n_groups + 1, np.int32)
+        check(lib().fvm_mesh_get(self._h, _vp(nodes), _vp(fn), _vp(fc), _vp(gp)))
+        return dict(nodes=nodes, face_nodes=fn, face_cells=fc, group_ptr=gp)
+
+    def metrics(self):
+        nf, nct = self.n_faces, self.n_total
+        fA = np.zeros(3 * nf); fAm = np.zeros(nf); fc = np.zeros(3 * nf)
+        cc = np.zeros(3 * nct); vol = np.zeros(nct)
+        check(lib().fvm_mesh_metrics(self._h, _vp(fA), _vp(fAm), _vp(fc), _vp(cc), _vp(vol)))
+        return dict(fA=fA, fAm=fAm, fc=fc, cc=cc, vol=vol)
+
+    def lognormal_field(self, out, sigma, block=1, seed=0):
+        """Fill a device tensor (torch, float64, n_total entries)."""
+        check(lib().fvm_mesh_lognormal_field(self._h, float(sigma), int(block), int(seed),
+                                             ctypes.c_void_p(out.data_ptr())))
+        return out
+
+    def uniform_field(self, out, lo=0.0, hi=1.0, seed=0):
+        check(lib().fvm_mesh_uniform_field(self._h, float(lo), float(hi), int(seed),
+                                           ctypes.c_void_p(out.data_ptr())))
+        return out
+
+
+class System:
+    """Assembled CRMatrix system in device memory (owned by the library)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        ns, nt = ctypes.c_int(), ctypes.c_int()
+        nz = ctypes.c_longlong()
+        check(lib().fvm_system_info(self._h, ctypes.byref(ns), ctypes.byref(nt), ctypes.byref(nz)))
+        self.n_self, self.n_total, self.nnz = ns.value, nt.value, nz.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().fvm_system_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def device_pointers(self):
+        ptrs = [ctypes.c_void_p() for _ in range(6)]
+        check(lib().fvm_system_device(self._h, *[ctypes.byref(p) for p in ptrs]))
+        return dict(zip(["row", "col", "diag", "off", "b", "is_boundary"], [p.value for p in ptrs]))
+
+    def to_host(self):
+        nt, nz = self.n_total, self.nnz
+        row = np.zeros(nt + 1, np.int32); col = np.zeros(max(nz, 1), np.int32)
+        diag = np.zeros(nt); off = np.zeros(max(nz, 1)); b = np.zeros(nt); isb = np.zeros(nt, np.uint8)
+        check(lib().fvm_system_get(self._h, _vp(row), _vp(col), _vp(diag), _vp(off), _vp(b), _vp(isb)))
+        return dict(n_self=self.n_self, n_total=nt, row=row, col=col[:nz], diag=diag, off=off[:nz], b=b,
+                    is_boundary=isb)
+
+    def linear_system(self):
+        h = self.to_host()
+        return LinearSystem(h["n_self"], h["n_total"], h["row"], h["col"], h["diag"], h["off"], h["b"],
+                            is_boundary=h["is_boundary"])
+
+
+def assemble(mesh, k, source, x, bc_types, bc_values):
+    """ThermalModel linearization on the device: k, source, x are torch float64
+    tensors with mesh.n_total entries; bc_types/bc_values per face group (index 0 unused)."""
+    t = np.ascontiguousarray(bc_types, np.int32)
+    v = np.ascontiguousarray(bc_values, np.float64)
+    if t.size != mesh.n_groups or v.size != mesh.n_groups:
+        raise ValueError("need one boundary condition per face group")
+    h = ctypes.c_void_p()
+    check(lib().fvm_thermal_assemble(mesh._h, ctypes.c_void_p(k.data_ptr()), ctypes.c_void_p(source.data_ptr()),
+                                     ctypes.c_void_p(x.data_ptr()), _vp(t), _vp(v), ctypes.byref(h)))
+    return System(h)

@@ -151,6 +151,14 @@ int fvm_amg_level_info(const fvm_amg *h, int level, int *n_rows, int *nnz);
  * coarsest level). */
 int fvm_amg_level_get(const fvm_amg *h, int level, int32_t *row, int32_t *col, double *diag,
                       double *off, int32_t *ci);
+/* Live timing of finest-level kernels with HIP events recorded around their
+ * launches on the handle's stream (no reference counterpart; for roofline
+ * reporting). kind 0: FP64 residual / SpMV launches; kind 1: smoother sweeps
+ * (one direction over all colours, or one Jacobi pass). Enabling resets the
+ * counters. */
+enum { FVM_KT_RESIDUAL64 = 0, FVM_KT_SWEEP = 1 };
+int fvm_amg_set_kernel_timing(fvm_amg *h, int enable);
+int fvm_amg_kernel_timing(const fvm_amg *h, int kind, long long *launches, double *total_ms);
 /* Setup statistics: seconds spent in setup, number of coarsening rounds of the finest level. */
 int fvm_amg_setup_stats(const fvm_amg *h, double *setup_seconds, int *finest_rounds);
 
@@ -184,6 +192,66 @@ int fvm_fgmres_solve(fvm_fgmres *s, const double *b, double *x, double *rnorm0,
 /* True-residual 1-norms of the last solve: hist[0] = r0, then one per restart
  * cycle. Returns the number of entries written (<= cap). */
 int fvm_fgmres_history(const fvm_fgmres *s, double *hist, int cap);
+
+
+/* ---------------------------------------------------------------------------
+ * Finite-volume assembly of the thermal (diffusion) equation on the device.
+ * Reference: ThermalModel linearization (ThermalModel_impl.h:236-398) with
+ * MeshMetricsCalculator (MeshMetricsCalculator_impl.h:59-462),
+ * DiffusionDiscretization (DiffusionDiscretization.h:65-232),
+ * SourceDiscretization (SourceDiscretization.h:40-60), GenericBCS Dirichlet /
+ * Neumann (GenericBCS.h:77-157) and LinearSystem::initSolve's boundary-row
+ * elimination (LinearSystem.cpp:40-64, CRMatrix.h:899-1085). The assembled
+ * system is the CRMatrix the AMG consumes: interior cells first, one ghost row
+ * per boundary face after them, columns in CRConnectivity::multiply order.
+ * Only uniform initial fields are supported (their least-squares gradient is
+ * zero, so the non-orthogonal secondary flux vanishes).
+ * ------------------------------------------------------------------------- */
+enum { FVM_BC_DIRICHLET = 0, FVM_BC_NEUMANN = 1 };
+
+typedef struct fvm_mesh fvm_mesh;
+typedef struct fvm_system fvm_system;
+
+/* Jittered n^3 hexahedral mesh of the unit cube built on the device: interior
+ * nodes moved by U[-jitter h, jitter h] per coordinate (walls stay planar).
+ * Faces: interior x-, y-, z-faces (lexicographic), then the boundary groups
+ * x=0, x=1, y=0, y=1, z=0, z=1 (group ids 1..6). Synthetic input: stands in
+ * for a mesh read by the reference's importers (FluentReader). */
+int fvm_mesh_create_hex(fvm_mesh **out, int device, int n, double jitter, uint64_t seed);
+/* Quad-face mesh from host arrays: face_nodes[4 f], face_cells[2 f] = (c0, c1),
+ * groups [group_ptr[g], group_ptr[g+1]) with group 0 the interior faces and
+ * ghost cell of boundary face f = n_cells + (f - group_ptr[1]). */
+int fvm_mesh_create(fvm_mesh **out, int device, int n_nodes, const double *nodes, int n_faces,
+                    const int32_t *face_nodes, const int32_t *face_cells, int n_cells, int n_ghost,
+                    int n_groups, const int32_t *group_ptr);
+int fvm_mesh_destroy(fvm_mesh *m);
+int fvm_mesh_info(const fvm_mesh *m, int *n_cells, int *n_ghost, int *n_faces, int *n_nodes,
+                  int *n_groups);
+/* host copies of the mesh arrays (any may be NULL) */
+int fvm_mesh_get(const fvm_mesh *m, double *nodes, int32_t *face_nodes, int32_t *face_cells,
+                 int32_t *group_ptr);
+/* MeshMetricsCalculator: face area vectors, |A|, face centroids, cell centroids
+ * and volumes (ghost cells included); host copies, any may be NULL. */
+int fvm_mesh_metrics(fvm_mesh *m, double *fA, double *fAm, double *fc, double *cc, double *vol);
+/* Cell fields on the device (n_cells + n_ghost entries, ghosts copy their
+ * interior cell): exp(sigma z), z standard normal, constant per block^3 cells
+ * on a hex mesh (block 1: per cell); and U[lo, hi). */
+int fvm_mesh_lognormal_field(fvm_mesh *m, double sigma, int block, uint64_t seed, double *d_out);
+int fvm_mesh_uniform_field(fvm_mesh *m, double lo, double hi, uint64_t seed, double *d_out);
+/* Assemble conductivity d_k, source d_source and current field d_x (device,
+ * n_cells + n_ghost entries) with boundary conditions per face group
+ * (bc_type/bc_value indexed by group id, entry 0 unused) into a new system. */
+int fvm_thermal_assemble(fvm_mesh *m, const double *d_k, const double *d_source, const double *d_x,
+                         const int32_t *bc_type, const double *bc_value, fvm_system **out);
+int fvm_system_info(const fvm_system *S, int *n_self, int *n_total, long long *nnz);
+int fvm_system_device(const fvm_system *S, const int32_t **row, const int32_t **col,
+                      const double **diag, const double **off, const double **b,
+                      const uint8_t **is_boundary);
+int fvm_system_get(const fvm_system *S, int32_t *row, int32_t *col, double *diag, double *off,
+                   double *b, uint8_t *is_boundary);
+int fvm_system_destroy(fvm_system *S);
+/* fvm_amg_setup_device on an assembled system */
+int fvm_amg_setup_system(fvm_amg *h, const fvm_system *S);
 
 #ifdef __cplusplus
 }

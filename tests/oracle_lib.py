@@ -51,6 +51,10 @@ def lib():
         L.om_residual_flat.argtypes = [_i, _p, _p, _p, _p, _p, _p, _p]
         L.om_gs_flat.argtypes = [_i, _p, _p, _p, _p, _p, _p, _i]
         L.om_coarsen_flat.argtypes = [_i, _i, _p, _p, _p, _p, _p, _i, _d, _p]
+        L.ofv_metrics.argtypes = [_i, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _p, _p]
+        L.ofv_thermal_nnz.argtypes = [_i, _p, _i, _i]
+        L.ofv_thermal.argtypes = [_i, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _p, _p,
+                                  _p, _p, _p, _p, _p, _p]
         _lib = L
     return _lib
 
@@ -160,3 +164,36 @@ def coarsen(m, group_size=2, theta=0.65, is_bnd=None):
     nc = lib().om_coarsen_flat(m["n_self"], m["n_total"], ptr(m["row"]), ptr(m["col"]),
                                ptr(m["diag"]), ptr(m["off"]), ptr(ib), group_size, theta, ptr(ci))
     return nc, ci
+
+
+def fv_metrics(mesh):
+    """Oracle mesh metrics of a mesh dict (tests/hexmesh.py layout)."""
+    nf = mesh["face_cells"].size // 2
+    nct = mesh["n_cells"] + mesh["n_ghost"]
+    out = dict(fA=np.zeros(3 * nf), fAm=np.zeros(nf), fc=np.zeros(3 * nf), cc=np.zeros(3 * nct),
+               vol=np.zeros(nct))
+    gp = np.ascontiguousarray(mesh["group_ptr"], np.int32)
+    lib().ofv_metrics(nf, ptr(mesh["nodes"]), ptr(mesh["face_nodes"]), ptr(mesh["face_cells"]),
+                      mesh["n_cells"], mesh["n_ghost"], gp.size - 1, ptr(gp), ptr(out["fA"]),
+                      ptr(out["fAm"]), ptr(out["fc"]), ptr(out["cc"]), ptr(out["vol"]))
+    return out
+
+
+def fv_thermal(mesh, k, src, x, bc_type, bc_value):
+    """Oracle thermal assembly -> CRMatrix system dict."""
+    L = lib()
+    nf = mesh["face_cells"].size // 2
+    nc, ng = mesh["n_cells"], mesh["n_ghost"]
+    nct = nc + ng
+    nnz = L.ofv_thermal_nnz(nf, ptr(mesh["face_cells"]), nc, ng)
+    row = np.zeros(nct + 1, np.int32); col = np.zeros(max(nnz, 1), np.int32)
+    diag = np.zeros(nct); off = np.zeros(max(nnz, 1)); b = np.zeros(nct); isb = np.zeros(nct, np.uint8)
+    gp = np.ascontiguousarray(mesh["group_ptr"], np.int32)
+    bt = np.ascontiguousarray(bc_type, np.int32); bv = np.ascontiguousarray(bc_value, np.float64)
+    k = np.ascontiguousarray(k, np.float64); src = np.ascontiguousarray(src, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    L.ofv_thermal(nf, ptr(mesh["nodes"]), ptr(mesh["face_nodes"]), ptr(mesh["face_cells"]), nc, ng,
+                  gp.size - 1, ptr(gp), ptr(bt), ptr(bv), ptr(k), ptr(src), ptr(x), ptr(row), ptr(col),
+                  ptr(diag), ptr(off), ptr(b), ptr(isb))
+    return dict(n_self=nc, n_total=nct, row=row, col=col[:nnz], diag=diag, off=off[:nnz], b=b,
+                is_boundary=isb)
