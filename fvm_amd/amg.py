@@ -169,6 +169,15 @@ class AMG:
         return dict(n=n.value, nnz=nnz.value, row=row, col=col[:nnz.value], diag=diag[:n.value],
                     off=off[:nnz.value], ci=ci[:n.value])
 
+    def set_tuning(self, key, value):
+        """Performance knob that does not change results (see fvm_amg_set_tuning)."""
+        check(lib().fvm_amg_set_tuning(self._h, key.encode(), int(value)))
+
+    def get_tuning(self, key):
+        v = ctypes.c_longlong()
+        check(lib().fvm_amg_get_tuning(self._h, key.encode(), ctypes.byref(v)))
+        return v.value
+
     def setup_stats(self):
         t = ctypes.c_double()
         r = ctypes.c_int()

@@ -45,6 +45,7 @@ struct Level {
     DArr<int> perm, iperm;
     int ncolor = 0;
     std::vector<int> color_ptr;        // host: row ranges of each colour (coloured index)
+    DArr<int> d_color_ptr;             // the same on the device
     DArr<int> sell_off, sell_col;
     DArr<double> sell_val, pdiag;
     DArr<int> pci, pc2f_row, pc2f_col;
@@ -69,6 +70,7 @@ struct Workspace {
 struct fvm_amg {
     int device = 0;
     hipStream_t stream = nullptr;
+    fvm::Pool pool;              // declared first: destroyed after every array it lends
     fvm_amg_params prm{};
     std::vector<fvm::Level> lev;
     fvm::Workspace ws;
@@ -83,6 +85,14 @@ struct fvm_amg {
     // mixed-precision coarsest dense inverse (FP64)
     int direct_level = -1;
     fvm::DArr<double> dense_inv;
+    // tuning knobs (fvm_amg_set_tuning): rows below which a level runs in the
+    // single-workgroup tail of the V-cycle / is coarsened by one workgroup
+    long long tail_max_rows = 40000;
+    long long coarsen_small_rows = 16384;
+    // single-workgroup tail of the V-cycle: levels >= tail_level run in one launch
+    int tail_level = -1;
+    int tail_prec = -1;                // precision the descriptors were built for
+    fvm::DArr<unsigned char> tail_desc;
     // live timing of finest-level kernels with HIP events on the handle's stream
     bool ktime = false;
     std::vector<hipEvent_t> ev_pool;

@@ -49,6 +49,13 @@ struct DeviceGuard {
     }
 };
 
+// device of the handle + its allocation pool for the duration of a call
+struct HandleScope {
+    DeviceGuard dg;
+    fvm::PoolScope ps;
+    explicit HandleScope(fvm_amg *h) : dg(h->device), ps(&h->pool) {}
+};
+
 void check_params(const fvm_amg_params *p) {
     FVM_CHECK(p, FVM_E_INVALID, "null parameter struct");
     FVM_CHECK(p->maxCoarseLevels >= 0, FVM_E_INVALID, "maxCoarseLevels must be >= 0");
@@ -130,7 +137,7 @@ int fvm_amg_destroy(fvm_amg *h) {
     FVM_API_BEGIN
     if (!h) return FVM_OK;
     {
-        DeviceGuard g(h->device);
+        HandleScope g(h);
         (void)hipStreamSynchronize(h->stream);
         h->lev.clear();
         h->ws = fvm::Workspace();
@@ -140,6 +147,7 @@ int fvm_amg_destroy(fvm_amg *h) {
         fvm::timing_harvest(h);
         for (auto e : h->ev_pool) (void)hipEventDestroy(e);
         h->ev_pool.clear();
+        h->pool.trim();
         (void)hipStreamDestroy(h->stream);
     }
     delete h;
@@ -158,7 +166,7 @@ int fvm_amg_set_params(fvm_amg *h, const fvm_amg_params *p) {
     h->prm = *p;
     if (rebuild) {
         // hierarchy parameters changed: the next solve needs a new setup (AMG rebuilds per solve)
-        DeviceGuard g(h->device);
+        HandleScope g(h);
         h->lev.clear();
         h->ready = false;
     }
@@ -178,7 +186,7 @@ int fvm_amg_setup_device(fvm_amg *h, int n_self, int n_total, const int32_t *d_r
     FVM_CHECK(h, FVM_E_INVALID, "null handle");
     FVM_CHECK(n_self >= 0 && n_total >= n_self, FVM_E_INVALID, "need 0 <= n_self <= n_total");
     FVM_CHECK(d_row && (n_total == 0 || (d_diag)), FVM_E_INVALID, "null matrix array");
-    DeviceGuard g(h->device);
+    HandleScope g(h);
     fvm::build_hierarchy(h, n_self, n_total, d_row, d_col, d_diag, d_off, d_is_boundary);
     return FVM_OK;
     FVM_API_END
@@ -193,7 +201,7 @@ int fvm_amg_setup(fvm_amg *h, int n_self, int n_total, const int32_t *row, const
     const long long nnz = n_total > 0 ? row[n_total] : 0;
     FVM_CHECK(nnz >= 0 && row[0] == 0, FVM_E_INVALID, "row pointers must start at 0");
     FVM_CHECK(nnz == 0 || (col && off), FVM_E_INVALID, "null col/off arrays");
-    DeviceGuard g(h->device);
+    HandleScope g(h);
     DArr<int> d_row((size_t)n_total + 1), d_col(std::max<long long>(nnz, 1));
     DArr<double> d_diag(std::max(n_total, 1)), d_off(std::max<long long>(nnz, 1));
     DArr<uint8_t> d_bnd;
@@ -221,7 +229,7 @@ int fvm_amg_solve_device(fvm_amg *h, const double *d_b, double *d_x, double *rno
     FVM_CHECK(h, FVM_E_INVALID, "null handle");
     FVM_CHECK(h->ready, FVM_E_NOT_SETUP, "solve called before setup");
     FVM_CHECK(h->n_self == 0 || (d_b && d_x), FVM_E_INVALID, "null vector");
-    DeviceGuard g(h->device);
+    HandleScope g(h);
     fvm::solve_device(h, d_b, d_x, rnorm0, rnorm_final, cycles);
     return FVM_OK;
     FVM_API_END
@@ -232,7 +240,7 @@ int fvm_amg_solve(fvm_amg *h, const double *b, double *x, double *rnorm0, double
     FVM_CHECK(h, FVM_E_INVALID, "null handle");
     FVM_CHECK(h->ready, FVM_E_NOT_SETUP, "solve called before setup");
     FVM_CHECK(h->n_total == 0 || (b && x), FVM_E_INVALID, "null vector");
-    DeviceGuard g(h->device);
+    HandleScope g(h);
     const size_t n = std::max(h->n_total, 1);
     h->h_stage_b.ensure(n);
     h->h_stage_x.ensure(n);
@@ -254,7 +262,7 @@ int fvm_amg_smooth_device(fvm_amg *h, const double *d_b, double *d_x) {
     FVM_API_BEGIN
     FVM_CHECK(h, FVM_E_INVALID, "null handle");
     FVM_CHECK(h->ready, FVM_E_NOT_SETUP, "smooth called before setup");
-    DeviceGuard g(h->device);
+    HandleScope g(h);
     fvm::smooth_device(h, d_b, d_x);
     return FVM_OK;
     FVM_API_END
@@ -265,7 +273,7 @@ int fvm_amg_smooth(fvm_amg *h, const double *b, double *x) {
     FVM_CHECK(h, FVM_E_INVALID, "null handle");
     FVM_CHECK(h->ready, FVM_E_NOT_SETUP, "smooth called before setup");
     FVM_CHECK(h->n_total == 0 || (b && x), FVM_E_INVALID, "null vector");
-    DeviceGuard g(h->device);
+    HandleScope g(h);
     const size_t n = std::max(h->n_total, 1);
     h->h_stage_b.ensure(n);
     h->h_stage_x.ensure(n);
@@ -286,7 +294,7 @@ int fvm_amg_smooth(fvm_amg *h, const double *b, double *x) {
 int fvm_amg_cleanup(fvm_amg *h) {
     FVM_API_BEGIN
     FVM_CHECK(h, FVM_E_INVALID, "null handle");
-    DeviceGuard g(h->device);
+    HandleScope g(h);
     FVM_HIP(hipStreamSynchronize(h->stream));
     h->lev.clear();
     h->dense_inv.release();
@@ -339,7 +347,7 @@ int fvm_amg_level_get(const fvm_amg *h, int level, int32_t *row, int32_t *col, d
 int fvm_amg_set_kernel_timing(fvm_amg *h, int enable) {
     FVM_API_BEGIN
     FVM_CHECK(h, FVM_E_INVALID, "null handle");
-    DeviceGuard g(h->device);
+    HandleScope g(h);
     FVM_HIP(hipStreamSynchronize(h->stream));
     fvm::timing_harvest(h);
     h->ktime = enable != 0;
@@ -352,6 +360,32 @@ int fvm_amg_kernel_timing(const fvm_amg *h, int kind, long long *launches, doubl
     if (!h || kind < 0 || kind > 3) return FVM_E_INVALID;
     if (launches) *launches = h->k_launches[kind];
     if (total_ms) *total_ms = h->k_ms[kind];
+    return FVM_OK;
+}
+
+int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value) {
+    FVM_API_BEGIN
+    FVM_CHECK(h && key, FVM_E_INVALID, "null argument");
+    const std::string k(key);
+    if (k == "tail_max_rows") {
+        FVM_CHECK(value >= 0, FVM_E_INVALID, "tail_max_rows must be >= 0");
+        h->tail_max_rows = value;
+    } else if (k == "coarsen_small_rows") {
+        FVM_CHECK(value >= 0, FVM_E_INVALID, "coarsen_small_rows must be >= 0");
+        h->coarsen_small_rows = value;
+    } else {
+        throw fvm::Error(FVM_E_INVALID, "unknown tuning key: " + k);
+    }
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value) {
+    if (!h || !key || !value) return FVM_E_INVALID;
+    const std::string k(key);
+    if (k == "tail_max_rows") *value = h->tail_max_rows;
+    else if (k == "coarsen_small_rows") *value = h->coarsen_small_rows;
+    else return FVM_E_INVALID;
     return FVM_OK;
 }
 
@@ -389,7 +423,7 @@ int fvm_fgmres_destroy(fvm_fgmres *s) {
     FVM_API_BEGIN
     if (!s) return FVM_OK;
     if (s->pc) {
-        DeviceGuard g(s->pc->device);
+        HandleScope g(s->pc);
         fvm::delete_krylov_work(s->work);
     }
     delete s;
@@ -408,7 +442,7 @@ int fvm_fgmres_solve_device(fvm_fgmres *s, const double *d_b, double *d_x, doubl
     FVM_API_BEGIN
     FVM_CHECK(s && s->pc, FVM_E_INVALID, "null solver");
     FVM_CHECK(s->pc->ready, FVM_E_NOT_SETUP, "preconditioner not set up");
-    DeviceGuard g(s->pc->device);
+    HandleScope g(s->pc);
     fvm::fgmres_solve(s->pc, *s->work, s->prm, d_b, d_x, rnorm0, rnorm_final, iterations, s->hist);
     return FVM_OK;
     FVM_API_END
@@ -420,7 +454,7 @@ int fvm_fgmres_solve(fvm_fgmres *s, const double *b, double *x, double *rnorm0, 
     FVM_CHECK(s && s->pc, FVM_E_INVALID, "null solver");
     fvm_amg *h = s->pc;
     FVM_CHECK(h->ready, FVM_E_NOT_SETUP, "preconditioner not set up");
-    DeviceGuard g(h->device);
+    HandleScope g(h);
     const size_t n = std::max(h->n_total, 1);
     h->h_stage_b.ensure(n);
     h->h_stage_x.ensure(n);

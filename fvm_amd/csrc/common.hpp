@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -35,37 +36,113 @@ void set_last_error(const std::string &msg);
         if (!(cond)) throw ::fvm::Error((code), (msg));                                      \
     } while (0)
 
-// Owning device array (hipMalloc'd). Not copyable; movable.
+// Caching device allocator of one handle. All work of a handle runs on its one
+// stream, so a block returned here can be handed out again at once: later
+// kernels on that stream are ordered after the earlier users of the block.
+// Avoids hipMalloc/hipFree (which synchronize and map pages) on every setup.
+struct Pool {
+    std::multimap<size_t, void *> free_blocks;
+    size_t cached = 0;
+    static size_t round(size_t b) {
+        if (b <= (1u << 20)) {
+            size_t r = 256;
+            while (r < b) r <<= 1;
+            return r;
+        }
+        const size_t g = 2u << 20;
+        return (b + g - 1) / g * g;
+    }
+    void *get(size_t bytes, size_t *cap) {
+        const size_t r = round(bytes);
+        auto it = free_blocks.lower_bound(r);
+        if (it != free_blocks.end() && it->first <= r + r / 4) {
+            void *p = it->second;
+            *cap = it->first;
+            cached -= it->first;
+            free_blocks.erase(it);
+            return p;
+        }
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, r);
+        if (e != hipSuccess) {
+            trim();   // give cached blocks back and retry once
+            e = hipMalloc(&p, r);
+        }
+        if (e != hipSuccess)
+            throw Error(-5, std::string("hipMalloc of ") + std::to_string(r) + " bytes failed: " +
+                                hipGetErrorString(e));
+        *cap = r;
+        return p;
+    }
+    void put(void *p, size_t cap) {
+        free_blocks.emplace(cap, p);
+        cached += cap;
+    }
+    void trim() {
+        for (auto &kv : free_blocks) (void)hipFree(kv.second);
+        free_blocks.clear();
+        cached = 0;
+    }
+    ~Pool() { trim(); }
+};
+
+// The pool that DArr allocations of the current thread draw from (nullptr:
+// plain hipMalloc). Set for the duration of an entry point by PoolScope.
+inline Pool *&current_pool() {
+    static thread_local Pool *p = nullptr;
+    return p;
+}
+struct PoolScope {
+    Pool *prev;
+    explicit PoolScope(Pool *p) : prev(current_pool()) { current_pool() = p; }
+    ~PoolScope() { current_pool() = prev; }
+};
+
+// Owning device array. Not copyable; movable.
 template <class T>
 struct DArr {
     T *p = nullptr;
     size_t n = 0;
+    size_t cap = 0;
+    Pool *pool = nullptr;
     DArr() = default;
     explicit DArr(size_t count) { alloc(count); }
     DArr(const DArr &) = delete;
     DArr &operator=(const DArr &) = delete;
-    DArr(DArr &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DArr(DArr &&o) noexcept : p(o.p), n(o.n), cap(o.cap), pool(o.pool) { o.p = nullptr; o.n = 0; o.cap = 0; o.pool = nullptr; }
     DArr &operator=(DArr &&o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        if (this != &o) {
+            release();
+            p = o.p; n = o.n; cap = o.cap; pool = o.pool;
+            o.p = nullptr; o.n = 0; o.cap = 0; o.pool = nullptr;
+        }
         return *this;
     }
     ~DArr() { release(); }
     void alloc(size_t count) {
         release();
         n = count;
-        if (count) {
+        if (!count) return;
+        pool = current_pool();
+        if (pool) {
+            p = (T *)pool->get(count * sizeof(T), &cap);
+        } else {
             hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
             if (e != hipSuccess) {
                 p = nullptr; n = 0;
                 throw Error(-5, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
                                     " bytes failed: " + hipGetErrorString(e));
             }
+            cap = count * sizeof(T);
         }
     }
     void ensure(size_t count) { if (n < count) alloc(count); }
     void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr; n = 0;
+        if (p) {
+            if (pool) pool->put(p, cap);
+            else (void)hipFree(p);
+        }
+        p = nullptr; n = 0; cap = 0; pool = nullptr;
     }
     size_t bytes() const { return n * sizeof(T); }
 };

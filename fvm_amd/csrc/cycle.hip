@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 namespace fvm {
 
@@ -315,6 +316,146 @@ __global__ void k_dense_apply(int n, const double *inv, const int *perm, const i
     }
 }
 
+// ------------------------------------------------------------------ single-workgroup tail
+// The small levels of a V-cycle in one launch: every step (restriction, a
+// colour of a sweep, prolongation) is a loop of the workgroup over the rows of
+// one level, steps separated by workgroup barriers. Same arithmetic per row as
+// the per-level kernels; it only removes the launches.
+struct TailLv {
+    int n, ncolor;
+    const int *soff, *scol, *cptr, *perm, *iperm;
+    const void *val, *diag;
+    const int *pci;                 // fine -> coarse (level + 1), coloured indices
+    const int *c2f_row, *c2f_col;   // rows of level + 1 -> rows of this level
+    void *b, *x, *t;
+};
+
+template <class V, class D, class X>
+__device__ __forceinline__ void tail_rows(const TailLv &L, int beg, int end, const X *xin, X *xout,
+                                          const X *b) {
+    const V *val = (const V *)L.val;
+    const D *dg = (const D *)L.diag;
+    for (int p = beg + threadIdx.x; p < end; p += blockDim.x) {
+        const int s = p >> 6, lane = p & 63;
+        const int o = L.soff[s];
+        const int w = (L.soff[s + 1] - o) >> 6;
+        X acc = b[p];
+        for (int k = 0; k < w; k++) {
+            const int e = o + k * 64 + lane;
+            acc += (X)ldv(val, e) * xin[L.scol[e]];
+        }
+        xout[p] = -acc / (X)dg[p];
+    }
+}
+
+template <class V, class D, class X>
+__device__ void tail_sweeps(const TailLv &L, int ns, int jacobi) {
+    X *x = (X *)L.x;
+    const X *b = (const X *)L.b;
+    for (int k = 0; k < ns; k++) {
+        if (!jacobi) {
+            for (int c = 0; c < L.ncolor; c++) {
+                tail_rows<V, D, X>(L, L.cptr[c], L.cptr[c + 1], x, x, b);
+                __syncthreads();
+            }
+            for (int c = L.ncolor - 1; c >= 0; c--) {
+                tail_rows<V, D, X>(L, L.cptr[c], L.cptr[c + 1], x, x, b);
+                __syncthreads();
+            }
+        } else {
+            X *t = (X *)L.t;
+            tail_rows<V, D, X>(L, 0, L.n, x, t, b);
+            __syncthreads();
+            tail_rows<V, D, X>(L, 0, L.n, t, x, b);
+            __syncthreads();
+        }
+    }
+}
+
+template <class V, class D, class X>
+__device__ void tail_fill0(const TailLv &L) {
+    X *x = (X *)L.x;
+    for (int p = threadIdx.x; p < L.n; p += blockDim.x) x[p] = (X)0;
+}
+
+template <class V, class D, class X>
+__global__ void __launch_bounds__(1024)
+k_tail(const TailLv *lv, int l0, int last, int npre, int npost, int jacobi, const double *inv, int direct) {
+    // descend: level l0 holds its b, its x is zero
+    for (int l = l0; l < last; l++) {
+        const TailLv &L = lv[l];
+        const X *rr = (const X *)L.b;
+        if (npre > 0) {
+            tail_fill0<V, D, X>(L);
+            __syncthreads();
+            tail_sweeps<V, D, X>(L, npre, jacobi);
+            // residual r = b + A x into t
+            const V *val = (const V *)L.val;
+            const D *dg = (const D *)L.diag;
+            const X *x = (const X *)L.x;
+            const X *b = (const X *)L.b;
+            X *r = (X *)L.t;
+            for (int p = threadIdx.x; p < L.n; p += blockDim.x) {
+                const int s = p >> 6, lane = p & 63;
+                const int o = L.soff[s];
+                const int w = (L.soff[s + 1] - o) >> 6;
+                X acc = b[p] + (X)dg[p] * x[p];
+                for (int k = 0; k < w; k++) {
+                    const int e = o + k * 64 + lane;
+                    acc += (X)ldv(val, e) * x[L.scol[e]];
+                }
+                r[p] = acc;
+            }
+            __syncthreads();
+            rr = r;
+        }
+        const TailLv &C = lv[l + 1];
+        X *bc = (X *)C.b;
+        for (int I = threadIdx.x; I < C.n; I += blockDim.x) {
+            X sum = 0;
+            for (int q = L.c2f_row[I]; q < L.c2f_row[I + 1]; q++) sum += rr[L.c2f_col[q]];
+            bc[I] = sum;
+        }
+        __syncthreads();
+    }
+    // coarsest level of the tail
+    {
+        const TailLv &L = lv[last];
+        if (direct) {
+            const X *b = (const X *)L.b;
+            X *x = (X *)L.x;
+            const int n = L.n;
+            for (int p = threadIdx.x; p < n; p += blockDim.x) {
+                const int i = L.perm[p];
+                double sum = 0;
+                for (int j = 0; j < n; j++) sum += inv[(size_t)i * n + j] * (double)b[L.iperm[j]];
+                x[p] = (X)(-sum);
+            }
+        } else {
+            tail_fill0<V, D, X>(L);
+            __syncthreads();
+            tail_sweeps<V, D, X>(L, npre + npost, jacobi);
+        }
+        __syncthreads();
+    }
+    // ascend
+    for (int l = last - 1; l >= l0; l--) {
+        const TailLv &L = lv[l];
+        const X *xc = (const X *)lv[l + 1].x;
+        X *x = (X *)L.x;
+        for (int i = threadIdx.x; i < L.n; i += blockDim.x) {
+            const int I = L.pci[i];
+            if (npre > 0) {
+                if (I >= 0) x[i] = x[i] + xc[I];
+            } else {
+                x[i] = I >= 0 ? (X)0 + xc[I] : (X)0;
+            }
+        }
+        __syncthreads();
+        tail_sweeps<V, D, X>(L, npost, jacobi);
+    }
+}
+
 }  // namespace
 
 // ====================================================================== host side
@@ -425,7 +566,7 @@ void jacobi_pass(Run &R, int l) {
     FVM_DISPATCH_VALS(R, l, L, X,
         (k_sell_jacobi<<<grid_for(n, BS), BS, 0, s>>>(n, L.sell_off.p, L.sell_col.p, val, dg, xa.p, b, ta.p)),
         (k_csr_jacobi<<<grid_for(n, BS), BS, 0, s>>>(n, L.row.p, L.col.p, val, dg, xa.p, b, ta.p)));
-    std::swap(xa.p, ta.p);   // xnew becomes x (MultiFieldMatrix.cpp:203-215 copies it back)
+    std::swap(xa, ta);   // xnew becomes x (MultiFieldMatrix.cpp:203-215 copies it back)
 }
 
 template <class X>
@@ -470,11 +611,53 @@ void symmetric_scale(Run &R, int l, double *scale_out) {
     k_scale<<<1, 1, 0, s>>>(scratch_scalar(h, 8), scratch_scalar(h, 9), scale_out);
 }
 
+// Descriptors of every level for the tail kernel (coloured layout), built once per precision.
+template <class X>
+void build_tail_desc(Run &R) {
+    fvm_amg *h = R.h;
+    const int prec = R.mixed ? 1 : 0;
+    if (h->tail_prec == prec && h->tail_desc.p) return;
+    std::vector<TailLv> d(h->lev.size());
+    for (size_t l = 0; l < h->lev.size(); l++) {
+        Level &L = h->lev[l];
+        TailLv &t = d[l];
+        t.n = L.n;
+        t.ncolor = L.ncolor;
+        t.soff = L.sell_off.p; t.scol = L.sell_col.p; t.cptr = L.d_color_ptr.p;
+        t.perm = L.perm.p; t.iperm = L.iperm.p;
+        const int vt = value_type(R, (int)l);
+        t.val = vt == V64 ? (const void *)L.sell_val.p : vt == V32 ? (const void *)L.sell_val32.p
+                                                                      : (const void *)L.sell_bf16.p;
+        t.diag = vt == V64 ? (const void *)L.pdiag.p : (const void *)L.pdiag32.p;
+        t.pci = L.pci.p; t.c2f_row = L.pc2f_row.p; t.c2f_col = L.pc2f_col.p;
+        t.b = vb<X>(L); t.x = vx<X>(L); t.t = vt_arr<X>(L).p;
+    }
+    h->tail_desc.alloc(sizeof(TailLv) * d.size());
+    FVM_HIP(hipMemcpyAsync(h->tail_desc.p, d.data(), sizeof(TailLv) * d.size(), hipMemcpyHostToDevice, h->stream));
+    FVM_HIP(hipStreamSynchronize(h->stream));
+    h->tail_prec = prec;
+}
+
 template <class X>
 void cycle(Run &R, int type, int l, bool x_zero) {
     fvm_amg *h = R.h;
     Level &L = h->lev[l];
     hipStream_t s = h->stream;
+    if (R.fast && l == h->tail_level && type == FVM_V_CYCLE && x_zero && l <= R.last &&
+        !(h->prm.isSymmetric && h->prm.scaleCorrections)) {
+        build_tail_desc<X>(R);
+        const TailLv *d = (const TailLv *)h->tail_desc.p;
+        const int direct = (R.mixed && R.last == h->direct_level) ? 1 : 0;
+        const int jac = h->prm.smootherType == FVM_JACOBI ? 1 : 0;
+        if (R.mixed)
+            k_tail<uint16_t, float, float><<<1, 1024, 0, s>>>(d, l, R.last, h->prm.nPreSweeps, h->prm.nPostSweeps,
+                                                               jac, h->dense_inv.p, direct);
+        else
+            k_tail<double, double, double><<<1, 1024, 0, s>>>(d, l, R.last, h->prm.nPreSweeps,
+                                                                h->prm.nPostSweeps, jac, h->dense_inv.p, direct);
+        FVM_HIP(hipGetLastError());
+        return;
+    }
     if (R.mixed && l == h->direct_level) {
         const bool perm = R.fast;
         k_dense_apply<X><<<grid_for(L.n, 64), 64, 0, s>>>(L.n, h->dense_inv.p, perm ? L.perm.p : nullptr,
@@ -539,6 +722,11 @@ Run make_run(fvm_amg *h) {
     }
     R.last = (int)h->lev.size() - 1;
     if (R.mixed && h->direct_level >= 0) R.last = h->direct_level;
+    h->tail_level = -1;
+    if (R.fast) {
+        for (int l = 1; l <= R.last; l++)
+            if (h->lev[l].n <= h->tail_max_rows) { h->tail_level = l; break; }
+    }
     return R;
 }
 
@@ -825,6 +1013,7 @@ void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const d
     const int n = F.n;
     const int m = std::max(1, p.restart);
     if (W.n != n || W.m != m) {
+        PoolScope no_pool(nullptr);   // the Krylov basis may outlive the preconditioner's pool
         W.n = n; W.m = m;
         W.V.alloc((size_t)(m + 1) * std::max(n, 1));
         W.Z.alloc((size_t)m * std::max(n, 1));

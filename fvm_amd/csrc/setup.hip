@@ -17,6 +17,7 @@
 #include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 
 namespace fvm {
 
@@ -63,151 +64,207 @@ __global__ void k_compact_fill(int n_self, const int *row, const int *col, const
 }
 
 // ------------------------------------------------------------------ coarsening
-__global__ void k_coarsen_init(int n, const uint8_t *bnd, int *ldr, int *dround, int *memr,
-                               int *mcnt, int *stamp, int *remaining) {
-    int local = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const bool b = isb(bnd, i);
-        ldr[i] = -1;
-        dround[i] = b ? -1 : INT_MAX;
-        mcnt[i] = 0;
-        stamp[i] = 0;
-        for (int k = 0; k < MEMW; k++) memr[(size_t)i * MEMW + k] = INT_MAX;
-        local += b ? 0 : 1;
+// ------------------------------------------------------------------ coarsening
+//
+// Exact parallel form of CRMatrix::createCoarsening for groupSize 2
+// (CRMatrix.h:485-583). In the sequential loop, when row r's turn comes every
+// non-boundary row < r is grouped, and a row c > r is grouped iff an earlier
+// seed s < r with c in row(s) paired it. With rowT(c) = {s : c in row(s)}, the
+// decision of r can only depend on the rows of
+//   D(r) = {s < r : s in rowT(r)}              (s could pair r)
+//        + {c < r : c in row(r)}               (r needs c's group)
+//        + {s < r : s in rowT(c), c in row(r), c > r}   (s could pair c first)
+// counted as a multiset. cnt[r] starts at |D(r)| and is decremented when a row
+// of D(r) decides; r enters the frontier when it reaches 0 (Kahn's algorithm
+// on this DAG). One round decides the whole frontier; all its inputs were
+// decided in earlier rounds (state stamped with a round < t), so rows of one
+// round never read what another writes, and the result equals the sequential
+// one. A row that would join the group g of its strongest grouped neighbour
+// also needs every s < r adjacent (through rowT) to a member of g decided
+// (s could still join g and change its count); if not, it retries next round.
+// Boundary rows never decide and are dropped from all dependency sets.
+
+#ifndef FVM_COARSEN_SUB
+#define FVM_COARSEN_SUB 16
+#endif
+constexpr int SUB = FVM_COARSEN_SUB;   // lanes that process one frontier row
+
+__device__ __forceinline__ int sub_lane() { return threadIdx.x & (SUB - 1); }
+
+// append x to the queue when want; must be reached by every active lane
+#ifdef FVM_PUSH_PLAIN
+__device__ __forceinline__ void push_agg(bool want, int x, int *q, int *qcnt) {
+    if (want) q[atomicAdd(qcnt, 1)] = x;
+}
+#else
+__device__ __forceinline__ void push_agg(bool want, int x, int *q, int *qcnt) {
+    const unsigned long long m = __ballot(want);
+    if (!m) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(qcnt, __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (want) q[base + __popcll(m & ((1ull << lane) - 1ull))] = x;
+}
+#endif
+
+__global__ void k_cnt_init(int n, const int *row, const int *col, const int *rowT, const int *colT,
+                           int sym, const uint8_t *bnd, int *cnt, int *dround, int *ldr, int *memr,
+                           int *mcnt, int *q, int *qcnt, int *remaining) {
+    const long long gt = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const int r = (int)(gt / SUB);
+    const int lane = sub_lane();
+    const bool valid = r < n;
+    int k = 0;
+    bool b = true;
+    if (valid) {
+        b = isb(bnd, r);
+        if (!b) {
+            if (!sym)
+                for (int qq = rowT[r] + lane; qq < rowT[r + 1]; qq += SUB) {
+                    const int s = colT[qq];
+                    k += (s < r && !isb(bnd, s));
+                }
+            for (int p = row[r] + lane; p < row[r + 1]; p += SUB) {
+                const int c = col[p];
+                if (isb(bnd, c)) continue;
+                if (c < r) {
+                    k += sym ? 2 : 1;   // symmetric: the rowT part is the same set
+                } else {
+                    for (int qq = rowT[c]; qq < rowT[c + 1]; qq++) {
+                        const int s = colT[qq];
+                        k += (s < r && !isb(bnd, s));
+                    }
+                }
+            }
+        }
     }
-    if (local) atomicAdd(remaining, local);
+    for (int o = SUB / 2; o > 0; o >>= 1) k += __shfl_xor(k, o, 64);
+    const bool head = valid && lane == 0;
+    if (head) {
+        cnt[r] = k;
+        ldr[r] = -1;
+        dround[r] = b ? -1 : INT_MAX;
+        mcnt[r] = 0;
+        for (int z = 0; z < MEMW; z++) memr[(size_t)r * MEMW + z] = INT_MAX;
+    }
+    push_agg(head && !b && k == 0, r, q, qcnt);
+    const unsigned long long live = __ballot(head && !b);
+    if (__lane_id() == 0 && live) atomicAdd(remaining, __popcll(live));
 }
 
-__device__ __forceinline__ void push_cand(int x, int t, const uint8_t *bnd, const int *dround,
-                                          int *stamp, int *next, int *next_cnt) {
-    if (isb(bnd, x)) return;
-    if (dround[x] <= t) return;               // already decided
-    if (stamp[x] >= t + 1) return;            // already queued for round t+1
-    if (atomicMax(&stamp[x], t + 1) < t + 1) {
-        const int k = atomicAdd(next_cnt, 1);
-        next[k] = x;
+// decrement the dependents of d (rows whose D-set contains d)
+__device__ void fan_out(int d, const int *row, const int *col, const int *rowT, const int *colT, int sym,
+                        const uint8_t *bnd, int *cnt, int *q, int *qcnt) {
+    const int lane = sub_lane();
+    // A: r in row(d), r > d (d in rowT(r))
+    const int dec = sym ? 2 : 1;
+    for (int base = row[d]; base < row[d + 1]; base += SUB) {
+        const int p = base + lane;
+        bool want = false;
+        int x = -1;
+        if (p < row[d + 1]) {
+            x = col[p];
+            if (x > d && !isb(bnd, x)) want = (atomicSub(&cnt[x], dec) == dec);
+        }
+        push_agg(want, x, q, qcnt);
+    }
+    // B: r in rowT(d), r > d (d in row(r)); for a symmetric pattern it is the set
+    // of A again, counted twice in cnt: decrement by 2 there
+    if (!sym) {
+        for (int base = rowT[d]; base < rowT[d + 1]; base += SUB) {
+            const int p = base + lane;
+            bool want = false;
+            int x = -1;
+            if (p < rowT[d + 1]) {
+                x = colT[p];
+                if (x > d && !isb(bnd, x)) want = (atomicSub(&cnt[x], 1) == 1);
+            }
+            push_agg(want, x, q, qcnt);
+        }
+    }
+    // C: for c in row(d), c > d: r in rowT(c) with d < r < c
+    for (int pc = row[d]; pc < row[d + 1]; pc++) {
+        const int c = col[pc];
+        if (c <= d || isb(bnd, c)) continue;
+        for (int base = rowT[c]; base < rowT[c + 1]; base += SUB) {
+            const int p = base + lane;
+            bool want = false;
+            int x = -1;
+            if (p < rowT[c + 1]) {
+                x = colT[p];
+                if (x > d && x < c && !isb(bnd, x)) want = (atomicSub(&cnt[x], 1) == 1);
+            }
+            push_agg(want, x, q, qcnt);
+        }
     }
 }
 
-// Rows whose readiness may depend on d having decided (see k_coarsen_round):
-// rows > d in row(d), in rowT(d), and in rowT(c) for c in row(d).
-__device__ void notify(int d, int t, const int *row, const int *col, const int *rowT,
-                       const int *colT, int sym, const uint8_t *bnd, const int *dround, int *stamp,
-                       int *next, int *next_cnt) {
-    for (int p = row[d]; p < row[d + 1]; p++) {
+// Decide one frontier row with the SUB lanes of its sub-group. Returns via the
+// queue (fan-out) and the shared state arrays.
+__device__ void decide_row(int r, int t, double theta, const int *row, const int *col, const int *rowT,
+                           const int *colT, int sym, const double *off, const double *diag,
+                           const uint8_t *bnd, int *ldr, int *dround, int *mem, int *memr, int *mcnt,
+                           int *cnt, int *q, int *qcnt, int *remaining) {
+    const int lane = sub_lane();
+    if (dround[r] < t) return;   // already paired by an earlier seed (its fan-out is done)
+    // strongest ungrouped (u) and grouped (g) neighbours; ties: first in CSR order
+    const double d0 = fabs(diag[r]);
+    double mu = -1.0, mg = -1.0;
+    int pu = INT_MAX, pg = INT_MAX;
+    for (int p = row[r] + lane; p < row[r + 1]; p += SUB) {
         const int c = col[p];
-        if (c > d) push_cand(c, t, bnd, dround, stamp, next, next_cnt);
-        for (int q = rowT[c]; q < rowT[c + 1]; q++) {
-            const int c2 = colT[q];
-            if (c2 > d) push_cand(c2, t, bnd, dround, stamp, next, next_cnt);
+        if (isb(bnd, c)) continue;
+        const double d1 = fabs(diag[c]);
+        const double w = fabs(fabs(off[p]) / (d0 < d1 ? d1 : d0));
+        if (!(dround[c] < t)) {
+            if (pu == INT_MAX || w > mu) { mu = w; pu = p; }
+        } else {
+            if (pg == INT_MAX || w > mg) { mg = w; pg = p; }
         }
     }
-    if (!sym)
-        for (int q = rowT[d]; q < rowT[d + 1]; q++) {
-            const int c = colT[q];
-            if (c > d) push_cand(c, t, bnd, dround, stamp, next, next_cnt);
-        }
-}
-
-// One round of the exact parallel form of CRMatrix::createCoarsening for
-// groupSize 2 (CRMatrix.h:485-583).
-//
-// State: dround[i] = round in which row i became grouped (INT_MAX: not yet,
-// -1: boundary row); ldr[i] = the seed row ("leader") of i's group; per leader
-// the member list mem/memr (member row, round it joined). Only state stamped
-// with a round < t is read as decided, so every decision of round t is a
-// function of the state after round t-1.
-//
-// In the sequential order, when row r comes up every non-boundary row < r is
-// grouped, and a row c > r is grouped iff an earlier seed s < r with c in
-// row(s) paired it. rowT is the transpose pattern: rowT(c) = {s : c in row(s)}.
-// Row r may decide in round t when
-//   R1  every s < r in rowT(r) has decided (s could still pair r);
-//   R1a every c < r in row(r) has decided (r needs c's group);
-//   R2  for every undecided c > r in row(r), every s < r in rowT(c) has decided
-//       (s could still pair c before r's turn);
-//   R3  (only when r would join the group g of its strongest grouped neighbour)
-//       every s < r in rowT(m) of a member m of g has decided (s could still
-//       join g and change g's count).
-// Two rows that are both ready in one round never write state the other reads,
-// so the result equals the sequential one (checked against the oracle on every
-// level, on symmetric and non-symmetric patterns).
-__global__ void k_coarsen_round(int t, int all_rows, int n, const int *row, const int *col,
-                                const int *rowT, const int *colT, int sym, const double *off,
-                                const double *diag, const uint8_t *bnd, double theta, int *ldr,
-                                int *dround, int *mem, int *memr, int *mcnt, int *stamp,
-                                const int *cand, const int *cand_cnt, int *next, int *next_cnt,
-                                int *zero_cnt, int *remaining) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *zero_cnt = 0;
-    const int ncand = all_rows ? n : *cand_cnt;
-    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < ncand;
-         idx += gridDim.x * blockDim.x) {
-        const int r = all_rows ? idx : cand[idx];
-        if (dround[r] < t) continue;
-        bool ready = true;
-        // R1
-        if (!sym)
-            for (int q = rowT[r]; q < rowT[r + 1]; q++) {
-                const int s = colT[q];
-                if (s < r && !isb(bnd, s) && !(dround[s] < t)) { ready = false; break; }
-            }
-        // R1a / R2
-        for (int p = row[r]; p < row[r + 1] && ready; p++) {
-            const int c = col[p];
-            if (isb(bnd, c)) continue;
-            if (c < r) {
-                if (!(dround[c] < t)) ready = false;
-            } else if (!(dround[c] < t)) {
-                for (int q = rowT[c]; q < rowT[c + 1]; q++) {
-                    const int s = colT[q];
-                    if (s < r && !isb(bnd, s) && !(dround[s] < t)) { ready = false; break; }
-                }
+    for (int o = SUB / 2; o > 0; o >>= 1) {
+        const double mu2 = __shfl_xor(mu, o, 64);
+        const int pu2 = __shfl_xor(pu, o, 64);
+        if (pu2 != INT_MAX && (pu == INT_MAX || mu2 > mu || (mu2 == mu && pu2 < pu))) { mu = mu2; pu = pu2; }
+        const double mg2 = __shfl_xor(mg, o, 64);
+        const int pg2 = __shfl_xor(pg, o, 64);
+        if (pg2 != INT_MAX && (pg == INT_MAX || mg2 > mg || (mg2 == mg && pg2 < pg))) { mg = mg2; pg = pg2; }
+    }
+    const int cu = pu == INT_MAX ? -1 : col[pu];
+    const int cg = pg == INT_MAX ? -1 : col[pg];
+    int action;  // 0 pair, 1 new singleton group, 2 join cg's group, 3 retry next round
+    int g = -1;
+    if (cu != -1 && (cg == -1 || mu > theta * mg)) {
+        action = 0;
+    } else if (cg == -1) {
+        action = 1;
+    } else {
+        g = ldr[cg];
+        int count = 0;
+        bool blocked = false;
+        for (int k = 0; k < MEMW; k++) {
+            if (!(memr[(size_t)g * MEMW + k] < t)) continue;
+            count++;
+            const int m = mem[(size_t)g * MEMW + k];
+            for (int p = rowT[m] + lane; p < rowT[m + 1]; p += SUB) {
+                const int s = colT[p];
+                if (s < r && !isb(bnd, s) && !(dround[s] < t)) blocked = true;
             }
         }
-        if (!ready) continue;
-        // strongest ungrouped / grouped neighbours; weight uses the seed's diagonal
-        const double d0 = fabs(diag[r]);
-        int cu = -1, cg = -1;
-        double mu = 0, mg = 0;
-        for (int p = row[r]; p < row[r + 1]; p++) {
-            const int c = col[p];
-            if (isb(bnd, c)) continue;
-            const double d1 = fabs(diag[c]);
-            const double w = fabs(fabs(off[p]) / (d0 < d1 ? d1 : d0));
-            if (!(dround[c] < t)) {
-                if (cu == -1 || w > mu) { cu = c; mu = w; }
-            } else {
-                if (cg == -1 || w > mg) { cg = c; mg = w; }
-            }
-        }
-        int action;  // 0 pair, 1 new singleton group, 2 join cg's group
-        int g = -1;
-        if (cu != -1 && (cg == -1 || mu > theta * mg)) {
-            action = 0;
-        } else if (cg == -1) {
-            action = 1;
-        } else {
-            g = ldr[cg];
-            int count = 0;
-            bool blocked = false;
-            for (int k = 0; k < MEMW && !blocked; k++) {
-                if (!(memr[(size_t)g * MEMW + k] < t)) continue;
-                count++;
-                const int m = mem[(size_t)g * MEMW + k];
-                for (int q = rowT[m]; q < rowT[m + 1]; q++) {
-                    const int s = colT[q];
-                    if (s < r && !isb(bnd, s) && !(dround[s] < t)) { blocked = true; break; }
-                }
-            }
-            if (blocked) {  // R3: retry next round
-                if (atomicMax(&stamp[r], t + 1) < t + 1) {
-                    const int k = atomicAdd(next_cnt, 1);
-                    next[k] = r;
-                }
-                continue;
-            }
-            action = (count > 2 + 2) ? 1 : 2;   // coarseCount > groupSize+2
-        }
+        // every lane must take part in each shuffle (no short-circuit): a lane
+        // that skipped it would leave its partner reading an inactive lane
+        int bl = blocked ? 1 : 0;
+        for (int o = SUB / 2; o > 0; o >>= 1) bl |= __shfl_xor(bl, o, 64);
+        blocked = bl != 0;
+        action = blocked ? 3 : ((count > 2 + 2) ? 1 : 2);   // coarseCount > groupSize+2
+    }
+    if (action == 3) {
+        push_agg(lane == 0, r, q, qcnt);
+        return;
+    }
+    if (lane == 0) {
         if (action == 0) {
             ldr[r] = r; ldr[cu] = r;
             mem[(size_t)r * MEMW] = r; memr[(size_t)r * MEMW] = t;
@@ -229,9 +286,68 @@ __global__ void k_coarsen_round(int t, int all_rows, int n, const int *row, cons
             dround[r] = t;
             atomicSub(remaining, 1);
         }
-        notify(r, t, row, col, rowT, colT, sym, bnd, dround, stamp, next, next_cnt);
-        if (action == 0) notify(cu, t, row, col, rowT, colT, sym, bnd, dround, stamp, next, next_cnt);
     }
+    fan_out(r, row, col, rowT, colT, sym, bnd, cnt, q, qcnt);
+    if (action == 0) fan_out(cu, row, col, rowT, colT, sym, bnd, cnt, q, qcnt);
+}
+
+// One round over the frontier (multi-block). Reads frontier t from cur, appends
+// frontier t+1 to nxt; zeroes the counter of round t+2.
+__global__ void __launch_bounds__(256)
+k_coarsen_round(int t, const int *row, const int *col, const int *rowT, const int *colT, int sym,
+                const double *off, const double *diag, const uint8_t *bnd, double theta, int *ldr,
+                int *dround, int *mem, int *memr, int *mcnt, int *cnt, const int *cur, const int *cur_cnt,
+                int *nxt, int *nxt_cnt, int *zero_cnt, int *remaining) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *zero_cnt = 0;
+    const int nf = *cur_cnt;
+    const int per_wave = 64 / SUB;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int base = wave * per_wave; base < nf; base += nwaves * per_wave) {
+        const int i = base + (__lane_id() / SUB);
+        if (i < nf)
+            decide_row(cur[i], t, theta, row, col, rowT, colT, sym, off, diag, bnd, ldr, dround, mem, memr,
+                       mcnt, cnt, nxt, nxt_cnt, remaining);
+    }
+}
+
+// All rounds of a small level in one workgroup: rounds are separated by
+// __syncthreads instead of kernel launches. Frontier buffers and counters live
+// in global memory (L2-resident at these sizes).
+__global__ void __launch_bounds__(1024)
+k_coarsen_small(int n, int max_rounds, const int *row, const int *col, const int *rowT, const int *colT,
+                int sym, const double *off, const double *diag, const uint8_t *bnd, double theta, int *ldr,
+                int *dround, int *mem, int *memr, int *mcnt, int *cnt, int *qa, int *qb, int *qcnt,
+                int *remaining, int *rounds_out) {
+    __shared__ int s_nf, s_rem;
+    int *cur = qa, *nxt = qb;
+    int t = 1;
+    for (; t <= max_rounds; t++) {
+        if (threadIdx.x == 0) {
+            // the counters are updated by atomics (performed in L2): read and reset them
+            // atomically too, a plain load could return a stale L1 copy
+            s_nf = atomicAdd(&qcnt[t & 1], 0);
+            s_rem = atomicAdd(remaining, 0);
+            atomicExch(&qcnt[(t + 1) & 1], 0);
+        }
+        __syncthreads();
+        if (s_rem == 0) break;
+        __threadfence();   // see every store of the previous round (device-scope acquire)
+        const int nf = s_nf;
+        const int per_wave = 64 / SUB;
+        const int wave = threadIdx.x >> 6;
+        const int nwaves = blockDim.x >> 6;
+        for (int base = wave * per_wave; base < nf; base += nwaves * per_wave) {
+            const int i = base + (__lane_id() / SUB);
+            if (i < nf)
+                decide_row(cur[i], t, theta, row, col, rowT, colT, sym, off, diag, bnd, ldr, dround, mem,
+                           memr, mcnt, cnt, nxt, &qcnt[(t + 1) & 1], remaining);
+        }
+        __threadfence();   // publish this round's stores before the barrier
+        __syncthreads();
+        int *tmp = cur; cur = nxt; nxt = tmp;
+    }
+    if (threadIdx.x == 0) *rounds_out = t;
 }
 
 // transpose pattern (order inside a row is irrelevant to its users)
@@ -470,9 +586,16 @@ __global__ void k_invert_perm(int n, const int *perm, int *iperm) {
         iperm[perm[p]] = p;
 }
 
+// colour histogram: per-block counts in LDS, one global atomic per colour and block
 __global__ void k_color_hist(int n, const int *color, int *hist) {
+    __shared__ int h[257];
+    for (int c = threadIdx.x; c < 257; c += blockDim.x) h[c] = 0;
+    __syncthreads();
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        atomicAdd(&hist[color[i]], 1);
+        atomicAdd(&h[color[i]], 1);
+    __syncthreads();
+    for (int c = threadIdx.x; c < 257; c += blockDim.x)
+        if (h[c]) atomicAdd(&hist[c], h[c]);
 }
 
 // ------------------------------------------------------------------ SELL-64
@@ -614,6 +737,23 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Phase timing of the setup, printed to stderr when FVM_AMD_SETUP_TIMING is set.
+struct PhaseTimer {
+    fvm_amg *h;
+    bool on;
+    double t;
+    explicit PhaseTimer(fvm_amg *hh) : h(hh), on(getenv("FVM_AMD_SETUP_TIMING") != nullptr), t(0) {
+        if (on) { (void)hipStreamSynchronize(h->stream); t = now_s(); }
+    }
+    void mark(const char *what, int level) {
+        if (!on) return;
+        (void)hipStreamSynchronize(h->stream);
+        const double n = now_s();
+        fprintf(stderr, "[setup] level %2d %-18s %9.3f ms\n", level, what, 1e3 * (n - t));
+        t = n;
+    }
+};
+
 // Structural symmetry test and, for non-symmetric patterns, the transpose pattern.
 void ensure_transpose(fvm_amg *h, Level &L) {
     if (L.sym >= 0) return;
@@ -644,33 +784,51 @@ int coarsen_level(fvm_amg *h, Level &L, double theta) {
     const int sym = L.sym;
     const int *rowT = sym ? L.row.p : L.rowT.p;
     const int *colT = sym ? L.col.p : L.colT.p;
-    DArr<int> ldr(n), dround(n), mem((size_t)n * MEMW), memr((size_t)n * MEMW), mcnt(n),
-        stamp(n), lists(2 * (size_t)n + 2), ctr(8);
+    DArr<int> ldr(n), dround(n), mem((size_t)n * MEMW), memr((size_t)n * MEMW), mcnt(n), cnt(n),
+        lists(2 * (size_t)n + 2), ctr(8);
     FVM_HIP(hipMemsetAsync(ctr.p, 0, ctr.bytes(), s));
-    int *cnt = ctr.p;            // cnt[0..2] rotating candidate counters
+    int *qc = ctr.p;             // qc[0..2] rotating frontier counters
     int *remaining = ctr.p + 3;
+    int *rounds_out = ctr.p + 4;
     const uint8_t *bnd = L.bnd.p;
-    k_coarsen_init<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, bnd, ldr.p, dround.p, memr.p, mcnt.p,
-                                                       stamp.p, remaining);
-    FVM_HIP(hipGetLastError());
-    const unsigned g_all = grid_for(n, BS);
-    const unsigned g_round = grid_for(n, BS, 2048);
+    int *q0 = lists.p, *q1 = lists.p + n;
+    // initial counts; the frontier of round 1 goes to q1 (round t reads list t&1)
+    {
+        const long long threads = (long long)n * SUB;
+        const unsigned g = (unsigned)((threads + 255) / 256);
+        k_cnt_init<<<g, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, cnt.p, dround.p, ldr.p,
+                                     memr.p, mcnt.p, q1, qc + 1, remaining);
+        FVM_HIP(hipGetLastError());
+    }
     int t = 1;
-    const int batch = 16;
     const long long cap = 4LL * n + 64;
-    for (;;) {
-        for (int k = 0; k < batch; k++, t++) {
-            const bool first = (t == 1);
-            k_coarsen_round<<<first ? g_all : g_round, BS, 0, s>>>(
-                t, first ? 1 : 0, n, L.row.p, L.col.p, rowT, colT, sym, L.off.p, L.diag.p, bnd,
-                theta, ldr.p, dround.p, mem.p, memr.p, mcnt.p, stamp.p,
-                lists.p + (size_t)(t & 1) * n, cnt + (t % 3), lists.p + (size_t)((t + 1) & 1) * n,
-                cnt + ((t + 1) % 3), cnt + ((t + 2) % 3), remaining);
-        }
+    if (n <= h->coarsen_small_rows) {
+        k_coarsen_small<<<1, 1024, 0, s>>>(n, (int)std::min<long long>(cap, INT_MAX - 1), L.row.p, L.col.p,
+                                           rowT, colT, sym, L.off.p, L.diag.p, bnd, theta, ldr.p, dround.p,
+                                           mem.p, memr.p, mcnt.p, cnt.p, q1, q0, qc, remaining, rounds_out);
         FVM_HIP(hipGetLastError());
         const int rem = read_scalar(remaining, s);
-        if (rem == 0) break;
-        FVM_CHECK(t < cap, FVM_E_NUMERIC, "coarsening did not terminate");
+        t = read_scalar(rounds_out, s);
+        FVM_CHECK(rem == 0, FVM_E_NUMERIC,
+                  "coarsening did not terminate (single-workgroup path, n=" + std::to_string(n) + ", rounds=" +
+                      std::to_string(t) + ", undecided=" + std::to_string(rem) + ")");
+    } else {
+        const unsigned g_round = 1024;
+        const int batch = 32;
+        for (;;) {
+            for (int k = 0; k < batch; k++, t++) {
+                k_coarsen_round<<<g_round, 256, 0, s>>>(
+                    t, L.row.p, L.col.p, rowT, colT, sym, L.off.p, L.diag.p, bnd, theta, ldr.p, dround.p,
+                    mem.p, memr.p, mcnt.p, cnt.p, (t & 1) ? q1 : q0, qc + (t % 3), (t & 1) ? q0 : q1,
+                    qc + ((t + 1) % 3), qc + ((t + 2) % 3), remaining);
+            }
+            FVM_HIP(hipGetLastError());
+            const int rem = read_scalar(remaining, s);
+            if (rem == 0) break;
+            FVM_CHECK(t < cap, FVM_E_NUMERIC,
+                      "coarsening did not terminate (multi-block path, n=" + std::to_string(n) + ", rounds=" +
+                          std::to_string(t) + ", undecided=" + std::to_string(rem) + ")");
+        }
     }
     // leaders -> contiguous ids in seed order
     DArr<int> flag((size_t)n + 1), gid((size_t)n + 1);
@@ -739,6 +897,8 @@ void build_hierarchy(fvm_amg *h, int n_self, int n_total, const int *d_row, cons
     h->lev.clear();
     h->ready = false;
     h->direct_level = -1;
+    h->tail_prec = -1;
+    h->tail_desc.release();
     h->n_self = n_self;
     h->n_total = n_total;
     const long long nnz_in = n_total > 0 ? (long long)read_scalar(d_row + n_total, s) : 0;
@@ -781,17 +941,22 @@ void build_hierarchy(fvm_amg *h, int n_self, int n_total, const int *d_row, cons
         }
         FVM_HIP(hipGetLastError());
     }
+    PhaseTimer pt(h);
+    pt.mark("validate+compact", 0);
     int fine_size = n_total;  // getLocalSize() of the finest level counts ghosts
     for (int k = 0; k < h->prm.maxCoarseLevels && n_self > 0; k++) {
         Level &A = h->lev.back();
         const int rounds = coarsen_level(h, A, h->prm.weightRatioThreshold);
+        pt.mark("coarsen", k);
         if (k == 0) h->finest_rounds = rounds;
         const int nc = A.nc;
         if (nc == fine_size) { A.ci.release(); A.nc = 0; break; }       // AMG.cpp:163-168
         if (!h->prm.parallelStopRule && nc <= 3) { A.ci.release(); A.nc = 0; break; }
         coarse_to_fine(h, A);
+        pt.mark("coarse-to-fine", k);
         Level C;
         coarse_operator(h, A, C);
+        pt.mark("galerkin", k);
         h->lev.push_back(std::move(C));
         if (h->prm.verbosity > 1)
             printf("Created coarse level %d of size %d\n", k, nc);
@@ -803,13 +968,18 @@ void build_hierarchy(fvm_amg *h, int n_self, int n_total, const int *d_row, cons
     h->ws.scal.ensure(64);
     FVM_HIP(hipStreamSynchronize(s));
     h->ready = true;
+    pt.mark("vectors", -1);
     // mode-specific preparation
     if (h->prm.smootherType == FVM_GAUSS_SEIDEL && h->prm.gsOrdering == FVM_GS_EXACT)
         for (auto &L : h->lev) prepare_exact_schedule(h, L);
+    pt.mark("gs-schedule", -1);
     if (h->prm.gsOrdering == FVM_GS_MULTICOLOR) prepare_fast(h);
+    pt.mark("colour+sell", -1);
     if (h->prm.precision == FVM_PREC_MIXED) {
         prepare_mixed(h, h->prm.gsOrdering == FVM_GS_MULTICOLOR);
+        pt.mark("mixed", -1);
         prepare_direct(h);
+        pt.mark("direct", -1);
     }
     FVM_HIP(hipStreamSynchronize(s));
     h->setup_seconds = now_s() - t0;
@@ -908,13 +1078,19 @@ void prepare_fast(fvm_amg *h) {
         k_invert_perm<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.perm.p, L.iperm.p);
         DArr<int> hist(257);
         FVM_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), s));
-        k_color_hist<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, color.p, hist.p);
+        k_color_hist<<<grid_for(n, BS, 1024), BS, 0, s>>>(n, color.p, hist.p);
         std::vector<int> hh(257);
         FVM_HIP(hipMemcpyAsync(hh.data(), hist.p, hist.bytes(), hipMemcpyDeviceToHost, s));
         FVM_HIP(hipStreamSynchronize(s));
         L.color_ptr.assign(1, 0);
         for (int c = 0; c < 257 && L.color_ptr.back() < n; c++) L.color_ptr.push_back(L.color_ptr.back() + hh[c]);
         L.ncolor = (int)L.color_ptr.size() - 1;
+        L.d_color_ptr.alloc(L.color_ptr.size());
+        FVM_HIP(hipMemcpyAsync(L.d_color_ptr.p, L.color_ptr.data(), sizeof(int) * L.color_ptr.size(),
+                               hipMemcpyHostToDevice, s));
+        FVM_HIP(hipStreamSynchronize(s));
+        if (getenv("FVM_AMD_SETUP_TIMING"))
+            fprintf(stderr, "[setup] level %2d n=%d nnz=%lld colours=%d\n", l, n, L.nnz, L.ncolor);
         // SELL-64
         const int nslice = (n + 63) / 64;
         DArr<int> w64((size_t)nslice + 1);

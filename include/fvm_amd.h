@@ -159,6 +159,12 @@ int fvm_amg_level_get(const fvm_amg *h, int level, int32_t *row, int32_t *col, d
 enum { FVM_KT_RESIDUAL64 = 0, FVM_KT_SWEEP = 1 };
 int fvm_amg_set_kernel_timing(fvm_amg *h, int enable);
 int fvm_amg_kernel_timing(const fvm_amg *h, int kind, long long *launches, double *total_ms);
+/* Performance knobs that never change results (no reference counterpart; set
+ * by the autotuner): "tail_max_rows" -- V-cycle levels with at most this many
+ * rows run in one single-workgroup launch (0 disables); "coarsen_small_rows" --
+ * levels with at most this many rows are agglomerated by one workgroup. */
+int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value);
+int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value);
 /* Setup statistics: seconds spent in setup, number of coarsening rounds of the finest level. */
 int fvm_amg_setup_stats(const fvm_amg *h, double *setup_seconds, int *finest_rounds);
 

@@ -285,3 +285,31 @@ def test_device_entry_points_match_host():
     torch.cuda.synchronize()
     assert cyc == a.lastCycles
     assert np.array_equal(x.cpu().numpy(), ls.delta)
+
+
+@pytest.mark.parametrize("small", [0, 1 << 30])
+def test_coarsening_paths_agree(small):
+    # multi-block rounds (small=0) and single-workgroup rounds (all levels) give the oracle's maps
+    m = matgen.grid3d(14, seed=21, contrast=2.0)
+    oa = oracle_for(m)
+    ga = gpu_amg()
+    ga.set_tuning("coarsen_small_rows", small)
+    ls = ls_from(m)
+    ga.setup(ls)
+    check_hierarchy(ga, oa, m)
+
+
+@pytest.mark.parametrize("prec,smoother,npre", [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)])
+def test_tail_kernel_is_bit_identical(prec, smoother, npre):
+    # the single-workgroup tail must not change a single bit of the coloured cycle
+    m = matgen.grid3d(18, seed=22, contrast=1.5)
+    res = []
+    for tail in (0, 1 << 30):
+        ga = gpu_amg(gsOrdering=1, precision=prec, smootherType=smoother, nPreSweeps=npre, nMaxIterations=12,
+                     relativeTolerance=1e-30)
+        ga.set_tuning("tail_max_rows", tail)
+        ls = ls_from(m)
+        ga.solve(ls)
+        res.append((ls.delta.copy(), ga.history()))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
