@@ -100,6 +100,134 @@ def cpu_baseline(S, cycles_needed):
                 setup_s=t1 - t0, per_cycle_s=per_cycle, host_threads=os.cpu_count())
 
 
+def fib_direction(i, K):
+    import math
+    z = 1.0 - (2 * i + 1) / K
+    phi = i * math.pi * (3.0 - math.sqrt(5.0))
+    r = math.sqrt(max(0.0, 1.0 - z * z))
+    return [r * math.cos(phi), r * math.sin(phi), z]
+
+
+def build_batch_workload(torch, K=256, lo=1e4, hi=5e5):
+    """K discrete-ordinate systems (gray phonon BTE): m^3 jittered hex cubes with
+    n = round(exp(U[ln lo, ln hi])), m = round(n^(1/3)); direction i on a
+    Fibonacci sphere; tau_i ~ U[0.1, 10]; e0 = 1, field 0, inflow value 1 on the
+    x=0 wall (0 elsewhere). Meshes and their metrics are built once."""
+    import numpy as np
+    from fvm_amd.thermal import Mesh
+    rng = np.random.default_rng([SEED, 4])
+    n = np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), K)))
+    m = np.maximum(2, np.round(np.cbrt(n))).astype(int)
+    tau = rng.uniform(0.1, 10.0, K)
+    meshes, fields = [], []
+    for i in range(K):
+        mesh = Mesh.hex(int(m[i]), 0.2, SEED + 1000 + i)
+        mesh.metrics_device()
+        nc = mesh.n_cells
+        x = torch.zeros(mesh.n_total, dtype=torch.float64, device="cuda:0")
+        x[nc: nc + int(m[i]) ** 2] = 1.0          # ghosts of the x=0 wall (first boundary group)
+        e0 = torch.ones(nc, dtype=torch.float64, device="cuda:0")
+        meshes.append(mesh)
+        fields.append((fib_direction(i, K), float(tau[i]), e0, x))
+    return meshes, fields, m
+
+
+def batch_params(obj):
+    obj.relativeTolerance = 1e-8
+    obj.nMaxIterations = 200
+    obj.verbosity = 0
+
+
+def run_batch(args):
+    """Per sweep of all K directions: assemble every system, build every
+    hierarchy, solve every system to 1e-8 (reference-order FP64 AMG per system,
+    PhononModel::advance), all systems in one batch handle."""
+    import numpy as np
+    import torch
+    import fvm_amd
+    from fvm_amd.thermal import assemble_phonon
+    K = args.batch
+    log("building %d direction systems" % K)
+    meshes, fields, m = build_batch_workload(torch, K)
+    ncells = int(sum(me.n_cells for me in meshes))
+    B = fvm_amd.BatchAMG()
+    batch_params(B)
+
+    def sweep():
+        systems = [assemble_phonon(me, *f) for me, f in zip(meshes, fields)]
+        devs = [dict(S.device_pointers(), n_self=S.n_self, n_total=S.n_total) for S in systems]
+        B.setup_device(devs)
+        xs = [torch.zeros(S.n_total, dtype=torch.float64, device="cuda:0") for S in systems]
+        r0, rf, cyc = B.solve_device([d["b"] for d in devs], xs)
+        return systems, xs, r0, rf, cyc
+
+    for w in range(args.warmup):
+        _, _, r0, rf, cyc = sweep()
+        log("warmup %d: cycles min %d max %d" % (w, cyc.min(), cyc.max()))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        systems, xs, r0, rf, cyc = sweep()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = 1e3 * (t1 - t0) / args.steps
+    st = B.setup_stats()
+    # phases of one more sweep
+    torch.cuda.synchronize(); ta = time.perf_counter()
+    systems = [assemble_phonon(me, *f) for me, f in zip(meshes, fields)]
+    torch.cuda.synchronize(); tb = time.perf_counter()
+    devs = [dict(S.device_pointers(), n_self=S.n_self, n_total=S.n_total) for S in systems]
+    B.setup_device(devs)
+    torch.cuda.synchronize(); tc = time.perf_counter()
+    xs = [torch.zeros(S.n_total, dtype=torch.float64, device="cuda:0") for S in systems]
+    B.solve_device([d["b"] for d in devs], xs)
+    torch.cuda.synchronize(); td = time.perf_counter()
+    # the same systems one handle at a time (unbatched loop, as the reference calls AMG::solve)
+    t_loop = None
+    if not args.skip_plain:
+        a = fvm_amd.AMG()
+        batch_params(a)
+        torch.cuda.synchronize(); tl0 = time.perf_counter()
+        for S, d in zip(systems, devs):
+            x = torch.zeros(S.n_total, dtype=torch.float64, device="cuda:0")
+            a.setup_device(S.n_self, S.n_total, d["row"], d["col"], d["diag"], d["off"], d["is_boundary"])
+            a.solve_device(d["b"], x)
+        torch.cuda.synchronize(); t_loop = time.perf_counter() - tl0
+        log("unbatched loop over %d systems: %.3f s" % (K, t_loop))
+    cpu = None
+    if not args.skip_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        order = np.argsort([me.n_cells for me in meshes])
+        sample = [int(order[j]) for j in np.linspace(0, K - 1, 6).astype(int)]
+        t_cpu, cells = 0.0, 0
+        for i in sample:
+            h = systems[i].to_host()
+            a = O.OracleAMG(relative_tolerance=1e-8, n_max_iterations=200)
+            c0 = time.perf_counter()
+            a.setup(h["n_self"], h["n_total"], h["row"], h["col"], h["diag"], h["off"], h["is_boundary"])
+            a.solve(h["b"])
+            t_cpu += time.perf_counter() - c0
+            cells += h["n_self"]
+        cpu = dict(value=round(t_cpu * ncells / cells, 3), unit="s", cores=1, kind="port",
+                   sample="oracle setup+solve of %d of the %d systems (%d cells, %.2f s), scaled by cell count "
+                          "to all %d cells" % (len(sample), K, cells, t_cpu, ncells))
+    out = {
+        "metric": "time per sweep of %d direction systems (assemble + AMG setup + solve to 1e-8)" % K,
+        "value": round(ms / 1e3, 6), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded meshes, directions and relaxation times)",
+        "config": {"workload": "%d phonon-BTE directions, m^3 jittered hex cubes, m in [%d, %d], %d cells total; "
+                               "upwind convection + relaxation, reference-order FP64 AMG per system, batched"
+                               % (K, int(m.min()), int(m.max()), ncells)},
+        "cycles_min": int(cyc.min()), "cycles_max": int(cyc.max()), "cycles_mean": float(cyc.mean()),
+        "assemble_s": round(tb - ta, 4), "setup_s": round(tc - tb, 4), "solve_s": round(td - tc, 4),
+        "levels": st["levels"], "unbatched_loop_s": None if t_loop is None else round(t_loop, 4),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,11 +237,16 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-exact", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="instead of the 8M-cell solve: time a sweep of this many phonon direction systems")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the row-sharded design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.batch:
+        run_batch(args)
+        return
     import torch
     import fvm_amd
     from fvm_amd._lib import lib

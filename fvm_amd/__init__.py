@@ -4,6 +4,6 @@ Host-side mirror of the c-PRIMED/fvm ``LinearSolver``/``AMG`` interface over the
 C ABI of ``libfvm_amd.so`` (hand-written HIP kernels for gfx950).
 """
 from .linear_system import LinearSystem, read_matrix_market
-from .amg import AMG, FGMRES
+from .amg import AMG, FGMRES, BatchAMG
 
 __all__ = ["AMG", "FGMRES", "LinearSystem", "read_matrix_market"]

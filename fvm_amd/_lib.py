@@ -80,6 +80,15 @@ SIGNATURES = {
     "fvm_mesh_lognormal_field": (_i, [_p, _d, _i, ctypes.c_uint64, _p]),
     "fvm_mesh_uniform_field": (_i, [_p, _d, _d, ctypes.c_uint64, _p]),
     "fvm_thermal_assemble": (_i, [_p, _p, _p, _p, _p, _p, ctypes.POINTER(_p)]),
+    "fvm_phonon_assemble": (_i, [_p, _p, _d, _p, _p, ctypes.POINTER(_p)]),
+    "fvm_batch_create": (_i, [ctypes.POINTER(_p), _i]),
+    "fvm_batch_destroy": (_i, [_p]),
+    "fvm_batch_set_params": (_i, [_p, ctypes.POINTER(AmgParams)]),
+    "fvm_batch_setup_device": (_i, [_p, _i, _p, _p, _p, _p, _p, _p, _p]),
+    "fvm_batch_solve_device": (_i, [_p, _p, _p, _p, _p, _p]),
+    "fvm_batch_level_sizes": (_i, [_p, _i, _p, _i]),
+    "fvm_batch_set_tuning": (_i, [_p, ctypes.c_char_p, _ll]),
+    "fvm_batch_setup_stats": (_i, [_p, _p, _p, _p]),
     "fvm_system_info": (_i, [_p, _p, _p, _p]),
     "fvm_system_device": (_i, [_p, _p, _p, _p, _p, _p, _p]),
     "fvm_system_get": (_i, [_p, _p, _p, _p, _p, _p, _p]),

@@ -267,3 +267,86 @@ class FGMRES:
         buf = np.zeros(cap)
         k = check(lib().fvm_fgmres_history(self._h, _ptr(buf), cap))
         return buf[:k].copy()
+
+
+class BatchAMG:
+    """Many independent systems through one handle (the per-direction solves of
+    PhononModel::advance, PhononModel.h:530-600): per system the same result as
+    an ``AMG`` solve with the same parameters (exact Gauss-Seidel order, FP64).
+    Parameter attributes are those of ``AMG``."""
+
+    _FIELDS = AMG._FIELDS
+
+    def __init__(self, device=0):
+        L = lib()
+        h = ctypes.c_void_p()
+        check(L.fvm_batch_create(ctypes.byref(h), int(device)))
+        self._h = h
+        p = AmgParams()
+        check(L.fvm_amg_default_params(ctypes.byref(p)))
+        p.verbosity = 0
+        object.__setattr__(self, "_p", p)
+        self.K = 0
+
+    def __getattr__(self, name):
+        if name in BatchAMG._FIELDS:
+            return getattr(self.__dict__["_p"], name)
+        raise AttributeError(name)
+
+    def __setattr__(self, name, value):
+        if name in BatchAMG._FIELDS:
+            setattr(self._p, name, value)
+        else:
+            object.__setattr__(self, name, value)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().fvm_batch_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def _push_params(self):
+        check(lib().fvm_batch_set_params(self._h, ctypes.byref(self._p)))
+
+    def setup_device(self, systems):
+        """systems: list of dicts with n_self, n_total and device tensors row, col,
+        diag, off and optionally is_boundary (e.g. ``System.device_pointers()``)."""
+        K = len(systems)
+        ns = np.array([s["n_self"] for s in systems], np.int32)
+        nt = np.array([s["n_total"] for s in systems], np.int32)
+
+        def parr(key):
+            a = (ctypes.c_void_p * K)()
+            for k, s in enumerate(systems):
+                v = s.get(key)
+                a[k] = None if v is None else (v if isinstance(v, int) else v.data_ptr())
+            return a
+        self._keep = [parr(k) for k in ("row", "col", "diag", "off", "is_boundary")]
+        self._push_params()
+        check(lib().fvm_batch_setup_device(self._h, K, _ptr(ns), _ptr(nt), *self._keep))
+        self.K = K
+
+    def solve_device(self, bs, xs):
+        """bs, xs: lists of device tensors (or addresses); returns (r0, r_final, cycles) arrays."""
+        K = self.K
+        pb = (ctypes.c_void_p * K)(*[_dptr(b).value for b in bs])
+        px = (ctypes.c_void_p * K)(*[_dptr(x).value for x in xs])
+        r0 = np.zeros(K); rf = np.zeros(K); cyc = np.zeros(K, np.int32)
+        self._push_params()
+        check(lib().fvm_batch_solve_device(self._h, pb, px, _ptr(r0), _ptr(rf), _ptr(cyc)))
+        return r0, rf, cyc
+
+    def level_sizes(self, k, cap=64):
+        buf = np.zeros(cap, np.int32)
+        d = check(lib().fvm_batch_level_sizes(self._h, int(k), _ptr(buf), cap))
+        return buf[:d].copy()
+
+    def set_tuning(self, key, value):
+        check(lib().fvm_batch_set_tuning(self._h, key.encode(), int(value)))
+
+    def setup_stats(self):
+        t = ctypes.c_double(); n = ctypes.c_int(); lv = ctypes.c_int()
+        check(lib().fvm_batch_setup_stats(self._h, ctypes.byref(t), ctypes.byref(n), ctypes.byref(lv)))
+        return dict(setup_seconds=t.value, total_rows=n.value, levels=lv.value)

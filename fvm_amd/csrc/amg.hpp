@@ -58,6 +58,46 @@ struct Level {
     DArr<float> b32, x32, r32, t32;
 };
 
+// Several independent systems solved as one block-diagonal hierarchy: the
+// agglomeration, Galerkin product and Gauss-Seidel schedules are local to a
+// system's rows, so level l of the combined hierarchy is the concatenation of
+// every system's own level l (systems in input order). Systems stop coarsening
+// independently; a system whose hierarchy has ended keeps its coarsest rows but
+// is excluded from further coarsening.
+struct BatchSys {
+    const int *row, *col;
+    const double *diag, *off;
+    const uint8_t *bnd;
+    int n_self, n_total;
+};
+
+struct BatchInfo {
+    int K = 0;
+    std::vector<BatchSys> sys;
+    std::vector<std::vector<int>> off;   // per level: K+1 row offsets of the systems
+    std::vector<int> depth;              // per system: levels of its hierarchy
+    DArr<BatchSys> d_sys;
+    DArr<int> d_off0;                    // finest-level offsets on the device
+    // solve state
+    DArr<uint8_t> act;                   // per system: still cycling
+    DArr<uint8_t> mask0;                 // per finest row: act of its system
+    DArr<int> blk_off;                   // norm partial blocks per system (K+1)
+    int nblk = 0;
+    DArr<double> norm, r0, rn;           // per system
+    DArr<int> cyc, counters;             // cycles per system; [0] iteration, [1] active count
+    DArr<const double *> d_bptr;
+    DArr<double *> d_xptr;
+    DArr<double> partials;
+    hipGraphExec_t graph = nullptr;
+    bool graph_ok = false;
+    bool use_graph = true;
+    int *h_active = nullptr;             // pinned: active count after an iteration
+    ~BatchInfo() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (h_active) (void)hipHostFree(h_active);
+    }
+};
+
 struct Workspace {
     Scanner scan;
     DArr<double> partials;   // norm / dot partial sums
@@ -115,10 +155,13 @@ void prepare_exact_schedule(fvm_amg *h, Level &L);
 void prepare_fast(fvm_amg *h);
 void prepare_mixed(fvm_amg *h, bool fast);
 void prepare_direct(fvm_amg *h);
+void build_hierarchy_batched(fvm_amg *h, BatchInfo &B);
 // cycle.hip
 void solve_device(fvm_amg *h, const double *d_b, double *d_x, double *rnorm0, double *rnorm_final,
                   int *cycles);
 void smooth_device(fvm_amg *h, const double *d_b, double *d_x);
+void batch_solve(fvm_amg *h, BatchInfo &B, const double *const *d_b, double *const *d_x, double *rnorm0,
+                 double *rnorm_final, int *cycles);
 void apply_precond(fvm_amg *h, const double *d_v, double *d_z);   // z = -M^{-1}... see cycle.hip
 void spmv_residual64(fvm_amg *h, const double *d_x, const double *d_b, double *d_r, double *norm1);
 double norm1_device(fvm_amg *h, const double *d_r, int n);

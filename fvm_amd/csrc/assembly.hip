@@ -409,6 +409,104 @@ __global__ void k_eliminate(int n_cells, const int *cf_row, const int *cf_col, c
     }
 }
 
+
+// ------------------------------------------------------------------ phonon direction
+// Upwind convection of a constant direction vg plus relaxation towards e0
+// (PhononConvectionDiscretization.h:150-175, PhononCollisionDiscretization.h:54-68,
+// collision linearized first as in PhononModel.h:449-459).
+__global__ void k_face_conv(int nf, const double *fA, double v0, double v1, double v2, double *F) {
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x)
+        F[f] = fA[3 * f] * v0 + fA[3 * f + 1] * v1 + fA[3 * f + 2] * v2;
+}
+
+// per cell: relaxation, then the upwind terms of its faces in face order (the
+// order in which the reference's face loop reaches this cell's entries)
+__global__ void k_cell_phonon(int n_cells, int nct, const int *cf_row, const int *cf_col, const int *fcells,
+                              const int *pos01, const int *pos10, const int *row, const double *F,
+                              const double *vol, const double *e0, const double *x, double tau, double *diag,
+                              double *off, double *b) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nct; c += gridDim.x * blockDim.x) {
+        for (int p = row[c]; p < row[c + 1]; p++) off[p] = 0.0;
+        double d = 0.0, r = 0.0;
+        if (c < n_cells) {
+            r -= vol[c] * (x[c] - e0[c]) / tau;
+            d -= vol[c] / tau;
+        }
+        for (int q = cf_row[c]; q < cf_row[c + 1]; q++) {
+            const int f = cf_col[q];
+            const int c0 = fcells[2 * f], c1 = fcells[2 * f + 1];
+            const double Ff = F[f];
+            double flux;
+            if (Ff > 0.0) {
+                flux = Ff * x[c0];
+                if (c == c0) d -= Ff;
+                else off[pos10[f]] += Ff;
+            } else {
+                flux = Ff * x[c1];
+                if (c == c1) d += Ff;
+                else off[pos01[f]] -= Ff;
+            }
+            if (c == c0) r -= flux;
+            else r += flux;
+        }
+        diag[c] = d;
+        b[c] = r;
+    }
+}
+
+__device__ __forceinline__ bool phonon_inflow(const double *fA, const double *fAm, int f, double v0, double v1,
+                                              double v2) {
+    const double e0 = fA[3 * f] / fAm[f], e1 = fA[3 * f + 1] / fAm[f], e2 = fA[3 * f + 2] / fAm[f];
+    return v0 * e0 + v1 * e1 + v2 * e2 < 0.0;
+}
+
+// boundary faces, interior side (GenericPhononBCS.h:67-88 Dirichlet with the
+// ghost's own value, :106-124 extrapolation), faces in order per cell; reads the
+// ghost diagonal before k_bc_phonon_ghost resets it
+__global__ void k_bc_phonon_interior(int n_cells, const int *cf_row, const int *cf_col, const int *fcells,
+                                     const int *pos01, int first_bface, const double *fA, const double *fAm,
+                                     double v0, double v1, double v2, const double *x, double *diag,
+                                     double *off, double *b) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n_cells; c += gridDim.x * blockDim.x) {
+        for (int q = cf_row[c]; q < cf_row[c + 1]; q++) {
+            const int f = cf_col[q];
+            if (f < first_bface) continue;
+            const int c1 = fcells[2 * f + 1];
+            if (phonon_inflow(fA, fAm, f, v0, v1, v2)) {
+                const double bval = x[c1];
+                const double dX = bval - x[c1];
+                b[c] += off[pos01[f]] * dX;
+                off[pos01[f]] = 0.0;
+            } else {
+                const double dFdX1 = -diag[c1];
+                const double dx01 = x[c] - x[c1];
+                diag[c] += dFdX1;
+                b[c] += dFdX1 * dx01;
+                off[pos01[f]] = 0.0;
+            }
+        }
+    }
+}
+
+__global__ void k_bc_phonon_ghost(int n_cells, int nct, const int *cf_row, const int *cf_col, const int *fcells,
+                                  const int *pos10, const double *fA, const double *fAm, double v0, double v1,
+                                  double v2, const double *x, double *diag, double *off, double *b,
+                                  uint8_t *isb) {
+    for (int gc = n_cells + blockIdx.x * blockDim.x + threadIdx.x; gc < nct; gc += gridDim.x * blockDim.x) {
+        const int f = cf_col[cf_row[gc]];
+        diag[gc] = -1.0;
+        if (phonon_inflow(fA, fAm, f, v0, v1, v2)) {
+            off[pos10[f]] = 0.0;
+            b[gc] = 0.0;
+            isb[gc] = 0;
+        } else {
+            off[pos10[f]] = 1.0;
+            b[gc] = x[fcells[2 * f]] - x[gc];
+            isb[gc] = 1;
+        }
+    }
+}
+
 // ------------------------------------------------------------------ fields
 __global__ void k_lognormal(int N, int n_cells, int block, double sigma, unsigned long long seed,
                             double *k) {
@@ -753,6 +851,55 @@ int fvm_thermal_assemble(fvm_mesh *m, const double *d_k, const double *d_source,
             fvm::k_bc_ghost<<<fvm::grid_for(m->n_ghost, 256), 256, 0, s>>>(nc, nct, m->cf_row.p, m->cf_col.p,
                                                                            m->pos10.p, m->fAm.p, G, S->diag.p,
                                                                            S->off.p, S->b.p, S->isb.p);
+        fvm::k_eliminate<<<fvm::grid_for(nc, 256), 256, 0, s>>>(nc, m->cf_row.p, m->cf_col.p, m->fcells.p,
+                                                                m->pos01.p, m->pos10.p, S->isb.p, S->diag.p,
+                                                                S->off.p, S->b.p);
+        FVM_HIP(hipGetLastError());
+        FVM_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        delete S;
+        throw;
+    }
+    *out = S;
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_phonon_assemble(fvm_mesh *m, const double *vg, double tau, const double *d_e0, const double *d_x,
+                        fvm_system **out) {
+    FVM_API_BEGIN
+    FVM_CHECK(m && vg && d_e0 && d_x && out, FVM_E_INVALID, "null argument");
+    FVM_CHECK(tau != 0.0 && std::isfinite(tau), FVM_E_INVALID, "tau must be finite and non-zero");
+    fvm::Dev d(m->device);
+    fvm::mesh_metrics(m);
+    hipStream_t s = m->stream;
+    const int nc = m->n_cells, nct = m->n_cells + m->n_ghost, nf = m->n_faces;
+    fvm_system *S = new fvm_system();
+    try {
+        S->device = m->device;
+        S->n_self = nc;
+        S->n_total = nct;
+        S->nnz = m->nnz;
+        S->row.alloc((size_t)nct + 1);
+        S->col.alloc(std::max<long long>(m->nnz, 1));
+        S->off.alloc(std::max<long long>(m->nnz, 1));
+        S->diag.alloc(nct); S->b.alloc(nct); S->isb.alloc(nct);
+        FVM_HIP(hipMemcpyAsync(S->row.p, m->cc_row.p, S->row.bytes(), hipMemcpyDeviceToDevice, s));
+        FVM_HIP(hipMemcpyAsync(S->col.p, m->cc_col.p, sizeof(int) * m->nnz, hipMemcpyDeviceToDevice, s));
+        DArr<double> F(std::max(nf, 1));
+        fvm::k_face_conv<<<fvm::grid_for(nf, 256), 256, 0, s>>>(nf, m->fA.p, vg[0], vg[1], vg[2], F.p);
+        fvm::k_cell_phonon<<<fvm::grid_for(nct, 256), 256, 0, s>>>(
+            nc, nct, m->cf_row.p, m->cf_col.p, m->fcells.p, m->pos01.p, m->pos10.p, m->cc_row.p, F.p, m->vol.p,
+            d_e0, d_x, tau, S->diag.p, S->off.p, S->b.p);
+        const int first_bface = m->gptr.size() > 1 ? m->gptr[1] : nf;
+        fvm::k_bc_phonon_interior<<<fvm::grid_for(nc, 256), 256, 0, s>>>(
+            nc, m->cf_row.p, m->cf_col.p, m->fcells.p, m->pos01.p, first_bface, m->fA.p, m->fAm.p, vg[0], vg[1],
+            vg[2], d_x, S->diag.p, S->off.p, S->b.p);
+        FVM_HIP(hipMemsetAsync(S->isb.p, 0, nct, s));
+        if (m->n_ghost)
+            fvm::k_bc_phonon_ghost<<<fvm::grid_for(m->n_ghost, 256), 256, 0, s>>>(
+                nc, nct, m->cf_row.p, m->cf_col.p, m->fcells.p, m->pos10.p, m->fA.p, m->fAm.p, vg[0], vg[1],
+                vg[2], d_x, S->diag.p, S->off.p, S->b.p, S->isb.p);
         fvm::k_eliminate<<<fvm::grid_for(nc, 256), 256, 0, s>>>(nc, m->cf_row.p, m->cf_col.p, m->fcells.p,
                                                                 m->pos01.p, m->pos10.p, S->isb.p, S->diag.p,
                                                                 S->off.p, S->b.p);

@@ -79,6 +79,11 @@ struct fvm_fgmres {
     std::vector<double> hist;
 };
 
+struct fvm_batch {
+    fvm_amg *h = nullptr;     // owns the combined hierarchy, stream and pool
+    fvm::BatchInfo B;
+};
+
 namespace fvm {
 struct KrylovWork;
 KrylovWork *new_krylov_work();
@@ -393,6 +398,131 @@ int fvm_amg_setup_stats(const fvm_amg *h, double *setup_seconds, int *finest_rou
     if (!h) return FVM_E_INVALID;
     if (setup_seconds) *setup_seconds = h->setup_seconds;
     if (finest_rounds) *finest_rounds = h->finest_rounds;
+    return FVM_OK;
+}
+
+// ----------------------------------------------------------------------- batched systems
+static void batch_check_params(const fvm_amg_params *p) {
+    check_params(p);
+    FVM_CHECK(p->gsOrdering == FVM_GS_EXACT, FVM_E_INVALID, "batched solve: gsOrdering must be FVM_GS_EXACT");
+    FVM_CHECK(p->precision == FVM_PREC_FP64, FVM_E_INVALID, "batched solve: precision must be FVM_PREC_FP64");
+    FVM_CHECK(!(p->isSymmetric && p->scaleCorrections), FVM_E_INVALID,
+              "batched solve: scaled corrections (isSymmetric && scaleCorrections) are not supported");
+}
+
+static void batch_invalidate(fvm_batch *b) {
+    if (b->B.graph) { (void)hipGraphExecDestroy(b->B.graph); b->B.graph = nullptr; }
+    b->B.graph_ok = false;
+    b->B.blk_off.release();
+}
+
+int fvm_batch_create(fvm_batch **out, int device) {
+    FVM_API_BEGIN
+    FVM_CHECK(out, FVM_E_INVALID, "null output handle");
+    fvm_amg *h = nullptr;
+    const int rc = fvm_amg_create(&h, device);
+    if (rc) return rc;
+    fvm_batch *b = new fvm_batch();
+    b->h = h;
+    b->h->prm.verbosity = 0;
+    *out = b;
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_batch_destroy(fvm_batch *b) {
+    FVM_API_BEGIN
+    if (!b) return FVM_OK;
+    fvm_amg *h = b->h;
+    {
+        HandleScope g(h);
+        batch_invalidate(b);
+        delete b;   // batch arrays go back to the pool of h, destroyed next
+    }
+    return fvm_amg_destroy(h);
+    FVM_API_END
+}
+
+int fvm_batch_set_params(fvm_batch *b, const fvm_amg_params *p) {
+    FVM_API_BEGIN
+    FVM_CHECK(b, FVM_E_INVALID, "null handle");
+    batch_check_params(p);
+    const fvm_amg_params old = b->h->prm;
+    const bool rebuild = b->h->ready && (p->coarseGroupSize != old.coarseGroupSize ||
+                                         p->weightRatioThreshold != old.weightRatioThreshold ||
+                                         p->maxCoarseLevels != old.maxCoarseLevels ||
+                                         p->parallelStopRule != old.parallelStopRule ||
+                                         p->smootherType != old.smootherType);
+    HandleScope g(b->h);
+    b->h->prm = *p;
+    batch_invalidate(b);   // the captured iteration depends on every cycle parameter
+    if (rebuild) {
+        b->h->lev.clear();
+        b->h->ready = false;
+    }
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_batch_setup_device(fvm_batch *b, int nsys, const int32_t *n_self, const int32_t *n_total,
+                           const int32_t *const *d_row, const int32_t *const *d_col, const double *const *d_diag,
+                           const double *const *d_off, const uint8_t *const *d_is_boundary) {
+    FVM_API_BEGIN
+    FVM_CHECK(b && nsys > 0 && n_self && n_total && d_row && d_col && d_diag && d_off, FVM_E_INVALID,
+              "null argument or empty batch");
+    batch_check_params(&b->h->prm);
+    HandleScope g(b->h);
+    batch_invalidate(b);
+    b->B.K = nsys;
+    b->B.sys.resize(nsys);
+    for (int k = 0; k < nsys; k++) {
+        fvm::BatchSys &S = b->B.sys[k];
+        S.row = d_row[k]; S.col = d_col[k]; S.diag = d_diag[k]; S.off = d_off[k];
+        S.bnd = d_is_boundary ? d_is_boundary[k] : nullptr;
+        S.n_self = n_self[k]; S.n_total = n_total[k];
+    }
+    fvm::build_hierarchy_batched(b->h, b->B);
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_batch_solve_device(fvm_batch *b, const double *const *d_b, double *const *d_x, double *rnorm0,
+                           double *rnorm_final, int *cycles) {
+    FVM_API_BEGIN
+    FVM_CHECK(b && d_b && d_x, FVM_E_INVALID, "null argument");
+    FVM_CHECK(b->h->ready, FVM_E_NOT_SETUP, "batched solve called before setup");
+    batch_check_params(&b->h->prm);
+    HandleScope g(b->h);
+    fvm::batch_solve(b->h, b->B, d_b, d_x, rnorm0, rnorm_final, cycles);
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_batch_level_sizes(const fvm_batch *b, int sys, int32_t *sizes, int cap) {
+    if (!b || sys < 0 || sys >= b->B.K || !b->h->ready) return FVM_E_INVALID;
+    const int d = b->B.depth[sys];
+    for (int l = 0; l < d && l < cap; l++) sizes[l] = b->B.off[l][sys + 1] - b->B.off[l][sys];
+    return d;
+}
+
+int fvm_batch_set_tuning(fvm_batch *b, const char *key, long long value) {
+    FVM_API_BEGIN
+    FVM_CHECK(b && key, FVM_E_INVALID, "null argument");
+    if (std::string(key) == "graph") {
+        HandleScope g(b->h);
+        batch_invalidate(b);
+        b->B.use_graph = value != 0;
+        return FVM_OK;
+    }
+    return fvm_amg_set_tuning(b->h, key, value);
+    FVM_API_END
+}
+
+int fvm_batch_setup_stats(const fvm_batch *b, double *setup_seconds, int *total_rows, int *levels) {
+    if (!b) return FVM_E_INVALID;
+    if (setup_seconds) *setup_seconds = b->h->setup_seconds;
+    if (total_rows) *total_rows = b->h->ready ? b->h->lev[0].n : 0;
+    if (levels) *levels = (int)b->h->lev.size();
     return FVM_OK;
 }
 

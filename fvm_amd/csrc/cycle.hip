@@ -90,10 +90,11 @@ template <class V, class D, class X>
 __global__ void k_csr_gs_level(const int *__restrict__ ptr, int lvl, const int *__restrict__ rows,
                                const int *__restrict__ row, const int *__restrict__ col,
                                const V *__restrict__ off, const D *__restrict__ diag, X *x,
-                               const X *__restrict__ b) {
+                               const X *__restrict__ b, const uint8_t *__restrict__ mask) {
     const int beg = ptr[lvl], end = ptr[lvl + 1];
     for (int k = beg + blockIdx.x * blockDim.x + threadIdx.x; k < end; k += gridDim.x * blockDim.x) {
         const int i = rows[k];
+        if (mask && !mask[i]) continue;   // row of a system that has stopped cycling
         X s = b[i];
         for (int p = row[i]; p < row[i + 1]; p++) s += (X)ldv(off, p) * x[col[p]];
         x[i] = -s / (X)diag[i];
@@ -103,8 +104,10 @@ __global__ void k_csr_gs_level(const int *__restrict__ ptr, int lvl, const int *
 template <class V, class D, class X>
 __global__ void k_csr_jacobi(int n, const int *__restrict__ row, const int *__restrict__ col,
                              const V *__restrict__ off, const D *__restrict__ diag,
-                             const X *__restrict__ xo, const X *__restrict__ b, X *__restrict__ xn) {
+                             const X *__restrict__ xo, const X *__restrict__ b, X *__restrict__ xn,
+                             const uint8_t *__restrict__ mask) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (mask && !mask[i]) { xn[i] = xo[i]; continue; }
         X s = b[i];
         for (int p = row[i]; p < row[i + 1]; p++) s += (X)ldv(off, p) * xo[col[p]];
         xn[i] = -s / (X)diag[i];
@@ -180,8 +183,9 @@ __global__ void k_restrict(int nc, const int *__restrict__ c2f_row, const int *_
 // x_i += X_c[ci_i] (correct, Array.h:438-467); set != 0: x was zero, write X_c[ci_i]
 template <class X>
 __global__ void k_prolong(int n, const int *__restrict__ ci, const X *__restrict__ xc, X *x, int set,
-                          const double *scale) {
+                          const double *scale, const uint8_t *__restrict__ mask) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (mask && !mask[i]) continue;
         const int I = ci[i];
         if (scale) {
             const X sc = (X)*scale;
@@ -470,6 +474,7 @@ struct Run {
     bool mixed;
     int last;       // last level of the recursion
     bool r0_valid;  // level-0 residual holds b + A x for the current x
+    const uint8_t *mask0 = nullptr;   // batched solve: finest rows still cycling
 };
 
 inline int value_type(const Run &R, int l) {
@@ -548,7 +553,8 @@ void gs_sweep(Run &R, int l, X *x, const X *b) {
             const unsigned g = grid_for(maxw, BS);
             for (int k = 0; k < nlev; k++) {
                 FVM_DISPATCH_VALS(R, l, L, X, (void)0,
-                    (k_csr_gs_level<<<g, BS, 0, s>>>(ptr, k, rows, L.row.p, L.col.p, val, dg, x, b)));
+                    (k_csr_gs_level<<<g, BS, 0, s>>>(ptr, k, rows, L.row.p, L.col.p, val, dg, x, b,
+                                                      l == 0 ? R.mask0 : nullptr)));
             }
         }
     }
@@ -565,7 +571,8 @@ void jacobi_pass(Run &R, int l) {
     const X *b = vb<X>(L);
     FVM_DISPATCH_VALS(R, l, L, X,
         (k_sell_jacobi<<<grid_for(n, BS), BS, 0, s>>>(n, L.sell_off.p, L.sell_col.p, val, dg, xa.p, b, ta.p)),
-        (k_csr_jacobi<<<grid_for(n, BS), BS, 0, s>>>(n, L.row.p, L.col.p, val, dg, xa.p, b, ta.p)));
+        (k_csr_jacobi<<<grid_for(n, BS), BS, 0, s>>>(n, L.row.p, L.col.p, val, dg, xa.p, b, ta.p,
+                                                      l == 0 ? R.mask0 : nullptr)));
     std::swap(xa, ta);   // xnew becomes x (MultiFieldMatrix.cpp:203-215 copies it back)
 }
 
@@ -694,7 +701,9 @@ void cycle(Run &R, int type, int l, bool x_zero) {
             scale = scratch_scalar(h, 10);
         }
         const int *ci = R.fast ? L.pci.p : L.ci.p;
-        if (L.n) k_prolong<X><<<grid_for(L.n, BS), BS, 0, s>>>(L.n, ci, vx<X>(C), vx<X>(L), x_zero ? 1 : 0, scale);
+        if (L.n)
+            k_prolong<X><<<grid_for(L.n, BS), BS, 0, s>>>(L.n, ci, vx<X>(C), vx<X>(L), x_zero ? 1 : 0, scale,
+                                                          l == 0 ? R.mask0 : nullptr);
         x_zero = false;
     }
     if (h->prm.nPostSweeps > 0) {
@@ -822,6 +831,229 @@ void smooth_device(fvm_amg *h, const double *d_b, double *d_x) {
     one_cycle(R);
     store_finest(R, d_x);
     FVM_HIP(hipStreamSynchronize(h->stream));
+}
+
+
+// ===========================================================================
+// Batched solve of independent systems on one block-diagonal hierarchy.
+// Each system follows AMG::solve (AMG.cpp:219-282) on its own: its own initial
+// norm, stop test and cycle count. A system that has met its test stops
+// changing (its finest rows are masked out of the smoother and the
+// correction), so its result is the one its own unbatched solve returns.
+// Per-system 1-norms use the same blocks and summation tree as the unbatched
+// norm of a system of that size.
+// ===========================================================================
+namespace {
+
+__device__ __forceinline__ int batch_sys(const int *off, int K, int i) {
+    int lo = 0, hi = K;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_batch_gather(int total, int K, const int *off, const double *const *bp, double *const *xp,
+                               double *b, double *x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int k = batch_sys(off, K, i);
+        const int li = i - off[k];
+        b[i] = bp[k][li];
+        x[i] = xp[k][li];
+    }
+}
+
+__global__ void k_batch_scatter(int total, int K, const int *off, const double *x, double *const *xp) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int k = batch_sys(off, K, i);
+        xp[k][i - off[k]] = x[i];
+    }
+}
+
+// block j of system k covers local rows j*BS + t, stride g_k*BS: the unbatched
+// k_norm1_partial on that system's rows with g_k = its block count
+__global__ void __launch_bounds__(BS) k_seg_norm1_partial(int K, const int *off, const int *blk_off,
+                                                          const double *__restrict__ r, double *part) {
+    const int k = batch_sys(blk_off, K, blockIdx.x);
+    const int j = blockIdx.x - blk_off[k];
+    const int gk = blk_off[k + 1] - blk_off[k];
+    const int n = off[k + 1] - off[k];
+    const double *rk = r + off[k];
+    double s = 0;
+    for (int i = j * BS + threadIdx.x; i < n; i += gk * BS) s += fabs(rk[i]);
+    s = block_sum<BS>(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(BS) k_seg_final(const int *blk_off, const double *part, double *out) {
+    const int k = blockIdx.x;
+    const int m = blk_off[k + 1] - blk_off[k];
+    const double *pk = part + blk_off[k];
+    double s = 0;
+    for (int i = threadIdx.x; i < m; i += BS) s += pk[i];
+    s = block_sum<BS>(s);
+    if (threadIdx.x == 0) out[k] = s;
+}
+
+__global__ void __launch_bounds__(BS) k_batch_init(int K, const int *off, const double *norm, double abs_tol,
+                                                   double *r0, double *rn, uint8_t *act, int *cyc, int *ctr) {
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    int c = 0;
+    for (int k = threadIdx.x; k < K; k += BS) {
+        r0[k] = norm[k];
+        rn[k] = norm[k];
+        cyc[k] = 0;
+        const bool a = !(norm[k] < abs_tol);
+        act[k] = a ? 1 : 0;
+        c += a;
+    }
+    atomicAdd(&cnt, c);
+    __syncthreads();
+    if (threadIdx.x == 0) { ctr[0] = 0; ctr[1] = cnt; }
+}
+
+// stop test of AMG.cpp:262-272 for every system still cycling
+__global__ void __launch_bounds__(BS) k_batch_check(int K, const double *norm, double rel_tol, double abs_tol,
+                                                    const double *r0, double *rn, uint8_t *act, int *cyc,
+                                                    int *ctr) {
+    __shared__ int cnt, it;
+    if (threadIdx.x == 0) { cnt = 0; it = ctr[0] + 1; }
+    __syncthreads();
+    int c = 0;
+    for (int k = threadIdx.x; k < K; k += BS) {
+        if (!act[k]) continue;
+        const double v = norm[k];
+        double ratio = v;
+        if (r0[k] != 0) ratio /= r0[k];
+        rn[k] = v;
+        cyc[k] = it;
+        if (v < abs_tol || ratio < rel_tol) act[k] = 0;
+        else c++;
+    }
+    atomicAdd(&cnt, c);
+    __syncthreads();
+    if (threadIdx.x == 0) { ctr[0] = it; ctr[1] = cnt; }
+}
+
+__global__ void k_batch_mask(int total, int K, const int *off, const uint8_t *act, uint8_t *mask) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x)
+        mask[i] = act[batch_sys(off, K, i)];
+}
+
+// finest residual and per-system norms into B.norm
+void batch_norms(Run &R, BatchInfo &B) {
+    fvm_amg *h = R.h;
+    Level &F = h->lev[0];
+    hipStream_t s = h->stream;
+    Run R64 = R;
+    R64.mixed = false;
+    residual<double>(R64, 0, F.x.p, F.b.p, F.r.p);
+    if (B.nblk > 0)
+        k_seg_norm1_partial<<<B.nblk, BS, 0, s>>>(B.K, B.d_off0.p, B.blk_off.p, F.r.p, B.partials.p);
+    k_seg_final<<<B.K, BS, 0, s>>>(B.blk_off.p, B.partials.p, B.norm.p);
+    R.r0_valid = true;
+}
+
+void batch_iteration(Run &R, BatchInfo &B) {
+    fvm_amg *h = R.h;
+    hipStream_t s = h->stream;
+    one_cycle(R);
+    batch_norms(R, B);
+    k_batch_check<<<1, BS, 0, s>>>(B.K, B.norm.p, h->prm.relativeTolerance, h->prm.absoluteTolerance, B.r0.p,
+                                   B.rn.p, B.act.p, B.cyc.p, B.counters.p);
+    const int total = h->lev[0].n;
+    if (total) k_batch_mask<<<grid_for(total, BS), BS, 0, s>>>(total, B.K, B.d_off0.p, B.act.p, B.mask0.p);
+}
+
+}  // namespace
+
+void batch_solve(fvm_amg *h, BatchInfo &B, const double *const *d_b, double *const *d_x, double *rnorm0,
+                 double *rnorm_final, int *cycles) {
+    FVM_CHECK(h->ready, FVM_E_NOT_SETUP, "batched solve called before setup");
+    Run R = make_run(h);
+    FVM_CHECK(!R.fast && !R.mixed, FVM_E_INVALID, "batched solve supports the exact FP64 cycle only");
+    FVM_CHECK(!(h->prm.isSymmetric && h->prm.scaleCorrections), FVM_E_INVALID,
+              "batched solve does not support scaled corrections");
+    hipStream_t s = h->stream;
+    const int K = B.K;
+    const int total = h->lev[0].n;
+    const std::vector<int> &off0 = B.off[0];
+    if (B.blk_off.n != (size_t)K + 1) {
+        std::vector<int> blk(K + 1, 0);
+        for (int k = 0; k < K; k++) {
+            const int n = off0[k + 1] - off0[k];
+            blk[k + 1] = blk[k] + (n > 0 ? (int)grid_for(n, BS, NORM_BLOCKS) : 0);
+        }
+        B.nblk = blk[K];
+        B.blk_off.alloc(K + 1);
+        FVM_HIP(hipMemcpyAsync(B.blk_off.p, blk.data(), sizeof(int) * (K + 1), hipMemcpyHostToDevice, s));
+        B.partials.alloc(std::max(B.nblk, 1));
+        B.norm.alloc(K); B.r0.alloc(K); B.rn.alloc(K);
+        B.cyc.alloc(K); B.counters.alloc(4);
+        B.act.alloc(K); B.mask0.alloc(std::max(total, 1));
+        B.d_bptr.alloc(K); B.d_xptr.alloc(K);
+        if (!B.h_active) FVM_HIP(hipHostMalloc((void **)&B.h_active, sizeof(int) * 4, hipHostMallocDefault));
+    }
+    FVM_HIP(hipMemcpyAsync(B.d_bptr.p, d_b, sizeof(double *) * K, hipMemcpyHostToDevice, s));
+    FVM_HIP(hipMemcpyAsync(B.d_xptr.p, d_x, sizeof(double *) * K, hipMemcpyHostToDevice, s));
+    Level &F = h->lev[0];
+    if (total)
+        k_batch_gather<<<grid_for(total, BS), BS, 0, s>>>(total, K, B.d_off0.p, B.d_bptr.p, B.d_xptr.p, F.b.p,
+                                                          F.x.p);
+    R.mask0 = B.mask0.p;
+    batch_norms(R, B);
+    k_batch_init<<<1, BS, 0, s>>>(K, B.d_off0.p, B.norm.p, h->prm.absoluteTolerance, B.r0.p, B.rn.p, B.act.p,
+                                  B.cyc.p, B.counters.p);
+    if (total) k_batch_mask<<<grid_for(total, BS), BS, 0, s>>>(total, K, B.d_off0.p, B.act.p, B.mask0.p);
+    FVM_HIP(hipGetLastError());
+    FVM_HIP(hipMemcpyAsync(B.h_active, B.counters.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    FVM_HIP(hipStreamSynchronize(s));
+    int active = *B.h_active;
+    // One iteration (cycle, residual, norms, stop test, masks) is the same launch
+    // sequence every time: captured once into a graph and replayed.
+    const bool saved_ktime = h->ktime;
+    for (int i = 1; i < h->prm.nMaxIterations && active > 0; i++) {
+        if (B.use_graph) {
+            if (!B.graph_ok) {
+                h->ktime = false;
+                hipGraph_t g = nullptr;
+                FVM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                Run Rc = R;
+                try {
+                    batch_iteration(Rc, B);
+                } catch (...) {
+                    (void)hipStreamEndCapture(s, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    h->ktime = saved_ktime;
+                    throw;
+                }
+                FVM_HIP(hipStreamEndCapture(s, &g));
+                h->ktime = saved_ktime;
+                if (B.graph) { (void)hipGraphExecDestroy(B.graph); B.graph = nullptr; }
+                hipError_t e = hipGraphInstantiate(&B.graph, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                FVM_HIP(e);
+                B.graph_ok = true;
+            }
+            FVM_HIP(hipGraphLaunch(B.graph, s));
+        } else {
+            batch_iteration(R, B);
+        }
+        h->total_iterations++;
+        FVM_HIP(hipMemcpyAsync(B.h_active, B.counters.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        FVM_HIP(hipStreamSynchronize(s));
+        active = *B.h_active;
+    }
+    if (total) k_batch_scatter<<<grid_for(total, BS), BS, 0, s>>>(total, K, B.d_off0.p, F.x.p, B.d_xptr.p);
+    FVM_HIP(hipGetLastError());
+    if (rnorm0) FVM_HIP(hipMemcpyAsync(rnorm0, B.r0.p, sizeof(double) * K, hipMemcpyDeviceToHost, s));
+    if (rnorm_final) FVM_HIP(hipMemcpyAsync(rnorm_final, B.rn.p, sizeof(double) * K, hipMemcpyDeviceToHost, s));
+    if (cycles) FVM_HIP(hipMemcpyAsync(cycles, B.cyc.p, sizeof(int) * K, hipMemcpyDeviceToHost, s));
+    FVM_HIP(hipStreamSynchronize(s));
 }
 
 

@@ -1188,4 +1188,243 @@ void prepare_direct(fvm_amg *h) {
     if (read_scalar(st.p, s) == 0) h->direct_level = lv;
 }
 
+// ====================================================================== batched hierarchy
+namespace {
+
+// system of combined row i: the k with off[k] <= i < off[k+1] (empty systems skipped)
+__device__ __forceinline__ int find_sys(const int *off, int K, int i) {
+    int lo = 0, hi = K;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_batch_validate(int total, int K, const BatchSys *sys, const int *off, int *flag) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int k = find_sys(off, K, i);
+        const BatchSys S = sys[k];
+        const int li = i - off[k];
+        const int b = S.row[li], e = S.row[li + 1];
+        if (b < 0 || e < b) { atomicOr(flag, 1); continue; }
+        for (int p = b; p < e; p++) {
+            const int c = S.col[p];
+            if (c < 0 || c >= S.n_total || c == li) atomicOr(flag, 2);
+        }
+        const double d = S.diag[li];
+        if (!(d != 0.0) || !isfinite(d)) atomicOr(flag, 4);
+    }
+}
+
+__global__ void k_batch_compact_count(int total, int K, const BatchSys *sys, const int *off, int *cnt) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int k = find_sys(off, K, i);
+        const BatchSys S = sys[k];
+        const int li = i - off[k];
+        int c = 0;
+        for (int p = S.row[li]; p < S.row[li + 1]; p++) c += (S.col[p] < S.n_self);
+        cnt[i] = c;
+    }
+}
+
+__global__ void k_batch_compact_fill(int total, int K, const BatchSys *sys, const int *off, const int *nrow,
+                                     int *ncol, double *noff, double *ndiag, uint8_t *nbnd) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int k = find_sys(off, K, i);
+        const BatchSys S = sys[k];
+        const int li = i - off[k];
+        int q = nrow[i];
+        for (int p = S.row[li]; p < S.row[li + 1]; p++) {
+            const int c = S.col[p];
+            if (c < S.n_self) { ncol[q] = c + off[k]; noff[q] = S.off[p]; q++; }
+        }
+        ndiag[i] = S.diag[li];
+        if (nbnd) nbnd[i] = S.bnd ? S.bnd[li] : 0;
+    }
+}
+
+// rows of systems that no longer coarsen take no part in the agglomeration
+__global__ void k_batch_mark(int n, int K, const int *off, const uint8_t *stopped, uint8_t *bnd) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (stopped[find_sys(off, K, i)]) bnd[i] = 1;
+}
+
+__global__ void k_batch_group_range(int n, int K, const int *off, const int *ci, int *gmin, int *gmax) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int I = ci[i];
+        if (I < 0) continue;
+        const int k = find_sys(off, K, i);
+        atomicMin(&gmin[k], I);
+        atomicMax(&gmax[k], I);
+    }
+}
+
+__global__ void k_batch_renumber(int n, int K, const int *off, const uint8_t *drop, const int *shift, int *ci) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int I = ci[i];
+        if (I < 0) continue;
+        const int k = find_sys(off, K, i);
+        ci[i] = drop[k] ? -1 : I - shift[k];
+    }
+}
+
+}  // namespace
+
+// Block-diagonal hierarchy of B.K systems with each system's own stop rule
+// (AMG.cpp:149-210 applied per system). Level l holds the level-l rows of every
+// system whose hierarchy reaches l, in system order.
+void build_hierarchy_batched(fvm_amg *h, BatchInfo &B) {
+    const double t0 = now_s();
+    hipStream_t s = h->stream;
+    const int K = B.K;
+    FVM_CHECK(K > 0, FVM_E_INVALID, "empty batch");
+    FVM_CHECK(h->prm.coarseGroupSize == 2, FVM_E_INVALID, "the GPU agglomeration supports coarseGroupSize == 2 only");
+    h->lev.clear();
+    h->ready = false;
+    h->direct_level = -1;
+    h->tail_prec = -1;
+    h->tail_desc.release();
+    std::vector<int> off0(K + 1, 0);
+    bool any_bnd = false;
+    for (int k = 0; k < K; k++) {
+        const BatchSys &S = B.sys[k];
+        FVM_CHECK(S.n_self >= 0 && S.n_total >= S.n_self, FVM_E_INVALID,
+                  "system " + std::to_string(k) + ": need 0 <= n_self <= n_total");
+        FVM_CHECK(S.n_self == 0 || (S.row && S.col && S.diag && S.off), FVM_E_INVALID,
+                  "system " + std::to_string(k) + ": null array");
+        FVM_CHECK((long long)off0[k] + S.n_self < INT_MAX, FVM_E_INVALID, "batch exceeds 2^31 rows");
+        off0[k + 1] = off0[k] + S.n_self;
+        any_bnd = any_bnd || S.bnd;
+    }
+    const int total = off0[K];
+    h->n_self = total;
+    h->n_total = total;
+    B.d_sys.alloc(K);
+    FVM_HIP(hipMemcpyAsync(B.d_sys.p, B.sys.data(), sizeof(BatchSys) * K, hipMemcpyHostToDevice, s));
+    B.d_off0.alloc(K + 1);
+    FVM_HIP(hipMemcpyAsync(B.d_off0.p, off0.data(), sizeof(int) * (K + 1), hipMemcpyHostToDevice, s));
+    {
+        DArr<int> flag(1);
+        FVM_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+        if (total > 0)
+            k_batch_validate<<<grid_for(total, BS, 4096), BS, 0, s>>>(total, K, B.d_sys.p, B.d_off0.p, flag.p);
+        FVM_HIP(hipGetLastError());
+        const int f = read_scalar(flag.p, s);
+        FVM_CHECK(!(f & 1), FVM_E_INVALID, "row pointers are not a valid CSR");
+        FVM_CHECK(!(f & 2), FVM_E_INVALID, "column index out of range or on the diagonal");
+        FVM_CHECK(!(f & 4), FVM_E_NUMERIC, "zero or non-finite diagonal on a self row");
+    }
+    h->lev.emplace_back();
+    {
+        Level &F = h->lev[0];
+        F.n = total;
+        DArr<int> cnt((size_t)total + 1);
+        FVM_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes(), s));
+        if (total > 0)
+            k_batch_compact_count<<<grid_for(total, BS, 4096), BS, 0, s>>>(total, K, B.d_sys.p, B.d_off0.p, cnt.p);
+        F.row.alloc((size_t)total + 1);
+        h->ws.scan.exclusive(cnt.p, F.row.p, total + 1, s);
+        F.nnz = read_scalar(F.row.p + total, s);
+        F.col.alloc(std::max<long long>(F.nnz, 1));
+        F.off.alloc(std::max<long long>(F.nnz, 1));
+        F.diag.alloc(std::max(total, 1));
+        if (any_bnd) F.bnd.alloc(std::max(total, 1));
+        if (total > 0)
+            k_batch_compact_fill<<<grid_for(total, BS, 4096), BS, 0, s>>>(total, K, B.d_sys.p, B.d_off0.p, F.row.p,
+                                                                         F.col.p, F.off.p, F.diag.p,
+                                                                         any_bnd ? F.bnd.p : nullptr);
+        FVM_HIP(hipGetLastError());
+    }
+    B.off.assign(1, off0);
+    B.depth.assign(K, 1);
+    std::vector<long long> fine_size(K);
+    std::vector<char> active(K);
+    for (int k = 0; k < K; k++) {
+        fine_size[k] = B.sys[k].n_total;   // getLocalSize() of the finest level counts ghosts
+        active[k] = B.sys[k].n_self > 0;
+    }
+    DArr<int> d_off(K + 1), gmin(K), gmax(K), shift(K);
+    DArr<uint8_t> flags(K);
+    std::vector<int> hmin(K), hmax(K), hshift(K);
+    std::vector<uint8_t> hflag(K);
+    for (int lvl = 0; lvl < h->prm.maxCoarseLevels; lvl++) {
+        if (std::none_of(active.begin(), active.end(), [](char a) { return a != 0; })) break;
+        Level &A = h->lev.back();
+        const std::vector<int> offl = B.off[lvl];
+        FVM_HIP(hipMemcpyAsync(d_off.p, offl.data(), sizeof(int) * (K + 1), hipMemcpyHostToDevice, s));
+        bool any_stopped = false;
+        for (int k = 0; k < K; k++) {
+            hflag[k] = (!active[k] && offl[k + 1] > offl[k]) ? 1 : 0;
+            any_stopped = any_stopped || hflag[k];
+        }
+        if (any_stopped) {
+            if (!A.bnd.p) {
+                A.bnd.alloc(std::max(A.n, 1));
+                FVM_HIP(hipMemsetAsync(A.bnd.p, 0, A.bnd.bytes(), s));
+            }
+            FVM_HIP(hipMemcpyAsync(flags.p, hflag.data(), K, hipMemcpyHostToDevice, s));
+            k_batch_mark<<<grid_for(A.n, BS, 4096), BS, 0, s>>>(A.n, K, d_off.p, flags.p, A.bnd.p);
+        }
+        coarsen_level(h, A, h->prm.weightRatioThreshold);
+        if (A.nc == 0) { A.ci.release(); break; }
+        k_fill_int<<<grid_for(K, BS), BS, 0, s>>>(K, gmin.p, INT_MAX);
+        k_fill_int<<<grid_for(K, BS), BS, 0, s>>>(K, gmax.p, -1);
+        k_batch_group_range<<<grid_for(A.n, BS, 4096), BS, 0, s>>>(A.n, K, d_off.p, A.ci.p, gmin.p, gmax.p);
+        FVM_HIP(hipGetLastError());
+        FVM_HIP(hipMemcpyAsync(hmin.data(), gmin.p, sizeof(int) * K, hipMemcpyDeviceToHost, s));
+        FVM_HIP(hipMemcpyAsync(hmax.data(), gmax.p, sizeof(int) * K, hipMemcpyDeviceToHost, s));
+        FVM_HIP(hipStreamSynchronize(s));
+        std::vector<int> nck(K, 0);
+        bool any_drop = false;
+        int dropped = 0;
+        for (int k = 0; k < K; k++) {
+            nck[k] = hmax[k] >= 0 ? hmax[k] - hmin[k] + 1 : 0;
+            hflag[k] = 0;
+            hshift[k] = dropped;
+            if (!active[k]) continue;
+            if (nck[k] == fine_size[k] || (!h->prm.parallelStopRule && nck[k] <= 3)) {   // AMG.cpp:163-168
+                hflag[k] = 1;
+                any_drop = true;
+                dropped += nck[k];
+                active[k] = 0;
+            }
+        }
+        if (any_drop) {
+            FVM_HIP(hipMemcpyAsync(flags.p, hflag.data(), K, hipMemcpyHostToDevice, s));
+            FVM_HIP(hipMemcpyAsync(shift.p, hshift.data(), sizeof(int) * K, hipMemcpyHostToDevice, s));
+            k_batch_renumber<<<grid_for(A.n, BS, 4096), BS, 0, s>>>(A.n, K, d_off.p, flags.p, shift.p, A.ci.p);
+            FVM_HIP(hipGetLastError());
+            A.nc -= dropped;
+        }
+        if (A.nc == 0) { A.ci.release(); break; }
+        coarse_to_fine(h, A);
+        Level C;
+        coarse_operator(h, A, C);
+        h->lev.push_back(std::move(C));
+        std::vector<int> offn(K + 1, 0);
+        for (int k = 0; k < K; k++) {
+            const bool kept = active[k] && !hflag[k];
+            offn[k + 1] = offn[k] + (kept ? nck[k] : 0);
+            if (kept) {
+                fine_size[k] = nck[k];
+                B.depth[k] = lvl + 2;
+                if (h->prm.parallelStopRule && nck[k] <= 3) active[k] = 0;   // AMG.cpp:180-181
+            }
+        }
+        FVM_CHECK(offn[K] == h->lev.back().n, FVM_E_INVALID, "batched coarse level size mismatch");
+        B.off.push_back(offn);
+    }
+    for (auto &L : h->lev) alloc_vectors(L);
+    h->ws.partials.ensure(4096);
+    h->ws.scal.ensure(64);
+    FVM_HIP(hipStreamSynchronize(s));
+    h->ready = true;
+    if (h->prm.smootherType == FVM_GAUSS_SEIDEL)
+        for (auto &L : h->lev) prepare_exact_schedule(h, L);
+    FVM_HIP(hipStreamSynchronize(s));
+    h->setup_seconds = now_s() - t0;
+}
+
 }  // namespace fvm

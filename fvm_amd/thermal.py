@@ -78,6 +78,10 @@ class Mesh:
         check(lib().fvm_mesh_metrics(self._h, _vp(fA), _vp(fAm), _vp(fc), _vp(cc), _vp(vol)))
         return dict(fA=fA, fAm=fAm, fc=fc, cc=cc, vol=vol)
 
+    def metrics_device(self):
+        """Compute connectivity and metrics on the device without copying them out."""
+        check(lib().fvm_mesh_metrics(self._h, None, None, None, None, None))
+
     def lognormal_field(self, out, sigma, block=1, seed=0):
         """Fill a device tensor (torch, float64, n_total entries)."""
         check(lib().fvm_mesh_lognormal_field(self._h, float(sigma), int(block), int(seed),
@@ -137,4 +141,18 @@ def assemble(mesh, k, source, x, bc_types, bc_values):
     h = ctypes.c_void_p()
     check(lib().fvm_thermal_assemble(mesh._h, ctypes.c_void_p(k.data_ptr()), ctypes.c_void_p(source.data_ptr()),
                                      ctypes.c_void_p(x.data_ptr()), _vp(t), _vp(v), ctypes.byref(h)))
+    return System(h)
+
+
+def assemble_phonon(mesh, vg, tau, e0, x):
+    """One discrete-ordinate direction of the gray phonon BTE on the device
+    (PhononModel::linearizePhononModel, PhononModel.h:437-523): vg is the group
+    velocity (3 floats), tau the relaxation time, e0 (n_cells) and x
+    (mesh.n_total, ghosts hold boundary values) torch float64 tensors."""
+    v = np.ascontiguousarray(vg, np.float64)
+    if v.size != 3:
+        raise ValueError("vg needs 3 components")
+    h = ctypes.c_void_p()
+    check(lib().fvm_phonon_assemble(mesh._h, _vp(v), float(tau), ctypes.c_void_p(e0.data_ptr()),
+                                    ctypes.c_void_p(x.data_ptr()), ctypes.byref(h)))
     return System(h)

@@ -201,6 +201,38 @@ int fvm_fgmres_history(const fvm_fgmres *s, double *hist, int cap);
 
 
 /* ---------------------------------------------------------------------------
+ * Batched solve of many independent scalar systems (the per-direction systems
+ * of PhononModel::advance, PhononModel.h:530-600, which calls AMG::solve once
+ * per direction). One handle builds every system's hierarchy at once (a
+ * block-diagonal hierarchy; each system keeps its own maps, levels and stop
+ * rule, AMG.cpp:149-210) and cycles all systems together, each with its own
+ * stop test (AMG.cpp:219-282): per system the result equals that system's own
+ * fvm_amg solve with the same parameters. One cycle of all systems is captured
+ * into a HIP graph and replayed. Supported: gsOrdering FVM_GS_EXACT, precision
+ * FVM_PREC_FP64, no scaled corrections.
+ * All array arguments are host arrays of nsys entries; the pointers they hold
+ * are device pointers (CSR as in fvm_amg_setup_device; is_boundary may be NULL,
+ * or hold NULL entries). rnorm0/rnorm_final/cycles: host arrays of nsys.
+ * ------------------------------------------------------------------------- */
+typedef struct fvm_batch fvm_batch;
+int fvm_batch_create(fvm_batch **out, int device);
+int fvm_batch_destroy(fvm_batch *b);
+int fvm_batch_set_params(fvm_batch *b, const fvm_amg_params *p);
+int fvm_batch_setup_device(fvm_batch *b, int nsys, const int32_t *n_self, const int32_t *n_total,
+                           const int32_t *const *d_row, const int32_t *const *d_col,
+                           const double *const *d_diag, const double *const *d_off,
+                           const uint8_t *const *d_is_boundary);
+int fvm_batch_solve_device(fvm_batch *b, const double *const *d_b, double *const *d_x,
+                           double *rnorm0, double *rnorm_final, int *cycles);
+/* rows of each level of system sys; returns that system's number of levels */
+int fvm_batch_level_sizes(const fvm_batch *b, int sys, int32_t *sizes, int cap);
+/* "graph" (1: replay the captured cycle, default; 0: plain launches) and the
+ * fvm_amg_set_tuning keys */
+int fvm_batch_set_tuning(fvm_batch *b, const char *key, long long value);
+int fvm_batch_setup_stats(const fvm_batch *b, double *setup_seconds, int *total_rows, int *levels);
+
+
+/* ---------------------------------------------------------------------------
  * Finite-volume assembly of the thermal (diffusion) equation on the device.
  * Reference: ThermalModel linearization (ThermalModel_impl.h:236-398) with
  * MeshMetricsCalculator (MeshMetricsCalculator_impl.h:59-462),
@@ -249,6 +281,17 @@ int fvm_mesh_uniform_field(fvm_mesh *m, double lo, double hi, uint64_t seed, dou
  * (bc_type/bc_value indexed by group id, entry 0 unused) into a new system. */
 int fvm_thermal_assemble(fvm_mesh *m, const double *d_k, const double *d_source, const double *d_x,
                          const int32_t *bc_type, const double *bc_value, fvm_system **out);
+/* One discrete-ordinate direction of a gray phonon BTE (scalar system per
+ * direction, PhononModel::linearizePhononModel, PhononModel.h:437-523):
+ * relaxation -V (x - e0)/tau (PhononCollisionDiscretization.h:54-68), upwind
+ * convection with face flux F = A . vg (PhononConvectionDiscretization.h:150-175),
+ * per boundary face Dirichlet with the ghost's current value where vg points
+ * into the domain and extrapolation elsewhere (PhononModel.h:498-523,
+ * GenericPhononBCS.h:67-124), then initSolve's elimination of the extrapolated
+ * ghost rows. vg: host double[3]; d_e0: device, n_cells entries; d_x: device,
+ * n_cells + n_ghost entries (ghosts hold the boundary values). */
+int fvm_phonon_assemble(fvm_mesh *m, const double *vg, double tau, const double *d_e0,
+                        const double *d_x, fvm_system **out);
 int fvm_system_info(const fvm_system *S, int *n_self, int *n_total, long long *nnz);
 int fvm_system_device(const fvm_system *S, const int32_t **row, const int32_t **col,
                       const double **diag, const double **off, const double **b,

@@ -273,3 +273,116 @@ int ofv_thermal(int n_faces, const double *nodes, const int *face_nodes, const i
     free(fA); free(fAm); free(fc); free(cc); free(vol); free(p01); free(p10);
     return 0;
 }
+
+/*
+ * Gray phonon-BTE style scalar system of one direction (discrete ordinate):
+ *   collision   PhononCollisionDiscretization.h:54-68  (phononbase/):
+ *               r[c] -= V (x - e0) / tau,  diag[c] -= V / tau       (self cells)
+ *   convection  PhononConvectionDiscretization.h:150-175 (upwind, no grid flux):
+ *               F = A . vg;  F > 0: r0 -= F x0, r1 += F x0, d0 -= F, a10 += F
+ *                            else : r0 -= F x1, r1 += F x1, d1 += F, a01 -= F
+ *   BCs         PhononModel.h:498-523 + GenericPhononBCS.h:67-131: per boundary
+ *               face, vg . (A/|A|) < 0 -> Dirichlet with the ghost's own value,
+ *               else extrapolation (ghost row marked boundary)
+ *   initSolve   eliminate the marked ghost rows (CRMatrix.h:899-944)
+ * The linearizer runs collision before convection (PhononModel.h:449-459).
+ * Loops run in the reference's order (faces ascending, groups in order).
+ */
+int ofv_phonon(int n_faces, const double *nodes, const int *face_nodes, const int *face_cells,
+               int n_cells, int n_ghost, int n_groups, const int *gptr, const double *vg, double tau,
+               const double *e0, const double *x, int *row, int *col, double *diag, double *off,
+               double *b, unsigned char *is_bnd)
+{
+    const int nct = n_cells + n_ghost;
+    double *fA = (double *)malloc(sizeof(double) * 3 * (size_t)n_faces);
+    double *fAm = (double *)malloc(sizeof(double) * (size_t)n_faces);
+    double *fc = (double *)malloc(sizeof(double) * 3 * (size_t)n_faces);
+    double *cc = (double *)malloc(sizeof(double) * 3 * (size_t)nct);
+    double *vol = (double *)malloc(sizeof(double) * (size_t)nct);
+    ofv_metrics(n_faces, nodes, face_nodes, face_cells, n_cells, n_ghost, n_groups, gptr, fA, fAm, fc, cc, vol);
+    int *p01 = (int *)malloc(sizeof(int) * (size_t)n_faces);
+    int *p10 = (int *)malloc(sizeof(int) * (size_t)n_faces);
+    int *c2;
+    cell_cells(n_faces, face_cells, nct, row, &c2, p01, p10);
+    memcpy(col, c2, sizeof(int) * (size_t)row[nct]);
+    free(c2);
+    memset(diag, 0, sizeof(double) * (size_t)nct);
+    memset(off, 0, sizeof(double) * (size_t)row[nct]);
+    memset(b, 0, sizeof(double) * (size_t)nct);
+    memset(is_bnd, 0, (size_t)nct);
+    /* the boundary-condition code writes the field (Dirichlet ghosts) */
+    double *xf = (double *)malloc(sizeof(double) * (size_t)nct);
+    memcpy(xf, x, sizeof(double) * (size_t)nct);
+    for (int c = 0; c < n_cells; c++) {
+        b[c] -= vol[c] * (xf[c] - e0[c]) / tau;
+        diag[c] -= vol[c] / tau;
+    }
+    for (int f = 0; f < n_faces; f++) {
+        const int c0 = face_cells[2 * f], c1 = face_cells[2 * f + 1];
+        const double F = fA[3 * f] * vg[0] + fA[3 * f + 1] * vg[1] + fA[3 * f + 2] * vg[2];
+        double flux;
+        if (F > 0.0) {
+            flux = F * xf[c0];
+            diag[c0] -= F;
+            off[p10[f]] += F;
+        } else {
+            flux = F * xf[c1];
+            diag[c1] += F;
+            off[p01[f]] -= F;
+        }
+        b[c0] -= flux;
+        b[c1] += flux;
+    }
+    for (int g = 1; g < n_groups; g++)
+        for (int f = gptr[g]; f < gptr[g + 1]; f++) {
+            const int c0 = face_cells[2 * f], c1 = face_cells[2 * f + 1];
+            double en[3];
+            for (int k = 0; k < 3; k++) en[k] = fA[3 * f + k] / fAm[f];
+            const double vn = vg[0] * en[0] + vg[1] * en[1] + vg[2] * en[2];
+            if (vn < 0.0) {
+                const double bval = xf[c1];
+                const double dX = bval - xf[c1];
+                b[c0] += off[p01[f]] * dX;
+                off[p01[f]] = 0.0;
+                xf[c1] = bval;
+                off[p10[f]] = 0.0;
+                b[c1] = 0.0;
+                diag[c1] = -1.0;
+            } else {
+                const double dFdX1 = -diag[c1];
+                const double dx01 = xf[c0] - xf[c1];
+                diag[c0] += dFdX1;
+                b[c0] += dFdX1 * dx01;
+                off[p01[f]] = 0.0;
+                diag[c1] = -1.0;
+                off[p10[f]] = 1.0;
+                b[c1] = dx01;
+                is_bnd[c1] = 1;
+            }
+        }
+    for (int j = n_cells; j < nct; j++) {
+        if (!is_bnd[j]) continue;
+        const double ajj = diag[j];
+        for (int nb = row[j]; nb < row[j + 1]; nb++) {
+            const int i = col[nb];
+            int pij = -1;
+            for (int e = row[i]; e < row[i + 1]; e++) if (col[e] == j) { pij = e; break; }
+            if (pij < 0) continue;
+            const double aij = off[pij];
+            for (int nb2 = row[j]; nb2 < row[j + 1]; nb2++) {
+                const int k = col[nb2];
+                const double ajk = off[nb2];
+                if (i != k) {
+                    for (int e = row[i]; e < row[i + 1]; e++)
+                        if (col[e] == k) { off[e] -= aij * (ajk / ajj); break; }
+                } else {
+                    diag[i] -= aij * (ajk / ajj);
+                }
+            }
+            b[i] -= aij * (b[j] / ajj);
+            off[pij] = 0.0;
+        }
+    }
+    free(xf); free(fA); free(fAm); free(fc); free(cc); free(vol); free(p01); free(p10);
+    return 0;
+}

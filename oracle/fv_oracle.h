@@ -13,6 +13,10 @@ int ofv_thermal(int n_faces, const double *nodes, const int *face_nodes, const i
                 int n_cells, int n_ghost, int n_groups, const int *gptr, const int *bc_type,
                 const double *bc_value, const double *kcell, const double *src, const double *x,
                 int *row, int *col, double *diag, double *off, double *b, unsigned char *is_bnd);
+int ofv_phonon(int n_faces, const double *nodes, const int *face_nodes, const int *face_cells,
+               int n_cells, int n_ghost, int n_groups, const int *gptr, const double *vg, double tau,
+               const double *e0, const double *x, int *row, int *col, double *diag, double *off,
+               double *b, unsigned char *is_bnd);
 #ifdef __cplusplus
 }
 #endif

@@ -55,6 +55,8 @@ def lib():
         L.ofv_thermal_nnz.argtypes = [_i, _p, _i, _i]
         L.ofv_thermal.argtypes = [_i, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _p, _p,
                                   _p, _p, _p, _p, _p, _p]
+        L.ofv_phonon.argtypes = [_i, _p, _p, _p, _i, _i, _i, _p, _p, _d, _p, _p, _p, _p, _p, _p,
+                                 _p, _p]
         _lib = L
     return _lib
 
@@ -89,7 +91,10 @@ class OracleAMG:
         p = OmParams()
         self._L.om_default_params(ctypes.byref(p))
         self._p = p
+        names = {f for f, _ in OmParams._fields_}
         for k, v in kw.items():
+            if k not in names:
+                raise TypeError("unknown oracle parameter %r" % k)
             setattr(p, k, v)
 
     def __del__(self):
@@ -195,5 +200,25 @@ def fv_thermal(mesh, k, src, x, bc_type, bc_value):
     L.ofv_thermal(nf, ptr(mesh["nodes"]), ptr(mesh["face_nodes"]), ptr(mesh["face_cells"]), nc, ng,
                   gp.size - 1, ptr(gp), ptr(bt), ptr(bv), ptr(k), ptr(src), ptr(x), ptr(row), ptr(col),
                   ptr(diag), ptr(off), ptr(b), ptr(isb))
+    return dict(n_self=nc, n_total=nct, row=row, col=col[:nnz], diag=diag, off=off[:nnz], b=b,
+                is_boundary=isb)
+
+
+def fv_phonon(mesh, vg, tau, e0, x):
+    """Oracle phonon-direction assembly (upwind convection + relaxation) -> system dict."""
+    L = lib()
+    nf = mesh["face_cells"].size // 2
+    nc, ng = mesh["n_cells"], mesh["n_ghost"]
+    nct = nc + ng
+    nnz = L.ofv_thermal_nnz(nf, ptr(mesh["face_cells"]), nc, ng)
+    row = np.zeros(nct + 1, np.int32); col = np.zeros(max(nnz, 1), np.int32)
+    diag = np.zeros(nct); off = np.zeros(max(nnz, 1)); b = np.zeros(nct); isb = np.zeros(nct, np.uint8)
+    gp = np.ascontiguousarray(mesh["group_ptr"], np.int32)
+    v = np.ascontiguousarray(vg, np.float64)
+    e0 = np.ascontiguousarray(e0, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    L.ofv_phonon(nf, ptr(mesh["nodes"]), ptr(mesh["face_nodes"]), ptr(mesh["face_cells"]), nc, ng,
+                 gp.size - 1, ptr(gp), ptr(v), ctypes.c_double(tau), ptr(e0), ptr(x), ptr(row), ptr(col),
+                 ptr(diag), ptr(off), ptr(b), ptr(isb))
     return dict(n_self=nc, n_total=nct, row=row, col=col[:nnz], diag=diag, off=off[:nnz], b=b,
                 is_boundary=isb)

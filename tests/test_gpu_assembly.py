@@ -104,3 +104,33 @@ def test_amg_on_device_assembled_system_exact():
     assert amg.num_levels() == oa.num_levels()
     for l in range(1, oa.num_levels()):
         assert np.array_equal(amg.level(l)["diag"], oa.level(l)["diag"])
+
+
+def _fib(i, n=256):
+    z = 1.0 - (2 * i + 1) / n
+    phi = i * np.pi * (3.0 - np.sqrt(5.0))
+    r = np.sqrt(1.0 - z * z)
+    return np.array([r * np.cos(phi), r * np.sin(phi), z])
+
+
+@pytest.mark.parametrize("N,jit,seed,i,tau", [(4, 0.0, 0, 0, 0.5), (6, 0.2, 3, 77, 3.0), (7, 0.2, 9, 200, 9.5)])
+def test_phonon_system_bit_identical(N, jit, seed, i, tau):
+    import torch
+    from fvm_amd.thermal import Mesh, assemble_phonon
+    m = Mesh.hex(N, jit, seed)
+    ref = hexmesh.hex_mesh(N, jit, seed)
+    nc = ref["n_cells"]
+    rng = np.random.default_rng(seed)
+    e0 = rng.uniform(0.5, 1.5, nc)
+    x = np.zeros(m.n_total)
+    fc = ref["face_cells"].reshape(-1, 2)
+    gp = ref["group_ptr"]
+    x[fc[gp[1]:gp[2], 1]] = 1.0
+    x[:nc] = rng.uniform(0, 0.1, nc)
+    te0 = torch.tensor(e0, dtype=torch.float64, device="cuda:0")
+    tx = torch.tensor(x, dtype=torch.float64, device="cuda:0")
+    vg = _fib(i)
+    S = assemble_phonon(m, vg, tau, te0, tx).to_host()
+    o = O.fv_phonon(ref, vg, tau, e0, x)
+    for key in ("row", "col", "diag", "off", "b", "is_boundary"):
+        assert np.array_equal(S[key], o[key]), key
