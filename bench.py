@@ -182,6 +182,8 @@ def run_batch(args):
     xs = [torch.zeros(S.n_total, dtype=torch.float64, device="cuda:0") for S in systems]
     B.solve_device([d["b"] for d in devs], xs)
     torch.cuda.synchronize(); td = time.perf_counter()
+    log("batched sweep: %.1f ms (assemble %.1f, setup %.1f, solve %.1f ms), cycles %d..%d"
+        % (ms, 1e3 * (tb - ta), 1e3 * (tc - tb), 1e3 * (td - tc), cyc.min(), cyc.max()))
     # the same systems one handle at a time (unbatched loop, as the reference calls AMG::solve)
     t_loop = None
     if not args.skip_plain:
@@ -369,6 +371,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "setup_ms": round(1e3 * st["setup_seconds"], 2),
+        "finest_coarsening_rounds": st["finest_rounds"],
         "fgmres_iterations": it,
         "final_ratio": rf / r0,
         "plain_fp64_gpu_amg_s": None if t_plain is None else round(t_plain, 4),

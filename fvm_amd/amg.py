@@ -87,7 +87,7 @@ class AMG:
         """Build from device tensors (torch, on this handle's device)."""
         self.isSymmetric = int(isSymmetric)
         self._push_params()
-        dp = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+        dp = lambda t: None if t is None else _dptr(t)
         check(lib().fvm_amg_setup_device(self._h, int(n_self), int(n_total), dp(row), dp(col), dp(diag),
                                          dp(off), dp(is_boundary)))
         self._finest = "device"

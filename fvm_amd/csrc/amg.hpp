@@ -129,6 +129,8 @@ struct fvm_amg {
     // single-workgroup tail of the V-cycle / is coarsened by one workgroup
     long long tail_max_rows = 40000;
     long long coarsen_small_rows = 16384;
+    int coarsen_coop = 1;              // larger levels: all rounds in one cooperative launch
+    int coop_blocks = -1;              // co-resident blocks for that launch (-1: not queried)
     // single-workgroup tail of the V-cycle: levels >= tail_level run in one launch
     int tail_level = -1;
     int tail_prec = -1;                // precision the descriptors were built for

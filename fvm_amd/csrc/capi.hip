@@ -378,6 +378,8 @@ int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value) {
     } else if (k == "coarsen_small_rows") {
         FVM_CHECK(value >= 0, FVM_E_INVALID, "coarsen_small_rows must be >= 0");
         h->coarsen_small_rows = value;
+    } else if (k == "coarsen_coop") {
+        h->coarsen_coop = value != 0;
     } else {
         throw fvm::Error(FVM_E_INVALID, "unknown tuning key: " + k);
     }
@@ -390,6 +392,7 @@ int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value) {
     const std::string k(key);
     if (k == "tail_max_rows") *value = h->tail_max_rows;
     else if (k == "coarsen_small_rows") *value = h->coarsen_small_rows;
+    else if (k == "coarsen_coop") *value = h->coarsen_coop;
     else return FVM_E_INVALID;
     return FVM_OK;
 }

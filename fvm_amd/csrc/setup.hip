@@ -13,6 +13,8 @@
 // yields exactly the sequential result (see k_coarsen_round).
 #include "amg.hpp"
 
+#include <hip/hip_cooperative_groups.h>
+
 #include <algorithm>
 #include <chrono>
 #include <climits>
@@ -348,6 +350,41 @@ k_coarsen_small(int n, int max_rounds, const int *row, const int *col, const int
         int *tmp = cur; cur = nxt; nxt = tmp;
     }
     if (threadIdx.x == 0) *rounds_out = t;
+}
+
+// All rounds of a large level in one cooperative launch: the rounds are
+// separated by grid-wide barriers instead of kernel launches. Every block leaves
+// the loop on the same value: the size of the next frontier, which is final at
+// the barrier and not written again until the round after it.
+// Counters: qc[t % 3] = size of frontier t; round t appends to qc[(t+1) % 3] and
+// clears qc[(t+2) % 3] (read for the last time in round t-1).
+__global__ void __launch_bounds__(256)
+k_coarsen_coop(int max_rounds, const int *row, const int *col, const int *rowT, const int *colT, int sym,
+               const double *off, const double *diag, const uint8_t *bnd, double theta, int *ldr, int *dround,
+               int *mem, int *memr, int *mcnt, int *cnt, int *qa, int *qb, int *qc, int *remaining,
+               int *rounds_out) {
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    int *cur = qa, *nxt = qb;
+    const int per_wave = 64 / SUB;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    int t = 1;
+    for (; t <= max_rounds; t++) {
+        const int nf = atomicAdd(&qc[t % 3], 0);   // performed in L2: never a stale copy
+        if (nf == 0) break;
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicExch(&qc[(t + 2) % 3], 0);
+        for (int base = wave * per_wave; base < nf; base += nwaves * per_wave) {
+            const int i = base + (__lane_id() / SUB);
+            if (i < nf)
+                decide_row(cur[i], t, theta, row, col, rowT, colT, sym, off, diag, bnd, ldr, dround, mem, memr,
+                           mcnt, cnt, nxt, &qc[(t + 1) % 3], remaining);
+        }
+        __threadfence();   // publish this round's stores
+        grid.sync();
+        __threadfence();   // and see the other blocks' ones
+        int *tmp = cur; cur = nxt; nxt = tmp;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *rounds_out = t;
 }
 
 // transpose pattern (order inside a row is irrelevant to its users)
@@ -776,6 +813,21 @@ void ensure_transpose(fvm_amg *h, Level &L) {
     FVM_HIP(hipGetLastError());
 }
 
+// co-resident 256-thread blocks for k_coarsen_coop on the handle's device (0: no
+// cooperative launch)
+int coop_blocks(fvm_amg *h) {
+    if (h->coop_blocks >= 0) return h->coop_blocks;
+    int coop = 0, ncu = 0, per_cu = 0;
+    h->coop_blocks = 0;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, h->device) != hipSuccess || !coop)
+        return 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_coarsen_coop, 256, 0) != hipSuccess)
+        return 0;
+    h->coop_blocks = std::max(0, std::min(ncu * std::min(per_cu, 4), 2048));
+    return h->coop_blocks;
+}
+
 // Agglomerate level L (reference order); fills L.ci, L.nc and returns the rounds used.
 int coarsen_level(fvm_amg *h, Level &L, double theta) {
     hipStream_t s = h->stream;
@@ -811,6 +863,22 @@ int coarsen_level(fvm_amg *h, Level &L, double theta) {
         t = read_scalar(rounds_out, s);
         FVM_CHECK(rem == 0, FVM_E_NUMERIC,
                   "coarsening did not terminate (single-workgroup path, n=" + std::to_string(n) + ", rounds=" +
+                      std::to_string(t) + ", undecided=" + std::to_string(rem) + ")");
+    } else if (h->coarsen_coop && coop_blocks(h) > 0) {
+        // k_cnt_init put frontier 1 in q1 with its size in qc[1]
+        int max_rounds = (int)std::min<long long>(cap, INT_MAX - 1);
+        void *args[] = {&max_rounds, (void *)&L.row.p, (void *)&L.col.p, (void *)&rowT, (void *)&colT,
+                        (void *)&sym, (void *)&L.off.p, (void *)&L.diag.p, (void *)&bnd, (void *)&theta,
+                        (void *)&ldr.p, (void *)&dround.p, (void *)&mem.p, (void *)&memr.p, (void *)&mcnt.p,
+                        (void *)&cnt.p, (void *)&q1, (void *)&q0, (void *)&qc, (void *)&remaining,
+                        (void *)&rounds_out};
+        FVM_HIP(hipLaunchCooperativeKernel((const void *)k_coarsen_coop, dim3(coop_blocks(h)), dim3(256), args, 0,
+                                           s));
+        FVM_HIP(hipGetLastError());
+        const int rem = read_scalar(remaining, s);
+        t = read_scalar(rounds_out, s);
+        FVM_CHECK(rem == 0, FVM_E_NUMERIC,
+                  "coarsening did not terminate (cooperative path, n=" + std::to_string(n) + ", rounds=" +
                       std::to_string(t) + ", undecided=" + std::to_string(rem) + ")");
     } else {
         const unsigned g_round = 1024;

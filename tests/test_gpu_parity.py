@@ -287,13 +287,18 @@ def test_device_entry_points_match_host():
     assert np.array_equal(x.cpu().numpy(), ls.delta)
 
 
-@pytest.mark.parametrize("small", [0, 1 << 30])
-def test_coarsening_paths_agree(small):
-    # multi-block rounds (small=0) and single-workgroup rounds (all levels) give the oracle's maps
-    m = matgen.grid3d(14, seed=21, contrast=2.0)
+@pytest.mark.parametrize("small,coop,kat", [(0, 1, 0), (0, 0, 0), (1 << 30, 1, 0), (0, 1, 1), (0, 0, 1)])
+def test_coarsening_paths_agree(small, coop, kat):
+    # cooperative single-launch rounds, one launch per round, and single-workgroup
+    # rounds all give the oracle's maps (symmetric grid and the non-symmetric KAT-226)
+    if kat:
+        m = O.load_mm(os.path.join(G, "MatrixMarket226.dat"), os.path.join(G, "rhs226.dat"))
+    else:
+        m = matgen.grid3d(14, seed=21, contrast=2.0)
     oa = oracle_for(m)
     ga = gpu_amg()
     ga.set_tuning("coarsen_small_rows", small)
+    ga.set_tuning("coarsen_coop", coop)
     ls = ls_from(m)
     ga.setup(ls)
     check_hierarchy(ga, oa, m)
