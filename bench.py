@@ -132,16 +132,17 @@ def build_batch_workload(torch, K=256, lo=1e4, hi=5e5):
     return meshes, fields, m
 
 
-def batch_params(obj):
+def batch_params(obj, gs_order=1):
     obj.relativeTolerance = 1e-8
-    obj.nMaxIterations = 200
+    obj.nMaxIterations = 500
     obj.verbosity = 0
+    obj.gsOrdering = gs_order
 
 
 def run_batch(args):
     """Per sweep of all K directions: assemble every system, build every
-    hierarchy, solve every system to 1e-8 (reference-order FP64 AMG per system,
-    PhononModel::advance), all systems in one batch handle."""
+    hierarchy, solve every system to 1e-8 (FP64 AMG per system with coloured
+    Gauss-Seidel, PhononModel::advance), all systems in one batch handle."""
     import numpy as np
     import torch
     import fvm_amd
@@ -220,7 +221,7 @@ def run_batch(args):
         "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seeded meshes, directions and relaxation times)",
         "config": {"workload": "%d phonon-BTE directions, m^3 jittered hex cubes, m in [%d, %d], %d cells total; "
-                               "upwind convection + relaxation, reference-order FP64 AMG per system, batched"
+                               "upwind convection + relaxation, FP64 AMG (coloured GS) per system, batched"
                                % (K, int(m.min()), int(m.max()), ncells)},
         "cycles_min": int(cyc.min()), "cycles_max": int(cyc.max()), "cycles_mean": float(cyc.mean()),
         "assemble_s": round(tb - ta, 4), "setup_s": round(tc - tb, 4), "solve_s": round(td - tc, 4),
@@ -228,6 +229,35 @@ def run_batch(args):
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def run_profile(args):
+    """Short program for rocprofv3 (kernel trace and PMC passes): the 8M-cell
+    system, one mixed-precision setup, `--steps` preconditioner cycles of each
+    kind (mixed FP32/BF16 and plain FP64, coloured layout) and one 1 GiB device
+    copy (512 MiB read + 512 MiB written) that calibrates the HBM byte counters."""
+    import torch
+    N = args.n
+    mesh, S, _ = build_workload(N, torch)
+    n_total = S.n_total
+    b = S.device_pointers()["b"]
+    for prec in (1, 0):
+        amg = make_amg(1, prec)
+        setup_on(amg, S)
+        x = torch.zeros(n_total, dtype=torch.float64, device="cuda:0")
+        from fvm_amd._lib import check, lib
+        import ctypes
+        for _ in range(args.steps):
+            check(lib().fvm_amg_smooth_device(amg._h, ctypes.c_void_p(b), ctypes.c_void_p(x.data_ptr())))
+        torch.cuda.synchronize()
+        lv = level_sizes(amg)
+        log("profiled %s cycles: n=%d nnz=%d" % ("mixed" if prec else "fp64", lv[0][0], lv[0][1]))
+        del amg
+    src = torch.ones(64 << 20, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log("calibration copy: %d bytes read, %d bytes written" % (src.numel() * 8, dst.numel() * 8))
 
 
 def main():
@@ -239,6 +269,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-exact", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="short program for rocprofv3: a few cycles of the hot kernels and a calibration copy")
     ap.add_argument("--batch", type=int, default=0,
                     help="instead of the 8M-cell solve: time a sweep of this many phonon direction systems")
     args = ap.parse_args()
@@ -248,6 +280,9 @@ def main():
         sys.exit(2)
     if args.batch:
         run_batch(args)
+        return
+    if args.profile:
+        run_profile(args)
         return
     import torch
     import fvm_amd
@@ -273,6 +308,9 @@ def main():
         r0, rf, it = fg.solve_device(b, x)
         return x, r0, rf, it
 
+    setup_on(amg, S)
+    tune = amg.autotune(reps=2)                 # launch shapes only: results do not change
+    log("autotune: %s" % json.dumps(tune["chosen"]))
     for w in range(args.warmup):
         x, r0, rf, it = step()
         log("warmup %d: iterations %d ratio %.3e" % (w, it, rf / r0))
@@ -299,6 +337,8 @@ def main():
     kp_s = (0, 0.0)
     if not args.skip_plain:
         plain = make_amg(1, 0)
+        for key in ("sell_block", "tail_max_rows"):
+            plain.set_tuning(key, amg.get_tuning(key))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         setup_on(plain, S)
@@ -381,6 +421,7 @@ def main():
         "exact_order_cycles": cyc_exact,
         "rel_error_vs_fp64_amg": err,
         "assembly_s": round(t_asm, 4),
+        "autotune": tune,
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)

@@ -81,6 +81,7 @@ SIGNATURES = {
     "fvm_mesh_uniform_field": (_i, [_p, _d, _d, ctypes.c_uint64, _p]),
     "fvm_thermal_assemble": (_i, [_p, _p, _p, _p, _p, _p, ctypes.POINTER(_p)]),
     "fvm_phonon_assemble": (_i, [_p, _p, _d, _p, _p, ctypes.POINTER(_p)]),
+    "fvm_amg_autotune": (_i, [_p, _i, ctypes.c_char_p, _i]),
     "fvm_batch_create": (_i, [ctypes.POINTER(_p), _i]),
     "fvm_batch_destroy": (_i, [_p]),
     "fvm_batch_set_params": (_i, [_p, ctypes.POINTER(AmgParams)]),

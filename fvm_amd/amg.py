@@ -178,6 +178,15 @@ class AMG:
         check(lib().fvm_amg_get_tuning(self._h, key.encode(), ctypes.byref(v)))
         return v.value
 
+    def autotune(self, reps=3):
+        """Time the result-preserving launch shapes on this hierarchy and keep the
+        fastest (fvm_amg_autotune); returns the report (candidates, choice)."""
+        import json
+        self._push_params()
+        buf = ctypes.create_string_buffer(8192)
+        check(lib().fvm_amg_autotune(self._h, int(reps), buf, len(buf)))
+        return json.loads(buf.value.decode())
+
     def setup_stats(self):
         t = ctypes.c_double()
         r = ctypes.c_int()

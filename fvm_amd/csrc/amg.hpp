@@ -78,6 +78,8 @@ struct BatchInfo {
     std::vector<int> depth;              // per system: levels of its hierarchy
     DArr<BatchSys> d_sys;
     DArr<int> d_off0;                    // finest-level offsets on the device
+    std::vector<DArr<int>> d_off;        // every level's offsets on the device
+    DArr<int> sysperm;                   // coloured layout: finest rows grouped by system
     // solve state
     DArr<uint8_t> act;                   // per system: still cycling
     DArr<uint8_t> mask0;                 // per finest row: act of its system
@@ -130,6 +132,9 @@ struct fvm_amg {
     long long tail_max_rows = 40000;
     long long coarsen_small_rows = 16384;
     int coarsen_coop = 1;              // larger levels: all rounds in one cooperative launch
+    int sell_block = 256;              // threads per block of the SELL-64 cycle kernels
+    fvm::BatchInfo *batch = nullptr;   // set while the hierarchy is a batched one
+    std::string tune_report;           // last autotuner result (JSON)
     int coop_blocks = -1;              // co-resident blocks for that launch (-1: not queried)
     // single-workgroup tail of the V-cycle: levels >= tail_level run in one launch
     int tail_level = -1;
@@ -167,4 +172,5 @@ void batch_solve(fvm_amg *h, BatchInfo &B, const double *const *d_b, double *con
 void apply_precond(fvm_amg *h, const double *d_v, double *d_z);   // z = -M^{-1}... see cycle.hip
 void spmv_residual64(fvm_amg *h, const double *d_x, const double *d_b, double *d_r, double *norm1);
 double norm1_device(fvm_amg *h, const double *d_r, int n);
+std::string autotune(fvm_amg *h, int reps);
 }  // namespace fvm

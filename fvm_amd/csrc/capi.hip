@@ -378,8 +378,10 @@ int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value) {
     } else if (k == "coarsen_small_rows") {
         FVM_CHECK(value >= 0, FVM_E_INVALID, "coarsen_small_rows must be >= 0");
         h->coarsen_small_rows = value;
-    } else if (k == "coarsen_coop") {
-        h->coarsen_coop = value != 0;
+    } else if (k == "sell_block") {
+        FVM_CHECK(value == 64 || value == 128 || value == 256 || value == 512, FVM_E_INVALID,
+                  "sell_block must be 64, 128, 256 or 512");
+        h->sell_block = (int)value;
     } else {
         throw fvm::Error(FVM_E_INVALID, "unknown tuning key: " + k);
     }
@@ -392,9 +394,23 @@ int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value) {
     const std::string k(key);
     if (k == "tail_max_rows") *value = h->tail_max_rows;
     else if (k == "coarsen_small_rows") *value = h->coarsen_small_rows;
-    else if (k == "coarsen_coop") *value = h->coarsen_coop;
+    else if (k == "sell_block") *value = h->sell_block;
     else return FVM_E_INVALID;
     return FVM_OK;
+}
+
+int fvm_amg_autotune(fvm_amg *h, int reps, char *report, int cap) {
+    FVM_API_BEGIN
+    FVM_CHECK(h && reps >= 1, FVM_E_INVALID, "null handle or reps < 1");
+    HandleScope g(h);
+    const std::string r = fvm::autotune(h, reps);
+    if (report && cap > 0) {
+        const size_t k = std::min(r.size(), (size_t)cap - 1);
+        memcpy(report, r.data(), k);
+        report[k] = 0;
+    }
+    return (int)r.size();
+    FVM_API_END
 }
 
 int fvm_amg_setup_stats(const fvm_amg *h, double *setup_seconds, int *finest_rounds) {
@@ -407,7 +423,6 @@ int fvm_amg_setup_stats(const fvm_amg *h, double *setup_seconds, int *finest_rou
 // ----------------------------------------------------------------------- batched systems
 static void batch_check_params(const fvm_amg_params *p) {
     check_params(p);
-    FVM_CHECK(p->gsOrdering == FVM_GS_EXACT, FVM_E_INVALID, "batched solve: gsOrdering must be FVM_GS_EXACT");
     FVM_CHECK(p->precision == FVM_PREC_FP64, FVM_E_INVALID, "batched solve: precision must be FVM_PREC_FP64");
     FVM_CHECK(!(p->isSymmetric && p->scaleCorrections), FVM_E_INVALID,
               "batched solve: scaled corrections (isSymmetric && scaleCorrections) are not supported");
@@ -417,6 +432,7 @@ static void batch_invalidate(fvm_batch *b) {
     if (b->B.graph) { (void)hipGraphExecDestroy(b->B.graph); b->B.graph = nullptr; }
     b->B.graph_ok = false;
     b->B.blk_off.release();
+    b->B.sysperm.release();
 }
 
 int fvm_batch_create(fvm_batch **out, int device) {
@@ -455,7 +471,8 @@ int fvm_batch_set_params(fvm_batch *b, const fvm_amg_params *p) {
                                          p->weightRatioThreshold != old.weightRatioThreshold ||
                                          p->maxCoarseLevels != old.maxCoarseLevels ||
                                          p->parallelStopRule != old.parallelStopRule ||
-                                         p->smootherType != old.smootherType);
+                                         p->smootherType != old.smootherType ||
+                                         p->gsOrdering != old.gsOrdering);
     HandleScope g(b->h);
     b->h->prm = *p;
     batch_invalidate(b);   // the captured iteration depends on every cycle parameter

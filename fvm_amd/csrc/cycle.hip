@@ -115,56 +115,87 @@ __global__ void k_csr_jacobi(int n, const int *__restrict__ row, const int *__re
 }
 
 // ------------------------------------------------------------------ SELL-64 (coloured order)
+
+// The hardware hands workgroups to the 8 XCDs of the chip round-robin (block b
+// to XCD b mod 8), and each XCD has its own L2. Renumbering the blocks so that
+// XCD j processes the j-th contiguous eighth of the rows keeps the gathers of a
+// row's neighbours (nearby rows, also in the coloured order) inside one L2.
+// A bijection of [0, gridDim.x) for any grid size.
+__device__ __forceinline__ int xcd_block() {
+    const int G = gridDim.x, b = blockIdx.x;
+    const int j = b & 7, q = G >> 3, r = G & 7;
+    return j * q + (j < r ? j : r) + (b >> 3);
+}
+
+// acc + sum over the SELL-64 entries of one row, in storage order (the row's CSR
+// order). Loads of a group of U entries are issued before their products are
+// added, so a lane keeps U gathers in flight; the additions stay in order.
+template <int U, class V, class X>
+__device__ __forceinline__ X sell_row(X acc, const int *__restrict__ scol, const V *__restrict__ sval, int o,
+                                      int w, int lane, const X *x) {
+    for (int k = 0; k < w; k += U) {
+        int c[U];
+        X v[U], xv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (k + u < w) {
+                const int e = o + (k + u) * 64 + lane;
+                c[u] = __builtin_nontemporal_load(scol + e);
+                v[u] = (X)ldv(sval, e);
+            }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (k + u < w) xv[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (k + u < w) acc += v[u] * xv[u];
+    }
+    return acc;
+}
+constexpr int SELL_U = 8;
+
 template <class V, class D, class X>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(512)
 k_sell_residual(int n, const int *__restrict__ soff, const int *__restrict__ scol,
                 const V *__restrict__ sval, const D *__restrict__ diag, const X *__restrict__ x,
                 const X *__restrict__ b, X *__restrict__ r) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = xcd_block() * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int s = p >> 6, lane = p & 63;
     const int o = soff[s];
     const int w = (soff[s + 1] - o) >> 6;
-    X acc = b[p] + (X)diag[p] * x[p];
-    for (int k = 0; k < w; k++) {
-        const int e = o + k * 64 + lane;
-        acc += (X)ldv(sval, e) * x[scol[e]];
-    }
-    r[p] = acc;
+    const X acc = b[p] + (X)diag[p] * x[p];
+    r[p] = sell_row<SELL_U>(acc, scol, sval, o, w, lane, x);
 }
 
 template <class V, class D, class X>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(512)
 k_sell_gs(int beg, int end, const int *__restrict__ soff, const int *__restrict__ scol,
-          const V *__restrict__ sval, const D *__restrict__ diag, X *x, const X *__restrict__ b) {
-    const int p = beg + blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= end) return;
+          const V *__restrict__ sval, const D *__restrict__ diag, X *x, const X *__restrict__ b,
+          const uint8_t *__restrict__ mask) {
+    // threads start at the slice holding beg: each wavefront covers exactly one slice
+    const int p = (beg & ~63) + xcd_block() * blockDim.x + threadIdx.x;
+    if (p < beg || p >= end) return;
+    if (mask && !mask[p]) return;   // row of a system that has stopped cycling
     const int s = p >> 6, lane = p & 63;
     const int o = soff[s];
     const int w = (soff[s + 1] - o) >> 6;
-    X acc = b[p];
-    for (int k = 0; k < w; k++) {
-        const int e = o + k * 64 + lane;
-        acc += (X)ldv(sval, e) * x[scol[e]];
-    }
+    const X acc = sell_row<SELL_U>(b[p], scol, sval, o, w, lane, (const X *)x);
     x[p] = -acc / (X)diag[p];
 }
 
 template <class V, class D, class X>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(512)
 k_sell_jacobi(int n, const int *__restrict__ soff, const int *__restrict__ scol,
               const V *__restrict__ sval, const D *__restrict__ diag, const X *__restrict__ xo,
-              const X *__restrict__ b, X *__restrict__ xn) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+              const X *__restrict__ b, X *__restrict__ xn, const uint8_t *__restrict__ mask) {
+    const int p = xcd_block() * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    if (mask && !mask[p]) { xn[p] = xo[p]; return; }
     const int s = p >> 6, lane = p & 63;
     const int o = soff[s];
     const int w = (soff[s + 1] - o) >> 6;
-    X acc = b[p];
-    for (int k = 0; k < w; k++) {
-        const int e = o + k * 64 + lane;
-        acc += (X)ldv(sval, e) * xo[scol[e]];
-    }
+    const X acc = sell_row<SELL_U>(b[p], scol, sval, o, w, lane, xo);
     xn[p] = -acc / (X)diag[p];
 }
 
@@ -173,7 +204,7 @@ k_sell_jacobi(int n, const int *__restrict__ soff, const int *__restrict__ scol,
 template <class X>
 __global__ void k_restrict(int nc, const int *__restrict__ c2f_row, const int *__restrict__ c2f_col,
                            const X *__restrict__ r, X *__restrict__ bc) {
-    for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < nc; I += gridDim.x * blockDim.x) {
+    for (int I = xcd_block() * blockDim.x + threadIdx.x; I < nc; I += gridDim.x * blockDim.x) {
         X s = 0;
         for (int q = c2f_row[I]; q < c2f_row[I + 1]; q++) s += r[c2f_col[q]];
         bc[I] = s;
@@ -184,7 +215,7 @@ __global__ void k_restrict(int nc, const int *__restrict__ c2f_row, const int *_
 template <class X>
 __global__ void k_prolong(int n, const int *__restrict__ ci, const X *__restrict__ xc, X *x, int set,
                           const double *scale, const uint8_t *__restrict__ mask) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int i = xcd_block() * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (mask && !mask[i]) continue;
         const int I = ci[i];
         if (scale) {
@@ -343,11 +374,7 @@ __device__ __forceinline__ void tail_rows(const TailLv &L, int beg, int end, con
         const int s = p >> 6, lane = p & 63;
         const int o = L.soff[s];
         const int w = (L.soff[s + 1] - o) >> 6;
-        X acc = b[p];
-        for (int k = 0; k < w; k++) {
-            const int e = o + k * 64 + lane;
-            acc += (X)ldv(val, e) * xin[L.scol[e]];
-        }
+        const X acc = sell_row<SELL_U>(b[p], L.scol, val, o, w, lane, xin);
         xout[p] = -acc / (X)dg[p];
     }
 }
@@ -403,12 +430,8 @@ k_tail(const TailLv *lv, int l0, int last, int npre, int npost, int jacobi, cons
                 const int s = p >> 6, lane = p & 63;
                 const int o = L.soff[s];
                 const int w = (L.soff[s + 1] - o) >> 6;
-                X acc = b[p] + (X)dg[p] * x[p];
-                for (int k = 0; k < w; k++) {
-                    const int e = o + k * 64 + lane;
-                    acc += (X)ldv(val, e) * x[L.scol[e]];
-                }
-                r[p] = acc;
+                const X acc = b[p] + (X)dg[p] * x[p];
+                r[p] = sell_row<SELL_U>(acc, L.scol, val, o, w, lane, x);
             }
             __syncthreads();
             rr = r;
@@ -517,11 +540,12 @@ template <class X>
 void residual(Run &R, int l, const X *x, const X *b, X *r) {
     Level &L = R.h->lev[l];
     hipStream_t s = R.h->stream;
+    const int sb = R.h->sell_block;
     const int n = L.n;
     if (n == 0) return;
     hipEvent_t t0 = (l == 0 && value_type(R, l) == V64) ? timing_begin(R.h) : nullptr;
     FVM_DISPATCH_VALS(R, l, L, X,
-        (k_sell_residual<<<grid_for(n, BS), BS, 0, s>>>(n, L.sell_off.p, L.sell_col.p, val, dg, x, b, r)),
+        (k_sell_residual<<<grid_for(n, sb), sb, 0, s>>>(n, L.sell_off.p, L.sell_col.p, val, dg, x, b, r)),
         (k_csr_residual<<<grid_for(n, BS), BS, 0, s>>>(n, L.row.p, L.col.p, val, dg, x, b, r)));
     if (t0) timing_end(R.h, KT_RESID64, t0);
 }
@@ -530,6 +554,7 @@ template <class X>
 void gs_sweep(Run &R, int l, X *x, const X *b) {
     Level &L = R.h->lev[l];
     hipStream_t s = R.h->stream;
+    const int sb = R.h->sell_block;
     if (L.n == 0) return;
     if (R.fast) {
         for (int dir = 0; dir < 2; dir++) {
@@ -539,7 +564,8 @@ void gs_sweep(Run &R, int l, X *x, const X *b) {
                 const int beg = L.color_ptr[c], end = L.color_ptr[c + 1];
                 if (end <= beg) continue;
                 FVM_DISPATCH_VALS(R, l, L, X,
-                    (k_sell_gs<<<grid_for(end - beg, BS), BS, 0, s>>>(beg, end, L.sell_off.p, L.sell_col.p, val, dg, x, b)),
+                    (k_sell_gs<<<grid_for(end - (beg & ~63), sb), sb, 0, s>>>(beg, end, L.sell_off.p, L.sell_col.p, val, dg, x, b,
+                                                                                l == 0 ? R.mask0 : nullptr)),
                     (void)0);
             }
             if (t0) timing_end(R.h, KT_SWEEP, t0);
@@ -564,13 +590,15 @@ template <class X>
 void jacobi_pass(Run &R, int l) {
     Level &L = R.h->lev[l];
     hipStream_t s = R.h->stream;
+    const int sb = R.h->sell_block;
     const int n = L.n;
     if (n == 0) return;
     DArr<X> &xa = vx_arr<X>(L);
     DArr<X> &ta = vt_arr<X>(L);
     const X *b = vb<X>(L);
     FVM_DISPATCH_VALS(R, l, L, X,
-        (k_sell_jacobi<<<grid_for(n, BS), BS, 0, s>>>(n, L.sell_off.p, L.sell_col.p, val, dg, xa.p, b, ta.p)),
+        (k_sell_jacobi<<<grid_for(n, sb), sb, 0, s>>>(n, L.sell_off.p, L.sell_col.p, val, dg, xa.p, b, ta.p,
+                                                       l == 0 ? R.mask0 : nullptr)),
         (k_csr_jacobi<<<grid_for(n, BS), BS, 0, s>>>(n, L.row.p, L.col.p, val, dg, xa.p, b, ta.p,
                                                       l == 0 ? R.mask0 : nullptr)));
     std::swap(xa, ta);   // xnew becomes x (MultiFieldMatrix.cpp:203-215 copies it back)
@@ -835,6 +863,97 @@ void smooth_device(fvm_amg *h, const double *d_b, double *d_x) {
 
 
 // ===========================================================================
+// Autotuner: launch shapes that cannot change a result (every row keeps its
+// arithmetic; the single-workgroup tail is bit-identical to per-level launches)
+// are timed on the handle's own hierarchy and parameters, and the fastest kept.
+//   sell_block     threads per block of the SELL-64 kernels (row blocks of
+//                  sell_block/64 slices)
+//   tail_max_rows  levels at or below this size run in the single-workgroup tail
+// Each candidate: one warm-up cycle, then the minimum over `reps` cycles from
+// x = 0 on a fixed right-hand side, timed with HIP events on the handle's stream.
+// ===========================================================================
+namespace {
+__global__ void k_tune_rhs(int n, double *b) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        unsigned long long z = (unsigned long long)i * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+        z = (z ^ (z >> 31)) * 0xBF58476D1CE4E5B9ull;
+        z ^= z >> 29;
+        b[i] = (double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+}
+}  // namespace
+
+std::string autotune(fvm_amg *h, int reps) {
+    FVM_CHECK(h->ready, FVM_E_NOT_SETUP, "autotune called before setup");
+    hipStream_t s = h->stream;
+    Level &F = h->lev[0];
+    const int n = F.n;
+    const int keep_block = h->sell_block;
+    const long long keep_tail = h->tail_max_rows;
+    const bool keep_kt = h->ktime;
+    h->ktime = false;
+    std::vector<int> blocks = {128, 256, 512};
+    std::vector<long long> tails = {0, 2500, 10000, 40000};
+    hipEvent_t e0, e1;
+    FVM_HIP(hipEventCreate(&e0));
+    FVM_HIP(hipEventCreate(&e1));
+    std::string rep = "{\"candidates\": [";
+    double best = 1e300;
+    int best_b = keep_block;
+    long long best_t = keep_tail;
+    bool first = true;
+    try {
+        for (int bsz : blocks)
+            for (long long tl : tails) {
+                h->sell_block = bsz;
+                h->tail_max_rows = tl;
+                Run R = make_run(h);
+                if (!R.fast && tl != tails[0]) continue;   // the tail only exists in the coloured layout
+                double tmin = 1e300;
+                for (int r = 0; r <= reps; r++) {
+                    if (n) {
+                        k_tune_rhs<<<grid_for(n, BS), BS, 0, s>>>(n, F.b.p);
+                        k_fill<double><<<grid_for(n, BS, 4096), BS, 0, s>>>(n, F.x.p, 0.0);
+                    }
+                    R.r0_valid = false;
+                    FVM_HIP(hipEventRecord(e0, s));
+                    one_cycle(R);
+                    FVM_HIP(hipEventRecord(e1, s));
+                    FVM_HIP(hipEventSynchronize(e1));
+                    float ms = 0;
+                    FVM_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (r > 0) tmin = std::min(tmin, (double)ms);
+                }
+                char buf[160];
+                snprintf(buf, sizeof buf, "%s{\"sell_block\": %d, \"tail_max_rows\": %lld, \"cycle_ms\": %.4f}",
+                         first ? "" : ", ", bsz, tl, tmin);
+                rep += buf;
+                first = false;
+                if (tmin < best) { best = tmin; best_b = bsz; best_t = tl; }
+            }
+    } catch (...) {
+        h->sell_block = keep_block;
+        h->tail_max_rows = keep_tail;
+        h->ktime = keep_kt;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->sell_block = best_b;
+    h->tail_max_rows = best_t;
+    h->ktime = keep_kt;
+    char buf[200];
+    snprintf(buf, sizeof buf, "], \"chosen\": {\"sell_block\": %d, \"tail_max_rows\": %lld, \"cycle_ms\": %.4f}}",
+             best_b, best_t, best);
+    rep += buf;
+    h->tune_report = rep;
+    return rep;
+}
+
+
+// ===========================================================================
 // Batched solve of independent systems on one block-diagonal hierarchy.
 // Each system follows AMG::solve (AMG.cpp:219-282) on its own: its own initial
 // norm, stop test and cycle count. A system that has met its test stops
@@ -855,34 +974,47 @@ __device__ __forceinline__ int batch_sys(const int *off, int K, int i) {
     return lo;
 }
 
-__global__ void k_batch_gather(int total, int K, const int *off, const double *const *bp, double *const *xp,
-                               double *b, double *x) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+// finest vectors in the hierarchy's layout (perm: coloured position -> row; null: identity)
+__global__ void k_batch_gather(int total, int K, const int *off, const int *perm, const double *const *bp,
+                               double *const *xp, double *b, double *x) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
+        const int i = perm ? perm[p] : p;
         const int k = batch_sys(off, K, i);
         const int li = i - off[k];
-        b[i] = bp[k][li];
-        x[i] = xp[k][li];
+        b[p] = bp[k][li];
+        x[p] = xp[k][li];
     }
 }
 
-__global__ void k_batch_scatter(int total, int K, const int *off, const double *x, double *const *xp) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+__global__ void k_batch_scatter(int total, int K, const int *off, const int *perm, const double *x,
+                                double *const *xp) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
+        const int i = perm ? perm[p] : p;
         const int k = batch_sys(off, K, i);
-        xp[k][i - off[k]] = x[i];
+        xp[k][i - off[k]] = x[p];
+    }
+}
+
+__global__ void k_batch_syskey(int total, int K, const int *off, const int *perm, int *key, int *idx) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
+        key[p] = batch_sys(off, K, perm[p]);
+        idx[p] = p;
     }
 }
 
 // block j of system k covers local rows j*BS + t, stride g_k*BS: the unbatched
 // k_norm1_partial on that system's rows with g_k = its block count
+// sysperm (coloured layout): positions of system k's rows, in its own coloured order
 __global__ void __launch_bounds__(BS) k_seg_norm1_partial(int K, const int *off, const int *blk_off,
-                                                          const double *__restrict__ r, double *part) {
+                                                          const int *sysperm, const double *__restrict__ r,
+                                                          double *part) {
     const int k = batch_sys(blk_off, K, blockIdx.x);
     const int j = blockIdx.x - blk_off[k];
     const int gk = blk_off[k + 1] - blk_off[k];
     const int n = off[k + 1] - off[k];
-    const double *rk = r + off[k];
+    const int o = off[k];
     double s = 0;
-    for (int i = j * BS + threadIdx.x; i < n; i += gk * BS) s += fabs(rk[i]);
+    for (int i = j * BS + threadIdx.x; i < n; i += gk * BS) s += fabs(r[sysperm ? sysperm[o + i] : o + i]);
     s = block_sum<BS>(s);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
@@ -939,9 +1071,10 @@ __global__ void __launch_bounds__(BS) k_batch_check(int K, const double *norm, d
     if (threadIdx.x == 0) { ctr[0] = it; ctr[1] = cnt; }
 }
 
-__global__ void k_batch_mask(int total, int K, const int *off, const uint8_t *act, uint8_t *mask) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x)
-        mask[i] = act[batch_sys(off, K, i)];
+__global__ void k_batch_mask(int total, int K, const int *off, const int *perm, const uint8_t *act,
+                             uint8_t *mask) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x)
+        mask[p] = act[batch_sys(off, K, perm ? perm[p] : p)];
 }
 
 // finest residual and per-system norms into B.norm
@@ -953,7 +1086,8 @@ void batch_norms(Run &R, BatchInfo &B) {
     R64.mixed = false;
     residual<double>(R64, 0, F.x.p, F.b.p, F.r.p);
     if (B.nblk > 0)
-        k_seg_norm1_partial<<<B.nblk, BS, 0, s>>>(B.K, B.d_off0.p, B.blk_off.p, F.r.p, B.partials.p);
+        k_seg_norm1_partial<<<B.nblk, BS, 0, s>>>(B.K, B.d_off0.p, B.blk_off.p, R.fast ? B.sysperm.p : nullptr,
+                                                   F.r.p, B.partials.p);
     k_seg_final<<<B.K, BS, 0, s>>>(B.blk_off.p, B.partials.p, B.norm.p);
     R.r0_valid = true;
 }
@@ -966,7 +1100,9 @@ void batch_iteration(Run &R, BatchInfo &B) {
     k_batch_check<<<1, BS, 0, s>>>(B.K, B.norm.p, h->prm.relativeTolerance, h->prm.absoluteTolerance, B.r0.p,
                                    B.rn.p, B.act.p, B.cyc.p, B.counters.p);
     const int total = h->lev[0].n;
-    if (total) k_batch_mask<<<grid_for(total, BS), BS, 0, s>>>(total, B.K, B.d_off0.p, B.act.p, B.mask0.p);
+    if (total)
+        k_batch_mask<<<grid_for(total, BS), BS, 0, s>>>(total, B.K, B.d_off0.p, R.fast ? h->lev[0].perm.p : nullptr,
+                                                        B.act.p, B.mask0.p);
 }
 
 }  // namespace
@@ -975,7 +1111,7 @@ void batch_solve(fvm_amg *h, BatchInfo &B, const double *const *d_b, double *con
                  double *rnorm_final, int *cycles) {
     FVM_CHECK(h->ready, FVM_E_NOT_SETUP, "batched solve called before setup");
     Run R = make_run(h);
-    FVM_CHECK(!R.fast && !R.mixed, FVM_E_INVALID, "batched solve supports the exact FP64 cycle only");
+    FVM_CHECK(!R.mixed, FVM_E_INVALID, "batched solve supports the FP64 cycle only");
     FVM_CHECK(!(h->prm.isSymmetric && h->prm.scaleCorrections), FVM_E_INVALID,
               "batched solve does not support scaled corrections");
     hipStream_t s = h->stream;
@@ -1001,23 +1137,43 @@ void batch_solve(fvm_amg *h, BatchInfo &B, const double *const *d_b, double *con
     FVM_HIP(hipMemcpyAsync(B.d_bptr.p, d_b, sizeof(double *) * K, hipMemcpyHostToDevice, s));
     FVM_HIP(hipMemcpyAsync(B.d_xptr.p, d_x, sizeof(double *) * K, hipMemcpyHostToDevice, s));
     Level &F = h->lev[0];
+    const int *perm0 = R.fast ? F.perm.p : nullptr;
+    if (R.fast && B.sysperm.n != (size_t)std::max(total, 1)) {
+        // positions of the coloured layout grouped by system (stable: each system's own order)
+        B.sysperm.alloc(std::max(total, 1));
+        if (total) {
+            DArr<int> key(total), key_out(total), idx(total);
+            k_batch_syskey<<<grid_for(total, BS), BS, 0, s>>>(total, K, B.d_off0.p, perm0, key.p, idx.p);
+            int bits = 1;
+            while ((1 << bits) < K) bits++;
+            size_t bytes = 0;
+            FVM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key.p, key_out.p, idx.p, B.sysperm.p, total,
+                                                       0, bits, s));
+            DArr<unsigned char> tmp(bytes + 16);
+            FVM_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, key.p, key_out.p, idx.p, B.sysperm.p, total, 0,
+                                                       bits, s));
+            FVM_HIP(hipStreamSynchronize(s));
+        }
+    }
     if (total)
-        k_batch_gather<<<grid_for(total, BS), BS, 0, s>>>(total, K, B.d_off0.p, B.d_bptr.p, B.d_xptr.p, F.b.p,
-                                                          F.x.p);
+        k_batch_gather<<<grid_for(total, BS), BS, 0, s>>>(total, K, B.d_off0.p, perm0, B.d_bptr.p, B.d_xptr.p,
+                                                          F.b.p, F.x.p);
     R.mask0 = B.mask0.p;
     batch_norms(R, B);
     k_batch_init<<<1, BS, 0, s>>>(K, B.d_off0.p, B.norm.p, h->prm.absoluteTolerance, B.r0.p, B.rn.p, B.act.p,
                                   B.cyc.p, B.counters.p);
-    if (total) k_batch_mask<<<grid_for(total, BS), BS, 0, s>>>(total, K, B.d_off0.p, B.act.p, B.mask0.p);
+    if (total) k_batch_mask<<<grid_for(total, BS), BS, 0, s>>>(total, K, B.d_off0.p, perm0, B.act.p, B.mask0.p);
     FVM_HIP(hipGetLastError());
     FVM_HIP(hipMemcpyAsync(B.h_active, B.counters.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
     FVM_HIP(hipStreamSynchronize(s));
     int active = *B.h_active;
     // One iteration (cycle, residual, norms, stop test, masks) is the same launch
-    // sequence every time: captured once into a graph and replayed.
+    // sequence every time: captured once into a graph and replayed. The first
+    // iteration runs uncaptured, so that one-time preparations (which allocate)
+    // happen outside the capture.
     const bool saved_ktime = h->ktime;
     for (int i = 1; i < h->prm.nMaxIterations && active > 0; i++) {
-        if (B.use_graph) {
+        if (B.use_graph && i > 1) {
             if (!B.graph_ok) {
                 h->ktime = false;
                 hipGraph_t g = nullptr;
@@ -1048,7 +1204,8 @@ void batch_solve(fvm_amg *h, BatchInfo &B, const double *const *d_b, double *con
         FVM_HIP(hipStreamSynchronize(s));
         active = *B.h_active;
     }
-    if (total) k_batch_scatter<<<grid_for(total, BS), BS, 0, s>>>(total, K, B.d_off0.p, F.x.p, B.d_xptr.p);
+    if (total)
+        k_batch_scatter<<<grid_for(total, BS), BS, 0, s>>>(total, K, B.d_off0.p, perm0, F.x.p, B.d_xptr.p);
     FVM_HIP(hipGetLastError());
     if (rnorm0) FVM_HIP(hipMemcpyAsync(rnorm0, B.r0.p, sizeof(double) * K, hipMemcpyDeviceToHost, s));
     if (rnorm_final) FVM_HIP(hipMemcpyAsync(rnorm_final, B.rn.p, sizeof(double) * K, hipMemcpyDeviceToHost, s));
@@ -1066,20 +1223,16 @@ void batch_solve(fvm_amg *h, BatchInfo &B, const double *const *d_b, double *con
 namespace {
 
 template <class V, class D>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(512)
 k_sell_spmv(int n, const int *__restrict__ soff, const int *__restrict__ scol, const V *__restrict__ sval,
             const D *__restrict__ diag, const double *__restrict__ x, double *__restrict__ y) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = xcd_block() * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int s = p >> 6, lane = p & 63;
     const int o = soff[s];
     const int w = (soff[s + 1] - o) >> 6;
-    double acc = (double)diag[p] * x[p];
-    for (int k = 0; k < w; k++) {
-        const int e = o + k * 64 + lane;
-        acc += (double)ldv(sval, e) * x[scol[e]];
-    }
-    y[p] = acc;
+    const double acc = (double)diag[p] * x[p];
+    y[p] = sell_row<SELL_U>(acc, scol, sval, o, w, lane, x);
 }
 
 __global__ void k_csr_spmv(int n, const int *__restrict__ row, const int *__restrict__ col,
@@ -1166,7 +1319,7 @@ void spmv64(Run &R, const double *x, double *y) {
         ~T() { if (e) timing_end(h, KT_RESID64, e); }
     } timer(h);
     if (R.fast)
-        k_sell_spmv<<<grid_for(n, BS), BS, 0, h->stream>>>(n, F.sell_off.p, F.sell_col.p, F.sell_val.p,
+        k_sell_spmv<<<grid_for(n, h->sell_block), h->sell_block, 0, h->stream>>>(n, F.sell_off.p, F.sell_col.p, F.sell_val.p,
                                                            F.pdiag.p, x, y);
     else
         k_csr_spmv<<<grid_for(n, BS), BS, 0, h->stream>>>(n, F.row.p, F.col.p, F.off.p, F.diag.p, x, y);

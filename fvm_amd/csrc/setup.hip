@@ -13,8 +13,6 @@
 // yields exactly the sequential result (see k_coarsen_round).
 #include "amg.hpp"
 
-#include <hip/hip_cooperative_groups.h>
-
 #include <algorithm>
 #include <chrono>
 #include <climits>
@@ -86,19 +84,11 @@ __global__ void k_compact_fill(int n_self, const int *row, const int *col, const
 // (s could still join g and change its count); if not, it retries next round.
 // Boundary rows never decide and are dropped from all dependency sets.
 
-#ifndef FVM_COARSEN_SUB
-#define FVM_COARSEN_SUB 16
-#endif
-constexpr int SUB = FVM_COARSEN_SUB;   // lanes that process one frontier row
-
-__device__ __forceinline__ int sub_lane() { return threadIdx.x & (SUB - 1); }
+constexpr int INIT_SUB = 4;    // lanes per row in k_cnt_init
+constexpr int ROUND_SUB = 64;  // lanes per frontier row in k_coarsen_round (one wavefront)
+constexpr int SMALL_SUB = 16;  // lanes per frontier row in k_coarsen_small
 
 // append x to the queue when want; must be reached by every active lane
-#ifdef FVM_PUSH_PLAIN
-__device__ __forceinline__ void push_agg(bool want, int x, int *q, int *qcnt) {
-    if (want) q[atomicAdd(qcnt, 1)] = x;
-}
-#else
 __device__ __forceinline__ void push_agg(bool want, int x, int *q, int *qcnt) {
     const unsigned long long m = __ballot(want);
     if (!m) return;
@@ -109,14 +99,53 @@ __device__ __forceinline__ void push_agg(bool want, int x, int *q, int *qcnt) {
     base = __shfl(base, leader, 64);
     if (want) q[base + __popcll(m & ((1ull << lane) - 1ull))] = x;
 }
-#endif
+
+// Rows that became ready, buffered per lane and appended to the queue with one
+// atomic per wavefront (the queue tail is a single hot counter).
+constexpr int PB = 4;
+struct PushBuf {
+    int v[PB];
+    int n;
+};
+
+__device__ __forceinline__ void pb_flush(PushBuf &pb, int *q, int *qcnt) {
+    unsigned long long m[PB];
+    int before[PB];
+    int tot = 0;
+#pragma unroll
+    for (int k = 0; k < PB; k++) {
+        m[k] = __ballot(pb.n > k);
+        before[k] = tot;
+        tot += __popcll(m[k]);
+    }
+    if (tot == 0) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)__ballot(1)) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(qcnt, tot);
+    base = __shfl(base, leader, 64);
+    const unsigned long long lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int k = 0; k < PB; k++)
+        if (pb.n > k) q[base + before[k] + __popcll(m[k] & lt)] = pb.v[k];
+    pb.n = 0;
+}
+
+// must be reached by every active lane
+__device__ __forceinline__ void pb_push(PushBuf &pb, bool want, int x, int *q, int *qcnt) {
+    if (__ballot(pb.n == PB)) pb_flush(pb, q, qcnt);
+#pragma unroll
+    for (int k = 0; k < PB; k++)
+        if (want && pb.n == k) pb.v[k] = x;
+    pb.n += want ? 1 : 0;
+}
 
 __global__ void k_cnt_init(int n, const int *row, const int *col, const int *rowT, const int *colT,
                            int sym, const uint8_t *bnd, int *cnt, int *dround, int *ldr, int *memr,
-                           int *mcnt, int *q, int *qcnt, int *remaining) {
+                           int *mcnt, int *q, int *qcnt) {
     const long long gt = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    const int r = (int)(gt / SUB);
-    const int lane = sub_lane();
+    const int r = (int)(gt / INIT_SUB);
+    const int lane = threadIdx.x & (INIT_SUB - 1);
     const bool valid = r < n;
     int k = 0;
     bool b = true;
@@ -124,11 +153,11 @@ __global__ void k_cnt_init(int n, const int *row, const int *col, const int *row
         b = isb(bnd, r);
         if (!b) {
             if (!sym)
-                for (int qq = rowT[r] + lane; qq < rowT[r + 1]; qq += SUB) {
+                for (int qq = rowT[r] + lane; qq < rowT[r + 1]; qq += INIT_SUB) {
                     const int s = colT[qq];
                     k += (s < r && !isb(bnd, s));
                 }
-            for (int p = row[r] + lane; p < row[r + 1]; p += SUB) {
+            for (int p = row[r] + lane; p < row[r + 1]; p += INIT_SUB) {
                 const int c = col[p];
                 if (isb(bnd, c)) continue;
                 if (c < r) {
@@ -142,7 +171,7 @@ __global__ void k_cnt_init(int n, const int *row, const int *col, const int *row
             }
         }
     }
-    for (int o = SUB / 2; o > 0; o >>= 1) k += __shfl_xor(k, o, 64);
+    for (int o = INIT_SUB / 2; o > 0; o >>= 1) k += __shfl_xor(k, o, 64);
     const bool head = valid && lane == 0;
     if (head) {
         cnt[r] = k;
@@ -152,30 +181,33 @@ __global__ void k_cnt_init(int n, const int *row, const int *col, const int *row
         for (int z = 0; z < MEMW; z++) memr[(size_t)r * MEMW + z] = INT_MAX;
     }
     push_agg(head && !b && k == 0, r, q, qcnt);
-    const unsigned long long live = __ballot(head && !b);
-    if (__lane_id() == 0 && live) atomicAdd(remaining, __popcll(live));
 }
 
-// decrement the dependents of d (rows whose D-set contains d)
+// Decrement the dependents of d (rows whose D-set contains d) with the S lanes of
+// a sub-group. Every part runs as a few lane-parallel steps: the C part is
+// flattened over all (c, x) pairs with a prefix sum of |rowT(c)| across lanes,
+// so the atomics of one row are not a chain of per-neighbour steps.
+template <int S>
 __device__ void fan_out(int d, const int *row, const int *col, const int *rowT, const int *colT, int sym,
-                        const uint8_t *bnd, int *cnt, int *q, int *qcnt) {
-    const int lane = sub_lane();
-    // A: r in row(d), r > d (d in rowT(r))
+                        const uint8_t *bnd, int *cnt, PushBuf &pb, int *q, int *qcnt) {
+    const int lane = threadIdx.x & (S - 1);
+    const int rb = row[d], re = row[d + 1];
+    // A: r in row(d), r > d (d in rowT(r)); for a symmetric pattern it is also
+    // the B set, counted twice in cnt: decrement by 2
     const int dec = sym ? 2 : 1;
-    for (int base = row[d]; base < row[d + 1]; base += SUB) {
+    for (int base = rb; base < re; base += S) {
         const int p = base + lane;
         bool want = false;
         int x = -1;
-        if (p < row[d + 1]) {
+        if (p < re) {
             x = col[p];
             if (x > d && !isb(bnd, x)) want = (atomicSub(&cnt[x], dec) == dec);
         }
-        push_agg(want, x, q, qcnt);
+        pb_push(pb, want, x, q, qcnt);
     }
-    // B: r in rowT(d), r > d (d in row(r)); for a symmetric pattern it is the set
-    // of A again, counted twice in cnt: decrement by 2 there
+    // B: r in rowT(d), r > d (d in row(r))
     if (!sym) {
-        for (int base = rowT[d]; base < rowT[d + 1]; base += SUB) {
+        for (int base = rowT[d]; base < rowT[d + 1]; base += S) {
             const int p = base + lane;
             bool want = false;
             int x = -1;
@@ -183,39 +215,62 @@ __device__ void fan_out(int d, const int *row, const int *col, const int *rowT, 
                 x = colT[p];
                 if (x > d && !isb(bnd, x)) want = (atomicSub(&cnt[x], 1) == 1);
             }
-            push_agg(want, x, q, qcnt);
+            pb_push(pb, want, x, q, qcnt);
         }
     }
-    // C: for c in row(d), c > d: r in rowT(c) with d < r < c
-    for (int pc = row[d]; pc < row[d + 1]; pc++) {
-        const int c = col[pc];
-        if (c <= d || isb(bnd, c)) continue;
-        for (int base = rowT[c]; base < rowT[c + 1]; base += SUB) {
-            const int p = base + lane;
+    // C: for c in row(d), c > d: every r in rowT(c) with d < r < c
+    for (int base = rb; base < re; base += S) {
+        const int p = base + lane;
+        int c = -1, tb = 0, m = 0;
+        if (p < re) {
+            c = col[p];
+            if (c > d && !isb(bnd, c)) { tb = rowT[c]; m = rowT[c + 1] - tb; }
+        }
+        int inc = m;   // inclusive prefix over the S lanes
+        for (int o = 1; o < S; o <<= 1) {
+            const int v = __shfl_up(inc, o, S);
+            if (lane >= o) inc += v;
+        }
+        const int total = __shfl(inc, S - 1, S);
+        const int exc = inc - m;
+        for (int k0 = 0; k0 < total; k0 += S) {
+            const int k = k0 + lane;
+            // pair k belongs to the lane j with exc_j <= k < inc_j
+            int lo = 0, hi = S - 1;
+            for (int it = 1; it < S; it <<= 1) {
+                const int mid = (lo + hi) >> 1;
+                const int v = __shfl(inc, mid, S);
+                if (v > k) hi = mid;
+                else lo = mid + 1;
+            }
+            const int cj = __shfl(c, lo, S);
+            const int tj = __shfl(tb, lo, S);
+            const int ej = __shfl(exc, lo, S);
             bool want = false;
             int x = -1;
-            if (p < rowT[c + 1]) {
-                x = colT[p];
-                if (x > d && x < c && !isb(bnd, x)) want = (atomicSub(&cnt[x], 1) == 1);
+            if (k < total) {
+                x = colT[tj + (k - ej)];
+                if (x > d && x < cj && !isb(bnd, x)) want = (atomicSub(&cnt[x], 1) == 1);
             }
-            push_agg(want, x, q, qcnt);
+            pb_push(pb, want, x, q, qcnt);
         }
     }
 }
 
-// Decide one frontier row with the SUB lanes of its sub-group. Returns via the
+// Decide one frontier row with the S lanes of its sub-group. Returns via the
 // queue (fan-out) and the shared state arrays.
+template <int S>
 __device__ void decide_row(int r, int t, double theta, const int *row, const int *col, const int *rowT,
                            const int *colT, int sym, const double *off, const double *diag,
                            const uint8_t *bnd, int *ldr, int *dround, int *mem, int *memr, int *mcnt,
-                           int *cnt, int *q, int *qcnt, int *remaining) {
-    const int lane = sub_lane();
+                           int *cnt, int *q, int *qcnt) {
+    const int lane = threadIdx.x & (S - 1);
     if (dround[r] < t) return;   // already paired by an earlier seed (its fan-out is done)
     // strongest ungrouped (u) and grouped (g) neighbours; ties: first in CSR order
     const double d0 = fabs(diag[r]);
     double mu = -1.0, mg = -1.0;
     int pu = INT_MAX, pg = INT_MAX;
-    for (int p = row[r] + lane; p < row[r + 1]; p += SUB) {
+    for (int p = row[r] + lane; p < row[r + 1]; p += S) {
         const int c = col[p];
         if (isb(bnd, c)) continue;
         const double d1 = fabs(diag[c]);
@@ -226,7 +281,7 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
             if (pg == INT_MAX || w > mg) { mg = w; pg = p; }
         }
     }
-    for (int o = SUB / 2; o > 0; o >>= 1) {
+    for (int o = S / 2; o > 0; o >>= 1) {
         const double mu2 = __shfl_xor(mu, o, 64);
         const int pu2 = __shfl_xor(pu, o, 64);
         if (pu2 != INT_MAX && (pu == INT_MAX || mu2 > mu || (mu2 == mu && pu2 < pu))) { mu = mu2; pu = pu2; }
@@ -250,7 +305,7 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
             if (!(memr[(size_t)g * MEMW + k] < t)) continue;
             count++;
             const int m = mem[(size_t)g * MEMW + k];
-            for (int p = rowT[m] + lane; p < rowT[m + 1]; p += SUB) {
+            for (int p = rowT[m] + lane; p < rowT[m + 1]; p += S) {
                 const int s = colT[p];
                 if (s < r && !isb(bnd, s) && !(dround[s] < t)) blocked = true;
             }
@@ -258,12 +313,15 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
         // every lane must take part in each shuffle (no short-circuit): a lane
         // that skipped it would leave its partner reading an inactive lane
         int bl = blocked ? 1 : 0;
-        for (int o = SUB / 2; o > 0; o >>= 1) bl |= __shfl_xor(bl, o, 64);
+        for (int o = S / 2; o > 0; o >>= 1) bl |= __shfl_xor(bl, o, 64);
         blocked = bl != 0;
         action = blocked ? 3 : ((count > 2 + 2) ? 1 : 2);   // coarseCount > groupSize+2
     }
+    PushBuf pb;
+    pb.n = 0;
     if (action == 3) {
-        push_agg(lane == 0, r, q, qcnt);
+        pb_push(pb, lane == 0, r, q, qcnt);
+        pb_flush(pb, q, qcnt);
         return;
     }
     if (lane == 0) {
@@ -273,44 +331,38 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
             mem[(size_t)r * MEMW + 1] = cu; memr[(size_t)r * MEMW + 1] = t;
             mcnt[r] = 2;
             dround[cu] = t; dround[r] = t;
-            atomicSub(remaining, 2);
         } else if (action == 1) {
             ldr[r] = r;
             mem[(size_t)r * MEMW] = r; memr[(size_t)r * MEMW] = t;
             mcnt[r] = 1;
             dround[r] = t;
-            atomicSub(remaining, 1);
         } else {
             const int k = mcnt[g];
             mem[(size_t)g * MEMW + k] = r; memr[(size_t)g * MEMW + k] = t;
             mcnt[g] = k + 1;
             ldr[r] = g;
             dround[r] = t;
-            atomicSub(remaining, 1);
         }
     }
-    fan_out(r, row, col, rowT, colT, sym, bnd, cnt, q, qcnt);
-    if (action == 0) fan_out(cu, row, col, rowT, colT, sym, bnd, cnt, q, qcnt);
+    fan_out<S>(r, row, col, rowT, colT, sym, bnd, cnt, pb, q, qcnt);
+    if (action == 0) fan_out<S>(cu, row, col, rowT, colT, sym, bnd, cnt, pb, q, qcnt);
+    pb_flush(pb, q, qcnt);
 }
 
-// One round over the frontier (multi-block). Reads frontier t from cur, appends
-// frontier t+1 to nxt; zeroes the counter of round t+2.
+// One round over the frontier (multi-block), one wavefront per frontier row.
+// Reads frontier t from cur, appends frontier t+1 to nxt; zeroes the counter of round t+2.
 __global__ void __launch_bounds__(256)
 k_coarsen_round(int t, const int *row, const int *col, const int *rowT, const int *colT, int sym,
                 const double *off, const double *diag, const uint8_t *bnd, double theta, int *ldr,
                 int *dround, int *mem, int *memr, int *mcnt, int *cnt, const int *cur, const int *cur_cnt,
-                int *nxt, int *nxt_cnt, int *zero_cnt, int *remaining) {
+                int *nxt, int *nxt_cnt, int *zero_cnt) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *zero_cnt = 0;
     const int nf = *cur_cnt;
-    const int per_wave = 64 / SUB;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (int base = wave * per_wave; base < nf; base += nwaves * per_wave) {
-        const int i = base + (__lane_id() / SUB);
-        if (i < nf)
-            decide_row(cur[i], t, theta, row, col, rowT, colT, sym, off, diag, bnd, ldr, dround, mem, memr,
-                       mcnt, cnt, nxt, nxt_cnt, remaining);
-    }
+    for (int i = wave; i < nf; i += nwaves)
+        decide_row<ROUND_SUB>(cur[i], t, theta, row, col, rowT, colT, sym, off, diag, bnd, ldr, dround, mem,
+                              memr, mcnt, cnt, nxt, nxt_cnt);
 }
 
 // All rounds of a small level in one workgroup: rounds are separated by
@@ -320,8 +372,8 @@ __global__ void __launch_bounds__(1024)
 k_coarsen_small(int n, int max_rounds, const int *row, const int *col, const int *rowT, const int *colT,
                 int sym, const double *off, const double *diag, const uint8_t *bnd, double theta, int *ldr,
                 int *dround, int *mem, int *memr, int *mcnt, int *cnt, int *qa, int *qb, int *qcnt,
-                int *remaining, int *rounds_out) {
-    __shared__ int s_nf, s_rem;
+                int *rounds_out) {
+    __shared__ int s_nf;
     int *cur = qa, *nxt = qb;
     int t = 1;
     for (; t <= max_rounds; t++) {
@@ -329,21 +381,20 @@ k_coarsen_small(int n, int max_rounds, const int *row, const int *col, const int
             // the counters are updated by atomics (performed in L2): read and reset them
             // atomically too, a plain load could return a stale L1 copy
             s_nf = atomicAdd(&qcnt[t & 1], 0);
-            s_rem = atomicAdd(remaining, 0);
             atomicExch(&qcnt[(t + 1) & 1], 0);
         }
         __syncthreads();
-        if (s_rem == 0) break;
+        if (s_nf == 0) break;   // every row reachable has decided
         __threadfence();   // see every store of the previous round (device-scope acquire)
         const int nf = s_nf;
-        const int per_wave = 64 / SUB;
+        const int per_wave = 64 / SMALL_SUB;
         const int wave = threadIdx.x >> 6;
         const int nwaves = blockDim.x >> 6;
         for (int base = wave * per_wave; base < nf; base += nwaves * per_wave) {
-            const int i = base + (__lane_id() / SUB);
+            const int i = base + (__lane_id() / SMALL_SUB);
             if (i < nf)
-                decide_row(cur[i], t, theta, row, col, rowT, colT, sym, off, diag, bnd, ldr, dround, mem,
-                           memr, mcnt, cnt, nxt, &qcnt[(t + 1) & 1], remaining);
+                decide_row<SMALL_SUB>(cur[i], t, theta, row, col, rowT, colT, sym, off, diag, bnd, ldr, dround,
+                                      mem, memr, mcnt, cnt, nxt, &qcnt[(t + 1) & 1]);
         }
         __threadfence();   // publish this round's stores before the barrier
         __syncthreads();
@@ -352,39 +403,14 @@ k_coarsen_small(int n, int max_rounds, const int *row, const int *col, const int
     if (threadIdx.x == 0) *rounds_out = t;
 }
 
-// All rounds of a large level in one cooperative launch: the rounds are
-// separated by grid-wide barriers instead of kernel launches. Every block leaves
-// the loop on the same value: the size of the next frontier, which is final at
-// the barrier and not written again until the round after it.
-// Counters: qc[t % 3] = size of frontier t; round t appends to qc[(t+1) % 3] and
-// clears qc[(t+2) % 3] (read for the last time in round t-1).
-__global__ void __launch_bounds__(256)
-k_coarsen_coop(int max_rounds, const int *row, const int *col, const int *rowT, const int *colT, int sym,
-               const double *off, const double *diag, const uint8_t *bnd, double theta, int *ldr, int *dround,
-               int *mem, int *memr, int *mcnt, int *cnt, int *qa, int *qb, int *qc, int *remaining,
-               int *rounds_out) {
-    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
-    int *cur = qa, *nxt = qb;
-    const int per_wave = 64 / SUB;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * blockDim.x) >> 6;
-    int t = 1;
-    for (; t <= max_rounds; t++) {
-        const int nf = atomicAdd(&qc[t % 3], 0);   // performed in L2: never a stale copy
-        if (nf == 0) break;
-        if (blockIdx.x == 0 && threadIdx.x == 0) atomicExch(&qc[(t + 2) % 3], 0);
-        for (int base = wave * per_wave; base < nf; base += nwaves * per_wave) {
-            const int i = base + (__lane_id() / SUB);
-            if (i < nf)
-                decide_row(cur[i], t, theta, row, col, rowT, colT, sym, off, diag, bnd, ldr, dround, mem, memr,
-                           mcnt, cnt, nxt, &qc[(t + 1) % 3], remaining);
-        }
-        __threadfence();   // publish this round's stores
-        grid.sync();
-        __threadfence();   // and see the other blocks' ones
-        int *tmp = cur; cur = nxt; nxt = tmp;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) *rounds_out = t;
+// rows that should have decided but have not (0 after a complete agglomeration)
+__global__ void k_count_undecided(int n, const uint8_t *bnd, const int *dround, int *out) {
+    int c = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        c += (!isb(bnd, i) && dround[i] == INT_MAX);
+    const unsigned long long m = __ballot(c != 0);
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+    if (__lane_id() == 0 && m) atomicAdd(out, c);
 }
 
 // transpose pattern (order inside a row is irrelevant to its users)
@@ -577,12 +603,30 @@ __device__ __forceinline__ unsigned hash32(unsigned x) {
 
 // Jones-Plassmann rounds with first-fit colours; a row is coloured in round t
 // if its (hash, index) priority beats every neighbour not coloured before t.
+// first row of the segment (system of a batch) holding row i; 0 without segments
+__device__ __forceinline__ int seg_base(const int *off, int K, int i) {
+    if (!off) return 0;
+    int lo = 0, hi = K;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return off[lo];
+}
+
+// Jones-Plassmann round: an uncoloured row whose priority beats every uncoloured
+// neighbour's takes the first colour its coloured neighbours leave free.
+// Priorities hash the row's index inside its segment, so each system of a batch
+// gets the colouring it gets alone (neighbours always share the segment).
 __global__ void k_color_round(int t, int n, const int *row, const int *col, const int *rowT,
-                              const int *colT, int *color, int *cround, int *left) {
+                              const int *colT, int *color, int *cround, int *left, const int *seg_off,
+                              int nseg) {
     int done = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (cround[i] < t) continue;
-        const unsigned pi = hash32((unsigned)i);
+        const int base = seg_base(seg_off, nseg, i);
+        const unsigned pi = hash32((unsigned)(i - base));
         bool best = true;
         unsigned long long used[4] = {0, 0, 0, 0};
         for (int pass = 0; pass < 2 && best; pass++) {
@@ -595,7 +639,7 @@ __global__ void k_color_round(int t, int n, const int *row, const int *col, cons
                     const int cj = color[j];
                     if (cj < 256) used[cj >> 6] |= 1ull << (cj & 63);
                 } else {
-                    const unsigned pj = hash32((unsigned)j);
+                    const unsigned pj = hash32((unsigned)(j - base));
                     if (pj > pi || (pj == pi && j > i)) { best = false; break; }
                 }
             }
@@ -813,21 +857,6 @@ void ensure_transpose(fvm_amg *h, Level &L) {
     FVM_HIP(hipGetLastError());
 }
 
-// co-resident 256-thread blocks for k_coarsen_coop on the handle's device (0: no
-// cooperative launch)
-int coop_blocks(fvm_amg *h) {
-    if (h->coop_blocks >= 0) return h->coop_blocks;
-    int coop = 0, ncu = 0, per_cu = 0;
-    h->coop_blocks = 0;
-    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, h->device) != hipSuccess || !coop)
-        return 0;
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_coarsen_coop, 256, 0) != hipSuccess)
-        return 0;
-    h->coop_blocks = std::max(0, std::min(ncu * std::min(per_cu, 4), 2048));
-    return h->coop_blocks;
-}
-
 // Agglomerate level L (reference order); fills L.ci, L.nc and returns the rounds used.
 int coarsen_level(fvm_amg *h, Level &L, double theta) {
     hipStream_t s = h->stream;
@@ -840,47 +869,30 @@ int coarsen_level(fvm_amg *h, Level &L, double theta) {
         lists(2 * (size_t)n + 2), ctr(8);
     FVM_HIP(hipMemsetAsync(ctr.p, 0, ctr.bytes(), s));
     int *qc = ctr.p;             // qc[0..2] rotating frontier counters
-    int *remaining = ctr.p + 3;
     int *rounds_out = ctr.p + 4;
     const uint8_t *bnd = L.bnd.p;
     int *q0 = lists.p, *q1 = lists.p + n;
     // initial counts; the frontier of round 1 goes to q1 (round t reads list t&1)
     {
-        const long long threads = (long long)n * SUB;
+        const long long threads = (long long)n * INIT_SUB;
         const unsigned g = (unsigned)((threads + 255) / 256);
         k_cnt_init<<<g, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, cnt.p, dround.p, ldr.p,
-                                     memr.p, mcnt.p, q1, qc + 1, remaining);
+                                     memr.p, mcnt.p, q1, qc + 1);
         FVM_HIP(hipGetLastError());
     }
     int t = 1;
     const long long cap = 4LL * n + 64;
+    const char *path;
     if (n <= h->coarsen_small_rows) {
+        path = "single-workgroup";
         k_coarsen_small<<<1, 1024, 0, s>>>(n, (int)std::min<long long>(cap, INT_MAX - 1), L.row.p, L.col.p,
                                            rowT, colT, sym, L.off.p, L.diag.p, bnd, theta, ldr.p, dround.p,
-                                           mem.p, memr.p, mcnt.p, cnt.p, q1, q0, qc, remaining, rounds_out);
+                                           mem.p, memr.p, mcnt.p, cnt.p, q1, q0, qc, rounds_out);
         FVM_HIP(hipGetLastError());
-        const int rem = read_scalar(remaining, s);
         t = read_scalar(rounds_out, s);
-        FVM_CHECK(rem == 0, FVM_E_NUMERIC,
-                  "coarsening did not terminate (single-workgroup path, n=" + std::to_string(n) + ", rounds=" +
-                      std::to_string(t) + ", undecided=" + std::to_string(rem) + ")");
-    } else if (h->coarsen_coop && coop_blocks(h) > 0) {
-        // k_cnt_init put frontier 1 in q1 with its size in qc[1]
-        int max_rounds = (int)std::min<long long>(cap, INT_MAX - 1);
-        void *args[] = {&max_rounds, (void *)&L.row.p, (void *)&L.col.p, (void *)&rowT, (void *)&colT,
-                        (void *)&sym, (void *)&L.off.p, (void *)&L.diag.p, (void *)&bnd, (void *)&theta,
-                        (void *)&ldr.p, (void *)&dround.p, (void *)&mem.p, (void *)&memr.p, (void *)&mcnt.p,
-                        (void *)&cnt.p, (void *)&q1, (void *)&q0, (void *)&qc, (void *)&remaining,
-                        (void *)&rounds_out};
-        FVM_HIP(hipLaunchCooperativeKernel((const void *)k_coarsen_coop, dim3(coop_blocks(h)), dim3(256), args, 0,
-                                           s));
-        FVM_HIP(hipGetLastError());
-        const int rem = read_scalar(remaining, s);
-        t = read_scalar(rounds_out, s);
-        FVM_CHECK(rem == 0, FVM_E_NUMERIC,
-                  "coarsening did not terminate (cooperative path, n=" + std::to_string(n) + ", rounds=" +
-                      std::to_string(t) + ", undecided=" + std::to_string(rem) + ")");
     } else {
+        // rounds until the frontier is empty, checked every `batch` launches
+        path = "multi-block";
         const unsigned g_round = 1024;
         const int batch = 32;
         for (;;) {
@@ -888,15 +900,21 @@ int coarsen_level(fvm_amg *h, Level &L, double theta) {
                 k_coarsen_round<<<g_round, 256, 0, s>>>(
                     t, L.row.p, L.col.p, rowT, colT, sym, L.off.p, L.diag.p, bnd, theta, ldr.p, dround.p,
                     mem.p, memr.p, mcnt.p, cnt.p, (t & 1) ? q1 : q0, qc + (t % 3), (t & 1) ? q0 : q1,
-                    qc + ((t + 1) % 3), qc + ((t + 2) % 3), remaining);
+                    qc + ((t + 1) % 3), qc + ((t + 2) % 3));
             }
             FVM_HIP(hipGetLastError());
-            const int rem = read_scalar(remaining, s);
-            if (rem == 0) break;
-            FVM_CHECK(t < cap, FVM_E_NUMERIC,
-                      "coarsening did not terminate (multi-block path, n=" + std::to_string(n) + ", rounds=" +
-                          std::to_string(t) + ", undecided=" + std::to_string(rem) + ")");
+            if (read_scalar(qc + (t % 3), s) == 0) break;   // frontier of round t (the next one)
+            if (t >= cap) break;
         }
+    }
+    {
+        DArr<int> und(1);
+        FVM_HIP(hipMemsetAsync(und.p, 0, sizeof(int), s));
+        k_count_undecided<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, bnd, dround.p, und.p);
+        const int rem = read_scalar(und.p, s);
+        FVM_CHECK(rem == 0, FVM_E_NUMERIC,
+                  std::string("coarsening did not terminate (") + path + " path, n=" + std::to_string(n) +
+                      ", rounds=" + std::to_string(t) + ", undecided=" + std::to_string(rem) + ")");
     }
     // leaders -> contiguous ids in seed order
     DArr<int> flag((size_t)n + 1), gid((size_t)n + 1);
@@ -964,6 +982,7 @@ void build_hierarchy(fvm_amg *h, int n_self, int n_total, const int *d_row, cons
               "the GPU agglomeration supports coarseGroupSize == 2 only");
     h->lev.clear();
     h->ready = false;
+    h->batch = nullptr;
     h->direct_level = -1;
     h->tail_prec = -1;
     h->tail_desc.release();
@@ -1119,6 +1138,13 @@ void prepare_fast(fvm_amg *h) {
         ensure_transpose(h, L);
         const int *rT = L.sym ? nullptr : L.rowT.p;
         const int *cT = L.sym ? nullptr : L.colT.p;
+        // batched hierarchy: colour each system as it is coloured alone
+        const int *seg_off = nullptr;
+        int nseg = 0;
+        if (h->batch && l < (int)h->batch->d_off.size()) {
+            seg_off = h->batch->d_off[l].p;
+            nseg = h->batch->K;
+        }
         DArr<int> color(n), cround(n), left(1);
         FVM_HIP(hipMemsetAsync(color.p, 0, color.bytes(), s));
         k_fill_int<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, cround.p, INT_MAX);
@@ -1127,7 +1153,7 @@ void prepare_fast(fvm_amg *h) {
         for (;;) {
             for (int k = 0; k < 8; k++, t++)
                 k_color_round<<<grid_for(n, BS, 4096), BS, 0, s>>>(t, n, L.row.p, L.col.p, rT, cT,
-                                                                   color.p, cround.p, left.p);
+                                                                   color.p, cround.p, left.p, seg_off, nseg);
             FVM_HIP(hipGetLastError());
             if (read_scalar(left.p, s) == 0) break;
             FVM_CHECK(t < 4 * n + 64, FVM_E_NUMERIC, "colouring did not terminate");
@@ -1351,6 +1377,7 @@ void build_hierarchy_batched(fvm_amg *h, BatchInfo &B) {
     FVM_CHECK(h->prm.coarseGroupSize == 2, FVM_E_INVALID, "the GPU agglomeration supports coarseGroupSize == 2 only");
     h->lev.clear();
     h->ready = false;
+    h->batch = nullptr;
     h->direct_level = -1;
     h->tail_prec = -1;
     h->tail_desc.release();
@@ -1487,10 +1514,19 @@ void build_hierarchy_batched(fvm_amg *h, BatchInfo &B) {
     for (auto &L : h->lev) alloc_vectors(L);
     h->ws.partials.ensure(4096);
     h->ws.scal.ensure(64);
+    B.d_off.clear();
+    for (size_t l = 0; l < B.off.size(); l++) {
+        B.d_off.emplace_back(K + 1);
+        FVM_HIP(hipMemcpyAsync(B.d_off.back().p, B.off[l].data(), sizeof(int) * (K + 1), hipMemcpyHostToDevice, s));
+    }
     FVM_HIP(hipStreamSynchronize(s));
     h->ready = true;
-    if (h->prm.smootherType == FVM_GAUSS_SEIDEL)
+    h->batch = &B;
+    if (h->prm.gsOrdering == FVM_GS_MULTICOLOR) {
+        prepare_fast(h);
+    } else if (h->prm.smootherType == FVM_GAUSS_SEIDEL) {
         for (auto &L : h->lev) prepare_exact_schedule(h, L);
+    }
     FVM_HIP(hipStreamSynchronize(s));
     h->setup_seconds = now_s() - t0;
 }

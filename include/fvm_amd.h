@@ -161,10 +161,19 @@ int fvm_amg_set_kernel_timing(fvm_amg *h, int enable);
 int fvm_amg_kernel_timing(const fvm_amg *h, int kind, long long *launches, double *total_ms);
 /* Performance knobs that never change results (no reference counterpart; set
  * by the autotuner): "tail_max_rows" -- V-cycle levels with at most this many
- * rows run in one single-workgroup launch (0 disables); "coarsen_small_rows" --
+ * rows run in one single-workgroup launch (0 disables); "sell_block" -- threads
+ * per block of the SELL-64 kernels (64/128/256/512); "coarsen_small_rows" --
  * levels with at most this many rows are agglomerated by one workgroup. */
 int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value);
 int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value);
+/* Autotuner (no counterpart in the reference): times every candidate launch
+ * shape that cannot change a result -- "sell_block" (threads per block of the
+ * coloured SELL-64 kernels: 128/256/512) x "tail_max_rows" (0/2500/1e4/4e4)
+ * -- with one warm-up and `reps` timed cycles each on the handle's current
+ * hierarchy and parameters, and keeps the fastest. Writes a JSON report of all
+ * candidates and the choice into report (cap bytes, may be NULL); returns the
+ * report length or an FVM_E_* code. Needs a completed setup. */
+int fvm_amg_autotune(fvm_amg *h, int reps, char *report, int cap);
 /* Setup statistics: seconds spent in setup, number of coarsening rounds of the finest level. */
 int fvm_amg_setup_stats(const fvm_amg *h, double *setup_seconds, int *finest_rounds);
 
@@ -208,7 +217,8 @@ int fvm_fgmres_history(const fvm_fgmres *s, double *hist, int cap);
  * rule, AMG.cpp:149-210) and cycles all systems together, each with its own
  * stop test (AMG.cpp:219-282): per system the result equals that system's own
  * fvm_amg solve with the same parameters. One cycle of all systems is captured
- * into a HIP graph and replayed. Supported: gsOrdering FVM_GS_EXACT, precision
+ * into a HIP graph and replayed. Supported: both Gauss-Seidel orderings
+ * (the coloured one colours each system as it is coloured alone), precision
  * FVM_PREC_FP64, no scaled corrections.
  * All array arguments are host arrays of nsys entries; the pointers they hold
  * are device pointers (CSR as in fvm_amg_setup_device; is_boundary may be NULL,

@@ -2,8 +2,9 @@
 
 Bar: for every system of a batch, the correction delta, the initial and final
 residual 1-norms, the cycle count and the level sizes are bit-identical to
-that system's own fvm_amg solve with the same parameters (exact Gauss-Seidel
-order, FP64), with the captured-graph replay and with plain launches; and each
+that system's own fvm_amg solve with the same parameters (FP64, reference or
+coloured Gauss-Seidel order), with the captured-graph replay and with plain
+launches; and each
 system agrees with the CPU oracle as the unbatched solve does (same cycles,
 same delta within 1e-12 relative). The batch mixes sizes, directions,
 symmetric and non-symmetric patterns, hierarchies of different depths and a
@@ -109,7 +110,8 @@ def _batched(devs, graph, **kw):
 
 
 @pytest.mark.parametrize("kw", [dict(), dict(cycleType=1), dict(cycleType=2, nPreSweeps=1),
-                                dict(smootherType=1, nMaxIterations=60)])
+                                dict(smootherType=1, nMaxIterations=60), dict(gsOrdering=1),
+                                dict(gsOrdering=1, cycleType=1), dict(gsOrdering=1, smootherType=1, nMaxIterations=60)])
 def test_batch_equals_unbatched_bitwise(kw):
     hosts = _batch_inputs()
     devs = [_to_dev(h) for h in hosts]

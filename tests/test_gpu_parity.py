@@ -287,10 +287,10 @@ def test_device_entry_points_match_host():
     assert np.array_equal(x.cpu().numpy(), ls.delta)
 
 
-@pytest.mark.parametrize("small,coop,kat", [(0, 1, 0), (0, 0, 0), (1 << 30, 1, 0), (0, 1, 1), (0, 0, 1)])
-def test_coarsening_paths_agree(small, coop, kat):
-    # cooperative single-launch rounds, one launch per round, and single-workgroup
-    # rounds all give the oracle's maps (symmetric grid and the non-symmetric KAT-226)
+@pytest.mark.parametrize("small,kat", [(0, 0), (1 << 30, 0), (0, 1), (1 << 30, 1)])
+def test_coarsening_paths_agree(small, kat):
+    # one launch per round (one wavefront per row) and single-workgroup rounds
+    # give the oracle's maps (symmetric grid and the non-symmetric KAT-226)
     if kat:
         m = O.load_mm(os.path.join(G, "MatrixMarket226.dat"), os.path.join(G, "rhs226.dat"))
     else:
@@ -298,7 +298,6 @@ def test_coarsening_paths_agree(small, coop, kat):
     oa = oracle_for(m)
     ga = gpu_amg()
     ga.set_tuning("coarsen_small_rows", small)
-    ga.set_tuning("coarsen_coop", coop)
     ls = ls_from(m)
     ga.setup(ls)
     check_hierarchy(ga, oa, m)
@@ -318,3 +317,32 @@ def test_tail_kernel_is_bit_identical(prec, smoother, npre):
         res.append((ls.delta.copy(), ga.history()))
     assert np.array_equal(res[0][0], res[1][0])
     assert np.array_equal(res[0][1], res[1][1])
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_autotune_keeps_results(prec):
+    # every launch shape the autotuner may pick gives the same bits
+    m = matgen.grid3d(20, seed=23, contrast=1.5)
+    ref = gpu_amg(gsOrdering=1, precision=prec, nMaxIterations=15, relativeTolerance=1e-30)
+    ls0 = ls_from(m)
+    ref.solve(ls0)
+    ga = gpu_amg(gsOrdering=1, precision=prec, nMaxIterations=15, relativeTolerance=1e-30)
+    ls1 = ls_from(m)
+    ga.setup(ls1)
+    rep = ga.autotune(reps=2)
+    assert len(rep["candidates"]) == 12
+    ch = rep["chosen"]
+    assert any(c["sell_block"] == ch["sell_block"] and c["tail_max_rows"] == ch["tail_max_rows"]
+               for c in rep["candidates"])
+    assert ga.get_tuning("sell_block") == ch["sell_block"]
+    assert ga.get_tuning("tail_max_rows") == ch["tail_max_rows"]
+    ga.solve(ls1)
+    assert np.array_equal(ls0.delta, ls1.delta)
+    assert np.array_equal(ref.history(), ga.history())
+    # and every block size by hand
+    for bs in (64, 128, 512):
+        gb = gpu_amg(gsOrdering=1, precision=prec, nMaxIterations=15, relativeTolerance=1e-30)
+        gb.set_tuning("sell_block", bs)
+        ls2 = ls_from(m)
+        gb.solve(ls2)
+        assert np.array_equal(ls0.delta, ls2.delta), bs
