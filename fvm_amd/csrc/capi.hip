@@ -378,10 +378,12 @@ int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value) {
     } else if (k == "coarsen_small_rows") {
         FVM_CHECK(value >= 0, FVM_E_INVALID, "coarsen_small_rows must be >= 0");
         h->coarsen_small_rows = value;
-    } else if (k == "sell_block") {
+    } else if (k == "sell_block" || k == "spmv_block") {
         FVM_CHECK(value == 64 || value == 128 || value == 256 || value == 512, FVM_E_INVALID,
-                  "sell_block must be 64, 128, 256 or 512");
-        h->sell_block = (int)value;
+                  k + " must be 64, 128, 256 or 512");
+        (k == "sell_block" ? h->sell_block : h->spmv_block) = (int)value;
+    } else if (k == "sell_xcd" || k == "spmv_xcd") {
+        (k == "sell_xcd" ? h->sell_xcd : h->spmv_xcd) = value != 0;
     } else {
         throw fvm::Error(FVM_E_INVALID, "unknown tuning key: " + k);
     }
@@ -395,6 +397,9 @@ int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value) {
     if (k == "tail_max_rows") *value = h->tail_max_rows;
     else if (k == "coarsen_small_rows") *value = h->coarsen_small_rows;
     else if (k == "sell_block") *value = h->sell_block;
+    else if (k == "spmv_block") *value = h->spmv_block;
+    else if (k == "sell_xcd") *value = h->sell_xcd;
+    else if (k == "spmv_xcd") *value = h->spmv_xcd;
     else return FVM_E_INVALID;
     return FVM_OK;
 }

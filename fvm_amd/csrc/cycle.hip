@@ -158,8 +158,8 @@ template <class V, class D, class X>
 __global__ void __launch_bounds__(512)
 k_sell_residual(int n, const int *__restrict__ soff, const int *__restrict__ scol,
                 const V *__restrict__ sval, const D *__restrict__ diag, const X *__restrict__ x,
-                const X *__restrict__ b, X *__restrict__ r) {
-    const int p = xcd_block() * blockDim.x + threadIdx.x;
+                const X *__restrict__ b, X *__restrict__ r, int xcd) {
+    const int p = (xcd ? xcd_block() : (int)blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int s = p >> 6, lane = p & 63;
     const int o = soff[s];
@@ -172,9 +172,9 @@ template <class V, class D, class X>
 __global__ void __launch_bounds__(512)
 k_sell_gs(int beg, int end, const int *__restrict__ soff, const int *__restrict__ scol,
           const V *__restrict__ sval, const D *__restrict__ diag, X *x, const X *__restrict__ b,
-          const uint8_t *__restrict__ mask) {
+          const uint8_t *__restrict__ mask, int xcd) {
     // threads start at the slice holding beg: each wavefront covers exactly one slice
-    const int p = (beg & ~63) + xcd_block() * blockDim.x + threadIdx.x;
+    const int p = (beg & ~63) + (xcd ? xcd_block() : (int)blockIdx.x) * blockDim.x + threadIdx.x;
     if (p < beg || p >= end) return;
     if (mask && !mask[p]) return;   // row of a system that has stopped cycling
     const int s = p >> 6, lane = p & 63;
@@ -188,8 +188,8 @@ template <class V, class D, class X>
 __global__ void __launch_bounds__(512)
 k_sell_jacobi(int n, const int *__restrict__ soff, const int *__restrict__ scol,
               const V *__restrict__ sval, const D *__restrict__ diag, const X *__restrict__ xo,
-              const X *__restrict__ b, X *__restrict__ xn, const uint8_t *__restrict__ mask) {
-    const int p = xcd_block() * blockDim.x + threadIdx.x;
+              const X *__restrict__ b, X *__restrict__ xn, const uint8_t *__restrict__ mask, int xcd) {
+    const int p = (xcd ? xcd_block() : (int)blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= n) return;
     if (mask && !mask[p]) { xn[p] = xo[p]; return; }
     const int s = p >> 6, lane = p & 63;
@@ -540,12 +540,13 @@ template <class X>
 void residual(Run &R, int l, const X *x, const X *b, X *r) {
     Level &L = R.h->lev[l];
     hipStream_t s = R.h->stream;
-    const int sb = R.h->sell_block;
+    const int sb = R.h->spmv_block;
     const int n = L.n;
     if (n == 0) return;
     hipEvent_t t0 = (l == 0 && value_type(R, l) == V64) ? timing_begin(R.h) : nullptr;
     FVM_DISPATCH_VALS(R, l, L, X,
-        (k_sell_residual<<<grid_for(n, sb), sb, 0, s>>>(n, L.sell_off.p, L.sell_col.p, val, dg, x, b, r)),
+        (k_sell_residual<<<grid_for(n, sb), sb, 0, s>>>(n, L.sell_off.p, L.sell_col.p, val, dg, x, b, r,
+                                                        R.h->spmv_xcd)),
         (k_csr_residual<<<grid_for(n, BS), BS, 0, s>>>(n, L.row.p, L.col.p, val, dg, x, b, r)));
     if (t0) timing_end(R.h, KT_RESID64, t0);
 }
@@ -565,7 +566,8 @@ void gs_sweep(Run &R, int l, X *x, const X *b) {
                 if (end <= beg) continue;
                 FVM_DISPATCH_VALS(R, l, L, X,
                     (k_sell_gs<<<grid_for(end - (beg & ~63), sb), sb, 0, s>>>(beg, end, L.sell_off.p, L.sell_col.p, val, dg, x, b,
-                                                                                l == 0 ? R.mask0 : nullptr)),
+                                                                                l == 0 ? R.mask0 : nullptr,
+                                                                                R.h->sell_xcd)),
                     (void)0);
             }
             if (t0) timing_end(R.h, KT_SWEEP, t0);
@@ -598,7 +600,7 @@ void jacobi_pass(Run &R, int l) {
     const X *b = vb<X>(L);
     FVM_DISPATCH_VALS(R, l, L, X,
         (k_sell_jacobi<<<grid_for(n, sb), sb, 0, s>>>(n, L.sell_off.p, L.sell_col.p, val, dg, xa.p, b, ta.p,
-                                                       l == 0 ? R.mask0 : nullptr)),
+                                                       l == 0 ? R.mask0 : nullptr, R.h->sell_xcd)),
         (k_csr_jacobi<<<grid_for(n, BS), BS, 0, s>>>(n, L.row.p, L.col.p, val, dg, xa.p, b, ta.p,
                                                       l == 0 ? R.mask0 : nullptr)));
     std::swap(xa, ta);   // xnew becomes x (MultiFieldMatrix.cpp:203-215 copies it back)
@@ -893,20 +895,23 @@ std::string autotune(fvm_amg *h, int reps) {
     const bool keep_kt = h->ktime;
     h->ktime = false;
     std::vector<int> blocks = {128, 256, 512};
+    const int keep_sx = h->sell_xcd, keep_px = h->spmv_xcd;
     std::vector<long long> tails = {0, 2500, 10000, 40000};
     hipEvent_t e0, e1;
     FVM_HIP(hipEventCreate(&e0));
     FVM_HIP(hipEventCreate(&e1));
     std::string rep = "{\"candidates\": [";
     double best = 1e300;
-    int best_b = keep_block;
+    int best_b = keep_block, best_x = keep_sx;
     long long best_t = keep_tail;
     bool first = true;
     try {
+        for (int xm = 0; xm < 2; xm++)
         for (int bsz : blocks)
             for (long long tl : tails) {
                 h->sell_block = bsz;
                 h->tail_max_rows = tl;
+                h->sell_xcd = xm;
                 Run R = make_run(h);
                 if (!R.fast && tl != tails[0]) continue;   // the tail only exists in the coloured layout
                 double tmin = 1e300;
@@ -924,29 +929,77 @@ std::string autotune(fvm_amg *h, int reps) {
                     FVM_HIP(hipEventElapsedTime(&ms, e0, e1));
                     if (r > 0) tmin = std::min(tmin, (double)ms);
                 }
-                char buf[160];
-                snprintf(buf, sizeof buf, "%s{\"sell_block\": %d, \"tail_max_rows\": %lld, \"cycle_ms\": %.4f}",
-                         first ? "" : ", ", bsz, tl, tmin);
+                char buf[200];
+                snprintf(buf, sizeof buf,
+                         "%s{\"sell_block\": %d, \"tail_max_rows\": %lld, \"sell_xcd\": %d, \"cycle_ms\": %.4f}",
+                         first ? "" : ", ", bsz, tl, xm, tmin);
                 rep += buf;
                 first = false;
-                if (tmin < best) { best = tmin; best_b = bsz; best_t = tl; }
+                if (tmin < best) { best = tmin; best_b = bsz; best_t = tl; best_x = xm; }
             }
     } catch (...) {
         h->sell_block = keep_block;
         h->tail_max_rows = keep_tail;
+        h->sell_xcd = keep_sx;
         h->ktime = keep_kt;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         throw;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     h->sell_block = best_b;
     h->tail_max_rows = best_t;
+    h->sell_xcd = best_x;
+    // residual / SpMV kernels of the finest level on their own
+    const int keep_spmv = h->spmv_block;
+    int best_s = keep_spmv, best_px = keep_px;
+    double best_sms = 1e300;
+    std::string srep;
+    if (n) {
+        try {
+            Run R = make_run(h);
+            Run R64 = R;
+            R64.mixed = false;
+            for (int xm = 0; xm < 2; xm++)
+            for (int bsz : blocks) {
+                h->spmv_block = bsz;
+                h->spmv_xcd = xm;
+                double tmin = 1e300;
+                for (int r = 0; r <= reps + 2; r++) {
+                    FVM_HIP(hipEventRecord(e0, s));
+                    residual<double>(R64, 0, F.x.p, F.b.p, F.r.p);
+                    FVM_HIP(hipEventRecord(e1, s));
+                    FVM_HIP(hipEventSynchronize(e1));
+                    float ms = 0;
+                    FVM_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (r > 0) tmin = std::min(tmin, (double)ms);
+                }
+                char b2[160];
+                snprintf(b2, sizeof b2, "%s{\"spmv_block\": %d, \"spmv_xcd\": %d, \"residual_ms\": %.4f}",
+                         srep.empty() ? "" : ", ", bsz, xm, tmin);
+                srep += b2;
+                if (tmin < best_sms) { best_sms = tmin; best_s = bsz; best_px = xm; }
+            }
+        } catch (...) {
+            h->spmv_block = keep_spmv;
+            h->spmv_xcd = keep_px;
+            h->ktime = keep_kt;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->spmv_block = best_s;
+    h->spmv_xcd = best_px;
     h->ktime = keep_kt;
-    char buf[200];
-    snprintf(buf, sizeof buf, "], \"chosen\": {\"sell_block\": %d, \"tail_max_rows\": %lld, \"cycle_ms\": %.4f}}",
-             best_b, best_t, best);
+    std::string tail_rep = "], \"spmv_candidates\": [" + srep + "], \"chosen\": ";
+    char buf[400];
+    snprintf(buf, sizeof buf,
+             "{\"sell_block\": %d, \"tail_max_rows\": %lld, \"sell_xcd\": %d, \"cycle_ms\": %.4f, "
+             "\"spmv_block\": %d, \"spmv_xcd\": %d, \"residual_ms\": %.4f}}",
+             best_b, best_t, best_x, best, best_s, best_px, best_sms == 1e300 ? 0.0 : best_sms);
+    rep += tail_rep;
     rep += buf;
     h->tune_report = rep;
     return rep;
@@ -1225,8 +1278,8 @@ namespace {
 template <class V, class D>
 __global__ void __launch_bounds__(512)
 k_sell_spmv(int n, const int *__restrict__ soff, const int *__restrict__ scol, const V *__restrict__ sval,
-            const D *__restrict__ diag, const double *__restrict__ x, double *__restrict__ y) {
-    const int p = xcd_block() * blockDim.x + threadIdx.x;
+            const D *__restrict__ diag, const double *__restrict__ x, double *__restrict__ y, int xcd) {
+    const int p = (xcd ? xcd_block() : (int)blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int s = p >> 6, lane = p & 63;
     const int o = soff[s];
@@ -1319,8 +1372,8 @@ void spmv64(Run &R, const double *x, double *y) {
         ~T() { if (e) timing_end(h, KT_RESID64, e); }
     } timer(h);
     if (R.fast)
-        k_sell_spmv<<<grid_for(n, h->sell_block), h->sell_block, 0, h->stream>>>(n, F.sell_off.p, F.sell_col.p, F.sell_val.p,
-                                                           F.pdiag.p, x, y);
+        k_sell_spmv<<<grid_for(n, h->spmv_block), h->spmv_block, 0, h->stream>>>(n, F.sell_off.p, F.sell_col.p, F.sell_val.p,
+                                                           F.pdiag.p, x, y, h->spmv_xcd);
     else
         k_csr_spmv<<<grid_for(n, BS), BS, 0, h->stream>>>(n, F.row.p, F.col.p, F.off.p, F.diag.p, x, y);
 }

@@ -162,15 +162,20 @@ int fvm_amg_kernel_timing(const fvm_amg *h, int kind, long long *launches, doubl
 /* Performance knobs that never change results (no reference counterpart; set
  * by the autotuner): "tail_max_rows" -- V-cycle levels with at most this many
  * rows run in one single-workgroup launch (0 disables); "sell_block" -- threads
- * per block of the SELL-64 kernels (64/128/256/512); "coarsen_small_rows" --
+ * per block of the SELL-64 smoother kernels, "spmv_block" -- of the SELL-64
+ * residual and SpMV kernels (64/128/256/512 each); "sell_xcd" / "spmv_xcd" --
+ * 1: blocks of those kernels are renumbered so that each of the chip's 8 XCDs
+ * works on a contiguous eighth of the rows, 0: hardware order; "coarsen_small_rows" --
  * levels with at most this many rows are agglomerated by one workgroup. */
 int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value);
 int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value);
 /* Autotuner (no counterpart in the reference): times every candidate launch
  * shape that cannot change a result -- "sell_block" (threads per block of the
- * coloured SELL-64 kernels: 128/256/512) x "tail_max_rows" (0/2500/1e4/4e4)
- * -- with one warm-up and `reps` timed cycles each on the handle's current
- * hierarchy and parameters, and keeps the fastest. Writes a JSON report of all
+ * coloured SELL-64 smoother kernels: 128/256/512) x "tail_max_rows"
+ * (0/2500/1e4/4e4) x "sell_xcd" (0/1) -- with one warm-up and `reps` timed
+ * cycles each on the handle's current hierarchy and parameters, keeps the
+ * fastest, then times the finest-level residual alone for "spmv_block"
+ * (128/256/512) x "spmv_xcd" (0/1). Writes a JSON report of all
  * candidates and the choice into report (cap bytes, may be NULL); returns the
  * report length or an FVM_E_* code. Needs a completed setup. */
 int fvm_amg_autotune(fvm_amg *h, int reps, char *report, int cap);

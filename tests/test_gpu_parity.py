@@ -330,7 +330,7 @@ def test_autotune_keeps_results(prec):
     ls1 = ls_from(m)
     ga.setup(ls1)
     rep = ga.autotune(reps=2)
-    assert len(rep["candidates"]) == 12
+    assert len(rep["candidates"]) == 24 and len(rep["spmv_candidates"]) == 6
     ch = rep["chosen"]
     assert any(c["sell_block"] == ch["sell_block"] and c["tail_max_rows"] == ch["tail_max_rows"]
                for c in rep["candidates"])
@@ -339,10 +339,13 @@ def test_autotune_keeps_results(prec):
     ga.solve(ls1)
     assert np.array_equal(ls0.delta, ls1.delta)
     assert np.array_equal(ref.history(), ga.history())
-    # and every block size by hand
-    for bs in (64, 128, 512):
+    # and every block size and block order by hand
+    for bs, xm in ((64, 0), (128, 1), (512, 0), (256, 1)):
         gb = gpu_amg(gsOrdering=1, precision=prec, nMaxIterations=15, relativeTolerance=1e-30)
         gb.set_tuning("sell_block", bs)
+        gb.set_tuning("spmv_block", bs)
+        gb.set_tuning("sell_xcd", xm)
+        gb.set_tuning("spmv_xcd", 1 - xm)
         ls2 = ls_from(m)
         gb.solve(ls2)
         assert np.array_equal(ls0.delta, ls2.delta), bs
