@@ -5,5 +5,6 @@ C ABI of ``libfvm_amd.so`` (hand-written HIP kernels for gfx950).
 """
 from .linear_system import LinearSystem, read_matrix_market
 from .amg import AMG, FGMRES, BatchAMG
+from .learned import LearnedCorrection
 
 __all__ = ["AMG", "FGMRES", "LinearSystem", "read_matrix_market"]

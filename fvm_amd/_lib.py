@@ -82,6 +82,18 @@ SIGNATURES = {
     "fvm_thermal_assemble": (_i, [_p, _p, _p, _p, _p, _p, ctypes.POINTER(_p)]),
     "fvm_phonon_assemble": (_i, [_p, _p, _d, _p, _p, ctypes.POINTER(_p)]),
     "fvm_amg_autotune": (_i, [_p, _i, ctypes.c_char_p, _i]),
+    "fvm_mpnn_num_params": (_i, []),
+    "fvm_mpnn_create": (_i, [ctypes.POINTER(_p), _i, ctypes.c_uint64]),
+    "fvm_mpnn_destroy": (_i, [_p]),
+    "fvm_mpnn_set_params": (_i, [_p, _p]),
+    "fvm_mpnn_get_params": (_i, [_p, _p]),
+    "fvm_mpnn_set_graph": (_i, [_p, _i, _p, _p, _p, _p]),
+    "fvm_mpnn_forward": (_i, [_p, _p, _p, _i]),
+    "fvm_mpnn_loss_grad": (_i, [_p, _p, _p]),
+    "fvm_mpnn_get_grad": (_i, [_p, _p, _i]),
+    "fvm_mpnn_adam_step": (_i, [_p, ctypes.c_float]),
+    "fvm_fgmres_set_learned": (_i, [_p, _p, _d]),
+    "fvm_fgmres_learned_stats": (_i, [_p, _p, _p]),
     "fvm_batch_create": (_i, [ctypes.POINTER(_p), _i]),
     "fvm_batch_destroy": (_i, [_p]),
     "fvm_batch_set_params": (_i, [_p, ctypes.POINTER(AmgParams)]),

@@ -277,6 +277,18 @@ class FGMRES:
         k = check(lib().fvm_fgmres_history(self._h, _ptr(buf), cap))
         return buf[:k].copy()
 
+    def set_learned(self, net, stall_ratio=0.9):
+        """Correct every preconditioner application with a trained LearnedCorrection
+        (None detaches); the network is switched off for the rest of a solve when the
+        residual estimate falls by less than stall_ratio over 5 iterations."""
+        self._net = net
+        check(lib().fvm_fgmres_set_learned(self._h, None if net is None else net._h, float(stall_ratio)))
+
+    def learned_stats(self):
+        u = ctypes.c_int(); f = ctypes.c_int()
+        check(lib().fvm_fgmres_learned_stats(self._h, ctypes.byref(u), ctypes.byref(f)))
+        return dict(iterations_used=u.value, fallback_at=f.value)
+
 
 class BatchAMG:
     """Many independent systems through one handle (the per-direction solves of

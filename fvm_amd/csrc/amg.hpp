@@ -175,4 +175,14 @@ void apply_precond(fvm_amg *h, const double *d_v, double *d_z);   // z = -M^{-1}
 void spmv_residual64(fvm_amg *h, const double *d_x, const double *d_b, double *d_r, double *norm1);
 double norm1_device(fvm_amg *h, const double *d_r, int n);
 std::string autotune(fvm_amg *h, int reps);
+// learned correction (mpnn.hip)
+void mpnn_bind(fvm_mpnn *m, int n, const int *row, const int *col, const double *off, const double *diag);
+void mpnn_correct(fvm_mpnn *m, hipStream_t st, const int *perm, const double *Az, const double *v, double *z,
+                  int n, int *bad_out);
+struct LearnedHook {
+    fvm_mpnn *m = nullptr;
+    double stall_ratio = 0.9;   // off when the 2-norm estimate falls by less over 5 iterations
+    int *used = nullptr;        // iterations that applied the network
+    int *fallback_at = nullptr; // iteration at which it was switched off (-1: never)
+};
 }  // namespace fvm

@@ -14,7 +14,8 @@ void set_last_error(const std::string &msg) { g_last_error = msg; }
 
 struct KrylovWork;
 void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const double *d_b, double *d_x,
-                  double *rnorm0, double *rnorm_final, int *iterations, std::vector<double> &hist);
+                  double *rnorm0, double *rnorm_final, int *iterations, std::vector<double> &hist,
+                  const LearnedHook &nn);
 
 }  // namespace fvm
 
@@ -77,6 +78,9 @@ struct fvm_fgmres {
     fvm_fgmres_params prm{};
     fvm::KrylovWork *work = nullptr;
     std::vector<double> hist;
+    fvm_mpnn *nn = nullptr;       // learned correction of the preconditioner (borrowed)
+    double stall_ratio = 0.9;
+    int nn_used = 0, nn_fallback_at = -1;
 };
 
 struct fvm_batch {
@@ -89,6 +93,15 @@ struct KrylovWork;
 KrylovWork *new_krylov_work();
 void delete_krylov_work(KrylovWork *w);
 }  // namespace fvm
+
+static fvm::LearnedHook hook_of(fvm_fgmres *s) {
+    fvm::LearnedHook k;
+    k.m = s->nn;
+    k.stall_ratio = s->stall_ratio;
+    k.used = &s->nn_used;
+    k.fallback_at = &s->nn_fallback_at;
+    return k;
+}
 
 extern "C" {
 
@@ -551,6 +564,23 @@ int fvm_batch_setup_stats(const fvm_batch *b, double *setup_seconds, int *total_
     return FVM_OK;
 }
 
+int fvm_fgmres_set_learned(fvm_fgmres *s, fvm_mpnn *m, double stall_ratio) {
+    FVM_API_BEGIN
+    FVM_CHECK(s, FVM_E_INVALID, "null handle");
+    FVM_CHECK(!m || (stall_ratio > 0 && stall_ratio <= 1), FVM_E_INVALID, "stall_ratio must be in (0, 1]");
+    s->nn = m;
+    s->stall_ratio = stall_ratio;
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_fgmres_learned_stats(const fvm_fgmres *s, int *iterations_used, int *fallback_at) {
+    if (!s) return FVM_E_INVALID;
+    if (iterations_used) *iterations_used = s->nn_used;
+    if (fallback_at) *fallback_at = s->nn_fallback_at;
+    return FVM_OK;
+}
+
 // ----------------------------------------------------------------------- FGMRES
 int fvm_fgmres_default_params(fvm_fgmres_params *p) {
     if (!p) return FVM_E_INVALID;
@@ -598,7 +628,7 @@ int fvm_fgmres_solve_device(fvm_fgmres *s, const double *d_b, double *d_x, doubl
     FVM_CHECK(s && s->pc, FVM_E_INVALID, "null solver");
     FVM_CHECK(s->pc->ready, FVM_E_NOT_SETUP, "preconditioner not set up");
     HandleScope g(s->pc);
-    fvm::fgmres_solve(s->pc, *s->work, s->prm, d_b, d_x, rnorm0, rnorm_final, iterations, s->hist);
+    fvm::fgmres_solve(s->pc, *s->work, s->prm, d_b, d_x, rnorm0, rnorm_final, iterations, s->hist, hook_of(s));
     return FVM_OK;
     FVM_API_END
 }
@@ -619,7 +649,7 @@ int fvm_fgmres_solve(fvm_fgmres *s, const double *b, double *x, double *rnorm0, 
         FVM_HIP(hipMemcpyAsync(h->h_stage_x.p, x, sizeof(double) * h->n_total, hipMemcpyHostToDevice, st));
     }
     fvm::fgmres_solve(h, *s->work, s->prm, h->h_stage_b.p, h->h_stage_x.p, rnorm0, rnorm_final, iterations,
-                      s->hist);
+                      s->hist, hook_of(s));
     if (h->n_self > 0) {
         FVM_HIP(hipMemcpyAsync(x, h->h_stage_x.p, sizeof(double) * h->n_self, hipMemcpyDeviceToHost, st));
         FVM_HIP(hipStreamSynchronize(st));

@@ -1444,7 +1444,8 @@ void multi_axpy(Run &R, KrylovWork &W, const double *basis, int k, const std::ve
 }  // namespace
 
 void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const double *d_b, double *d_x,
-                  double *rnorm0, double *rnorm_final, int *iterations, std::vector<double> &hist) {
+                  double *rnorm0, double *rnorm_final, int *iterations, std::vector<double> &hist,
+                  const LearnedHook &nn) {
     Run R = make_run(h);
     Level &F = h->lev[0];
     hipStream_t s = h->stream;
@@ -1475,6 +1476,12 @@ void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const d
     hist.assign(1, r0);
     double rn = r0;
     int total = 0;
+    // learned correction of the preconditioner: on until the residual stalls
+    bool nn_on = nn.m != nullptr && n > 0;
+    if (nn_on) mpnn_bind(nn.m, F.n, F.row.p, F.col.p, F.off.p, F.diag.p);
+    if (nn.used) *nn.used = 0;
+    if (nn.fallback_at) *nn.fallback_at = -1;
+    std::vector<double> est_hist;
     if (!(r0 < p.absoluteTolerance) && n > 0) {
         std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hc, hc2, y;
         for (;;) {
@@ -1488,6 +1495,19 @@ void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const d
                 double *vj = W.V.p + (size_t)j * n;
                 double *zj = W.Z.p + (size_t)j * n;
                 precond(R, vj, zj);
+                if (nn_on) {
+                    // z += NN(A z - v): the network's estimate of the remaining error
+                    spmv64(R, zj, W.w.p);
+                    int bad = 0;
+                    mpnn_correct(nn.m, s, perm, W.w.p, vj, zj, n, &bad);
+                    if (bad) {   // non-finite output: plain preconditioner from here on
+                        nn_on = false;
+                        if (nn.fallback_at) *nn.fallback_at = total;
+                        precond(R, vj, zj);
+                    } else if (nn.used) {
+                        (*nn.used)++;
+                    }
+                }
                 spmv64(R, zj, W.w.p);
                 multidot(R, W, j + 1, W.w.p, hc);
                 multi_axpy(R, W, W.V.p, j + 1, hc, -1.0, W.w.p);
@@ -1516,6 +1536,13 @@ void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const d
                 jend = j + 1;
                 const double est = std::fabs(g[j + 1]);
                 if (p.verbosity > 1) printf("fgmres %d: est2 %g\n", total, est);
+                est_hist.push_back(est);
+                const int win = 5;
+                if (nn_on && (int)est_hist.size() > win &&
+                    est > nn.stall_ratio * est_hist[est_hist.size() - 1 - win]) {
+                    nn_on = false;   // stalled: plain preconditioner for the rest of the solve
+                    if (nn.fallback_at) *nn.fallback_at = total;
+                }
                 if (est <= p.relativeTolerance * r0_2 || est < p.absoluteTolerance || !(hn > 0) ||
                     total >= p.nMaxIterations)
                     break;

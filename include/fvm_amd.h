@@ -248,6 +248,49 @@ int fvm_batch_setup_stats(const fvm_batch *b, double *setup_seconds, int *total_
 
 
 /* ---------------------------------------------------------------------------
+ * Learned correction (no counterpart in the reference): a 3-layer message-passing
+ * network over the matrix graph (hidden width 64, bf16 weights, FP32
+ * accumulation on the matrix cores) that maps a residual r to a correction e
+ * (delta form, A e ~ -r). Edge features are the off-diagonals scaled by the
+ * row's diagonal; node features the Jacobi correction -r/d (scaled by its RMS),
+ * its neighbour average, and the normalized log|d|. Trained on the GPU with the
+ * library's own forward and backward kernels on the loss ||r + A e||^2/||r||^2
+ * (A symmetric). Attached to an FGMRES handle it corrects every preconditioner
+ * application z = M(v) by z += NN(A z - v), and is switched off (plain AMG) for
+ * the rest of a solve when the residual estimate stalls or the output is not
+ * finite.
+ * Parameter vector layout (FP32, fvm_mpnn_num_params() entries): per layer
+ * k = 0..2: Ws_k (64 x fin_k, row-major), Wn_k (64 x fin_k), b_k (64), fin_0 = 16,
+ * fin_1 = fin_2 = 64; then wo (64), bo (1).
+ * ------------------------------------------------------------------------- */
+typedef struct fvm_mpnn fvm_mpnn;
+int fvm_mpnn_num_params(void);
+int fvm_mpnn_create(fvm_mpnn **out, int device, uint64_t seed);
+int fvm_mpnn_destroy(fvm_mpnn *m);
+int fvm_mpnn_set_params(fvm_mpnn *m, const float *theta);     /* host, num_params entries */
+int fvm_mpnn_get_params(const fvm_mpnn *m, float *theta);
+/* graph of a system: device CSR of its self rows (off-diagonals only, diagonal
+ * apart), structurally symmetric; the arrays are borrowed until the next call */
+int fvm_mpnn_set_graph(fvm_mpnn *m, int n, const int32_t *d_row, const int32_t *d_col,
+                       const double *d_off, const double *d_diag);
+/* d_e = NN(d_r) (accumulate: d_e += NN(d_r)); device vectors of n entries */
+int fvm_mpnn_forward(fvm_mpnn *m, const double *d_r, double *d_e, int accumulate);
+/* forward + backward on one residual: adds the gradient of the loss to the
+ * handle's gradient buffer and returns the loss */
+int fvm_mpnn_loss_grad(fvm_mpnn *m, const double *d_r, double *loss);
+/* host copy of the accumulated gradient (may be NULL); reset != 0 clears it */
+int fvm_mpnn_get_grad(fvm_mpnn *m, float *grad, int reset);
+/* one Adam step (beta 0.9/0.999) with the mean of the accumulated gradients */
+int fvm_mpnn_adam_step(fvm_mpnn *m, float lr);
+/* attach (m != NULL) or detach the network to an FGMRES handle; it is switched off
+ * for the rest of a solve when the 2-norm residual estimate falls by less than
+ * the factor stall_ratio over 5 iterations */
+int fvm_fgmres_set_learned(fvm_fgmres *s, fvm_mpnn *m, double stall_ratio);
+/* iterations of the last solve that applied the network, and the iteration at
+ * which it was switched off (-1: never) */
+int fvm_fgmres_learned_stats(const fvm_fgmres *s, int *iterations_used, int *fallback_at);
+
+/* ---------------------------------------------------------------------------
  * Finite-volume assembly of the thermal (diffusion) equation on the device.
  * Reference: ThermalModel linearization (ThermalModel_impl.h:236-398) with
  * MeshMetricsCalculator (MeshMetricsCalculator_impl.h:59-462),
