@@ -249,6 +249,7 @@ def run_profile(args):
         amg.set_tuning("tail_max_rows", args.tail_max_rows)
         amg.set_tuning("sell_xcd", args.sell_xcd)
         amg.set_tuning("spmv_xcd", args.spmv_xcd)
+        amg.set_tuning("cycle_graph", args.cycle_graph)
         setup_on(amg, S)
         x = torch.zeros(n_total, dtype=torch.float64, device="cuda:0")
         from fvm_amd._lib import check, lib
@@ -264,6 +265,171 @@ def run_profile(args):
     dst.copy_(src)
     torch.cuda.synchronize()
     log("calibration copy: %d bytes read, %d bytes written" % (src.numel() * 8, dst.numel() * 8))
+
+
+def thermal_system(torch, N, sigma, block, seed):
+    """Jittered N^3 heat-conduction system assembled on the GPU: lognormal k
+    (per cell for block 1, else constant in block^3 cells), source U[0,1],
+    T=1 at x=0, T=0 at x=1, zero flux elsewhere."""
+    from fvm_amd.thermal import Mesh, assemble
+    mesh = Mesh.hex(N, 0.2, seed)
+    nt = mesh.n_total
+    k = torch.empty(nt, dtype=torch.float64, device="cuda:0")
+    s = torch.empty(nt, dtype=torch.float64, device="cuda:0")
+    x = torch.zeros(nt, dtype=torch.float64, device="cuda:0")
+    mesh.lognormal_field(k, sigma, block, seed + 1)
+    mesh.uniform_field(s, 0.0, 1.0, seed + 2)
+    S = assemble(mesh, k, s, x, [0, 0, 0, 1, 1, 1, 1], [0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0])
+    return mesh, S
+
+
+def level0_graph(torch, amg):
+    """Device CSR of the hierarchy's finest level (self rows, natural order)."""
+    lv = amg.level(0)
+    t = lambda a, dt: torch.tensor(a, dtype=dt, device="cuda:0")
+    return (lv["n"], t(lv["row"], torch.int32), t(lv["col"], torch.int32), t(lv["off"], torch.float64),
+            t(lv["diag"], torch.float64))
+
+
+def residual_of(torch, g, x, b):
+    """r = b + A x on the self rows of graph g = (n, row, col, off, diag)."""
+    n, row, col, off, diag = g
+    rid = torch.repeat_interleave(torch.arange(n, device="cuda:0"), (row[1:] - row[:-1]).long())
+    y = diag * x[:n] + b[:n]
+    return y.index_add_(0, rid, off * x[col.long()])
+
+
+def run_learned(args):
+    """Training pairs from plain AMG on matrices of the 2.1M-cell workload's
+    distribution (at 32^3), training on the GPU, then the ablation on the
+    4M-cell system: FGMRES(30) + mixed AMG with and without the learned correction."""
+    import numpy as np
+    import torch
+    import fvm_amd
+    from fvm_amd import LearnedCorrection
+    rng = np.random.default_rng([SEED, 5])
+    n_mat, per_mat, Nt = args.train_matrices, args.train_pairs // args.train_matrices, 32
+    cycles = [int(c) for c in args.train_cycles.split(",")]
+    log("generating %d residual/error pairs from plain AMG on %d matrices (%d^3, sigma 1.5 per cell)"
+        % (n_mat * per_mat, n_mat, Nt))
+    t0 = time.perf_counter()
+    samples, keep = [], []
+    for j in range(n_mat):
+        mesh, S = thermal_system(torch, Nt, 1.5, 1, SEED + 100 + j)
+        amg = make_amg(1, 0, maxit=10)
+        setup_on(amg, S)
+        g = level0_graph(torch, amg)
+        d = S.device_pointers()
+        zero = torch.zeros(S.n_total, dtype=torch.float64, device="cuda:0")
+        for q in range(per_mat):
+            # exact solution first (white noise), b = -A xs, then one or two cycles of
+            # plain AMG from zero: error e = xs - x and residual r = b + A x (A e = -r)
+            xs = torch.tensor(rng.normal(size=S.n_total), dtype=torch.float64, device="cuda:0")
+            xs[S.n_self:] = 0
+            b = torch.zeros(S.n_total, dtype=torch.float64, device="cuda:0")
+            b[:S.n_self] = -residual_of(torch, g, xs, zero)
+            x = torch.zeros(S.n_total, dtype=torch.float64, device="cuda:0")
+            from fvm_amd._lib import check, lib
+            for _ in range(cycles[q % len(cycles)]):
+                check(lib().fvm_amg_smooth_device(amg._h, ctypes.c_void_p(b.data_ptr()),
+                                                  ctypes.c_void_p(x.data_ptr())))
+            samples.append((g, residual_of(torch, g, x, b).contiguous(), (xs[:S.n_self] - x[:S.n_self]).contiguous()))
+        keep.append((mesh, S, amg))
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t0
+    net = LearnedCorrection(seed=args.seed)
+    t0 = time.perf_counter()
+    curve = net.train(samples, steps=args.train_steps, lr=args.lr, batch=8, log=log)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del samples, keep
+    # ablation on the 4M-cell system (160^3, sigma 2.0 in 8^3 blocks)
+    N = args.n if args.n != 200 else 160
+    log("ablation on %d^3" % N)
+    mesh, S = thermal_system(torch, N, 2.0, 8, SEED + 7)
+    b = S.device_pointers()["b"]
+    amg = make_amg(1, 1)
+    setup_on(amg, S)
+    amg.autotune(reps=2)
+    # held-out pairs on the ablation system itself (plain FP64 AMG, one cycle):
+    # energy-norm error after the cycle, and after the cycle plus the correction
+    g = level0_graph(torch, amg)
+    plain = make_amg(1, 0, maxit=10)
+    setup_on(plain, S)
+    zero = torch.zeros(S.n_total, dtype=torch.float64, device="cuda:0")
+    net.set_graph(*g)
+    ratios = []
+    for q in range(4):
+        xs = torch.tensor(rng.normal(size=S.n_total), dtype=torch.float64, device="cuda:0")
+        xs[S.n_self:] = 0
+        bq = torch.zeros(S.n_total, dtype=torch.float64, device="cuda:0")
+        bq[:S.n_self] = -residual_of(torch, g, xs, zero)
+        x = torch.zeros(S.n_total, dtype=torch.float64, device="cuda:0")
+        from fvm_amd._lib import check, lib
+        check(lib().fvm_amg_smooth_device(plain._h, ctypes.c_void_p(bq.data_ptr()), ctypes.c_void_p(x.data_ptr())))
+        r = residual_of(torch, g, x, bq).contiguous()
+        e = (xs[:S.n_self] - x[:S.n_self]).contiguous()
+        en = lambda d: float(-(d @ residual_of(torch, g, d, torch.zeros_like(d))))
+        corr = torch.zeros(S.n_self, dtype=torch.float64, device="cuda:0")
+        net.forward(r, corr)
+        ratios.append((en(e) / en(xs[:S.n_self].contiguous()), en((e - corr).contiguous()) / en(e)))
+    heldout = dict(energy_after_cycle=float(np.mean([a for a, _ in ratios])),
+                   energy_after_cycle_and_correction_rel=float(np.mean([b_ for _, b_ in ratios])))
+    log("held-out pairs on %d^3: %s" % (N, heldout))
+    # stationary iteration x <- x + M(b + A x) with M = plain AMG cycle (+ correction)
+    stat = {}
+    bfull = torch.zeros(S.n_total, dtype=torch.float64, device="cuda:0")
+    bfull.copy_(torch.as_tensor(S.to_host()["b"], device="cuda:0"))
+    for label, use in (("without", False), ("with", True)):
+        x = torch.zeros(S.n_total, dtype=torch.float64, device="cuda:0")
+        r0 = float(residual_of(torch, g, x, bfull).abs().sum())
+        cyc, ratio = 0, 1.0
+        torch.cuda.synchronize(); t0 = time.perf_counter()
+        while cyc < args.stationary_max and ratio >= 1e-8:
+            check(lib().fvm_amg_smooth_device(plain._h, ctypes.c_void_p(bfull.data_ptr()), ctypes.c_void_p(x.data_ptr())))
+            if use:
+                r = residual_of(torch, g, x, bfull).contiguous()
+                xv = x[:S.n_self]
+                net.forward(r, xv, accumulate=True) if xv.is_contiguous() else None
+            cyc += 1
+            ratio = float(residual_of(torch, g, x, bfull).abs().sum()) / r0
+        torch.cuda.synchronize()
+        stat[label] = dict(cycles=cyc, ratio=ratio, seconds=round(time.perf_counter() - t0, 3))
+        log("stationary AMG %s correction: %s" % (label, stat[label]))
+    del plain
+    res = {}
+    for label, use in (("without", False), ("with", True)):
+        fg = fvm_amd.FGMRES(amg)
+        fg.restart = 30
+        fg.relativeTolerance = 1e-8
+        fg.nMaxIterations = 2000
+        if use:
+            fg.set_learned(net, args.stall)
+        times = []
+        for k in range(args.steps + 1):
+            x = torch.zeros(S.n_total, dtype=torch.float64, device="cuda:0")
+            torch.cuda.synchronize(); t0 = time.perf_counter()
+            r0, rf, it = fg.solve_device(b, x)
+            torch.cuda.synchronize(); times.append(time.perf_counter() - t0)
+        res[label] = dict(iterations=it, solve_s=round(min(times[1:]), 4), ratio=rf / r0,
+                          **(fg.learned_stats() if use else {}))
+        log("%s learned correction: %s" % (label, res[label]))
+    out = {
+        "metric": "FGMRES(30)+mixed AMG solve time to 1e-8 with vs without the learned correction",
+        "value": res["with"]["solve_s"], "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(1e3 * res["with"]["solve_s"], 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 outer, f32/bf16 V-cycle, bf16 network on MFMA (f32 accumulate)",
+        "data": "synthetic (seeded meshes and fields)",
+        "config": {"workload": "%d^3 jittered hex (n=%d), lognormal k sigma=2.0 in 8^3 blocks; network trained on "
+                               "%d residual/error pairs of plain AMG (after %s cycles) on %d matrices at 32^3 (sigma 1.5 per cell), "
+                               "energy-norm loss" % (N, S.n_self, n_mat * per_mat, args.train_cycles, n_mat)},
+        "without": res["without"], "with": res["with"], "heldout": heldout, "stationary": stat,
+        "gain": round(res["without"]["solve_s"] / res["with"]["solve_s"], 3),
+        "train": {"pairs": n_mat * per_mat, "steps": args.train_steps, "lr": args.lr,
+                  "loss_first": float(np.mean(curve[:10])), "loss_last": float(np.mean(curve[-10:])),
+                  "generate_s": round(t_gen, 2), "train_s": round(t_train, 2)},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -282,6 +448,18 @@ def main():
     ap.add_argument("--tail-max-rows", type=int, default=40000, help="--profile: single-workgroup tail size")
     ap.add_argument("--sell-xcd", type=int, default=1, help="--profile: XCD-contiguous smoother blocks")
     ap.add_argument("--spmv-xcd", type=int, default=0, help="--profile: XCD-contiguous residual/SpMV blocks")
+    ap.add_argument("--cycle-graph", type=int, default=1, help="--profile: replay levels >= 1 of the cycle from a graph")
+    ap.add_argument("--learned", action="store_true",
+                    help="instead of the 8M-cell solve: train the learned correction and run its ablation")
+    ap.add_argument("--train-matrices", type=int, default=20)
+    ap.add_argument("--train-pairs", type=int, default=2000)
+    ap.add_argument("--train-steps", type=int, default=600)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--stall", type=float, default=0.9)
+    ap.add_argument("--stationary-max", type=int, default=150)
+    ap.add_argument("--train-cycles", default="1,2",
+                    help="plain AMG cycles before a training pair is taken (cycled through)")
+    ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--batch", type=int, default=0,
                     help="instead of the 8M-cell solve: time a sweep of this many phonon direction systems")
     args = ap.parse_args()
@@ -294,6 +472,9 @@ def main():
         return
     if args.profile:
         run_profile(args)
+        return
+    if args.learned:
+        run_learned(args)
         return
     import torch
     import fvm_amd
@@ -348,7 +529,7 @@ def main():
     kp_s = (0, 0.0)
     if not args.skip_plain:
         plain = make_amg(1, 0)
-        for key in ("sell_block", "tail_max_rows", "spmv_block", "sell_xcd", "spmv_xcd"):
+        for key in ("sell_block", "tail_max_rows", "spmv_block", "sell_xcd", "spmv_xcd", "cycle_graph"):
             plain.set_tuning(key, amg.get_tuning(key))
         torch.cuda.synchronize()
         t0 = time.perf_counter()

@@ -90,6 +90,7 @@ SIGNATURES = {
     "fvm_mpnn_set_graph": (_i, [_p, _i, _p, _p, _p, _p]),
     "fvm_mpnn_forward": (_i, [_p, _p, _p, _i]),
     "fvm_mpnn_loss_grad": (_i, [_p, _p, _p]),
+    "fvm_mpnn_loss_grad_pair": (_i, [_p, _p, _p, _p]),
     "fvm_mpnn_get_grad": (_i, [_p, _p, _i]),
     "fvm_mpnn_adam_step": (_i, [_p, ctypes.c_float]),
     "fvm_fgmres_set_learned": (_i, [_p, _p, _d]),

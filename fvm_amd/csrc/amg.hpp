@@ -136,6 +136,9 @@ struct fvm_amg {
     int spmv_block = 256;              // threads per block of the SELL-64 residual / SpMV kernels
     int sell_xcd = 1, spmv_xcd = 0;    // XCD-contiguous block order for those kernel classes
     fvm::BatchInfo *batch = nullptr;   // set while the hierarchy is a batched one
+    // levels >= 1 of the coloured V-cycle replayed from a captured graph (per precision)
+    int cycle_graph = 1;
+    hipGraphExec_t cg_exec[2] = {nullptr, nullptr};
     std::string tune_report;           // last autotuner result (JSON)
     int coop_blocks = -1;              // co-resident blocks for that launch (-1: not queried)
     // single-workgroup tail of the V-cycle: levels >= tail_level run in one launch
@@ -152,6 +155,10 @@ struct fvm_amg {
 };
 
 namespace fvm {
+inline void invalidate_cycle_graphs(fvm_amg *h) {
+    for (auto &g : h->cg_exec)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
 // kernel timing kinds
 enum { KT_RESID64 = 0, KT_SWEEP = 1 };
 hipEvent_t timing_begin(fvm_amg *h);

@@ -665,7 +665,7 @@ int fvm_mesh_create_hex(fvm_mesh **out, int device, int n, double jitter, uint64
     fvm_mesh *m = new fvm_mesh();
     try {
         m->device = device;
-        FVM_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+        FVM_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamDefault));
         m->hexN = n;
         m->n_cells = n * n * n;
         m->n_ghost = 6 * n * n;
@@ -714,7 +714,7 @@ int fvm_mesh_create(fvm_mesh **out, int device, int n_nodes, const double *nodes
     fvm_mesh *m = new fvm_mesh();
     try {
         m->device = device;
-        FVM_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+        FVM_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamDefault));
         m->n_cells = n_cells; m->n_ghost = n_ghost; m->n_faces = n_faces; m->n_nodes = n_nodes;
         m->n_groups = n_groups;
         m->gptr.assign(group_ptr, group_ptr + n_groups + 1);

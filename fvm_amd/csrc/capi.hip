@@ -141,7 +141,10 @@ int fvm_amg_create(fvm_amg **out, int device) {
     fvm_amg *h = new fvm_amg();
     h->device = device;
     fvm_amg_default_params(&h->prm);
-    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    // a blocking stream: work queued on the legacy default stream (where PyTorch and
+    // other callers produce the device arrays they pass in) completes before the
+    // handle's work starts
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamDefault);
     if (e != hipSuccess) {
         delete h;
         throw fvm::Error(FVM_E_HIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
@@ -157,6 +160,7 @@ int fvm_amg_destroy(fvm_amg *h) {
     {
         HandleScope g(h);
         (void)hipStreamSynchronize(h->stream);
+        fvm::invalidate_cycle_graphs(h);
         h->lev.clear();
         h->ws = fvm::Workspace();
         h->dense_inv.release();
@@ -182,6 +186,10 @@ int fvm_amg_set_params(fvm_amg *h, const fvm_amg_params *p) {
                                       p->maxCoarseLevels != h->prm.maxCoarseLevels ||
                                       p->parallelStopRule != h->prm.parallelStopRule);
     h->prm = *p;
+    {
+        HandleScope g(h);
+        fvm::invalidate_cycle_graphs(h);   // captured cycles hold the old sweep counts and kernels
+    }
     if (rebuild) {
         // hierarchy parameters changed: the next solve needs a new setup (AMG rebuilds per solve)
         HandleScope g(h);
@@ -397,9 +405,13 @@ int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value) {
         (k == "sell_block" ? h->sell_block : h->spmv_block) = (int)value;
     } else if (k == "sell_xcd" || k == "spmv_xcd") {
         (k == "sell_xcd" ? h->sell_xcd : h->spmv_xcd) = value != 0;
+    } else if (k == "cycle_graph") {
+        h->cycle_graph = value != 0;
     } else {
         throw fvm::Error(FVM_E_INVALID, "unknown tuning key: " + k);
     }
+    HandleScope g(h);
+    fvm::invalidate_cycle_graphs(h);
     return FVM_OK;
     FVM_API_END
 }
@@ -413,6 +425,7 @@ int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value) {
     else if (k == "spmv_block") *value = h->spmv_block;
     else if (k == "sell_xcd") *value = h->sell_xcd;
     else if (k == "spmv_xcd") *value = h->spmv_xcd;
+    else if (k == "cycle_graph") *value = h->cycle_graph;
     else return FVM_E_INVALID;
     return FVM_OK;
 }

@@ -498,6 +498,7 @@ struct Run {
     int last;       // last level of the recursion
     bool r0_valid;  // level-0 residual holds b + A x for the current x
     const uint8_t *mask0 = nullptr;   // batched solve: finest rows still cycling
+    bool allow_graph = true;          // false while capturing or inside a captured batch iteration
 };
 
 inline int value_type(const Run &R, int l) {
@@ -676,10 +677,49 @@ void build_tail_desc(Run &R) {
 }
 
 template <class X>
+void cycle(Run &R, int type, int l, bool x_zero);
+
+// Levels >= 1 of a coloured V-cycle replayed from a HIP graph: the launch
+// sequence below the finest level is the same every cycle (fixed buffers), so it
+// is captured once per precision and replayed. Same kernels in the same order:
+// results are bit-identical to the launched cycle.
+template <class X>
+void graph_cycle(Run &R, int type, int l, bool x_zero) {
+    fvm_amg *h = R.h;
+    hipStream_t s = h->stream;
+    const int idx = R.mixed ? 1 : 0;
+    if (!h->cg_exec[idx]) {
+        if (h->tail_level >= 0) build_tail_desc<X>(R);   // allocates: outside the capture
+        hipGraph_t g = nullptr;
+        FVM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        Run Rc = R;
+        Rc.allow_graph = false;
+        try {
+            cycle<X>(Rc, type, l, x_zero);
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        FVM_HIP(hipStreamEndCapture(s, &g));
+        const hipError_t e = hipGraphInstantiate(&h->cg_exec[idx], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) h->cg_exec[idx] = nullptr;
+        FVM_HIP(e);
+    }
+    FVM_HIP(hipGraphLaunch(h->cg_exec[idx], s));
+}
+
+template <class X>
 void cycle(Run &R, int type, int l, bool x_zero) {
     fvm_amg *h = R.h;
     Level &L = h->lev[l];
     hipStream_t s = h->stream;
+    if (R.allow_graph && h->cycle_graph && R.fast && l == 1 && type == FVM_V_CYCLE && x_zero && l <= R.last &&
+        !(h->prm.isSymmetric && h->prm.scaleCorrections)) {
+        graph_cycle<X>(R, type, l, x_zero);
+        return;
+    }
     if (R.fast && l == h->tail_level && type == FVM_V_CYCLE && x_zero && l <= R.last &&
         !(h->prm.isSymmetric && h->prm.scaleCorrections)) {
         build_tail_desc<X>(R);
@@ -871,6 +911,7 @@ void smooth_device(fvm_amg *h, const double *d_b, double *d_x) {
 //   sell_block     threads per block of the SELL-64 kernels (row blocks of
 //                  sell_block/64 slices)
 //   tail_max_rows  levels at or below this size run in the single-workgroup tail
+//   cycle_graph    levels >= 1 of the coloured V-cycle replayed from a captured graph
 // Each candidate: one warm-up cycle, then the minimum over `reps` cycles from
 // x = 0 on a fixed right-hand side, timed with HIP events on the handle's stream.
 // ===========================================================================
@@ -895,25 +936,29 @@ std::string autotune(fvm_amg *h, int reps) {
     const bool keep_kt = h->ktime;
     h->ktime = false;
     std::vector<int> blocks = {128, 256, 512};
-    const int keep_sx = h->sell_xcd, keep_px = h->spmv_xcd;
+    const int keep_sx = h->sell_xcd, keep_px = h->spmv_xcd, keep_cg = h->cycle_graph;
     std::vector<long long> tails = {0, 2500, 10000, 40000};
     hipEvent_t e0, e1;
     FVM_HIP(hipEventCreate(&e0));
     FVM_HIP(hipEventCreate(&e1));
     std::string rep = "{\"candidates\": [";
     double best = 1e300;
-    int best_b = keep_block, best_x = keep_sx;
+    int best_b = keep_block, best_x = keep_sx, best_g = keep_cg;
     long long best_t = keep_tail;
     bool first = true;
     try {
+        for (int cg = 0; cg < 2; cg++)
         for (int xm = 0; xm < 2; xm++)
         for (int bsz : blocks)
             for (long long tl : tails) {
+                h->cycle_graph = cg;
                 h->sell_block = bsz;
                 h->tail_max_rows = tl;
                 h->sell_xcd = xm;
+                invalidate_cycle_graphs(h);
                 Run R = make_run(h);
-                if (!R.fast && tl != tails[0]) continue;   // the tail only exists in the coloured layout
+                // the tail and the captured cycle only exist in the coloured layout
+                if (!R.fast && (tl != tails[0] || cg)) continue;
                 double tmin = 1e300;
                 for (int r = 0; r <= reps; r++) {
                     if (n) {
@@ -929,18 +974,21 @@ std::string autotune(fvm_amg *h, int reps) {
                     FVM_HIP(hipEventElapsedTime(&ms, e0, e1));
                     if (r > 0) tmin = std::min(tmin, (double)ms);
                 }
-                char buf[200];
+                char buf[240];
                 snprintf(buf, sizeof buf,
-                         "%s{\"sell_block\": %d, \"tail_max_rows\": %lld, \"sell_xcd\": %d, \"cycle_ms\": %.4f}",
-                         first ? "" : ", ", bsz, tl, xm, tmin);
+                         "%s{\"sell_block\": %d, \"tail_max_rows\": %lld, \"sell_xcd\": %d, \"cycle_graph\": %d, "
+                         "\"cycle_ms\": %.4f}",
+                         first ? "" : ", ", bsz, tl, xm, cg, tmin);
                 rep += buf;
                 first = false;
-                if (tmin < best) { best = tmin; best_b = bsz; best_t = tl; best_x = xm; }
+                if (tmin < best) { best = tmin; best_b = bsz; best_t = tl; best_x = xm; best_g = cg; }
             }
     } catch (...) {
         h->sell_block = keep_block;
         h->tail_max_rows = keep_tail;
         h->sell_xcd = keep_sx;
+        h->cycle_graph = keep_cg;
+        invalidate_cycle_graphs(h);
         h->ktime = keep_kt;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -949,6 +997,8 @@ std::string autotune(fvm_amg *h, int reps) {
     h->sell_block = best_b;
     h->tail_max_rows = best_t;
     h->sell_xcd = best_x;
+    h->cycle_graph = best_g;
+    invalidate_cycle_graphs(h);
     // residual / SpMV kernels of the finest level on their own
     const int keep_spmv = h->spmv_block;
     int best_s = keep_spmv, best_px = keep_px;
@@ -996,9 +1046,9 @@ std::string autotune(fvm_amg *h, int reps) {
     std::string tail_rep = "], \"spmv_candidates\": [" + srep + "], \"chosen\": ";
     char buf[400];
     snprintf(buf, sizeof buf,
-             "{\"sell_block\": %d, \"tail_max_rows\": %lld, \"sell_xcd\": %d, \"cycle_ms\": %.4f, "
-             "\"spmv_block\": %d, \"spmv_xcd\": %d, \"residual_ms\": %.4f}}",
-             best_b, best_t, best_x, best, best_s, best_px, best_sms == 1e300 ? 0.0 : best_sms);
+             "{\"sell_block\": %d, \"tail_max_rows\": %lld, \"sell_xcd\": %d, \"cycle_graph\": %d, "
+             "\"cycle_ms\": %.4f, \"spmv_block\": %d, \"spmv_xcd\": %d, \"residual_ms\": %.4f}}",
+             best_b, best_t, best_x, best_g, best, best_s, best_px, best_sms == 1e300 ? 0.0 : best_sms);
     rep += tail_rep;
     rep += buf;
     h->tune_report = rep;
@@ -1164,6 +1214,7 @@ void batch_solve(fvm_amg *h, BatchInfo &B, const double *const *d_b, double *con
                  double *rnorm_final, int *cycles) {
     FVM_CHECK(h->ready, FVM_E_NOT_SETUP, "batched solve called before setup");
     Run R = make_run(h);
+    R.allow_graph = false;   // the batched iteration is captured as a whole
     FVM_CHECK(!R.mixed, FVM_E_INVALID, "batched solve supports the FP64 cycle only");
     FVM_CHECK(!(h->prm.isSymmetric && h->prm.scaleCorrections), FVM_E_INVALID,
               "batched solve does not support scaled corrections");

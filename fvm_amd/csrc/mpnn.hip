@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(BSZ) k_mp_layer(int n, const int *row, const i
             for (int j = 0; j < FQ; j++) {
                 const uint16_t a = f2bf(acc[j]);
                 Xw[nl * LDR + FIN + q * FQ + j] = a;
-                Aout[(size_t)i * FIN + q * FQ + j] = a;
+                if (Aout) Aout[(size_t)i * FIN + q * FQ + j] = a;   // kept for the backward pass
             }
         } else {
 #pragma unroll
@@ -244,13 +244,18 @@ __global__ void k_mp_readout(int n, const uint16_t *h3, const float *wo, const f
 }
 
 // ------------------------------------------------------------------ loss and backward
+// y = add + alpha A x (add may be null)
 __global__ void k_mp_spmv(int n, const int *row, const int *col, const double *off, const double *diag,
-                          const double *x, const double *add, double *y) {
+                          const double *x, const double *add, double alpha, double *y) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         double s = diag[i] * x[i];
         for (int p = row[i]; p < row[i + 1]; p++) s += off[p] * x[col[p]];
-        y[i] = add ? add[i] + s : s;
+        y[i] = add ? add[i] + alpha * s : alpha * s;
     }
+}
+
+__global__ void k_mp_sub(int n, const double *a, const double *b, double *y) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = a[i] - b[i];
 }
 
 __global__ void __launch_bounds__(BSZ) k_mp_dot_partial(int n, const double *a, const double *b, double *part) {
@@ -475,8 +480,9 @@ void mp_alloc_acts(fvm_mpnn *m) {
     m->flag.ensure(1);
 }
 
-// forward pass of the current graph on residual r (device, n); result in m->e or added to e_out
-void mp_forward(fvm_mpnn *m, const double *r, double *e_out, int accumulate) {
+// forward pass of the current graph on residual r (device, n); result in e_out or added to it;
+// keep: store the aggregates for a backward pass
+void mp_forward(fvm_mpnn *m, const double *r, double *e_out, int accumulate, bool keep = false) {
     hipStream_t st = m->stream;
     const int n = m->n;
     mp_alloc_acts(m);
@@ -488,11 +494,12 @@ void mp_forward(fvm_mpnn *m, const double *r, double *e_out, int accumulate) {
     const unsigned gl = (unsigned)((n + 63) / 64);
     for (int k = 0; k < MLAYERS; k++) {
         const float *bias = m->theta.p + layer_off(k) + 2 * MH * fin_of(k);
+        uint16_t *aout = keep ? m->A[k].p : nullptr;
         if (k == 0)
-            k_mp_layer<MF0><<<gl, BSZ, 0, st>>>(n, m->row, m->col, m->c.p, m->H[0].p, m->wbf[0].p, bias, m->A[0].p,
+            k_mp_layer<MF0><<<gl, BSZ, 0, st>>>(n, m->row, m->col, m->c.p, m->H[0].p, m->wbf[0].p, bias, aout,
                                                m->H[1].p);
         else
-            k_mp_layer<MH><<<gl, BSZ, 0, st>>>(n, m->row, m->col, m->c.p, m->H[k].p, m->wbf[k].p, bias, m->A[k].p,
+            k_mp_layer<MH><<<gl, BSZ, 0, st>>>(n, m->row, m->col, m->c.p, m->H[k].p, m->wbf[k].p, bias, aout,
                                               m->H[k + 1].p);
     }
     FVM_HIP(hipMemsetAsync(m->flag.p, 0, sizeof(int), st));
@@ -501,8 +508,9 @@ void mp_forward(fvm_mpnn *m, const double *r, double *e_out, int accumulate) {
     FVM_HIP(hipGetLastError());
 }
 
-// forward + backward on one residual; gradient added to m->grad; returns the loss
-double mp_loss_grad(fvm_mpnn *m, const double *r) {
+// forward + backward on one residual; gradient added to m->grad; returns the loss:
+// et null: ||r + A e||^2 / ||r||^2; else (e - et)^T (-A) (e - et) / et^T (-A) et
+double mp_loss_grad(fvm_mpnn *m, const double *r, const double *et) {
     hipStream_t st = m->stream;
     const int n = m->n;
     mp_alloc_acts(m);
@@ -510,16 +518,26 @@ double mp_loss_grad(fvm_mpnn *m, const double *r) {
     m->e.ensure(nn); m->t.ensure(nn); m->At.ensure(nn);
     m->dy.ensure(nn); m->dH.ensure(nn * MH); m->dH2.ensure(nn * MH);
     m->G.ensure(nn * MH); m->P.ensure(nn * MH);
-    mp_forward(m, r, m->e.p, 0);
+    mp_forward(m, r, m->e.p, 0, true);
     const unsigned gs = grid_for(n, BSZ);
-    // t = r + A e ; At = A t ; ||r||^2 and ||t||^2
-    k_mp_spmv<<<gs, BSZ, 0, st>>>(n, m->row, m->col, m->off, m->diag, m->e.p, r, m->t.p);
-    k_mp_spmv<<<gs, BSZ, 0, st>>>(n, m->row, m->col, m->off, m->diag, m->t.p, nullptr, m->At.p);
     const unsigned nb = grid_for(n, BSZ, 1024);
-    k_mp_dot_partial<<<nb, BSZ, 0, st>>>(n, r, r, m->dpart.p);
-    k_mp_sum<<<1, 64, 0, st>>>((int)nb, m->dpart.p, m->rr.p);                 // ||r||^2
-    k_mp_dot_partial<<<nb, BSZ, 0, st>>>(n, m->t.p, m->t.p, m->dpart.p + nb);
-    k_mp_sum<<<1, 64, 0, st>>>((int)nb, m->dpart.p + nb, m->rr.p + 1);        // ||t||^2
+    // rr[0] = denominator, rr[1] = numerator; At = d(numerator)/de / 2
+    if (!et) {
+        // t = r + A e ; At = A t (A symmetric)
+        k_mp_spmv<<<gs, BSZ, 0, st>>>(n, m->row, m->col, m->off, m->diag, m->e.p, r, 1.0, m->t.p);
+        k_mp_spmv<<<gs, BSZ, 0, st>>>(n, m->row, m->col, m->off, m->diag, m->t.p, nullptr, 1.0, m->At.p);
+        k_mp_dot_partial<<<nb, BSZ, 0, st>>>(n, r, r, m->dpart.p);
+        k_mp_dot_partial<<<nb, BSZ, 0, st>>>(n, m->t.p, m->t.p, m->dpart.p + nb);
+    } else {
+        // t = e - et ; At = -A t ; den = et^T (-A et) (the -A et product goes to e, no longer needed)
+        k_mp_sub<<<gs, BSZ, 0, st>>>(n, m->e.p, et, m->t.p);
+        k_mp_spmv<<<gs, BSZ, 0, st>>>(n, m->row, m->col, m->off, m->diag, m->t.p, nullptr, -1.0, m->At.p);
+        k_mp_spmv<<<gs, BSZ, 0, st>>>(n, m->row, m->col, m->off, m->diag, et, nullptr, -1.0, m->e.p);
+        k_mp_dot_partial<<<nb, BSZ, 0, st>>>(n, et, m->e.p, m->dpart.p);
+        k_mp_dot_partial<<<nb, BSZ, 0, st>>>(n, m->t.p, m->At.p, m->dpart.p + nb);
+    }
+    k_mp_sum<<<1, 64, 0, st>>>((int)nb, m->dpart.p, m->rr.p);
+    k_mp_sum<<<1, 64, 0, st>>>((int)nb, m->dpart.p + nb, m->rr.p + 1);
     k_mp_dy<<<gs, BSZ, 0, st>>>(n, m->At.p, m->s.p, m->rr.p, m->dy.p);
     // readout backward
     const unsigned nbo = (unsigned)((nn + BSZ - 1) / BSZ);
@@ -618,7 +636,7 @@ int fvm_mpnn_create(fvm_mpnn **out, int device, uint64_t seed) {
     m->device = device;
     try {
         fvm::MpScope g(m);
-        FVM_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+        FVM_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamDefault));
         const int np = fvm::NPARAM;
         m->theta.alloc(np); m->grad.alloc(np); m->adam_m.alloc(np); m->adam_v.alloc(np);
         for (int k = 0; k < fvm::MLAYERS; k++) m->wbf[k].alloc((size_t)fvm::MH * 2 * fvm::fin_of(k));
@@ -738,7 +756,19 @@ int fvm_mpnn_loss_grad(fvm_mpnn *m, const double *d_r, double *loss) {
     FVM_CHECK(m && d_r, FVM_E_INVALID, "null argument");
     FVM_CHECK(m->n > 0, FVM_E_NOT_SETUP, "no graph");
     fvm::MpScope g(m);
-    const double L = fvm::mp_loss_grad(m, d_r);
+    const double L = fvm::mp_loss_grad(m, d_r, nullptr);
+    FVM_HIP(hipStreamSynchronize(m->stream));
+    if (loss) *loss = L;
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_mpnn_loss_grad_pair(fvm_mpnn *m, const double *d_r, const double *d_e, double *loss) {
+    FVM_API_BEGIN
+    FVM_CHECK(m && d_r && d_e, FVM_E_INVALID, "null argument");
+    FVM_CHECK(m->n > 0, FVM_E_NOT_SETUP, "no graph");
+    fvm::MpScope g(m);
+    const double L = fvm::mp_loss_grad(m, d_r, d_e);
     FVM_HIP(hipStreamSynchronize(m->stream));
     if (loss) *loss = L;
     return FVM_OK;

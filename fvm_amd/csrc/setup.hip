@@ -986,6 +986,7 @@ void build_hierarchy(fvm_amg *h, int n_self, int n_total, const int *d_row, cons
     h->direct_level = -1;
     h->tail_prec = -1;
     h->tail_desc.release();
+    invalidate_cycle_graphs(h);
     h->n_self = n_self;
     h->n_total = n_total;
     const long long nnz_in = n_total > 0 ? (long long)read_scalar(d_row + n_total, s) : 0;
@@ -1381,6 +1382,7 @@ void build_hierarchy_batched(fvm_amg *h, BatchInfo &B) {
     h->direct_level = -1;
     h->tail_prec = -1;
     h->tail_desc.release();
+    invalidate_cycle_graphs(h);
     std::vector<int> off0(K + 1, 0);
     bool any_bnd = false;
     for (int k = 0; k < K; k++) {

@@ -67,6 +67,12 @@ class LearnedCorrection:
         check(lib().fvm_mpnn_loss_grad(self._h, _dp(r), ctypes.byref(L)))
         return L.value
 
+    def loss_grad_pair(self, r, e):
+        """Energy-norm loss of a residual/error pair (A e = -r); gradient accumulated."""
+        L = ctypes.c_double()
+        check(lib().fvm_mpnn_loss_grad_pair(self._h, _dp(r), _dp(e), ctypes.byref(L)))
+        return L.value
+
     def grad(self, reset=False):
         g = np.zeros(self.num_params(), np.float32)
         check(lib().fvm_mpnn_get_grad(self._h, ctypes.c_void_p(g.ctypes.data), 1 if reset else 0))
@@ -76,9 +82,11 @@ class LearnedCorrection:
         check(lib().fvm_mpnn_adam_step(self._h, float(lr)))
 
     def train(self, samples, steps, lr=1e-3, batch=4, seed=0, log=None):
-        """samples: list of (graph, residual) with graph = (n, row, col, off, diag)
-        device CSR and residual a device tensor. Minibatches of `batch` samples,
-        Adam with learning rate lr (cosine decay to lr/20). Returns the loss curve."""
+        """samples: list of (graph, residual) or (graph, residual, error) with
+        graph = (n, row, col, off, diag) device CSR and device tensors; pairs train
+        on the energy-norm error loss, residuals alone on the residual loss.
+        Minibatches of `batch` samples, Adam with learning rate lr (cosine decay to
+        lr/20). Returns the loss curve."""
         rng = np.random.default_rng(seed)
         curve = []
         cur = None
@@ -86,11 +94,11 @@ class LearnedCorrection:
             idx = rng.choice(len(samples), size=min(batch, len(samples)), replace=False)
             tot = 0.0
             for k in sorted(idx, key=lambda j: id(samples[j][0])):
-                g, r = samples[k]
+                g, r = samples[k][0], samples[k][1]
                 if g is not cur:
                     self.set_graph(*g)
                     cur = g
-                tot += self.loss_grad(r)
+                tot += self.loss_grad_pair(r, samples[k][2]) if len(samples[k]) > 2 else self.loss_grad(r)
             lr_t = lr * (0.05 + 0.95 * 0.5 * (1 + np.cos(np.pi * it / max(steps, 1))))
             self.adam_step(lr_t)
             curve.append(tot / len(idx))

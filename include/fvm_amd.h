@@ -25,7 +25,10 @@
  * fvm_last_error() returns a thread-local message for the last failure.
  * A handle is not thread-safe; it owns one HIP stream and all device memory of
  * its hierarchy until fvm_amg_cleanup() or fvm_amg_destroy(). Host pointers
- * passed in are only read (or written) during the call.
+ * passed in are only read (or written) during the call. The handles' streams are
+ * blocking streams: work the caller queued on the legacy default stream (e.g.
+ * PyTorch's default stream) finishes before a call reads device arrays, and
+ * every call has finished its device work when it returns.
  */
 #ifndef FVM_AMD_H
 #define FVM_AMD_H
@@ -165,14 +168,17 @@ int fvm_amg_kernel_timing(const fvm_amg *h, int kind, long long *launches, doubl
  * per block of the SELL-64 smoother kernels, "spmv_block" -- of the SELL-64
  * residual and SpMV kernels (64/128/256/512 each); "sell_xcd" / "spmv_xcd" --
  * 1: blocks of those kernels are renumbered so that each of the chip's 8 XCDs
- * works on a contiguous eighth of the rows, 0: hardware order; "coarsen_small_rows" --
+ * works on a contiguous eighth of the rows, 0: hardware order; "cycle_graph" --
+ * 1 (default): levels >= 1 of the coloured V-cycle are captured once into a HIP
+ * graph (first cycle after setup, parameter or tuning change) and replayed by
+ * every later cycle, 0: launched one by one; "coarsen_small_rows" --
  * levels with at most this many rows are agglomerated by one workgroup. */
 int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value);
 int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value);
 /* Autotuner (no counterpart in the reference): times every candidate launch
  * shape that cannot change a result -- "sell_block" (threads per block of the
  * coloured SELL-64 smoother kernels: 128/256/512) x "tail_max_rows"
- * (0/2500/1e4/4e4) x "sell_xcd" (0/1) -- with one warm-up and `reps` timed
+ * (0/2500/1e4/4e4) x "sell_xcd" (0/1) x "cycle_graph" (0/1) -- with one warm-up and `reps` timed
  * cycles each on the handle's current hierarchy and parameters, keeps the
  * fastest, then times the finest-level residual alone for "spmv_block"
  * (128/256/512) x "spmv_xcd" (0/1). Writes a JSON report of all
@@ -254,7 +260,8 @@ int fvm_batch_setup_stats(const fvm_batch *b, double *setup_seconds, int *total_
  * (delta form, A e ~ -r). Edge features are the off-diagonals scaled by the
  * row's diagonal; node features the Jacobi correction -r/d (scaled by its RMS),
  * its neighbour average, and the normalized log|d|. Trained on the GPU with the
- * library's own forward and backward kernels on the loss ||r + A e||^2/||r||^2
+ * library's own forward and backward kernels on the residual loss
+ * ||r + A e||^2/||r||^2 or, for residual/error pairs, the energy-norm error
  * (A symmetric). Attached to an FGMRES handle it corrects every preconditioner
  * application z = M(v) by z += NN(A z - v), and is switched off (plain AMG) for
  * the rest of a solve when the residual estimate stalls or the output is not
@@ -278,6 +285,9 @@ int fvm_mpnn_forward(fvm_mpnn *m, const double *d_r, double *d_e, int accumulate
 /* forward + backward on one residual: adds the gradient of the loss to the
  * handle's gradient buffer and returns the loss */
 int fvm_mpnn_loss_grad(fvm_mpnn *m, const double *d_r, double *loss);
+/* the same for a residual/error pair (A e = -r, device): energy loss
+ * (NN(r) - e)^T (-A) (NN(r) - e) / e^T (-A) e, -A positive definite */
+int fvm_mpnn_loss_grad_pair(fvm_mpnn *m, const double *d_r, const double *d_e, double *loss);
 /* host copy of the accumulated gradient (may be NULL); reset != 0 clears it */
 int fvm_mpnn_get_grad(fvm_mpnn *m, float *grad, int reset);
 /* one Adam step (beta 0.9/0.999) with the mean of the accumulated gradients */

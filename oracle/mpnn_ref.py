@@ -12,7 +12,9 @@ Network over the graph of a CRMatrix (delta form: A e = -r):
   h0_i  = [v_i, (C v)_i, l_i, 0 ... 0]    (F0 = 16 features, 3 used)
   layer k = 0, 1, 2:  a = C h ;  h' = relu(Ws_k h + Wn_k a + b_k)   (64 features)
   y_i   = wo . h3_i + bo ;   e_i = s y_i
-Loss of one (matrix, residual) sample: L = ||r + A e||^2 / ||r||^2 (A symmetric).
+Losses of one sample (A symmetric; -A positive definite for the FV matrices):
+  residual loss  L = ||r + A e||^2 / ||r||^2
+  energy loss    L = (e - e*)^T (-A) (e - e*) / (e*^T (-A) e*)  for a pair (r, e*), A e* = -r.
 Weights are used as the GPU uses them: bf16-rounded copies of FP32 masters.
 """
 import numpy as np
@@ -165,3 +167,33 @@ def grad(theta, G, r):
         if k > 0:
             dh = delta @ P["Ws%d" % k] + G.agg_t(delta @ P["Wn%d" % k])
     return float(t @ t / rr), pack(D)
+
+
+def loss_energy(theta, G, r, et):
+    e = forward(theta, G, r)
+    d = e - et
+    return float(-(d @ G.matvec(d)) / -(et @ G.matvec(et)))
+
+
+def grad_energy(theta, G, r, et):
+    """Analytic gradient of loss_energy() (A symmetric)."""
+    P = unpack(theta)
+    e, st = forward(theta, G, r, keep=True)
+    d = e - et
+    den = -(et @ G.matvec(et))
+    Md = -G.matvec(d)
+    ge = 2.0 * Md / den
+    dy = st["s"] * ge
+    D = {name: np.zeros(shp) for name, shp in layer_shapes()}
+    D["wo"] = st["h"].T @ dy
+    D["bo"] = np.array([dy.sum()])
+    dh = np.outer(dy, P["wo"])
+    for k in reversed(range(LAYERS)):
+        h, a, hn = st["cache"][k]
+        delta = dh * (hn > 0)
+        D["Ws%d" % k] = delta.T @ h
+        D["Wn%d" % k] = delta.T @ a
+        D["b%d" % k] = delta.sum(0)
+        if k > 0:
+            dh = delta @ P["Ws%d" % k] + G.agg_t(delta @ P["Wn%d" % k])
+    return float(d @ Md / den), pack(D)

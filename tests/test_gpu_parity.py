@@ -305,18 +305,25 @@ def test_coarsening_paths_agree(small, kat):
 
 @pytest.mark.parametrize("prec,smoother,npre", [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)])
 def test_tail_kernel_is_bit_identical(prec, smoother, npre):
-    # the single-workgroup tail must not change a single bit of the coloured cycle
+    # neither the single-workgroup tail nor replaying levels >= 1 from a captured
+    # graph may change a single bit of the coloured cycle
     m = matgen.grid3d(18, seed=22, contrast=1.5)
     res = []
-    for tail in (0, 1 << 30):
+    for tail, graph in ((0, 0), (1 << 30, 0), (0, 1), (1 << 30, 1), (2500, 1)):
         ga = gpu_amg(gsOrdering=1, precision=prec, smootherType=smoother, nPreSweeps=npre, nMaxIterations=12,
                      relativeTolerance=1e-30)
         ga.set_tuning("tail_max_rows", tail)
+        ga.set_tuning("cycle_graph", graph)
         ls = ls_from(m)
         ga.solve(ls)
         res.append((ls.delta.copy(), ga.history()))
-    assert np.array_equal(res[0][0], res[1][0])
-    assert np.array_equal(res[0][1], res[1][1])
+        # a second solve on the same handle replays the captured graph
+        ls = ls_from(m)
+        ga.solve(ls)
+        res.append((ls.delta.copy(), ga.history()))
+    for d, hist in res[1:]:
+        assert np.array_equal(res[0][0], d)
+        assert np.array_equal(res[0][1], hist)
 
 
 @pytest.mark.parametrize("prec", [0, 1])
@@ -330,12 +337,13 @@ def test_autotune_keeps_results(prec):
     ls1 = ls_from(m)
     ga.setup(ls1)
     rep = ga.autotune(reps=2)
-    assert len(rep["candidates"]) == 24 and len(rep["spmv_candidates"]) == 6
+    assert len(rep["candidates"]) == 48 and len(rep["spmv_candidates"]) == 6
     ch = rep["chosen"]
     assert any(c["sell_block"] == ch["sell_block"] and c["tail_max_rows"] == ch["tail_max_rows"]
                for c in rep["candidates"])
     assert ga.get_tuning("sell_block") == ch["sell_block"]
     assert ga.get_tuning("tail_max_rows") == ch["tail_max_rows"]
+    assert ga.get_tuning("cycle_graph") == ch["cycle_graph"]
     ga.solve(ls1)
     assert np.array_equal(ls0.delta, ls1.delta)
     assert np.array_equal(ref.history(), ga.history())

@@ -75,3 +75,29 @@ def test_bf16_rounding():
     assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0078125
     for a, b in zip(x, r):
         assert abs(a - b) <= abs(a) * 2.0 ** -8
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_energy_gradient_matches_finite_differences(seed):
+    G, _ = _graph(4, 20 + seed)
+    rng = np.random.default_rng(30 + seed)
+    th = M.init_params(seed) + 0.05 * rng.normal(size=M.n_params())
+    et = rng.normal(size=G.n)
+    r = -G.matvec(et)                      # the pair satisfies A e* = -r
+    L0, g = M.grad_energy(th, G, r, et)
+    assert abs(L0 - M.loss_energy(th, G, r, et)) < 1e-13 * max(1.0, L0)
+    for k in range(4):
+        v = rng.normal(size=th.size)
+        eps = 1e-6
+        fd = (M.loss_energy(th + eps * v, G, r, et) - M.loss_energy(th - eps * v, G, r, et)) / (2 * eps)
+        assert abs(fd - g @ v) <= 1e-6 * max(1.0, abs(fd)), (fd, g @ v)
+
+
+def test_energy_loss_of_exact_error_is_zero_and_positive_otherwise():
+    G, _ = _graph(4, 5)
+    rng = np.random.default_rng(3)
+    et = rng.normal(size=G.n)
+    # -A is positive definite: the energy of any nonzero vector is positive
+    for _ in range(5):
+        d = rng.normal(size=G.n)
+        assert -(d @ G.matvec(d)) > 0
