@@ -42,6 +42,7 @@ struct Level {
     std::vector<int> h_fwd_ptr, h_bwd_ptr;
     // coloured SELL layout
     bool fast_ready = false;
+    int sell_u = 8;              // load depth of the SELL-64 row kernels (mean slice width, rounded up to 8/16/32)
     DArr<int> perm, iperm;
     int ncolor = 0;
     std::vector<int> color_ptr;        // host: row ranges of each colour (coloured index)

@@ -130,7 +130,9 @@ __device__ __forceinline__ int xcd_block() {
 // acc + sum over the SELL-64 entries of one row, in storage order (the row's CSR
 // order). Loads of a group of U entries are issued before their products are
 // added, so a lane keeps U gathers in flight; the additions stay in order.
-template <int U, class V, class X>
+// NT: column indices read with non-temporal loads (the finest level streams its
+// matrix once per sweep; coarse levels keep theirs in cache from cycle to cycle).
+template <int U, bool NT, class V, class X>
 __device__ __forceinline__ X sell_row(X acc, const int *__restrict__ scol, const V *__restrict__ sval, int o,
                                       int w, int lane, const X *x) {
     for (int k = 0; k < w; k += U) {
@@ -140,7 +142,7 @@ __device__ __forceinline__ X sell_row(X acc, const int *__restrict__ scol, const
         for (int u = 0; u < U; u++)
             if (k + u < w) {
                 const int e = o + (k + u) * 64 + lane;
-                c[u] = __builtin_nontemporal_load(scol + e);
+                c[u] = NT ? __builtin_nontemporal_load(scol + e) : scol[e];
                 v[u] = (X)ldv(sval, e);
             }
 #pragma unroll
@@ -154,7 +156,24 @@ __device__ __forceinline__ X sell_row(X acc, const int *__restrict__ scol, const
 }
 constexpr int SELL_U = 8;
 
-template <class V, class D, class X>
+// Launch a SELL-64 row kernel with the load depth U of the level (all loads of a
+// row chunk of U entries are issued before the first is used: a wide coarse row
+// costs one round trip to memory per chunk, so coarse levels take U = 16 or 32).
+// The summation order does not depend on U.
+// Non-temporal column loads on the finest level only (SNT).
+#define FVM_SELL_U3(u_, CALL)                                    \
+    do {                                                         \
+        if ((u_) >= 32) { constexpr int SU = 32; CALL; }         \
+        else if ((u_) >= 16) { constexpr int SU = 16; CALL; }    \
+        else { constexpr int SU = 8; CALL; }                     \
+    } while (0)
+#define FVM_SELL_U(u_, l_, CALL)                                             \
+    do {                                                                     \
+        if ((l_) == 0) { constexpr bool SNT = true; FVM_SELL_U3(u_, CALL); }  \
+        else { constexpr bool SNT = false; FVM_SELL_U3(u_, CALL); }           \
+    } while (0)
+
+template <int U, bool NT, class V, class D, class X>
 __global__ void __launch_bounds__(512)
 k_sell_residual(int n, const int *__restrict__ soff, const int *__restrict__ scol,
                 const V *__restrict__ sval, const D *__restrict__ diag, const X *__restrict__ x,
@@ -165,10 +184,10 @@ k_sell_residual(int n, const int *__restrict__ soff, const int *__restrict__ sco
     const int o = soff[s];
     const int w = (soff[s + 1] - o) >> 6;
     const X acc = b[p] + (X)diag[p] * x[p];
-    r[p] = sell_row<SELL_U>(acc, scol, sval, o, w, lane, x);
+    r[p] = sell_row<U, NT>(acc, scol, sval, o, w, lane, x);
 }
 
-template <class V, class D, class X>
+template <int U, bool NT, class V, class D, class X>
 __global__ void __launch_bounds__(512)
 k_sell_gs(int beg, int end, const int *__restrict__ soff, const int *__restrict__ scol,
           const V *__restrict__ sval, const D *__restrict__ diag, X *x, const X *__restrict__ b,
@@ -180,11 +199,11 @@ k_sell_gs(int beg, int end, const int *__restrict__ soff, const int *__restrict_
     const int s = p >> 6, lane = p & 63;
     const int o = soff[s];
     const int w = (soff[s + 1] - o) >> 6;
-    const X acc = sell_row<SELL_U>(b[p], scol, sval, o, w, lane, (const X *)x);
+    const X acc = sell_row<U, NT>(b[p], scol, sval, o, w, lane, (const X *)x);
     x[p] = -acc / (X)diag[p];
 }
 
-template <class V, class D, class X>
+template <int U, bool NT, class V, class D, class X>
 __global__ void __launch_bounds__(512)
 k_sell_jacobi(int n, const int *__restrict__ soff, const int *__restrict__ scol,
               const V *__restrict__ sval, const D *__restrict__ diag, const X *__restrict__ xo,
@@ -195,7 +214,7 @@ k_sell_jacobi(int n, const int *__restrict__ soff, const int *__restrict__ scol,
     const int s = p >> 6, lane = p & 63;
     const int o = soff[s];
     const int w = (soff[s + 1] - o) >> 6;
-    const X acc = sell_row<SELL_U>(b[p], scol, sval, o, w, lane, xo);
+    const X acc = sell_row<U, NT>(b[p], scol, sval, o, w, lane, xo);
     xn[p] = -acc / (X)diag[p];
 }
 
@@ -374,7 +393,10 @@ __device__ __forceinline__ void tail_rows(const TailLv &L, int beg, int end, con
         const int s = p >> 6, lane = p & 63;
         const int o = L.soff[s];
         const int w = (L.soff[s + 1] - o) >> 6;
-        const X acc = sell_row<SELL_U>(b[p], L.scol, val, o, w, lane, xin);
+        // wide coarse rows: deeper load chunks (same summation order)
+        const X acc = w > 16 ? sell_row<32, false>(b[p], L.scol, val, o, w, lane, xin)
+                    : w > 8  ? sell_row<16, false>(b[p], L.scol, val, o, w, lane, xin)
+                             : sell_row<8, false>(b[p], L.scol, val, o, w, lane, xin);
         xout[p] = -acc / (X)dg[p];
     }
 }
@@ -389,7 +411,7 @@ __device__ void tail_sweeps(const TailLv &L, int ns, int jacobi) {
                 tail_rows<V, D, X>(L, L.cptr[c], L.cptr[c + 1], x, x, b);
                 __syncthreads();
             }
-            for (int c = L.ncolor - 1; c >= 0; c--) {
+            for (int c = L.ncolor - 2; c >= 0; c--) {   // the last colour is not repeated
                 tail_rows<V, D, X>(L, L.cptr[c], L.cptr[c + 1], x, x, b);
                 __syncthreads();
             }
@@ -431,7 +453,7 @@ k_tail(const TailLv *lv, int l0, int last, int npre, int npost, int jacobi, cons
                 const int o = L.soff[s];
                 const int w = (L.soff[s + 1] - o) >> 6;
                 const X acc = b[p] + (X)dg[p] * x[p];
-                r[p] = sell_row<SELL_U>(acc, L.scol, val, o, w, lane, x);
+                r[p] = sell_row<SELL_U, false>(acc, L.scol, val, o, w, lane, x);
             }
             __syncthreads();
             rr = r;
@@ -546,8 +568,8 @@ void residual(Run &R, int l, const X *x, const X *b, X *r) {
     if (n == 0) return;
     hipEvent_t t0 = (l == 0 && value_type(R, l) == V64) ? timing_begin(R.h) : nullptr;
     FVM_DISPATCH_VALS(R, l, L, X,
-        (k_sell_residual<<<grid_for(n, sb), sb, 0, s>>>(n, L.sell_off.p, L.sell_col.p, val, dg, x, b, r,
-                                                        R.h->spmv_xcd)),
+        FVM_SELL_U(L.sell_u, l, (k_sell_residual<SU, SNT><<<grid_for(n, sb), sb, 0, s>>>(n, L.sell_off.p, L.sell_col.p, val,
+                                                                               dg, x, b, r, R.h->spmv_xcd))),
         (k_csr_residual<<<grid_for(n, BS), BS, 0, s>>>(n, L.row.p, L.col.p, val, dg, x, b, r)));
     if (t0) timing_end(R.h, KT_RESID64, t0);
 }
@@ -561,14 +583,18 @@ void gs_sweep(Run &R, int l, X *x, const X *b) {
     if (R.fast) {
         for (int dir = 0; dir < 2; dir++) {
             hipEvent_t t0 = (l == 0) ? timing_begin(R.h) : nullptr;
-            for (int k = 0; k < L.ncolor; k++) {
+            // the backward pass starts with the colour the forward pass ended with:
+            // its rows read only rows of other colours, unchanged since, so that
+            // launch would rewrite the same values and is skipped
+            for (int k = dir ? 1 : 0; k < L.ncolor; k++) {
                 const int c = dir ? L.ncolor - 1 - k : k;
                 const int beg = L.color_ptr[c], end = L.color_ptr[c + 1];
                 if (end <= beg) continue;
                 FVM_DISPATCH_VALS(R, l, L, X,
-                    (k_sell_gs<<<grid_for(end - (beg & ~63), sb), sb, 0, s>>>(beg, end, L.sell_off.p, L.sell_col.p, val, dg, x, b,
-                                                                                l == 0 ? R.mask0 : nullptr,
-                                                                                R.h->sell_xcd)),
+                    FVM_SELL_U(L.sell_u, l,
+                        (k_sell_gs<SU, SNT><<<grid_for(end - (beg & ~63), sb), sb, 0, s>>>(
+                            beg, end, L.sell_off.p, L.sell_col.p, val, dg, x, b, l == 0 ? R.mask0 : nullptr,
+                            R.h->sell_xcd))),
                     (void)0);
             }
             if (t0) timing_end(R.h, KT_SWEEP, t0);
@@ -600,8 +626,9 @@ void jacobi_pass(Run &R, int l) {
     DArr<X> &ta = vt_arr<X>(L);
     const X *b = vb<X>(L);
     FVM_DISPATCH_VALS(R, l, L, X,
-        (k_sell_jacobi<<<grid_for(n, sb), sb, 0, s>>>(n, L.sell_off.p, L.sell_col.p, val, dg, xa.p, b, ta.p,
-                                                       l == 0 ? R.mask0 : nullptr, R.h->sell_xcd)),
+        FVM_SELL_U(L.sell_u, l, (k_sell_jacobi<SU, SNT><<<grid_for(n, sb), sb, 0, s>>>(n, L.sell_off.p, L.sell_col.p, val, dg,
+                                                                             xa.p, b, ta.p, l == 0 ? R.mask0 : nullptr,
+                                                                             R.h->sell_xcd))),
         (k_csr_jacobi<<<grid_for(n, BS), BS, 0, s>>>(n, L.row.p, L.col.p, val, dg, xa.p, b, ta.p,
                                                       l == 0 ? R.mask0 : nullptr)));
     std::swap(xa, ta);   // xnew becomes x (MultiFieldMatrix.cpp:203-215 copies it back)
@@ -1336,7 +1363,7 @@ k_sell_spmv(int n, const int *__restrict__ soff, const int *__restrict__ scol, c
     const int o = soff[s];
     const int w = (soff[s + 1] - o) >> 6;
     const double acc = (double)diag[p] * x[p];
-    y[p] = sell_row<SELL_U>(acc, scol, sval, o, w, lane, x);
+    y[p] = sell_row<SELL_U, true>(acc, scol, sval, o, w, lane, x);
 }
 
 __global__ void k_csr_spmv(int n, const int *__restrict__ row, const int *__restrict__ col,

@@ -86,7 +86,8 @@ __global__ void k_compact_fill(int n_self, const int *row, const int *col, const
 
 constexpr int INIT_SUB = 4;    // lanes per row in k_cnt_init
 constexpr int ROUND_SUB = 64;  // lanes per frontier row in k_coarsen_round (one wavefront)
-constexpr int SMALL_SUB = 16;  // lanes per frontier row in k_coarsen_small
+constexpr int SMALL_SUB = 64;  // lanes per frontier row in k_coarsen_small (one wavefront)
+constexpr int FU = 4;          // fan-out steps whose loads and atomics are issued together
 
 // append x to the queue when want; must be reached by every active lane
 __device__ __forceinline__ void push_agg(bool want, int x, int *q, int *qcnt) {
@@ -100,6 +101,16 @@ __device__ __forceinline__ void push_agg(bool want, int x, int *q, int *qcnt) {
     if (want) q[base + __popcll(m & ((1ull << lane) - 1ull))] = x;
 }
 
+// Integer atomics of the agglomeration rounds. WG = false: device scope (rounds
+// spread over the whole chip); WG = true: workgroup scope, for the rounds that one
+// workgroup runs alone (every access to the counters comes from its own
+// wavefronts, so the atomics need not be coherent beyond its CU).
+template <bool WG>
+__device__ __forceinline__ int atom_add(int *p, int v) {
+    if (WG) return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return atomicAdd(p, v);
+}
+
 // Rows that became ready, buffered per lane and appended to the queue with one
 // atomic per wavefront (the queue tail is a single hot counter).
 constexpr int PB = 4;
@@ -108,6 +119,7 @@ struct PushBuf {
     int n;
 };
 
+template <bool WG>
 __device__ __forceinline__ void pb_flush(PushBuf &pb, int *q, int *qcnt) {
     unsigned long long m[PB];
     int before[PB];
@@ -122,7 +134,7 @@ __device__ __forceinline__ void pb_flush(PushBuf &pb, int *q, int *qcnt) {
     const int lane = __lane_id();
     const int leader = __ffsll((long long)__ballot(1)) - 1;
     int base = 0;
-    if (lane == leader) base = atomicAdd(qcnt, tot);
+    if (lane == leader) base = atom_add<WG>(qcnt, tot);
     base = __shfl(base, leader, 64);
     const unsigned long long lt = (1ull << lane) - 1ull;
 #pragma unroll
@@ -132,8 +144,9 @@ __device__ __forceinline__ void pb_flush(PushBuf &pb, int *q, int *qcnt) {
 }
 
 // must be reached by every active lane
+template <bool WG>
 __device__ __forceinline__ void pb_push(PushBuf &pb, bool want, int x, int *q, int *qcnt) {
-    if (__ballot(pb.n == PB)) pb_flush(pb, q, qcnt);
+    if (__ballot(pb.n == PB)) pb_flush<WG>(pb, q, qcnt);
 #pragma unroll
     for (int k = 0; k < PB; k++)
         if (want && pb.n == k) pb.v[k] = x;
@@ -183,145 +196,217 @@ __global__ void k_cnt_init(int n, const int *row, const int *col, const int *row
     push_agg(head && !b && k == 0, r, q, qcnt);
 }
 
-// Decrement the dependents of d (rows whose D-set contains d) with the S lanes of
-// a sub-group. Every part runs as a few lane-parallel steps: the C part is
-// flattened over all (c, x) pairs with a prefix sum of |rowT(c)| across lanes,
-// so the atomics of one row are not a chain of per-neighbour steps.
+// Mask of the lanes of this lane's sub-group of S lanes within the wavefront.
 template <int S>
-__device__ void fan_out(int d, const int *row, const int *col, const int *rowT, const int *colT, int sym,
-                        const uint8_t *bnd, int *cnt, PushBuf &pb, int *q, int *qcnt) {
+__device__ __forceinline__ unsigned long long sub_mask() {
+    if (S == 64) return ~0ull;
+    return ((1ull << (S & 63)) - 1ull) << (__lane_id() & ~(S - 1));
+}
+
+// Walk the concatenation of the per-lane ranges [beg, beg + len) of the S lanes
+// of a sub-group, S elements per step: f(owner, idx) runs on every lane, with the
+// lane that owns element idx (idx = -1 past the end). The ranges are flattened
+// with a prefix sum across lanes, so a set of short lists costs a few
+// lane-parallel steps instead of one chain per list.
+template <int S, class F>
+__device__ __forceinline__ void walk_ranges(int beg, int len, F &&f) {
     const int lane = threadIdx.x & (S - 1);
-    const int rb = row[d], re = row[d + 1];
-    // A: r in row(d), r > d (d in rowT(r)); for a symmetric pattern it is also
+    int inc = len;   // inclusive prefix over the S lanes
+    for (int o = 1; o < S; o <<= 1) {
+        const int v = __shfl_up(inc, o, S);
+        if (lane >= o) inc += v;
+    }
+    const int total = __shfl(inc, S - 1, S);
+    const int exc = inc - len;
+    for (int k0 = 0; k0 < total; k0 += S) {
+        const int k = k0 + lane;
+        // element k belongs to the lane j with exc_j <= k < inc_j
+        int lo = 0, hi = S - 1;
+        for (int it = 1; it < S; it <<= 1) {
+            const int mid = (lo + hi) >> 1;
+            const int v = __shfl(inc, mid, S);
+            if (v > k) hi = mid;
+            else lo = mid + 1;
+        }
+        const int bj = __shfl(beg, lo, S);
+        const int ej = __shfl(exc, lo, S);
+        f(lo, k < total ? bj + (k - ej) : -1);
+    }
+}
+
+// Decrement the dependents of d0 and, when d1 >= 0, of d1 (rows whose D-set
+// contains them) with the S lanes of a sub-group; rows whose count reaches zero
+// are appended to the next frontier. [rb0, re0) is row(d0). The A and C parts of
+// both rows are one flattened pass, and the A decrement of an entry goes out
+// with the first C decrements, so one pass costs a single atomic round trip.
+template <int S, bool WG>
+__device__ void fan_out(int d0, int rb0, int re0, int d1, const int *row, const int *col, const int *rowT,
+                        const int *colT, int sym, const uint8_t *bnd, int *cnt, PushBuf &pb, int *q, int *qcnt) {
+    const int lane = threadIdx.x & (S - 1);
+    int rb1 = 0, re1 = 0;
+    if (d1 >= 0) { rb1 = row[d1]; re1 = row[d1 + 1]; }
+    const int m0 = re0 - rb0, mt = m0 + (re1 - rb1);
+    // A: x in row(d), x > d (d in rowT(x)); for a symmetric pattern it is also
     // the B set, counted twice in cnt: decrement by 2
     const int dec = sym ? 2 : 1;
-    for (int base = rb; base < re; base += S) {
-        const int p = base + lane;
-        bool want = false;
-        int x = -1;
-        if (p < re) {
-            x = col[p];
-            if (x > d && !isb(bnd, x)) want = (atomicSub(&cnt[x], dec) == dec);
-        }
-        pb_push(pb, want, x, q, qcnt);
-    }
-    // B: r in rowT(d), r > d (d in row(r))
-    if (!sym) {
-        for (int base = rowT[d]; base < rowT[d + 1]; base += S) {
-            const int p = base + lane;
-            bool want = false;
-            int x = -1;
-            if (p < rowT[d + 1]) {
-                x = colT[p];
-                if (x > d && !isb(bnd, x)) want = (atomicSub(&cnt[x], 1) == 1);
+    for (int base = 0; base < mt; base += S) {
+        const int k = base + lane;
+        int d = -1, c = -1, tb = 0, tl = 0;
+        bool a_ok = false;
+        if (k < mt) {
+            d = k < m0 ? d0 : d1;
+            c = col[k < m0 ? rb0 + k : rb1 + (k - m0)];
+            if (c > d && !isb(bnd, c)) {
+                a_ok = true;
+                tb = rowT[c];
+                tl = rowT[c + 1] - tb;
             }
-            pb_push(pb, want, x, q, qcnt);
         }
-    }
-    // C: for c in row(d), c > d: every r in rowT(c) with d < r < c
-    for (int base = rb; base < re; base += S) {
-        const int p = base + lane;
-        int c = -1, tb = 0, m = 0;
-        if (p < re) {
-            c = col[p];
-            if (c > d && !isb(bnd, c)) { tb = rowT[c]; m = rowT[c + 1] - tb; }
-        }
-        int inc = m;   // inclusive prefix over the S lanes
+        // C: for those c: every x in rowT(c) with d < x < c, flattened over the
+        // lanes (prefix sum of the list lengths, owner found by binary search);
+        // FU steps at a time, so that their loads and atomics are in flight together
+        int inc = tl;
         for (int o = 1; o < S; o <<= 1) {
             const int v = __shfl_up(inc, o, S);
             if (lane >= o) inc += v;
         }
         const int total = __shfl(inc, S - 1, S);
-        const int exc = inc - m;
-        for (int k0 = 0; k0 < total; k0 += S) {
-            const int k = k0 + lane;
-            // pair k belongs to the lane j with exc_j <= k < inc_j
-            int lo = 0, hi = S - 1;
-            for (int it = 1; it < S; it <<= 1) {
-                const int mid = (lo + hi) >> 1;
-                const int v = __shfl(inc, mid, S);
-                if (v > k) hi = mid;
-                else lo = mid + 1;
+        const int exc = inc - tl;
+        const int steps = total > 0 ? (total + S - 1) / S : 1;
+        for (int st0 = 0; st0 < steps; st0 += FU) {
+            int xs[FU];
+            bool ok[FU];
+#pragma unroll
+            for (int u = 0; u < FU; u++) {
+                xs[u] = -1;
+                ok[u] = false;
+                if (st0 + u >= steps) continue;   // uniform over the sub-group
+                const int kk = (st0 + u) * S + lane;
+                int lo = 0, hi = S - 1;
+                for (int it = 1; it < S; it <<= 1) {
+                    const int mid = (lo + hi) >> 1;
+                    const int v = __shfl(inc, mid, S);
+                    if (v > kk) hi = mid;
+                    else lo = mid + 1;
+                }
+                const int dj = __shfl(d, lo, S);
+                const int cj = __shfl(c, lo, S);
+                const int tj = __shfl(tb, lo, S);
+                const int ej = __shfl(exc, lo, S);
+                if (kk < total) {
+                    const int x = colT[tj + (kk - ej)];
+                    xs[u] = x;
+                    ok[u] = x > dj && x < cj && !isb(bnd, x);
+                }
             }
-            const int cj = __shfl(c, lo, S);
-            const int tj = __shfl(tb, lo, S);
-            const int ej = __shfl(exc, lo, S);
-            bool want = false;
-            int x = -1;
-            if (k < total) {
-                x = colT[tj + (k - ej)];
-                if (x > d && x < cj && !isb(bnd, x)) want = (atomicSub(&cnt[x], 1) == 1);
+            bool wa = false;
+            if (st0 == 0 && a_ok) wa = (atom_add<WG>(&cnt[c], -dec) == dec);
+            bool wc[FU];
+#pragma unroll
+            for (int u = 0; u < FU; u++) wc[u] = ok[u] && (atom_add<WG>(&cnt[xs[u]], -1) == 1);
+            pb_push<WG>(pb, wa, c, q, qcnt);
+#pragma unroll
+            for (int u = 0; u < FU; u++) pb_push<WG>(pb, wc[u], xs[u], q, qcnt);
+        }
+    }
+    // B: x in rowT(d), x > d (d in row(x))
+    if (!sym) {
+        for (int j = 0; j < 2; j++) {
+            const int d = j ? d1 : d0;
+            if (d < 0) continue;
+            const int tb = rowT[d], te = rowT[d + 1];
+            for (int base = tb; base < te; base += S) {
+                const int p = base + lane;
+                bool want = false;
+                int x = -1;
+                if (p < te) {
+                    x = colT[p];
+                    if (x > d && !isb(bnd, x)) want = (atom_add<WG>(&cnt[x], -1) == 1);
+                }
+                pb_push<WG>(pb, want, x, q, qcnt);
             }
-            pb_push(pb, want, x, q, qcnt);
         }
     }
 }
 
 // Decide one frontier row with the S lanes of its sub-group. Returns via the
 // queue (fan-out) and the shared state arrays.
-template <int S>
+template <int S, bool WG>
 __device__ void decide_row(int r, int t, double theta, const int *row, const int *col, const int *rowT,
                            const int *colT, int sym, const double *off, const double *diag,
                            const uint8_t *bnd, int *ldr, int *dround, int *mem, int *memr, int *mcnt,
                            int *cnt, int *q, int *qcnt) {
     const int lane = threadIdx.x & (S - 1);
-    if (dround[r] < t) return;   // already paired by an earlier seed (its fan-out is done)
-    // strongest ungrouped (u) and grouped (g) neighbours; ties: first in CSR order
+    // state, row range and diagonal of r in one step
+    const int dr = dround[r];
+    const int rb = row[r], re = row[r + 1];
     const double d0 = fabs(diag[r]);
+    if (dr < t) return;   // already paired by an earlier seed (its fan-out is done)
+    // strongest ungrouped (u) and grouped (g) neighbours; ties: first in CSR order
     double mu = -1.0, mg = -1.0;
-    int pu = INT_MAX, pg = INT_MAX;
-    for (int p = row[r] + lane; p < row[r + 1]; p += S) {
+    int pu = INT_MAX, pg = INT_MAX, cu = -1, cg = -1;
+    for (int p = rb + lane; p < re; p += S) {
         const int c = col[p];
+        const double o = off[p];
         if (isb(bnd, c)) continue;
         const double d1 = fabs(diag[c]);
-        const double w = fabs(fabs(off[p]) / (d0 < d1 ? d1 : d0));
+        const double w = fabs(fabs(o) / (d0 < d1 ? d1 : d0));
         if (!(dround[c] < t)) {
-            if (pu == INT_MAX || w > mu) { mu = w; pu = p; }
+            if (pu == INT_MAX || w > mu) { mu = w; pu = p; cu = c; }
         } else {
-            if (pg == INT_MAX || w > mg) { mg = w; pg = p; }
+            if (pg == INT_MAX || w > mg) { mg = w; pg = p; cg = c; }
         }
     }
     for (int o = S / 2; o > 0; o >>= 1) {
         const double mu2 = __shfl_xor(mu, o, 64);
         const int pu2 = __shfl_xor(pu, o, 64);
-        if (pu2 != INT_MAX && (pu == INT_MAX || mu2 > mu || (mu2 == mu && pu2 < pu))) { mu = mu2; pu = pu2; }
+        const int cu2 = __shfl_xor(cu, o, 64);
+        if (pu2 != INT_MAX && (pu == INT_MAX || mu2 > mu || (mu2 == mu && pu2 < pu))) { mu = mu2; pu = pu2; cu = cu2; }
         const double mg2 = __shfl_xor(mg, o, 64);
         const int pg2 = __shfl_xor(pg, o, 64);
-        if (pg2 != INT_MAX && (pg == INT_MAX || mg2 > mg || (mg2 == mg && pg2 < pg))) { mg = mg2; pg = pg2; }
+        const int cg2 = __shfl_xor(cg, o, 64);
+        if (pg2 != INT_MAX && (pg == INT_MAX || mg2 > mg || (mg2 == mg && pg2 < pg))) { mg = mg2; pg = pg2; cg = cg2; }
     }
-    const int cu = pu == INT_MAX ? -1 : col[pu];
-    const int cg = pg == INT_MAX ? -1 : col[pg];
     int action;  // 0 pair, 1 new singleton group, 2 join cg's group, 3 retry next round
-    int g = -1;
+    int g = -1, slot = 0;
     if (cu != -1 && (cg == -1 || mu > theta * mg)) {
         action = 0;
     } else if (cg == -1) {
         action = 1;
     } else {
         g = ldr[cg];
-        int count = 0;
-        bool blocked = false;
-        for (int k = 0; k < MEMW; k++) {
-            if (!(memr[(size_t)g * MEMW + k] < t)) continue;
-            count++;
-            const int m = mem[(size_t)g * MEMW + k];
-            for (int p = rowT[m] + lane; p < rowT[m + 1]; p += S) {
-                const int s = colT[p];
-                if (s < r && !isb(bnd, s) && !(dround[s] < t)) blocked = true;
+        // members of g placed before round t (one lane each), and every s < r
+        // adjacent to one of them through rowT: r must wait while such an s is
+        // undecided (it could still join g and change its count)
+        int tb = 0, tl = 0;
+        bool is_mem = false;
+        if (lane == 0) slot = mcnt[g];
+        if (lane < MEMW) {
+            const size_t e = (size_t)g * MEMW + lane;
+            if (memr[e] < t) {
+                const int m = mem[e];
+                is_mem = true;
+                tb = rowT[m];
+                tl = rowT[m + 1] - tb;
             }
         }
-        // every lane must take part in each shuffle (no short-circuit): a lane
-        // that skipped it would leave its partner reading an inactive lane
-        int bl = blocked ? 1 : 0;
-        for (int o = S / 2; o > 0; o >>= 1) bl |= __shfl_xor(bl, o, 64);
-        blocked = bl != 0;
+        const int count = __popcll(__ballot(is_mem) & sub_mask<S>());
+        bool blocked = false;
+        walk_ranges<S>(tb, tl, [&](int, int idx) {
+            if (idx >= 0) {
+                const int s = colT[idx];
+                if (s < r && !isb(bnd, s) && !(dround[s] < t)) blocked = true;
+            }
+        });
+        // every lane of the sub-group takes part in the vote
+        blocked = (__ballot(blocked) & sub_mask<S>()) != 0;
         action = blocked ? 3 : ((count > 2 + 2) ? 1 : 2);   // coarseCount > groupSize+2
     }
     PushBuf pb;
     pb.n = 0;
     if (action == 3) {
-        pb_push(pb, lane == 0, r, q, qcnt);
-        pb_flush(pb, q, qcnt);
+        pb_push<WG>(pb, lane == 0, r, q, qcnt);
+        pb_flush<WG>(pb, q, qcnt);
         return;
     }
     if (lane == 0) {
@@ -337,16 +422,14 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
             mcnt[r] = 1;
             dround[r] = t;
         } else {
-            const int k = mcnt[g];
-            mem[(size_t)g * MEMW + k] = r; memr[(size_t)g * MEMW + k] = t;
-            mcnt[g] = k + 1;
+            mem[(size_t)g * MEMW + slot] = r; memr[(size_t)g * MEMW + slot] = t;
+            mcnt[g] = slot + 1;
             ldr[r] = g;
             dround[r] = t;
         }
     }
-    fan_out<S>(r, row, col, rowT, colT, sym, bnd, cnt, pb, q, qcnt);
-    if (action == 0) fan_out<S>(cu, row, col, rowT, colT, sym, bnd, cnt, pb, q, qcnt);
-    pb_flush(pb, q, qcnt);
+    fan_out<S, WG>(r, rb, re, action == 0 ? cu : -1, row, col, rowT, colT, sym, bnd, cnt, pb, q, qcnt);
+    pb_flush<WG>(pb, q, qcnt);
 }
 
 // One round over the frontier (multi-block), one wavefront per frontier row.
@@ -361,42 +444,44 @@ k_coarsen_round(int t, const int *row, const int *col, const int *rowT, const in
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int nwaves = (gridDim.x * blockDim.x) >> 6;
     for (int i = wave; i < nf; i += nwaves)
-        decide_row<ROUND_SUB>(cur[i], t, theta, row, col, rowT, colT, sym, off, diag, bnd, ldr, dround, mem,
+        decide_row<ROUND_SUB, false>(cur[i], t, theta, row, col, rowT, colT, sym, off, diag, bnd, ldr, dround, mem,
                               memr, mcnt, cnt, nxt, nxt_cnt);
 }
 
-// All rounds of a small level in one workgroup: rounds are separated by
-// __syncthreads instead of kernel launches. Frontier buffers and counters live
-// in global memory (L2-resident at these sizes).
+// All rounds of a level in one workgroup: rounds are separated by workgroup
+// barriers instead of kernel launches. The barrier orders the workgroup's own
+// global stores and loads (all its wavefronts share one CU), so no device-scope
+// fence is needed, and every atomic is a workgroup-scope one: the frontier
+// sizes are LDS counters, the dependency counters stay in global memory.
 __global__ void __launch_bounds__(1024)
 k_coarsen_small(int n, int max_rounds, const int *row, const int *col, const int *rowT, const int *colT,
                 int sym, const double *off, const double *diag, const uint8_t *bnd, double theta, int *ldr,
-                int *dround, int *mem, int *memr, int *mcnt, int *cnt, int *qa, int *qb, int *qcnt,
+                int *dround, int *mem, int *memr, int *mcnt, int *cnt, int *qa, int *qb, const int *qcnt0,
                 int *rounds_out) {
-    __shared__ int s_nf;
+    __shared__ int s_q[2];   // s_q[t & 1]: size of the frontier of round t
     int *cur = qa, *nxt = qb;
+    if (threadIdx.x == 0) {
+        s_q[1] = *qcnt0;     // frontier of round 1, counted by the initialisation kernel
+        s_q[0] = 0;
+    }
+    __syncthreads();
     int t = 1;
     for (; t <= max_rounds; t++) {
-        if (threadIdx.x == 0) {
-            // the counters are updated by atomics (performed in L2): read and reset them
-            // atomically too, a plain load could return a stale L1 copy
-            s_nf = atomicAdd(&qcnt[t & 1], 0);
-            atomicExch(&qcnt[(t + 1) & 1], 0);
-        }
-        __syncthreads();
-        if (s_nf == 0) break;   // every row reachable has decided
-        __threadfence();   // see every store of the previous round (device-scope acquire)
-        const int nf = s_nf;
+        const int nf = s_q[t & 1];
+        // every row reachable has decided; nf > n cannot happen (each row is queued
+        // at most once per round) and only guards the reads of the list
+        if (nf == 0 || nf > n) break;
         const int per_wave = 64 / SMALL_SUB;
         const int wave = threadIdx.x >> 6;
         const int nwaves = blockDim.x >> 6;
         for (int base = wave * per_wave; base < nf; base += nwaves * per_wave) {
             const int i = base + (__lane_id() / SMALL_SUB);
             if (i < nf)
-                decide_row<SMALL_SUB>(cur[i], t, theta, row, col, rowT, colT, sym, off, diag, bnd, ldr, dround,
-                                      mem, memr, mcnt, cnt, nxt, &qcnt[(t + 1) & 1]);
+                decide_row<SMALL_SUB, true>(cur[i], t, theta, row, col, rowT, colT, sym, off, diag, bnd, ldr,
+                                            dround, mem, memr, mcnt, cnt, nxt, &s_q[(t + 1) & 1]);
         }
-        __threadfence();   // publish this round's stores before the barrier
+        __syncthreads();
+        if (threadIdx.x == 0) s_q[t & 1] = 0;   // every thread has read it: the counter of round t + 2
         __syncthreads();
         int *tmp = cur; cur = nxt; nxt = tmp;
     }
@@ -463,15 +548,35 @@ __global__ void k_c2f_fill(int n, const int *ci, const int *c2f_row, int *fill, 
     }
 }
 
-__global__ void k_segsort(int nseg, const int *ptr, int *v) {
+// sort each segment of v ascending by v, or by key[v] when key is given
+__global__ void k_segsort(int nseg, const int *ptr, int *v, const int *key) {
     for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < nseg; I += gridDim.x * blockDim.x) {
         const int b = ptr[I], e = ptr[I + 1];
         for (int a = b + 1; a < e; a++) {
-            const int key = v[a];
+            const int x = v[a];
+            const int kx = key ? key[x] : x;
             int j = a - 1;
-            while (j >= b && v[j] > key) { v[j + 1] = v[j]; j--; }
-            v[j + 1] = key;
+            while (j >= b && (key ? key[v[j]] : v[j]) > kx) { v[j + 1] = v[j]; j--; }
+            v[j + 1] = x;
         }
+    }
+}
+
+// Sort key of the coloured layout: the colour, then (coarse levels) rows by
+// decreasing length inside windows of SIGMA consecutive rows, so that the 64
+// rows of a SELL slice have similar lengths while rows stay near their index
+// order (locality of the gathers). Rows of one colour never read each other, so
+// their order changes no result.
+constexpr int SIGMA = 256;
+__global__ void k_color_key(int n, const int *color, const int *row, int by_len, int wbits,
+                            unsigned long long *key) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        unsigned long long k = (unsigned long long)color[i];
+        if (by_len) {
+            k = (k << wbits) | (unsigned long long)(i / SIGMA);
+            k = (k << 8) | (unsigned long long)(255 - min(row[i + 1] - row[i], 255));
+        }
+        key[i] = k;
     }
 }
 
@@ -887,7 +992,7 @@ int coarsen_level(fvm_amg *h, Level &L, double theta) {
         path = "single-workgroup";
         k_coarsen_small<<<1, 1024, 0, s>>>(n, (int)std::min<long long>(cap, INT_MAX - 1), L.row.p, L.col.p,
                                            rowT, colT, sym, L.off.p, L.diag.p, bnd, theta, ldr.p, dround.p,
-                                           mem.p, memr.p, mcnt.p, cnt.p, q1, q0, qc, rounds_out);
+                                           mem.p, memr.p, mcnt.p, cnt.p, q1, q0, qc + 1, rounds_out);
         FVM_HIP(hipGetLastError());
         t = read_scalar(rounds_out, s);
     } else {
@@ -938,7 +1043,7 @@ void coarse_to_fine(fvm_amg *h, Level &L) {
     h->ws.scan.exclusive(cnt.p, L.c2f_row.p, nc + 1, s);
     L.c2f_col.alloc(std::max(n, 1));
     k_c2f_fill<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.ci.p, L.c2f_row.p, fill.p, L.c2f_col.p);
-    k_segsort<<<grid_for(nc, BS, 4096), BS, 0, s>>>(nc, L.c2f_row.p, L.c2f_col.p);
+    k_segsort<<<grid_for(nc, BS, 4096), BS, 0, s>>>(nc, L.c2f_row.p, L.c2f_col.p, nullptr);
     FVM_HIP(hipGetLastError());
 }
 
@@ -1036,6 +1141,7 @@ void build_hierarchy(fvm_amg *h, int n_self, int n_total, const int *d_row, cons
         Level &A = h->lev.back();
         const int rounds = coarsen_level(h, A, h->prm.weightRatioThreshold);
         pt.mark("coarsen", k);
+        if (pt.on) fprintf(stderr, "[setup] level %2d n=%d nnz=%lld rounds=%d\n", k, A.n, A.nnz, rounds);
         if (k == 0) h->finest_rounds = rounds;
         const int nc = A.nc;
         if (nc == fine_size) { A.ci.release(); A.nc = 0; break; }       // AMG.cpp:163-168
@@ -1159,17 +1265,23 @@ void prepare_fast(fvm_amg *h) {
             if (read_scalar(left.p, s) == 0) break;
             FVM_CHECK(t < 4 * n + 64, FVM_E_NUMERIC, "colouring did not terminate");
         }
-        // stable sort of rows by colour -> perm
-        DArr<int> idx(n), keys_out(n);
+        // stable sort of rows by colour (coarse levels: then by decreasing length) -> perm
+        DArr<int> idx(n);
+        DArr<unsigned long long> keys_in(n), keys_out(n);
         L.perm.alloc(n);
         L.iperm.alloc(n);
         k_iota<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, idx.p);
+        const int by_len = l > 0 ? 1 : 0;
+        int wbits = 1;
+        while ((1LL << wbits) <= (long long)(n / SIGMA)) wbits++;
+        const int key_bits = by_len ? 9 + wbits + 8 : 9;
+        k_color_key<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, color.p, L.row.p, by_len, wbits, keys_in.p);
         size_t bytes = 0;
-        FVM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, color.p, keys_out.p, idx.p,
-                                                   L.perm.p, n, 0, 9, s));
+        FVM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys_in.p, keys_out.p, idx.p,
+                                                   L.perm.p, n, 0, key_bits, s));
         h->ws.sort_tmp.ensure(bytes + 16);
-        FVM_HIP(hipcub::DeviceRadixSort::SortPairs(h->ws.sort_tmp.p, bytes, color.p, keys_out.p,
-                                                   idx.p, L.perm.p, n, 0, 9, s));
+        FVM_HIP(hipcub::DeviceRadixSort::SortPairs(h->ws.sort_tmp.p, bytes, keys_in.p, keys_out.p,
+                                                   idx.p, L.perm.p, n, 0, key_bits, s));
         k_invert_perm<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.perm.p, L.iperm.p);
         DArr<int> hist(257);
         FVM_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), s));
@@ -1193,6 +1305,13 @@ void prepare_fast(fvm_amg *h) {
         L.sell_off.alloc((size_t)nslice + 1);
         h->ws.scan.exclusive(w64.p, L.sell_off.p, nslice + 1, s);
         const int total = read_scalar(L.sell_off.p + nslice, s);
+        {
+            const double wmean = (double)total / (64.0 * nslice);
+            L.sell_u = wmean > 16 ? 32 : (wmean > 8 ? 16 : 8);
+            if (getenv("FVM_AMD_SETUP_TIMING"))
+                fprintf(stderr, "[setup] level %2d SELL-64 entries %d for nnz %lld (%.2fx), mean width %.1f\n", l,
+                        total, L.nnz, L.nnz ? (double)total / (double)L.nnz : 0.0, wmean);
+        }
         L.sell_col.alloc(std::max(total, 1));
         L.sell_val.alloc(std::max(total, 1));
         L.pdiag.alloc(n);
@@ -1217,7 +1336,9 @@ void prepare_fast(fvm_amg *h) {
         L.pc2f_col.alloc(std::max(n, 1));
         k_c2f_fill<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.pci.p, L.pc2f_row.p, fill.p,
                                                         L.pc2f_col.p);
-        k_segsort<<<grid_for(nc, BS, 4096), BS, 0, s>>>(nc, L.pc2f_row.p, L.pc2f_col.p);
+        // fine rows of a coarse row in their original order: the restriction sums
+        // in the same order whatever the layout of the level
+        k_segsort<<<grid_for(nc, BS, 4096), BS, 0, s>>>(nc, L.pc2f_row.p, L.pc2f_col.p, L.perm.p);
         FVM_HIP(hipGetLastError());
     }
     for (auto &L : h->lev) L.fast_ready = true;
