@@ -525,12 +525,13 @@ def main():
     x_mixed = x
 
     # ---------------- plain FP64 GPU AMG (coloured GS), same system
-    t_plain, cyc_plain, err = None, None, None
+    t_plain, cyc_plain, err, tune_plain = None, None, None, None
     kp_s = (0, 0.0)
     if not args.skip_plain:
         plain = make_amg(1, 0)
-        for key in ("sell_block", "tail_max_rows", "spmv_block", "sell_xcd", "spmv_xcd", "cycle_graph"):
-            plain.set_tuning(key, amg.get_tuning(key))
+        # its own launch shapes: the FP64 hierarchy is tuned like the mixed one (untimed)
+        setup_on(plain, S)
+        tune_plain = plain.autotune(reps=2)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         setup_on(plain, S)
@@ -619,6 +620,7 @@ def main():
         "rel_error_vs_fp64_amg": err,
         "assembly_s": round(t_asm, 4),
         "autotune": tune,
+        "plain_fp64_autotune": None if tune_plain is None else tune_plain["chosen"],
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)

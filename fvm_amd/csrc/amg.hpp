@@ -141,6 +141,7 @@ struct fvm_amg {
     int cycle_graph = 1;
     hipGraphExec_t cg_exec[2] = {nullptr, nullptr};
     std::string tune_report;           // last autotuner result (JSON)
+    fvm::CounterProbe probe;           // pipelined termination checks of the setup loops
     int coop_blocks = -1;              // co-resident blocks for that launch (-1: not queried)
     // single-workgroup tail of the V-cycle: levels >= tail_level run in one launch
     int tail_level = -1;

@@ -161,6 +161,7 @@ int fvm_amg_destroy(fvm_amg *h) {
         HandleScope g(h);
         (void)hipStreamSynchronize(h->stream);
         fvm::invalidate_cycle_graphs(h);
+        h->probe.release();
         h->lev.clear();
         h->ws = fvm::Workspace();
         h->dense_inv.release();

@@ -171,12 +171,49 @@ struct Scanner {
     }
 };
 
+// Pinned staging word of the calling host thread (a copy into pageable memory
+// goes through a bounce buffer); held until the process ends.
+inline void *pinned_word() {
+    static thread_local void *p = nullptr;
+    if (!p) FVM_HIP(hipHostMalloc(&p, 64, hipHostMallocDefault));
+    return p;
+}
+
 template <class T>
 inline T read_scalar(const T *dptr, hipStream_t s) {
-    T v;
-    FVM_HIP(hipMemcpyAsync(&v, dptr, sizeof(T), hipMemcpyDeviceToHost, s));
+    static_assert(sizeof(T) <= 64, "scalar too large");
+    T *v = (T *)pinned_word();
+    FVM_HIP(hipMemcpyAsync(v, dptr, sizeof(T), hipMemcpyDeviceToHost, s));
     FVM_HIP(hipStreamSynchronize(s));
-    return v;
+    return *v;
 }
+
+// Pipelined reads of a device counter for loops that run batches of kernels
+// until the counter reaches zero: the counter after batch b is copied to a
+// pinned slot and read while batch b + 1 is already queued, so the device never
+// waits for the host.
+struct CounterProbe {
+    int *hv = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    void init() {
+        if (hv) return;
+        FVM_HIP(hipHostMalloc((void **)&hv, 2 * sizeof(int), hipHostMallocDefault));
+        FVM_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        FVM_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    }
+    void post(int slot, const int *d, hipStream_t s) {
+        FVM_HIP(hipMemcpyAsync(&hv[slot & 1], d, sizeof(int), hipMemcpyDeviceToHost, s));
+        FVM_HIP(hipEventRecord(ev[slot & 1], s));
+    }
+    int wait(int slot) {
+        FVM_HIP(hipEventSynchronize(ev[slot & 1]));
+        return hv[slot & 1];
+    }
+    void release() {   // with the device of the events current
+        for (auto &e : ev)
+            if (e) { (void)hipEventDestroy(e); e = nullptr; }
+        if (hv) { (void)hipHostFree(hv); hv = nullptr; }
+    }
+};
 
 }  // namespace fvm

@@ -86,8 +86,8 @@ __global__ void k_compact_fill(int n_self, const int *row, const int *col, const
 
 constexpr int INIT_SUB = 4;    // lanes per row in k_cnt_init
 constexpr int ROUND_SUB = 64;  // lanes per frontier row in k_coarsen_round (one wavefront)
+constexpr int THIN_SUB = 16;   // the same for levels of at most 8 entries per row
 constexpr int SMALL_SUB = 64;  // lanes per frontier row in k_coarsen_small (one wavefront)
-constexpr int FU = 4;          // fan-out steps whose loads and atomics are issued together
 
 // append x to the queue when want; must be reached by every active lane
 __device__ __forceinline__ void push_agg(bool want, int x, int *q, int *qcnt) {
@@ -234,138 +234,120 @@ __device__ __forceinline__ void walk_ranges(int beg, int len, F &&f) {
     }
 }
 
-// Decrement the dependents of d0 and, when d1 >= 0, of d1 (rows whose D-set
-// contains them) with the S lanes of a sub-group; rows whose count reaches zero
-// are appended to the next frontier. [rb0, re0) is row(d0). The A and C parts of
-// both rows are one flattened pass, and the A decrement of an entry goes out
-// with the first C decrements, so one pass costs a single atomic round trip.
-template <int S, bool WG>
-__device__ void fan_out(int d0, int rb0, int re0, int d1, const int *row, const int *col, const int *rowT,
-                        const int *colT, int sym, const uint8_t *bnd, int *cnt, PushBuf &pb, int *q, int *qcnt) {
-    const int lane = threadIdx.x & (S - 1);
-    int rb1 = 0, re1 = 0;
-    if (d1 >= 0) { rb1 = row[d1]; re1 = row[d1 + 1]; }
-    const int m0 = re0 - rb0, mt = m0 + (re1 - rb1);
-    // A: x in row(d), x > d (d in rowT(x)); for a symmetric pattern it is also
-    // the B set, counted twice in cnt: decrement by 2
-    const int dec = sym ? 2 : 1;
-    for (int base = 0; base < mt; base += S) {
-        const int k = base + lane;
-        int d = -1, c = -1, tb = 0, tl = 0;
-        bool a_ok = false;
-        if (k < mt) {
-            d = k < m0 ? d0 : d1;
-            c = col[k < m0 ? rb0 + k : rb1 + (k - m0)];
-            if (c > d && !isb(bnd, c)) {
-                a_ok = true;
-                tb = rowT[c];
-                tl = rowT[c + 1] - tb;
+// Dependents lists: for every row d, the rows x whose D-set holds d, one entry
+// per decrement of cnt[x] that the decision of d makes (bit 31 set: a decrement
+// by 2, the A and B parts of a symmetric pattern being the same set). They
+// depend on the pattern only, so they are built once per level; a decision then
+// finds its dependents with one contiguous load instead of walking row, rowT
+// and colT. The parts, as counted from the other side in k_cnt_init:
+//   A: x in row(d), x > d                       (d in rowT(x))
+//   B: x in rowT(d), x > d, non-symmetric only  (d in row(x))
+//   C: c in row(d), c > d: x in rowT(c), d < x < c
+// Boundary rows neither decide nor wait, and are in no list.
+constexpr int DEP_SUB = 4;
+template <bool FILL>
+__global__ void k_dep(int n, const int *row, const int *col, const int *rowT, const int *colT, int sym,
+                      const uint8_t *bnd, const int *dptr, int *out) {
+    const long long gt = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const int d = (int)(gt / DEP_SUB);
+    const int lane = threadIdx.x & (DEP_SUB - 1);
+    const bool valid = d < n && !isb(bnd, d);
+    const unsigned two = sym ? 0x80000000u : 0u;
+    int k = 0;     // entries of this lane
+    int pos = 0;   // FILL: next slot of this lane
+    for (int pass = 0; pass < (FILL ? 2 : 1); pass++) {
+        if (pass == 1) {
+            int inc = k;   // exclusive prefix over the lanes of the row
+            for (int o = 1; o < DEP_SUB; o <<= 1) {
+                const int v = __shfl_up(inc, o, DEP_SUB);
+                if (lane >= o) inc += v;
             }
+            pos = (valid ? dptr[d] : 0) + inc - k;
         }
-        // C: for those c: every x in rowT(c) with d < x < c, flattened over the
-        // lanes (prefix sum of the list lengths, owner found by binary search);
-        // FU steps at a time, so that their loads and atomics are in flight together
-        int inc = tl;
-        for (int o = 1; o < S; o <<= 1) {
-            const int v = __shfl_up(inc, o, S);
-            if (lane >= o) inc += v;
-        }
-        const int total = __shfl(inc, S - 1, S);
-        const int exc = inc - tl;
-        const int steps = total > 0 ? (total + S - 1) / S : 1;
-        for (int st0 = 0; st0 < steps; st0 += FU) {
-            int xs[FU];
-            bool ok[FU];
-#pragma unroll
-            for (int u = 0; u < FU; u++) {
-                xs[u] = -1;
-                ok[u] = false;
-                if (st0 + u >= steps) continue;   // uniform over the sub-group
-                const int kk = (st0 + u) * S + lane;
-                int lo = 0, hi = S - 1;
-                for (int it = 1; it < S; it <<= 1) {
-                    const int mid = (lo + hi) >> 1;
-                    const int v = __shfl(inc, mid, S);
-                    if (v > kk) hi = mid;
-                    else lo = mid + 1;
-                }
-                const int dj = __shfl(d, lo, S);
-                const int cj = __shfl(c, lo, S);
-                const int tj = __shfl(tb, lo, S);
-                const int ej = __shfl(exc, lo, S);
-                if (kk < total) {
-                    const int x = colT[tj + (kk - ej)];
-                    xs[u] = x;
-                    ok[u] = x > dj && x < cj && !isb(bnd, x);
+        int m = 0;
+        if (valid) {
+            for (int p = row[d] + lane; p < row[d + 1]; p += DEP_SUB) {
+                const int c = col[p];
+                if (c <= d || isb(bnd, c)) continue;
+                if (pass) out[pos++] = (int)((unsigned)c | two);
+                m++;
+                for (int qq = rowT[c]; qq < rowT[c + 1]; qq++) {
+                    const int x = colT[qq];
+                    if (x > d && x < c && !isb(bnd, x)) {
+                        if (pass) out[pos++] = x;
+                        m++;
+                    }
                 }
             }
-            bool wa = false;
-            if (st0 == 0 && a_ok) wa = (atom_add<WG>(&cnt[c], -dec) == dec);
-            bool wc[FU];
-#pragma unroll
-            for (int u = 0; u < FU; u++) wc[u] = ok[u] && (atom_add<WG>(&cnt[xs[u]], -1) == 1);
-            pb_push<WG>(pb, wa, c, q, qcnt);
-#pragma unroll
-            for (int u = 0; u < FU; u++) pb_push<WG>(pb, wc[u], xs[u], q, qcnt);
+            if (!sym)
+                for (int p = rowT[d] + lane; p < rowT[d + 1]; p += DEP_SUB) {
+                    const int x = colT[p];
+                    if (x > d && !isb(bnd, x)) {
+                        if (pass) out[pos++] = x;
+                        m++;
+                    }
+                }
         }
+        if (pass == 0) k = m;
     }
-    // B: x in rowT(d), x > d (d in row(x))
-    if (!sym) {
-        for (int j = 0; j < 2; j++) {
-            const int d = j ? d1 : d0;
-            if (d < 0) continue;
-            const int tb = rowT[d], te = rowT[d + 1];
-            for (int base = tb; base < te; base += S) {
-                const int p = base + lane;
-                bool want = false;
-                int x = -1;
-                if (p < te) {
-                    x = colT[p];
-                    if (x > d && !isb(bnd, x)) want = (atom_add<WG>(&cnt[x], -1) == 1);
-                }
-                pb_push<WG>(pb, want, x, q, qcnt);
-            }
-        }
+    if (!FILL) {
+        int tot = k;
+        for (int o = DEP_SUB / 2; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
+        if (d < n && lane == 0) out[d] = tot;
     }
 }
 
 // Decide one frontier row with the S lanes of its sub-group. Returns via the
-// queue (fan-out) and the shared state arrays.
+// queue (fan-out) and the shared state arrays. Loads that a later step may need
+// (the first dependents of r, the dependents range of every neighbour, the group
+// of every neighbour) go out with the loads that the decision itself needs, so
+// a decision costs few round trips to memory.
 template <int S, bool WG>
 __device__ void decide_row(int r, int t, double theta, const int *row, const int *col, const int *rowT,
-                           const int *colT, int sym, const double *off, const double *diag,
-                           const uint8_t *bnd, int *ldr, int *dround, int *mem, int *memr, int *mcnt,
-                           int *cnt, int *q, int *qcnt) {
+                           const int *colT, const double *off, const double *diag, const uint8_t *bnd,
+                           const int *dptr, const int *dep, int *ldr, int *dround, int *mem, int *memr,
+                           int *mcnt, int *cnt, int *q, int *qcnt) {
     const int lane = threadIdx.x & (S - 1);
-    // state, row range and diagonal of r in one step
+    // state, row range, dependents range and diagonal of r in one step
     const int dr = dround[r];
     const int rb = row[r], re = row[r + 1];
+    const int db = dptr[r], de = dptr[r + 1];
     const double d0 = fabs(diag[r]);
     if (dr < t) return;   // already paired by an earlier seed (its fan-out is done)
+    const int pre = db + lane < de ? dep[db + lane] : 0;   // first S dependents of r
     // strongest ungrouped (u) and grouped (g) neighbours; ties: first in CSR order
     double mu = -1.0, mg = -1.0;
-    int pu = INT_MAX, pg = INT_MAX, cu = -1, cg = -1;
+    int pu = INT_MAX, pg = INT_MAX, cu = -1, cg = -1, ub = 0, ue = 0, lg = -1;
     for (int p = rb + lane; p < re; p += S) {
         const int c = col[p];
         const double o = off[p];
         if (isb(bnd, c)) continue;
         const double d1 = fabs(diag[c]);
+        const int dc = dround[c];
+        const int cb = dptr[c], ce = dptr[c + 1], lc = ldr[c];
         const double w = fabs(fabs(o) / (d0 < d1 ? d1 : d0));
-        if (!(dround[c] < t)) {
-            if (pu == INT_MAX || w > mu) { mu = w; pu = p; cu = c; }
+        if (!(dc < t)) {
+            if (pu == INT_MAX || w > mu) { mu = w; pu = p; cu = c; ub = cb; ue = ce; }
         } else {
-            if (pg == INT_MAX || w > mg) { mg = w; pg = p; cg = c; }
+            if (pg == INT_MAX || w > mg) { mg = w; pg = p; cg = c; lg = lc; }
         }
     }
     for (int o = S / 2; o > 0; o >>= 1) {
         const double mu2 = __shfl_xor(mu, o, 64);
         const int pu2 = __shfl_xor(pu, o, 64);
         const int cu2 = __shfl_xor(cu, o, 64);
-        if (pu2 != INT_MAX && (pu == INT_MAX || mu2 > mu || (mu2 == mu && pu2 < pu))) { mu = mu2; pu = pu2; cu = cu2; }
+        const int ub2 = __shfl_xor(ub, o, 64);
+        const int ue2 = __shfl_xor(ue, o, 64);
+        if (pu2 != INT_MAX && (pu == INT_MAX || mu2 > mu || (mu2 == mu && pu2 < pu))) {
+            mu = mu2; pu = pu2; cu = cu2; ub = ub2; ue = ue2;
+        }
         const double mg2 = __shfl_xor(mg, o, 64);
         const int pg2 = __shfl_xor(pg, o, 64);
         const int cg2 = __shfl_xor(cg, o, 64);
-        if (pg2 != INT_MAX && (pg == INT_MAX || mg2 > mg || (mg2 == mg && pg2 < pg))) { mg = mg2; pg = pg2; cg = cg2; }
+        const int lg2 = __shfl_xor(lg, o, 64);
+        if (pg2 != INT_MAX && (pg == INT_MAX || mg2 > mg || (mg2 == mg && pg2 < pg))) {
+            mg = mg2; pg = pg2; cg = cg2; lg = lg2;
+        }
     }
     int action;  // 0 pair, 1 new singleton group, 2 join cg's group, 3 retry next round
     int g = -1, slot = 0;
@@ -374,7 +356,7 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
     } else if (cg == -1) {
         action = 1;
     } else {
-        g = ldr[cg];
+        g = lg;   // ldr[cg], final since cg was grouped in an earlier round
         // members of g placed before round t (one lane each), and every s < r
         // adjacent to one of them through rowT: r must wait while such an s is
         // undecided (it could still join g and change its count)
@@ -428,24 +410,47 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
             dround[r] = t;
         }
     }
-    fan_out<S, WG>(r, rb, re, action == 0 ? cu : -1, row, col, rowT, colT, sym, bnd, cnt, pb, q, qcnt);
+    // fan-out: the dependents of r and, for a pair, of its partner
+    const int l0 = de - db, lt = l0 + (action == 0 ? ue - ub : 0);
+    for (int base = 0; base < lt; base += S) {
+        const int k = base + lane;
+        bool want = false;
+        int x = -1;
+        if (k < lt) {
+            const int e = k < l0 ? (base == 0 ? pre : dep[db + k]) : dep[ub + (k - l0)];
+            x = e & 0x7fffffff;
+            const int dec = 1 + (int)((unsigned)e >> 31);
+            want = (atom_add<WG>(&cnt[x], -dec) == dec);
+        }
+        pb_push<WG>(pb, want, x, q, qcnt);
+    }
     pb_flush<WG>(pb, q, qcnt);
 }
 
 // One round over the frontier (multi-block), one wavefront per frontier row.
 // Reads frontier t from cur, appends frontier t+1 to nxt; zeroes the counter of round t+2.
+template <int S>
 __global__ void __launch_bounds__(256)
-k_coarsen_round(int t, const int *row, const int *col, const int *rowT, const int *colT, int sym,
-                const double *off, const double *diag, const uint8_t *bnd, double theta, int *ldr,
-                int *dround, int *mem, int *memr, int *mcnt, int *cnt, const int *cur, const int *cur_cnt,
-                int *nxt, int *nxt_cnt, int *zero_cnt) {
+k_coarsen_round(int t, int n, const int *row, const int *col, const int *rowT, const int *colT,
+                const double *off, const double *diag, const uint8_t *bnd, const int *dptr, const int *dep,
+                double theta, int *ldr, int *dround, int *mem, int *memr, int *mcnt, int *cnt, const int *cur,
+                const int *cur_cnt, int *nxt, int *nxt_cnt, int *zero_cnt) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *zero_cnt = 0;
+    // one sub-group of S lanes per frontier row
+    const int sub = (blockIdx.x * blockDim.x + threadIdx.x) / S;
+    const int nsub = (gridDim.x * blockDim.x) / S;
+    // the frontier size and this sub-group's first entry in one step (the list
+    // holds n entries; an entry past the frontier is not used)
     const int nf = *cur_cnt;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (int i = wave; i < nf; i += nwaves)
-        decide_row<ROUND_SUB, false>(cur[i], t, theta, row, col, rowT, colT, sym, off, diag, bnd, ldr, dround, mem,
-                              memr, mcnt, cnt, nxt, nxt_cnt);
+    const int r0 = sub < n ? cur[sub] : 0;
+    // the loop bound is uniform over each wavefront's sub-groups only up to the
+    // last pass: sub-groups past the frontier simply skip the row
+    for (int base = sub - (sub % (64 / S)); base < nf; base += nsub) {
+        const int i = base + (sub % (64 / S));
+        if (i < nf)
+            decide_row<S, false>(i == sub ? r0 : cur[i], t, theta, row, col, rowT, colT, off, diag, bnd, dptr,
+                                 dep, ldr, dround, mem, memr, mcnt, cnt, nxt, nxt_cnt);
+    }
 }
 
 // All rounds of a level in one workgroup: rounds are separated by workgroup
@@ -455,9 +460,9 @@ k_coarsen_round(int t, const int *row, const int *col, const int *rowT, const in
 // sizes are LDS counters, the dependency counters stay in global memory.
 __global__ void __launch_bounds__(1024)
 k_coarsen_small(int n, int max_rounds, const int *row, const int *col, const int *rowT, const int *colT,
-                int sym, const double *off, const double *diag, const uint8_t *bnd, double theta, int *ldr,
-                int *dround, int *mem, int *memr, int *mcnt, int *cnt, int *qa, int *qb, const int *qcnt0,
-                int *rounds_out) {
+                const double *off, const double *diag, const uint8_t *bnd, const int *dptr, const int *dep,
+                double theta, int *ldr, int *dround, int *mem, int *memr, int *mcnt, int *cnt, int *qa, int *qb,
+                const int *qcnt0, int *rounds_out) {
     __shared__ int s_q[2];   // s_q[t & 1]: size of the frontier of round t
     int *cur = qa, *nxt = qb;
     if (threadIdx.x == 0) {
@@ -477,8 +482,8 @@ k_coarsen_small(int n, int max_rounds, const int *row, const int *col, const int
         for (int base = wave * per_wave; base < nf; base += nwaves * per_wave) {
             const int i = base + (__lane_id() / SMALL_SUB);
             if (i < nf)
-                decide_row<SMALL_SUB, true>(cur[i], t, theta, row, col, rowT, colT, sym, off, diag, bnd, ldr,
-                                            dround, mem, memr, mcnt, cnt, nxt, &s_q[(t + 1) & 1]);
+                decide_row<SMALL_SUB, true>(cur[i], t, theta, row, col, rowT, colT, off, diag, bnd, dptr, dep,
+                                            ldr, dround, mem, memr, mcnt, cnt, nxt, &s_q[(t + 1) & 1]);
         }
         __syncthreads();
         if (threadIdx.x == 0) s_q[t & 1] = 0;   // every thread has read it: the counter of round t + 2
@@ -985,30 +990,64 @@ int coarsen_level(fvm_amg *h, Level &L, double theta) {
                                      memr.p, mcnt.p, q1, qc + 1);
         FVM_HIP(hipGetLastError());
     }
+    // dependents lists (count, scan, fill)
+    DArr<int> dptr((size_t)n + 1);
+    {
+        DArr<int> dcnt((size_t)n + 1);
+        FVM_HIP(hipMemsetAsync(dcnt.p + n, 0, sizeof(int), s));
+        const long long threads = (long long)n * DEP_SUB;
+        const unsigned g = (unsigned)((threads + 255) / 256);
+        k_dep<false><<<g, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, nullptr, dcnt.p);
+        h->ws.scan.exclusive(dcnt.p, dptr.p, n + 1, s);
+    }
+    const int ndep = read_scalar(dptr.p + n, s);
+    FVM_CHECK(ndep >= 0, FVM_E_INVALID, "dependents lists exceed 2^31 entries");
+    DArr<int> dep((size_t)std::max(ndep, 1));
+    {
+        const long long threads = (long long)n * DEP_SUB;
+        const unsigned g = (unsigned)((threads + 255) / 256);
+        k_dep<true><<<g, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, dptr.p, dep.p);
+        FVM_HIP(hipGetLastError());
+    }
     int t = 1;
     const long long cap = 4LL * n + 64;
     const char *path;
     if (n <= h->coarsen_small_rows) {
         path = "single-workgroup";
         k_coarsen_small<<<1, 1024, 0, s>>>(n, (int)std::min<long long>(cap, INT_MAX - 1), L.row.p, L.col.p,
-                                           rowT, colT, sym, L.off.p, L.diag.p, bnd, theta, ldr.p, dround.p,
-                                           mem.p, memr.p, mcnt.p, cnt.p, q1, q0, qc + 1, rounds_out);
+                                           rowT, colT, L.off.p, L.diag.p, bnd, dptr.p, dep.p, theta, ldr.p,
+                                           dround.p, mem.p, memr.p, mcnt.p, cnt.p, q1, q0, qc + 1, rounds_out);
         FVM_HIP(hipGetLastError());
         t = read_scalar(rounds_out, s);
     } else {
-        // rounds until the frontier is empty, checked every `batch` launches
+        // rounds until the frontier is empty, checked every `batch` launches; the
+        // check of batch b runs while batch b + 1 is queued (a round on an empty
+        // frontier only zeroes a counter)
         path = "multi-block";
-        const unsigned g_round = 1024;
+        // lanes per row: 16 for thin rows (finest FV levels, four rows per
+        // wavefront), 32 for large levels of moderate rows (their frontiers are
+        // thousands of rows wide), else one wavefront per row
+        const int sub = L.nnz <= 8LL * n ? THIN_SUB : (L.nnz <= 16LL * n && n >= (1 << 20)) ? 32 : ROUND_SUB;
+        const unsigned g_round = sub < ROUND_SUB ? 2048 : 1024;
         const int batch = 32;
-        for (;;) {
+        h->probe.init();
+        int t_prev = t;
+        for (int b = 0;; b++) {
             for (int k = 0; k < batch; k++, t++) {
-                k_coarsen_round<<<g_round, 256, 0, s>>>(
-                    t, L.row.p, L.col.p, rowT, colT, sym, L.off.p, L.diag.p, bnd, theta, ldr.p, dround.p,
-                    mem.p, memr.p, mcnt.p, cnt.p, (t & 1) ? q1 : q0, qc + (t % 3), (t & 1) ? q0 : q1,
-                    qc + ((t + 1) % 3), qc + ((t + 2) % 3));
+                auto launch = [&](auto kern) {
+                    kern<<<g_round, 256, 0, s>>>(
+                        t, n, L.row.p, L.col.p, rowT, colT, L.off.p, L.diag.p, bnd, dptr.p, dep.p, theta, ldr.p,
+                        dround.p, mem.p, memr.p, mcnt.p, cnt.p, (t & 1) ? q1 : q0, qc + (t % 3),
+                        (t & 1) ? q0 : q1, qc + ((t + 1) % 3), qc + ((t + 2) % 3));
+                };
+                if (sub == THIN_SUB) launch(k_coarsen_round<THIN_SUB>);
+                else if (sub == 32) launch(k_coarsen_round<32>);
+                else launch(k_coarsen_round<ROUND_SUB>);
             }
             FVM_HIP(hipGetLastError());
-            if (read_scalar(qc + (t % 3), s) == 0) break;   // frontier of round t (the next one)
+            h->probe.post(b, qc + (t % 3), s);   // frontier of round t (the next one)
+            if (b > 0 && h->probe.wait(b - 1) == 0) { t = t_prev; break; }
+            t_prev = t;
             if (t >= cap) break;
         }
     }
