@@ -73,6 +73,10 @@ __device__ __forceinline__ float ldv(const float *p, size_t i) { return p[i]; }
 __device__ __forceinline__ float ldv(const uint16_t *p, size_t i) {
     return __uint_as_float(((unsigned)p[i]) << 16);
 }
+// value of a stored matrix entry (bf16 entries are the high half of an FP32)
+__device__ __forceinline__ double cvt_val(double v) { return v; }
+__device__ __forceinline__ float cvt_val(float v) { return v; }
+__device__ __forceinline__ float cvt_val(uint16_t v) { return __uint_as_float(((unsigned)v) << 16); }
 
 // ------------------------------------------------------------------ CSR (reference order)
 template <class V, class D, class X>
@@ -130,27 +134,29 @@ __device__ __forceinline__ int xcd_block() {
 // acc + sum over the SELL-64 entries of one row, in storage order (the row's CSR
 // order). Loads of a group of U entries are issued before their products are
 // added, so a lane keeps U gathers in flight; the additions stay in order.
-// NT: column indices read with non-temporal loads (the finest level streams its
-// matrix once per sweep; coarse levels keep theirs in cache from cycle to cycle).
+// NT: matrix entries read with non-temporal loads (the finest levels stream
+// their matrices once per sweep; coarse levels keep theirs in cache from cycle
+// to cycle).
 template <int U, bool NT, class V, class X>
 __device__ __forceinline__ X sell_row(X acc, const int *__restrict__ scol, const V *__restrict__ sval, int o,
                                       int w, int lane, const X *x) {
     for (int k = 0; k < w; k += U) {
         int c[U];
-        X v[U], xv[U];
+        V v[U];   // raw storage values: a conversion here would wait for each load in turn
+        X xv[U];
 #pragma unroll
         for (int u = 0; u < U; u++)
             if (k + u < w) {
                 const int e = o + (k + u) * 64 + lane;
                 c[u] = NT ? __builtin_nontemporal_load(scol + e) : scol[e];
-                v[u] = (X)ldv(sval, e);
+                v[u] = NT ? __builtin_nontemporal_load(sval + e) : sval[e];
             }
 #pragma unroll
         for (int u = 0; u < U; u++)
             if (k + u < w) xv[u] = x[c[u]];
 #pragma unroll
         for (int u = 0; u < U; u++)
-            if (k + u < w) acc += v[u] * xv[u];
+            if (k + u < w) acc += (X)cvt_val(v[u]) * xv[u];
     }
     return acc;
 }
@@ -160,7 +166,9 @@ constexpr int SELL_U = 8;
 // row chunk of U entries are issued before the first is used: a wide coarse row
 // costs one round trip to memory per chunk, so coarse levels take U = 16 or 32).
 // The summation order does not depend on U.
-// Non-temporal column loads on the finest level only (SNT).
+// Non-temporal matrix loads on the two finest levels (SNT): they stream far more
+// than the caches hold and would evict the coarse levels, which are re-read
+// every cycle.
 #define FVM_SELL_U3(u_, CALL)                                    \
     do {                                                         \
         if ((u_) >= 32) { constexpr int SU = 32; CALL; }         \
@@ -169,7 +177,7 @@ constexpr int SELL_U = 8;
     } while (0)
 #define FVM_SELL_U(u_, l_, CALL)                                             \
     do {                                                                     \
-        if ((l_) == 0) { constexpr bool SNT = true; FVM_SELL_U3(u_, CALL); }  \
+        if ((l_) <= 1) { constexpr bool SNT = true; FVM_SELL_U3(u_, CALL); }  \
         else { constexpr bool SNT = false; FVM_SELL_U3(u_, CALL); }           \
     } while (0)
 
