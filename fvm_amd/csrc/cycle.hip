@@ -1384,14 +1384,18 @@ __global__ void k_csr_spmv(int n, const int *__restrict__ row, const int *__rest
     }
 }
 
-// partial dots of up to 8 basis vectors V_j (stride n) with w
+// partial dots of up to 8 basis vectors V_j (stride n) with w; nslot >= 0: also
+// the partial sums of w.w into slot nslot
 __global__ void __launch_bounds__(BS) k_multidot8(int n, const double *__restrict__ V, int j0, int k,
-                                                  const double *__restrict__ w, double *part, int nblk) {
+                                                  const double *__restrict__ w, double *part, int nblk,
+                                                  int nslot) {
     double acc[8];
+    double ww = 0;
 #pragma unroll
     for (int q = 0; q < 8; q++) acc[q] = 0;
     for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
         const double wi = w[i];
+        ww += wi * wi;
 #pragma unroll
         for (int q = 0; q < 8; q++)
             if (q < k) acc[q] += V[(size_t)(j0 + q) * n + i] * wi;
@@ -1403,14 +1407,35 @@ __global__ void __launch_bounds__(BS) k_multidot8(int n, const double *__restric
             if (threadIdx.x == 0) part[(size_t)(j0 + q) * nblk + blockIdx.x] = t;
         }
     }
+    if (nslot >= 0) {
+        const double t = block_sum<BS>(ww);
+        if (threadIdx.x == 0) part[(size_t)nslot * nblk + blockIdx.x] = t;
+    }
 }
 
-__global__ void __launch_bounds__(BS) k_multi_final(int nblk, const double *part, double *out) {
-    const int j = blockIdx.x;
+// out[b] = sum of the partials of slot b (b < k), out[k] = those of slot nslot
+__global__ void __launch_bounds__(BS) k_multi_final(int nblk, const double *part, double *out, int k, int nslot) {
+    const int j = (int)blockIdx.x < k ? (int)blockIdx.x : nslot;
     double s = 0;
     for (int i = threadIdx.x; i < nblk; i += BS) s += part[(size_t)j * nblk + i];
     s = block_sum<BS>(s);
-    if (threadIdx.x == 0) out[j] = s;
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// w += sign * sum_j c_j V_j, and the partial sums of the new w.w into slot nslot
+__global__ void __launch_bounds__(BS) k_multi_axpy_norm(int n, const double *__restrict__ V, int k,
+                                                        const double *__restrict__ c, double sign, double *w,
+                                                        double *part, int nblk, int nslot) {
+    double ww = 0;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        double s = 0;
+        for (int j = 0; j < k; j++) s += c[j] * V[(size_t)j * n + i];
+        const double v = w[i] + sign * s;
+        w[i] = v;
+        ww += v * v;
+    }
+    const double t = block_sum<BS>(ww);
+    if (threadIdx.x == 0) part[(size_t)nslot * nblk + blockIdx.x] = t;
 }
 
 // w += sign * sum_j c_j V_j
@@ -1503,19 +1528,28 @@ double norm2_of(Run &R, const double *r, int n) {
     return std::sqrt(reduce_to_host(h, g, 2));
 }
 
-// h_out[0..k) = V_{0..k}^T w
-void multidot(Run &R, KrylovWork &W, int k, const double *w, std::vector<double> &h_out) {
+// out[0..k) = V_{0..k}^T w, on the device
+void multidot_dev(Run &R, KrylovWork &W, int k, const double *w, double *out) {
     fvm_amg *h = R.h;
     hipStream_t s = h->stream;
     const int n = W.n;
     const unsigned g = grid_for(n, BS, 512);
     for (int j0 = 0; j0 < k; j0 += 8)
-        k_multidot8<<<g, BS, 0, s>>>(n, W.V.p, j0, std::min(8, k - j0), w, W.part.p, (int)g);
-    k_multi_final<<<k, BS, 0, s>>>((int)g, W.part.p, W.dev_h.p);
+        k_multidot8<<<g, BS, 0, s>>>(n, W.V.p, j0, std::min(8, k - j0), w, W.part.p, (int)g, -1);
+    k_multi_final<<<k, BS, 0, s>>>((int)g, W.part.p, out, k, 0);
     FVM_HIP(hipGetLastError());
-    h_out.resize(k);
-    FVM_HIP(hipMemcpyAsync(h_out.data(), W.dev_h.p, sizeof(double) * k, hipMemcpyDeviceToHost, s));
-    FVM_HIP(hipStreamSynchronize(s));
+}
+
+// w -= sum_j c_j V_j over the first k basis vectors (c on the device); ww != null:
+// the new w.w into *ww (device)
+void axpy_dev(Run &R, KrylovWork &W, int k, const double *c, double *w, double *ww) {
+    fvm_amg *h = R.h;
+    hipStream_t s = h->stream;
+    const unsigned g = grid_for(W.n, BS, 512);
+    const int nslot = W.m + 1;
+    k_multi_axpy_norm<<<g, BS, 0, s>>>(W.n, W.V.p, k, c, -1.0, w, W.part.p, (int)g, nslot);
+    if (ww) k_multi_final<<<1, BS, 0, s>>>((int)g, W.part.p, ww, 0, nslot);
+    FVM_HIP(hipGetLastError());
 }
 
 void multi_axpy(Run &R, KrylovWork &W, const double *basis, int k, const std::vector<double> &c, double sign,
@@ -1544,8 +1578,8 @@ void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const d
         W.Z.alloc((size_t)m * std::max(n, 1));
         W.w.alloc(std::max(n, 1)); W.b.alloc(std::max(n, 1)); W.x.alloc(std::max(n, 1));
         W.r.alloc(std::max(n, 1));
-        W.part.alloc((size_t)(m + 1) * 512);
-        W.dev_h.alloc(m + 1);
+        W.part.alloc((size_t)(m + 2) * 512);
+        W.dev_h.alloc(2 * (size_t)(m + 1) + 1);
     }
     const int *perm = R.fast ? F.perm.p : nullptr;
     k_gather<<<grid_for(n, BS), BS, 0, s>>>(n, perm, d_b, W.b.p);
@@ -1558,7 +1592,6 @@ void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const d
     };
     residual_into_r();
     const double r0 = norm1_of(R, W.r.p, n);
-    const double r0_2 = norm2_of(R, W.r.p, n);
     hist.assign(1, r0);
     double rn = r0;
     int total = 0;
@@ -1569,10 +1602,15 @@ void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const d
     if (nn.fallback_at) *nn.fallback_at = -1;
     std::vector<double> est_hist;
     if (!(r0 < p.absoluteTolerance) && n > 0) {
-        std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hc, hc2, y;
+        std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hc, hbuf, y;
         for (;;) {
             const double beta = norm2_of(R, W.r.p, n);
             if (!(beta > 0)) break;
+            // The stop rule is on the 1-norm of the true residual, the Arnoldi
+            // estimate is a 2-norm: aim the estimate at the reduction the 1-norm
+            // still needs (rn -> tol * r0), with a margin for the ratio of the two
+            // norms drifting, so that a restart rarely ends just short of it.
+            const double goal2 = beta * (p.relativeTolerance * r0 / rn) * 0.5;
             k_scale_into<<<grid_for(n, BS), BS, 0, s>>>(n, W.r.p, -1.0 / beta, W.V.p);
             std::fill(g.begin(), g.end(), 0.0);
             g[0] = beta;
@@ -1595,12 +1633,22 @@ void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const d
                     }
                 }
                 spmv64(R, zj, W.w.p);
-                multidot(R, W, j + 1, W.w.p, hc);
-                multi_axpy(R, W, W.V.p, j + 1, hc, -1.0, W.w.p);
-                multidot(R, W, j + 1, W.w.p, hc2);
-                multi_axpy(R, W, W.V.p, j + 1, hc2, -1.0, W.w.p);
-                for (int q = 0; q <= j; q++) hc[q] += hc2[q];
-                const double hn = norm2_of(R, W.w.p, n);
+                // classical Gram-Schmidt applied twice ("twice is enough": Giraud,
+                // Langou, Rozloznik, Comput. Math. Appl. 50 (2005)); coefficients stay
+                // on the device, the norm of w comes from the last pass, and the
+                // host reads them once per iteration
+                double *h1 = W.dev_h.p, *h2 = W.dev_h.p + (m + 1), *hw = W.dev_h.p + 2 * (m + 1);
+                multidot_dev(R, W, j + 1, W.w.p, h1);
+                axpy_dev(R, W, j + 1, h1, W.w.p, nullptr);
+                multidot_dev(R, W, j + 1, W.w.p, h2);
+                axpy_dev(R, W, j + 1, h2, W.w.p, hw);
+                hbuf.resize(2 * (size_t)(m + 1) + 1);
+                FVM_HIP(hipMemcpyAsync(hbuf.data(), W.dev_h.p, sizeof(double) * hbuf.size(), hipMemcpyDeviceToHost, s));
+                FVM_HIP(hipStreamSynchronize(s));
+                hc.assign(j + 1, 0.0);
+                for (int q = 0; q <= j; q++) hc[q] = hbuf[q] + hbuf[(size_t)(m + 1) + q];
+                const double ww1 = hbuf[2 * (size_t)(m + 1)];
+                const double hn = std::sqrt(ww1);
                 for (int q = 0; q <= j; q++) H[(size_t)q * m + j] = hc[q];
                 H[(size_t)(j + 1) * m + j] = hn;
                 if (hn > 0)
@@ -1629,8 +1677,7 @@ void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const d
                     nn_on = false;   // stalled: plain preconditioner for the rest of the solve
                     if (nn.fallback_at) *nn.fallback_at = total;
                 }
-                if (est <= p.relativeTolerance * r0_2 || est < p.absoluteTolerance || !(hn > 0) ||
-                    total >= p.nMaxIterations)
+                if (est <= goal2 || est < p.absoluteTolerance || !(hn > 0) || total >= p.nMaxIterations)
                     break;
             }
             y.assign(jend, 0.0);
