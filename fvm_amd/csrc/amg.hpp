@@ -114,9 +114,15 @@ struct fvm_amg {
     int device = 0;
     hipStream_t stream = nullptr;
     fvm::Pool pool;              // declared first: destroyed after every array it lends
+    // second stream of the setup (colouring and SELL layout of finished levels
+    // while later levels are agglomerated), with its own pool and workspace so
+    // that a block is only handed out again to work ordered on the same stream
+    fvm::Pool pool2;
+    hipStream_t stream2 = nullptr;
     fvm_amg_params prm{};
     std::vector<fvm::Level> lev;
     fvm::Workspace ws;
+    fvm::Workspace ws2;          // of stream2
     bool ready = false;
     int n_self = 0, n_total = 0;
     long long total_iterations = 0;
@@ -171,6 +177,8 @@ void build_hierarchy(fvm_amg *h, int n_self, int n_total, const int *d_row, cons
                      const double *d_diag, const double *d_off, const uint8_t *d_bnd);
 void prepare_exact_schedule(fvm_amg *h, Level &L);
 void prepare_fast(fvm_amg *h);
+void prepare_fast_level(fvm_amg *h, Level &L, int l, hipStream_t s, Workspace &ws, const int *seg_off, int nseg);
+void prepare_fast_maps(fvm_amg *h);
 void prepare_mixed(fvm_amg *h, bool fast);
 void prepare_direct(fvm_amg *h);
 void build_hierarchy_batched(fvm_amg *h, BatchInfo &B);

@@ -164,6 +164,9 @@ int fvm_amg_destroy(fvm_amg *h) {
         h->probe.release();
         h->lev.clear();
         h->ws = fvm::Workspace();
+        h->ws2 = fvm::Workspace();
+        if (h->stream2) (void)hipStreamDestroy(h->stream2);
+        h->pool2.trim();
         h->dense_inv.release();
         h->h_stage_b.release();
         h->h_stage_x.release();

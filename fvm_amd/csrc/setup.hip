@@ -15,6 +15,12 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <exception>
+#include <memory>
+#include <mutex>
+#include <thread>
 #include <climits>
 #include <cmath>
 #include <cstdlib>
@@ -1117,6 +1123,97 @@ void alloc_vectors(Level &L) {
 
 // Build the reference hierarchy (AMG::createCoarseLevels, AMG.cpp:149-210) from a
 // device CSR of the finest level; ghost rows/columns are dropped first.
+// Colouring and SELL layout of levels as soon as they are agglomerated: a host
+// thread drives them on the handle's second stream while the setup thread goes
+// on with the next levels (agglomeration rounds are latency-bound and leave the
+// device mostly idle). A level is handed over with an event of the main stream
+// after which its matrix and transpose pattern are final; the worker touches
+// nothing else of the hierarchy. The level vector is reserved beforehand, so
+// the addresses handed over stay valid.
+class LevelPipe {
+  public:
+    explicit LevelPipe(fvm_amg *h) : h_(h) {
+        if (!h_->stream2) FVM_HIP(hipStreamCreateWithFlags(&h_->stream2, hipStreamDefault));
+        // blocks of pool2 freed by the last setup may still be read by earlier
+        // work of the main stream: the second stream starts after it
+        hipEvent_t e;
+        FVM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        FVM_HIP(hipEventRecord(e, h_->stream));
+        FVM_HIP(hipStreamWaitEvent(h_->stream2, e, 0));
+        FVM_HIP(hipEventDestroy(e));
+        th_ = std::thread([this] { run(); });
+    }
+    ~LevelPipe() { stop(); }
+    void push(Level *L, int l) {
+        hipEvent_t e;
+        FVM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        FVM_HIP(hipEventRecord(e, h_->stream));
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            q_.push_back(Item{L, l, e});
+        }
+        cv_.notify_one();
+    }
+    // waits for every level handed over; the main stream then follows the second
+    void finish() {
+        stop();
+        if (err_) std::rethrow_exception(err_);
+        hipEvent_t e;
+        FVM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        FVM_HIP(hipEventRecord(e, h_->stream2));
+        FVM_HIP(hipStreamWaitEvent(h_->stream, e, 0));
+        FVM_HIP(hipEventDestroy(e));
+    }
+
+  private:
+    struct Item {
+        Level *L;
+        int l;
+        hipEvent_t ev;
+    };
+    void stop() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            done_ = true;
+        }
+        cv_.notify_one();
+        if (th_.joinable()) th_.join();
+        for (auto &it : q_) (void)hipEventDestroy(it.ev);   // left over after an error
+        q_.clear();
+    }
+    void run() {
+        (void)hipSetDevice(h_->device);
+        PoolScope ps(&h_->pool2);
+        for (;;) {
+            Item it;
+            {
+                std::unique_lock<std::mutex> g(mu_);
+                cv_.wait(g, [this] { return done_ || !q_.empty(); });
+                if (q_.empty()) return;
+                it = q_.front();
+                q_.pop_front();
+            }
+            try {
+                if (!err_) {
+                    FVM_HIP(hipStreamWaitEvent(h_->stream2, it.ev, 0));
+                    prepare_fast_level(h_, *it.L, it.l, h_->stream2, h_->ws2, nullptr, 0);
+                    FVM_HIP(hipStreamSynchronize(h_->stream2));
+                }
+            } catch (...) {
+                err_ = std::current_exception();
+            }
+            (void)hipEventDestroy(it.ev);
+        }
+    }
+    fvm_amg *h_;
+    std::thread th_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<Item> q_;
+    bool done_ = false;
+    std::exception_ptr err_;
+};
+
 void build_hierarchy(fvm_amg *h, int n_self, int n_total, const int *d_row, const int *d_col,
                      const double *d_diag, const double *d_off, const uint8_t *d_bnd) {
     const double t0 = now_s();
@@ -1176,9 +1273,17 @@ void build_hierarchy(fvm_amg *h, int n_self, int n_total, const int *d_row, cons
     PhaseTimer pt(h);
     pt.mark("validate+compact", 0);
     int fine_size = n_total;  // getLocalSize() of the finest level counts ghosts
+    // coloured layout: levels are coloured on the second stream as they are finished
+    std::unique_ptr<LevelPipe> pipe;
+    int handed = 0;
+    if (h->prm.gsOrdering == FVM_GS_MULTICOLOR && n_self > 0) {
+        h->lev.reserve((size_t)std::max(h->prm.maxCoarseLevels, 0) + 2);
+        pipe.reset(new LevelPipe(h));
+    }
     for (int k = 0; k < h->prm.maxCoarseLevels && n_self > 0; k++) {
         Level &A = h->lev.back();
         const int rounds = coarsen_level(h, A, h->prm.weightRatioThreshold);
+        if (pipe) { pipe->push(&A, k); handed = k + 1; }   // its pattern is analysed and final
         pt.mark("coarsen", k);
         if (pt.on) fprintf(stderr, "[setup] level %2d n=%d nnz=%lld rounds=%d\n", k, A.n, A.nnz, rounds);
         if (k == 0) h->finest_rounds = rounds;
@@ -1195,6 +1300,16 @@ void build_hierarchy(fvm_amg *h, int n_self, int n_total, const int *d_row, cons
             printf("Created coarse level %d of size %d\n", k, nc);
         fine_size = nc;
         if (h->prm.parallelStopRule && nc <= 3) break;                  // AMG.cpp:180-181
+    }
+    if (pipe) {
+        for (int l = handed; l < (int)h->lev.size(); l++) {
+            ensure_transpose(h, h->lev[l]);
+            pipe->push(&h->lev[l], l);
+        }
+        pipe->finish();
+        pipe.reset();
+        prepare_fast_maps(h);
+        pt.mark("colour+sell (overlapped)", -1);
     }
     for (auto &L : h->lev) alloc_vectors(L);
     h->ws.partials.ensure(4096);
@@ -1271,95 +1386,89 @@ void prepare_exact_schedule(fvm_amg *h, Level &L) {
 }
 
 // Graph colouring and coloured SELL-64 layout of every level.
-void prepare_fast(fvm_amg *h) {
+// Colouring and SELL-64 layout of one level, on stream s with workspace ws
+// (arrays from the calling thread's pool). The level's transpose pattern must
+// exist (ensure_transpose).
+void prepare_fast_level(fvm_amg *h, Level &L, int l, hipStream_t s, Workspace &ws, const int *seg_off, int nseg) {
+    const int n = L.n;
+    if (n == 0) return;
+    FVM_CHECK(L.sym >= 0, FVM_E_INVALID, "level pattern not analysed before colouring");
+    const int *rT = L.sym ? nullptr : L.rowT.p;
+    const int *cT = L.sym ? nullptr : L.colT.p;
+    DArr<int> color(n), cround(n), left(1);
+    FVM_HIP(hipMemsetAsync(color.p, 0, color.bytes(), s));
+    k_fill_int<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, cround.p, INT_MAX);
+    k_fill_int<<<1, 1, 0, s>>>(1, left.p, n);
+    int t = 1;
+    for (;;) {
+        for (int k = 0; k < 8; k++, t++)
+            k_color_round<<<grid_for(n, BS, 4096), BS, 0, s>>>(t, n, L.row.p, L.col.p, rT, cT,
+                                                               color.p, cround.p, left.p, seg_off, nseg);
+        FVM_HIP(hipGetLastError());
+        if (read_scalar(left.p, s) == 0) break;
+        FVM_CHECK(t < 4 * n + 64, FVM_E_NUMERIC, "colouring did not terminate");
+    }
+    // stable sort of rows by colour (coarse levels: then by decreasing length) -> perm
+    DArr<int> idx(n);
+    DArr<unsigned long long> keys_in(n), keys_out(n);
+    L.perm.alloc(n);
+    L.iperm.alloc(n);
+    k_iota<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, idx.p);
+    const int by_len = l > 0 ? 1 : 0;
+    int wbits = 1;
+    while ((1LL << wbits) <= (long long)(n / SIGMA)) wbits++;
+    const int key_bits = by_len ? 9 + wbits + 8 : 9;
+    k_color_key<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, color.p, L.row.p, by_len, wbits, keys_in.p);
+    size_t bytes = 0;
+    FVM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys_in.p, keys_out.p, idx.p,
+                                               L.perm.p, n, 0, key_bits, s));
+    ws.sort_tmp.ensure(bytes + 16);
+    FVM_HIP(hipcub::DeviceRadixSort::SortPairs(ws.sort_tmp.p, bytes, keys_in.p, keys_out.p,
+                                               idx.p, L.perm.p, n, 0, key_bits, s));
+    k_invert_perm<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.perm.p, L.iperm.p);
+    DArr<int> hist(257);
+    FVM_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), s));
+    k_color_hist<<<grid_for(n, BS, 1024), BS, 0, s>>>(n, color.p, hist.p);
+    std::vector<int> hh(257);
+    FVM_HIP(hipMemcpyAsync(hh.data(), hist.p, hist.bytes(), hipMemcpyDeviceToHost, s));
+    FVM_HIP(hipStreamSynchronize(s));
+    L.color_ptr.assign(1, 0);
+    for (int c = 0; c < 257 && L.color_ptr.back() < n; c++) L.color_ptr.push_back(L.color_ptr.back() + hh[c]);
+    L.ncolor = (int)L.color_ptr.size() - 1;
+    L.d_color_ptr.alloc(L.color_ptr.size());
+    FVM_HIP(hipMemcpyAsync(L.d_color_ptr.p, L.color_ptr.data(), sizeof(int) * L.color_ptr.size(),
+                           hipMemcpyHostToDevice, s));
+    FVM_HIP(hipStreamSynchronize(s));
+    if (getenv("FVM_AMD_SETUP_TIMING"))
+        fprintf(stderr, "[setup] level %2d n=%d nnz=%lld colours=%d\n", l, n, L.nnz, L.ncolor);
+    // SELL-64
+    const int nslice = (n + 63) / 64;
+    DArr<int> w64((size_t)nslice + 1);
+    k_sell_width<<<grid_for(nslice, BS, 4096), BS, 0, s>>>(n, nslice, L.perm.p, L.row.p, w64.p);
+    L.sell_off.alloc((size_t)nslice + 1);
+    ws.scan.exclusive(w64.p, L.sell_off.p, nslice + 1, s);
+    const int total = read_scalar(L.sell_off.p + nslice, s);
+    {
+        const double wmean = (double)total / (64.0 * nslice);
+        L.sell_u = wmean > 16 ? 32 : (wmean > 8 ? 16 : 8);
+        if (getenv("FVM_AMD_SETUP_TIMING"))
+            fprintf(stderr, "[setup] level %2d SELL-64 entries %d for nnz %lld (%.2fx), mean width %.1f\n", l,
+                    total, L.nnz, L.nnz ? (double)total / (double)L.nnz : 0.0, wmean);
+    }
+    L.sell_col.alloc(std::max(total, 1));
+    L.sell_val.alloc(std::max(total, 1));
+    L.pdiag.alloc(n);
+    k_sell_fill<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.perm.p, L.iperm.p, L.row.p, L.col.p,
+                                                     L.off.p, L.diag.p, L.sell_off.p,
+                                                     L.sell_col.p, L.sell_val.p, L.pdiag.p);
+    k_sell_tail<<<1, 64, 0, s>>>(n, nslice, L.sell_off.p, L.sell_col.p, L.sell_val.p);
+    FVM_HIP(hipGetLastError());
+}
+
+// Coloured-index transfer maps between consecutive levels (every level coloured).
+void prepare_fast_maps(fvm_amg *h) {
     hipStream_t s = h->stream;
     const int nl = (int)h->lev.size();
-    bool all = true;
-    for (auto &L : h->lev) all = all && L.fast_ready;
-    if (all) return;
-    for (int l = 0; l < nl; l++) {
-        Level &L = h->lev[l];
-        const int n = L.n;
-        if (n == 0) continue;
-        ensure_transpose(h, L);
-        const int *rT = L.sym ? nullptr : L.rowT.p;
-        const int *cT = L.sym ? nullptr : L.colT.p;
-        // batched hierarchy: colour each system as it is coloured alone
-        const int *seg_off = nullptr;
-        int nseg = 0;
-        if (h->batch && l < (int)h->batch->d_off.size()) {
-            seg_off = h->batch->d_off[l].p;
-            nseg = h->batch->K;
-        }
-        DArr<int> color(n), cround(n), left(1);
-        FVM_HIP(hipMemsetAsync(color.p, 0, color.bytes(), s));
-        k_fill_int<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, cround.p, INT_MAX);
-        k_fill_int<<<1, 1, 0, s>>>(1, left.p, n);
-        int t = 1;
-        for (;;) {
-            for (int k = 0; k < 8; k++, t++)
-                k_color_round<<<grid_for(n, BS, 4096), BS, 0, s>>>(t, n, L.row.p, L.col.p, rT, cT,
-                                                                   color.p, cround.p, left.p, seg_off, nseg);
-            FVM_HIP(hipGetLastError());
-            if (read_scalar(left.p, s) == 0) break;
-            FVM_CHECK(t < 4 * n + 64, FVM_E_NUMERIC, "colouring did not terminate");
-        }
-        // stable sort of rows by colour (coarse levels: then by decreasing length) -> perm
-        DArr<int> idx(n);
-        DArr<unsigned long long> keys_in(n), keys_out(n);
-        L.perm.alloc(n);
-        L.iperm.alloc(n);
-        k_iota<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, idx.p);
-        const int by_len = l > 0 ? 1 : 0;
-        int wbits = 1;
-        while ((1LL << wbits) <= (long long)(n / SIGMA)) wbits++;
-        const int key_bits = by_len ? 9 + wbits + 8 : 9;
-        k_color_key<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, color.p, L.row.p, by_len, wbits, keys_in.p);
-        size_t bytes = 0;
-        FVM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys_in.p, keys_out.p, idx.p,
-                                                   L.perm.p, n, 0, key_bits, s));
-        h->ws.sort_tmp.ensure(bytes + 16);
-        FVM_HIP(hipcub::DeviceRadixSort::SortPairs(h->ws.sort_tmp.p, bytes, keys_in.p, keys_out.p,
-                                                   idx.p, L.perm.p, n, 0, key_bits, s));
-        k_invert_perm<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.perm.p, L.iperm.p);
-        DArr<int> hist(257);
-        FVM_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), s));
-        k_color_hist<<<grid_for(n, BS, 1024), BS, 0, s>>>(n, color.p, hist.p);
-        std::vector<int> hh(257);
-        FVM_HIP(hipMemcpyAsync(hh.data(), hist.p, hist.bytes(), hipMemcpyDeviceToHost, s));
-        FVM_HIP(hipStreamSynchronize(s));
-        L.color_ptr.assign(1, 0);
-        for (int c = 0; c < 257 && L.color_ptr.back() < n; c++) L.color_ptr.push_back(L.color_ptr.back() + hh[c]);
-        L.ncolor = (int)L.color_ptr.size() - 1;
-        L.d_color_ptr.alloc(L.color_ptr.size());
-        FVM_HIP(hipMemcpyAsync(L.d_color_ptr.p, L.color_ptr.data(), sizeof(int) * L.color_ptr.size(),
-                               hipMemcpyHostToDevice, s));
-        FVM_HIP(hipStreamSynchronize(s));
-        if (getenv("FVM_AMD_SETUP_TIMING"))
-            fprintf(stderr, "[setup] level %2d n=%d nnz=%lld colours=%d\n", l, n, L.nnz, L.ncolor);
-        // SELL-64
-        const int nslice = (n + 63) / 64;
-        DArr<int> w64((size_t)nslice + 1);
-        k_sell_width<<<grid_for(nslice, BS, 4096), BS, 0, s>>>(n, nslice, L.perm.p, L.row.p, w64.p);
-        L.sell_off.alloc((size_t)nslice + 1);
-        h->ws.scan.exclusive(w64.p, L.sell_off.p, nslice + 1, s);
-        const int total = read_scalar(L.sell_off.p + nslice, s);
-        {
-            const double wmean = (double)total / (64.0 * nslice);
-            L.sell_u = wmean > 16 ? 32 : (wmean > 8 ? 16 : 8);
-            if (getenv("FVM_AMD_SETUP_TIMING"))
-                fprintf(stderr, "[setup] level %2d SELL-64 entries %d for nnz %lld (%.2fx), mean width %.1f\n", l,
-                        total, L.nnz, L.nnz ? (double)total / (double)L.nnz : 0.0, wmean);
-        }
-        L.sell_col.alloc(std::max(total, 1));
-        L.sell_val.alloc(std::max(total, 1));
-        L.pdiag.alloc(n);
-        k_sell_fill<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.perm.p, L.iperm.p, L.row.p, L.col.p,
-                                                         L.off.p, L.diag.p, L.sell_off.p,
-                                                         L.sell_col.p, L.sell_val.p, L.pdiag.p);
-        k_sell_tail<<<1, 64, 0, s>>>(n, nslice, L.sell_off.p, L.sell_col.p, L.sell_val.p);
-        FVM_HIP(hipGetLastError());
-    }
     for (int l = 0; l + 1 < nl; l++) {
         Level &L = h->lev[l];
         Level &C = h->lev[l + 1];
@@ -1382,6 +1491,27 @@ void prepare_fast(fvm_amg *h) {
     }
     for (auto &L : h->lev) L.fast_ready = true;
     FVM_HIP(hipStreamSynchronize(s));
+}
+
+void prepare_fast(fvm_amg *h) {
+    const int nl = (int)h->lev.size();
+    bool all = true;
+    for (auto &L : h->lev) all = all && L.fast_ready;
+    if (all) return;
+    for (int l = 0; l < nl; l++) {
+        Level &L = h->lev[l];
+        if (L.n == 0) continue;
+        ensure_transpose(h, L);
+        // batched hierarchy: colour each system as it is coloured alone
+        const int *seg_off = nullptr;
+        int nseg = 0;
+        if (h->batch && l < (int)h->batch->d_off.size()) {
+            seg_off = h->batch->d_off[l].p;
+            nseg = h->batch->K;
+        }
+        prepare_fast_level(h, L, l, h->stream, h->ws, seg_off, nseg);
+    }
+    prepare_fast_maps(h);
 }
 
 // FP32 values on the finest level, BF16 off-diagonals (FP32 diagonal) on coarse levels.
