@@ -250,6 +250,7 @@ def run_profile(args):
         amg.set_tuning("sell_xcd", args.sell_xcd)
         amg.set_tuning("spmv_xcd", args.spmv_xcd)
         amg.set_tuning("cycle_graph", args.cycle_graph)
+        amg.set_tuning("nt_levels", args.nt_levels)
         setup_on(amg, S)
         x = torch.zeros(n_total, dtype=torch.float64, device="cuda:0")
         from fvm_amd._lib import check, lib
@@ -449,6 +450,7 @@ def main():
     ap.add_argument("--sell-xcd", type=int, default=1, help="--profile: XCD-contiguous smoother blocks")
     ap.add_argument("--spmv-xcd", type=int, default=0, help="--profile: XCD-contiguous residual/SpMV blocks")
     ap.add_argument("--cycle-graph", type=int, default=1, help="--profile: replay levels >= 1 of the cycle from a graph")
+    ap.add_argument("--nt-levels", type=int, default=2, help="--profile: finest levels read with non-temporal loads")
     ap.add_argument("--learned", action="store_true",
                     help="instead of the 8M-cell solve: train the learned correction and run its ablation")
     ap.add_argument("--train-matrices", type=int, default=20)

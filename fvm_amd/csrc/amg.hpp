@@ -142,6 +142,7 @@ struct fvm_amg {
     int sell_block = 256;              // threads per block of the SELL-64 smoother kernels
     int spmv_block = 256;              // threads per block of the SELL-64 residual / SpMV kernels
     int sell_xcd = 1, spmv_xcd = 0;    // XCD-contiguous block order for those kernel classes
+    int nt_levels = 2;                 // finest levels read with non-temporal matrix loads
     fvm::BatchInfo *batch = nullptr;   // set while the hierarchy is a batched one
     // levels >= 1 of the coloured V-cycle replayed from a captured graph (per precision)
     int cycle_graph = 1;

@@ -166,19 +166,19 @@ constexpr int SELL_U = 8;
 // row chunk of U entries are issued before the first is used: a wide coarse row
 // costs one round trip to memory per chunk, so coarse levels take U = 16 or 32).
 // The summation order does not depend on U.
-// Non-temporal matrix loads on the two finest levels (SNT): they stream far more
-// than the caches hold and would evict the coarse levels, which are re-read
-// every cycle.
+// Non-temporal matrix loads on the nt_levels finest levels (SNT): they stream
+// far more than the caches hold and would evict the coarse levels, which are
+// re-read every cycle.
 #define FVM_SELL_U3(u_, CALL)                                    \
     do {                                                         \
         if ((u_) >= 32) { constexpr int SU = 32; CALL; }         \
         else if ((u_) >= 16) { constexpr int SU = 16; CALL; }    \
         else { constexpr int SU = 8; CALL; }                     \
     } while (0)
-#define FVM_SELL_U(u_, l_, CALL)                                             \
-    do {                                                                     \
-        if ((l_) <= 1) { constexpr bool SNT = true; FVM_SELL_U3(u_, CALL); }  \
-        else { constexpr bool SNT = false; FVM_SELL_U3(u_, CALL); }           \
+#define FVM_SELL_U(u_, l_, CALL)                                                        \
+    do {                                                                                \
+        if ((l_) < R.h->nt_levels) { constexpr bool SNT = true; FVM_SELL_U3(u_, CALL); }  \
+        else { constexpr bool SNT = false; FVM_SELL_U3(u_, CALL); }                      \
     } while (0)
 
 template <int U, bool NT, class V, class D, class X>
@@ -947,6 +947,7 @@ void smooth_device(fvm_amg *h, const double *d_b, double *d_x) {
 //                  sell_block/64 slices)
 //   tail_max_rows  levels at or below this size run in the single-workgroup tail
 //   cycle_graph    levels >= 1 of the coloured V-cycle replayed from a captured graph
+//   nt_levels      finest levels whose matrices are read with non-temporal loads
 // Each candidate: one warm-up cycle, then the minimum over `reps` cycles from
 // x = 0 on a fixed right-hand side, timed with HIP events on the handle's stream.
 // ===========================================================================
@@ -1034,6 +1035,48 @@ std::string autotune(fvm_amg *h, int reps) {
     h->sell_xcd = best_x;
     h->cycle_graph = best_g;
     invalidate_cycle_graphs(h);
+    // levels with non-temporal matrix loads, with the shapes chosen above
+    std::string nrep;
+    int best_nt = h->nt_levels;
+    {
+        const int keep_nt = h->nt_levels;
+        try {
+            double bnt = 1e300;
+            for (int nt = 0; nt <= 3; nt++) {
+                h->nt_levels = nt;
+                invalidate_cycle_graphs(h);
+                Run R = make_run(h);
+                double tmin = 1e300;
+                for (int r = 0; r <= reps; r++) {
+                    if (n) {
+                        k_tune_rhs<<<grid_for(n, BS), BS, 0, s>>>(n, F.b.p);
+                        k_fill<double><<<grid_for(n, BS, 4096), BS, 0, s>>>(n, F.x.p, 0.0);
+                    }
+                    R.r0_valid = false;
+                    FVM_HIP(hipEventRecord(e0, s));
+                    one_cycle(R);
+                    FVM_HIP(hipEventRecord(e1, s));
+                    FVM_HIP(hipEventSynchronize(e1));
+                    float ms = 0;
+                    FVM_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (r > 0) tmin = std::min(tmin, (double)ms);
+                }
+                char b3[120];
+                snprintf(b3, sizeof b3, "%s{\"nt_levels\": %d, \"cycle_ms\": %.4f}", nrep.empty() ? "" : ", ", nt, tmin);
+                nrep += b3;
+                if (tmin < bnt) { bnt = tmin; best_nt = nt; }
+            }
+        } catch (...) {
+            h->nt_levels = keep_nt;
+            invalidate_cycle_graphs(h);
+            h->ktime = keep_kt;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        h->nt_levels = best_nt;
+        invalidate_cycle_graphs(h);
+    }
     // residual / SpMV kernels of the finest level on their own
     const int keep_spmv = h->spmv_block;
     int best_s = keep_spmv, best_px = keep_px;
@@ -1078,12 +1121,12 @@ std::string autotune(fvm_amg *h, int reps) {
     h->spmv_block = best_s;
     h->spmv_xcd = best_px;
     h->ktime = keep_kt;
-    std::string tail_rep = "], \"spmv_candidates\": [" + srep + "], \"chosen\": ";
+    std::string tail_rep = "], \"nt_candidates\": [" + nrep + "], \"spmv_candidates\": [" + srep + "], \"chosen\": ";
     char buf[400];
     snprintf(buf, sizeof buf,
              "{\"sell_block\": %d, \"tail_max_rows\": %lld, \"sell_xcd\": %d, \"cycle_graph\": %d, "
-             "\"cycle_ms\": %.4f, \"spmv_block\": %d, \"spmv_xcd\": %d, \"residual_ms\": %.4f}}",
-             best_b, best_t, best_x, best_g, best, best_s, best_px, best_sms == 1e300 ? 0.0 : best_sms);
+             "\"nt_levels\": %d, \"cycle_ms\": %.4f, \"spmv_block\": %d, \"spmv_xcd\": %d, \"residual_ms\": %.4f}}",
+             best_b, best_t, best_x, best_g, best_nt, best, best_s, best_px, best_sms == 1e300 ? 0.0 : best_sms);
     rep += tail_rep;
     rep += buf;
     h->tune_report = rep;

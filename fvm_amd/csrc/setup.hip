@@ -312,7 +312,7 @@ template <int S, bool WG>
 __device__ void decide_row(int r, int t, double theta, const int *row, const int *col, const int *rowT,
                            const int *colT, const double *off, const double *diag, const uint8_t *bnd,
                            const int *dptr, const int *dep, int *ldr, int *dround, int *mem, int *memr,
-                           int *mcnt, int *cnt, int *q, int *qcnt) {
+                           int *mcnt, int *cnt, int *q, int *qcnt, int *nretry) {
     const int lane = threadIdx.x & (S - 1);
     // state, row range, dependents range and diagonal of r in one step
     const int dr = dround[r];
@@ -393,6 +393,7 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
     PushBuf pb;
     pb.n = 0;
     if (action == 3) {
+        if (lane == 0) atom_add<WG>(nretry, 1);   // statistics: blocked joins
         pb_push<WG>(pb, lane == 0, r, q, qcnt);
         pb_flush<WG>(pb, q, qcnt);
         return;
@@ -440,7 +441,7 @@ __global__ void __launch_bounds__(256)
 k_coarsen_round(int t, int n, const int *row, const int *col, const int *rowT, const int *colT,
                 const double *off, const double *diag, const uint8_t *bnd, const int *dptr, const int *dep,
                 double theta, int *ldr, int *dround, int *mem, int *memr, int *mcnt, int *cnt, const int *cur,
-                const int *cur_cnt, int *nxt, int *nxt_cnt, int *zero_cnt) {
+                const int *cur_cnt, int *nxt, int *nxt_cnt, int *zero_cnt, int *nretry) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *zero_cnt = 0;
     // one sub-group of S lanes per frontier row
     const int sub = (blockIdx.x * blockDim.x + threadIdx.x) / S;
@@ -455,7 +456,7 @@ k_coarsen_round(int t, int n, const int *row, const int *col, const int *rowT, c
         const int i = base + (sub % (64 / S));
         if (i < nf)
             decide_row<S, false>(i == sub ? r0 : cur[i], t, theta, row, col, rowT, colT, off, diag, bnd, dptr,
-                                 dep, ldr, dround, mem, memr, mcnt, cnt, nxt, nxt_cnt);
+                                 dep, ldr, dround, mem, memr, mcnt, cnt, nxt, nxt_cnt, nretry);
     }
 }
 
@@ -468,7 +469,7 @@ __global__ void __launch_bounds__(1024)
 k_coarsen_small(int n, int max_rounds, const int *row, const int *col, const int *rowT, const int *colT,
                 const double *off, const double *diag, const uint8_t *bnd, const int *dptr, const int *dep,
                 double theta, int *ldr, int *dround, int *mem, int *memr, int *mcnt, int *cnt, int *qa, int *qb,
-                const int *qcnt0, int *rounds_out) {
+                const int *qcnt0, int *rounds_out, int *nretry) {
     __shared__ int s_q[2];   // s_q[t & 1]: size of the frontier of round t
     int *cur = qa, *nxt = qb;
     if (threadIdx.x == 0) {
@@ -489,7 +490,7 @@ k_coarsen_small(int n, int max_rounds, const int *row, const int *col, const int
             const int i = base + (__lane_id() / SMALL_SUB);
             if (i < nf)
                 decide_row<SMALL_SUB, true>(cur[i], t, theta, row, col, rowT, colT, off, diag, bnd, dptr, dep,
-                                            ldr, dround, mem, memr, mcnt, cnt, nxt, &s_q[(t + 1) & 1]);
+                                            ldr, dround, mem, memr, mcnt, cnt, nxt, &s_q[(t + 1) & 1], nretry);
         }
         __syncthreads();
         if (threadIdx.x == 0) s_q[t & 1] = 0;   // every thread has read it: the counter of round t + 2
@@ -974,7 +975,7 @@ void ensure_transpose(fvm_amg *h, Level &L) {
 }
 
 // Agglomerate level L (reference order); fills L.ci, L.nc and returns the rounds used.
-int coarsen_level(fvm_amg *h, Level &L, double theta) {
+int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
     hipStream_t s = h->stream;
     const int n = L.n;
     ensure_transpose(h, L);
@@ -1022,7 +1023,8 @@ int coarsen_level(fvm_amg *h, Level &L, double theta) {
         path = "single-workgroup";
         k_coarsen_small<<<1, 1024, 0, s>>>(n, (int)std::min<long long>(cap, INT_MAX - 1), L.row.p, L.col.p,
                                            rowT, colT, L.off.p, L.diag.p, bnd, dptr.p, dep.p, theta, ldr.p,
-                                           dround.p, mem.p, memr.p, mcnt.p, cnt.p, q1, q0, qc + 1, rounds_out);
+                                           dround.p, mem.p, memr.p, mcnt.p, cnt.p, q1, q0, qc + 1, rounds_out,
+                                           ctr.p + 6);
         FVM_HIP(hipGetLastError());
         t = read_scalar(rounds_out, s);
     } else {
@@ -1044,7 +1046,7 @@ int coarsen_level(fvm_amg *h, Level &L, double theta) {
                     kern<<<g_round, 256, 0, s>>>(
                         t, n, L.row.p, L.col.p, rowT, colT, L.off.p, L.diag.p, bnd, dptr.p, dep.p, theta, ldr.p,
                         dround.p, mem.p, memr.p, mcnt.p, cnt.p, (t & 1) ? q1 : q0, qc + (t % 3),
-                        (t & 1) ? q0 : q1, qc + ((t + 1) % 3), qc + ((t + 2) % 3));
+                        (t & 1) ? q0 : q1, qc + ((t + 1) % 3), qc + ((t + 2) % 3), ctr.p + 6);
                 };
                 if (sub == THIN_SUB) launch(k_coarsen_round<THIN_SUB>);
                 else if (sub == 32) launch(k_coarsen_round<32>);
@@ -1074,6 +1076,7 @@ int coarsen_level(fvm_amg *h, Level &L, double theta) {
     k_assign_ci<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, ldr.p, bnd, gid.p, L.ci.p);
     FVM_HIP(hipGetLastError());
     L.nc = read_scalar(gid.p + n, s);
+    if (retries) *retries = read_scalar(ctr.p + 6, s);
     return t - 1;
 }
 
@@ -1282,10 +1285,11 @@ void build_hierarchy(fvm_amg *h, int n_self, int n_total, const int *d_row, cons
     }
     for (int k = 0; k < h->prm.maxCoarseLevels && n_self > 0; k++) {
         Level &A = h->lev.back();
-        const int rounds = coarsen_level(h, A, h->prm.weightRatioThreshold);
+        int retries = 0;
+        const int rounds = coarsen_level(h, A, h->prm.weightRatioThreshold, pt.on ? &retries : nullptr);
         if (pipe) { pipe->push(&A, k); handed = k + 1; }   // its pattern is analysed and final
         pt.mark("coarsen", k);
-        if (pt.on) fprintf(stderr, "[setup] level %2d n=%d nnz=%lld rounds=%d\n", k, A.n, A.nnz, rounds);
+        if (pt.on) fprintf(stderr, "[setup] level %2d n=%d nnz=%lld rounds=%d blocked-joins=%d\n", k, A.n, A.nnz, rounds, retries);
         if (k == 0) h->finest_rounds = rounds;
         const int nc = A.nc;
         if (nc == fine_size) { A.ci.release(); A.nc = 0; break; }       // AMG.cpp:163-168

@@ -171,7 +171,9 @@ int fvm_amg_kernel_timing(const fvm_amg *h, int kind, long long *launches, doubl
  * works on a contiguous eighth of the rows, 0: hardware order; "cycle_graph" --
  * 1 (default): levels >= 1 of the coloured V-cycle are captured once into a HIP
  * graph (first cycle after setup, parameter or tuning change) and replayed by
- * every later cycle, 0: launched one by one; "coarsen_small_rows" --
+ * every later cycle, 0: launched one by one; "nt_levels" -- the matrices of
+ * this many finest levels are read with non-temporal loads (default 2);
+ * "coarsen_small_rows" --
  * levels with at most this many rows are agglomerated by one workgroup. */
 int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value);
 int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value);
@@ -180,7 +182,8 @@ int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value);
  * coloured SELL-64 smoother kernels: 128/256/512) x "tail_max_rows"
  * (0/2500/1e4/4e4) x "sell_xcd" (0/1) x "cycle_graph" (0/1) -- with one warm-up and `reps` timed
  * cycles each on the handle's current hierarchy and parameters, keeps the
- * fastest, then times the finest-level residual alone for "spmv_block"
+ * fastest, then "nt_levels" (0..3) with those shapes, then times the
+ * finest-level residual alone for "spmv_block"
  * (128/256/512) x "spmv_xcd" (0/1). Writes a JSON report of all
  * candidates and the choice into report (cap bytes, may be NULL); returns the
  * report length or an FVM_E_* code. Needs a completed setup. */
