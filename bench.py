@@ -585,7 +585,7 @@ def main():
     # HBM bytes per launch of these kernels on the 200^3 workload, measured with two
     # rocprofv3 passes: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the scales a 512 MiB copy
     # gives); launch shapes as chosen by the autotuner
-    measured_traffic = {"finest_fp64_spmv_residual": 1.392e9, "finest_fp32_gs_sweep": 1.058e9}
+    measured_traffic = {"finest_fp64_spmv_residual": 1.389e9, "finest_fp32_gs_sweep": 6.31e8}
     roofline = dict(bound="hbm", kernel=dom, achieved=round(kd["gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(kd["gbs"] / HBM_PEAK_GBS, 4),
                     traffic=measured_traffic.get(dom) if N == 200 else None,

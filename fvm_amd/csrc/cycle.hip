@@ -354,6 +354,9 @@ __global__ void k_d2f(int n, const double *a, float *b) {
 __global__ void k_d2f_neg(int n, const double *a, float *b) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = (float)(-a[i]);
 }
+__global__ void k_f2d(int n, const float *e, double *x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = (double)e[i];
+}
 __global__ void k_add_f2d(int n, const float *e, double *x) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] += (double)e[i];
 }
@@ -1546,8 +1549,7 @@ void precond(Run &R, const double *v, double *z) {
     } else {
         k_d2f_neg<<<grid_for(n, BS), BS, 0, s>>>(n, v, F.b32.p);
         cycle<float>(R, h->prm.cycleType, 0, true);
-        k_fill<double><<<grid_for(n, BS, 4096), BS, 0, s>>>(n, z, 0.0);
-        k_add_f2d<<<grid_for(n, BS), BS, 0, s>>>(n, F.x32.p, z);
+        k_f2d<<<grid_for(n, BS), BS, 0, s>>>(n, F.x32.p, z);
     }
     FVM_HIP(hipGetLastError());
 }

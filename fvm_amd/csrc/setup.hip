@@ -593,70 +593,121 @@ __global__ void k_color_key(int n, const int *color, const int *row, int by_len,
 }
 
 // ------------------------------------------------------------------ coarse operator
-__global__ void k_cc_ub(int nc, const int *c2f_row, const int *c2f_col, const int *row, int *ub) {
-    for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < nc; I += gridDim.x * blockDim.x) {
-        int s = 0;
-        for (int q = c2f_row[I]; q < c2f_row[I + 1]; q++) {
-            const int i = c2f_col[q];
-            s += row[i + 1] - row[i];
-        }
-        ub[I] = s;
+// Coarse operator with one sub-group of S lanes per coarse row I (the same results as one
+// thread per row, bit for bit). The entries of I's fine rows are flattened over the
+// lanes in the reference's order: for each fine row (ascending), a diagonal entry and
+// then its CSR entries. Distinct coarse columns are kept in order of first appearance
+// (CRMatrix.h:616-640); each value is summed in flattened order by broadcasting the
+// entries one by one (CRMatrix.h:722-755), so every sum adds the same terms in the
+// same order as the sequential loop.
+struct GalEntry {
+    int J;        // coarse column of an off-diagonal entry (-1: diagonal entry or dropped)
+    int isd;      // 1: diagonal entry of a fine row, or an entry inside the aggregate (J == I)
+    double v;
+};
+
+__device__ __forceinline__ GalEntry gal_entry(int K, int m, int nf, const int *fi, const int *fo,
+                                              const int *fb, int I, const int *col, const double *diag,
+                                              const double *off, const int *ci) {
+    GalEntry e{-1, 0, 0.0};
+    if (K >= m) return e;
+    // owner fine row: the last t with fo[t] <= K (nf <= MEMW); selected without
+    // indexing the arrays at run time, so that they stay in registers
+    int ofi = fi[0], ofo = fo[0], ofb = fb[0];
+#pragma unroll
+    for (int t = 1; t < MEMW; t++)
+        if (t < nf && fo[t] <= K) { ofi = fi[t]; ofo = fo[t]; ofb = fb[t]; }
+    if (K == ofo) {
+        e.isd = 1;
+        e.v = diag[ofi];
+    } else {
+        const int p = ofb + (K - ofo - 1);
+        const int J = ci[col[p]];
+        e.v = off[p];
+        if (J == I) e.isd = 1;
+        else if (J >= 0) e.J = J;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) ub[nc] = 0;
+    return e;
 }
 
-// Distinct coarse neighbours J != I of coarse row I in order of first
-// appearance (fine rows ascending, then CSR order): CRMatrix.h:616-640.
-__global__ void k_cc_list(int nc, const int *c2f_row, const int *c2f_col, const int *row,
-                          const int *col, const int *ci, const int *uboff, int *scratch, int *cnt) {
-    for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < nc; I += gridDim.x * blockDim.x) {
-        int *lst = scratch + uboff[I];
-        int k = 0;
-        for (int q = c2f_row[I]; q < c2f_row[I + 1]; q++) {
-            const int i = c2f_col[q];
-            for (int p = row[i]; p < row[i + 1]; p++) {
-                const int J = ci[col[p]];
-                if (J < 0 || J == I) continue;
-                bool seen = false;
-                for (int e = 0; e < k; e++)
-                    if (lst[e] == J) { seen = true; break; }
-                if (!seen) lst[k++] = J;
-            }
+template <int S, bool FILL>
+__global__ void k_gal(int nc, const int *c2f_row, const int *c2f_col, const int *row, const int *col,
+                      const double *diag, const double *off, const int *ci, int *cnt, const int *crow,
+                      int *ccol, double *cdiag, double *coff) {
+    const int lane = threadIdx.x & (S - 1);
+    const int sub = (blockIdx.x * blockDim.x + threadIdx.x) / S;
+    const int nsub = (gridDim.x * blockDim.x) / S;
+    for (int I = sub; I < nc; I += nsub) {
+        // fine rows of I (at most MEMW: one lane each), their rows and flattened offsets
+        const int q0 = c2f_row[I];
+        const int nf = min(c2f_row[I + 1] - q0, MEMW);
+        int fi_l = 0, fb_l = 0, len_l = 0;
+        if (lane < nf) {
+            fi_l = c2f_col[q0 + lane];
+            fb_l = row[fi_l];
+            len_l = row[fi_l + 1] - fb_l;
         }
-        cnt[I] = k;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[nc] = 0;
-}
-
-// Galerkin values with the reference's summation order (CRMatrix.h:722-755).
-__global__ void k_cc_galerkin(int nc, const int *c2f_row, const int *c2f_col, const int *row,
-                              const int *col, const double *diag, const double *off, const int *ci,
-                              const int *uboff, const int *scratch, const int *crow, int *ccol,
-                              double *cdiag, double *coff) {
-    for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < nc; I += gridDim.x * blockDim.x) {
-        const int base = crow[I];
-        const int k = crow[I + 1] - base;
-        const int *lst = scratch + uboff[I];
-        for (int e = 0; e < k; e++) { ccol[base + e] = lst[e]; coff[base + e] = 0.0; }
+        int fi[MEMW], fo[MEMW], fb[MEMW];
+        int m = 0;
+#pragma unroll
+        for (int t = 0; t < MEMW; t++) {
+            fi[t] = __shfl(fi_l, t, S);
+            fb[t] = __shfl(fb_l, t, S);
+            const int ln = __shfl(len_l, t, S);
+            fo[t] = m;
+            if (t < nf) m += 1 + ln;
+        }
+        // pass over the chunks of S entries: distinct columns in first-appearance order
+        int ndist = 0;   // distinct columns in the chunks before this one
         double d = 0.0;
-        for (int q = c2f_row[I]; q < c2f_row[I + 1]; q++) {
-            const int i = c2f_col[q];
-            d += diag[i];
-            for (int p = row[i]; p < row[i + 1]; p++) {
-                const int J = ci[col[p]];
-                if (J < 0) continue;
-                if (J != I) {
-                    int e = 0;
-                    while (lst[e] != J) e++;
-                    coff[base + e] += off[p];
-                } else {
-                    d += off[p];
+        for (int c0 = 0; c0 < m; c0 += S) {
+            const GalEntry me = gal_entry(c0 + lane, m, nf, fi, fo, fb, I, col, diag, off, ci);
+            // first appearance: no earlier entry (earlier chunks, then earlier lanes) with the same J
+            bool first = me.J >= 0;
+            for (int e0 = 0; e0 < c0; e0 += S) {
+                const GalEntry o = gal_entry(e0 + lane, m, nf, fi, fo, fb, I, col, diag, off, ci);
+                for (int k = 0; k < S; k++) {
+                    const int Jo = __shfl(o.J, k, S);
+                    if (Jo >= 0 && Jo == me.J) first = false;
                 }
             }
+            for (int k = 0; k < S; k++) {
+                const int Jo = __shfl(me.J, k, S);
+                if (k < lane && Jo >= 0 && Jo == me.J) first = false;
+            }
+            const unsigned long long bal = __ballot(first) & sub_mask<S>();
+            const int below = __popcll(bal & ((1ull << __lane_id()) - 1ull));
+            if (FILL) {
+                // the value of a first appearance: every entry of its column, in flattened order
+                double acc = 0.0;
+                for (int e0 = 0; e0 < m; e0 += S) {
+                    const GalEntry o = gal_entry(e0 + lane, m, nf, fi, fo, fb, I, col, diag, off, ci);
+                    for (int k = 0; k < S; k++) {
+                        // every lane takes part in each shuffle
+                        const int Jo = __shfl(o.J, k, S);
+                        const double vo = __shfl(o.v, k, S);
+                        const int isdo = __shfl(o.isd, k, S);
+                        if (first && Jo == me.J) acc += vo;
+                        // the diagonal: lane 0, during the first chunk's pass
+                        if (c0 == 0 && lane == 0 && isdo) d += vo;
+                    }
+                }
+                if (first) {
+                    const int pos = crow[I] + ndist + below;
+                    ccol[pos] = me.J;
+                    coff[pos] = acc;
+                }
+            }
+            ndist += __popcll(bal);
         }
-        cdiag[I] = d;
+        if (lane == 0) {
+            if (FILL) cdiag[I] = d;
+            else cnt[I] = ndist;
+        }
     }
+    if (!FILL && blockIdx.x == 0 && threadIdx.x == 0) cnt[nc] = 0;
 }
+
 
 // ------------------------------------------------------------------ GS level schedule
 // Rows i < j are ordered by the sweep whenever j is in row(i) or i is in row(j):
@@ -1098,12 +1149,19 @@ void coarse_to_fine(fvm_amg *h, Level &L) {
 void coarse_operator(fvm_amg *h, Level &F, Level &C) {
     hipStream_t s = h->stream;
     const int nc = F.nc;
-    DArr<int> ub((size_t)nc + 1), uboff((size_t)nc + 1), cnt((size_t)nc + 1);
-    k_cc_ub<<<grid_for(nc, BS, 4096), BS, 0, s>>>(nc, F.c2f_row.p, F.c2f_col.p, F.row.p, ub.p);
-    h->ws.scan.exclusive(ub.p, uboff.p, nc + 1, s);
-    DArr<int> scratch(std::max<long long>(F.nnz, 1));
-    k_cc_list<<<grid_for(nc, BS, 4096), BS, 0, s>>>(nc, F.c2f_row.p, F.c2f_col.p, F.row.p, F.col.p,
-                                                    F.ci.p, uboff.p, scratch.p, cnt.p);
+    // lanes per coarse row from the mean number of flattened entries per coarse row
+    const double mavg = nc ? (double)(F.nnz + F.n) / nc : 0.0;
+    const int S = mavg <= 12 ? 16 : (mavg <= 28 ? 32 : 64);
+    const unsigned g = (unsigned)std::min<long long>(((long long)nc * S + 255) / 256, 65536);
+    auto launch = [&](auto kern, int *cnt, const int *crow, int *ccol, double *cdiag, double *coff) {
+        kern<<<std::max(g, 1u), 256, 0, s>>>(nc, F.c2f_row.p, F.c2f_col.p, F.row.p, F.col.p, F.diag.p, F.off.p,
+                                             F.ci.p, cnt, crow, ccol, cdiag, coff);
+        FVM_HIP(hipGetLastError());
+    };
+    DArr<int> cnt((size_t)nc + 1);
+    if (S == 16) launch(k_gal<16, false>, cnt.p, nullptr, nullptr, nullptr, nullptr);
+    else if (S == 32) launch(k_gal<32, false>, cnt.p, nullptr, nullptr, nullptr, nullptr);
+    else launch(k_gal<64, false>, cnt.p, nullptr, nullptr, nullptr, nullptr);
     C.n = nc;
     C.row.alloc((size_t)nc + 1);
     h->ws.scan.exclusive(cnt.p, C.row.p, nc + 1, s);
@@ -1111,10 +1169,9 @@ void coarse_operator(fvm_amg *h, Level &F, Level &C) {
     C.col.alloc(std::max<long long>(C.nnz, 1));
     C.off.alloc(std::max<long long>(C.nnz, 1));
     C.diag.alloc(std::max(nc, 1));
-    k_cc_galerkin<<<grid_for(nc, BS, 4096), BS, 0, s>>>(
-        nc, F.c2f_row.p, F.c2f_col.p, F.row.p, F.col.p, F.diag.p, F.off.p, F.ci.p, uboff.p,
-        scratch.p, C.row.p, C.col.p, C.diag.p, C.off.p);
-    FVM_HIP(hipGetLastError());
+    if (S == 16) launch(k_gal<16, true>, nullptr, C.row.p, C.col.p, C.diag.p, C.off.p);
+    else if (S == 32) launch(k_gal<32, true>, nullptr, C.row.p, C.col.p, C.diag.p, C.off.p);
+    else launch(k_gal<64, true>, nullptr, C.row.p, C.col.p, C.diag.p, C.off.p);
 }
 
 void alloc_vectors(Level &L) {
