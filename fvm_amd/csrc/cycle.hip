@@ -169,11 +169,13 @@ constexpr int SELL_U = 8;
 // Non-temporal matrix loads on the nt_levels finest levels (SNT): they stream
 // far more than the caches hold and would evict the coarse levels, which are
 // re-read every cycle.
-#define FVM_SELL_U3(u_, CALL)                                    \
-    do {                                                         \
-        if ((u_) >= 32) { constexpr int SU = 32; CALL; }         \
-        else if ((u_) >= 16) { constexpr int SU = 16; CALL; }    \
-        else { constexpr int SU = 8; CALL; }                     \
+// FP64 values take at most U = 16 (32 doubles of values and gathers per lane would
+// cost occupancy on the large levels).
+#define FVM_SELL_U3(u_, CALL)                                                          \
+    do {                                                                               \
+        if ((u_) >= 32 && sizeof(*val) < 8) { constexpr int SU = 32; CALL; }           \
+        else if ((u_) >= 16) { constexpr int SU = 16; CALL; }                          \
+        else { constexpr int SU = 8; CALL; }                                           \
     } while (0)
 #define FVM_SELL_U(u_, l_, CALL)                                                        \
     do {                                                                                \
