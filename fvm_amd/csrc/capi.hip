@@ -75,6 +75,7 @@ void check_params(const fvm_amg_params *p) {
 
 struct fvm_fgmres {
     fvm_amg *pc = nullptr;
+    int device = 0;               // of pc, kept so that destroy never reads pc (it may be gone)
     fvm_fgmres_params prm{};
     fvm::KrylovWork *work = nullptr;
     std::vector<double> hist;
@@ -618,6 +619,7 @@ int fvm_fgmres_create(fvm_fgmres **out, fvm_amg *pc) {
     FVM_CHECK(out && pc, FVM_E_INVALID, "null argument");
     fvm_fgmres *s = new fvm_fgmres();
     s->pc = pc;
+    s->device = pc->device;
     fvm_fgmres_default_params(&s->prm);
     s->work = fvm::new_krylov_work();
     *out = s;
@@ -628,8 +630,10 @@ int fvm_fgmres_create(fvm_fgmres **out, fvm_amg *pc) {
 int fvm_fgmres_destroy(fvm_fgmres *s) {
     FVM_API_BEGIN
     if (!s) return FVM_OK;
-    if (s->pc) {
-        HandleScope g(s->pc);
+    {
+        // the Krylov arrays are plain allocations of s->device (not from the
+        // preconditioner's pool): freeing them needs no access to pc
+        DeviceGuard g(s->device);
         fvm::delete_krylov_work(s->work);
     }
     delete s;

@@ -919,6 +919,9 @@ void solve_device(fvm_amg *h, const double *d_b, double *d_x, double *rnorm0, do
             h->total_iterations++;
             one_cycle(R);
             rn = finest_residual_norm(R);
+            FVM_CHECK(std::isfinite(rn), FVM_E_NUMERIC,
+                      "the cycle produced a non-finite residual (cycle " + std::to_string(i) +
+                          "); in mixed precision the W-cycle can be unstable on deep hierarchies: use V or F, or FP64");
             double ratio = rn;
             if (r0 != 0) ratio /= r0;
             it = i;
@@ -1639,6 +1642,7 @@ void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const d
     };
     residual_into_r();
     const double r0 = norm1_of(R, W.r.p, n);
+    FVM_CHECK(std::isfinite(r0), FVM_E_NUMERIC, "FGMRES: non-finite initial residual");
     hist.assign(1, r0);
     double rn = r0;
     int total = 0;
@@ -1652,6 +1656,7 @@ void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const d
         std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hc, hbuf, y;
         for (;;) {
             const double beta = norm2_of(R, W.r.p, n);
+            FVM_CHECK(std::isfinite(beta), FVM_E_NUMERIC, "FGMRES: non-finite residual at a restart");
             if (!(beta > 0)) break;
             // The stop rule is on the 1-norm of the true residual, the Arnoldi
             // estimate is a 2-norm: aim the estimate at the reduction the 1-norm
@@ -1736,6 +1741,10 @@ void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const d
             multi_axpy(R, W, W.Z.p, jend, y, 1.0, W.x.p);
             residual_into_r();
             rn = norm1_of(R, W.r.p, n);
+            FVM_CHECK(std::isfinite(rn), FVM_E_NUMERIC,
+                      "FGMRES produced a non-finite residual (iteration " + std::to_string(total) +
+                          "); in mixed precision the W-cycle preconditioner can be unstable on deep hierarchies: "
+                          "use V or F, or FP64");
             hist.push_back(rn);
             if (p.verbosity > 0) printf("%d: %g\n", total, rn);
             double ratio = rn;

@@ -209,7 +209,9 @@ typedef struct fvm_fgmres_params {
 typedef struct fvm_fgmres fvm_fgmres;
 
 int fvm_fgmres_default_params(fvm_fgmres_params *p);
-/* The preconditioner handle must have been set up; it is borrowed, not owned. */
+/* The preconditioner handle must have been set up; it is borrowed, not owned: do
+ * not solve after it is destroyed. Destroying the solver never touches it, so the
+ * two may be destroyed in either order. */
 int fvm_fgmres_create(fvm_fgmres **out, fvm_amg *preconditioner);
 int fvm_fgmres_destroy(fvm_fgmres *s);
 int fvm_fgmres_set_params(fvm_fgmres *s, const fvm_fgmres_params *p);
