@@ -406,10 +406,11 @@ __device__ __forceinline__ void tail_rows(const TailLv &L, int beg, int end, con
         const int s = p >> 6, lane = p & 63;
         const int o = L.soff[s];
         const int w = (L.soff[s + 1] - o) >> 6;
-        // wide coarse rows: deeper load chunks (same summation order)
-        const X acc = w > 16 ? sell_row<32, false>(b[p], L.scol, val, o, w, lane, xin)
-                    : w > 8  ? sell_row<16, false>(b[p], L.scol, val, o, w, lane, xin)
-                             : sell_row<8, false>(b[p], L.scol, val, o, w, lane, xin);
+        // wide coarse rows: deeper load chunks (same summation order), at most 16 deep
+        // and 8 for FP64 values (deeper chunks spill registers of this 1024-thread kernel)
+        X acc;
+        if (sizeof(V) < 8 && w > 8) acc = sell_row<16, false>(b[p], L.scol, val, o, w, lane, xin);
+        else acc = sell_row<8, false>(b[p], L.scol, val, o, w, lane, xin);
         xout[p] = -acc / (X)dg[p];
     }
 }
