@@ -1465,6 +1465,43 @@ __global__ void __launch_bounds__(BS) k_multidot8(int n, const double *__restric
     }
 }
 
+// First classical Gram-Schmidt update and the dots of the second pass in one sweep
+// over the basis: w' = w - sum_j c_j V_j, then the partial sums of V_j . w' into
+// slots j (the second pass reads the updated w element by element, so its dots
+// need no sweep of their own). Up to KMAX basis vectors.
+template <int KMAX>
+__global__ void __launch_bounds__(BS) k_axpy_dots(int n, const double *__restrict__ V, int k,
+                                                  const double *__restrict__ c, double *w, double *part,
+                                                  int nblk) {
+    double cj[KMAX], acc[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) {
+        cj[j] = j < k ? c[j] : 0.0;
+        acc[j] = 0.0;
+    }
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        double vj[KMAX];
+        double s = 0;
+#pragma unroll
+        for (int j = 0; j < KMAX; j++) {
+            vj[j] = j < k ? V[(size_t)j * n + i] : 0.0;
+            if (j < k) s += cj[j] * vj[j];
+        }
+        const double v = w[i] - s;
+        w[i] = v;
+#pragma unroll
+        for (int j = 0; j < KMAX; j++)
+            if (j < k) acc[j] += vj[j] * v;
+    }
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) {
+        if (j < k) {
+            const double t = block_sum<BS>(acc[j]);
+            if (threadIdx.x == 0) part[(size_t)j * nblk + blockIdx.x] = t;
+        }
+    }
+}
+
 // out[b] = sum of the partials of slot b (b < k), out[k] = those of slot nslot
 __global__ void __launch_bounds__(BS) k_multi_final(int nblk, const double *part, double *out, int k, int nslot) {
     const int j = (int)blockIdx.x < k ? (int)blockIdx.x : nslot;
@@ -1481,6 +1518,7 @@ __global__ void __launch_bounds__(BS) k_multi_axpy_norm(int n, const double *__r
     double ww = 0;
     for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
         double s = 0;
+#pragma unroll 4
         for (int j = 0; j < k; j++) s += c[j] * V[(size_t)j * n + i];
         const double v = w[i] + sign * s;
         w[i] = v;
@@ -1516,6 +1554,10 @@ __global__ void __launch_bounds__(BS) k_norm2sq_partial(int n, const double *__r
 }
 
 }  // namespace
+
+// blocks of the Krylov basis kernels (dots, updates): enough resident wavefronts
+// per CU to keep HBM busy with their independent streams
+constexpr unsigned KRYLOV_BLOCKS = 2048;
 
 struct KrylovWork {
     int n = 0, m = 0;
@@ -1584,9 +1626,21 @@ void multidot_dev(Run &R, KrylovWork &W, int k, const double *w, double *out) {
     fvm_amg *h = R.h;
     hipStream_t s = h->stream;
     const int n = W.n;
-    const unsigned g = grid_for(n, BS, 512);
+    const unsigned g = grid_for(n, BS, KRYLOV_BLOCKS);
     for (int j0 = 0; j0 < k; j0 += 8)
         k_multidot8<<<g, BS, 0, s>>>(n, W.V.p, j0, std::min(8, k - j0), w, W.part.p, (int)g, -1);
+    k_multi_final<<<k, BS, 0, s>>>((int)g, W.part.p, out, k, 0);
+    FVM_HIP(hipGetLastError());
+}
+
+// w -= sum_j c_j V_j (c on the device), and out[0..k) = V_j . (the new w)
+void axpy_dots_dev(Run &R, KrylovWork &W, int k, const double *c, double *w, double *out) {
+    fvm_amg *h = R.h;
+    hipStream_t s = h->stream;
+    const unsigned g = grid_for(W.n, BS, KRYLOV_BLOCKS);
+    if (k <= 8) k_axpy_dots<8><<<g, BS, 0, s>>>(W.n, W.V.p, k, c, w, W.part.p, (int)g);
+    else if (k <= 16) k_axpy_dots<16><<<g, BS, 0, s>>>(W.n, W.V.p, k, c, w, W.part.p, (int)g);
+    else k_axpy_dots<32><<<g, BS, 0, s>>>(W.n, W.V.p, k, c, w, W.part.p, (int)g);
     k_multi_final<<<k, BS, 0, s>>>((int)g, W.part.p, out, k, 0);
     FVM_HIP(hipGetLastError());
 }
@@ -1596,7 +1650,7 @@ void multidot_dev(Run &R, KrylovWork &W, int k, const double *w, double *out) {
 void axpy_dev(Run &R, KrylovWork &W, int k, const double *c, double *w, double *ww) {
     fvm_amg *h = R.h;
     hipStream_t s = h->stream;
-    const unsigned g = grid_for(W.n, BS, 512);
+    const unsigned g = grid_for(W.n, BS, KRYLOV_BLOCKS);
     const int nslot = W.m + 1;
     k_multi_axpy_norm<<<g, BS, 0, s>>>(W.n, W.V.p, k, c, -1.0, w, W.part.p, (int)g, nslot);
     if (ww) k_multi_final<<<1, BS, 0, s>>>((int)g, W.part.p, ww, 0, nslot);
@@ -1629,7 +1683,7 @@ void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const d
         W.Z.alloc((size_t)m * std::max(n, 1));
         W.w.alloc(std::max(n, 1)); W.b.alloc(std::max(n, 1)); W.x.alloc(std::max(n, 1));
         W.r.alloc(std::max(n, 1));
-        W.part.alloc((size_t)(m + 2) * 512);
+        W.part.alloc((size_t)(m + 2) * KRYLOV_BLOCKS);
         W.dev_h.alloc(2 * (size_t)(m + 1) + 1);
     }
     const int *perm = R.fast ? F.perm.p : nullptr;
@@ -1692,8 +1746,12 @@ void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const d
                 // host reads them once per iteration
                 double *h1 = W.dev_h.p, *h2 = W.dev_h.p + (m + 1), *hw = W.dev_h.p + 2 * (m + 1);
                 multidot_dev(R, W, j + 1, W.w.p, h1);
-                axpy_dev(R, W, j + 1, h1, W.w.p, nullptr);
-                multidot_dev(R, W, j + 1, W.w.p, h2);
+                if (j + 1 <= 32) {
+                    axpy_dots_dev(R, W, j + 1, h1, W.w.p, h2);   // first update + second dots
+                } else {
+                    axpy_dev(R, W, j + 1, h1, W.w.p, nullptr);
+                    multidot_dev(R, W, j + 1, W.w.p, h2);
+                }
                 axpy_dev(R, W, j + 1, h2, W.w.p, hw);
                 hbuf.resize(2 * (size_t)(m + 1) + 1);
                 FVM_HIP(hipMemcpyAsync(hbuf.data(), W.dev_h.p, sizeof(double) * hbuf.size(), hipMemcpyDeviceToHost, s));

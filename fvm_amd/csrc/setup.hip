@@ -1172,6 +1172,10 @@ void coarse_operator(fvm_amg *h, Level &F, Level &C) {
     if (S == 16) launch(k_gal<16, true>, nullptr, C.row.p, C.col.p, C.diag.p, C.off.p);
     else if (S == 32) launch(k_gal<32, true>, nullptr, C.row.p, C.col.p, C.diag.p, C.off.p);
     else launch(k_gal<64, true>, nullptr, C.row.p, C.col.p, C.diag.p, C.off.p);
+    // a structurally symmetric fine pattern gives a symmetric coarse one: (I, J) is an
+    // entry iff some a_ij with i in I, j in J is, and a_ji is then one too (rows
+    // outside every aggregate drop out of both sides alike)
+    if (F.sym == 1) C.sym = 1;
 }
 
 void alloc_vectors(Level &L) {
