@@ -137,7 +137,8 @@ struct fvm_amg {
     // tuning knobs (fvm_amg_set_tuning): rows below which a level runs in the
     // single-workgroup tail of the V-cycle / is coarsened by one workgroup
     long long tail_max_rows = 40000;
-    long long coarsen_small_rows = 16384;
+    long long coarsen_small_rows = 8192;
+    int coarsen_lanes = 0;             // lanes per row of the agglomeration rounds (0: by row width)
     int coarsen_coop = 1;              // larger levels: all rounds in one cooperative launch
     int sell_block = 256;              // threads per block of the SELL-64 smoother kernels
     int spmv_block = 256;              // threads per block of the SELL-64 residual / SpMV kernels

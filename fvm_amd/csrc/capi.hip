@@ -412,6 +412,10 @@ int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value) {
         (k == "sell_xcd" ? h->sell_xcd : h->spmv_xcd) = value != 0;
     } else if (k == "cycle_graph") {
         h->cycle_graph = value != 0;
+    } else if (k == "coarsen_lanes") {
+        FVM_CHECK(value == 0 || value == 16 || value == 32 || value == 64, FVM_E_INVALID,
+                  "coarsen_lanes must be 0, 16, 32 or 64");
+        h->coarsen_lanes = (int)value;
     } else if (k == "nt_levels") {
         FVM_CHECK(value >= 0 && value <= 64, FVM_E_INVALID, "nt_levels must be in [0, 64]");
         h->nt_levels = (int)value;
@@ -435,6 +439,7 @@ int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value) {
     else if (k == "spmv_xcd") *value = h->spmv_xcd;
     else if (k == "cycle_graph") *value = h->cycle_graph;
     else if (k == "nt_levels") *value = h->nt_levels;
+    else if (k == "coarsen_lanes") *value = h->coarsen_lanes;
     else return FVM_E_INVALID;
     return FVM_OK;
 }

@@ -92,7 +92,7 @@ __global__ void k_compact_fill(int n_self, const int *row, const int *col, const
 
 constexpr int INIT_SUB = 4;    // lanes per row in k_cnt_init
 constexpr int ROUND_SUB = 64;  // lanes per frontier row in k_coarsen_round (one wavefront)
-constexpr int THIN_SUB = 16;   // the same for levels of at most 8 entries per row
+constexpr int THIN_SUB = 16;   // the same for levels of at most 10 entries per row
 constexpr int SMALL_SUB = 64;  // lanes per frontier row in k_coarsen_small (one wavefront)
 
 // append x to the queue when want; must be reached by every active lane
@@ -1083,10 +1083,11 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
         // check of batch b runs while batch b + 1 is queued (a round on an empty
         // frontier only zeroes a counter)
         path = "multi-block";
-        // lanes per row: 16 for thin rows (finest FV levels, four rows per
-        // wavefront), 32 for large levels of moderate rows (their frontiers are
-        // thousands of rows wide), else one wavefront per row
-        const int sub = L.nnz <= 8LL * n ? THIN_SUB : (L.nnz <= 16LL * n && n >= (1 << 20)) ? 32 : ROUND_SUB;
+        // lanes per row (measured on the 200^3 hierarchy): 16 for rows of at most 10
+        // entries (four rows per wavefront), 32 for large levels of moderate rows
+        // (their frontiers are thousands of rows wide), else one wavefront per row
+        int sub = L.nnz <= 10LL * n ? THIN_SUB : (L.nnz <= 16LL * n && n >= (1 << 20)) ? 32 : ROUND_SUB;
+        if (h->coarsen_lanes) sub = h->coarsen_lanes;   // tuning override (16, 32 or 64)
         const unsigned g_round = sub < ROUND_SUB ? 2048 : 1024;
         const int batch = 32;
         h->probe.init();
