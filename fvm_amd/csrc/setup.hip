@@ -935,19 +935,41 @@ __global__ void k_dense_from_csr(int n, const int *row, const int *col, const do
 __global__ void k_gauss_jordan(int n, double *A, double *Inv, int *status) {
     __shared__ int piv;
     __shared__ double pval;
+    __shared__ double s_v[16];
+    __shared__ int s_i[16];
     for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x)
         Inv[idx] = ((idx / n) == (idx % n)) ? 1.0 : 0.0;
     __syncthreads();
     for (int k = 0; k < n; k++) {
-        if (threadIdx.x == 0) {
-            int best = k;
-            double bv = fabs(A[(size_t)k * n + k]);
-            for (int i = k + 1; i < n; i++) {
-                const double v = fabs(A[(size_t)i * n + k]);
-                if (v > bv) { bv = v; best = i; }
+        // pivot: the first row i >= k of largest |A[i][k]| (rows below k with a NaN
+        // are skipped; a NaN on the diagonal keeps k and stops below), by a
+        // block-wide argmax (n <= blockDim.x)
+        {
+            const int i = threadIdx.x;
+            double v = -1.0;
+            int bi = INT_MAX;
+            if (i >= k && i < n) {
+                const double a = fabs(A[(size_t)i * n + k]);
+                if (i == k) v = isnan(a) ? INFINITY : a;
+                else v = isnan(a) ? -1.0 : a;
+                bi = i;
             }
-            piv = best;
-            pval = bv;
+            for (int o = 32; o > 0; o >>= 1) {
+                const double v2 = __shfl_xor(v, o, 64);
+                const int i2 = __shfl_xor(bi, o, 64);
+                if (v2 > v || (v2 == v && i2 < bi)) { v = v2; bi = i2; }
+            }
+            const int wv = threadIdx.x >> 6;
+            if ((threadIdx.x & 63) == 0) { s_v[wv] = v; s_i[wv] = bi; }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double bv = s_v[0];
+                int best = s_i[0];
+                for (int w = 1; w < (int)(blockDim.x >> 6); w++)
+                    if (s_v[w] > bv || (s_v[w] == bv && s_i[w] < best)) { bv = s_v[w]; best = s_i[w]; }
+                piv = best;
+                pval = fabs(A[(size_t)best * n + k]);
+            }
         }
         __syncthreads();
         if (!(pval > 0.0) || !isfinite(pval)) {
