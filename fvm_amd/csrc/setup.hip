@@ -794,7 +794,9 @@ __global__ void k_color_round(int t, int n, const int *row, const int *col, cons
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (cround[i] < t) continue;
         const int base = seg_base(seg_off, nseg, i);
-        const unsigned pi = hash32((unsigned)(i - base));
+        // priority: larger row first (largest-first ordering needs fewer colours),
+        // then the hash of the row's index inside its system
+        const unsigned long long pi = ((unsigned long long)(row[i + 1] - row[i]) << 32) | hash32((unsigned)(i - base));
         bool best = true;
         unsigned long long used[4] = {0, 0, 0, 0};
         for (int pass = 0; pass < 2 && best; pass++) {
@@ -807,7 +809,8 @@ __global__ void k_color_round(int t, int n, const int *row, const int *col, cons
                     const int cj = color[j];
                     if (cj < 256) used[cj >> 6] |= 1ull << (cj & 63);
                 } else {
-                    const unsigned pj = hash32((unsigned)(j - base));
+                    const unsigned long long pj =
+                        ((unsigned long long)(row[j + 1] - row[j]) << 32) | hash32((unsigned)(j - base));
                     if (pj > pi || (pj == pi && j > i)) { best = false; break; }
                 }
             }
