@@ -303,6 +303,39 @@ __global__ void k_dep(int n, const int *row, const int *col, const int *rowT, co
     }
 }
 
+// Per-row state of the agglomeration rounds in one 32-byte record, so that reading
+// a neighbour (its diagonal, state, group and dependents range) touches one cache
+// line instead of four: the decisions of the large levels are bound by such random
+// line fetches.
+struct alignas(32) RowRec {
+    double ad;      // |diagonal|
+    int dround;     // round the row was grouped in (INT_MAX: not yet, -1: boundary)
+    int ldr;        // leader (seed) of its group, -1 while ungrouped
+    int db, de;     // range of its dependents list
+    int pad0, pad1;
+};
+
+__global__ void k_rec_pack(int n, const double *diag, const int *dptr, const int *dround, const int *ldr,
+                           RowRec *rec) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        RowRec r;
+        r.ad = fabs(diag[i]);
+        r.dround = dround[i];
+        r.ldr = ldr[i];
+        r.db = dptr[i];
+        r.de = dptr[i + 1];
+        r.pad0 = r.pad1 = 0;
+        rec[i] = r;
+    }
+}
+
+__global__ void k_rec_unpack(int n, const RowRec *rec, int *dround, int *ldr) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        dround[i] = rec[i].dround;
+        ldr[i] = rec[i].ldr;
+    }
+}
+
 // Decide one frontier row with the S lanes of its sub-group. Returns via the
 // queue (fan-out) and the shared state arrays. Loads that a later step may need
 // (the first dependents of r, the dependents range of every neighbour, the group
@@ -310,15 +343,15 @@ __global__ void k_dep(int n, const int *row, const int *col, const int *rowT, co
 // a decision costs few round trips to memory.
 template <int S, bool WG>
 __device__ void decide_row(int r, int t, double theta, const int *row, const int *col, const int *rowT,
-                           const int *colT, const double *off, const double *diag, const uint8_t *bnd,
-                           const int *dptr, const int *dep, int *ldr, int *dround, int *mem, int *memr,
-                           int *mcnt, int *cnt, int *q, int *qcnt, int *nretry) {
+                           const int *colT, const double *off, const uint8_t *bnd, RowRec *rec,
+                           const int *dep, int *mem, int *memr, int *mcnt, int *cnt, int *q, int *qcnt,
+                           int *nretry) {
     const int lane = threadIdx.x & (S - 1);
     // state, row range, dependents range and diagonal of r in one step
-    const int dr = dround[r];
+    const RowRec rr = rec[r];
     const int rb = row[r], re = row[r + 1];
-    const int db = dptr[r], de = dptr[r + 1];
-    const double d0 = fabs(diag[r]);
+    const int dr = rr.dround, db = rr.db, de = rr.de;
+    const double d0 = rr.ad;
     if (dr < t) return;   // already paired by an earlier seed (its fan-out is done)
     const int pre = db + lane < de ? dep[db + lane] : 0;   // first S dependents of r
     // strongest ungrouped (u) and grouped (g) neighbours; ties: first in CSR order
@@ -328,9 +361,10 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
         const int c = col[p];
         const double o = off[p];
         if (isb(bnd, c)) continue;
-        const double d1 = fabs(diag[c]);
-        const int dc = dround[c];
-        const int cb = dptr[c], ce = dptr[c + 1], lc = ldr[c];
+        const RowRec rc = rec[c];
+        const double d1 = rc.ad;
+        const int dc = rc.dround;
+        const int cb = rc.db, ce = rc.de, lc = rc.ldr;
         const double w = fabs(fabs(o) / (d0 < d1 ? d1 : d0));
         if (!(dc < t)) {
             if (pu == INT_MAX || w > mu) { mu = w; pu = p; cu = c; ub = cb; ue = ce; }
@@ -383,7 +417,7 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
         walk_ranges<S>(tb, tl, [&](int, int idx) {
             if (idx >= 0) {
                 const int s = colT[idx];
-                if (s < r && !isb(bnd, s) && !(dround[s] < t)) blocked = true;
+                if (s < r && !isb(bnd, s) && !(rec[s].dround < t)) blocked = true;
             }
         });
         // every lane of the sub-group takes part in the vote
@@ -400,21 +434,21 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
     }
     if (lane == 0) {
         if (action == 0) {
-            ldr[r] = r; ldr[cu] = r;
+            rec[r].ldr = r; rec[cu].ldr = r;
             mem[(size_t)r * MEMW] = r; memr[(size_t)r * MEMW] = t;
             mem[(size_t)r * MEMW + 1] = cu; memr[(size_t)r * MEMW + 1] = t;
             mcnt[r] = 2;
-            dround[cu] = t; dround[r] = t;
+            rec[cu].dround = t; rec[r].dround = t;
         } else if (action == 1) {
-            ldr[r] = r;
+            rec[r].ldr = r;
             mem[(size_t)r * MEMW] = r; memr[(size_t)r * MEMW] = t;
             mcnt[r] = 1;
-            dround[r] = t;
+            rec[r].dround = t;
         } else {
             mem[(size_t)g * MEMW + slot] = r; memr[(size_t)g * MEMW + slot] = t;
             mcnt[g] = slot + 1;
-            ldr[r] = g;
-            dround[r] = t;
+            rec[r].ldr = g;
+            rec[r].dround = t;
         }
     }
     // fan-out: the dependents of r and, for a pair, of its partner
@@ -439,9 +473,9 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
 template <int S>
 __global__ void __launch_bounds__(256)
 k_coarsen_round(int t, int n, const int *row, const int *col, const int *rowT, const int *colT,
-                const double *off, const double *diag, const uint8_t *bnd, const int *dptr, const int *dep,
-                double theta, int *ldr, int *dround, int *mem, int *memr, int *mcnt, int *cnt, const int *cur,
-                const int *cur_cnt, int *nxt, int *nxt_cnt, int *zero_cnt, int *nretry) {
+                const double *off, const uint8_t *bnd, RowRec *rec, const int *dep, double theta, int *mem,
+                int *memr, int *mcnt, int *cnt, const int *cur, const int *cur_cnt, int *nxt, int *nxt_cnt,
+                int *zero_cnt, int *nretry) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *zero_cnt = 0;
     // one sub-group of S lanes per frontier row
     const int sub = (blockIdx.x * blockDim.x + threadIdx.x) / S;
@@ -455,8 +489,8 @@ k_coarsen_round(int t, int n, const int *row, const int *col, const int *rowT, c
     for (int base = sub - (sub % (64 / S)); base < nf; base += nsub) {
         const int i = base + (sub % (64 / S));
         if (i < nf)
-            decide_row<S, false>(i == sub ? r0 : cur[i], t, theta, row, col, rowT, colT, off, diag, bnd, dptr,
-                                 dep, ldr, dround, mem, memr, mcnt, cnt, nxt, nxt_cnt, nretry);
+            decide_row<S, false>(i == sub ? r0 : cur[i], t, theta, row, col, rowT, colT, off, bnd, rec, dep,
+                                 mem, memr, mcnt, cnt, nxt, nxt_cnt, nretry);
     }
 }
 
@@ -467,9 +501,9 @@ k_coarsen_round(int t, int n, const int *row, const int *col, const int *rowT, c
 // sizes are LDS counters, the dependency counters stay in global memory.
 __global__ void __launch_bounds__(1024)
 k_coarsen_small(int n, int max_rounds, const int *row, const int *col, const int *rowT, const int *colT,
-                const double *off, const double *diag, const uint8_t *bnd, const int *dptr, const int *dep,
-                double theta, int *ldr, int *dround, int *mem, int *memr, int *mcnt, int *cnt, int *qa, int *qb,
-                const int *qcnt0, int *rounds_out, int *nretry) {
+                const double *off, const uint8_t *bnd, RowRec *rec, const int *dep, double theta, int *mem,
+                int *memr, int *mcnt, int *cnt, int *qa, int *qb, const int *qcnt0, int *rounds_out,
+                int *nretry) {
     __shared__ int s_q[2];   // s_q[t & 1]: size of the frontier of round t
     int *cur = qa, *nxt = qb;
     if (threadIdx.x == 0) {
@@ -489,8 +523,8 @@ k_coarsen_small(int n, int max_rounds, const int *row, const int *col, const int
         for (int base = wave * per_wave; base < nf; base += nwaves * per_wave) {
             const int i = base + (__lane_id() / SMALL_SUB);
             if (i < nf)
-                decide_row<SMALL_SUB, true>(cur[i], t, theta, row, col, rowT, colT, off, diag, bnd, dptr, dep,
-                                            ldr, dround, mem, memr, mcnt, cnt, nxt, &s_q[(t + 1) & 1], nretry);
+                decide_row<SMALL_SUB, true>(cur[i], t, theta, row, col, rowT, colT, off, bnd, rec, dep, mem,
+                                            memr, mcnt, cnt, nxt, &s_q[(t + 1) & 1], nretry);
         }
         __syncthreads();
         if (threadIdx.x == 0) s_q[t & 1] = 0;   // every thread has read it: the counter of round t + 2
@@ -1092,15 +1126,17 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
         k_dep<true><<<g, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, dptr.p, dep.p);
         FVM_HIP(hipGetLastError());
     }
+    DArr<RowRec> rec(std::max(n, 1));
+    k_rec_pack<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.diag.p, dptr.p, dround.p, ldr.p, rec.p);
+    FVM_HIP(hipGetLastError());
     int t = 1;
     const long long cap = 4LL * n + 64;
     const char *path;
     if (n <= h->coarsen_small_rows) {
         path = "single-workgroup";
         k_coarsen_small<<<1, 1024, 0, s>>>(n, (int)std::min<long long>(cap, INT_MAX - 1), L.row.p, L.col.p,
-                                           rowT, colT, L.off.p, L.diag.p, bnd, dptr.p, dep.p, theta, ldr.p,
-                                           dround.p, mem.p, memr.p, mcnt.p, cnt.p, q1, q0, qc + 1, rounds_out,
-                                           ctr.p + 6);
+                                           rowT, colT, L.off.p, bnd, rec.p, dep.p, theta, mem.p, memr.p, mcnt.p,
+                                           cnt.p, q1, q0, qc + 1, rounds_out, ctr.p + 6);
         FVM_HIP(hipGetLastError());
         t = read_scalar(rounds_out, s);
     } else {
@@ -1121,9 +1157,9 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
             for (int k = 0; k < batch; k++, t++) {
                 auto launch = [&](auto kern) {
                     kern<<<g_round, 256, 0, s>>>(
-                        t, n, L.row.p, L.col.p, rowT, colT, L.off.p, L.diag.p, bnd, dptr.p, dep.p, theta, ldr.p,
-                        dround.p, mem.p, memr.p, mcnt.p, cnt.p, (t & 1) ? q1 : q0, qc + (t % 3),
-                        (t & 1) ? q0 : q1, qc + ((t + 1) % 3), qc + ((t + 2) % 3), ctr.p + 6);
+                        t, n, L.row.p, L.col.p, rowT, colT, L.off.p, bnd, rec.p, dep.p, theta, mem.p, memr.p,
+                        mcnt.p, cnt.p, (t & 1) ? q1 : q0, qc + (t % 3), (t & 1) ? q0 : q1, qc + ((t + 1) % 3),
+                        qc + ((t + 2) % 3), ctr.p + 6);
                 };
                 if (sub == THIN_SUB) launch(k_coarsen_round<THIN_SUB>);
                 else if (sub == 32) launch(k_coarsen_round<32>);
@@ -1136,6 +1172,7 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
             if (t >= cap) break;
         }
     }
+    k_rec_unpack<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, rec.p, dround.p, ldr.p);
     {
         DArr<int> und(1);
         FVM_HIP(hipMemsetAsync(und.p, 0, sizeof(int), s));
