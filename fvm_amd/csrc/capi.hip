@@ -413,8 +413,8 @@ int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value) {
     } else if (k == "cycle_graph") {
         h->cycle_graph = value != 0;
     } else if (k == "coarsen_lanes") {
-        FVM_CHECK(value == 0 || value == 16 || value == 32 || value == 64, FVM_E_INVALID,
-                  "coarsen_lanes must be 0, 16, 32 or 64");
+        FVM_CHECK(value == 0 || value == 8 || value == 16 || value == 32 || value == 64, FVM_E_INVALID,
+                  "coarsen_lanes must be 0, 8, 16, 32 or 64");
         h->coarsen_lanes = (int)value;
     } else if (k == "nt_levels") {
         FVM_CHECK(value >= 0 && value <= 64, FVM_E_INVALID, "nt_levels must be in [0, 64]");

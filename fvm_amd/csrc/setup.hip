@@ -1144,10 +1144,13 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
         // check of batch b runs while batch b + 1 is queued (a round on an empty
         // frontier only zeroes a counter)
         path = "multi-block";
-        // lanes per row (measured on the 200^3 hierarchy): 16 for rows of at most 10
-        // entries (four rows per wavefront), 32 for large levels of moderate rows
-        // (their frontiers are thousands of rows wide), else one wavefront per row
-        int sub = L.nnz <= 10LL * n ? THIN_SUB : (L.nnz <= 16LL * n && n >= (1 << 20)) ? 32 : ROUND_SUB;
+        // lanes per row (measured on the 200^3 hierarchy): 8 for rows of at most 7
+        // entries (eight rows per wavefront), 16 up to 10 entries, 32 for large levels
+        // of moderate rows (their frontiers are thousands of rows wide), else one
+        // wavefront per row
+        int sub = L.nnz <= 7LL * n ? 8
+                : L.nnz <= 10LL * n ? THIN_SUB
+                : (L.nnz <= 16LL * n && n >= (1 << 20)) ? 32 : ROUND_SUB;
         if (h->coarsen_lanes) sub = h->coarsen_lanes;   // tuning override (16, 32 or 64)
         const unsigned g_round = sub < ROUND_SUB ? 2048 : 1024;
         const int batch = 32;
@@ -1161,7 +1164,8 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
                         mcnt.p, cnt.p, (t & 1) ? q1 : q0, qc + (t % 3), (t & 1) ? q0 : q1, qc + ((t + 1) % 3),
                         qc + ((t + 2) % 3), ctr.p + 6);
                 };
-                if (sub == THIN_SUB) launch(k_coarsen_round<THIN_SUB>);
+                if (sub == 8) launch(k_coarsen_round<8>);
+                else if (sub == THIN_SUB) launch(k_coarsen_round<THIN_SUB>);
                 else if (sub == 32) launch(k_coarsen_round<32>);
                 else launch(k_coarsen_round<ROUND_SUB>);
             }

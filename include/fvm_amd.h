@@ -173,7 +173,7 @@ int fvm_amg_kernel_timing(const fvm_amg *h, int kind, long long *launches, doubl
  * graph (first cycle after setup, parameter or tuning change) and replayed by
  * every later cycle, 0: launched one by one; "nt_levels" -- the matrices of
  * this many finest levels are read with non-temporal loads (default 2);
- * "coarsen_lanes" -- lanes per row in the agglomeration rounds (16/32/64; 0,
+ * "coarsen_lanes" -- lanes per row in the agglomeration rounds (8/16/32/64; 0,
  * the default: chosen per level from its row width); "coarsen_small_rows" --
  * levels with at most this many rows are agglomerated by one workgroup. */
 int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value);
