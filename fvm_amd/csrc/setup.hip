@@ -1218,7 +1218,7 @@ void coarse_operator(fvm_amg *h, Level &F, Level &C) {
     const int nc = F.nc;
     // lanes per coarse row from the mean number of flattened entries per coarse row
     const double mavg = nc ? (double)(F.nnz + F.n) / nc : 0.0;
-    const int S = mavg <= 12 ? 16 : (mavg <= 28 ? 32 : 64);
+    const int S = mavg <= 24 ? 16 : (mavg <= 40 ? 32 : 64);
     const unsigned g = (unsigned)std::min<long long>(((long long)nc * S + 255) / 256, 65536);
     auto launch = [&](auto kern, int *cnt, const int *crow, int *ccol, double *cdiag, double *coff) {
         kern<<<std::max(g, 1u), 256, 0, s>>>(nc, F.c2f_row.p, F.c2f_col.p, F.row.p, F.col.p, F.diag.p, F.off.p,
