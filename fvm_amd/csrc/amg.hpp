@@ -139,7 +139,6 @@ struct fvm_amg {
     long long tail_max_rows = 40000;
     long long coarsen_small_rows = 8192;
     int coarsen_lanes = 0;             // lanes per row of the agglomeration rounds (0: by row width)
-    int coarsen_coop = 1;              // larger levels: all rounds in one cooperative launch
     int sell_block = 256;              // threads per block of the SELL-64 smoother kernels
     int spmv_block = 256;              // threads per block of the SELL-64 residual / SpMV kernels
     int sell_xcd = 1, spmv_xcd = 0;    // XCD-contiguous block order for those kernel classes
@@ -150,7 +149,6 @@ struct fvm_amg {
     hipGraphExec_t cg_exec[2] = {nullptr, nullptr};
     std::string tune_report;           // last autotuner result (JSON)
     fvm::CounterProbe probe;           // pipelined termination checks of the setup loops
-    int coop_blocks = -1;              // co-resident blocks for that launch (-1: not queried)
     // single-workgroup tail of the V-cycle: levels >= tail_level run in one launch
     int tail_level = -1;
     int tail_prec = -1;                // precision the descriptors were built for
