@@ -303,6 +303,72 @@ __global__ void k_dep(int n, const int *row, const int *col, const int *rowT, co
     }
 }
 
+// Dependents lists, fill pass with S lanes per row: the same entries as k_dep<true>
+// (any order inside a row is fine: decrements commute), written as contiguous runs.
+// The (c, x) pairs of the C part are flattened over the lanes, and each step's kept
+// entries are compacted with a ballot, so one store instruction writes one run.
+template <int S>
+__global__ void k_dep_fill(int n, const int *row, const int *col, const int *rowT, const int *colT, int sym,
+                           const uint8_t *bnd, const int *dptr, int *out) {
+    const int lane = threadIdx.x & (S - 1);
+    const int sub = (blockIdx.x * blockDim.x + threadIdx.x) / S;
+    const int nsub = (gridDim.x * blockDim.x) / S;
+    const unsigned long long smask = sub_mask<S>();
+    const unsigned long long below = (1ull << __lane_id()) - 1ull;
+    const unsigned two = sym ? 0x80000000u : 0u;
+    for (int d = sub; d < n; d += nsub) {
+        if (isb(bnd, d)) continue;   // the same for every lane of the sub-group
+        int pos = dptr[d];
+        const int rb = row[d], re = row[d + 1];
+        for (int b0 = rb; b0 < re; b0 += S) {
+            const int p = b0 + lane;
+            int c = -1, tb = 0, tl = 0;
+            bool a_ok = false;
+            if (p < re) {
+                c = col[p];
+                if (c > d && !isb(bnd, c)) {
+                    a_ok = true;
+                    tb = rowT[c];
+                    tl = rowT[c + 1] - tb;
+                }
+            }
+            // A: x in row(d), x > d
+            const unsigned long long ma = __ballot(a_ok) & smask;
+            if (a_ok) out[pos + __popcll(ma & below)] = (int)((unsigned)c | two);
+            pos += __popcll(ma);
+            // C: x in rowT(c) with d < x < c
+            walk_ranges<S>(tb, tl, [&](int owner, int idx) {
+                const int cj = __shfl(c, owner, S);
+                bool w = false;
+                int x = -1;
+                if (idx >= 0) {
+                    x = colT[idx];
+                    w = x > d && x < cj && !isb(bnd, x);
+                }
+                const unsigned long long mc = __ballot(w) & smask;
+                if (w) out[pos + __popcll(mc & below)] = x;
+                pos += __popcll(mc);
+            });
+        }
+        // B: x in rowT(d), x > d (non-symmetric patterns)
+        if (!sym) {
+            const int tb = rowT[d], te = rowT[d + 1];
+            for (int b0 = tb; b0 < te; b0 += S) {
+                const int p = b0 + lane;
+                bool w = false;
+                int x = -1;
+                if (p < te) {
+                    x = colT[p];
+                    w = x > d && !isb(bnd, x);
+                }
+                const unsigned long long mb = __ballot(w) & smask;
+                if (w) out[pos + __popcll(mb & below)] = x;
+                pos += __popcll(mb);
+            }
+        }
+    }
+}
+
 // Per-row state of the agglomeration rounds in one 32-byte record, so that reading
 // a neighbour (its diagonal, state, group and dependents range) touches one cache
 // line instead of four: the decisions of the large levels are bound by such random
@@ -1123,7 +1189,8 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
     {
         const long long threads = (long long)n * DEP_SUB;
         const unsigned g = (unsigned)((threads + 255) / 256);
-        k_dep<true><<<g, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, dptr.p, dep.p);
+        const unsigned gf = (unsigned)std::max<long long>(1, std::min<long long>(((long long)n * 16 + 255) / 256, 65536));
+        k_dep_fill<16><<<gf, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, dptr.p, dep.p);
         FVM_HIP(hipGetLastError());
     }
     DArr<RowRec> rec(std::max(n, 1));
