@@ -250,64 +250,13 @@ __device__ __forceinline__ void walk_ranges(int beg, int len, F &&f) {
 //   B: x in rowT(d), x > d, non-symmetric only  (d in row(x))
 //   C: c in row(d), c > d: x in rowT(c), d < x < c
 // Boundary rows neither decide nor wait, and are in no list.
-constexpr int DEP_SUB = 4;
-template <bool FILL>
-__global__ void k_dep(int n, const int *row, const int *col, const int *rowT, const int *colT, int sym,
-                      const uint8_t *bnd, const int *dptr, int *out) {
-    const long long gt = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    const int d = (int)(gt / DEP_SUB);
-    const int lane = threadIdx.x & (DEP_SUB - 1);
-    const bool valid = d < n && !isb(bnd, d);
-    const unsigned two = sym ? 0x80000000u : 0u;
-    int k = 0;     // entries of this lane
-    int pos = 0;   // FILL: next slot of this lane
-    for (int pass = 0; pass < (FILL ? 2 : 1); pass++) {
-        if (pass == 1) {
-            int inc = k;   // exclusive prefix over the lanes of the row
-            for (int o = 1; o < DEP_SUB; o <<= 1) {
-                const int v = __shfl_up(inc, o, DEP_SUB);
-                if (lane >= o) inc += v;
-            }
-            pos = (valid ? dptr[d] : 0) + inc - k;
-        }
-        int m = 0;
-        if (valid) {
-            for (int p = row[d] + lane; p < row[d + 1]; p += DEP_SUB) {
-                const int c = col[p];
-                if (c <= d || isb(bnd, c)) continue;
-                if (pass) out[pos++] = (int)((unsigned)c | two);
-                m++;
-                for (int qq = rowT[c]; qq < rowT[c + 1]; qq++) {
-                    const int x = colT[qq];
-                    if (x > d && x < c && !isb(bnd, x)) {
-                        if (pass) out[pos++] = x;
-                        m++;
-                    }
-                }
-            }
-            if (!sym)
-                for (int p = rowT[d] + lane; p < rowT[d + 1]; p += DEP_SUB) {
-                    const int x = colT[p];
-                    if (x > d && !isb(bnd, x)) {
-                        if (pass) out[pos++] = x;
-                        m++;
-                    }
-                }
-        }
-        if (pass == 0) k = m;
-    }
-    if (!FILL) {
-        int tot = k;
-        for (int o = DEP_SUB / 2; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
-        if (d < n && lane == 0) out[d] = tot;
-    }
-}
 
-// Dependents lists, fill pass with S lanes per row: the same entries as k_dep<true>
-// (any order inside a row is fine: decrements commute), written as contiguous runs.
-// The (c, x) pairs of the C part are flattened over the lanes, and each step's kept
-// entries are compacted with a ballot, so one store instruction writes one run.
-template <int S>
+// Dependents lists with S lanes per row: FILL false counts the entries of each row,
+// FILL true writes them (any order inside a row is fine: decrements commute) as
+// contiguous runs. The (c, x) pairs of the C part are flattened over the lanes, and
+// each step's kept entries are compacted with a ballot, so one store instruction
+// writes one run.
+template <int S, bool FILL>
 __global__ void k_dep_fill(int n, const int *row, const int *col, const int *rowT, const int *colT, int sym,
                            const uint8_t *bnd, const int *dptr, int *out) {
     const int lane = threadIdx.x & (S - 1);
@@ -317,8 +266,11 @@ __global__ void k_dep_fill(int n, const int *row, const int *col, const int *row
     const unsigned long long below = (1ull << __lane_id()) - 1ull;
     const unsigned two = sym ? 0x80000000u : 0u;
     for (int d = sub; d < n; d += nsub) {
-        if (isb(bnd, d)) continue;   // the same for every lane of the sub-group
-        int pos = dptr[d];
+        if (isb(bnd, d)) {   // the same for every lane of the sub-group
+            if (!FILL && lane == 0) out[d] = 0;
+            continue;
+        }
+        int pos = FILL ? dptr[d] : 0;   // count pass (FILL false): out[d] = the number of entries
         const int rb = row[d], re = row[d + 1];
         for (int b0 = rb; b0 < re; b0 += S) {
             const int p = b0 + lane;
@@ -334,7 +286,7 @@ __global__ void k_dep_fill(int n, const int *row, const int *col, const int *row
             }
             // A: x in row(d), x > d
             const unsigned long long ma = __ballot(a_ok) & smask;
-            if (a_ok) out[pos + __popcll(ma & below)] = (int)((unsigned)c | two);
+            if (FILL && a_ok) out[pos + __popcll(ma & below)] = (int)((unsigned)c | two);
             pos += __popcll(ma);
             // C: x in rowT(c) with d < x < c
             walk_ranges<S>(tb, tl, [&](int owner, int idx) {
@@ -346,7 +298,7 @@ __global__ void k_dep_fill(int n, const int *row, const int *col, const int *row
                     w = x > d && x < cj && !isb(bnd, x);
                 }
                 const unsigned long long mc = __ballot(w) & smask;
-                if (w) out[pos + __popcll(mc & below)] = x;
+                if (FILL && w) out[pos + __popcll(mc & below)] = x;
                 pos += __popcll(mc);
             });
         }
@@ -362,10 +314,11 @@ __global__ void k_dep_fill(int n, const int *row, const int *col, const int *row
                     w = x > d && !isb(bnd, x);
                 }
                 const unsigned long long mb = __ballot(w) & smask;
-                if (w) out[pos + __popcll(mb & below)] = x;
+                if (FILL && w) out[pos + __popcll(mb & below)] = x;
                 pos += __popcll(mb);
             }
         }
+        if (!FILL && lane == 0) out[d] = pos;
     }
 }
 
@@ -1178,19 +1131,16 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
     {
         DArr<int> dcnt((size_t)n + 1);
         FVM_HIP(hipMemsetAsync(dcnt.p + n, 0, sizeof(int), s));
-        const long long threads = (long long)n * DEP_SUB;
-        const unsigned g = (unsigned)((threads + 255) / 256);
-        k_dep<false><<<g, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, nullptr, dcnt.p);
+        const unsigned gc = (unsigned)std::max<long long>(1, std::min<long long>(((long long)n * 16 + 255) / 256, 65536));
+        k_dep_fill<16, false><<<gc, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, nullptr, dcnt.p);
         h->ws.scan.exclusive(dcnt.p, dptr.p, n + 1, s);
     }
     const int ndep = read_scalar(dptr.p + n, s);
     FVM_CHECK(ndep >= 0, FVM_E_INVALID, "dependents lists exceed 2^31 entries");
     DArr<int> dep((size_t)std::max(ndep, 1));
     {
-        const long long threads = (long long)n * DEP_SUB;
-        const unsigned g = (unsigned)((threads + 255) / 256);
         const unsigned gf = (unsigned)std::max<long long>(1, std::min<long long>(((long long)n * 16 + 255) / 256, 65536));
-        k_dep_fill<16><<<gf, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, dptr.p, dep.p);
+        k_dep_fill<16, true><<<gf, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, dptr.p, dep.p);
         FVM_HIP(hipGetLastError());
     }
     DArr<RowRec> rec(std::max(n, 1));
