@@ -836,20 +836,26 @@ __device__ __forceinline__ int seg_base(const int *off, int K, int i) {
     return off[lo];
 }
 
+// Colouring priority of each row: larger row first (largest-first ordering needs
+// fewer colours), then the hash of the row's index inside its segment, so each
+// system of a batch gets the colouring it gets alone (neighbours always share the
+// segment). Computed once per level, so that a round reads one word per neighbour.
+__global__ void k_color_prio(int n, const int *row, const int *seg_off, int nseg, unsigned long long *prio) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int base = seg_base(seg_off, nseg, i);
+        prio[i] = ((unsigned long long)(row[i + 1] - row[i]) << 32) | hash32((unsigned)(i - base));
+    }
+}
+
 // Jones-Plassmann round: an uncoloured row whose priority beats every uncoloured
 // neighbour's takes the first colour its coloured neighbours leave free.
-// Priorities hash the row's index inside its segment, so each system of a batch
-// gets the colouring it gets alone (neighbours always share the segment).
 __global__ void k_color_round(int t, int n, const int *row, const int *col, const int *rowT,
-                              const int *colT, int *color, int *cround, int *left, const int *seg_off,
-                              int nseg) {
+                              const int *colT, const unsigned long long *prio, int *color, int *cround,
+                              int *left) {
     int done = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (cround[i] < t) continue;
-        const int base = seg_base(seg_off, nseg, i);
-        // priority: larger row first (largest-first ordering needs fewer colours),
-        // then the hash of the row's index inside its system
-        const unsigned long long pi = ((unsigned long long)(row[i + 1] - row[i]) << 32) | hash32((unsigned)(i - base));
+        const unsigned long long pi = prio[i];
         bool best = true;
         unsigned long long used[4] = {0, 0, 0, 0};
         for (int pass = 0; pass < 2 && best; pass++) {
@@ -862,8 +868,7 @@ __global__ void k_color_round(int t, int n, const int *row, const int *col, cons
                     const int cj = color[j];
                     if (cj < 256) used[cj >> 6] |= 1ull << (cj & 63);
                 } else {
-                    const unsigned long long pj =
-                        ((unsigned long long)(row[j + 1] - row[j]) << 32) | hash32((unsigned)(j - base));
+                    const unsigned long long pj = prio[j];
                     if (pj > pi || (pj == pi && j > i)) { best = false; break; }
                 }
             }
@@ -1545,14 +1550,16 @@ void prepare_fast_level(fvm_amg *h, Level &L, int l, hipStream_t s, Workspace &w
     const int *rT = L.sym ? nullptr : L.rowT.p;
     const int *cT = L.sym ? nullptr : L.colT.p;
     DArr<int> color(n), cround(n), left(1);
+    DArr<unsigned long long> prio(n);
+    k_color_prio<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.row.p, seg_off, nseg, prio.p);
     FVM_HIP(hipMemsetAsync(color.p, 0, color.bytes(), s));
     k_fill_int<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, cround.p, INT_MAX);
     k_fill_int<<<1, 1, 0, s>>>(1, left.p, n);
     int t = 1;
     for (;;) {
         for (int k = 0; k < 8; k++, t++)
-            k_color_round<<<grid_for(n, BS, 4096), BS, 0, s>>>(t, n, L.row.p, L.col.p, rT, cT,
-                                                               color.p, cround.p, left.p, seg_off, nseg);
+            k_color_round<<<grid_for(n, BS, 4096), BS, 0, s>>>(t, n, L.row.p, L.col.p, rT, cT, prio.p,
+                                                               color.p, cround.p, left.p);
         FVM_HIP(hipGetLastError());
         if (read_scalar(left.p, s) == 0) break;
         FVM_CHECK(t < 4 * n + 64, FVM_E_NUMERIC, "colouring did not terminate");
