@@ -922,7 +922,7 @@ void solve_device(fvm_amg *h, const double *d_b, double *d_x, double *rnorm0, do
             rn = finest_residual_norm(R);
             FVM_CHECK(std::isfinite(rn), FVM_E_NUMERIC,
                       "the cycle produced a non-finite residual (cycle " + std::to_string(i) +
-                          "); in mixed precision the W-cycle can be unstable on deep hierarchies: use V or F, or FP64");
+                          "): the system holds non-finite or zero-diagonal entries, or the cycle diverged");
             double ratio = rn;
             if (r0 != 0) ratio /= r0;
             it = i;
@@ -1802,8 +1802,7 @@ void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const d
             rn = norm1_of(R, W.r.p, n);
             FVM_CHECK(std::isfinite(rn), FVM_E_NUMERIC,
                       "FGMRES produced a non-finite residual (iteration " + std::to_string(total) +
-                          "); in mixed precision the W-cycle preconditioner can be unstable on deep hierarchies: "
-                          "use V or F, or FP64");
+                          "): the system holds non-finite or zero-diagonal entries, or the preconditioner diverged");
             hist.push_back(rn);
             if (p.verbosity > 0) printf("%d: %g\n", total, rn);
             double ratio = rn;

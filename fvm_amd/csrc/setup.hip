@@ -982,6 +982,32 @@ __global__ void k_to_bf16(long long n, const double *a, uint16_t *b) {
         b[i] = f32_to_bf16_rne((float)a[i]);
 }
 
+// FP32 diagonal of a level whose off-diagonals are stored in BF16: the diagonal takes
+// the rounding error of its row's off-diagonals, so the stored operator keeps the
+// row sums (A times the constant vector) of the FP64 operator. Without it, rounding
+// can tip a nearly singular coarse diffusion row out of diagonal dominance.
+__device__ __forceinline__ double bf16_err(double v) {
+    const uint16_t h = f32_to_bf16_rne((float)v);
+    return v - (double)__uint_as_float(((unsigned)h) << 16);
+}
+
+__global__ void k_diag_comp_csr(int n, const int *row, const double *off, const double *diag, float *d32) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double d = diag[i];
+        for (int e = row[i]; e < row[i + 1]; e++) d += bf16_err(off[e]);
+        d32[i] = (float)d;
+    }
+}
+
+__global__ void k_diag_comp_sell(int n, const int *soff, const double *sval, const double *pdiag, float *d32) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const int o = soff[p >> 6], w = (soff[(p >> 6) + 1] - o) >> 6, lane = p & 63;
+        double d = pdiag[p];
+        for (int k = 0; k < w; k++) d += bf16_err(sval[o + k * 64 + lane]);   // padding holds 0
+        d32[p] = (float)d;
+    }
+}
+
 // ------------------------------------------------------------------ dense inverse
 // Gauss-Jordan with partial pivoting of a dense n x n FP64 matrix in one block.
 __global__ void k_dense_from_csr(int n, const int *row, const int *col, const double *diag,
@@ -1670,7 +1696,8 @@ void prepare_fast(fvm_amg *h) {
     prepare_fast_maps(h);
 }
 
-// FP32 values on the finest level, BF16 off-diagonals (FP32 diagonal) on coarse levels.
+// FP32 values on the finest level, BF16 off-diagonals (FP32 diagonal that keeps the
+// FP64 row sums) on coarse levels.
 void prepare_mixed(fvm_amg *h, bool fast) {
     hipStream_t s = h->stream;
     for (size_t l = 0; l < h->lev.size(); l++) {
@@ -1679,11 +1706,13 @@ void prepare_mixed(fvm_amg *h, bool fast) {
         if (!L.b32.p) { L.b32.alloc(n); L.x32.alloc(n); L.r32.alloc(n); L.t32.alloc(n); }
         if (!fast && !L.mixed_ready_exact) {
             L.diag32.alloc(n);
-            k_to_f32<<<grid_for(L.n, BS, 4096), BS, 0, s>>>(L.n, L.diag.p, L.diag32.p);
             if (l == 0) {
+                k_to_f32<<<grid_for(L.n, BS, 4096), BS, 0, s>>>(L.n, L.diag.p, L.diag32.p);
                 L.off32.alloc(std::max<long long>(L.nnz, 1));
                 k_to_f32<<<grid_for(L.nnz, BS, 4096), BS, 0, s>>>(L.nnz, L.off.p, L.off32.p);
             } else {
+                k_diag_comp_csr<<<grid_for(L.n, BS, 4096), BS, 0, s>>>(L.n, L.row.p, L.off.p, L.diag.p,
+                                                                       L.diag32.p);
                 L.off16.alloc(std::max<long long>(L.nnz, 1));
                 k_to_bf16<<<grid_for(L.nnz, BS, 4096), BS, 0, s>>>(L.nnz, L.off.p, L.off16.p);
             }
@@ -1693,11 +1722,13 @@ void prepare_mixed(fvm_amg *h, bool fast) {
             const int nslice = (L.n + 63) / 64;
             const int total = read_scalar(L.sell_off.p + nslice, s);
             L.pdiag32.alloc(n);
-            k_to_f32<<<grid_for(L.n, BS, 4096), BS, 0, s>>>(L.n, L.pdiag.p, L.pdiag32.p);
             if (l == 0) {
+                k_to_f32<<<grid_for(L.n, BS, 4096), BS, 0, s>>>(L.n, L.pdiag.p, L.pdiag32.p);
                 L.sell_val32.alloc(std::max(total, 1));
                 k_to_f32<<<grid_for(total, BS, 4096), BS, 0, s>>>(total, L.sell_val.p, L.sell_val32.p);
             } else {
+                k_diag_comp_sell<<<grid_for(L.n, BS, 4096), BS, 0, s>>>(L.n, L.sell_off.p, L.sell_val.p,
+                                                                        L.pdiag.p, L.pdiag32.p);
                 L.sell_bf16.alloc(std::max(total, 1));
                 k_to_bf16<<<grid_for(total, BS, 4096), BS, 0, s>>>(total, L.sell_val.p, L.sell_bf16.p);
             }

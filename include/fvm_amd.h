@@ -63,7 +63,8 @@ enum {
 /* arithmetic of the V-cycle (no counterpart in the reference, which is FP64 only) */
 enum {
     FVM_PREC_FP64 = 0,     /* FP64 values and vectors on every level */
-    FVM_PREC_MIXED = 1     /* FP32 vectors, FP32 finest values, BF16 coarse off-diagonals;
+    FVM_PREC_MIXED = 1     /* FP32 vectors, FP32 finest values, BF16 coarse off-diagonals
+                              with an FP32 diagonal that keeps each row's FP64 sum;
                               the outer residual stays FP64 (iterative refinement) */
 };
 
