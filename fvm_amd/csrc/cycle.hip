@@ -371,15 +371,28 @@ __global__ void k_scatter(int n, const int *perm, const double *src, double *dst
         dst[perm ? perm[p] : p] = src[p];
 }
 
-// x = -A^{-1} b on the direct level (dense FP64 inverse in reference order)
+// Row i of A^{-1} b (dense FP64 inverse in reference order) by one wavefront: lane l
+// sums j = l, l + 64, ... in order, then a butterfly over the lanes leaves the same
+// total in every lane. Used by the per-level path and the tail alike, so both give
+// the same bits. The caller's loop over rows must be uniform across the wavefront.
+template <class X>
+__device__ __forceinline__ double dense_row(const double *__restrict__ inv, int n, int i, const X *b,
+                                           const int *iperm, int lane) {
+    double s = 0;
+    for (int j = lane; j < n; j += 64) s += inv[(size_t)i * n + j] * (double)b[iperm ? iperm[j] : j];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    return s;
+}
+
+// x = -A^{-1} b on the direct level, one wavefront per row
 template <class X>
 __global__ void k_dense_apply(int n, const double *inv, const int *perm, const int *iperm, const X *b,
                               X *x) {
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        const int i = perm ? perm[p] : p;
-        double s = 0;
-        for (int j = 0; j < n; j++) s += inv[(size_t)i * n + j] * (double)b[iperm ? iperm[j] : j];
-        x[p] = (X)(-s);
+    const int lane = threadIdx.x & 63;
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    for (int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; p < n; p += nw) {
+        const double s = dense_row(inv, n, perm ? perm[p] : p, b, iperm, lane);
+        if (lane == 0) x[p] = (X)(-s);
     }
 }
 
@@ -488,11 +501,10 @@ k_tail(const TailLv *lv, int l0, int last, int npre, int npost, int jacobi, cons
             const X *b = (const X *)L.b;
             X *x = (X *)L.x;
             const int n = L.n;
-            for (int p = threadIdx.x; p < n; p += blockDim.x) {
-                const int i = L.perm[p];
-                double sum = 0;
-                for (int j = 0; j < n; j++) sum += inv[(size_t)i * n + j] * (double)b[L.iperm[j]];
-                x[p] = (X)(-sum);
+            const int lane = threadIdx.x & 63;
+            for (int p = threadIdx.x >> 6; p < n; p += blockDim.x >> 6) {
+                const double sum = dense_row(inv, n, L.perm[p], b, L.iperm, lane);
+                if (lane == 0) x[p] = (X)(-sum);
             }
         } else {
             tail_fill0<V, D, X>(L);
@@ -778,7 +790,7 @@ void cycle(Run &R, int type, int l, bool x_zero) {
     }
     if (R.mixed && l == h->direct_level) {
         const bool perm = R.fast;
-        k_dense_apply<X><<<grid_for(L.n, 64), 64, 0, s>>>(L.n, h->dense_inv.p, perm ? L.perm.p : nullptr,
+        k_dense_apply<X><<<grid_for((long long)L.n * 64, 256), 256, 0, s>>>(L.n, h->dense_inv.p, perm ? L.perm.p : nullptr,
                                                           perm ? L.iperm.p : nullptr, vb<X>(L), vx<X>(L));
         return;
     }

@@ -1009,7 +1009,7 @@ __global__ void k_diag_comp_sell(int n, const int *soff, const double *sval, con
 }
 
 // ------------------------------------------------------------------ dense inverse
-// Gauss-Jordan with partial pivoting of a dense n x n FP64 matrix in one block.
+// Gauss-Jordan with partial pivoting of a dense n x n FP64 matrix (n <= 256) in one block.
 __global__ void k_dense_from_csr(int n, const int *row, const int *col, const double *diag,
                                  const double *off, double *A) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -1024,6 +1024,7 @@ __global__ void k_gauss_jordan(int n, double *A, double *Inv, int *status) {
     __shared__ double pval;
     __shared__ double s_v[16];
     __shared__ int s_i[16];
+    __shared__ double s_prow[256], s_pinv[256], s_f[256];
     for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x)
         Inv[idx] = ((idx / n) == (idx % n)) ? 1.0 : 0.0;
     __syncthreads();
@@ -1071,22 +1072,36 @@ __global__ void k_gauss_jordan(int n, double *A, double *Inv, int *status) {
         }
         __syncthreads();
         const double inv_p = 1.0 / A[(size_t)k * n + k];
-        __syncthreads();
+        __syncthreads();   // every thread has read the pivot before row k is rescaled
+        // scaled pivot row and the multipliers of column k, staged in LDS: the
+        // elimination then reads only its own elements from memory
         for (int j = threadIdx.x; j < n; j += blockDim.x) {
-            A[(size_t)k * n + j] *= inv_p;
-            Inv[(size_t)k * n + j] *= inv_p;
+            const double a = A[(size_t)k * n + j] * inv_p, v = Inv[(size_t)k * n + j] * inv_p;
+            s_prow[j] = a; s_pinv[j] = v;
+            A[(size_t)k * n + j] = a; Inv[(size_t)k * n + j] = v;
         }
+        for (int i = threadIdx.x; i < n; i += blockDim.x) s_f[i] = A[(size_t)i * n + k];
         __syncthreads();
-        for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
-            const int i = idx / n, j = idx % n;
-            if (i == k) continue;
-            const double f = A[(size_t)i * n + k];
-            if (j != k) A[(size_t)i * n + j] -= f * A[(size_t)k * n + j];
-            Inv[(size_t)i * n + j] -= f * Inv[(size_t)k * n + j];
+        // eight elements per thread at a time: all loads issued before the first store
+        constexpr int B = 8;
+        for (int base = threadIdx.x; base < n * n; base += B * blockDim.x) {
+            double a[B], v[B];
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                const int idx = base + u * blockDim.x;
+                if (idx < n * n) { a[u] = A[idx]; v[u] = Inv[idx]; }
+            }
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                const int idx = base + u * blockDim.x;
+                if (idx >= n * n) continue;
+                const int i = idx / n, j = idx - i * n;
+                if (i == k) continue;
+                const double f = s_f[i];
+                A[idx] = j == k ? 0.0 : a[u] - f * s_prow[j];
+                Inv[idx] = v[u] - f * s_pinv[j];
+            }
         }
-        __syncthreads();
-        for (int i = threadIdx.x; i < n; i += blockDim.x)
-            if (i != k) A[(size_t)i * n + k] = 0.0;
         __syncthreads();
     }
 }
