@@ -683,10 +683,27 @@ __device__ __forceinline__ GalEntry gal_entry(int K, int m, int nf, const int *f
     return e;
 }
 
-template <int S, bool FILL>
+// Flattened entries of each coarse row (fine rows' lengths plus one diagonal each):
+// a bound on its distinct columns, used to place the rows of the one-pass product.
+__global__ void k_gal_bound(int nc, const int *c2f_row, const int *c2f_col, const int *row, int *ub) {
+    for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < nc; I += gridDim.x * blockDim.x) {
+        const int q0 = c2f_row[I], nf = min(c2f_row[I + 1] - q0, MEMW);
+        int m = 0;
+        for (int t = 0; t < nf; t++) {
+            const int f = c2f_col[q0 + t];
+            m += 1 + row[f + 1] - row[f];
+        }
+        ub[I] = m;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ub[nc] = 0;
+}
+
+// One pass: row I's distinct columns and sums go to tcol/tval from toff[I] on (a
+// bound-sized slot), its count to cnt[I] and its diagonal to cdiag[I].
+template <int S>
 __global__ void k_gal(int nc, const int *c2f_row, const int *c2f_col, const int *row, const int *col,
-                      const double *diag, const double *off, const int *ci, int *cnt, const int *crow,
-                      int *ccol, double *cdiag, double *coff) {
+                      const double *diag, const double *off, const int *ci, int *cnt, const int *toff,
+                      int *tcol, double *cdiag, double *tval) {
     const int lane = threadIdx.x & (S - 1);
     const int sub = (blockIdx.x * blockDim.x + threadIdx.x) / S;
     const int nsub = (gridDim.x * blockDim.x) / S;
@@ -730,35 +747,49 @@ __global__ void k_gal(int nc, const int *c2f_row, const int *c2f_col, const int 
             }
             const unsigned long long bal = __ballot(first) & sub_mask<S>();
             const int below = __popcll(bal & ((1ull << __lane_id()) - 1ull));
-            if (FILL) {
-                // the value of a first appearance: every entry of its column, in flattened order
-                double acc = 0.0;
-                for (int e0 = 0; e0 < m; e0 += S) {
-                    const GalEntry o = gal_entry(e0 + lane, m, nf, fi, fo, fb, I, col, diag, off, ci);
-                    for (int k = 0; k < S; k++) {
-                        // every lane takes part in each shuffle
-                        const int Jo = __shfl(o.J, k, S);
-                        const double vo = __shfl(o.v, k, S);
-                        const int isdo = __shfl(o.isd, k, S);
-                        if (first && Jo == me.J) acc += vo;
-                        // the diagonal: lane 0, during the first chunk's pass
-                        if (c0 == 0 && lane == 0 && isdo) d += vo;
-                    }
+            // the value of a first appearance: every entry of its column, in flattened order
+            double acc = 0.0;
+            for (int e0 = 0; e0 < m; e0 += S) {
+                const GalEntry o = gal_entry(e0 + lane, m, nf, fi, fo, fb, I, col, diag, off, ci);
+                for (int k = 0; k < S; k++) {
+                    // every lane takes part in each shuffle
+                    const int Jo = __shfl(o.J, k, S);
+                    const double vo = __shfl(o.v, k, S);
+                    const int isdo = __shfl(o.isd, k, S);
+                    if (first && Jo == me.J) acc += vo;
+                    // the diagonal: lane 0, during the first chunk's pass
+                    if (c0 == 0 && lane == 0 && isdo) d += vo;
                 }
-                if (first) {
-                    const int pos = crow[I] + ndist + below;
-                    ccol[pos] = me.J;
-                    coff[pos] = acc;
-                }
+            }
+            if (first) {
+                const int pos = toff[I] + ndist + below;
+                tcol[pos] = me.J;
+                tval[pos] = acc;
             }
             ndist += __popcll(bal);
         }
         if (lane == 0) {
-            if (FILL) cdiag[I] = d;
-            else cnt[I] = ndist;
+            cdiag[I] = d;
+            cnt[I] = ndist;
         }
     }
-    if (!FILL && blockIdx.x == 0 && threadIdx.x == 0) cnt[nc] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[nc] = 0;
+}
+
+// Packs the bound-sized slots of the one-pass product into the coarse CSR.
+template <int S>
+__global__ void k_gal_compact(int nc, const int *toff, const int *crow, const int *tcol, const double *tval,
+                              int *ccol, double *coff) {
+    const int lane = threadIdx.x & (S - 1);
+    const int sub = (blockIdx.x * blockDim.x + threadIdx.x) / S;
+    const int nsub = (gridDim.x * blockDim.x) / S;
+    for (int I = sub; I < nc; I += nsub) {
+        const int b = toff[I], o = crow[I], c = crow[I + 1] - o;
+        for (int k = lane; k < c; k += S) {
+            ccol[o + k] = tcol[b + k];
+            coff[o + k] = tval[b + k];
+        }
+    }
 }
 
 
@@ -1283,25 +1314,32 @@ void coarse_operator(fvm_amg *h, Level &F, Level &C) {
     const double mavg = nc ? (double)(F.nnz + F.n) / nc : 0.0;
     const int S = mavg <= 24 ? 16 : (mavg <= 40 ? 32 : 64);
     const unsigned g = (unsigned)std::min<long long>(((long long)nc * S + 255) / 256, 65536);
-    auto launch = [&](auto kern, int *cnt, const int *crow, int *ccol, double *cdiag, double *coff) {
+    // slots of the one-pass product: row I gets as many as it has flattened entries
+    // (their total is at most nnz + n of the fine level)
+    DArr<int> ub((size_t)nc + 1), toff((size_t)nc + 1), cnt((size_t)nc + 1);
+    k_gal_bound<<<grid_for(nc, BS, 4096), BS, 0, s>>>(nc, F.c2f_row.p, F.c2f_col.p, F.row.p, ub.p);
+    h->ws.scan.exclusive(ub.p, toff.p, nc + 1, s);
+    const size_t tsize = (size_t)std::max<long long>(F.nnz + F.n, 1);
+    DArr<int> tcol(tsize);
+    DArr<double> tval(tsize);
+    C.diag.alloc(std::max(nc, 1));
+    auto launch = [&](auto kern) {
         kern<<<std::max(g, 1u), 256, 0, s>>>(nc, F.c2f_row.p, F.c2f_col.p, F.row.p, F.col.p, F.diag.p, F.off.p,
-                                             F.ci.p, cnt, crow, ccol, cdiag, coff);
+                                             F.ci.p, cnt.p, toff.p, tcol.p, C.diag.p, tval.p);
         FVM_HIP(hipGetLastError());
     };
-    DArr<int> cnt((size_t)nc + 1);
-    if (S == 16) launch(k_gal<16, false>, cnt.p, nullptr, nullptr, nullptr, nullptr);
-    else if (S == 32) launch(k_gal<32, false>, cnt.p, nullptr, nullptr, nullptr, nullptr);
-    else launch(k_gal<64, false>, cnt.p, nullptr, nullptr, nullptr, nullptr);
+    if (S == 16) launch(k_gal<16>);
+    else if (S == 32) launch(k_gal<32>);
+    else launch(k_gal<64>);
     C.n = nc;
     C.row.alloc((size_t)nc + 1);
     h->ws.scan.exclusive(cnt.p, C.row.p, nc + 1, s);
     C.nnz = read_scalar(C.row.p + nc, s);
     C.col.alloc(std::max<long long>(C.nnz, 1));
     C.off.alloc(std::max<long long>(C.nnz, 1));
-    C.diag.alloc(std::max(nc, 1));
-    if (S == 16) launch(k_gal<16, true>, nullptr, C.row.p, C.col.p, C.diag.p, C.off.p);
-    else if (S == 32) launch(k_gal<32, true>, nullptr, C.row.p, C.col.p, C.diag.p, C.off.p);
-    else launch(k_gal<64, true>, nullptr, C.row.p, C.col.p, C.diag.p, C.off.p);
+    k_gal_compact<16><<<std::max((unsigned)std::min<long long>(((long long)nc * 16 + 255) / 256, 65536), 1u), 256, 0,
+                        s>>>(nc, toff.p, C.row.p, tcol.p, tval.p, C.col.p, C.off.p);
+    FVM_HIP(hipGetLastError());
     // a structurally symmetric fine pattern gives a symmetric coarse one: (I, J) is an
     // entry iff some a_ij with i in I, j in J is, and a_ji is then one too (rows
     // outside every aggregate drop out of both sides alike)
