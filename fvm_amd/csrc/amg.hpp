@@ -48,6 +48,7 @@ struct Level {
     std::vector<int> color_ptr;        // host: row ranges of each colour (coloured index)
     DArr<int> d_color_ptr;             // the same on the device
     DArr<int> sell_off, sell_col;
+    int sell_total = 0;                // SELL-64 elements (padding included) = sell_off[nslices]
     DArr<double> sell_val, pdiag;
     DArr<int> pci, pc2f_row, pc2f_col;
     // mixed-precision values

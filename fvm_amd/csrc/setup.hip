@@ -1683,6 +1683,7 @@ void prepare_fast_level(fvm_amg *h, Level &L, int l, hipStream_t s, Workspace &w
     L.sell_off.alloc((size_t)nslice + 1);
     ws.scan.exclusive(w64.p, L.sell_off.p, nslice + 1, s);
     const int total = read_scalar(L.sell_off.p + nslice, s);
+    L.sell_total = total;
     {
         const double wmean = (double)total / (64.0 * nslice);
         L.sell_u = wmean > 16 ? 32 : (wmean > 8 ? 16 : 8);
@@ -1772,8 +1773,7 @@ void prepare_mixed(fvm_amg *h, bool fast) {
             L.mixed_ready_exact = true;
         }
         if (fast && !L.mixed_ready_fast && L.n > 0) {
-            const int nslice = (L.n + 63) / 64;
-            const int total = read_scalar(L.sell_off.p + nslice, s);
+            const int total = L.sell_total;
             L.pdiag32.alloc(n);
             if (l == 0) {
                 k_to_f32<<<grid_for(L.n, BS, 4096), BS, 0, s>>>(L.n, L.pdiag.p, L.pdiag32.p);
