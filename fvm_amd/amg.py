@@ -18,7 +18,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import AmgParams, FgmresParams, check, lib
+from ._lib import AmgParams, BcgstabParams, FgmresParams, check, lib
 
 
 def _ptr(a):
@@ -206,6 +206,55 @@ class AMG:
 def _fmt6(v):
     """C++ ostream default formatting (precision 6, %g-like)."""
     return "%g" % v
+
+
+class BCGStab:
+    """BiCGStab with an AMG preconditioner (BCGStab.cpp:27-171), FP64 on the device.
+
+    Fields are LinearSolver's (LinearSolver.h:19-24); ``preconditioner`` is an AMG
+    whose hierarchy is built, applied as AMG::smooth (one cycle from a zero delta).
+    ``solve_device`` returns (r0, final true-residual 1-norm, iterations); the
+    per-iteration 1-norms are in ``history()``.
+    """
+
+    _FIELDS = [f for f, _ in BcgstabParams._fields_]
+
+    def __init__(self, preconditioner=None):
+        self.preconditioner = preconditioner
+        p = BcgstabParams()
+        check(lib().fvm_bcgstab_default_params(ctypes.byref(p)))
+        self._p = p
+        self._hist = np.zeros(0)
+
+    def __getattr__(self, name):
+        if name in BCGStab._FIELDS:
+            return getattr(self.__dict__["_p"], name)
+        raise AttributeError(name)
+
+    def __setattr__(self, name, value):
+        if name in BCGStab._FIELDS:
+            setattr(self._p, name, value)
+        else:
+            object.__setattr__(self, name, value)
+
+    def solve_device(self, b, x, hist_cap=1 << 16):
+        pc = self.preconditioner
+        if pc is None:
+            raise ValueError("BCGStab needs a preconditioner")
+        pc._push_params()
+        r0 = ctypes.c_double()
+        rf = ctypes.c_double()
+        it = ctypes.c_int()
+        buf = np.zeros(hist_cap)
+        k = check(lib().fvm_bcgstab_solve_device(pc._h, ctypes.byref(self._p), _dptr(b), _dptr(x), ctypes.byref(r0),
+                                                  ctypes.byref(rf), ctypes.byref(it), _ptr(buf), hist_cap))
+        self._hist = buf[:k].copy()
+        self.lastIterations = it.value
+        self.lastResidual = rf.value
+        return r0.value, rf.value, it.value
+
+    def history(self):
+        return self._hist.copy()
 
 
 class FGMRES:

@@ -688,6 +688,32 @@ int fvm_fgmres_solve(fvm_fgmres *s, const double *b, double *x, double *rnorm0, 
     FVM_API_END
 }
 
+// ---------------------------------------------------------------------- BCGStab
+int fvm_bcgstab_default_params(fvm_bcgstab_params *p) {
+    if (!p) return FVM_E_INVALID;
+    p->nMaxIterations = 100;
+    p->verbosity = 0;
+    p->relativeTolerance = 1e-8;
+    p->absoluteTolerance = 1e-50;
+    return FVM_OK;
+}
+
+int fvm_bcgstab_solve_device(fvm_amg *pc, const fvm_bcgstab_params *p, const double *d_b, double *d_x,
+                             double *rnorm0, double *rnorm_final, int *iterations, double *hist, int cap) {
+    FVM_API_BEGIN
+    FVM_CHECK(pc && p, FVM_E_INVALID, "null argument");
+    FVM_CHECK(p->nMaxIterations >= 0 && cap >= 0 && (cap == 0 || hist), FVM_E_INVALID, "invalid argument");
+    FVM_CHECK(pc->ready, FVM_E_NOT_SETUP, "preconditioner not set up");
+    FVM_CHECK(pc->n_total == 0 || (d_b && d_x), FVM_E_INVALID, "null vector");
+    HandleScope g(pc);
+    std::vector<double> h;
+    fvm::bcgstab_solve(pc, *p, d_b, d_x, rnorm0, rnorm_final, iterations, h);
+    const int k = std::min<int>(cap, (int)h.size());
+    for (int i = 0; i < k; i++) hist[i] = h[i];
+    return k;
+    FVM_API_END
+}
+
 int fvm_fgmres_history(const fvm_fgmres *s, double *hist, int cap) {
     if (!s || (cap > 0 && !hist)) return FVM_E_INVALID;
     const int k = std::min<int>(cap, (int)s->hist.size());

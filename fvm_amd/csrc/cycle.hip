@@ -1831,6 +1831,122 @@ void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const d
     if (iterations) *iterations = total;
 }
 
+// ===========================================================================
+// BiCGStab around AMG::smooth (BCGStab.cpp:27-171): the reference's own Krylov
+// loop, in FP64 on the device, with the handle's cycle as the preconditioner.
+// The reference's operations in its order: rho = r.rt; p = (p - omega v) beta + r;
+// p^ = smooth(b = p, x = 0); v = A p^; alpha = rho / (rt.v); x -= alpha p^,
+// r -= alpha v; ||r||_1 < absTol ends the loop; s^ = smooth(b = r, x = 0);
+// t = A s^; omega = (t.r) / (t.t); x -= omega s^, r -= omega t; the 1-norm of r,
+// relative to r0, is the stop rule; at the end the true residual is recomputed.
+// ===========================================================================
+namespace {
+
+__global__ void k_bicg_p(int n, double *p, const double *v, const double *r, double omega, double beta) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double t = p[i] - omega * v[i];
+        t *= beta;
+        p[i] = t + r[i];
+    }
+}
+
+// x -= a * xs, r -= a * rs
+__global__ void k_bicg_update(int n, double a, const double *xs, const double *rs, double *x, double *r) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        x[i] -= a * xs[i];
+        r[i] -= a * rs[i];
+    }
+}
+
+double dot_of(Run &R, const double *a, const double *b, int n) {
+    fvm_amg *h = R.h;
+    const unsigned g = grid_for(n, BS, NORM_BLOCKS);
+    k_dot_partial<double><<<g, BS, 0, h->stream>>>(n, a, b, h->ws.partials.p);
+    return reduce_to_host(h, g, 3);
+}
+
+// y = the handle's cycle on A y + b = 0 from y = 0 (AMG::smooth with a zero delta)
+void smooth_zero(Run &R, const double *b, double *y) {
+    fvm_amg *h = R.h;
+    Level &F = h->lev[0];
+    hipStream_t s = h->stream;
+    FVM_HIP(hipMemcpyAsync(F.b.p, b, sizeof(double) * F.n, hipMemcpyDeviceToDevice, s));
+    FVM_HIP(hipMemsetAsync(F.x.p, 0, sizeof(double) * F.n, s));
+    R.r0_valid = false;
+    one_cycle(R);
+    FVM_HIP(hipMemcpyAsync(y, F.x.p, sizeof(double) * F.n, hipMemcpyDeviceToDevice, s));
+}
+
+}  // namespace
+
+void bcgstab_solve(fvm_amg *h, const fvm_bcgstab_params &p, const double *d_b, double *d_x, double *rnorm0,
+                   double *rnorm_final, int *iterations, std::vector<double> &hist) {
+    Run R = make_run(h);
+    Level &F = h->lev[0];
+    hipStream_t s = h->stream;
+    const int n = F.n;
+    const size_t n1 = std::max(n, 1);
+    const int *perm = R.fast ? F.perm.p : nullptr;
+    DArr<double> b(n1), x(n1), r(n1), rt(n1), pv(n1), ph(n1), v(n1), t(n1);
+    const unsigned gb = grid_for(n, BS);
+    if (n > 0) {
+        k_gather<<<gb, BS, 0, s>>>(n, perm, d_b, b.p);
+        k_gather<<<gb, BS, 0, s>>>(n, perm, d_x, x.p);
+    }
+    Run R64 = R;
+    R64.mixed = false;
+    residual<double>(R64, 0, x.p, b.p, r.p);
+    const double r0 = norm1_of(R, r.p, n);
+    FVM_CHECK(std::isfinite(r0), FVM_E_NUMERIC, "BCGStab: non-finite initial residual");
+    hist.assign(1, r0);
+    if (p.verbosity > 0) printf("0: %g\n", r0);
+    FVM_HIP(hipMemcpyAsync(rt.p, r.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    double rho = 0, alpha = 0, omega = 0;
+    bool have_p = false;
+    int it = 0;
+    for (int i = 0; i < p.nMaxIterations && n > 0; i++) {
+        it = i + 1;
+        const double rho_prev = rho;
+        rho = dot_of(R, r.p, rt.p, n);
+        if (!have_p) {
+            FVM_HIP(hipMemcpyAsync(pv.p, r.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            have_p = true;
+        } else {
+            const double beta = (rho / rho_prev) * (alpha / omega);
+            k_bicg_p<<<gb, BS, 0, s>>>(n, pv.p, v.p, r.p, omega, beta);
+        }
+        smooth_zero(R, pv.p, ph.p);
+        spmv64(R, ph.p, v.p);
+        alpha = rho / dot_of(R, rt.p, v.p, n);
+        k_bicg_update<<<gb, BS, 0, s>>>(n, alpha, ph.p, v.p, x.p, r.p);
+        double rn = norm1_of(R, r.p, n);
+        FVM_CHECK(std::isfinite(rn), FVM_E_NUMERIC,
+                  "BCGStab produced a non-finite residual (iteration " + std::to_string(i + 1) + ")");
+        if (rn < p.absoluteTolerance) break;
+        smooth_zero(R, r.p, ph.p);   // s^ takes p^'s storage, as in the reference
+        spmv64(R, ph.p, t.p);
+        const double tdotr = dot_of(R, t.p, r.p, n);
+        const double tdott = dot_of(R, t.p, t.p, n);
+        omega = tdotr / tdott;
+        k_bicg_update<<<gb, BS, 0, s>>>(n, omega, ph.p, t.p, x.p, r.p);
+        rn = norm1_of(R, r.p, n);
+        FVM_CHECK(std::isfinite(rn), FVM_E_NUMERIC,
+                  "BCGStab produced a non-finite residual (iteration " + std::to_string(i + 1) + ")");
+        hist.push_back(rn);
+        if (p.verbosity > 0) printf("%d: %g\n", i + 1, rn);
+        if (rn < p.absoluteTolerance || rn / r0 < p.relativeTolerance) break;
+    }
+    residual<double>(R64, 0, x.p, b.p, r.p);
+    const double rf = norm1_of(R, r.p, n);
+    if (p.verbosity > 0) printf("n: %g\n", rf);
+    if (n > 0) k_scatter<<<gb, BS, 0, s>>>(n, perm, x.p, d_x);
+    FVM_HIP(hipGetLastError());
+    FVM_HIP(hipStreamSynchronize(s));
+    if (rnorm0) *rnorm0 = r0;
+    if (rnorm_final) *rnorm_final = rf;
+    if (iterations) *iterations = it;
+}
+
 }  // namespace fvm
 
 namespace fvm {

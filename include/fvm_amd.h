@@ -226,6 +226,29 @@ int fvm_fgmres_solve(fvm_fgmres *s, const double *b, double *x, double *rnorm0,
  * cycle. Returns the number of entries written (<= cap). */
 int fvm_fgmres_history(const fvm_fgmres *s, double *hist, int cap);
 
+/* ---------------------------------------------------------------------------
+ * BiCGStab with the AMG handle's cycle as preconditioner: the reference's own
+ * Krylov loop, BCGStab::solve (BCGStab.cpp:27-171), in FP64 on the device, with
+ * AMG::smooth (one cycle from a zero delta) as the preconditioner application.
+ * Fields and defaults are LinearSolver's (LinearSolver.h:19-24; verbosity 0
+ * here). b, x: device vectors of n_total entries, delta form (r = b + A x);
+ * x holds the initial delta and receives the solution on the n_self rows.
+ * hist (may be NULL): r0, then the 1-norm of r after every completed iteration,
+ * as the reference prints them; returns the number of entries written (<= cap)
+ * or an FVM_E_* code. rnorm0 = the initial 1-norm (the reference's return
+ * value); rnorm_final = the 1-norm of the recomputed true residual.
+ * ------------------------------------------------------------------------- */
+typedef struct fvm_bcgstab_params {
+    int nMaxIterations;           /* 100 */
+    int verbosity;                /* 0 */
+    double relativeTolerance;     /* 1e-8 */
+    double absoluteTolerance;     /* 1e-50 */
+} fvm_bcgstab_params;
+
+int fvm_bcgstab_default_params(fvm_bcgstab_params *p);
+int fvm_bcgstab_solve_device(fvm_amg *pc, const fvm_bcgstab_params *p, const double *d_b, double *d_x,
+                             double *rnorm0, double *rnorm_final, int *iterations, double *hist, int cap);
+
 
 /* ---------------------------------------------------------------------------
  * Batched solve of many independent scalar systems (the per-direction systems
