@@ -4,7 +4,7 @@ Host-side mirror of the c-PRIMED/fvm ``LinearSolver``/``AMG`` interface over the
 C ABI of ``libfvm_amd.so`` (hand-written HIP kernels for gfx950).
 """
 from .linear_system import LinearSystem, read_matrix_market
-from .amg import AMG, BCGStab, FGMRES, BatchAMG
+from .amg import AMG, BCGStab, CG, FGMRES, BatchAMG
 from .learned import LearnedCorrection
 
-__all__ = ["AMG", "BCGStab", "FGMRES", "LinearSystem", "read_matrix_market"]
+__all__ = ["AMG", "BCGStab", "CG", "FGMRES", "LinearSystem", "read_matrix_market"]

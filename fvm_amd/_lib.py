@@ -38,7 +38,7 @@ class FgmresParams(ctypes.Structure):
                 ("relativeTolerance", _d), ("absoluteTolerance", _d)]
 
 
-class BcgstabParams(ctypes.Structure):
+class LinSolverParams(ctypes.Structure):
     _fields_ = [("nMaxIterations", _i), ("verbosity", _i), ("relativeTolerance", _d),
                 ("absoluteTolerance", _d)]
 
@@ -76,8 +76,9 @@ SIGNATURES = {
     "fvm_fgmres_solve_device": (_i, [_p, _p, _p, _p, _p, _p]),
     "fvm_fgmres_solve": (_i, [_p, _p, _p, _p, _p, _p]),
     "fvm_fgmres_history": (_i, [_p, _p, _i]),
-    "fvm_bcgstab_default_params": (_i, [ctypes.POINTER(BcgstabParams)]),
-    "fvm_bcgstab_solve_device": (_i, [_p, ctypes.POINTER(BcgstabParams), _p, _p, _p, _p, _p, _p, _i]),
+    "fvm_linsolver_default_params": (_i, [ctypes.POINTER(LinSolverParams)]),
+    "fvm_bcgstab_solve_device": (_i, [_p, ctypes.POINTER(LinSolverParams), _p, _p, _p, _p, _p, _p, _i]),
+    "fvm_cg_solve_device": (_i, [_p, ctypes.POINTER(LinSolverParams), _p, _p, _p, _p, _p, _p, _i]),
     "fvm_mesh_create_hex": (_i, [ctypes.POINTER(_p), _i, _i, _d, ctypes.c_uint64]),
     "fvm_mesh_create": (_i, [ctypes.POINTER(_p), _i, _i, _p, _i, _p, _p, _i, _i, _i, _p]),
     "fvm_mesh_destroy": (_i, [_p]),

@@ -18,7 +18,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import AmgParams, BcgstabParams, FgmresParams, check, lib
+from ._lib import AmgParams, LinSolverParams, FgmresParams, check, lib
 
 
 def _ptr(a):
@@ -211,18 +211,21 @@ def _fmt6(v):
 class BCGStab:
     """BiCGStab with an AMG preconditioner (BCGStab.cpp:27-171), FP64 on the device.
 
+    (``CG`` below is the reference's CG.cpp:25-137 with the same interface.)
+
     Fields are LinearSolver's (LinearSolver.h:19-24); ``preconditioner`` is an AMG
     whose hierarchy is built, applied as AMG::smooth (one cycle from a zero delta).
     ``solve_device`` returns (r0, final true-residual 1-norm, iterations); the
     per-iteration 1-norms are in ``history()``.
     """
 
-    _FIELDS = [f for f, _ in BcgstabParams._fields_]
+    _FIELDS = [f for f, _ in LinSolverParams._fields_]
+    _ENTRY = "fvm_bcgstab_solve_device"
 
     def __init__(self, preconditioner=None):
         self.preconditioner = preconditioner
-        p = BcgstabParams()
-        check(lib().fvm_bcgstab_default_params(ctypes.byref(p)))
+        p = LinSolverParams()
+        check(lib().fvm_linsolver_default_params(ctypes.byref(p)))
         self._p = p
         self._hist = np.zeros(0)
 
@@ -246,8 +249,8 @@ class BCGStab:
         rf = ctypes.c_double()
         it = ctypes.c_int()
         buf = np.zeros(hist_cap)
-        k = check(lib().fvm_bcgstab_solve_device(pc._h, ctypes.byref(self._p), _dptr(b), _dptr(x), ctypes.byref(r0),
-                                                  ctypes.byref(rf), ctypes.byref(it), _ptr(buf), hist_cap))
+        k = check(getattr(lib(), self._ENTRY)(pc._h, ctypes.byref(self._p), _dptr(b), _dptr(x), ctypes.byref(r0),
+                                               ctypes.byref(rf), ctypes.byref(it), _ptr(buf), hist_cap))
         self._hist = buf[:k].copy()
         self.lastIterations = it.value
         self.lastResidual = rf.value
@@ -255,6 +258,12 @@ class BCGStab:
 
     def history(self):
         return self._hist.copy()
+
+
+class CG(BCGStab):
+    """Preconditioned CG with an AMG preconditioner (CG.cpp:25-137), FP64 on the device."""
+
+    _ENTRY = "fvm_cg_solve_device"
 
 
 class FGMRES:

@@ -187,8 +187,10 @@ void build_hierarchy_batched(fvm_amg *h, BatchInfo &B);
 void solve_device(fvm_amg *h, const double *d_b, double *d_x, double *rnorm0, double *rnorm_final,
                   int *cycles);
 void smooth_device(fvm_amg *h, const double *d_b, double *d_x);
-void bcgstab_solve(fvm_amg *h, const fvm_bcgstab_params &p, const double *d_b, double *d_x, double *rnorm0,
+void bcgstab_solve(fvm_amg *h, const fvm_linsolver_params &p, const double *d_b, double *d_x, double *rnorm0,
                    double *rnorm_final, int *iterations, std::vector<double> &hist);
+void cg_solve(fvm_amg *h, const fvm_linsolver_params &p, const double *d_b, double *d_x, double *rnorm0,
+              double *rnorm_final, int *iterations, std::vector<double> &hist);
 void batch_solve(fvm_amg *h, BatchInfo &B, const double *const *d_b, double *const *d_x, double *rnorm0,
                  double *rnorm_final, int *cycles);
 void apply_precond(fvm_amg *h, const double *d_v, double *d_z);   // z = -M^{-1}... see cycle.hip

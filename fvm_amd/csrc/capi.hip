@@ -689,7 +689,7 @@ int fvm_fgmres_solve(fvm_fgmres *s, const double *b, double *x, double *rnorm0, 
 }
 
 // ---------------------------------------------------------------------- BCGStab
-int fvm_bcgstab_default_params(fvm_bcgstab_params *p) {
+int fvm_linsolver_default_params(fvm_linsolver_params *p) {
     if (!p) return FVM_E_INVALID;
     p->nMaxIterations = 100;
     p->verbosity = 0;
@@ -698,8 +698,12 @@ int fvm_bcgstab_default_params(fvm_bcgstab_params *p) {
     return FVM_OK;
 }
 
-int fvm_bcgstab_solve_device(fvm_amg *pc, const fvm_bcgstab_params *p, const double *d_b, double *d_x,
-                             double *rnorm0, double *rnorm_final, int *iterations, double *hist, int cap) {
+namespace {
+using KrylovFn = void (*)(fvm_amg *, const fvm_linsolver_params &, const double *, double *, double *, double *,
+                          int *, std::vector<double> &);
+
+int krylov_device(KrylovFn fn, fvm_amg *pc, const fvm_linsolver_params *p, const double *d_b, double *d_x,
+                  double *rnorm0, double *rnorm_final, int *iterations, double *hist, int cap) {
     FVM_API_BEGIN
     FVM_CHECK(pc && p, FVM_E_INVALID, "null argument");
     FVM_CHECK(p->nMaxIterations >= 0 && cap >= 0 && (cap == 0 || hist), FVM_E_INVALID, "invalid argument");
@@ -707,11 +711,22 @@ int fvm_bcgstab_solve_device(fvm_amg *pc, const fvm_bcgstab_params *p, const dou
     FVM_CHECK(pc->n_total == 0 || (d_b && d_x), FVM_E_INVALID, "null vector");
     HandleScope g(pc);
     std::vector<double> h;
-    fvm::bcgstab_solve(pc, *p, d_b, d_x, rnorm0, rnorm_final, iterations, h);
+    fn(pc, *p, d_b, d_x, rnorm0, rnorm_final, iterations, h);
     const int k = std::min<int>(cap, (int)h.size());
     for (int i = 0; i < k; i++) hist[i] = h[i];
     return k;
     FVM_API_END
+}
+}  // namespace
+
+int fvm_bcgstab_solve_device(fvm_amg *pc, const fvm_linsolver_params *p, const double *d_b, double *d_x,
+                             double *rnorm0, double *rnorm_final, int *iterations, double *hist, int cap) {
+    return krylov_device(fvm::bcgstab_solve, pc, p, d_b, d_x, rnorm0, rnorm_final, iterations, hist, cap);
+}
+
+int fvm_cg_solve_device(fvm_amg *pc, const fvm_linsolver_params *p, const double *d_b, double *d_x,
+                        double *rnorm0, double *rnorm_final, int *iterations, double *hist, int cap) {
+    return krylov_device(fvm::cg_solve, pc, p, d_b, d_x, rnorm0, rnorm_final, iterations, hist, cap);
 }
 
 int fvm_fgmres_history(const fvm_fgmres *s, double *hist, int cap) {

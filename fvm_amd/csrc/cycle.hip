@@ -1879,7 +1879,7 @@ void smooth_zero(Run &R, const double *b, double *y) {
 
 }  // namespace
 
-void bcgstab_solve(fvm_amg *h, const fvm_bcgstab_params &p, const double *d_b, double *d_x, double *rnorm0,
+void bcgstab_solve(fvm_amg *h, const fvm_linsolver_params &p, const double *d_b, double *d_x, double *rnorm0,
                    double *rnorm_final, int *iterations, std::vector<double> &hist) {
     Run R = make_run(h);
     Level &F = h->lev[0];
@@ -1932,6 +1932,80 @@ void bcgstab_solve(fvm_amg *h, const fvm_bcgstab_params &p, const double *d_b, d
         rn = norm1_of(R, r.p, n);
         FVM_CHECK(std::isfinite(rn), FVM_E_NUMERIC,
                   "BCGStab produced a non-finite residual (iteration " + std::to_string(i + 1) + ")");
+        hist.push_back(rn);
+        if (p.verbosity > 0) printf("%d: %g\n", i + 1, rn);
+        if (rn < p.absoluteTolerance || rn / r0 < p.relativeTolerance) break;
+    }
+    residual<double>(R64, 0, x.p, b.p, r.p);
+    const double rf = norm1_of(R, r.p, n);
+    if (p.verbosity > 0) printf("n: %g\n", rf);
+    if (n > 0) k_scatter<<<gb, BS, 0, s>>>(n, perm, x.p, d_x);
+    FVM_HIP(hipGetLastError());
+    FVM_HIP(hipStreamSynchronize(s));
+    if (rnorm0) *rnorm0 = r0;
+    if (rnorm_final) *rnorm_final = rf;
+    if (iterations) *iterations = it;
+}
+
+// ===========================================================================
+// Preconditioned CG around AMG::smooth (CG.cpp:25-137), FP64 on the device, in
+// the reference's order: z = smooth(b = r, x = 0); rho = r.z; p = z, or
+// p = p beta + z with beta = rho / rho_prev; q = A p; alpha = rho / (p.q);
+// x -= alpha p, r -= alpha q; the 1-norm of r is checked after every iteration.
+// ===========================================================================
+namespace {
+
+// p = p * beta + z
+__global__ void k_cg_p(int n, double *p, const double *z, double beta) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double t = p[i] * beta;
+        p[i] = t + z[i];
+    }
+}
+
+}  // namespace
+
+void cg_solve(fvm_amg *h, const fvm_linsolver_params &p, const double *d_b, double *d_x, double *rnorm0,
+              double *rnorm_final, int *iterations, std::vector<double> &hist) {
+    Run R = make_run(h);
+    Level &F = h->lev[0];
+    hipStream_t s = h->stream;
+    const int n = F.n;
+    const size_t n1 = std::max(n, 1);
+    const int *perm = R.fast ? F.perm.p : nullptr;
+    DArr<double> b(n1), x(n1), r(n1), z(n1), pv(n1), q(n1);
+    const unsigned gb = grid_for(n, BS);
+    if (n > 0) {
+        k_gather<<<gb, BS, 0, s>>>(n, perm, d_b, b.p);
+        k_gather<<<gb, BS, 0, s>>>(n, perm, d_x, x.p);
+    }
+    Run R64 = R;
+    R64.mixed = false;
+    residual<double>(R64, 0, x.p, b.p, r.p);
+    const double r0 = norm1_of(R, r.p, n);
+    FVM_CHECK(std::isfinite(r0), FVM_E_NUMERIC, "CG: non-finite initial residual");
+    hist.assign(1, r0);
+    if (p.verbosity > 0) printf("0: %g\n", r0);
+    double rho = 0;
+    bool have_p = false;
+    int it = 0;
+    for (int i = 0; i < p.nMaxIterations && n > 0; i++) {
+        it = i + 1;
+        smooth_zero(R, r.p, z.p);
+        const double rho_prev = rho;
+        rho = dot_of(R, r.p, z.p, n);
+        if (!have_p) {
+            FVM_HIP(hipMemcpyAsync(pv.p, z.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            have_p = true;
+        } else {
+            k_cg_p<<<gb, BS, 0, s>>>(n, pv.p, z.p, rho / rho_prev);
+        }
+        spmv64(R, pv.p, q.p);
+        const double alpha = rho / dot_of(R, pv.p, q.p, n);
+        k_bicg_update<<<gb, BS, 0, s>>>(n, alpha, pv.p, q.p, x.p, r.p);
+        const double rn = norm1_of(R, r.p, n);
+        FVM_CHECK(std::isfinite(rn), FVM_E_NUMERIC,
+                  "CG produced a non-finite residual (iteration " + std::to_string(i + 1) + ")");
         hist.push_back(rn);
         if (p.verbosity > 0) printf("%d: %g\n", i + 1, rn);
         if (rn < p.absoluteTolerance || rn / r0 < p.relativeTolerance) break;

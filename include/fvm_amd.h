@@ -227,9 +227,11 @@ int fvm_fgmres_solve(fvm_fgmres *s, const double *b, double *x, double *rnorm0,
 int fvm_fgmres_history(const fvm_fgmres *s, double *hist, int cap);
 
 /* ---------------------------------------------------------------------------
- * BiCGStab with the AMG handle's cycle as preconditioner: the reference's own
- * Krylov loop, BCGStab::solve (BCGStab.cpp:27-171), in FP64 on the device, with
- * AMG::smooth (one cycle from a zero delta) as the preconditioner application.
+ * The reference's Krylov loops with the AMG handle's cycle as preconditioner,
+ * in FP64 on the device; AMG::smooth (one cycle from a zero delta) is the
+ * preconditioner application:
+ *   fvm_bcgstab_solve_device  BCGStab::solve (BCGStab.cpp:27-171)
+ *   fvm_cg_solve_device       CG::solve (CG.cpp:25-137)
  * Fields and defaults are LinearSolver's (LinearSolver.h:19-24; verbosity 0
  * here). b, x: device vectors of n_total entries, delta form (r = b + A x);
  * x holds the initial delta and receives the solution on the n_self rows.
@@ -238,16 +240,18 @@ int fvm_fgmres_history(const fvm_fgmres *s, double *hist, int cap);
  * or an FVM_E_* code. rnorm0 = the initial 1-norm (the reference's return
  * value); rnorm_final = the 1-norm of the recomputed true residual.
  * ------------------------------------------------------------------------- */
-typedef struct fvm_bcgstab_params {
+typedef struct fvm_linsolver_params {
     int nMaxIterations;           /* 100 */
     int verbosity;                /* 0 */
     double relativeTolerance;     /* 1e-8 */
     double absoluteTolerance;     /* 1e-50 */
-} fvm_bcgstab_params;
+} fvm_linsolver_params;
 
-int fvm_bcgstab_default_params(fvm_bcgstab_params *p);
-int fvm_bcgstab_solve_device(fvm_amg *pc, const fvm_bcgstab_params *p, const double *d_b, double *d_x,
+int fvm_linsolver_default_params(fvm_linsolver_params *p);
+int fvm_bcgstab_solve_device(fvm_amg *pc, const fvm_linsolver_params *p, const double *d_b, double *d_x,
                              double *rnorm0, double *rnorm_final, int *iterations, double *hist, int cap);
+int fvm_cg_solve_device(fvm_amg *pc, const fvm_linsolver_params *p, const double *d_b, double *d_x,
+                        double *rnorm0, double *rnorm_final, int *iterations, double *hist, int cap);
 
 
 /* ---------------------------------------------------------------------------

@@ -55,3 +55,35 @@ def bcgstab(matvec, smooth, b, x, n_max=100, rel_tol=1e-8, abs_tol=1e-50):
             break
     rf = np.abs(b + matvec(x)).sum()                   # :153-154
     return dict(x=x, r0=r0, rfinal=rf, iterations=it, history=np.array(hist))
+
+
+def cg(matvec, smooth, b, x, n_max=100, rel_tol=1e-8, abs_tol=1e-50):
+    """CG::solve (CG.cpp:25-137); returns what bcgstab returns (history: r0, then
+    ||r||_1 after every iteration)."""
+    b = np.asarray(b, np.float64)
+    x = np.array(x, np.float64)
+    r = b + matvec(x)                                  # CG.cpp:34
+    r0 = np.abs(r).sum()                               # :36
+    hist = [r0]
+    p = None
+    rho = 0.0
+    it = 0
+    for i in range(n_max):                             # :61
+        it = i + 1
+        z = smooth(r)                                  # :63-67
+        rho_prev = rho
+        rho = float(r @ z)                             # :71
+        if p is None:
+            p = z.copy()                               # :75-78
+        else:
+            p = p * (rho / rho_prev) + z               # :81-83
+        q = matvec(p)                                  # :86
+        alpha = rho / float(p @ q)                     # :88-91
+        x = x - alpha * p                              # :93
+        r = r - alpha * q                              # :94
+        rn = np.abs(r).sum()                           # :96
+        hist.append(rn)
+        if rn < abs_tol or rn / r0 < rel_tol:          # :99, 111
+            break
+    rf = np.abs(b + matvec(x)).sum()                   # :122-123
+    return dict(x=x, r0=r0, rfinal=rf, iterations=it, history=np.array(hist))

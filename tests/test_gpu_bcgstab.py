@@ -1,5 +1,5 @@
-"""Device BiCGStab (fvm_bcgstab_solve_device, BCGStab.cpp:27-171) against the
-BiCGStab restatement in oracle/krylov_ref.py, with the oracle AMG's smooth as its
+"""Device BiCGStab (fvm_bcgstab_solve_device, BCGStab.cpp:27-171) and CG
+(fvm_cg_solve_device, CG.cpp:25-137) against their restatements in oracle/krylov_ref.py, with the oracle AMG's smooth as its
 preconditioner. In the reference-order FP64 mode the preconditioner is bit-identical
 on both sides; the dot products and 1-norms are summed in a different order, so the
 bar is: the same iteration count, every history entry within 1e-7 relative, and the
@@ -45,27 +45,29 @@ def _gpu(m, gs, prec):
     return amg
 
 
-def _oracle(m):
+def _oracle(m, method=None):
+    method = method or K.bcgstab
     n = m["n_self"]
     oa = O.OracleAMG(n_max_iterations=1)
     oa.setup(m["n_self"], m["n_total"], m["row"], m["col"], m["diag"], m["off"])
     smooth = lambda b: oa.smooth(np.concatenate([b, np.zeros(m["n_total"] - n)]), np.zeros(m["n_total"]))[:n]
     matvec = lambda v: O.residual(m, np.concatenate([v, np.zeros(m["n_total"] - n)]), np.zeros(n))
-    return K.bcgstab(matvec, smooth, m["b"][:n], np.zeros(n), n_max=100, rel_tol=1e-8)
+    return method(matvec, smooth, m["b"][:n], np.zeros(n), n_max=100, rel_tol=1e-8)
 
 
 @pytest.mark.parametrize("kind", ["kat226", "grid"])
-def test_bcgstab_reference_order_matches_oracle(kind):
+@pytest.mark.parametrize("method", ["bcgstab", "cg"])
+def test_reference_order_matches_oracle(kind, method):
     import torch
     import fvm_amd
     m = _system(kind)
     n = m["n_self"]
-    ref = _oracle(m)
+    ref = _oracle(m, K.bcgstab if method == "bcgstab" else K.cg)
     amg = _gpu(m, 0, 0)
-    solver = fvm_amd.BCGStab(amg)
+    solver = (fvm_amd.BCGStab if method == "bcgstab" else fvm_amd.CG)(amg)
     x = torch.zeros(m["n_total"], dtype=torch.float64, device="cuda:0")
     r0, rf, it = solver.solve_device(_dev(m["b"], torch.float64), x)
-    print(kind, "iterations", it, ref["iterations"], "history", solver.history(), ref["history"], flush=True)
+    print(kind, method, "iterations", it, ref["iterations"], "history", solver.history(), ref["history"], flush=True)
     np.testing.assert_allclose(r0, ref["r0"], rtol=1e-12)
     assert it == ref["iterations"]
     np.testing.assert_allclose(solver.history(), ref["history"], rtol=1e-7)

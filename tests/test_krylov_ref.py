@@ -80,3 +80,19 @@ def test_kat226_converges_with_oracle_amg():
     assert ref["history"][-1] / ref["r0"] < 1e-8
     assert ref["iterations"] < 40
     assert ref["rfinal"] / ref["r0"] < 1e-7
+
+
+@pytest.mark.parametrize("kind", ["kat226", "grid"])
+@pytest.mark.parametrize("k", [1, 2, 5])
+def test_cg_matches_scipy_cg(kind, k):
+    """CG::solve (CG.cpp:25-137) against scipy's CG with M = -smooth on A x = -b."""
+    from scipy.sparse.linalg import cg as scipy_cg
+    m, A, smooth = _system(kind)
+    n = m["n_self"]
+    b = m["b"][:n]
+    ref = K.cg(lambda v: A @ v, smooth, b, np.zeros(n), n_max=k, rel_tol=0.0)
+    assert ref["iterations"] == k
+    M = LinearOperator((n, n), matvec=lambda v: -smooth(np.asarray(v).ravel()))
+    xs, info = scipy_cg(A, -b, x0=np.zeros(n), rtol=1e-300, atol=0.0, maxiter=k, M=M)
+    assert np.abs(ref["x"] - xs).max() <= 1e-9 * np.abs(xs).max()
+    assert abs(ref["history"][-1] - ref["rfinal"]) <= 1e-12 * ref["r0"]
