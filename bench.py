@@ -526,6 +526,23 @@ def main():
         % (ms_per_step, 1e3 * st["setup_seconds"], it, rf / r0))
     x_mixed = x
 
+    # ---------------- the reference's own Krylov loops (BCGStab.cpp, CG.cpp) with the same
+    # mixed AMG as preconditioner: setup + solve to the same stop rule, once each
+    krylov = {}
+    for name, cls in (("bcgstab", fvm_amd.BCGStab), ("cg", fvm_amd.CG)):
+        ks_ = cls(amg)
+        ks_.relativeTolerance = 1e-8
+        ks_.nMaxIterations = 500
+        xk = torch.zeros(n_total, dtype=torch.float64, device="cuda:0")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        setup_on(amg, S)
+        r0k, rfk, itk = ks_.solve_device(b, xk)
+        torch.cuda.synchronize()
+        krylov[name] = {"s": round(time.perf_counter() - t0, 4), "iterations": itk, "final_ratio": rfk / r0k}
+        log("%s + mixed AMG: %.1f ms (setup included), %d iterations, ratio %.3e"
+            % (name, 1e3 * krylov[name]["s"], itk, rfk / r0k))
+
     # ---------------- plain FP64 GPU AMG (coloured GS), same system
     t_plain, cyc_plain, err, tune_plain = None, None, None, None
     kp_s = (0, 0.0)
@@ -614,6 +631,7 @@ def main():
         "finest_coarsening_rounds": st["finest_rounds"],
         "fgmres_iterations": it,
         "final_ratio": rf / r0,
+        "reference_krylov_mixed_amg": krylov,
         "plain_fp64_gpu_amg_s": None if t_plain is None else round(t_plain, 4),
         "plain_fp64_cycles": cyc_plain,
         "speedup_vs_plain_fp64_gpu": None if t_plain is None else round(t_plain / (ms_per_step / 1e3), 3),
