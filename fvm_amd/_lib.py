@@ -114,6 +114,7 @@ SIGNATURES = {
     "fvm_system_info": (_i, [_p, _p, _p, _p]),
     "fvm_system_device": (_i, [_p, _p, _p, _p, _p, _p, _p]),
     "fvm_system_get": (_i, [_p, _p, _p, _p, _p, _p, _p]),
+    "fvm_system_post_solve": (_i, [_p, _p]),
     "fvm_system_destroy": (_i, [_p]),
     "fvm_amg_setup_system": (_i, [_p, _p]),
 }

@@ -391,6 +391,34 @@ __global__ void k_bc_ghost(int n_cells, int nct, const int *cf_row, const int *c
 
 // LinearSystem::initSolve: eliminate the marked ghost rows (one neighbour each)
 // into their interior cell, ghosts in ascending order (CRMatrix.h:899-944).
+// LinearSystem::postSolve -> CRMatrix::solveBoundary (LinearSystem.cpp:250-252,
+// CRMatrix.h:433-454): every boundary row nr >= n_self takes
+// x[nr] = -(b[nr] + sum off x[col]) / diag[nr], the sum in CSR order.
+// flag: some boundary row reads another boundary row (then rows go in ascending order).
+__global__ void k_boundary_dep(int n_self, int n_total, const int *row, const int *col, const uint8_t *isb,
+                               int *flag) {
+    for (int nr = n_self + blockIdx.x * blockDim.x + threadIdx.x; nr < n_total; nr += gridDim.x * blockDim.x) {
+        if (!isb[nr]) continue;
+        for (int nb = row[nr]; nb < row[nr + 1]; nb++) {
+            const int j = col[nb];
+            if (j >= n_self && j != nr && isb[j]) { atomicOr(flag, 1); break; }
+        }
+    }
+}
+
+template <bool SEQ>
+__global__ void k_solve_boundary(int n_self, int n_total, const int *row, const int *col, const double *diag,
+                                 const double *off, const uint8_t *isb, const double *b, double *x) {
+    const int start = SEQ ? n_self : n_self + blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = SEQ ? 1 : gridDim.x * blockDim.x;
+    for (int nr = start; nr < n_total; nr += stride) {
+        if (!isb[nr]) continue;
+        double sum = b[nr];
+        for (int nb = row[nr]; nb < row[nr + 1]; nb++) sum += off[nb] * x[col[nb]];
+        x[nr] = -sum / diag[nr];
+    }
+}
+
 __global__ void k_eliminate(int n_cells, const int *cf_row, const int *cf_col, const int *fcells,
                             const int *pos01, const int *pos10, const uint8_t *isb, double *diag,
                             double *off, double *b) {
@@ -932,6 +960,31 @@ int fvm_system_device(const fvm_system *S, const int32_t **row, const int32_t **
     if (b) *b = S->b.p;
     if (is_boundary) *is_boundary = S->isb.p;
     return FVM_OK;
+}
+
+int fvm_system_post_solve(const fvm_system *S, double *d_delta) {
+    FVM_API_BEGIN
+    FVM_CHECK(S, FVM_E_INVALID, "null system");
+    FVM_CHECK(S->n_total == 0 || d_delta, FVM_E_INVALID, "null vector");
+    const int ng = S->n_total - S->n_self;
+    if (ng <= 0) return FVM_OK;
+    fvm::Dev d(S->device);
+    DArr<int> flag(1);
+    FVM_HIP(hipMemset(flag.p, 0, sizeof(int)));
+    const unsigned g = fvm::grid_for(ng, 256);
+    fvm::k_boundary_dep<<<g, 256>>>(S->n_self, S->n_total, S->row.p, S->col.p, S->isb.p, flag.p);
+    int dep = 0;
+    FVM_HIP(hipMemcpy(&dep, flag.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (dep)   // a boundary row reads another one: the reference's ascending row order
+        fvm::k_solve_boundary<true><<<1, 1>>>(S->n_self, S->n_total, S->row.p, S->col.p, S->diag.p, S->off.p,
+                                              S->isb.p, S->b.p, d_delta);
+    else
+        fvm::k_solve_boundary<false><<<g, 256>>>(S->n_self, S->n_total, S->row.p, S->col.p, S->diag.p, S->off.p,
+                                                 S->isb.p, S->b.p, d_delta);
+    FVM_HIP(hipGetLastError());
+    FVM_HIP(hipDeviceSynchronize());
+    return FVM_OK;
+    FVM_API_END
 }
 
 int fvm_system_get(const fvm_system *S, int32_t *row, int32_t *col, double *diag, double *off, double *b,

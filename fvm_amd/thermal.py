@@ -13,6 +13,8 @@ import numpy as np
 from ._lib import check, lib
 from .linear_system import LinearSystem
 
+FVM_BC_DIRICHLET, FVM_BC_NEUMANN = 0, 1   # enum of fvm_amd.h
+
 BC_DIRICHLET = 0
 BC_NEUMANN = 1
 
@@ -125,6 +127,11 @@ class System:
         return dict(n_self=self.n_self, n_total=nt, row=row, col=col[:nz], diag=diag, off=off[:nz], b=b,
                     is_boundary=isb)
 
+    def post_solve(self, delta):
+        """LinearSystem::postSolve (LinearSystem.cpp:250-252): the boundary rows'
+        entries of the device vector delta (n_total) from the self rows' ones."""
+        check(lib().fvm_system_post_solve(self._h, ctypes.c_void_p(delta.data_ptr())))
+
     def linear_system(self):
         h = self.to_host()
         return LinearSystem(h["n_self"], h["n_total"], h["row"], h["col"], h["diag"], h["off"], h["b"],
@@ -156,3 +163,64 @@ def assemble_phonon(mesh, vg, tau, e0, x):
     check(lib().fvm_phonon_assemble(mesh._h, _vp(v), float(tau), ctypes.c_void_p(e0.data_ptr()),
                                     ctypes.c_void_p(x.data_ptr()), ctypes.byref(h)))
     return System(h)
+
+
+class ThermalModel:
+    """ThermalModel::advance (ThermalModel_impl.h:424-456) on the device.
+
+    Each outer iteration: linearize around the current temperature x (assemble,
+    with initSolve's elimination), solve for the delta with ``solver`` (an
+    fvm_amd.AMG; the hierarchy is rebuilt per iteration as after the reference's
+    cleanup), postSolve (boundary rows), x += delta. The stop rule is the
+    reference's: the solver's returned initial norm against the first one.
+    """
+
+    def __init__(self, mesh, k, source, x, bc_types, bc_values, solver, relative_tolerance=1e-8,
+                 absolute_tolerance=1e-20):
+        self.mesh, self.k, self.source, self.x = mesh, k, source, x
+        self.bc_types, self.bc_values = bc_types, bc_values
+        self.solver = solver
+        self.relativeTolerance = relative_tolerance      # ThermalModelOptions
+        self.absoluteTolerance = absolute_tolerance
+        self.niters = 0
+        self.initial_norm = None
+        self.norms = []
+        # Dirichlet ghost cells: the linearization sets them to their boundary values
+        # (GenericBCS.h:104); the assembly itself reads x only
+        import torch
+        a = mesh.arrays()
+        gp = np.asarray(a["group_ptr"])
+        fc = np.asarray(a["face_cells"]).reshape(-1, 2)
+        idx, val = [], []
+        for g in range(1, len(gp) - 1):
+            if int(bc_types[g]) == FVM_BC_DIRICHLET:
+                idx.append(fc[gp[g]:gp[g + 1], 1])
+                val.append(np.full(gp[g + 1] - gp[g], float(bc_values[g])))
+        self._dir_idx = torch.tensor(np.concatenate(idx) if idx else np.zeros(0, np.int64), dtype=torch.int64,
+                                     device=x.device)
+        self._dir_val = torch.tensor(np.concatenate(val) if val else np.zeros(0), dtype=torch.float64,
+                                     device=x.device)
+
+    def advance(self, niter):
+        import torch
+        from ._lib import check as _check
+        for _ in range(niter):
+            S = assemble(self.mesh, self.k, self.source, self.x, self.bc_types, self.bc_values)
+            if self._dir_idx.numel():
+                self.x[self._dir_idx] = self._dir_val          # _x[c1] = bValue
+            torch.cuda.synchronize(self.x.device)              # torch's stream, then the library's
+            self.solver._push_params()
+            _check(lib().fvm_amg_setup_system(self.solver._h, S._h))
+            delta = torch.zeros_like(self.x)
+            r_norm = self.solver.solve_device(S.device_pointers()["b"], delta)[0]
+            if self.initial_norm is None:
+                self.initial_norm = r_norm
+            ratio = r_norm / self.initial_norm if self.initial_norm else 0.0
+            self.norms.append(r_norm)
+            S.post_solve(delta)
+            self.x += delta                                # updateSolution
+            torch.cuda.synchronize(self.x.device)
+            self.niters += 1
+            if r_norm < self.absoluteTolerance or ratio < self.relativeTolerance:
+                break
+        return self.norms

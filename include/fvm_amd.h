@@ -400,6 +400,11 @@ int fvm_system_device(const fvm_system *S, const int32_t **row, const int32_t **
                       const uint8_t **is_boundary);
 int fvm_system_get(const fvm_system *S, int32_t *row, int32_t *col, double *diag, double *off,
                    double *b, uint8_t *is_boundary);
+/* LinearSystem::postSolve (LinearSystem.cpp:250-252) -> CRMatrix::solveBoundary
+ * (CRMatrix.h:433-454): after a solve for the self rows, every boundary row
+ * nr >= n_self gets d_delta[nr] = -(b[nr] + sum_j off * d_delta[col]) / diag[nr]
+ * (device vector of n_total entries). */
+int fvm_system_post_solve(const fvm_system *S, double *d_delta);
 int fvm_system_destroy(fvm_system *S);
 /* fvm_amg_setup_device on an assembled system */
 int fvm_amg_setup_system(fvm_amg *h, const fvm_system *S);
