@@ -170,8 +170,8 @@ class ThermalModel:
 
     Each outer iteration: linearize around the current temperature x (assemble,
     with initSolve's elimination), solve for the delta with ``solver`` (an
-    fvm_amd.AMG; the hierarchy is rebuilt per iteration as after the reference's
-    cleanup), postSolve (boundary rows), x += delta. The stop rule is the
+    fvm_amd.AMG, or a BCGStab / CG / FGMRES around one; the hierarchy is rebuilt per
+    iteration as after the reference's cleanup), postSolve (boundary rows), x += delta. The stop rule is the
     reference's: the solver's returned initial norm against the first one.
     """
 
@@ -209,8 +209,9 @@ class ThermalModel:
             if self._dir_idx.numel():
                 self.x[self._dir_idx] = self._dir_val          # _x[c1] = bValue
             torch.cuda.synchronize(self.x.device)              # torch's stream, then the library's
-            self.solver._push_params()
-            _check(lib().fvm_amg_setup_system(self.solver._h, S._h))
+            pc = getattr(self.solver, "preconditioner", self.solver)   # an AMG, or a Krylov solver's AMG
+            pc._push_params()
+            _check(lib().fvm_amg_setup_system(pc._h, S._h))
             delta = torch.zeros_like(self.x)
             r_norm = self.solver.solve_device(S.device_pointers()["b"], delta)[0]
             if self.initial_norm is None:

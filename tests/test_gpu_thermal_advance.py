@@ -46,3 +46,33 @@ def test_advance_matches_oracle(N, jit, seed):
     assert len(norms) == len(norms_o) == model.niters
     np.testing.assert_allclose(norms, norms_o, rtol=1e-12)
     assert np.array_equal(x.cpu().numpy(), xo)
+
+
+def test_advance_with_bcgstab_around_the_amg():
+    """The same loop with the reference's BiCGStab around the AMG: not the same
+    arithmetic as the oracle's AMG-only loop, so the bar is the physics: the loop
+    stops at its second linearization, and the temperatures agree with the oracle's
+    to the solver tolerance."""
+    import torch
+    import fvm_amd
+    from fvm_amd.thermal import Mesh, ThermalModel
+    N, jit, seed = 6, 0.2, 5
+    mesh = hexmesh.hex_mesh(N, jit, seed)
+    nct = mesh["n_cells"] + mesh["n_ghost"]
+    rng = np.random.default_rng(seed)
+    k = np.exp(rng.normal(0, 1.0, nct))
+    s = rng.uniform(0, 1, nct)
+    assemble, solve = _oracle_parts(mesh, k, s)
+    xo, _ = MR.thermal_advance(assemble, solve, np.zeros(nct), niter=4, rel_tol=1e-8, abs_tol=1e-20,
+                               dirichlet=dirichlet_ghosts(mesh))
+    amg = fvm_amd.AMG()
+    amg.verbosity = 0
+    solver = fvm_amd.BCGStab(amg)
+    solver.relativeTolerance = 1e-12
+    dev = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda:0")
+    x = torch.zeros(nct, dtype=torch.float64, device="cuda:0")
+    model = ThermalModel(Mesh.hex(N, jit, seed), dev(k), dev(s), x, BC_T, BC_V, solver)
+    norms = model.advance(4)
+    assert len(norms) == 2 and norms[1] < 1e-8 * norms[0]
+    xg = x.cpu().numpy()
+    assert np.abs(xg - xo).max() <= 1e-9 * np.abs(xo).max()
