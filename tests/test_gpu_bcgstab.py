@@ -25,6 +25,8 @@ G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 def _system(kind):
     if kind == "kat226":
         return O.load_mm(os.path.join(G, "MatrixMarket226.dat"), os.path.join(G, "rhs226.dat"))
+    if kind == "cav32":
+        return O.load_mm(os.path.join(G, "cav32_matrix_mesh0.mat"), os.path.join(G, "cav32_matrix.rhs"))
     return matgen.grid3d(12, seed=6, ghosts=False, contrast=1.5)
 
 
@@ -55,7 +57,7 @@ def _oracle(m, method=None):
     return method(matvec, smooth, m["b"][:n], np.zeros(n), n_max=100, rel_tol=1e-8)
 
 
-@pytest.mark.parametrize("kind", ["kat226", "grid"])
+@pytest.mark.parametrize("kind", ["kat226", "cav32", "grid"])
 @pytest.mark.parametrize("method", ["bcgstab", "cg"])
 def test_reference_order_matches_oracle(kind, method):
     import torch
