@@ -50,6 +50,7 @@ class AMG:
         object.__setattr__(self, "_p", p)
         self._finest = None
         self._print_file = None
+        self.device = int(device)
 
     # parameter attributes map onto the C struct
     def __getattr__(self, name):
@@ -228,6 +229,7 @@ class BCGStab:
         check(lib().fvm_linsolver_default_params(ctypes.byref(p)))
         self._p = p
         self._hist = np.zeros(0)
+        self._total = 0
 
     def __getattr__(self, name):
         if name in BCGStab._FIELDS:
@@ -254,10 +256,38 @@ class BCGStab:
         self._hist = buf[:k].copy()
         self.lastIterations = it.value
         self.lastResidual = rf.value
+        self._total += it.value
         return r0.value, rf.value, it.value
+
+    def solve(self, ls):
+        """Host LinearSystem: ls.delta receives the correction; returns the initial
+        1-norm, as the reference's solve does."""
+        import torch
+        pc = self.preconditioner
+        if pc is None:
+            raise ValueError("%s needs a preconditioner" % type(self).__name__)
+        if pc._finest is not ls:
+            pc.setup(ls)
+        b = torch.from_numpy(ls.b).to("cuda:%d" % pc.device)
+        x = torch.from_numpy(ls.delta).to("cuda:%d" % pc.device)
+        r0, _, _ = self.solve_device(b, x)
+        ls.delta[:ls.n_self] = x.cpu().numpy()[:ls.n_self]
+        return r0
 
     def history(self):
         return self._hist.copy()
+
+    def getTotalIterations(self):
+        """Iterations over all solves (BCGStab.h:31)."""
+        return self._total
+
+    def cleanup(self):
+        """The preconditioner's cleanup (BCGStab.cpp:21-24)."""
+        if self.preconditioner is not None:
+            self.preconditioner.cleanup()
+
+    def smooth(self, ls):
+        raise RuntimeError("cannot use %s as preconditioner" % type(self).__name__)   # BCGStab.cpp:172-176
 
 
 class CG(BCGStab):

@@ -101,3 +101,31 @@ def test_bcgstab_coloured_reaches_stop_rule(prec):
     x2 = torch.zeros_like(x)
     r0b, rfb, itb = solver.solve_device(_dev(m["b"], torch.float64), x2)
     assert (r0b, rfb, itb) == (r0, rf, it) and torch.equal(x, x2)
+
+
+@pytest.mark.parametrize("cls", ["BCGStab", "CG"])
+def test_host_solve_and_reference_methods(cls):
+    """solve(ls) on a host LinearSystem gives the device solve's result; the
+    reference's getTotalIterations (BCGStab.h:31) adds up, cleanup reaches the
+    preconditioner, and smooth refuses (BCGStab.cpp:172-176)."""
+    import torch
+    import fvm_amd
+    m = _system("kat226")
+    ls = fvm_amd.LinearSystem(m["n_self"], m["n_total"], m["row"], m["col"], m["diag"], m["off"], m["b"])
+    amg = fvm_amd.AMG()
+    amg.verbosity = 0
+    solver = getattr(fvm_amd, cls)(amg)
+    r0 = solver.solve(ls)
+    it1 = solver.lastIterations
+    amg2 = _gpu(m, 0, 0)
+    ref = getattr(fvm_amd, cls)(amg2)
+    x = torch.zeros(m["n_total"], dtype=torch.float64, device="cuda:0")
+    r0d, _, itd = ref.solve_device(_dev(m["b"], torch.float64), x)
+    assert r0 == r0d and it1 == itd
+    assert np.array_equal(ls.delta[:m["n_self"]], x.cpu().numpy()[:m["n_self"]])
+    ls.delta[:] = 0
+    solver.solve(ls)
+    assert solver.getTotalIterations() == 2 * it1
+    solver.cleanup()
+    with pytest.raises(RuntimeError):
+        solver.smooth(ls)
