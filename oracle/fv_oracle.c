@@ -22,9 +22,11 @@
  *
  * Pinned by tests/test_fv_oracle.py: closed forms on uniform meshes (|A| = h^2,
  * V = h^3, interior coefficient k h, Dirichlet coefficient 2 k h, Neumann rows
- * eliminate to zero flux), divergence-theorem identities on jittered meshes
- * (sum of volumes = domain volume, closed-surface sum of A = 0), and the
- * THERMAL_MATRIX golden matrix (2-D cavity) through the same face loops.
+ * eliminate to zero flux) and divergence-theorem identities on jittered meshes
+ * (sum of volumes = domain volume, closed-surface sum of A = 0); by
+ * tests/test_fv_closed_forms.py: the exact series-resistance solution of a
+ * layered conductivity (harmonic face mean) and the affine closed form of the
+ * coefficient k |A|^2 / (A . ds) on sheared parallelepiped meshes.
  */
 #include <math.h>
 #include <stdlib.h>
