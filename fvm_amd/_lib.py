@@ -65,6 +65,8 @@ SIGNATURES = {
     "fvm_amg_level_info": (_i, [_p, _i, _p, _p]),
     "fvm_amg_level_get": (_i, [_p, _i, _p, _p, _p, _p, _p]),
     "fvm_amg_setup_stats": (_i, [_p, _p, _p]),
+    "fvm_amg_direct_level": (_i, [_p, _p, _p, _p, _p]),
+    "fvm_amg_direct_apply": (_i, [_p, _p, _p]),
     "fvm_amg_set_kernel_timing": (_i, [_p, _i]),
     "fvm_amg_set_tuning": (_i, [_p, ctypes.c_char_p, _ll]),
     "fvm_amg_get_tuning": (_i, [_p, ctypes.c_char_p, ctypes.POINTER(_ll)]),

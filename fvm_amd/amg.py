@@ -170,6 +170,24 @@ class AMG:
         return dict(n=n.value, nnz=nnz.value, row=row, col=col[:nnz.value], diag=diag[:n.value],
                     off=off[:nnz.value], ci=ci[:n.value])
 
+    def direct_level(self, with_inverse=False):
+        """Coarsest dense solve of the mixed cycle: dict(level, n, attempts[, inv])."""
+        lv, n, att = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().fvm_amg_direct_level(self._h, ctypes.byref(lv), ctypes.byref(n), ctypes.byref(att), None))
+        out = dict(level=lv.value, n=n.value, attempts=att.value)
+        if with_inverse and n.value > 0:
+            inv = np.zeros((n.value, n.value))
+            check(lib().fvm_amg_direct_level(self._h, None, None, None, _ptr(inv)))
+            out["inv"] = inv
+        return out
+
+    def direct_apply(self, b):
+        """x = -A^{-1} b on the direct level (the cycle's dense-apply kernel)."""
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros_like(b)
+        check(lib().fvm_amg_direct_apply(self._h, _ptr(b), _ptr(x)))
+        return x
+
     def set_tuning(self, key, value):
         """Performance knob that does not change results (see fvm_amg_set_tuning)."""
         check(lib().fvm_amg_set_tuning(self._h, key.encode(), int(value)))

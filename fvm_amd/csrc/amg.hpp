@@ -134,6 +134,8 @@ struct fvm_amg {
     fvm::DArr<double> h_stage_b, h_stage_x;
     // mixed-precision coarsest dense inverse (FP64)
     int direct_level = -1;
+    bool direct_tried = false;         // factorization attempted since the last setup
+    int direct_attempts = 0;           // Gauss-Jordan runs since the handle was created
     fvm::DArr<double> dense_inv;
     // tuning knobs (fvm_amg_set_tuning): rows below which a level runs in the
     // single-workgroup tail of the V-cycle / is coarsened by one workgroup
@@ -187,6 +189,7 @@ void build_hierarchy_batched(fvm_amg *h, BatchInfo &B);
 void solve_device(fvm_amg *h, const double *d_b, double *d_x, double *rnorm0, double *rnorm_final,
                   int *cycles);
 void smooth_device(fvm_amg *h, const double *d_b, double *d_x);
+void direct_apply(fvm_amg *h, const double *d_b, double *d_x);
 void bcgstab_solve(fvm_amg *h, const fvm_linsolver_params &p, const double *d_b, double *d_x, double *rnorm0,
                    double *rnorm_final, int *iterations, std::vector<double> &hist);
 void cg_solve(fvm_amg *h, const fvm_linsolver_params &p, const double *d_b, double *d_x, double *rnorm0,

@@ -330,6 +330,7 @@ int fvm_amg_cleanup(fvm_amg *h) {
     h->lev.clear();
     h->dense_inv.release();
     h->direct_level = -1;
+    h->direct_tried = false;
     h->ready = false;
     return FVM_OK;
     FVM_API_END
@@ -371,6 +372,39 @@ int fvm_amg_level_get(const fvm_amg *h, int level, int32_t *row, int32_t *col, d
             for (int i = 0; i < L.n; i++) ci[i] = -1;
     }
     FVM_HIP(hipStreamSynchronize(s));
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_amg_direct_level(const fvm_amg *h, int *level, int *n_rows, int *attempts, double *inv) {
+    FVM_API_BEGIN
+    FVM_CHECK(h, FVM_E_INVALID, "null handle");
+    const int lv = h->ready ? h->direct_level : -1;
+    const int n = lv >= 0 ? h->lev[lv].n : 0;
+    if (level) *level = lv;
+    if (n_rows) *n_rows = n;
+    if (attempts) *attempts = h->direct_attempts;
+    if (inv && n > 0) {
+        DeviceGuard g(h->device);
+        FVM_HIP(hipMemcpyAsync(inv, h->dense_inv.p, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost,
+                               h->stream));
+        FVM_HIP(hipStreamSynchronize(h->stream));
+    }
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_amg_direct_apply(fvm_amg *h, const double *b, double *x) {
+    FVM_API_BEGIN
+    FVM_CHECK(h && b && x, FVM_E_INVALID, "null argument");
+    FVM_CHECK(h->ready && h->direct_level >= 0, FVM_E_INVALID, "the hierarchy has no direct level");
+    HandleScope g(h);
+    const int n = h->lev[h->direct_level].n;
+    DArr<double> db(n), dx(n);
+    FVM_HIP(hipMemcpyAsync(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    fvm::direct_apply(h, db.p, dx.p);
+    FVM_HIP(hipMemcpyAsync(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    FVM_HIP(hipStreamSynchronize(h->stream));
     return FVM_OK;
     FVM_API_END
 }

@@ -850,7 +850,7 @@ Run make_run(fvm_amg *h) {
         for (auto &L : h->lev) prepare_exact_schedule(h, L);
     if (R.mixed) {
         prepare_mixed(h, R.fast);
-        if (h->direct_level < 0) prepare_direct(h);
+        if (h->direct_level < 0 && !h->direct_tried) prepare_direct(h);
     }
     R.last = (int)h->lev.size() - 1;
     if (R.mixed && h->direct_level >= 0) R.last = h->direct_level;
@@ -956,6 +956,17 @@ void smooth_device(fvm_amg *h, const double *d_b, double *d_x) {
     load_finest(R, d_b, d_x);
     one_cycle(R);
     store_finest(R, d_x);
+    FVM_HIP(hipStreamSynchronize(h->stream));
+}
+
+// x = -A^{-1} b on the direct level with the cycle's own kernel, rows in the
+// level's reference order (no colour permutation); device vectors of n entries.
+void direct_apply(fvm_amg *h, const double *d_b, double *d_x) {
+    FVM_CHECK(h->direct_level >= 0, FVM_E_INVALID, "the hierarchy has no direct level");
+    const int n = h->lev[h->direct_level].n;
+    k_dense_apply<double><<<grid_for((long long)n * 64, 256), 256, 0, h->stream>>>(n, h->dense_inv.p, nullptr,
+                                                                                   nullptr, d_b, d_x);
+    FVM_HIP(hipGetLastError());
     FVM_HIP(hipStreamSynchronize(h->stream));
 }
 

@@ -1457,6 +1457,7 @@ void build_hierarchy(fvm_amg *h, int n_self, int n_total, const int *d_row, cons
     h->ready = false;
     h->batch = nullptr;
     h->direct_level = -1;
+    h->direct_tried = false;
     h->tail_prec = -1;
     h->tail_desc.release();
     invalidate_cycle_graphs(h);
@@ -1797,6 +1798,7 @@ void prepare_mixed(fvm_amg *h, bool fast) {
 void prepare_direct(fvm_amg *h) {
     hipStream_t s = h->stream;
     h->direct_level = -1;
+    h->direct_tried = true;   // a singular level is not factored again until the next setup
     int lv = -1;
     for (size_t l = 1; l < h->lev.size(); l++)
         if (h->lev[l].n <= 256 && h->lev[l].n > 0) { lv = (int)l; break; }
@@ -1810,6 +1812,7 @@ void prepare_direct(fvm_amg *h) {
     k_dense_from_csr<<<1, 256, 0, s>>>(n, L.row.p, L.col.p, L.diag.p, L.off.p, A.p);
     k_gauss_jordan<<<1, 1024, 0, s>>>(n, A.p, h->dense_inv.p, st.p);
     FVM_HIP(hipGetLastError());
+    h->direct_attempts++;
     if (read_scalar(st.p, s) == 0) h->direct_level = lv;
 }
 
@@ -1910,6 +1913,7 @@ void build_hierarchy_batched(fvm_amg *h, BatchInfo &B) {
     h->ready = false;
     h->batch = nullptr;
     h->direct_level = -1;
+    h->direct_tried = false;
     h->tail_prec = -1;
     h->tail_desc.release();
     invalidate_cycle_graphs(h);

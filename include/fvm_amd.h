@@ -155,6 +155,16 @@ int fvm_amg_level_info(const fvm_amg *h, int level, int *n_rows, int *nnz);
  * coarsest level). */
 int fvm_amg_level_get(const fvm_amg *h, int level, int32_t *row, int32_t *col, double *diag,
                       double *off, int32_t *ci);
+/* Coarsest dense solve of the mixed-precision cycle (no reference counterpart):
+ * the first coarse level with at most 256 rows is inverted in FP64 by Gauss-Jordan
+ * with partial pivoting; a singular level keeps the smoother instead (level -1)
+ * and is not factored again until the next setup. *level, *n_rows: the inverted
+ * level and its rows (-1 / 0 if none); *attempts: factorizations run by this
+ * handle; inv (may be NULL): host copy of the n_rows x n_rows inverse (row-major,
+ * rows in the level's reference order as returned by fvm_amg_level_get). */
+int fvm_amg_direct_level(const fvm_amg *h, int *level, int *n_rows, int *attempts, double *inv);
+/* x = -A^{-1} b on that level with the cycle's kernel (host vectors of n_rows). */
+int fvm_amg_direct_apply(fvm_amg *h, const double *b, double *x);
 /* Live timing of finest-level kernels with HIP events recorded around their
  * launches on the handle's stream (no reference counterpart; for roofline
  * reporting). kind 0: FP64 residual / SpMV launches; kind 1: smoother sweeps

@@ -51,12 +51,12 @@ def lib():
         L.om_residual_flat.argtypes = [_i, _p, _p, _p, _p, _p, _p, _p]
         L.om_gs_flat.argtypes = [_i, _p, _p, _p, _p, _p, _p, _i]
         L.om_coarsen_flat.argtypes = [_i, _i, _p, _p, _p, _p, _p, _i, _d, _p]
-        L.ofv_metrics.argtypes = [_i, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _p, _p]
+        mesh_args = [_i, _i, _p, _p, _p, _p, _i, _i, _i, _p]
+        L.ofv_metrics.argtypes = mesh_args + [_p, _p, _p, _p, _p]
         L.ofv_thermal_nnz.argtypes = [_i, _p, _i, _i]
-        L.ofv_thermal.argtypes = [_i, _p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _p, _p,
-                                  _p, _p, _p, _p, _p, _p]
-        L.ofv_phonon.argtypes = [_i, _p, _p, _p, _i, _i, _i, _p, _p, _d, _p, _p, _p, _p, _p, _p,
-                                 _p, _p]
+        L.ofv_gradient.argtypes = mesh_args + [_p, _p]
+        L.ofv_thermal.argtypes = mesh_args + [_p, _p, _p, _p, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p]
+        L.ofv_phonon.argtypes = mesh_args + [_p, _d, _p, _p, _p, _p, _p, _p, _p, _p]
         _lib = L
     return _lib
 
@@ -171,35 +171,60 @@ def coarsen(m, group_size=2, theta=0.65, is_bnd=None):
     return nc, ci
 
 
+def _mesh_args(mesh):
+    """(dim, n_faces, nodes, fnptr, face_nodes, face_cells, n_cells, n_ghost, n_groups, gptr)
+    of a mesh dict (tests/hexmesh.py layout; "dim" defaults to 3 and "face_nptr" to
+    four nodes per face)."""
+    nf = mesh["face_cells"].size // 2
+    fnptr = mesh.get("face_nptr")
+    if fnptr is None:
+        fnptr = np.arange(nf + 1, dtype=np.int32) * 4
+    keep = [np.ascontiguousarray(fnptr, np.int32), np.ascontiguousarray(mesh["nodes"], np.float64),
+            np.ascontiguousarray(mesh["face_nodes"], np.int32), np.ascontiguousarray(mesh["face_cells"], np.int32),
+            np.ascontiguousarray(mesh["group_ptr"], np.int32)]
+    fp, nodes, fn, fcl, gp = keep
+    args = [int(mesh.get("dim", 3)), nf, ptr(nodes), ptr(fp), ptr(fn), ptr(fcl), mesh["n_cells"], mesh["n_ghost"],
+            gp.size - 1, ptr(gp)]
+    return args, keep
+
+
 def fv_metrics(mesh):
     """Oracle mesh metrics of a mesh dict (tests/hexmesh.py layout)."""
     nf = mesh["face_cells"].size // 2
     nct = mesh["n_cells"] + mesh["n_ghost"]
     out = dict(fA=np.zeros(3 * nf), fAm=np.zeros(nf), fc=np.zeros(3 * nf), cc=np.zeros(3 * nct),
                vol=np.zeros(nct))
-    gp = np.ascontiguousarray(mesh["group_ptr"], np.int32)
-    lib().ofv_metrics(nf, ptr(mesh["nodes"]), ptr(mesh["face_nodes"]), ptr(mesh["face_cells"]),
-                      mesh["n_cells"], mesh["n_ghost"], gp.size - 1, ptr(gp), ptr(out["fA"]),
-                      ptr(out["fAm"]), ptr(out["fc"]), ptr(out["cc"]), ptr(out["vol"]))
+    args, keep = _mesh_args(mesh)
+    lib().ofv_metrics(*args, ptr(out["fA"]), ptr(out["fAm"]), ptr(out["fc"]), ptr(out["cc"]), ptr(out["vol"]))
     return out
 
 
-def fv_thermal(mesh, k, src, x, bc_type, bc_value):
-    """Oracle thermal assembly -> CRMatrix system dict."""
+def fv_gradient(mesh, x):
+    """Oracle least-squares gradient (GradientModel) of a cell field -> (n_total, 3)."""
+    nct = mesh["n_cells"] + mesh["n_ghost"]
+    g = np.zeros(3 * nct)
+    args, keep = _mesh_args(mesh)
+    lib().ofv_gradient(*args, ptr(np.ascontiguousarray(x, np.float64)), ptr(g))
+    return g.reshape(-1, 3)
+
+
+def fv_thermal(mesh, k, src, x, bc_type, bc_value, conv_flux=None, cont_res=None, central=False):
+    """Oracle thermal (convection-diffusion) assembly -> CRMatrix system dict."""
     L = lib()
     nf = mesh["face_cells"].size // 2
     nc, ng = mesh["n_cells"], mesh["n_ghost"]
     nct = nc + ng
-    nnz = L.ofv_thermal_nnz(nf, ptr(mesh["face_cells"]), nc, ng)
+    nnz = L.ofv_thermal_nnz(nf, ptr(np.ascontiguousarray(mesh["face_cells"], np.int32)), nc, ng)
     row = np.zeros(nct + 1, np.int32); col = np.zeros(max(nnz, 1), np.int32)
     diag = np.zeros(nct); off = np.zeros(max(nnz, 1)); b = np.zeros(nct); isb = np.zeros(nct, np.uint8)
-    gp = np.ascontiguousarray(mesh["group_ptr"], np.int32)
     bt = np.ascontiguousarray(bc_type, np.int32); bv = np.ascontiguousarray(bc_value, np.float64)
     k = np.ascontiguousarray(k, np.float64); src = np.ascontiguousarray(src, np.float64)
     x = np.ascontiguousarray(x, np.float64)
-    L.ofv_thermal(nf, ptr(mesh["nodes"]), ptr(mesh["face_nodes"]), ptr(mesh["face_cells"]), nc, ng,
-                  gp.size - 1, ptr(gp), ptr(bt), ptr(bv), ptr(k), ptr(src), ptr(x), ptr(row), ptr(col),
-                  ptr(diag), ptr(off), ptr(b), ptr(isb))
+    cf = None if conv_flux is None else np.ascontiguousarray(conv_flux, np.float64)
+    cr = None if cont_res is None else np.ascontiguousarray(cont_res, np.float64)
+    args, keep = _mesh_args(mesh)
+    L.ofv_thermal(*args, ptr(bt), ptr(bv), ptr(k), ptr(src), ptr(x), ptr(cf), ptr(cr), int(bool(central)),
+                  ptr(row), ptr(col), ptr(diag), ptr(off), ptr(b), ptr(isb))
     return dict(n_self=nc, n_total=nct, row=row, col=col[:nnz], diag=diag, off=off[:nnz], b=b,
                 is_boundary=isb)
 
@@ -210,15 +235,14 @@ def fv_phonon(mesh, vg, tau, e0, x):
     nf = mesh["face_cells"].size // 2
     nc, ng = mesh["n_cells"], mesh["n_ghost"]
     nct = nc + ng
-    nnz = L.ofv_thermal_nnz(nf, ptr(mesh["face_cells"]), nc, ng)
+    nnz = L.ofv_thermal_nnz(nf, ptr(np.ascontiguousarray(mesh["face_cells"], np.int32)), nc, ng)
     row = np.zeros(nct + 1, np.int32); col = np.zeros(max(nnz, 1), np.int32)
     diag = np.zeros(nct); off = np.zeros(max(nnz, 1)); b = np.zeros(nct); isb = np.zeros(nct, np.uint8)
-    gp = np.ascontiguousarray(mesh["group_ptr"], np.int32)
     v = np.ascontiguousarray(vg, np.float64)
     e0 = np.ascontiguousarray(e0, np.float64)
     x = np.ascontiguousarray(x, np.float64)
-    L.ofv_phonon(nf, ptr(mesh["nodes"]), ptr(mesh["face_nodes"]), ptr(mesh["face_cells"]), nc, ng,
-                 gp.size - 1, ptr(gp), ptr(v), ctypes.c_double(tau), ptr(e0), ptr(x), ptr(row), ptr(col),
+    args, keep = _mesh_args(mesh)
+    L.ofv_phonon(*args, ptr(v), ctypes.c_double(tau), ptr(e0), ptr(x), ptr(row), ptr(col),
                  ptr(diag), ptr(off), ptr(b), ptr(isb))
     return dict(n_self=nc, n_total=nct, row=row, col=col[:nnz], diag=diag, off=off[:nnz], b=b,
                 is_boundary=isb)

@@ -1,9 +1,9 @@
 """Pins oracle/model_ref.py (LinearSystem::postSolve -> CRMatrix::solveBoundary,
 ThermalModel::advance) with the C oracle's assembly and AMG: solveBoundary leaves
-every boundary row's equation satisfied; the advance loop of the linear problem
-stops at its second linearization, whose initial norm is the first one's times the
-solver tolerance, the self rows' residual at the final temperature is small, and
-Dirichlet ghosts hold their boundary values."""
+every boundary row's equation satisfied; on a jittered mesh the advance loop
+converges geometrically (the secondary flux is a deferred correction from the
+current field's gradient), the self rows' residual at the final temperature is
+small, and Dirichlet ghosts hold their boundary values."""
 import os
 import sys
 
@@ -52,10 +52,12 @@ def test_advance_dirichlet_ghosts_hold_boundary_values():
     s = rng.uniform(0, 1, nc + ng)
     x0 = np.zeros(nc + ng)
     assemble, solve = _oracle_parts(mesh, k, s)
-    x, norms = MR.thermal_advance(assemble, solve, x0, niter=4, rel_tol=1e-8, abs_tol=1e-20,
+    x, norms = MR.thermal_advance(assemble, solve, x0, niter=40, rel_tol=1e-8, abs_tol=1e-20,
                                   dirichlet=dirichlet_ghosts(mesh))
-    # linear problem: the second linearization starts from the converged field
-    assert len(norms) == 2 and norms[1] < 1e-8 * norms[0]
+    # the non-orthogonal secondary flux enters b from the gradient of the current
+    # field (deferred correction): the outer norms fall geometrically to the tolerance
+    assert 2 < len(norms) < 40 and norms[-1] < 1e-8 * norms[0]
+    assert all(norms[i + 1] < 0.5 * norms[i] for i in range(len(norms) - 1))
     gp = mesh["group_ptr"]
     fc = mesh["face_cells"].reshape(-1, 2)
     for g in range(1, len(gp) - 1):
