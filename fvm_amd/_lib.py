@@ -83,7 +83,11 @@ SIGNATURES = {
     "fvm_cg_solve_device": (_i, [_p, ctypes.POINTER(LinSolverParams), _p, _p, _p, _p, _p, _p, _i]),
     "fvm_mesh_create_hex": (_i, [ctypes.POINTER(_p), _i, _i, _d, ctypes.c_uint64]),
     "fvm_mesh_create": (_i, [ctypes.POINTER(_p), _i, _i, _p, _i, _p, _p, _i, _i, _i, _p]),
+    "fvm_mesh_create_general": (_i, [ctypes.POINTER(_p), _i, _i, _i, _p, _i, _p, _p, _p, _i, _i, _i, _p]),
     "fvm_mesh_destroy": (_i, [_p]),
+    "fvm_mesh_topology": (_i, [_p, _p, _p, _p]),
+    "fvm_mesh_gradient": (_i, [_p, _p, _p]),
+    "fvm_convdiff_assemble": (_i, [_p, _p, _p, _p, _p, _p, _i, _p, _p, ctypes.POINTER(_p)]),
     "fvm_mesh_info": (_i, [_p, _p, _p, _p, _p, _p]),
     "fvm_mesh_get": (_i, [_p, _p, _p, _p, _p]),
     "fvm_mesh_metrics": (_i, [_p, _p, _p, _p, _p, _p]),

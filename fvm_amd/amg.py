@@ -117,6 +117,8 @@ class AMG:
         cyc = ctypes.c_int()
         check(lib().fvm_amg_solve_device(self._h, _dptr(b), _dptr(x), ctypes.byref(r0), ctypes.byref(rf),
                                          ctypes.byref(cyc)))
+        self.lastCycles = cyc.value
+        self.lastResidual = rf.value
         return r0.value, rf.value, cyc.value
 
     def smooth(self, ls):

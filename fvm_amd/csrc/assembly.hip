@@ -213,47 +213,81 @@ __global__ void k_cc_build(int nct, const int *cf_row, const int *cf_col, const 
 }
 
 // ------------------------------------------------------------------ metrics
-__global__ void k_face_metrics(int nf, const double *xyz, const int *fnodes, double *fA, double *fAm,
-                               double *fc) {
+// Face area vectors by node count (MeshMetricsCalculator_impl.h:239-305: 2 nodes
+// in 2-D, triangles, quads, general polygons), |A|, and face centroids (node
+// average, then the correction of faces with more than 3 nodes, :59-118).
+__global__ void k_face_metrics(int nf, const double *xyz, const int *fnptr, const int *fnodes, double *fA,
+                               double *fAm, double *fc) {
     const double twoThirds = 2. / 3.;
     for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
-        const int *q = fnodes + 4 * f;
-        double x[4][3];
-        for (int t = 0; t < 4; t++)
-            for (int a = 0; a < 3; a++) x[t][a] = xyz[3 * q[t] + a];
-        double d20[3], d31[3], A[3];
-        for (int a = 0; a < 3; a++) { d20[a] = x[2][a] - x[0][a]; d31[a] = x[3][a] - x[1][a]; }
-        A[0] = 0.5 * (d20[1] * d31[2] - d20[2] * d31[1]);
-        A[1] = 0.5 * (d20[2] * d31[0] - d20[0] * d31[2]);
-        A[2] = 0.5 * (d20[0] * d31[1] - d20[1] * d31[0]);
+        const int *q = fnodes + fnptr[f];
+        const int nn = fnptr[f + 1] - fnptr[f];
+        double A[3];
+        if (nn == 2) {
+            const double *x0 = xyz + 3 * q[0], *x1 = xyz + 3 * q[1];
+            A[0] = x1[1] - x0[1];
+            A[1] = -(x1[0] - x0[0]);
+            A[2] = 0.;
+        } else if (nn == 3) {
+            const double *x0 = xyz + 3 * q[0], *x1 = xyz + 3 * q[1], *x2 = xyz + 3 * q[2];
+            double d10[3], d20[3];
+            for (int a = 0; a < 3; a++) { d10[a] = x1[a] - x0[a]; d20[a] = x2[a] - x0[a]; }
+            A[0] = 0.5 * (d10[1] * d20[2] - d10[2] * d20[1]);
+            A[1] = 0.5 * (d10[2] * d20[0] - d10[0] * d20[2]);
+            A[2] = 0.5 * (d10[0] * d20[1] - d10[1] * d20[0]);
+        } else if (nn == 4) {
+            const double *x0 = xyz + 3 * q[0], *x1 = xyz + 3 * q[1], *x2 = xyz + 3 * q[2], *x3 = xyz + 3 * q[3];
+            double d20[3], d31[3];
+            for (int a = 0; a < 3; a++) { d20[a] = x2[a] - x0[a]; d31[a] = x3[a] - x1[a]; }
+            A[0] = 0.5 * (d20[1] * d31[2] - d20[2] * d31[1]);
+            A[1] = 0.5 * (d20[2] * d31[0] - d20[0] * d31[2]);
+            A[2] = 0.5 * (d20[0] * d31[1] - d20[1] * d31[0]);
+        } else {
+            A[0] = A[1] = A[2] = 0.0;
+            for (int t = 0; t < nn; t++) {
+                const double *u = xyz + 3 * q[t], *v = xyz + 3 * q[(t + 1) % nn];
+                double xm[3], dr[3];
+                for (int a = 0; a < 3; a++) { xm[a] = 0.5 * (v[a] + u[a]); dr[a] = v[a] - u[a]; }
+                A[0] += xm[1] * dr[2];
+                A[1] += xm[2] * dr[0];
+                A[2] += xm[0] * dr[1];
+            }
+        }
         const double Am = sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]);
         double p[3];
-        for (int a = 0; a < 3; a++) p[a] = x[0][a];
-        for (int t = 1; t < 4; t++)
-            for (int a = 0; a < 3; a++) p[a] += x[t][a];
-        for (int a = 0; a < 3; a++) p[a] /= 4.0;
-        double en[3];
-        for (int a = 0; a < 3; a++) en[a] = A[a] / Am;
-        double denom = 0.0, cfc[3] = {0.0, 0.0, 0.0};
-        for (int t = 0; t < 4; t++) {
-            const double *u = x[t], *v = x[(t + 1) & 3];
-            double r0[3], r1[3], tri[3], xm[3];
-            for (int a = 0; a < 3; a++) { r0[a] = u[a] - p[a]; r1[a] = v[a] - p[a]; }
-            tri[0] = 0.5 * (r0[1] * r1[2] - r0[2] * r1[1]);
-            tri[1] = 0.5 * (r0[2] * r1[0] - r0[0] * r1[2]);
-            tri[2] = 0.5 * (r0[0] * r1[1] - r0[1] * r1[0]);
-            const double triP = tri[0] * en[0] + tri[1] * en[1] + tri[2] * en[2];
-            for (int a = 0; a < 3; a++) xm[a] = 0.5 * (u[a] + v[a]);
-            for (int a = 0; a < 3; a++) cfc[a] += (twoThirds * (xm[a] - p[a])) * triP;
-            denom += triP;
+        for (int a = 0; a < 3; a++) p[a] = xyz[3 * q[0] + a];
+        for (int t = 1; t < nn; t++)
+            for (int a = 0; a < 3; a++) p[a] += xyz[3 * q[t] + a];
+        for (int a = 0; a < 3; a++) p[a] /= (double)nn;
+        if (nn > 3) {
+            double en[3];
+            for (int a = 0; a < 3; a++) en[a] = A[a] / Am;
+            double denom = 0.0, cfc[3] = {0.0, 0.0, 0.0};
+            for (int t = 0; t < nn; t++) {
+                const double *u = xyz + 3 * q[t], *v = xyz + 3 * q[(t + 1) % nn];
+                double r0[3], r1[3], tri[3], xm[3];
+                for (int a = 0; a < 3; a++) { r0[a] = u[a] - p[a]; r1[a] = v[a] - p[a]; }
+                tri[0] = 0.5 * (r0[1] * r1[2] - r0[2] * r1[1]);
+                tri[1] = 0.5 * (r0[2] * r1[0] - r0[0] * r1[2]);
+                tri[2] = 0.5 * (r0[0] * r1[1] - r0[1] * r1[0]);
+                const double triP = tri[0] * en[0] + tri[1] * en[1] + tri[2] * en[2];
+                for (int a = 0; a < 3; a++) xm[a] = 0.5 * (u[a] + v[a]);
+                for (int a = 0; a < 3; a++) cfc[a] += (twoThirds * (xm[a] - p[a])) * triP;
+                denom += triP;
+            }
+            for (int a = 0; a < 3; a++) cfc[a] /= denom;
+            for (int a = 0; a < 3; a++) p[a] += cfc[a];
         }
-        for (int a = 0; a < 3; a++) cfc[a] /= denom;
         for (int a = 0; a < 3; a++) {
             fA[3 * f + a] = A[a];
-            fc[3 * f + a] = p[a] + cfc[a];
+            fc[3 * f + a] = p[a];
         }
         fAm[f] = Am;
     }
+}
+
+__global__ void k_iota_stride(int n, int stride, int *v) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += gridDim.x * blockDim.x) v[i] = stride * i;
 }
 
 __global__ void k_cell_centroids(int n_cells, int nct, const int *cf_row, const int *cf_col,
@@ -274,7 +308,7 @@ __global__ void k_cell_centroids(int n_cells, int nct, const int *cf_row, const 
     }
 }
 
-__global__ void k_cell_volumes(int n_cells, const int *cf_row, const int *cf_col, const int *fcells,
+__global__ void k_cell_volumes(int n_cells, double rdim, const int *cf_row, const int *cf_col, const int *fcells,
                                const double *fA, const double *fc, const double *cc, double *vol) {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n_cells; c += gridDim.x * blockDim.x) {
         double v = 0.0;
@@ -282,7 +316,7 @@ __global__ void k_cell_volumes(int n_cells, const int *cf_row, const int *cf_col
             const int f = cf_col[q];
             double d[3];
             for (int a = 0; a < 3; a++) d[a] = fc[3 * f + a] - cc[3 * c + a];
-            const double t = (d[0] * fA[3 * f] + d[1] * fA[3 * f + 1] + d[2] * fA[3 * f + 2]) / 3.0;
+            const double t = (d[0] * fA[3 * f] + d[1] * fA[3 * f + 1] + d[2] * fA[3 * f + 2]) / rdim;
             if (fcells[2 * f] == c) v += t;
             else v -= t;
         }
@@ -303,10 +337,118 @@ __device__ __forceinline__ double harmonic(double x0, double x1) {
     return sum;
 }
 
-// per face: diffusion coefficient and flux (DiffusionDiscretization.h:172-212)
+// ------------------------------------------------------------------ least-squares gradient
+// GradientModel::getLeastSquaresGradientMatrix3D / 2D (GradientModel.h:128-420):
+// per cellCells entry of a self cell, K s^ / |ds| with s^ = ds / |ds| the unit step to
+// the neighbour and K the inverse of the cell's normal matrix sum s^ s^T (summed in
+// CSR order, used when its determinant exceeds 1e-6 in 3-D / 1e-26 in 2-D);
+// Green-Gauss face terms 0.5 A / V (sign of the face side) for degenerate cells;
+// ghost rows keep -s^ / |ds|. Each row's entries are its faces in ascending order
+// (conformal meshes: one face per neighbour), so the sums follow the CSR order.
+__device__ __forceinline__ void unit_step(const double *cc, int c0, int c1, bool first, double *u,
+                                          double &dsMag) {
+    double ds[3];
+    for (int a = 0; a < 3; a++) ds[a] = cc[3 * c1 + a] - cc[3 * c0 + a];
+    dsMag = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]);
+    for (int a = 0; a < 3; a++) u[a] = first ? ds[a] / dsMag : -ds[a] / dsMag;
+}
+
+__global__ void k_lsq_coeffs(int n_cells, int nct, int dim, const int *cf_row, const int *cf_col,
+                             const int *fcells, const int *pos01, const int *pos10, const double *cc,
+                             const double *fA, const double *vol, double *gc) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nct; c += gridDim.x * blockDim.x) {
+        const int qb = cf_row[c], qe = cf_row[c + 1];
+        bool degen = false;
+        double K[6] = {0, 0, 0, 0, 0, 0};   // xx xy xz yy yz zz
+        if (c < n_cells) {
+            double Ixx = 0, Iyy = 0, Izz = 0, Ixy = 0, Ixz = 0, Iyz = 0;
+            for (int q = qb; q < qe; q++) {
+                const int f = cf_col[q];
+                double u[3], m;
+                unit_step(cc, fcells[2 * f], fcells[2 * f + 1], fcells[2 * f] == c, u, m);
+                Ixx += u[0] * u[0];
+                Iyy += u[1] * u[1];
+                Izz += u[2] * u[2];
+                Ixy += u[0] * u[1];
+                Ixz += u[0] * u[2];
+                Iyz += u[1] * u[2];
+            }
+            if (dim == 2) {
+                const double det = Ixx * Iyy - Ixy * Ixy;
+                if (det > 1e-26) { K[0] = Iyy / det; K[1] = -Ixy / det; K[3] = Ixx / det; }
+                else degen = true;
+            } else {
+                const double det = Ixx * (Iyy * Izz - Iyz * Iyz) - Ixy * (Ixy * Izz - Iyz * Ixz)
+                                   + Ixz * (Ixy * Iyz - Iyy * Ixz);
+                if (det > 1e-6) {
+                    K[0] = (Iyy * Izz - Iyz * Iyz) / det;
+                    K[1] = -(Ixy * Izz - Iyz * Ixz) / det;
+                    K[2] = (Ixy * Iyz - Iyy * Ixz) / det;
+                    K[3] = (Ixx * Izz - Ixz * Ixz) / det;
+                    K[4] = -(Ixx * Iyz - Ixy * Ixz) / det;
+                    K[5] = (Ixx * Iyy - Ixy * Ixy) / det;
+                } else {
+                    degen = true;
+                }
+            }
+        }
+        for (int q = qb; q < qe; q++) {
+            const int f = cf_col[q];
+            const bool first = fcells[2 * f] == c;
+            double u[3], m, w[3];
+            unit_step(cc, fcells[2 * f], fcells[2 * f + 1], first, u, m);
+            if (c >= n_cells) {
+                for (int a = 0; a < 3; a++) w[a] = u[a] / m;
+            } else if (degen) {
+                const double sgn = first ? 0.5 : -0.5;
+                for (int a = 0; a < 3; a++) w[a] = sgn * fA[3 * f + a] / vol[c];
+            } else if (dim == 2) {
+                w[0] = (K[0] * u[0] + K[1] * u[1]) / m;
+                w[1] = (K[1] * u[0] + K[3] * u[1]) / m;
+                w[2] = 0.0 / m;
+            } else {
+                w[0] = (K[0] * u[0] + K[1] * u[1] + K[2] * u[2]) / m;
+                w[1] = (K[1] * u[0] + K[3] * u[1] + K[4] * u[2]) / m;
+                w[2] = (K[2] * u[0] + K[4] * u[1] + K[5] * u[2]) / m;
+            }
+            const int p = first ? pos01[f] : pos10[f];
+            for (int a = 0; a < 3; a++) gc[3 * (size_t)p + a] = w[a];
+        }
+    }
+}
+
+// GradientMatrix::getGradient (GradientMatrix.h:55-76) of the self cells, entries in
+// CSR order; a ghost cell then takes its face's cell's gradient (GradientModel.h:500-515)
+__global__ void k_gradient(int n_cells, const int *row, const int *col, const double *gc, const double *x,
+                           double *grad) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n_cells; c += gridDim.x * blockDim.x) {
+        double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+        const double xc = x[c];
+        for (int nb = row[c]; nb < row[c + 1]; nb++) {
+            const double v = x[col[nb]] - xc;
+            g0 += gc[3 * (size_t)nb] * v;
+            g1 += gc[3 * (size_t)nb + 1] * v;
+            g2 += gc[3 * (size_t)nb + 2] * v;
+        }
+        grad[3 * c] = g0;
+        grad[3 * c + 1] = g1;
+        grad[3 * c + 2] = g2;
+    }
+}
+
+__global__ void k_grad_ghost(int n_cells, int nct, const int *cf_row, const int *cf_col, const int *fcells,
+                             double *grad) {
+    for (int g = n_cells + blockIdx.x * blockDim.x + threadIdx.x; g < nct; g += gridDim.x * blockDim.x) {
+        const int c0 = fcells[2 * cf_col[cf_row[g]]];
+        for (int a = 0; a < 3; a++) grad[3 * g + a] = grad[3 * c0 + a];
+    }
+}
+
+// per face: diffusion coefficient and flux with the secondary (non-orthogonal)
+// term from the volume-weighted face gradient (DiffusionDiscretization.h:172-212)
 __global__ void k_face_diffusion(int nf, const int *fcells, const double *fA, const double *fAm,
                                  const double *cc, const double *vol, const double *kc,
-                                 const double *x, double *dcoef, double *dflux) {
+                                 const double *x, const double *grad, double *dcoef, double *dflux) {
     for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
         const int c0 = fcells[2 * f], c1 = fcells[2 * f + 1];
         const double vol0 = vol[c0], vol1 = vol[c1];
@@ -321,7 +463,7 @@ __global__ void k_face_diffusion(int nf, const int *fcells, const double *fA, co
         const double diffCoeff = kf * diffMetric;
         double sec[3], gradF[3];
         for (int a = 0; a < 3; a++) sec[a] = kf * (A[a] - ds[a] * diffMetric);
-        for (int a = 0; a < 3; a++) gradF[a] = (0.0 * vol0 + 0.0 * vol1) / (vol0 + vol1);
+        for (int a = 0; a < 3; a++) gradF[a] = (grad[3 * c0 + a] * vol0 + grad[3 * c1 + a] * vol1) / (vol0 + vol1);
         double dsec = 0.0;
         for (int a = 0; a < 3; a++) dsec += gradF[a] * sec[a];
         dcoef[f] = diffCoeff;
@@ -329,10 +471,15 @@ __global__ void k_face_diffusion(int nf, const int *fcells, const double *fA, co
     }
 }
 
-// per cell: diffusion sums in face order, then source (interior cells)
+// per cell: diffusion sums in face order, then the convection terms of its faces in
+// face order (ConvectionDiscretization.h:160-185 upwind; :131-147 central, whose
+// face value is 0.5 (x0 + x0) with the upwind linearization), the continuity
+// residual, then the source (interior cells) -- the order of the reference's
+// discretization list (ThermalModel_impl.h:238-265)
 __global__ void k_cell_assemble(int n_cells, int nct, const int *cf_row, const int *cf_col,
                                 const int *fcells, const int *pos01, const int *pos10,
                                 const int *row, const double *dcoef, const double *dflux,
+                                const double *F, const double *cont_res, int central, const double *x,
                                 const double *vol, const double *src, double *diag, double *off,
                                 double *b) {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nct; c += gridDim.x * blockDim.x) {
@@ -346,41 +493,84 @@ __global__ void k_cell_assemble(int n_cells, int nct, const int *cf_row, const i
             if (first) r += dflux[f];
             else r -= dflux[f];
         }
+        if (F) {
+            for (int q = cf_row[c]; q < cf_row[c + 1]; q++) {
+                const int f = cf_col[q];
+                const int c0 = fcells[2 * f], c1 = fcells[2 * f + 1];
+                const double Ff = F[f];
+                double flux;
+                if (central) flux = 0.5 * Ff * (x[c0] + x[c0]);
+                else flux = Ff > 0.0 ? Ff * x[c0] : Ff * x[c1];
+                if (Ff > 0.0) {
+                    if (c == c0) d -= Ff;
+                    else off[pos10[f]] += Ff;
+                } else {
+                    if (c == c1) d += Ff;
+                    else off[pos01[f]] -= Ff;
+                }
+                if (c == c0) r -= flux;
+                else r += flux;
+            }
+            if (c < n_cells) d += cont_res ? cont_res[c] : 0.0;
+        }
         if (c < n_cells) r += vol[c] * src[c];
         diag[c] = d;
         b[c] = r;
     }
 }
 
-// GenericBCS, interior side: Dirichlet adds coeff01*(Tb - x_ghost) to b[c0]
-// and drops the coefficient (GenericBCS.h:77-100), faces in order.
+// A specified-temperature face with an outflow convecting flux extrapolates
+// (ThermalModel_impl.h:306-318).
+__device__ __forceinline__ bool bc_dirichlet(const Groups &G, int g, const double *F, int f) {
+    return G.type[g] == FVM_BC_DIRICHLET && !(F && F[f] > 0.);
+}
+
+// GenericBCS, interior side, faces in order: Dirichlet adds coeff01 (Tb - x_ghost)
+// to b[c0] and drops the coefficient (GenericBCS.h:77-100); extrapolation folds
+// the ghost's diagonal into the cell (:170-185). Reads the ghost diagonal before
+// k_bc_ghost resets it.
 __global__ void k_bc_interior(int n_cells, const int *cf_row, const int *cf_col, const int *fcells,
-                             const int *pos01, Groups G, const double *x, double *off, double *b) {
+                             const int *pos01, Groups G, const double *F, const double *x, double *diag,
+                             double *off, double *b) {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n_cells; c += gridDim.x * blockDim.x) {
         for (int q = cf_row[c]; q < cf_row[c + 1]; q++) {
             const int f = cf_col[q];
             if (f < G.ptr[1]) continue;
             const int g = group_of(G, f);
-            if (G.type[g] != FVM_BC_DIRICHLET) continue;
-            const double dX = G.value[g] - x[fcells[2 * f + 1]];
-            b[c] += off[pos01[f]] * dX;
-            off[pos01[f]] = 0.0;
+            const int c1 = fcells[2 * f + 1];
+            if (bc_dirichlet(G, g, F, f)) {
+                const double dX = G.value[g] - x[c1];
+                b[c] += off[pos01[f]] * dX;
+                off[pos01[f]] = 0.0;
+            } else if (G.type[g] == FVM_BC_DIRICHLET) {
+                const double dFdX1 = -diag[c1];
+                const double dx01 = x[c] - x[c1];
+                diag[c] += dFdX1;
+                b[c] += dFdX1 * dx01;
+                off[pos01[f]] = 0.0;
+            }
         }
     }
 }
 
-// GenericBCS, ghost side (GenericBCS.h:100-110 Dirichlet, :129-150 Neumann)
-__global__ void k_bc_ghost(int n_cells, int nct, const int *cf_row, const int *cf_col,
-                           const int *pos10, const double *fAm, Groups G, double *diag, double *off,
-                           double *b, uint8_t *isb) {
+// GenericBCS, ghost side (GenericBCS.h:100-110 Dirichlet, :129-150 Neumann,
+// :187-192 extrapolation)
+__global__ void k_bc_ghost(int n_cells, int nct, const int *cf_row, const int *cf_col, const int *fcells,
+                           const int *pos10, const double *fAm, Groups G, const double *F, const double *x,
+                           double *diag, double *off, double *b, uint8_t *isb) {
     for (int gc = n_cells + blockIdx.x * blockDim.x + threadIdx.x; gc < nct; gc += gridDim.x * blockDim.x) {
         const int f = cf_col[cf_row[gc]];
         const int g = group_of(G, f);
         isb[gc] = 0;
-        if (G.type[g] == FVM_BC_DIRICHLET) {
+        if (bc_dirichlet(G, g, F, f)) {
             off[pos10[f]] = 0.0;
             b[gc] = 0.0;
             diag[gc] = -1.0;
+        } else if (G.type[g] == FVM_BC_DIRICHLET) {
+            diag[gc] = -1.0;
+            off[pos10[f]] = 1.0;
+            b[gc] = x[fcells[2 * f]] - x[gc];
+            isb[gc] = 1;
         } else {
             const double fluxB = -b[gc];
             b[gc] = G.value[g] * fAm[f] - fluxB;
@@ -572,15 +762,18 @@ struct fvm_mesh {
     int device = 0;
     hipStream_t stream = nullptr;
     int hexN = 0;
+    int dim = 3;
     int n_cells = 0, n_ghost = 0, n_faces = 0, n_nodes = 0, n_groups = 0;
     std::vector<int> gptr;
     DArr<double> nodes;
-    DArr<int> fnodes, fcells;
+    DArr<int> fnptr, fnodes, fcells;   // face nodes as a CSR (fnptr[n_faces + 1])
+    long long n_face_nodes = 0;
     // derived
-    bool conn = false, metrics = false;
+    bool conn = false, metrics = false, lsq = false;
     DArr<int> cf_row, cf_col, cc_row, cc_col, pos01, pos10;
     long long nnz = 0;
     DArr<double> fA, fAm, fc, cc, vol;
+    DArr<double> gcoef;                // least-squares gradient coefficients, 3 per cellCells entry
 };
 
 struct fvm_system {
@@ -642,10 +835,11 @@ void mesh_metrics(fvm_mesh *m) {
     const int nct = m->n_cells + m->n_ghost, nf = m->n_faces;
     m->fA.alloc(3 * (size_t)nf); m->fAm.alloc(nf); m->fc.alloc(3 * (size_t)nf);
     m->cc.alloc(3 * (size_t)nct); m->vol.alloc(nct);
-    k_face_metrics<<<grid_for(nf, BS), BS, 0, s>>>(nf, m->nodes.p, m->fnodes.p, m->fA.p, m->fAm.p, m->fc.p);
+    k_face_metrics<<<grid_for(nf, BS), BS, 0, s>>>(nf, m->nodes.p, m->fnptr.p, m->fnodes.p, m->fA.p, m->fAm.p,
+                                                   m->fc.p);
     k_cell_centroids<<<grid_for(nct, BS), BS, 0, s>>>(m->n_cells, nct, m->cf_row.p, m->cf_col.p, m->fAm.p,
                                                       m->fc.p, m->cc.p);
-    k_cell_volumes<<<grid_for(m->n_cells, BS), BS, 0, s>>>(m->n_cells, m->cf_row.p, m->cf_col.p,
+    k_cell_volumes<<<grid_for(m->n_cells, BS), BS, 0, s>>>(m->n_cells, (double)m->dim, m->cf_row.p, m->cf_col.p,
                                                            m->fcells.p, m->fA.p, m->fc.p, m->cc.p, m->vol.p);
     if (m->n_ghost)
         k_ghost_volumes<<<grid_for(m->n_ghost, BS), BS, 0, s>>>(m->n_cells, nct, m->cf_row.p, m->cf_col.p,
@@ -653,6 +847,34 @@ void mesh_metrics(fvm_mesh *m) {
     FVM_HIP(hipGetLastError());
     FVM_HIP(hipStreamSynchronize(s));
     m->metrics = true;
+}
+
+// least-squares gradient coefficients of the mesh (computed once)
+void mesh_gradient_coeffs(fvm_mesh *m) {
+    mesh_metrics(m);
+    if (m->lsq) return;
+    FVM_CHECK(m->nnz == 2LL * m->n_faces, FVM_E_INVALID,
+              "the gradient needs a conformal mesh: two cells share at most one face");
+    hipStream_t s = m->stream;
+    const int nct = m->n_cells + m->n_ghost;
+    m->gcoef.alloc(3 * (size_t)std::max<long long>(m->nnz, 1));
+    k_lsq_coeffs<<<grid_for(nct, BS), BS, 0, s>>>(m->n_cells, nct, m->dim, m->cf_row.p, m->cf_col.p, m->fcells.p,
+                                                  m->pos01.p, m->pos10.p, m->cc.p, m->fA.p, m->vol.p,
+                                                  m->gcoef.p);
+    FVM_HIP(hipGetLastError());
+    FVM_HIP(hipStreamSynchronize(s));
+    m->lsq = true;
+}
+
+// GradientModel::compute of a cell field into grad (3 per cell, ghosts included)
+void mesh_gradient(fvm_mesh *m, const double *x, double *grad, hipStream_t s) {
+    const int nct = m->n_cells + m->n_ghost;
+    if (m->n_cells)
+        k_gradient<<<grid_for(m->n_cells, BS), BS, 0, s>>>(m->n_cells, m->cc_row.p, m->cc_col.p, m->gcoef.p, x,
+                                                           grad);
+    if (m->n_ghost)
+        k_grad_ghost<<<grid_for(m->n_ghost, BS), BS, 0, s>>>(m->n_cells, nct, m->cf_row.p, m->cf_col.p,
+                                                             m->fcells.p, grad);
 }
 
 Groups make_groups(const fvm_mesh *m, const int32_t *type, const double *value) {
@@ -668,6 +890,63 @@ Groups make_groups(const fvm_mesh *m, const int32_t *type, const double *value) 
     return G;
 }
 
+}  // namespace
+}  // namespace fvm
+
+namespace fvm {
+namespace {
+__global__ void k_fill_u8(int n, uint8_t v, uint8_t *p) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// a system with the mesh's cellCells pattern
+void system_alloc(fvm_mesh *m, fvm_system *S) {
+    const int nct = m->n_cells + m->n_ghost;
+    S->device = m->device;
+    S->n_self = m->n_cells;
+    S->n_total = nct;
+    S->nnz = m->nnz;
+    S->row.alloc((size_t)nct + 1);
+    S->col.alloc(std::max<long long>(m->nnz, 1));
+    S->off.alloc(std::max<long long>(m->nnz, 1));
+    S->diag.alloc(std::max(nct, 1)); S->b.alloc(std::max(nct, 1)); S->isb.alloc(std::max(nct, 1));
+    FVM_HIP(hipMemcpyAsync(S->row.p, m->cc_row.p, S->row.bytes(), hipMemcpyDeviceToDevice, m->stream));
+    if (m->nnz)
+        FVM_HIP(hipMemcpyAsync(S->col.p, m->cc_col.p, sizeof(int) * m->nnz, hipMemcpyDeviceToDevice, m->stream));
+}
+
+// ThermalModel::linearize + LinearSystem::initSolve into S (pattern already set):
+// gradient of x, diffusion with secondary flux, convection (d_flux may be null),
+// source, boundary conditions, elimination. Scratch: dcoef, dflux (n_faces),
+// grad (3 (n_cells + n_ghost)). Queued on the mesh's stream; no allocation and
+// no host synchronization.
+void assemble_thermal(fvm_mesh *m, fvm_system *S, const double *d_k, const double *d_source, const double *d_x,
+                      const double *d_flux, const double *d_cont, int central, const Groups &G, double *dcoef,
+                      double *dflux, double *grad) {
+    hipStream_t s = m->stream;
+    const int nc = m->n_cells, nct = m->n_cells + m->n_ghost, nf = m->n_faces;
+    mesh_gradient(m, d_x, grad, s);
+    if (nf)
+        k_face_diffusion<<<grid_for(nf, 256), 256, 0, s>>>(nf, m->fcells.p, m->fA.p, m->fAm.p, m->cc.p, m->vol.p,
+                                                           d_k, d_x, grad, dcoef, dflux);
+    if (nct)
+        k_cell_assemble<<<grid_for(nct, 256), 256, 0, s>>>(nc, nct, m->cf_row.p, m->cf_col.p, m->fcells.p,
+                                                           m->pos01.p, m->pos10.p, m->cc_row.p, dcoef, dflux,
+                                                           d_flux, d_cont, central, d_x, m->vol.p, d_source,
+                                                           S->diag.p, S->off.p, S->b.p);
+    if (nc)
+        k_bc_interior<<<grid_for(nc, 256), 256, 0, s>>>(nc, m->cf_row.p, m->cf_col.p, m->fcells.p, m->pos01.p, G,
+                                                        d_flux, d_x, S->diag.p, S->off.p, S->b.p);
+    if (nct) k_fill_u8<<<grid_for(nct, 256), 256, 0, s>>>(nct, 0, S->isb.p);
+    if (m->n_ghost)
+        k_bc_ghost<<<grid_for(m->n_ghost, 256), 256, 0, s>>>(nc, nct, m->cf_row.p, m->cf_col.p, m->fcells.p,
+                                                             m->pos10.p, m->fAm.p, G, d_flux, d_x, S->diag.p,
+                                                             S->off.p, S->b.p, S->isb.p);
+    if (nc)
+        k_eliminate<<<grid_for(nc, 256), 256, 0, s>>>(nc, m->cf_row.p, m->cf_col.p, m->fcells.p, m->pos01.p,
+                                                      m->pos10.p, S->isb.p, S->diag.p, S->off.p, S->b.p);
+    FVM_HIP(hipGetLastError());
+}
 }  // namespace
 }  // namespace fvm
 
@@ -706,8 +985,11 @@ int fvm_mesh_create_hex(fvm_mesh **out, int device, int n, double jitter, uint64
         m->gptr[1] = (int)Fint;
         for (int g = 0; g < 6; g++) m->gptr[g + 2] = (int)(Fint + (long long)(g + 1) * n * n);
         m->nodes.alloc(3 * (size_t)m->n_nodes);
+        m->n_face_nodes = 4LL * m->n_faces;
+        m->fnptr.alloc((size_t)m->n_faces + 1);
         m->fnodes.alloc(4 * (size_t)m->n_faces);
         m->fcells.alloc(2 * (size_t)m->n_faces);
+        fvm::k_iota_stride<<<fvm::grid_for(m->n_faces + 1, 256), 256, 0, m->stream>>>(m->n_faces, 4, m->fnptr.p);
         fvm::k_hex_nodes<<<fvm::grid_for(m->n_nodes, 256), 256, 0, m->stream>>>(n, jitter, seed, m->nodes.p);
         fvm::k_hex_faces<<<fvm::grid_for(m->n_faces, 256), 256, 0, m->stream>>>(n, m->fnodes.p, m->fcells.p);
         FVM_HIP(hipGetLastError());
@@ -722,36 +1004,48 @@ int fvm_mesh_create_hex(fvm_mesh **out, int device, int n, double jitter, uint64
     FVM_API_END
 }
 
-int fvm_mesh_create(fvm_mesh **out, int device, int n_nodes, const double *nodes, int n_faces,
-                    const int32_t *face_nodes, const int32_t *face_cells, int n_cells, int n_ghost,
-                    int n_groups, const int32_t *group_ptr) {
+int fvm_mesh_create_general(fvm_mesh **out, int device, int dim, int n_nodes, const double *nodes, int n_faces,
+                            const int32_t *face_nptr, const int32_t *face_nodes, const int32_t *face_cells,
+                            int n_cells, int n_ghost, int n_groups, const int32_t *group_ptr) {
     FVM_API_BEGIN
-    FVM_CHECK(out && nodes && face_nodes && face_cells && group_ptr, FVM_E_INVALID, "null argument");
+    FVM_CHECK(out && nodes && face_nptr && face_nodes && face_cells && group_ptr, FVM_E_INVALID, "null argument");
+    FVM_CHECK(dim == 2 || dim == 3, FVM_E_INVALID, "dim must be 2 or 3");
+    FVM_CHECK(n_nodes >= 0 && n_faces >= 0 && n_cells >= 0 && n_ghost >= 0, FVM_E_INVALID, "negative count");
     FVM_CHECK(n_groups >= 1 && n_groups <= 64 && group_ptr[0] == 0 && group_ptr[n_groups] == n_faces,
               FVM_E_INVALID, "bad face groups");
+    for (int g = 0; g < n_groups; g++)
+        FVM_CHECK(group_ptr[g] <= group_ptr[g + 1], FVM_E_INVALID, "face groups must be ascending");
     FVM_CHECK(group_ptr[1] == n_faces - n_ghost, FVM_E_INVALID, "one ghost cell per boundary face");
+    FVM_CHECK(face_nptr[0] == 0, FVM_E_INVALID, "face_nptr[0] must be 0");
     const int nct = n_cells + n_ghost;
     for (int f = 0; f < n_faces; f++) {
         const int c0 = face_cells[2 * f], c1 = face_cells[2 * f + 1];
         FVM_CHECK(c0 >= 0 && c0 < n_cells && c1 >= 0 && c1 < nct && c0 != c1, FVM_E_INVALID, "bad face cells");
         if (f >= group_ptr[1]) FVM_CHECK(c1 == n_cells + (f - group_ptr[1]), FVM_E_INVALID, "ghost numbering");
-        for (int t = 0; t < 4; t++)
-            FVM_CHECK(face_nodes[4 * f + t] >= 0 && face_nodes[4 * f + t] < n_nodes, FVM_E_INVALID, "bad face node");
+        const int nn = face_nptr[f + 1] - face_nptr[f];
+        FVM_CHECK(dim == 2 ? nn == 2 : (nn >= 3 && nn <= 64), FVM_E_INVALID,
+                  "faces need 2 nodes in 2-D and 3..64 nodes in 3-D");
+        for (int t = face_nptr[f]; t < face_nptr[f + 1]; t++)
+            FVM_CHECK(face_nodes[t] >= 0 && face_nodes[t] < n_nodes, FVM_E_INVALID, "bad face node");
     }
     fvm::Dev d(device);
     fvm_mesh *m = new fvm_mesh();
     try {
         m->device = device;
         FVM_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamDefault));
+        m->dim = dim;
         m->n_cells = n_cells; m->n_ghost = n_ghost; m->n_faces = n_faces; m->n_nodes = n_nodes;
         m->n_groups = n_groups;
         m->gptr.assign(group_ptr, group_ptr + n_groups + 1);
-        m->nodes.alloc(3 * (size_t)n_nodes);
-        m->fnodes.alloc(4 * (size_t)n_faces);
-        m->fcells.alloc(2 * (size_t)n_faces);
-        FVM_HIP(hipMemcpy(m->nodes.p, nodes, m->nodes.bytes(), hipMemcpyHostToDevice));
-        FVM_HIP(hipMemcpy(m->fnodes.p, face_nodes, m->fnodes.bytes(), hipMemcpyHostToDevice));
-        FVM_HIP(hipMemcpy(m->fcells.p, face_cells, m->fcells.bytes(), hipMemcpyHostToDevice));
+        m->n_face_nodes = face_nptr[n_faces];
+        m->nodes.alloc(3 * (size_t)std::max(n_nodes, 1));
+        m->fnptr.alloc((size_t)n_faces + 1);
+        m->fnodes.alloc((size_t)std::max<long long>(m->n_face_nodes, 1));
+        m->fcells.alloc(2 * (size_t)std::max(n_faces, 1));
+        FVM_HIP(hipMemcpy(m->nodes.p, nodes, sizeof(double) * 3 * (size_t)n_nodes, hipMemcpyHostToDevice));
+        FVM_HIP(hipMemcpy(m->fnptr.p, face_nptr, sizeof(int) * ((size_t)n_faces + 1), hipMemcpyHostToDevice));
+        FVM_HIP(hipMemcpy(m->fnodes.p, face_nodes, sizeof(int) * (size_t)m->n_face_nodes, hipMemcpyHostToDevice));
+        FVM_HIP(hipMemcpy(m->fcells.p, face_cells, sizeof(int) * 2 * (size_t)n_faces, hipMemcpyHostToDevice));
     } catch (...) {
         if (m->stream) (void)hipStreamDestroy(m->stream);
         delete m;
@@ -762,13 +1056,23 @@ int fvm_mesh_create(fvm_mesh **out, int device, int n_nodes, const double *nodes
     FVM_API_END
 }
 
+int fvm_mesh_create(fvm_mesh **out, int device, int n_nodes, const double *nodes, int n_faces,
+                    const int32_t *face_nodes, const int32_t *face_cells, int n_cells, int n_ghost,
+                    int n_groups, const int32_t *group_ptr) {
+    if (n_faces < 0) return FVM_E_INVALID;
+    std::vector<int32_t> nptr((size_t)n_faces + 1);
+    for (int f = 0; f <= n_faces; f++) nptr[f] = 4 * f;
+    return fvm_mesh_create_general(out, device, 3, n_nodes, nodes, n_faces, nptr.data(), face_nodes, face_cells,
+                                   n_cells, n_ghost, n_groups, group_ptr);
+}
+
 int fvm_mesh_destroy(fvm_mesh *m) {
     FVM_API_BEGIN
     if (!m) return FVM_OK;
     {
         fvm::Dev d(m->device);
         (void)hipStreamSynchronize(m->stream);
-        m->nodes.release(); m->fnodes.release(); m->fcells.release();
+        m->nodes.release(); m->fnptr.release(); m->fnodes.release(); m->fcells.release(); m->gcoef.release();
         m->cf_row.release(); m->cf_col.release(); m->cc_row.release(); m->cc_col.release();
         m->pos01.release(); m->pos10.release();
         m->fA.release(); m->fAm.release(); m->fc.release(); m->cc.release(); m->vol.release();
@@ -789,13 +1093,40 @@ int fvm_mesh_info(const fvm_mesh *m, int *n_cells, int *n_ghost, int *n_faces, i
     return FVM_OK;
 }
 
+int fvm_mesh_topology(const fvm_mesh *m, int *dim, long long *n_face_nodes, int32_t *face_nptr) {
+    FVM_API_BEGIN
+    FVM_CHECK(m, FVM_E_INVALID, "null mesh");
+    if (dim) *dim = m->dim;
+    if (n_face_nodes) *n_face_nodes = m->n_face_nodes;
+    if (face_nptr) {
+        fvm::Dev d(m->device);
+        FVM_HIP(hipMemcpy(face_nptr, m->fnptr.p, sizeof(int) * ((size_t)m->n_faces + 1), hipMemcpyDeviceToHost));
+    }
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_mesh_gradient(fvm_mesh *m, const double *d_x, double *d_grad) {
+    FVM_API_BEGIN
+    FVM_CHECK(m && d_x && d_grad, FVM_E_INVALID, "null argument");
+    fvm::Dev d(m->device);
+    fvm::mesh_gradient_coeffs(m);
+    fvm::mesh_gradient(m, d_x, d_grad, m->stream);
+    FVM_HIP(hipGetLastError());
+    FVM_HIP(hipStreamSynchronize(m->stream));
+    return FVM_OK;
+    FVM_API_END
+}
+
 int fvm_mesh_get(const fvm_mesh *m, double *nodes, int32_t *face_nodes, int32_t *face_cells, int32_t *group_ptr) {
     FVM_API_BEGIN
     FVM_CHECK(m, FVM_E_INVALID, "null mesh");
     fvm::Dev d(m->device);
     if (nodes) FVM_HIP(hipMemcpy(nodes, m->nodes.p, m->nodes.bytes(), hipMemcpyDeviceToHost));
-    if (face_nodes) FVM_HIP(hipMemcpy(face_nodes, m->fnodes.p, m->fnodes.bytes(), hipMemcpyDeviceToHost));
-    if (face_cells) FVM_HIP(hipMemcpy(face_cells, m->fcells.p, m->fcells.bytes(), hipMemcpyDeviceToHost));
+    if (face_nodes && m->n_face_nodes)
+        FVM_HIP(hipMemcpy(face_nodes, m->fnodes.p, sizeof(int) * (size_t)m->n_face_nodes, hipMemcpyDeviceToHost));
+    if (face_cells && m->n_faces)
+        FVM_HIP(hipMemcpy(face_cells, m->fcells.p, sizeof(int) * 2 * (size_t)m->n_faces, hipMemcpyDeviceToHost));
     if (group_ptr) for (int g = 0; g <= m->n_groups; g++) group_ptr[g] = m->gptr[g];
     return FVM_OK;
     FVM_API_END
@@ -845,45 +1176,22 @@ int fvm_mesh_uniform_field(fvm_mesh *m, double lo, double hi, uint64_t seed, dou
     FVM_API_END
 }
 
-int fvm_thermal_assemble(fvm_mesh *m, const double *d_k, const double *d_source, const double *d_x,
-                         const int32_t *bc_type, const double *bc_value, fvm_system **out) {
+int fvm_convdiff_assemble(fvm_mesh *m, const double *d_k, const double *d_source, const double *d_x,
+                          const double *d_flux, const double *d_cont_res, int central, const int32_t *bc_type,
+                          const double *bc_value, fvm_system **out) {
     FVM_API_BEGIN
     FVM_CHECK(m && d_k && d_source && d_x && bc_type && bc_value && out, FVM_E_INVALID, "null argument");
+    FVM_CHECK(central == 0 || central == 1, FVM_E_INVALID, "central must be 0 or 1");
     fvm::Dev d(m->device);
-    fvm::mesh_metrics(m);
+    fvm::mesh_gradient_coeffs(m);
     const fvm::Groups G = fvm::make_groups(m, bc_type, bc_value);
-    hipStream_t s = m->stream;
-    const int nc = m->n_cells, nct = m->n_cells + m->n_ghost, nf = m->n_faces;
+    const int nct = m->n_cells + m->n_ghost, nf = m->n_faces;
     fvm_system *S = new fvm_system();
     try {
-        S->device = m->device;
-        S->n_self = nc;
-        S->n_total = nct;
-        S->nnz = m->nnz;
-        S->row.alloc((size_t)nct + 1);
-        S->col.alloc(std::max<long long>(m->nnz, 1));
-        S->off.alloc(std::max<long long>(m->nnz, 1));
-        S->diag.alloc(nct); S->b.alloc(nct); S->isb.alloc(nct);
-        FVM_HIP(hipMemcpyAsync(S->row.p, m->cc_row.p, S->row.bytes(), hipMemcpyDeviceToDevice, s));
-        FVM_HIP(hipMemcpyAsync(S->col.p, m->cc_col.p, sizeof(int) * m->nnz, hipMemcpyDeviceToDevice, s));
-        DArr<double> dcoef(std::max(nf, 1)), dflux(std::max(nf, 1));
-        fvm::k_face_diffusion<<<fvm::grid_for(nf, 256), 256, 0, s>>>(nf, m->fcells.p, m->fA.p, m->fAm.p, m->cc.p,
-                                                                     m->vol.p, d_k, d_x, dcoef.p, dflux.p);
-        fvm::k_cell_assemble<<<fvm::grid_for(nct, 256), 256, 0, s>>>(
-            nc, nct, m->cf_row.p, m->cf_col.p, m->fcells.p, m->pos01.p, m->pos10.p, m->cc_row.p, dcoef.p,
-            dflux.p, m->vol.p, d_source, S->diag.p, S->off.p, S->b.p);
-        fvm::k_bc_interior<<<fvm::grid_for(nc, 256), 256, 0, s>>>(nc, m->cf_row.p, m->cf_col.p, m->fcells.p,
-                                                                  m->pos01.p, G, d_x, S->off.p, S->b.p);
-        FVM_HIP(hipMemsetAsync(S->isb.p, 0, nct, s));
-        if (m->n_ghost)
-            fvm::k_bc_ghost<<<fvm::grid_for(m->n_ghost, 256), 256, 0, s>>>(nc, nct, m->cf_row.p, m->cf_col.p,
-                                                                           m->pos10.p, m->fAm.p, G, S->diag.p,
-                                                                           S->off.p, S->b.p, S->isb.p);
-        fvm::k_eliminate<<<fvm::grid_for(nc, 256), 256, 0, s>>>(nc, m->cf_row.p, m->cf_col.p, m->fcells.p,
-                                                                m->pos01.p, m->pos10.p, S->isb.p, S->diag.p,
-                                                                S->off.p, S->b.p);
-        FVM_HIP(hipGetLastError());
-        FVM_HIP(hipStreamSynchronize(s));
+        fvm::system_alloc(m, S);
+        DArr<double> dcoef(std::max(nf, 1)), dflux(std::max(nf, 1)), grad(3 * (size_t)std::max(nct, 1));
+        fvm::assemble_thermal(m, S, d_k, d_source, d_x, d_flux, d_cont_res, central, G, dcoef.p, dflux.p, grad.p);
+        FVM_HIP(hipStreamSynchronize(m->stream));
     } catch (...) {
         delete S;
         throw;
@@ -891,6 +1199,11 @@ int fvm_thermal_assemble(fvm_mesh *m, const double *d_k, const double *d_source,
     *out = S;
     return FVM_OK;
     FVM_API_END
+}
+
+int fvm_thermal_assemble(fvm_mesh *m, const double *d_k, const double *d_source, const double *d_x,
+                         const int32_t *bc_type, const double *bc_value, fvm_system **out) {
+    return fvm_convdiff_assemble(m, d_k, d_source, d_x, nullptr, nullptr, 0, bc_type, bc_value, out);
 }
 
 int fvm_phonon_assemble(fvm_mesh *m, const double *vg, double tau, const double *d_e0, const double *d_x,

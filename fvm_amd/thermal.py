@@ -34,6 +34,9 @@ class Mesh:
                                   ctypes.byref(nn), ctypes.byref(ngr)))
         self.n_cells, self.n_ghost, self.n_faces, self.n_nodes, self.n_groups = (
             nc.value, ng.value, nf.value, nn.value, ngr.value)
+        dim, nfn = ctypes.c_int(), ctypes.c_longlong()
+        check(lib().fvm_mesh_topology(self._h, ctypes.byref(dim), ctypes.byref(nfn), None))
+        self.dim, self.n_face_nodes = dim.value, nfn.value
 
     @classmethod
     def hex(cls, n, jitter=0.0, seed=0, device=0):
@@ -42,15 +45,22 @@ class Mesh:
         return cls(h, device)
 
     @classmethod
-    def from_arrays(cls, nodes, face_nodes, face_cells, n_cells, n_ghost, group_ptr, device=0):
+    def from_arrays(cls, nodes, face_nodes, face_cells, n_cells, n_ghost, group_ptr, device=0, face_nptr=None,
+                    dim=3):
+        """Mesh from host arrays: nodes (3 per node), face_nodes (a CSR over face_nptr;
+        four per face when face_nptr is None), face_cells (2 per face), group_ptr."""
         nodes = np.ascontiguousarray(nodes, np.float64).reshape(-1)
         face_nodes = np.ascontiguousarray(face_nodes, np.int32).reshape(-1)
         face_cells = np.ascontiguousarray(face_cells, np.int32).reshape(-1)
         group_ptr = np.ascontiguousarray(group_ptr, np.int32)
+        nf = face_cells.size // 2
+        if face_nptr is None:
+            face_nptr = np.arange(nf + 1, dtype=np.int32) * 4
+        face_nptr = np.ascontiguousarray(face_nptr, np.int32)
         h = ctypes.c_void_p()
-        check(lib().fvm_mesh_create(ctypes.byref(h), int(device), nodes.size // 3, _vp(nodes),
-                                    face_cells.size // 2, _vp(face_nodes), _vp(face_cells), int(n_cells),
-                                    int(n_ghost), group_ptr.size - 1, _vp(group_ptr)))
+        check(lib().fvm_mesh_create_general(ctypes.byref(h), int(device), int(dim), nodes.size // 3, _vp(nodes),
+                                            nf, _vp(face_nptr), _vp(face_nodes), _vp(face_cells), int(n_cells),
+                                            int(n_ghost), group_ptr.size - 1, _vp(group_ptr)))
         return cls(h, device)
 
     def __del__(self):
@@ -67,11 +77,23 @@ class Mesh:
 
     def arrays(self):
         nodes = np.zeros(3 * self.n_nodes)
-        fn = np.zeros(4 * self.n_faces, np.int32)
+        fn = np.zeros(self.n_face_nodes, np.int32)
         fc = np.zeros(2 * self.n_faces, np.int32)
         gp = np.zeros(self.n_groups + 1, np.int32)
+        fp = np.zeros(self.n_faces + 1, np.int32)
         check(lib().fvm_mesh_get(self._h, _vp(nodes), _vp(fn), _vp(fc), _vp(gp)))
-        return dict(nodes=nodes, face_nodes=fn, face_cells=fc, group_ptr=gp)
+        check(lib().fvm_mesh_topology(self._h, None, None, _vp(fp)))
+        return dict(nodes=nodes, face_nodes=fn, face_cells=fc, group_ptr=gp, face_nptr=fp, dim=self.dim,
+                    n_cells=self.n_cells, n_ghost=self.n_ghost)
+
+    def gradient(self, x, out=None):
+        """Least-squares gradient (GradientModel::compute) of a device cell field:
+        torch float64 tensor of n_total entries -> (n_total, 3) tensor."""
+        import torch
+        if out is None:
+            out = torch.empty((self.n_total, 3), dtype=torch.float64, device=x.device)
+        check(lib().fvm_mesh_gradient(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr())))
+        return out
 
     def metrics(self):
         nf, nct = self.n_faces, self.n_total
@@ -138,16 +160,24 @@ class System:
                             is_boundary=h["is_boundary"])
 
 
-def assemble(mesh, k, source, x, bc_types, bc_values):
+def assemble(mesh, k, source, x, bc_types, bc_values, flux=None, continuity_residual=None, central=False):
     """ThermalModel linearization on the device: k, source, x are torch float64
-    tensors with mesh.n_total entries; bc_types/bc_values per face group (index 0 unused)."""
+    tensors with mesh.n_total entries; bc_types/bc_values per face group (index 0
+    unused). flux (n_faces, optional): the convecting face flux of the
+    ConvectionDiscretization term; continuity_residual (n_cells, optional) is
+    added to the diagonal; central selects the reference's central-difference branch."""
     t = np.ascontiguousarray(bc_types, np.int32)
     v = np.ascontiguousarray(bc_values, np.float64)
     if t.size != mesh.n_groups or v.size != mesh.n_groups:
         raise ValueError("need one boundary condition per face group")
+    dp = lambda a: None if a is None else ctypes.c_void_p(a.data_ptr())
+    if flux is not None and flux.numel() != mesh.n_faces:
+        raise ValueError("flux needs one entry per face")
+    if continuity_residual is not None and continuity_residual.numel() < mesh.n_cells:
+        raise ValueError("continuity_residual needs one entry per cell")
     h = ctypes.c_void_p()
-    check(lib().fvm_thermal_assemble(mesh._h, ctypes.c_void_p(k.data_ptr()), ctypes.c_void_p(source.data_ptr()),
-                                     ctypes.c_void_p(x.data_ptr()), _vp(t), _vp(v), ctypes.byref(h)))
+    check(lib().fvm_convdiff_assemble(mesh._h, dp(k), dp(source), dp(x), dp(flux), dp(continuity_residual),
+                                      int(bool(central)), _vp(t), _vp(v), ctypes.byref(h)))
     return System(h)
 
 

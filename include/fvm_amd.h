@@ -354,8 +354,10 @@ int fvm_fgmres_learned_stats(const fvm_fgmres *s, int *iterations_used, int *fal
  * elimination (LinearSystem.cpp:40-64, CRMatrix.h:899-1085). The assembled
  * system is the CRMatrix the AMG consumes: interior cells first, one ghost row
  * per boundary face after them, columns in CRConnectivity::multiply order.
- * Only uniform initial fields are supported (their least-squares gradient is
- * zero, so the non-orthogonal secondary flux vanishes).
+ * The non-orthogonal secondary flux uses the least-squares gradient of the
+ * current field (GradientModel.h:128-520), as ThermalModel::linearize does.
+ * Boundary groups are walls (no symmetry or immersed-boundary faces); the
+ * gradient needs a conformal mesh (two cells share at most one face).
  * ------------------------------------------------------------------------- */
 enum { FVM_BC_DIRICHLET = 0, FVM_BC_NEUMANN = 1 };
 
@@ -374,10 +376,24 @@ int fvm_mesh_create_hex(fvm_mesh **out, int device, int n, double jitter, uint64
 int fvm_mesh_create(fvm_mesh **out, int device, int n_nodes, const double *nodes, int n_faces,
                     const int32_t *face_nodes, const int32_t *face_cells, int n_cells, int n_ghost,
                     int n_groups, const int32_t *group_ptr);
+/* General unstructured mesh (the meshes the reference's importers build,
+ * FluentReader.cpp:696-950): dim 2 (faces of 2 nodes, z = 0) or 3 (faces of 3..64
+ * nodes), faces as a node CSR face_nptr[n_faces + 1] / face_nodes, the same
+ * face_cells, ghost and group conventions as fvm_mesh_create. */
+int fvm_mesh_create_general(fvm_mesh **out, int device, int dim, int n_nodes, const double *nodes, int n_faces,
+                            const int32_t *face_nptr, const int32_t *face_nodes, const int32_t *face_cells,
+                            int n_cells, int n_ghost, int n_groups, const int32_t *group_ptr);
 int fvm_mesh_destroy(fvm_mesh *m);
+/* dimension, total face-node count and (may be NULL) a host copy of face_nptr */
+int fvm_mesh_topology(const fvm_mesh *m, int *dim, long long *n_face_nodes, int32_t *face_nptr);
+/* GradientModel::compute (GradientModel.h:472-520): least-squares gradient of the
+ * device cell field d_x (n_cells + n_ghost) into d_grad (3 per cell, ghosts take
+ * their face's cell's gradient) */
+int fvm_mesh_gradient(fvm_mesh *m, const double *d_x, double *d_grad);
 int fvm_mesh_info(const fvm_mesh *m, int *n_cells, int *n_ghost, int *n_faces, int *n_nodes,
                   int *n_groups);
-/* host copies of the mesh arrays (any may be NULL) */
+/* host copies of the mesh arrays (any may be NULL); face_nodes holds the
+ * n_face_nodes entries of fvm_mesh_topology */
 int fvm_mesh_get(const fvm_mesh *m, double *nodes, int32_t *face_nodes, int32_t *face_cells,
                  int32_t *group_ptr);
 /* MeshMetricsCalculator: face area vectors, |A|, face centroids, cell centroids
@@ -393,6 +409,17 @@ int fvm_mesh_uniform_field(fvm_mesh *m, double lo, double hi, uint64_t seed, dou
  * (bc_type/bc_value indexed by group id, entry 0 unused) into a new system. */
 int fvm_thermal_assemble(fvm_mesh *m, const double *d_k, const double *d_source, const double *d_x,
                          const int32_t *bc_type, const double *bc_value, fvm_system **out);
+/* The same with the convection term of ThermalModel::linearize
+ * (ConvectionDiscretization.h:53-200): d_flux, the convecting face flux (device,
+ * n_faces, positive from c0 to c1), upwind, or with central != 0 the reference's
+ * central-difference branch (face value 0.5 (x[c0] + x[c0]), upwind
+ * linearization); d_cont_res (device, n_cells, may be NULL = 0) is added to the
+ * diagonal; specified-temperature faces with an outflow flux extrapolate
+ * (ThermalModel_impl.h:306-318). d_flux NULL: no convection term (as
+ * fvm_thermal_assemble). */
+int fvm_convdiff_assemble(fvm_mesh *m, const double *d_k, const double *d_source, const double *d_x,
+                          const double *d_flux, const double *d_cont_res, int central, const int32_t *bc_type,
+                          const double *bc_value, fvm_system **out);
 /* One discrete-ordinate direction of a gray phonon BTE (scalar system per
  * direction, PhononModel::linearizePhononModel, PhononModel.h:437-523):
  * relaxation -V (x - e0)/tau (PhononCollisionDiscretization.h:54-68), upwind

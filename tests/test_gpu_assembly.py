@@ -134,3 +134,57 @@ def test_phonon_system_bit_identical(N, jit, seed, i, tau):
     o = O.fv_phonon(ref, vg, tau, e0, x)
     for key in ("row", "col", "diag", "off", "b", "is_boundary"):
         assert np.array_equal(S[key], o[key]), key
+
+
+@pytest.mark.parametrize("N,jit,seed", [(5, 0.2, 31), (8, 0.15, 32)])
+def test_gradient_bit_identical(N, jit, seed):
+    import torch
+    from fvm_amd.thermal import Mesh
+    m = Mesh.hex(N, jit, seed)
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0, 1, m.n_total)
+    g = m.gradient(torch.tensor(x, dtype=torch.float64, device="cuda:0")).cpu().numpy()
+    o = O.fv_gradient(hexmesh.hex_mesh(N, jit, seed), x)
+    assert np.array_equal(g, o)
+
+
+@pytest.mark.parametrize("N,jit,seed", [(5, 0.2, 41), (9, 0.2, 42)])
+def test_thermal_system_with_nonuniform_field_bit_identical(N, jit, seed):
+    """Secondary (non-orthogonal) flux from the least-squares gradient of a
+    non-uniform current field, as in every outer iteration after the first."""
+    import torch
+    from fvm_amd.thermal import Mesh, assemble
+    m = Mesh.hex(N, jit, seed)
+    k, s, _ = _dev_fields(m, 1.5, 1, seed)
+    rng = np.random.default_rng(seed)
+    xh = rng.normal(0, 1, m.n_total)
+    x = torch.tensor(xh, dtype=torch.float64, device="cuda:0")
+    S = assemble(m, k, s, x, BC_T, BC_V).to_host()
+    o = O.fv_thermal(hexmesh.hex_mesh(N, jit, seed), k.cpu().numpy(), s.cpu().numpy(), xh, BC_T, BC_V)
+    for key in ("row", "col", "diag", "off", "b", "is_boundary"):
+        assert np.array_equal(S[key], o[key]), key
+
+
+@pytest.mark.parametrize("central", [False, True])
+def test_convection_diffusion_system_bit_identical(central):
+    """ConvectionDiscretization with a caller-given face flux, a continuity
+    residual and outflow specified-temperature faces (extrapolation)."""
+    import torch
+    from fvm_amd.thermal import Mesh, assemble
+    N, jit, seed = 7, 0.2, 51
+    m = Mesh.hex(N, jit, seed)
+    ref = hexmesh.hex_mesh(N, jit, seed)
+    k, s, _ = _dev_fields(m, 1.0, 1, seed)
+    rng = np.random.default_rng(seed)
+    xh = rng.normal(0, 1, m.n_total)
+    fA = O.fv_metrics(ref)["fA"].reshape(-1, 3)
+    Fh = fA @ np.array([1.3, -0.4, 0.6]) + rng.normal(0, 0.01, m.n_faces)
+    crh = rng.normal(0, 0.05, m.n_cells)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda:0")
+    S = assemble(m, k, s, t(xh), BC_T, BC_V, flux=t(Fh), continuity_residual=t(crh), central=central).to_host()
+    o = O.fv_thermal(ref, k.cpu().numpy(), s.cpu().numpy(), xh, BC_T, BC_V, conv_flux=Fh, cont_res=crh,
+                     central=central)
+    for key in ("row", "col", "diag", "off", "b", "is_boundary"):
+        assert np.array_equal(S[key], o[key]), key
+    # both x walls have outflow faces somewhere: some of their ghosts extrapolate
+    assert S["is_boundary"].sum() > 4 * N * N

@@ -31,7 +31,7 @@ def test_advance_matches_oracle(N, jit, seed):
     k = np.exp(rng.normal(0, 1.0, nct))
     s = rng.uniform(0, 1, nct)
     assemble, solve = _oracle_parts(mesh, k, s)
-    xo, norms_o = MR.thermal_advance(assemble, solve, np.zeros(nct), niter=4, rel_tol=1e-8, abs_tol=1e-20,
+    xo, norms_o = MR.thermal_advance(assemble, solve, np.zeros(nct), niter=6, rel_tol=1e-8, abs_tol=1e-20,
                                      dirichlet=dirichlet_ghosts(mesh))
     m = Mesh.hex(N, jit, seed)
     amg = fvm_amd.AMG()
@@ -41,7 +41,7 @@ def test_advance_matches_oracle(N, jit, seed):
     dev = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda:0")
     x = torch.zeros(nct, dtype=torch.float64, device="cuda:0")
     model = ThermalModel(m, dev(k), dev(s), x, BC_T, BC_V, amg, relative_tolerance=1e-8, absolute_tolerance=1e-20)
-    norms = model.advance(4)
+    norms = model.advance(6)
     print("N", N, "norms", norms, norms_o, flush=True)
     assert len(norms) == len(norms_o) == model.niters
     np.testing.assert_allclose(norms, norms_o, rtol=1e-12)
@@ -50,9 +50,10 @@ def test_advance_matches_oracle(N, jit, seed):
 
 def test_advance_with_bcgstab_around_the_amg():
     """The same loop with the reference's BiCGStab around the AMG: not the same
-    arithmetic as the oracle's AMG-only loop, so the bar is the physics: the loop
-    stops at its second linearization, and the temperatures agree with the oracle's
-    to the solver tolerance."""
+    arithmetic as the oracle's AMG-only loop, so the bar is the physics: the outer
+    loop (deferred non-orthogonal correction) converges in the same number of
+    linearizations, and the temperatures agree with the oracle's to the solver
+    tolerance."""
     import torch
     import fvm_amd
     from fvm_amd.thermal import Mesh, ThermalModel
@@ -63,8 +64,8 @@ def test_advance_with_bcgstab_around_the_amg():
     k = np.exp(rng.normal(0, 1.0, nct))
     s = rng.uniform(0, 1, nct)
     assemble, solve = _oracle_parts(mesh, k, s)
-    xo, _ = MR.thermal_advance(assemble, solve, np.zeros(nct), niter=4, rel_tol=1e-8, abs_tol=1e-20,
-                               dirichlet=dirichlet_ghosts(mesh))
+    xo, norms_o = MR.thermal_advance(assemble, solve, np.zeros(nct), niter=40, rel_tol=1e-8, abs_tol=1e-20,
+                                     dirichlet=dirichlet_ghosts(mesh))
     amg = fvm_amd.AMG()
     amg.verbosity = 0
     solver = fvm_amd.BCGStab(amg)
@@ -72,7 +73,7 @@ def test_advance_with_bcgstab_around_the_amg():
     dev = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda:0")
     x = torch.zeros(nct, dtype=torch.float64, device="cuda:0")
     model = ThermalModel(Mesh.hex(N, jit, seed), dev(k), dev(s), x, BC_T, BC_V, solver)
-    norms = model.advance(4)
-    assert len(norms) == 2 and norms[1] < 1e-8 * norms[0]
+    norms = model.advance(40)
+    assert norms[-1] < 1e-8 * norms[0] and abs(len(norms) - len(norms_o)) <= 1
     xg = x.cpu().numpy()
     assert np.abs(xg - xo).max() <= 1e-9 * np.abs(xo).max()
