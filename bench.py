@@ -79,25 +79,96 @@ def kernel_stats(amg, kind):
     return n.value, ms.value
 
 
-def cpu_baseline(S, cycles_needed):
+def host_cpu():
+    """CPU model and logical core count of this host (/proc/cpuinfo, os.cpu_count)."""
+    model = None
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def _median(v):
+    v = sorted(v)
+    return v[len(v) // 2] if len(v) % 2 else 0.5 * (v[len(v) // 2 - 1] + v[len(v) // 2])
+
+
+def oracle_cycle_model(O, h, reps=1):
+    """Setup time, then the solve time of 1 and of 6 cycles on that hierarchy:
+    seconds per cycle = (T6 - T1) / 5, fixed cost = T1 - one cycle (initial
+    residual and norm). Medians over `reps`."""
+    a = O.OracleAMG(n_max_iterations=2, relative_tolerance=1e-30)
+    t0 = time.perf_counter()
+    a.setup(h["n_self"], h["n_total"], h["row"], h["col"], h["diag"], h["off"], h["is_boundary"])
+    setup = time.perf_counter() - t0
+    t1s, t6s = [], []
+    for _ in range(reps):
+        for k, acc in ((2, t1s), (7, t6s)):
+            a._p.n_max_iterations = k
+            t0 = time.perf_counter()
+            a.solve(h["b"])
+            acc.append(time.perf_counter() - t0)
+    per = (_median(t6s) - _median(t1s)) / 5
+    return setup, per, _median(t1s) - per
+
+
+def cpu_validation(torch, runs=5):
+    """The oracle (single thread) to convergence on the 32^3 uniform conductivity-1
+    system (T = 1 / 0 on the x walls, zero flux elsewhere, relTol 1e-8): median of
+    `runs` timed runs after one warm-up, setup and solve apart; and the
+    extrapolation model used for the 8M-cell system (oracle_cycle_model) applied
+    to the same system, compared with the measured time."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from fvm_amd.thermal import Mesh, assemble
+    mesh = Mesh.hex(32, 0.0, 0)
+    nt = mesh.n_total
+    k = torch.ones(nt, dtype=torch.float64, device="cuda:0")
+    z = torch.zeros(nt, dtype=torch.float64, device="cuda:0")
+    h = assemble(mesh, k, z, z, [0, 0, 0, 1, 1, 1, 1], [0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0]).to_host()
+    setups, solves, cycles = [], [], None
+    for r in range(runs + 1):
+        a = O.OracleAMG(n_max_iterations=10000, relative_tolerance=1e-8)
+        t0 = time.perf_counter()
+        a.setup(h["n_self"], h["n_total"], h["row"], h["col"], h["diag"], h["off"], h["is_boundary"])
+        t1 = time.perf_counter()
+        res = a.solve(h["b"])
+        t2 = time.perf_counter()
+        cycles = res["cycles"]
+        if r:
+            setups.append(t1 - t0)
+            solves.append(t2 - t1)
+    _, per, fixed = oracle_cycle_model(O, h, reps=runs)
+    measured = _median(setups) + _median(solves)
+    model = _median(setups) + fixed + cycles * per
+    return dict(workload="32^3 uniform, k=1", runs=runs, cycles=cycles, setup_s=round(_median(setups), 5),
+                solve_s=round(_median(solves), 5), measured_s=round(measured, 5), model_s=round(model, 5),
+                model_error=round(model / measured - 1.0, 4))
+
+
+def cpu_baseline(S, cycles_needed, torch=None):
     """Oracle (C restatement of the reference AMG, single thread) on the same
-    system: time the hierarchy build and 3 cycles, extrapolate to the number of
-    cycles the reference-order AMG needs for 1e-8."""
+    system: the hierarchy build, then 1 and 6 cycles, extrapolated to the number of
+    cycles the reference-order AMG needs for 1e-8 (setup + fixed + cycles x per
+    cycle); the model is checked against full timed runs on the 32^3 system
+    (cpu_validation)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     h = S.to_host()
-    a = O.OracleAMG(n_max_iterations=4, relative_tolerance=1e-30)
-    t0 = time.perf_counter()
-    a.setup(h["n_self"], h["n_total"], h["row"], h["col"], h["diag"], h["off"], h["is_boundary"])
-    t1 = time.perf_counter()
-    res = a.solve(h["b"])
-    t2 = time.perf_counter()
-    per_cycle = (t2 - t1) / max(res["cycles"], 1)
-    return dict(value=(t1 - t0) + cycles_needed * per_cycle, unit="s", cores=1, kind="port",
-                sample="oracle setup (%.2f s) + %d V-cycles (%.3f s/cycle) on the same 8M-cell system, "
-                       "extrapolated to the %d cycles of the reference-order AMG"
-                       % (t1 - t0, res["cycles"], per_cycle, cycles_needed),
-                setup_s=t1 - t0, per_cycle_s=per_cycle, host_threads=os.cpu_count())
+    setup, per_cycle, fixed = oracle_cycle_model(O, h)
+    model, ncpu = host_cpu()
+    out = dict(value=setup + fixed + cycles_needed * per_cycle, unit="s", cores=1, kind="port",
+               sample="oracle setup (%.2f s), then 1 and 6 V-cycles (%.3f s/cycle) on the same 8M-cell system, "
+                      "extrapolated to the %d cycles of the reference-order AMG"
+                      % (setup, per_cycle, cycles_needed),
+               setup_s=setup, per_cycle_s=per_cycle, fixed_s=fixed, host_threads=ncpu, cpu_model=model)
+    if torch is not None:
+        out["validation"] = cpu_validation(torch)
+    return out
 
 
 def fib_direction(i, K):
@@ -611,7 +682,8 @@ def main():
     cpu = None
     if not args.skip_cpu:
         log("cpu baseline (oracle, 1 thread)")
-        cpu = cpu_baseline(S, cyc_exact if cyc_exact else (cyc_plain or 0))
+        cpu = cpu_baseline(S, cyc_exact if cyc_exact else (cyc_plain or 0), torch)
+        log("cpu extrapolation check on 32^3: %s" % json.dumps(cpu["validation"]))
         log("cpu: %.1f s estimated time-to-1e-8" % cpu["value"])
 
     out = {

@@ -66,6 +66,7 @@ SIGNATURES = {
     "fvm_amg_level_get": (_i, [_p, _i, _p, _p, _p, _p, _p]),
     "fvm_amg_setup_stats": (_i, [_p, _p, _p]),
     "fvm_amg_direct_level": (_i, [_p, _p, _p, _p, _p]),
+    "fvm_amg_set_output": (_i, [_p, ctypes.c_char_p, ctypes.c_char_p, _i]),
     "fvm_amg_direct_apply": (_i, [_p, _p, _p]),
     "fvm_amg_set_kernel_timing": (_i, [_p, _i]),
     "fvm_amg_set_tuning": (_i, [_p, ctypes.c_char_p, _ll]),
@@ -123,6 +124,13 @@ SIGNATURES = {
     "fvm_system_post_solve": (_i, [_p, _p]),
     "fvm_system_destroy": (_i, [_p]),
     "fvm_amg_setup_system": (_i, [_p, _p]),
+    "fvm_phonon_create": (_i, [ctypes.POINTER(_p), _p, _i, _p, _p, _p, _p, _p, _p, _d]),
+    "fvm_phonon_destroy": (_i, [_p]),
+    "fvm_phonon_set_params": (_i, [_p, ctypes.POINTER(AmgParams)]),
+    "fvm_phonon_apply_walls": (_i, [_p]),
+    "fvm_phonon_advance": (_i, [_p, _i, _d, _d, _p, _p, _i, _p]),
+    "fvm_phonon_get": (_i, [_p, _p, _p, _p]),
+    "fvm_phonon_set_field": (_i, [_p, _p]),
 }
 
 _lib = None

@@ -49,7 +49,9 @@ class AMG:
         check(L.fvm_amg_default_params(ctypes.byref(p)))
         object.__setattr__(self, "_p", p)
         self._finest = None
-        self._print_file = None
+        self._print_path = None
+        self.fieldName = "x"          # the bracketed name of the convergence lines
+        self.printStyle = 0           # 0: every cycle (serial build), 1: first and last (parallel build)
         self.device = int(device)
 
     # parameter attributes map onto the C struct
@@ -63,6 +65,8 @@ class AMG:
             setattr(self._p, name, value)
         else:
             object.__setattr__(self, name, value)
+            if name in ("fieldName", "printStyle") and "printStyle" in self.__dict__:
+                self._push_output()
 
     def __del__(self):
         try:
@@ -106,7 +110,6 @@ class AMG:
                                   ctypes.byref(cyc)))
         self.lastCycles = cyc.value
         self.lastResidual = rf.value
-        self._emit_history()
         return r0.value
 
     def solve_device(self, b, x):
@@ -141,14 +144,19 @@ class AMG:
         single device; the value is recorded and has no effect."""
         self.mergeLevelSize = int(size)
 
+    def _push_output(self):
+        path = None if self._print_path is None else self._print_path.encode()
+        check(lib().fvm_amg_set_output(self._h, str(self.fieldName).encode(), path, int(self.printStyle)))
+
     def redirectPrintToFile(self, fname):
         """AMG.cpp:323-330: convergence lines go to ``fname`` instead of stdout."""
-        self._print_file = open(fname, "w")
+        self._print_path = str(fname)
+        self._push_output()
 
     def redirectPrintToScreen(self):
-        if self._print_file:
-            self._print_file.close()
-        self._print_file = None
+        """AMG.cpp:333-337"""
+        self._print_path = None
+        self._push_output()
 
     # ------------------------------------------------------------------ introspection
     def history(self, cap=1 << 20):
@@ -214,19 +222,6 @@ class AMG:
         check(lib().fvm_amg_setup_stats(self._h, ctypes.byref(t), ctypes.byref(r)))
         return dict(setup_seconds=t.value, finest_rounds=r.value)
 
-    def _emit_history(self, name="test"):
-        """Print 'i: [name : norm]' lines like AMG.cpp:243-272 when verbosity > 0."""
-        if self.verbosity <= 0 or self._print_file is None:
-            return
-        h = self.history()
-        for i, v in enumerate(h):
-            self._print_file.write("%d: [%s : %s]\n" % (i, name, _fmt6(v)))
-        self._print_file.flush()
-
-
-def _fmt6(v):
-    """C++ ostream default formatting (precision 6, %g-like)."""
-    return "%g" % v
 
 
 class BCGStab:

@@ -142,6 +142,9 @@ struct fvm_amg {
     long long tail_max_rows = 40000;
     long long coarsen_small_rows = 8192;
     int coarsen_lanes = 0;             // lanes per row of the agglomeration rounds (0: by row width)
+    int coarsen_small_lanes = 0;       // the same for the single-workgroup rounds (0: as coarsen_lanes)
+    int coarsen_grid = 0;              // blocks per launched agglomeration round (0: 2048 / 1024)
+    int bipartite_levels = 0;          // levels tried with a breadth-first two-colouring first
     int sell_block = 256;              // threads per block of the SELL-64 smoother kernels
     int spmv_block = 256;              // threads per block of the SELL-64 residual / SpMV kernels
     int sell_xcd = 1, spmv_xcd = 0;    // XCD-contiguous block order for those kernel classes
@@ -163,12 +166,23 @@ struct fvm_amg {
     long long k_launches[4] = {0, 0, 0, 0};
     double k_ms[4] = {0, 0, 0, 0};
     // double-buffer used by pointer-swapped Jacobi
+    // convergence lines at verbosity > 0 (AMG.cpp:237-272): "i: [field : norm]"
+    std::string field_name = "x";
+    FILE *out = nullptr;               // redirectPrintToFile target (nullptr: stdout)
+    std::string out_path;
+    int print_style = 0;               // 0: every cycle (serial build), 1: first and last (parallel build)
 };
 
 namespace fvm {
 inline void invalidate_cycle_graphs(fvm_amg *h) {
     for (auto &g : h->cg_exec)
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
+// one convergence line "label: [field : value]" (MultiFieldReduction.cpp:199-211)
+inline void print_norm(const fvm_amg *h, const std::string &label, double v) {
+    FILE *f = h->out ? h->out : stdout;
+    fprintf(f, "%s: [%s : %g]\n", label.c_str(), h->field_name.c_str(), v);
+    fflush(f);
 }
 // kernel timing kinds
 enum { KT_RESID64 = 0, KT_SWEEP = 1 };
@@ -200,6 +214,10 @@ void apply_precond(fvm_amg *h, const double *d_v, double *d_z);   // z = -M^{-1}
 void spmv_residual64(fvm_amg *h, const double *d_x, const double *d_b, double *d_r, double *norm1);
 double norm1_device(fvm_amg *h, const double *d_r, int n);
 std::string autotune(fvm_amg *h, int reps);
+// assembly.hip: device of a mesh; its face cells, area vectors and |A| on the
+// device (metrics computed if needed)
+int mesh_device(const fvm_mesh *m);
+void mesh_device_metrics(fvm_mesh *m, const int **fcells, const double **fA, const double **fAm);
 // learned correction (mpnn.hip)
 void mpnn_bind(fvm_mpnn *m, int n, const int *row, const int *col, const double *off, const double *diag);
 void mpnn_correct(fvm_mpnn *m, hipStream_t st, const int *perm, const double *Az, const double *v, double *z,

@@ -948,6 +948,16 @@ void assemble_thermal(fvm_mesh *m, fvm_system *S, const double *d_k, const doubl
     FVM_HIP(hipGetLastError());
 }
 }  // namespace
+
+int mesh_device(const fvm_mesh *m) { return m->device; }
+
+void mesh_device_metrics(fvm_mesh *m, const int **fcells, const double **fA, const double **fAm) {
+    Dev d(m->device);
+    mesh_metrics(m);
+    *fcells = m->fcells.p;
+    *fA = m->fA.p;
+    *fAm = m->fAm.p;
+}
 }  // namespace fvm
 
 #define FVM_API_BEGIN try {

@@ -162,6 +162,7 @@ int fvm_amg_destroy(fvm_amg *h) {
         HandleScope g(h);
         (void)hipStreamSynchronize(h->stream);
         fvm::invalidate_cycle_graphs(h);
+        if (h->out) { fclose(h->out); h->out = nullptr; }
         h->probe.release();
         h->lev.clear();
         h->ws = fvm::Workspace();
@@ -376,6 +377,26 @@ int fvm_amg_level_get(const fvm_amg *h, int level, int32_t *row, int32_t *col, d
     FVM_API_END
 }
 
+int fvm_amg_set_output(fvm_amg *h, const char *field_name, const char *path, int style) {
+    FVM_API_BEGIN
+    FVM_CHECK(h, FVM_E_INVALID, "null handle");
+    FVM_CHECK(style == 0 || style == 1, FVM_E_INVALID, "style must be 0 (every cycle) or 1 (first and last)");
+    if (!path || h->out_path != path) {      // the file stays open (and keeps its lines) while the path is the same
+        FILE *f = nullptr;
+        if (path) {
+            f = fopen(path, "w");
+            FVM_CHECK(f, FVM_E_INVALID, std::string("cannot open ") + path);
+        }
+        if (h->out) fclose(h->out);
+        h->out = f;
+        h->out_path = path ? path : "";
+    }
+    h->field_name = field_name ? field_name : "x";
+    h->print_style = style;
+    return FVM_OK;
+    FVM_API_END
+}
+
 int fvm_amg_direct_level(const fvm_amg *h, int *level, int *n_rows, int *attempts, double *inv) {
     FVM_API_BEGIN
     FVM_CHECK(h, FVM_E_INVALID, "null handle");
@@ -450,6 +471,16 @@ int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value) {
         FVM_CHECK(value == 0 || value == 8 || value == 16 || value == 32 || value == 64, FVM_E_INVALID,
                   "coarsen_lanes must be 0, 8, 16, 32 or 64");
         h->coarsen_lanes = (int)value;
+    } else if (k == "bipartite_levels") {
+        FVM_CHECK(value >= 0 && value <= 64, FVM_E_INVALID, "bipartite_levels must be in [0, 64]");
+        h->bipartite_levels = (int)value;
+    } else if (k == "coarsen_grid") {
+        FVM_CHECK(value >= 0 && value <= 65536, FVM_E_INVALID, "coarsen_grid must be in [0, 65536]");
+        h->coarsen_grid = (int)value;
+    } else if (k == "coarsen_small_lanes") {
+        FVM_CHECK(value == 0 || value == 8 || value == 16 || value == 32 || value == 64, FVM_E_INVALID,
+                  "coarsen_small_lanes must be 0, 8, 16, 32 or 64");
+        h->coarsen_small_lanes = (int)value;
     } else if (k == "nt_levels") {
         FVM_CHECK(value >= 0 && value <= 64, FVM_E_INVALID, "nt_levels must be in [0, 64]");
         h->nt_levels = (int)value;
@@ -474,6 +505,9 @@ int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value) {
     else if (k == "cycle_graph") *value = h->cycle_graph;
     else if (k == "nt_levels") *value = h->nt_levels;
     else if (k == "coarsen_lanes") *value = h->coarsen_lanes;
+    else if (k == "coarsen_small_lanes") *value = h->coarsen_small_lanes;
+    else if (k == "coarsen_grid") *value = h->coarsen_grid;
+    else if (k == "bipartite_levels") *value = h->bipartite_levels;
     else return FVM_E_INVALID;
     return FVM_OK;
 }

@@ -924,7 +924,7 @@ void solve_device(fvm_amg *h, const double *d_b, double *d_x, double *rnorm0, do
     load_finest(R, d_b, d_x);
     const double r0 = finest_residual_norm(R);
     h->hist.assign(1, r0);
-    if (h->prm.verbosity > 0) printf("0: %g\n", r0);
+    if (h->prm.verbosity > 0) print_norm(h, "0", r0);
     double rn = r0;
     int it = 0;
     if (!(r0 < h->prm.absoluteTolerance)) {
@@ -939,8 +939,11 @@ void solve_device(fvm_amg *h, const double *d_b, double *d_x, double *rnorm0, do
             if (r0 != 0) ratio /= r0;
             it = i;
             h->hist.push_back(rn);
-            if (h->prm.verbosity > 0) printf("%d: %g\n", i, rn);
-            if (rn < h->prm.absoluteTolerance || ratio < h->prm.relativeTolerance) break;
+            const bool stop = rn < h->prm.absoluteTolerance || ratio < h->prm.relativeTolerance;
+            // the parallel build prints only the line where the loop ends (AMG.cpp:262-268)
+            if (h->prm.verbosity > 0 && (h->print_style == 0 || stop || i == h->prm.nMaxIterations - 1))
+                print_norm(h, std::to_string(i), rn);
+            if (stop) break;
         }
     }
     store_finest(R, d_x);
@@ -1910,7 +1913,7 @@ void bcgstab_solve(fvm_amg *h, const fvm_linsolver_params &p, const double *d_b,
     const double r0 = norm1_of(R, r.p, n);
     FVM_CHECK(std::isfinite(r0), FVM_E_NUMERIC, "BCGStab: non-finite initial residual");
     hist.assign(1, r0);
-    if (p.verbosity > 0) printf("0: %g\n", r0);
+    if (p.verbosity > 0) print_norm(h, "0", r0);
     FVM_HIP(hipMemcpyAsync(rt.p, r.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     double rho = 0, alpha = 0, omega = 0;
     bool have_p = false;
@@ -1944,12 +1947,12 @@ void bcgstab_solve(fvm_amg *h, const fvm_linsolver_params &p, const double *d_b,
         FVM_CHECK(std::isfinite(rn), FVM_E_NUMERIC,
                   "BCGStab produced a non-finite residual (iteration " + std::to_string(i + 1) + ")");
         hist.push_back(rn);
-        if (p.verbosity > 0) printf("%d: %g\n", i + 1, rn);
+        if (p.verbosity > 0) print_norm(h, std::to_string(i + 1), rn);
         if (rn < p.absoluteTolerance || rn / r0 < p.relativeTolerance) break;
     }
     residual<double>(R64, 0, x.p, b.p, r.p);
     const double rf = norm1_of(R, r.p, n);
-    if (p.verbosity > 0) printf("n: %g\n", rf);
+    if (p.verbosity > 0) print_norm(h, "n", rf);
     if (n > 0) k_scatter<<<gb, BS, 0, s>>>(n, perm, x.p, d_x);
     FVM_HIP(hipGetLastError());
     FVM_HIP(hipStreamSynchronize(s));
@@ -1996,7 +1999,7 @@ void cg_solve(fvm_amg *h, const fvm_linsolver_params &p, const double *d_b, doub
     const double r0 = norm1_of(R, r.p, n);
     FVM_CHECK(std::isfinite(r0), FVM_E_NUMERIC, "CG: non-finite initial residual");
     hist.assign(1, r0);
-    if (p.verbosity > 0) printf("0: %g\n", r0);
+    if (p.verbosity > 0) print_norm(h, "0", r0);
     double rho = 0;
     bool have_p = false;
     int it = 0;
@@ -2018,12 +2021,12 @@ void cg_solve(fvm_amg *h, const fvm_linsolver_params &p, const double *d_b, doub
         FVM_CHECK(std::isfinite(rn), FVM_E_NUMERIC,
                   "CG produced a non-finite residual (iteration " + std::to_string(i + 1) + ")");
         hist.push_back(rn);
-        if (p.verbosity > 0) printf("%d: %g\n", i + 1, rn);
+        if (p.verbosity > 0) print_norm(h, std::to_string(i + 1), rn);
         if (rn < p.absoluteTolerance || rn / r0 < p.relativeTolerance) break;
     }
     residual<double>(R64, 0, x.p, b.p, r.p);
     const double rf = norm1_of(R, r.p, n);
-    if (p.verbosity > 0) printf("n: %g\n", rf);
+    if (p.verbosity > 0) print_norm(h, "n", rf);
     if (n > 0) k_scatter<<<gb, BS, 0, s>>>(n, perm, x.p, d_x);
     FVM_HIP(hipGetLastError());
     FVM_HIP(hipStreamSynchronize(s));

@@ -518,6 +518,8 @@ k_coarsen_round(int t, int n, const int *row, const int *col, const int *rowT, c
 // global stores and loads (all its wavefronts share one CU), so no device-scope
 // fence is needed, and every atomic is a workgroup-scope one: the frontier
 // sizes are LDS counters, the dependency counters stay in global memory.
+// S lanes per frontier row (64 / S rows per wavefront).
+template <int S>
 __global__ void __launch_bounds__(1024)
 k_coarsen_small(int n, int max_rounds, const int *row, const int *col, const int *rowT, const int *colT,
                 const double *off, const uint8_t *bnd, RowRec *rec, const int *dep, double theta, int *mem,
@@ -536,14 +538,14 @@ k_coarsen_small(int n, int max_rounds, const int *row, const int *col, const int
         // every row reachable has decided; nf > n cannot happen (each row is queued
         // at most once per round) and only guards the reads of the list
         if (nf == 0 || nf > n) break;
-        const int per_wave = 64 / SMALL_SUB;
+        const int per_wave = 64 / S;
         const int wave = threadIdx.x >> 6;
         const int nwaves = blockDim.x >> 6;
         for (int base = wave * per_wave; base < nf; base += nwaves * per_wave) {
-            const int i = base + (__lane_id() / SMALL_SUB);
+            const int i = base + (__lane_id() / S);
             if (i < nf)
-                decide_row<SMALL_SUB, true>(cur[i], t, theta, row, col, rowT, colT, off, bnd, rec, dep, mem,
-                                            memr, mcnt, cnt, nxt, &s_q[(t + 1) & 1], nretry);
+                decide_row<S, true>(cur[i], t, theta, row, col, rowT, colT, off, bnd, rec, dep, mem, memr, mcnt,
+                                    cnt, nxt, &s_q[(t + 1) & 1], nretry);
         }
         __syncthreads();
         if (threadIdx.x == 0) s_q[t & 1] = 0;   // every thread has read it: the counter of round t + 2
@@ -914,6 +916,52 @@ __global__ void k_color_round(int t, int n, const int *row, const int *col, cons
     if (done) atomicSub(left, done);
 }
 
+// Two-colouring by breadth-first parity (a graph without odd cycles, e.g. the
+// cell graph of a hexahedral mesh: red-black order). Seeds: the first row of
+// every segment (system of a batch), colour 0, so each system gets the colouring
+// it gets alone. A row takes the colour opposite to the frontier row that
+// reaches it first; the check below rejects the colouring if any edge joins
+// equal colours (odd cycle, or a row no seed reached).
+__global__ void k_bfs_seed(int n, const int *seg_off, int nseg, int *color, int *q, int *qcnt) {
+    const int ns = seg_off ? nseg : 1;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x) {
+        const int r = seg_off ? seg_off[s] : 0;
+        const int e = seg_off ? seg_off[s + 1] : n;
+        if (r < e) {
+            color[r] = 0;
+            q[atomicAdd(qcnt, 1)] = r;
+        }
+    }
+}
+
+__global__ void k_bfs_round(const int *row, const int *col, const int *rowT, const int *colT, int *color,
+                            const int *cur, const int *curcnt, int *nxt, int *nxtcnt, int *zero_cnt) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *zero_cnt = 0;
+    const int nf = *curcnt;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nf; t += gridDim.x * blockDim.x) {
+        const int i = cur[t];
+        const int want = 1 - color[i];
+        for (int pass = 0; pass < 2; pass++) {
+            const int *rp = pass ? rowT : row;
+            const int *cp = pass ? colT : col;
+            if (!rp) break;
+            for (int p = rp[i]; p < rp[i + 1]; p++) {
+                const int j = cp[p];
+                if (atomicCAS(&color[j], -1, want) == -1) nxt[atomicAdd(nxtcnt, 1)] = j;
+            }
+        }
+    }
+}
+
+__global__ void k_bipartite_check(int n, const int *row, const int *col, const int *color, int *bad) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int ci = color[i];
+        bool ok = ci >= 0;
+        for (int p = row[i]; p < row[i + 1] && ok; p++) ok = color[col[p]] != ci;
+        if (!ok) { atomicOr(bad, 1); return; }
+    }
+}
+
 __global__ void k_fill_int(int n, int *v, int val) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] = val;
 }
@@ -1226,11 +1274,26 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
     int t = 1;
     const long long cap = 4LL * n + 64;
     const char *path;
+    // lanes per row (measured on the 200^3 hierarchy): 8 for rows of at most 7
+    // entries (eight rows per wavefront), 16 up to 10 entries, 32 for large levels
+    // of moderate rows (their frontiers are thousands of rows wide), else one
+    // wavefront per row; the single-workgroup rounds take the same widths
+    int sub = L.nnz <= 7LL * n ? 8
+            : L.nnz <= 10LL * n ? THIN_SUB
+            : (L.nnz <= 16LL * n && n >= (1 << 20)) ? 32 : ROUND_SUB;
+    if (h->coarsen_lanes) sub = h->coarsen_lanes;   // tuning override (8, 16, 32 or 64)
     if (n <= h->coarsen_small_rows) {
         path = "single-workgroup";
-        k_coarsen_small<<<1, 1024, 0, s>>>(n, (int)std::min<long long>(cap, INT_MAX - 1), L.row.p, L.col.p,
-                                           rowT, colT, L.off.p, bnd, rec.p, dep.p, theta, mem.p, memr.p, mcnt.p,
-                                           cnt.p, q1, q0, qc + 1, rounds_out, ctr.p + 6);
+        auto launch_small = [&](auto kern) {
+            kern<<<1, 1024, 0, s>>>(n, (int)std::min<long long>(cap, INT_MAX - 1), L.row.p, L.col.p, rowT, colT,
+                                    L.off.p, bnd, rec.p, dep.p, theta, mem.p, memr.p, mcnt.p, cnt.p, q1, q0, qc + 1,
+                                    rounds_out, ctr.p + 6);
+        };
+        const int ssub = h->coarsen_small_lanes ? h->coarsen_small_lanes : sub;
+        if (ssub == 8) launch_small(k_coarsen_small<8>);
+        else if (ssub == THIN_SUB) launch_small(k_coarsen_small<THIN_SUB>);
+        else if (ssub == 32) launch_small(k_coarsen_small<32>);
+        else launch_small(k_coarsen_small<SMALL_SUB>);
         FVM_HIP(hipGetLastError());
         t = read_scalar(rounds_out, s);
     } else {
@@ -1238,15 +1301,9 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
         // check of batch b runs while batch b + 1 is queued (a round on an empty
         // frontier only zeroes a counter)
         path = "multi-block";
-        // lanes per row (measured on the 200^3 hierarchy): 8 for rows of at most 7
-        // entries (eight rows per wavefront), 16 up to 10 entries, 32 for large levels
-        // of moderate rows (their frontiers are thousands of rows wide), else one
-        // wavefront per row
-        int sub = L.nnz <= 7LL * n ? 8
-                : L.nnz <= 10LL * n ? THIN_SUB
-                : (L.nnz <= 16LL * n && n >= (1 << 20)) ? 32 : ROUND_SUB;
-        if (h->coarsen_lanes) sub = h->coarsen_lanes;   // tuning override (16, 32 or 64)
-        const unsigned g_round = sub < ROUND_SUB ? 2048 : 1024;
+        // blocks per round launch: grid-stride loops over the frontier, so any
+        // size is correct; "coarsen_grid" overrides the default
+        const unsigned g_round = h->coarsen_grid ? (unsigned)h->coarsen_grid : (sub < ROUND_SUB ? 2048u : 1024u);
         const int batch = 32;
         h->probe.init();
         int t_prev = t;
@@ -1630,19 +1687,44 @@ void prepare_fast_level(fvm_amg *h, Level &L, int l, hipStream_t s, Workspace &w
     const int *rT = L.sym ? nullptr : L.rowT.p;
     const int *cT = L.sym ? nullptr : L.colT.p;
     DArr<int> color(n), cround(n), left(1);
-    DArr<unsigned long long> prio(n);
-    k_color_prio<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.row.p, seg_off, nseg, prio.p);
-    FVM_HIP(hipMemsetAsync(color.p, 0, color.bytes(), s));
-    k_fill_int<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, cround.p, INT_MAX);
-    k_fill_int<<<1, 1, 0, s>>>(1, left.p, n);
-    int t = 1;
-    for (;;) {
-        for (int k = 0; k < 8; k++, t++)
-            k_color_round<<<grid_for(n, BS, 4096), BS, 0, s>>>(t, n, L.row.p, L.col.p, rT, cT, prio.p,
-                                                               color.p, cround.p, left.p);
-        FVM_HIP(hipGetLastError());
-        if (read_scalar(left.p, s) == 0) break;
-        FVM_CHECK(t < 4 * n + 64, FVM_E_NUMERIC, "colouring did not terminate");
+    bool two = false;
+    if (l < h->bipartite_levels) {
+        // red-black order when the level's graph has no odd cycle
+        DArr<int> qa(n), qb(n), cnt(4);
+        k_fill_int<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, color.p, -1);
+        FVM_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes(), s));
+        k_bfs_seed<<<grid_for(std::max(nseg, 1), BS), BS, 0, s>>>(n, seg_off, nseg, color.p, qa.p, cnt.p);
+        int t = 0;
+        for (;;) {
+            for (int k = 0; k < 16; k++, t++) {
+                const int *cur = (t & 1) ? qb.p : qa.p;
+                int *nxt = (t & 1) ? qa.p : qb.p;
+                k_bfs_round<<<1024, BS, 0, s>>>(L.row.p, L.col.p, rT, cT, color.p, cur, cnt.p + (t % 3), nxt,
+                                                cnt.p + ((t + 1) % 3), cnt.p + ((t + 2) % 3));
+            }
+            FVM_HIP(hipGetLastError());
+            if (read_scalar(cnt.p + (t % 3), s) == 0) break;
+            FVM_CHECK(t < 2 * n + 64, FVM_E_NUMERIC, "breadth-first colouring did not terminate");
+        }
+        FVM_HIP(hipMemsetAsync(cnt.p + 3, 0, sizeof(int), s));
+        k_bipartite_check<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.row.p, L.col.p, color.p, cnt.p + 3);
+        two = read_scalar(cnt.p + 3, s) == 0;
+    }
+    if (!two) {
+        DArr<unsigned long long> prio(n);
+        k_color_prio<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.row.p, seg_off, nseg, prio.p);
+        FVM_HIP(hipMemsetAsync(color.p, 0, color.bytes(), s));
+        k_fill_int<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, cround.p, INT_MAX);
+        k_fill_int<<<1, 1, 0, s>>>(1, left.p, n);
+        int t = 1;
+        for (;;) {
+            for (int k = 0; k < 8; k++, t++)
+                k_color_round<<<grid_for(n, BS, 4096), BS, 0, s>>>(t, n, L.row.p, L.col.p, rT, cT, prio.p,
+                                                                   color.p, cround.p, left.p);
+            FVM_HIP(hipGetLastError());
+            if (read_scalar(left.p, s) == 0) break;
+            FVM_CHECK(t < 4 * n + 64, FVM_E_NUMERIC, "colouring did not terminate");
+        }
     }
     // stable sort of rows by colour (coarse levels: then by decreasing length) -> perm
     DArr<int> idx(n);

@@ -142,6 +142,15 @@ int fvm_amg_smooth_device(fvm_amg *h, const double *d_b, double *d_x);
 int fvm_amg_cleanup(fvm_amg *h);
 /* AMG::getTotalIterations (AMG.h:69) */
 long long fvm_amg_total_iterations(const fvm_amg *h);
+/* Convergence lines printed at verbosity > 0, as the reference prints them
+ * (AMG.cpp:237-272, MultiFieldReduction.cpp:199-211, BCGStab.cpp:55-164):
+ * "i: [field_name : norm]". path: AMG::redirectPrintToFile (a new path is
+ * truncated, the same path keeps appending; NULL = AMG::redirectPrintToScreen,
+ * stdout); style 0 prints every
+ * cycle (serial build), 1 only the first line and the one where the loop ends
+ * (parallel build, as in the reference's convergence goldens). field_name NULL:
+ * "x". */
+int fvm_amg_set_output(fvm_amg *h, const char *field_name, const char *path, int style);
 /* Residual 1-norms of the last solve: hist[0] = r0, hist[i] after cycle i.
  * Returns the number of entries written (<= cap). */
 int fvm_amg_history(const fvm_amg *h, double *hist, int cap);
@@ -443,6 +452,42 @@ int fvm_system_get(const fvm_system *S, int32_t *row, int32_t *col, double *diag
  * (device vector of n_total entries). */
 int fvm_system_post_solve(const fvm_system *S, double *d_delta);
 int fvm_system_destroy(fvm_system *S);
+
+/* ---------------------------------------------------------------------------
+ * Gray phonon BTE model (PhononModel, phononbase/PhononModel.h) on one mesh:
+ * K discrete directions (modes) with group velocity vg[3k..3k+2], weight dk3[k],
+ * relaxation time tau[k] and the gray equilibrium e0_k(T) = cp[k] (T - tref[k])
+ * (pmode.h:146-165); every boundary face group is a temperature wall at
+ * wall_T[g] (PhononBoundary.h:165-210; entry 0 unused). init (PhononModel.h:
+ * 115-160): TL = T_init and e_k = e0_k(T_init) on every cell, ghosts included.
+ * The mesh is borrowed: keep it alive while the model is used.
+ * ------------------------------------------------------------------------- */
+typedef struct fvm_phonon fvm_phonon;
+int fvm_phonon_create(fvm_phonon **out, fvm_mesh *mesh, int K, const double *vg, const double *dk3,
+                      const double *tau, const double *cp, const double *tref, const double *wall_T,
+                      double T_init);
+int fvm_phonon_destroy(fvm_phonon *p);
+/* parameters of the per-direction AMG (the phonon linear solver); precision
+ * FVM_PREC_FP64, no scaled corrections */
+int fvm_phonon_set_params(fvm_phonon *p, const fvm_amg_params *prm);
+/* PhononModel::callBoundaryConditions: the temperature walls set the ghosts */
+int fvm_phonon_apply_walls(fvm_phonon *p);
+/* PhononModel::advance (PhononModel.h:530-600): up to niter outer iterations;
+ * every iteration assembles all directions (fvm_phonon_assemble), solves them
+ * together with one batched AMG (fvm_batch_*: each direction's result is its own
+ * AMG::solve's), sums the directions' initial 1-norms in direction order,
+ * postSolve + update, then -- unless the sum is below abs_tol or its ratio to
+ * the initial norm (the largest of the first five iterations') below rel_tol --
+ * updateTL (Newton, Kspace.h:425-441), updatee0 and the walls. norms (may be
+ * NULL when cap is 0): the iterations' norms; *converged: 1 if the stop test
+ * ended the call. */
+int fvm_phonon_advance(fvm_phonon *p, int niter, double rel_tol, double abs_tol, int *iterations,
+                       double *norms, int cap, int *converged);
+/* host copies (any may be NULL): TL (n_cells + n_ghost), e and e0 (K blocks of
+ * n_cells + n_ghost, direction-major) */
+int fvm_phonon_get(const fvm_phonon *p, double *TL, double *e, double *e0);
+/* replace every direction's field (host, K blocks of n_cells + n_ghost) */
+int fvm_phonon_set_field(fvm_phonon *p, const double *e);
 /* fvm_amg_setup_device on an assembled system */
 int fvm_amg_setup_system(fvm_amg *h, const fvm_system *S);
 

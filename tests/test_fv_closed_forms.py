@@ -267,3 +267,35 @@ def test_assembly_is_independent_of_interior_face_orientation():
         assert np.array_equal(out[key], ref[key]), key
     for key in ("diag", "off", "b"):
         np.testing.assert_allclose(out[key], ref[key], rtol=1e-12, atol=1e-13 * np.abs(ref[key]).max())
+
+
+@pytest.mark.parametrize("jit", [0.0, 0.15])
+def test_neumann_heat_flux_linear_solution(jit):
+    """GenericBCS.h:123-150: a specified heat flux q enters the domain through its
+    wall (the ghost row's residual is q |A| minus the flux the interior sends
+    into the domain). T = 0 on x = 0, flux q on x = 1, zero flux on y and z: the
+    exact solution T = (q / k) x is linear, which the scheme reproduces on a
+    uniform mesh; on a jittered one the heat through every x-section still
+    adds up to q."""
+    N, kc, q = 5, 1.7, 2.3
+    m = hexmesh.hex_mesh(N, jitter=jit, seed=3)
+    nc = m["n_cells"]; nct = nc + m["n_ghost"]
+    bt = [0, D, NM, NM, NM, NM, NM]
+    bv = [0.0, 0.0, q, 0.0, 0.0, 0.0, 0.0]
+    S = O.fv_thermal(m, np.full(nct, kc), np.zeros(nct), np.zeros(nct), bt, bv)
+    T = np.linalg.solve(_dense_interior(S), -S["b"][:nc])
+    cc = O.fv_metrics(m)["cc"].reshape(-1, 3)[:nc]
+    if jit == 0.0:
+        np.testing.assert_allclose(T, q / kc * cc[:, 0], rtol=1e-12, atol=1e-13)
+    else:
+        # total heat leaving through the Dirichlet wall equals the heat entering: q * 1
+        g = O.fv_metrics(m)
+        fcl = m["face_cells"].reshape(-1, 2)
+        gp = m["group_ptr"]
+        out = 0.0
+        for f in range(gp[1], gp[2]):
+            c0 = fcl[f, 0]
+            ds = g["cc"].reshape(-1, 3)[fcl[f, 1]] - g["cc"].reshape(-1, 3)[c0]
+            A = g["fA"].reshape(-1, 3)[f]
+            out += kc * (A.dot(A) / A.dot(ds)) * (T[c0] - 0.0)
+        np.testing.assert_allclose(out, q * 1.0, rtol=1e-10)

@@ -188,3 +188,21 @@ def test_convection_diffusion_system_bit_identical(central):
         assert np.array_equal(S[key], o[key]), key
     # both x walls have outflow faces somewhere: some of their ghosts extrapolate
     assert S["is_boundary"].sum() > 4 * N * N
+
+
+def test_neumann_heat_flux_system_bit_identical():
+    """Specified heat fluxes q != 0 on the y and z walls (GenericBCS.h:123-150)."""
+    import torch
+    from fvm_amd.thermal import Mesh, assemble
+    N, jit, seed = 6, 0.2, 61
+    m = Mesh.hex(N, jit, seed)
+    k, s, _ = _dev_fields(m, 1.0, 1, seed)
+    rng = np.random.default_rng(seed)
+    xh = rng.normal(0, 1, m.n_total)
+    bt = [0, D, NM, NM, NM, NM, NM]
+    bv = [0.0, 1.0, 0.7, -1.2, 2.5, 0.3, -0.4]
+    x = torch.tensor(xh, dtype=torch.float64, device="cuda:0")
+    S = assemble(m, k, s, x, bt, bv).to_host()
+    o = O.fv_thermal(hexmesh.hex_mesh(N, jit, seed), k.cpu().numpy(), s.cpu().numpy(), xh, bt, bv)
+    for key in ("row", "col", "diag", "off", "b", "is_boundary"):
+        assert np.array_equal(S[key], o[key]), key

@@ -287,9 +287,11 @@ def test_device_entry_points_match_host():
     assert np.array_equal(x.cpu().numpy(), ls.delta)
 
 
-@pytest.mark.parametrize("small,kat", [(0, 0), (1 << 30, 0), (0, 1), (1 << 30, 1)])
-def test_coarsening_paths_agree(small, kat):
-    # one launch per round (one wavefront per row) and single-workgroup rounds
+@pytest.mark.parametrize("small,kat,lanes", [(0, 0, 0), (1 << 30, 0, 0), (0, 1, 0), (1 << 30, 1, 0),
+                                              (1 << 30, 0, 8), (1 << 30, 0, 16), (1 << 30, 0, 32),
+                                              (1 << 30, 1, 8), (0, 0, 8), (0, 1, 16)])
+def test_coarsening_paths_agree(small, kat, lanes):
+    # launched rounds and single-workgroup rounds, with every lane width per row,
     # give the oracle's maps (symmetric grid and the non-symmetric KAT-226)
     if kat:
         m = O.load_mm(os.path.join(G, "MatrixMarket226.dat"), os.path.join(G, "rhs226.dat"))
@@ -298,6 +300,7 @@ def test_coarsening_paths_agree(small, kat):
     oa = oracle_for(m)
     ga = gpu_amg()
     ga.set_tuning("coarsen_small_rows", small)
+    ga.set_tuning("coarsen_lanes", lanes)
     ls = ls_from(m)
     ga.setup(ls)
     check_hierarchy(ga, oa, m)
@@ -357,3 +360,36 @@ def test_autotune_keeps_results(prec):
         ls2 = ls_from(m)
         gb.solve(ls2)
         assert np.array_equal(ls0.delta, ls2.delta), bs
+
+
+def test_convergence_lines_match_the_reference_golden(tmp_path):
+    """AMG.cpp:237-272 with MultiFieldReduction::print: the lines of the cavity
+    solve equal AMG_MERGING_THERMAL/proc1/GOLDEN/convergence.dat (parallel-build
+    style: the first line and the one where the loop ends); the serial style
+    prints every cycle with the same first and last lines."""
+    m = O.load_mm(os.path.join(G, "cav32_matrix_mesh0.mat"), os.path.join(G, "cav32_matrix.rhs"))
+    golden = open(os.path.join(G, "cav32_convergence_proc1.dat")).read().splitlines()[:2]
+    out = {}
+    for style in (1, 0):
+        ga = gpu_amg(relativeTolerance=1e-9, maxCoarseLevels=20, nMaxIterations=2000, verbosity=2)
+        ga.fieldName = "therm.temperature"
+        ga.printStyle = style
+        path = str(tmp_path / ("conv%d.dat" % style))
+        ga.redirectPrintToFile(path)
+        ga.solve(ls_from(m))
+        ga.redirectPrintToScreen()
+        out[style] = open(path).read().splitlines()
+    assert out[1] == golden
+    assert len(out[0]) == 57 and out[0][0] == golden[0] and out[0][-1] == golden[1]
+
+
+def test_serial_stop_rule_exact():
+    """AMG.cpp:199-204 (serial build): no coarse level of at most 3 rows is kept;
+    maps, cycles and delta equal the oracle's with the same rule."""
+    m = O.load_mm(os.path.join(G, "MatrixMarket226.dat"), os.path.join(G, "rhs226.dat"))
+    ga, oa, ls, r0, ores = run_pair(m, dict(parallelStopRule=0), dict(parallel_stop_rule=0))
+    check_hierarchy(ga, oa, m)
+    sizes = [ga.level(l)["n"] for l in range(1, ga.num_levels())]
+    assert sizes == [108, 48, 20, 8]
+    assert ga.lastCycles == ores["cycles"]
+    assert np.array_equal(ls.delta, ores["x"])

@@ -86,3 +86,91 @@ def test_solve_boundary_is_the_ghost_row_equation():
             lo, hi = sys["row"][nr], sys["row"][nr + 1]
             eq = sys["b"][nr] + sys["diag"][nr] * x[nr] + np.dot(sys["off"][lo:hi], x[sys["col"][lo:hi]])
             assert abs(eq) <= 1e-12 * (abs(sys["b"][nr]) + abs(sys["diag"][nr] * x[nr]) + 1.0)
+
+
+# ----------------------------------------------------------------- gray phonon model
+def _octants():
+    vg = np.array([[sx, sy, sz] for sx in (1, -1) for sy in (1, -1) for sz in (1, -1)], float) / np.sqrt(3.0)
+    return vg
+
+
+def _phonon_parts(mesh):
+    def assemble(vg, tau, e0, e):
+        return O.fv_phonon(mesh, vg, tau, e0, e)
+
+    def solve(sys):
+        oa = O.OracleAMG(n_max_iterations=400, relative_tolerance=1e-12)
+        oa.setup(sys["n_self"], sys["n_total"], sys["row"], sys["col"], sys["diag"], sys["off"], sys["is_boundary"])
+        res = oa.solve(sys["b"])
+        return res["r0"], res["x"]
+    return assemble, solve
+
+
+def test_phonon_newton_update_closed_form():
+    """Gray modes make sum_k e0_k(T) dk3_k / tau_k linear in T: Newton lands on
+    T = (e_sum + sum_k tref_k cp_k dk3_k / tau_k) / sum_k cp_k dk3_k / tau_k."""
+    rng = np.random.default_rng(3)
+    K, n = 7, 50
+    e = rng.uniform(0, 2, (K, n))
+    dk3 = rng.uniform(0.1, 1, K); tau = rng.uniform(0.1, 10, K)
+    cp = rng.uniform(0.5, 2, K); tref = rng.uniform(-1, 1, K)
+    TL = np.full(n, 300.0)
+    MR.phonon_update_TL(e, dk3, tau, cp, tref, TL)
+    a = (cp * dk3 / tau).sum()
+    want = ((e * (dk3 / tau)[:, None]).sum(0) + (tref * cp * dk3 / tau).sum()) / a
+    np.testing.assert_allclose(TL, want, rtol=1e-12)
+
+
+def test_phonon_equilibrium_is_a_fixed_point():
+    mesh = hexmesh.hex_mesh(3, 0.15, 4)
+    nct = mesh["n_cells"] + mesh["n_ghost"]
+    vg = _octants(); K = len(vg)
+    dk3 = np.full(K, 4 * np.pi / K); tau = np.linspace(0.5, 2, K); cp = np.ones(K); tref = np.zeros(K)
+    wall = [0.0] + [300.0] * 6
+    st = MR.phonon_init(nct, K, cp, tref, 300.0)
+    g = O.fv_metrics(mesh)
+    assemble, solve = _phonon_parts(mesh)
+    norms, conv = MR.phonon_advance(assemble, solve, mesh, g, vg, dk3, tau, cp, tref, wall, st, 3, 1e-30, 0.0)
+    assert max(norms) < 1e-10 * 300 * mesh["n_cells"]
+    np.testing.assert_allclose(st["TL"], 300.0, rtol=1e-13)
+    np.testing.assert_allclose(st["e"], 300.0, rtol=1e-13)
+
+
+def test_phonon_iteration_norm_is_the_sum_of_direction_residuals():
+    """PhononModel.h:557-572: the iteration's norm is the sum over the directions
+    of the initial 1-norm each direction's AMG::solve returns, i.e. of ||b_k||_1."""
+    mesh = hexmesh.hex_mesh(3, 0.1, 5)
+    nc = mesh["n_cells"]; nct = nc + mesh["n_ghost"]
+    vg = _octants(); K = len(vg)
+    dk3 = np.full(K, 4 * np.pi / K); tau = np.full(K, 1.0); cp = np.ones(K); tref = np.zeros(K)
+    wall = [0.0, 310.0, 290.0, 300.0, 300.0, 300.0, 300.0]
+    st = MR.phonon_init(nct, K, cp, tref, 300.0)
+    g = O.fv_metrics(mesh)
+    MR.phonon_walls(mesh, g["fA"], g["fAm"], vg, cp, tref, wall, st["e"])
+    want = sum(np.abs(O.fv_phonon(mesh, vg[k], tau[k], st["e0"][k][:nc], st["e"][k])["b"][:nc]).sum()
+               for k in range(K))
+    assemble, solve = _phonon_parts(mesh)
+    norms, _ = MR.phonon_advance(assemble, solve, mesh, g, vg, dk3, tau, cp, tref, wall, st, 1, 1e-30, 0.0)
+    np.testing.assert_allclose(norms[0], want, rtol=1e-13)
+
+
+def test_phonon_slab_maximum_principle_and_mirror_symmetry():
+    """Walls x0 = 310, x1 = 290, the others 300, a mirror-symmetric direction set on
+    a uniform mesh: the converged lattice temperature lies between the wall
+    temperatures and T - 300 is odd under x -> 1 - x."""
+    N = 4
+    mesh = hexmesh.hex_mesh(N)
+    nc = mesh["n_cells"]; nct = nc + mesh["n_ghost"]
+    vg = _octants(); K = len(vg)
+    dk3 = np.full(K, 4 * np.pi / K); tau = np.full(K, 0.05); cp = np.ones(K); tref = np.zeros(K)
+    wall = [0.0, 310.0, 290.0, 300.0, 300.0, 300.0, 300.0]
+    st = MR.phonon_init(nct, K, cp, tref, 300.0)
+    g = O.fv_metrics(mesh)
+    MR.phonon_walls(mesh, g["fA"], g["fAm"], vg, cp, tref, wall, st["e"])
+    assemble, solve = _phonon_parts(mesh)
+    norms, conv = MR.phonon_advance(assemble, solve, mesh, g, vg, dk3, tau, cp, tref, wall, st, 200, 1e-10, 0.0)
+    assert conv and norms[-1] < 1e-10 * max(norms[:5])
+    T = st["TL"][:nc].reshape(N, N, N)          # [k][j][i]
+    assert T.min() > 290 and T.max() < 310
+    np.testing.assert_allclose(T - 300, -(T[:, :, ::-1] - 300), rtol=0, atol=1e-7)
+    assert T[:, :, 0].mean() > 300 > T[:, :, -1].mean()
