@@ -79,6 +79,34 @@ def kernel_stats(amg, kind):
     return n.value, ms.value
 
 
+def kernel_source_hash():
+    """First 16 hex digits of the SHA-256 of the cycle kernels' source."""
+    import hashlib
+    with open(os.path.join(ROOT, "fvm_amd", "csrc", "cycle.hip"), "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
+
+
+def profiled_traffic(kernel, N):
+    """HBM bytes per launch of `kernel` from the committed counter profile
+    (profiles/hbm_traffic.json: two rocprofv3 passes, FETCH_SIZE and WRITE_SIZE,
+    scaled by a plain copy's counts), used only when the profile was taken on this
+    workload size with the current kernel source; otherwise (None, reason)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            prof = json.load(f)
+    except (OSError, ValueError):
+        return None, "no counter profile"
+    k = prof.get("kernels", {}).get(kernel)
+    if k is None:
+        return None, "kernel not in the counter profile"
+    if prof.get("n") != N:
+        return None, "counter profile taken on %s^3" % prof.get("n")
+    if prof.get("source_sha16") != kernel_source_hash():
+        return None, "counter profile stale (kernel source changed since it was taken)"
+    return k["bytes_per_launch"], "profiles/hbm_traffic.json"
+
+
 def host_cpu():
     """CPU model and logical core count of this host (/proc/cpuinfo, os.cpu_count)."""
     model = None
@@ -670,13 +698,9 @@ def main():
     # dominant kernel of the headline solve: the one with the larger total time
     dom = max([("finest_fp64_spmv_residual", kr[1]), ("finest_fp32_gs_sweep", ks[1])], key=lambda t: t[1])[0]
     kd = kernels[dom]
-    # HBM bytes per launch of these kernels on the 200^3 workload, measured with two
-    # rocprofv3 passes: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the scales a 512 MiB copy
-    # gives); launch shapes as chosen by the autotuner
-    measured_traffic = {"finest_fp64_spmv_residual": 1.389e9, "finest_fp32_gs_sweep": 6.31e8}
+    traffic, traffic_note = profiled_traffic(dom, N)
     roofline = dict(bound="hbm", kernel=dom, achieved=round(kd["gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(kd["gbs"] / HBM_PEAK_GBS, 4),
-                    traffic=measured_traffic.get(dom) if N == 200 else None,
+                    frac=round(kd["gbs"] / HBM_PEAK_GBS, 4), traffic=traffic, traffic_note=traffic_note,
                     algorithmic_bytes=kd["bytes"], avg_us=round(kd["avg_us"], 2))
 
     cpu = None
