@@ -89,6 +89,7 @@ SIGNATURES = {
     "fvm_mesh_topology": (_i, [_p, _p, _p, _p]),
     "fvm_mesh_gradient": (_i, [_p, _p, _p]),
     "fvm_convdiff_assemble": (_i, [_p, _p, _p, _p, _p, _p, _i, _p, _p, ctypes.POINTER(_p)]),
+    "fvm_convdiff_reassemble_noalloc": (_i, [_p, _p, _p, _p, _p, _p, _i, _p, _p, _p]),
     "fvm_mesh_info": (_i, [_p, _p, _p, _p, _p, _p]),
     "fvm_mesh_get": (_i, [_p, _p, _p, _p, _p]),
     "fvm_mesh_metrics": (_i, [_p, _p, _p, _p, _p, _p]),

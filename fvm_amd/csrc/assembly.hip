@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 namespace fvm {
 namespace {
@@ -949,6 +950,32 @@ void assemble_thermal(fvm_mesh *m, fvm_system *S, const double *d_k, const doubl
 }
 }  // namespace
 
+// Scratch of the allocation-free re-assembly (fvm_convdiff_reassemble_noalloc):
+// per-face diffusion coefficients and fluxes (2 n_faces) and the cell gradients
+// (3 (n_cells + n_ghost)). One copy per device, held until the process ends.
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = (size_t)1 << 22;   // 32 MiB
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[MAX_DEVICES] = {};
+
+// device address of this device's scratch copy (looked up once per device, with
+// that device current); the symbol's host-side name is not a device address
+double *scratch_on(int device) {
+    FVM_CHECK(device >= 0 && device < MAX_DEVICES, FVM_E_INVALID, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        FVM_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        FVM_CHECK(ptr != nullptr, FVM_E_HIP, "hipGetSymbolAddress returned a null scratch address");
+        g_scratch_ptr[device] = (double *)ptr;
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace
+
 int mesh_device(const fvm_mesh *m) { return m->device; }
 
 void mesh_device_metrics(fvm_mesh *m, const int **fcells, const double **fA, const double **fAm) {
@@ -1214,6 +1241,31 @@ int fvm_convdiff_assemble(fvm_mesh *m, const double *d_k, const double *d_source
 int fvm_thermal_assemble(fvm_mesh *m, const double *d_k, const double *d_source, const double *d_x,
                          const int32_t *bc_type, const double *bc_value, fvm_system **out) {
     return fvm_convdiff_assemble(m, d_k, d_source, d_x, nullptr, nullptr, 0, bc_type, bc_value, out);
+}
+
+int fvm_convdiff_reassemble_noalloc(fvm_mesh *m, const double *d_k, const double *d_source, const double *d_x,
+                                    const double *d_flux, const double *d_cont_res, int central,
+                                    const int32_t *bc_type, const double *bc_value, fvm_system *S) {
+    FVM_API_BEGIN
+    FVM_CHECK(m && d_k && d_source && d_x && bc_type && bc_value && S, FVM_E_INVALID, "null argument");
+    FVM_CHECK(central == 0 || central == 1, FVM_E_INVALID, "central must be 0 or 1");
+    FVM_CHECK(m->lsq, FVM_E_INVALID,
+              "the mesh has no metrics or gradient coefficients yet: assemble it once with fvm_convdiff_assemble");
+    const int nct = m->n_cells + m->n_ghost, nf = m->n_faces;
+    FVM_CHECK(S->device == m->device && S->n_self == m->n_cells && S->n_total == nct && S->nnz == m->nnz,
+              FVM_E_INVALID, "the system was not assembled from this mesh");
+    const size_t need = 2 * (size_t)nf + 3 * (size_t)nct;
+    FVM_CHECK(need <= fvm::NOALLOC_SCRATCH_DOUBLES, FVM_E_INVALID,
+              "mesh too large for the allocation-free re-assembly: 2 n_faces + 3 (n_cells + n_ghost) = " +
+                  std::to_string(need) + " doubles, at most " + std::to_string(fvm::NOALLOC_SCRATCH_DOUBLES));
+    fvm::Dev d(m->device);
+    double *scr = fvm::scratch_on(m->device);
+    const fvm::Groups G = fvm::make_groups(m, bc_type, bc_value);
+    fvm::assemble_thermal(m, S, d_k, d_source, d_x, d_flux, d_cont_res, central, G, scr, scr + nf,
+                          scr + 2 * (size_t)nf);
+    FVM_HIP(hipStreamSynchronize(m->stream));
+    return FVM_OK;
+    FVM_API_END
 }
 
 int fvm_phonon_assemble(fvm_mesh *m, const double *vg, double tau, const double *d_e0, const double *d_x,

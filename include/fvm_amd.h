@@ -429,6 +429,19 @@ int fvm_thermal_assemble(fvm_mesh *m, const double *d_k, const double *d_source,
 int fvm_convdiff_assemble(fvm_mesh *m, const double *d_k, const double *d_source, const double *d_x,
                           const double *d_flux, const double *d_cont_res, int central, const int32_t *bc_type,
                           const double *bc_value, fvm_system **out);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight time-stepping or serving loops).
+ * fvm_convdiff_assemble's linearization (same kernels, same results) into an
+ * existing system S that fvm_convdiff_assemble / fvm_thermal_assemble built from
+ * this mesh, with no allocation: its scratch (per-face coefficients and fluxes,
+ * cell gradients) is a per-device static array of 4,194,304 doubles (32 MiB, held
+ * until the process ends), so 2 n_faces + 3 (n_cells + n_ghost) must not exceed
+ * that (FVM_E_INVALID otherwise), and the mesh must have been assembled once
+ * (its metrics and gradient coefficients exist). Every call on a device shares
+ * that device's scratch: calls that overlap on one device (other threads) race. */
+int fvm_convdiff_reassemble_noalloc(fvm_mesh *m, const double *d_k, const double *d_source, const double *d_x,
+                                    const double *d_flux, const double *d_cont_res, int central,
+                                    const int32_t *bc_type, const double *bc_value, fvm_system *S);
 /* One discrete-ordinate direction of a gray phonon BTE (scalar system per
  * direction, PhononModel::linearizePhononModel, PhononModel.h:437-523):
  * relaxation -V (x - e0)/tau (PhononCollisionDiscretization.h:54-68), upwind
