@@ -144,7 +144,7 @@ struct fvm_amg {
     int coarsen_lanes = 0;             // lanes per row of the agglomeration rounds (0: by row width)
     int coarsen_small_lanes = 0;       // the same for the single-workgroup rounds (0: as coarsen_lanes)
     int coarsen_grid = 0;              // blocks per launched agglomeration round (0: 2048 / 1024)
-    int bipartite_levels = 0;          // levels tried with a breadth-first two-colouring first
+    int bipartite_levels = 1;          // levels tried with a breadth-first two-colouring first (red-black)
     int sell_block = 256;              // threads per block of the SELL-64 smoother kernels
     int spmv_block = 256;              // threads per block of the SELL-64 residual / SpMV kernels
     int sell_xcd = 1, spmv_xcd = 0;    // XCD-contiguous block order for those kernel classes

@@ -195,7 +195,15 @@ int fvm_amg_kernel_timing(const fvm_amg *h, int kind, long long *launches, doubl
  * this many finest levels are read with non-temporal loads (default 2);
  * "coarsen_lanes" -- lanes per row in the agglomeration rounds (8/16/32/64; 0,
  * the default: chosen per level from its row width); "coarsen_small_rows" --
- * levels with at most this many rows are agglomerated by one workgroup. */
+ * levels with at most this many rows are agglomerated by one workgroup;
+ * "coarsen_small_lanes" -- lanes per row in those single-workgroup rounds (0: as
+ * coarsen_lanes); "coarsen_grid" -- blocks per launched agglomeration round (0:
+ * 2048 for narrow rows, 1024 otherwise); "bipartite_levels" -- this many finest
+ * levels are first coloured by breadth-first parity (red-black order, two
+ * colours) and keep it when no edge joins equal colours, else the greedy
+ * colouring (default 1; takes effect at the next setup). Only the colouring
+ * keys change the coloured smoother's arithmetic (another Gauss-Seidel order);
+ * the others never change a result. */
 int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value);
 int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value);
 /* Autotuner (no counterpart in the reference): times every candidate launch
