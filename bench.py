@@ -72,11 +72,24 @@ def level_sizes(amg):
 
 
 def kernel_stats(amg, kind):
-    from fvm_amd._lib import lib
+    """(launches, total ms, total algorithmic bytes) of the handle's timed
+    finest-level kernels of one kind (0: FP64 residual/SpMV, 1: smoother sweep
+    directions), measured with HIP events on the handle's stream."""
+    from fvm_amd._lib import check, lib
     n = ctypes.c_longlong()
     ms = ctypes.c_double()
-    lib().fvm_amg_kernel_timing(amg._h, kind, ctypes.byref(n), ctypes.byref(ms))
-    return n.value, ms.value
+    nb = ctypes.c_double()
+    check(lib().fvm_amg_kernel_timing(amg._h, kind, ctypes.byref(n), ctypes.byref(ms)))
+    check(lib().fvm_amg_kernel_bytes(amg._h, kind, ctypes.byref(nb)))
+    return n.value, ms.value, nb.value
+
+
+def kernel_entry(st, formula):
+    """Per-launch figures of kernel_stats(): average duration, average algorithmic
+    bytes and their rate."""
+    n, ms, nb = st
+    return dict(launches=n, avg_us=1e3 * ms / n, bytes=nb / n, bytes_formula=formula,
+                gbs=nb / (ms * 1e-3) / 1e9)
 
 
 def kernel_source_hash():
@@ -330,41 +343,120 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def apply_knobs(amg, args):
+    """The launch shapes the autotuner chose in a timed run (printed in its JSON
+    line under autotune.chosen); none of them changes a result."""
+    for key in ("sell_block", "spmv_block", "tail_max_rows", "sell_xcd", "spmv_xcd", "cycle_graph", "nt_levels"):
+        amg.set_tuning(key, getattr(args, key))
+
+
 def run_profile(args):
     """Short program for rocprofv3 (kernel trace and PMC passes): the 8M-cell
-    system, one mixed-precision setup, `--steps` preconditioner cycles of each
-    kind (mixed FP32/BF16 and plain FP64, coloured layout) and one 1 GiB device
-    copy (512 MiB read + 512 MiB written) that calibrates the HBM byte counters."""
+    system, `--steps` headline solves (setup, FP64 FGMRES(30) with the mixed
+    FP32/BF16 AMG to 1e-8, with the given launch shapes) with the finest-level
+    kernels timed live as in the timed run, three plain FP64 cycles of the
+    coloured hierarchy, and one 1 GiB device copy (512 MiB read + 512 MiB
+    written) that calibrates the HBM byte counters."""
     import torch
+    import fvm_amd
+    from fvm_amd._lib import check, lib
     N = args.n
     mesh, S, _ = build_workload(N, torch)
     n_total = S.n_total
     b = S.device_pointers()["b"]
-    for prec in (1, 0):
-        amg = make_amg(1, prec)
-        # the launch shapes the autotuner chose in the timed run (printed in its JSON line)
-        amg.set_tuning("sell_block", args.sell_block)
-        amg.set_tuning("spmv_block", args.spmv_block)
-        amg.set_tuning("tail_max_rows", args.tail_max_rows)
-        amg.set_tuning("sell_xcd", args.sell_xcd)
-        amg.set_tuning("spmv_xcd", args.spmv_xcd)
-        amg.set_tuning("cycle_graph", args.cycle_graph)
-        amg.set_tuning("nt_levels", args.nt_levels)
-        setup_on(amg, S)
+    amg = make_amg(1, 1)
+    apply_knobs(amg, args)
+    fg = fvm_amd.FGMRES(amg)
+    fg.restart = 30
+    fg.relativeTolerance = 1e-8
+    fg.nMaxIterations = 2000
+    check(lib().fvm_amg_set_kernel_timing(amg._h, 1))
+    for _ in range(args.steps):
         x = torch.zeros(n_total, dtype=torch.float64, device="cuda:0")
-        from fvm_amd._lib import check, lib
-        import ctypes
-        for _ in range(args.steps):
-            check(lib().fvm_amg_smooth_device(amg._h, ctypes.c_void_p(b), ctypes.c_void_p(x.data_ptr())))
-        torch.cuda.synchronize()
-        lv = level_sizes(amg)
-        log("profiled %s cycles: n=%d nnz=%d" % ("mixed" if prec else "fp64", lv[0][0], lv[0][1]))
-        del amg
+        setup_on(amg, S)
+        r0, rf, it = fg.solve_device(b, x)
+    torch.cuda.synchronize()
+    lv = level_sizes(amg)
+    kernels = {"finest_fp64_spmv_residual": kernel_entry(kernel_stats(amg, 0), "28n+12nnz"),
+               "finest_fp32_gs_sweep": kernel_entry(kernel_stats(amg, 1), "20 per row + 8 per off-diagonal")}
+    log("profiled %d mixed solves: n=%d nnz=%d, %d FGMRES iterations; live: %s"
+        % (args.steps, lv[0][0], lv[0][1], it, json.dumps(kernels)))
+    del fg, amg
+    plain = make_amg(1, 0)
+    apply_knobs(plain, args)
+    setup_on(plain, S)
+    x = torch.zeros(n_total, dtype=torch.float64, device="cuda:0")
+    for _ in range(3):
+        check(lib().fvm_amg_smooth_device(plain._h, ctypes.c_void_p(b), ctypes.c_void_p(x.data_ptr())))
+    torch.cuda.synchronize()
+    log("profiled 3 plain FP64 cycles")
+    del plain
     src = torch.ones(64 << 20, dtype=torch.float64, device="cuda:0")
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
     log("calibration copy: %d bytes read, %d bytes written" % (src.numel() * 8, dst.numel() * 8))
+
+
+def _counter_rows(path, counter):
+    """Dispatches of a rocprofv3 counter-collection CSV: (dispatch id, kernel
+    name, grid size, value of `counter`), in dispatch order."""
+    import csv
+    rows = []
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            if r.get("Counter_Name") != counter:
+                continue
+            rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], int(r["Grid_Size"]), float(r["Counter_Value"])))
+    rows.sort()
+    return rows
+
+
+def write_traffic_profile(fetch_csv, write_csv, n, out_path):
+    """HBM bytes per launch of the finest-level kernels from the two counter
+    passes of `bench.py --profile` (FETCH_SIZE, then WRITE_SIZE, KiB per
+    dispatch), each scaled by what the counter reads for the program's final
+    512 MiB copy. The SpMV is its largest-grid dispatches; a smoother direction is
+    the FP32 colour launches with at least n^3/4 threads (the finest level's
+    colours: no coarser level has that many rows in one colour), summed per
+    headline solve cycle and halved (forward + backward)."""
+    fetch = _counter_rows(fetch_csv, "FETCH_SIZE")
+    write = _counter_rows(write_csv, "WRITE_SIZE")
+    copy_bytes = 512 << 20
+    scale = {}
+    for name, rows in (("FETCH_SIZE", fetch), ("WRITE_SIZE", write)):
+        cp = [r for r in rows if "copyBuffer" in r[1]]
+        if not cp:
+            raise SystemExit("no calibration copy in %s" % name)
+        scale[name] = copy_bytes / (cp[-1][3] * 1024.0)
+    N3 = n ** 3
+
+    def per_launch(pick, group=1):
+        tot = []
+        for rows, name in ((fetch, "FETCH_SIZE"), (write, "WRITE_SIZE")):
+            sel = [r for r in rows if pick(r)]
+            tot.append((len(sel), sum(r[3] for r in sel) * 1024.0 * scale[name]))
+        (nf, bf), (nw, bw) = tot
+        if nf != nw or nf == 0:
+            raise SystemExit("dispatch counts differ between the passes (%d, %d)" % (nf, nw))
+        return dict(dispatches=nf, fetch_bytes=bf / nf * group, write_bytes=bw / nf * group,
+                    bytes_per_launch=(bf + bw) / nf * group)
+
+    spmv = [r for r in fetch if "k_sell_spmv" in r[1]]
+    gmax = max(r[2] for r in spmv)
+    kern = {"finest_fp64_spmv_residual": per_launch(lambda r: "k_sell_spmv" in r[1] and r[2] == gmax)}
+    sweeps = lambda r: "k_sell_gs<" in r[1] and "float" in r[1] and "double" not in r[1] and r[2] >= N3 // 4
+    nsw = sum(1 for r in fetch if sweeps(r))
+    if nsw == 0 or nsw % 3:
+        raise SystemExit("finest smoother launches (%d) are not 3 per cycle (two colours expected)" % nsw)
+    # 2 forward colour launches and 1 backward launch per cycle with a two-colouring
+    kern["finest_fp32_gs_sweep"] = per_launch(sweeps, group=nsw / (2.0 * (nsw / 3.0)))
+    kern["finest_fp32_gs_sweep"]["note"] = "per direction: (2 forward + 1 backward colour launches) / 2"
+    prof = {"n": n, "source_sha16": kernel_source_hash(),
+            "scale": {k: round(v, 4) for k, v in scale.items()}, "kernels": kern}
+    with open(out_path, "w") as f:
+        json.dump(prof, f, indent=1)
+    print(json.dumps(prof, indent=1))
 
 
 def thermal_system(torch, N, sigma, block, seed):
@@ -543,6 +635,8 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="short program for rocprofv3: a few cycles of the hot kernels and a calibration copy")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="no GPU: write profiles/hbm_traffic.json from the two counter passes of --profile")
     ap.add_argument("--sell-block", type=int, default=256, help="--profile: smoother threads per block")
     ap.add_argument("--spmv-block", type=int, default=256, help="--profile: residual/SpMV threads per block")
     ap.add_argument("--tail-max-rows", type=int, default=40000, help="--profile: single-workgroup tail size")
@@ -568,6 +662,10 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(rows would shard by mesh slabs with halo exchange, not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.traffic:
+        write_traffic_profile(args.traffic[0], args.traffic[1], args.n,
+                              os.path.join(ROOT, "profiles", "hbm_traffic.json"))
+        return
     if args.batch:
         run_batch(args)
         return
@@ -644,7 +742,7 @@ def main():
 
     # ---------------- plain FP64 GPU AMG (coloured GS), same system
     t_plain, cyc_plain, err, tune_plain = None, None, None, None
-    kp_s = (0, 0.0)
+    kp_s = (0, 0.0, 0.0)
     if not args.skip_plain:
         plain = make_amg(1, 0)
         # its own launch shapes: the FP64 hierarchy is tuned like the mixed one (untimed)
@@ -680,21 +778,20 @@ def main():
         log("exact-order FP64 AMG: %.3f s, %d cycles" % (t_exact, cyc_exact))
         del ex
 
-    # ---------------- roofline of the finest-level kernels (algorithmic bytes)
-    bytes_sweep32 = 20 * n0 + 8 * nnz0          # one FP32 GS sweep: diag,b,x r/w,rowptr + col,val
-    bytes_resid64 = 36 * n0 + 12 * nnz0         # FP64 residual / SpMV
+    # ---------------- roofline of the finest-level kernels (algorithmic bytes counted by
+    # the library per timed launch: per row the diagonal, the vectors and a 4-byte row
+    # offset, per off-diagonal a 4-byte column and the value; a sweep direction counts
+    # the rows of the colours it updates)
     kernels = {}
     if kr[0]:
-        kernels["finest_fp64_spmv_residual"] = dict(launches=kr[0], avg_us=1e3 * kr[1] / kr[0],
-                                                    bytes=bytes_resid64,
-                                                    gbs=bytes_resid64 / (kr[1] / kr[0] * 1e-3) / 1e9)
+        kernels["finest_fp64_spmv_residual"] = kernel_entry(
+            kr, "residual 36n+12nnz, SpMV 28n+12nnz (FP64)")
     if ks[0]:
-        kernels["finest_fp32_gs_sweep"] = dict(launches=ks[0], avg_us=1e3 * ks[1] / ks[0], bytes=bytes_sweep32,
-                                               gbs=bytes_sweep32 / (ks[1] / ks[0] * 1e-3) / 1e9)
+        kernels["finest_fp32_gs_sweep"] = kernel_entry(
+            ks, "20 per updated row + 8 per its off-diagonal (FP32), per direction")
     if kp_s[0]:
-        b64 = 36 * n0 + 12 * nnz0
-        kernels["plain_finest_fp64_gs_sweep"] = dict(launches=kp_s[0], avg_us=1e3 * kp_s[1] / kp_s[0], bytes=b64,
-                                                     gbs=b64 / (kp_s[1] / kp_s[0] * 1e-3) / 1e9)
+        kernels["plain_finest_fp64_gs_sweep"] = kernel_entry(
+            kp_s, "36 per updated row + 12 per its off-diagonal (FP64), per direction")
     # dominant kernel of the headline solve: the one with the larger total time
     dom = max([("finest_fp64_spmv_residual", kr[1]), ("finest_fp32_gs_sweep", ks[1])], key=lambda t: t[1])[0]
     kd = kernels[dom]

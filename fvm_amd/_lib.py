@@ -72,6 +72,7 @@ SIGNATURES = {
     "fvm_amg_set_tuning": (_i, [_p, ctypes.c_char_p, _ll]),
     "fvm_amg_get_tuning": (_i, [_p, ctypes.c_char_p, ctypes.POINTER(_ll)]),
     "fvm_amg_kernel_timing": (_i, [_p, _i, _p, _p]),
+    "fvm_amg_kernel_bytes": (_i, [_p, _i, _p]),
     "fvm_fgmres_default_params": (_i, [ctypes.POINTER(FgmresParams)]),
     "fvm_fgmres_create": (_i, [ctypes.POINTER(_p), _p]),
     "fvm_fgmres_destroy": (_i, [_p]),

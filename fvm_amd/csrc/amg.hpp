@@ -46,6 +46,7 @@ struct Level {
     DArr<int> perm, iperm;
     int ncolor = 0;
     std::vector<int> color_ptr;        // host: row ranges of each colour (coloured index)
+    std::vector<long long> color_nnz;  // host: off-diagonal entries of each colour's rows
     DArr<int> d_color_ptr;             // the same on the device
     DArr<int> sell_off, sell_col;
     int sell_total = 0;                // SELL-64 elements (padding included) = sell_off[nslices]
@@ -162,9 +163,11 @@ struct fvm_amg {
     // live timing of finest-level kernels with HIP events on the handle's stream
     bool ktime = false;
     std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_open;
+    struct TimedLaunch { int kind; hipEvent_t start, stop; double bytes; };
+    std::vector<TimedLaunch> ev_open;
     long long k_launches[4] = {0, 0, 0, 0};
     double k_ms[4] = {0, 0, 0, 0};
+    double k_bytes[4] = {0, 0, 0, 0};  // algorithmic bytes of the timed launches
     // double-buffer used by pointer-swapped Jacobi
     // convergence lines at verbosity > 0 (AMG.cpp:237-272): "i: [field : norm]"
     std::string field_name = "x";
@@ -187,7 +190,8 @@ inline void print_norm(const fvm_amg *h, const std::string &label, double v) {
 // kernel timing kinds
 enum { KT_RESID64 = 0, KT_SWEEP = 1 };
 hipEvent_t timing_begin(fvm_amg *h);
-void timing_end(fvm_amg *h, int kind, hipEvent_t start);
+// bytes: the algorithmic HBM bytes of the timed work (every operand read or written once)
+void timing_end(fvm_amg *h, int kind, hipEvent_t start, double bytes);
 void timing_harvest(fvm_amg *h);
 // setup.hip
 void build_hierarchy(fvm_amg *h, int n_self, int n_total, const int *d_row, const int *d_col,

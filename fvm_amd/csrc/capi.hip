@@ -437,7 +437,7 @@ int fvm_amg_set_kernel_timing(fvm_amg *h, int enable) {
     FVM_HIP(hipStreamSynchronize(h->stream));
     fvm::timing_harvest(h);
     h->ktime = enable != 0;
-    for (int k = 0; k < 4; k++) { h->k_launches[k] = 0; h->k_ms[k] = 0.0; }
+    for (int k = 0; k < 4; k++) { h->k_launches[k] = 0; h->k_ms[k] = 0.0; h->k_bytes[k] = 0.0; }
     return FVM_OK;
     FVM_API_END
 }
@@ -446,6 +446,12 @@ int fvm_amg_kernel_timing(const fvm_amg *h, int kind, long long *launches, doubl
     if (!h || kind < 0 || kind > 3) return FVM_E_INVALID;
     if (launches) *launches = h->k_launches[kind];
     if (total_ms) *total_ms = h->k_ms[kind];
+    return FVM_OK;
+}
+
+int fvm_amg_kernel_bytes(const fvm_amg *h, int kind, double *total_bytes) {
+    if (!h || kind < 0 || kind > 3 || !total_bytes) return FVM_E_INVALID;
+    *total_bytes = h->k_bytes[kind];
     return FVM_OK;
 }
 

@@ -37,7 +37,7 @@ hipEvent_t timing_begin(fvm_amg *h) {
     return e;
 }
 
-void timing_end(fvm_amg *h, int kind, hipEvent_t start) {
+void timing_end(fvm_amg *h, int kind, hipEvent_t start, double bytes) {
     if (!h->ktime || !start) return;
     hipEvent_t e;
     if (h->ev_pool.empty()) {
@@ -47,18 +47,19 @@ void timing_end(fvm_amg *h, int kind, hipEvent_t start) {
         h->ev_pool.pop_back();
     }
     FVM_HIP(hipEventRecord(e, h->stream));
-    h->ev_open.push_back({kind, {start, e}});
+    h->ev_open.push_back({kind, start, e, bytes});
 }
 
 // after the stream is synchronized: accumulate and recycle the events
 void timing_harvest(fvm_amg *h) {
     for (auto &it : h->ev_open) {
         float ms = 0.f;
-        FVM_HIP(hipEventElapsedTime(&ms, it.second.first, it.second.second));
-        h->k_launches[it.first]++;
-        h->k_ms[it.first] += ms;
-        h->ev_pool.push_back(it.second.first);
-        h->ev_pool.push_back(it.second.second);
+        FVM_HIP(hipEventElapsedTime(&ms, it.start, it.stop));
+        h->k_launches[it.kind]++;
+        h->k_ms[it.kind] += ms;
+        h->k_bytes[it.kind] += it.bytes;
+        h->ev_pool.push_back(it.start);
+        h->ev_pool.push_back(it.stop);
     }
     h->ev_open.clear();
 }
@@ -554,6 +555,15 @@ inline int value_type(const Run &R, int l) {
     return l == 0 ? V32 : V16;
 }
 
+// Algorithmic bytes of one pass over `rows` rows holding `nnz` off-diagonal
+// entries: per row the diagonal, `vecs` vectors of xsize bytes (read or written
+// once each) and a 4-byte row offset; per entry a 4-byte column and its value.
+inline double pass_bytes(int vt, int xsize, int vecs, long long rows, long long nnz) {
+    const int vsize = vt == V64 ? 8 : (vt == V32 ? 4 : 2);
+    const int dsize = vt == V64 ? 8 : 4;
+    return (double)rows * (dsize + vecs * xsize + 4) + (double)nnz * (4 + vsize);
+}
+
 template <class X> X *vb(Level &L);
 template <class X> X *vx(Level &L);
 template <class X> X *vr(Level &L);
@@ -597,7 +607,7 @@ void residual(Run &R, int l, const X *x, const X *b, X *r) {
         FVM_SELL_U(L.sell_u, l, (k_sell_residual<SU, SNT><<<grid_for(n, sb), sb, 0, s>>>(n, L.sell_off.p, L.sell_col.p, val,
                                                                                dg, x, b, r, R.h->spmv_xcd))),
         (k_csr_residual<<<grid_for(n, BS), BS, 0, s>>>(n, L.row.p, L.col.p, val, dg, x, b, r)));
-    if (t0) timing_end(R.h, KT_RESID64, t0);
+    if (t0) timing_end(R.h, KT_RESID64, t0, pass_bytes(V64, sizeof(X), 3, n, L.nnz));   // x, b read; r written
 }
 
 template <class X>
@@ -609,6 +619,7 @@ void gs_sweep(Run &R, int l, X *x, const X *b) {
     if (R.fast) {
         for (int dir = 0; dir < 2; dir++) {
             hipEvent_t t0 = (l == 0) ? timing_begin(R.h) : nullptr;
+            long long rows = 0, nnz = 0;
             // the backward pass starts with the colour the forward pass ended with:
             // its rows read only rows of other colours, unchanged since, so that
             // launch would rewrite the same values and is skipped
@@ -616,6 +627,8 @@ void gs_sweep(Run &R, int l, X *x, const X *b) {
                 const int c = dir ? L.ncolor - 1 - k : k;
                 const int beg = L.color_ptr[c], end = L.color_ptr[c + 1];
                 if (end <= beg) continue;
+                rows += end - beg;
+                nnz += L.color_nnz[c];
                 FVM_DISPATCH_VALS(R, l, L, X,
                     FVM_SELL_U(L.sell_u, l,
                         (k_sell_gs<SU, SNT><<<grid_for(end - (beg & ~63), sb), sb, 0, s>>>(
@@ -623,7 +636,8 @@ void gs_sweep(Run &R, int l, X *x, const X *b) {
                             R.h->sell_xcd))),
                     (void)0);
             }
-            if (t0) timing_end(R.h, KT_SWEEP, t0);
+            // b read, x read and written (the neighbours' x are counted once per updated row)
+            if (t0) timing_end(R.h, KT_SWEEP, t0, pass_bytes(value_type(R, l), sizeof(X), 3, rows, nnz));
         }
     } else {
         for (int dir = 0; dir < 2; dir++) {
@@ -1599,9 +1613,9 @@ void spmv64(Run &R, const double *x, double *y) {
     if (n == 0) return;
     struct T {
         fvm_amg *h; hipEvent_t e;
-        T(fvm_amg *hh) : h(hh), e(timing_begin(hh)) {}
-        ~T() { if (e) timing_end(h, KT_RESID64, e); }
-    } timer(h);
+        long long n, nnz;
+        ~T() { if (e) timing_end(h, KT_RESID64, e, pass_bytes(V64, 8, 2, n, nnz)); }   // x read, y written
+    } timer{h, timing_begin(h), n, F.nnz};
     if (R.fast)
         k_sell_spmv<<<grid_for(n, h->spmv_block), h->spmv_block, 0, h->stream>>>(n, F.sell_off.p, F.sell_col.p, F.sell_val.p,
                                                            F.pdiag.p, x, y, h->spmv_xcd);

@@ -976,15 +976,19 @@ __global__ void k_invert_perm(int n, const int *perm, int *iperm) {
 }
 
 // colour histogram: per-block counts in LDS, one global atomic per colour and block
-__global__ void k_color_hist(int n, const int *color, int *hist) {
+// rows and off-diagonal entries of each colour
+__global__ void k_color_hist(int n, const int *color, const int *row, int *hist, unsigned long long *hnnz) {
     __shared__ int h[257];
-    for (int c = threadIdx.x; c < 257; c += blockDim.x) h[c] = 0;
+    __shared__ unsigned long long z[257];
+    for (int c = threadIdx.x; c < 257; c += blockDim.x) { h[c] = 0; z[c] = 0; }
     __syncthreads();
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         atomicAdd(&h[color[i]], 1);
+        atomicAdd(&z[color[i]], (unsigned long long)(row[i + 1] - row[i]));
+    }
     __syncthreads();
     for (int c = threadIdx.x; c < 257; c += blockDim.x)
-        if (h[c]) atomicAdd(&hist[c], h[c]);
+        if (h[c]) { atomicAdd(&hist[c], h[c]); atomicAdd(&hnnz[c], z[c]); }
 }
 
 // ------------------------------------------------------------------ SELL-64
@@ -1745,13 +1749,21 @@ void prepare_fast_level(fvm_amg *h, Level &L, int l, hipStream_t s, Workspace &w
                                                idx.p, L.perm.p, n, 0, key_bits, s));
     k_invert_perm<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.perm.p, L.iperm.p);
     DArr<int> hist(257);
+    DArr<unsigned long long> hnnz(257);
     FVM_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), s));
-    k_color_hist<<<grid_for(n, BS, 1024), BS, 0, s>>>(n, color.p, hist.p);
+    FVM_HIP(hipMemsetAsync(hnnz.p, 0, hnnz.bytes(), s));
+    k_color_hist<<<grid_for(n, BS, 1024), BS, 0, s>>>(n, color.p, L.row.p, hist.p, hnnz.p);
     std::vector<int> hh(257);
+    std::vector<unsigned long long> hz(257);
     FVM_HIP(hipMemcpyAsync(hh.data(), hist.p, hist.bytes(), hipMemcpyDeviceToHost, s));
+    FVM_HIP(hipMemcpyAsync(hz.data(), hnnz.p, hnnz.bytes(), hipMemcpyDeviceToHost, s));
     FVM_HIP(hipStreamSynchronize(s));
     L.color_ptr.assign(1, 0);
-    for (int c = 0; c < 257 && L.color_ptr.back() < n; c++) L.color_ptr.push_back(L.color_ptr.back() + hh[c]);
+    L.color_nnz.clear();
+    for (int c = 0; c < 257 && L.color_ptr.back() < n; c++) {
+        L.color_ptr.push_back(L.color_ptr.back() + hh[c]);
+        L.color_nnz.push_back((long long)hz[c]);
+    }
     L.ncolor = (int)L.color_ptr.size() - 1;
     L.d_color_ptr.alloc(L.color_ptr.size());
     FVM_HIP(hipMemcpyAsync(L.d_color_ptr.p, L.color_ptr.data(), sizeof(int) * L.color_ptr.size(),

@@ -182,6 +182,12 @@ int fvm_amg_direct_apply(fvm_amg *h, const double *b, double *x);
 enum { FVM_KT_RESIDUAL64 = 0, FVM_KT_SWEEP = 1 };
 int fvm_amg_set_kernel_timing(fvm_amg *h, int enable);
 int fvm_amg_kernel_timing(const fvm_amg *h, int kind, long long *launches, double *total_ms);
+/* Algorithmic HBM bytes of the same timed launches, summed: every operand read or
+ * written once (per row: the diagonal, the vectors, a 4-byte row offset; per
+ * off-diagonal entry: a 4-byte column and its value, at the precision the level
+ * is stored in). A smoother direction counts the rows of the colours it updates,
+ * so a backward direction that skips a colour counts fewer bytes. */
+int fvm_amg_kernel_bytes(const fvm_amg *h, int kind, double *total_bytes);
 /* Performance knobs that never change results (no reference counterpart; set
  * by the autotuner): "tail_max_rows" -- V-cycle levels with at most this many
  * rows run in one single-workgroup launch (0 disables); "sell_block" -- threads
