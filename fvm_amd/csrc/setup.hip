@@ -858,7 +858,8 @@ __device__ __forceinline__ unsigned hash32(unsigned x) {
 // Jones-Plassmann rounds with first-fit colours; a row is coloured in round t
 // if its (hash, index) priority beats every neighbour not coloured before t.
 // first row of the segment (system of a batch) holding row i; 0 without segments
-__device__ __forceinline__ int seg_base(const int *off, int K, int i) {
+// index of the segment holding row i; 0 without segments
+__device__ __forceinline__ int seg_index(const int *off, int K, int i) {
     if (!off) return 0;
     int lo = 0, hi = K;
     while (hi - lo > 1) {
@@ -866,7 +867,10 @@ __device__ __forceinline__ int seg_base(const int *off, int K, int i) {
         if (off[mid] <= i) lo = mid;
         else hi = mid;
     }
-    return off[lo];
+    return lo;
+}
+__device__ __forceinline__ int seg_base(const int *off, int K, int i) {
+    return off ? off[seg_index(off, K, i)] : 0;
 }
 
 // Colouring priority of each row: larger row first (largest-first ordering needs
@@ -953,13 +957,22 @@ __global__ void k_bfs_round(const int *row, const int *col, const int *rowT, con
     }
 }
 
-__global__ void k_bipartite_check(int n, const int *row, const int *col, const int *color, int *bad) {
+// bad[s] != 0: segment s is not two-coloured (each system of a batch decides alone)
+__global__ void k_bipartite_check(int n, const int *row, const int *col, const int *color, const int *seg_off,
+                                  int nseg, int *bad) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int ci = color[i];
         bool ok = ci >= 0;
         for (int p = row[i]; p < row[i + 1] && ok; p++) ok = color[col[p]] != ci;
-        if (!ok) { atomicOr(bad, 1); return; }
+        if (!ok) atomicOr(&bad[seg_index(seg_off, nseg, i)], 1);
     }
+}
+
+// rows of two-coloured segments take their breadth-first colour, the others keep
+// the greedy one (no edge joins two segments, so the union is a valid colouring)
+__global__ void k_merge_colours(int n, const int *seg_off, int nseg, const int *bad, const int *two, int *color) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (!bad[seg_index(seg_off, nseg, i)]) color[i] = two[i];
 }
 
 __global__ void k_fill_int(int n, int *v, int val) {
@@ -1691,30 +1704,42 @@ void prepare_fast_level(fvm_amg *h, Level &L, int l, hipStream_t s, Workspace &w
     const int *rT = L.sym ? nullptr : L.rowT.p;
     const int *cT = L.sym ? nullptr : L.colT.p;
     DArr<int> color(n), cround(n), left(1);
-    bool two = false;
+    // segments (systems) of the level: with every one two-coloured the greedy
+    // colouring is skipped; with some, it runs and the others keep their red-black
+    const int nsys = seg_off ? nseg : 1;
+    int n_two = 0;
+    DArr<int> bfs_color, bad;
     if (l < h->bipartite_levels) {
         // red-black order when the level's graph has no odd cycle
-        DArr<int> qa(n), qb(n), cnt(4);
-        k_fill_int<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, color.p, -1);
+        DArr<int> qa(n), qb(n), cnt(3);
+        bfs_color.alloc(n);
+        bad.alloc(nsys);
+        k_fill_int<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, bfs_color.p, -1);
         FVM_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes(), s));
-        k_bfs_seed<<<grid_for(std::max(nseg, 1), BS), BS, 0, s>>>(n, seg_off, nseg, color.p, qa.p, cnt.p);
+        k_bfs_seed<<<grid_for(std::max(nseg, 1), BS), BS, 0, s>>>(n, seg_off, nseg, bfs_color.p, qa.p, cnt.p);
         int t = 0;
         for (;;) {
             for (int k = 0; k < 16; k++, t++) {
                 const int *cur = (t & 1) ? qb.p : qa.p;
                 int *nxt = (t & 1) ? qa.p : qb.p;
-                k_bfs_round<<<1024, BS, 0, s>>>(L.row.p, L.col.p, rT, cT, color.p, cur, cnt.p + (t % 3), nxt,
+                k_bfs_round<<<1024, BS, 0, s>>>(L.row.p, L.col.p, rT, cT, bfs_color.p, cur, cnt.p + (t % 3), nxt,
                                                 cnt.p + ((t + 1) % 3), cnt.p + ((t + 2) % 3));
             }
             FVM_HIP(hipGetLastError());
             if (read_scalar(cnt.p + (t % 3), s) == 0) break;
             FVM_CHECK(t < 2 * n + 64, FVM_E_NUMERIC, "breadth-first colouring did not terminate");
         }
-        FVM_HIP(hipMemsetAsync(cnt.p + 3, 0, sizeof(int), s));
-        k_bipartite_check<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.row.p, L.col.p, color.p, cnt.p + 3);
-        two = read_scalar(cnt.p + 3, s) == 0;
+        FVM_HIP(hipMemsetAsync(bad.p, 0, bad.bytes(), s));
+        k_bipartite_check<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.row.p, L.col.p, bfs_color.p, seg_off, nseg,
+                                                               bad.p);
+        std::vector<int> hb(nsys);
+        FVM_HIP(hipMemcpyAsync(hb.data(), bad.p, bad.bytes(), hipMemcpyDeviceToHost, s));
+        FVM_HIP(hipStreamSynchronize(s));
+        for (int v : hb) n_two += v == 0;
     }
-    if (!two) {
+    if (n_two == nsys) {
+        FVM_HIP(hipMemcpyAsync(color.p, bfs_color.p, color.bytes(), hipMemcpyDeviceToDevice, s));
+    } else {
         DArr<unsigned long long> prio(n);
         k_color_prio<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.row.p, seg_off, nseg, prio.p);
         FVM_HIP(hipMemsetAsync(color.p, 0, color.bytes(), s));
@@ -1729,6 +1754,8 @@ void prepare_fast_level(fvm_amg *h, Level &L, int l, hipStream_t s, Workspace &w
             if (read_scalar(left.p, s) == 0) break;
             FVM_CHECK(t < 4 * n + 64, FVM_E_NUMERIC, "colouring did not terminate");
         }
+        if (n_two > 0)
+            k_merge_colours<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, seg_off, nseg, bad.p, bfs_color.p, color.p);
     }
     // stable sort of rows by colour (coarse levels: then by decreasing length) -> perm
     DArr<int> idx(n);
