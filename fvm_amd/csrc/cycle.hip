@@ -1001,6 +1001,16 @@ void direct_apply(fvm_amg *h, const double *d_b, double *d_x) {
 // x = 0 on a fixed right-hand side, timed with HIP events on the handle's stream.
 // ===========================================================================
 namespace {
+// Reads m doubles (a buffer larger than the last-level cache), so that the launch
+// timed after it finds its matrix and vectors in HBM, as it does inside a solve
+// where Gram-Schmidt and the cycle stream gigabytes between two products.
+__global__ void k_tune_evict(long long m, const double *__restrict__ a, double *__restrict__ sink) {
+    double v = 0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < m; i += (long long)gridDim.x * blockDim.x)
+        v += a[i];
+    if (v == 1.0) sink[0] = v;   // never true for the zero buffer: keeps the loads
+}
+
 __global__ void k_tune_rhs(int n, double *b) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         unsigned long long z = (unsigned long long)i * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
@@ -1136,12 +1146,17 @@ std::string autotune(fvm_amg *h, int reps) {
             Run R = make_run(h);
             Run R64 = R;
             R64.mixed = false;
+            // timed with cold caches: 512 MiB read before every launch
+            const long long m_ev = 64LL << 20;
+            DArr<double> ev((size_t)m_ev), sink(1);
+            FVM_HIP(hipMemsetAsync(ev.p, 0, ev.bytes(), s));
             for (int xm = 0; xm < 2; xm++)
             for (int bsz : blocks) {
                 h->spmv_block = bsz;
                 h->spmv_xcd = xm;
                 double tmin = 1e300;
                 for (int r = 0; r <= reps + 2; r++) {
+                    k_tune_evict<<<2048, BS, 0, s>>>(m_ev, ev.p, sink.p);
                     FVM_HIP(hipEventRecord(e0, s));
                     residual<double>(R64, 0, F.x.p, F.b.p, F.r.p);
                     FVM_HIP(hipEventRecord(e1, s));
