@@ -557,6 +557,96 @@ int ofv_thermal(int dim, int n_faces, const double *nodes, const int *fnptr, con
  * The linearizer runs collision before convection (PhononModel.h:449-459).
  * Loops run in the reference's order (faces ascending, groups in order).
  */
+/* One velocity direction of the ESBGK kinetic model: KineticModel::linearizeKineticModel
+ * (esbgkbase/KineticModel.h:1505-1674). Discretizations in the reference's order:
+ * the collision term first (CollisionTermDiscretization.h:46-62:
+ * r -= V nu (f - feq), diag -= V nu, self cells), then upwind convection with the
+ * face flux A.c (ConvectionDiscretization_Kmodel.h:171-197, no grid flux, no central
+ * difference), then the boundary faces group by group (KineticModel.h:1596-1661):
+ * bc_kind[g] = 1 (ZeroGradBC): every direction takes the Dirichlet form with the
+ * ghost's own value; bc_kind[g] = 0 (WallBC, SymmetryBC and the other wall-like
+ * groups): Dirichlet when c.en - uw.en < eps (incoming), else extrapolation
+ * (GenericKineticBCS.h:76-116 and :139-173), uw = wall_u[3g..3g+2]. Then the
+ * elimination of LinearSystem::initSolve. */
+int ofv_kinetic(int dim, int n_faces, const double *nodes, const int *fnptr, const int *face_nodes,
+                const int *face_cells, int n_cells, int n_ghost, int n_groups, const int *gptr,
+                const double *c, const double *nue, const double *feq, const double *x, const int *bc_kind,
+                const double *wall_u, double eps, int *row, int *col, double *diag, double *off, double *b,
+                unsigned char *is_bnd)
+{
+    const mesh_t M = {dim, n_faces, n_cells, n_ghost, n_groups, nodes, fnptr, face_nodes, face_cells, gptr};
+    const int nct = n_cells + n_ghost;
+    geom_t G;
+    geom_build(&M, &G);
+    const int *p01 = G.p01, *p10 = G.p10;
+    memcpy(row, G.row, sizeof(int) * ((size_t)nct + 1));
+    memcpy(col, G.col, sizeof(int) * (size_t)G.row[nct]);
+    memset(diag, 0, sizeof(double) * (size_t)nct);
+    memset(off, 0, sizeof(double) * (size_t)row[nct]);
+    memset(b, 0, sizeof(double) * (size_t)nct);
+    memset(is_bnd, 0, (size_t)nct);
+    double *xf = (double *)malloc(sizeof(double) * (size_t)(nct + 1));
+    memcpy(xf, x, sizeof(double) * (size_t)nct);
+    for (int cc = 0; cc < n_cells; cc++) {
+        b[cc] -= G.vol[cc] * nue[cc] * (xf[cc] - feq[cc]);
+        diag[cc] -= G.vol[cc] * nue[cc];
+    }
+    for (int f = 0; f < n_faces; f++) {
+        const int c0 = face_cells[2 * f], c1 = face_cells[2 * f + 1];
+        const double *A = G.fA + 3 * f;
+        const double F = A[0] * c[0] + A[1] * c[1] + A[2] * c[2];
+        double flux;
+        if (F > 0.0) {
+            flux = F * xf[c0];
+            diag[c0] -= F;
+            off[p10[f]] += F;
+        } else {
+            flux = F * xf[c1];
+            diag[c1] += F;
+            off[p01[f]] -= F;
+        }
+        b[c0] -= flux;
+        b[c1] += flux;
+    }
+    for (int g = 1; g < n_groups; g++)
+        for (int f = gptr[g]; f < gptr[g + 1]; f++) {
+            const int c0 = face_cells[2 * f], c1 = face_cells[2 * f + 1];
+            int incoming = 1;
+            if (bc_kind[g] != 1) {
+                double en[3];
+                for (int k = 0; k < 3; k++) en[k] = G.fA[3 * f + k] / G.fAm[f];
+                const double cn = c[0] * en[0] + c[1] * en[1] + c[2] * en[2];
+                const double *uw = wall_u + 3 * g;
+                const double un = uw[0] * en[0] + uw[1] * en[1] + uw[2] * en[2];
+                incoming = cn - un < eps;
+            }
+            if (incoming) {
+                const double bval = xf[c1];
+                const double dX = bval - xf[c1];
+                b[c0] += off[p01[f]] * dX;
+                off[p01[f]] = 0.0;
+                xf[c1] = bval;
+                off[p10[f]] = 0.0;
+                b[c1] = 0.0;
+                diag[c1] = -1.0;
+            } else {
+                const double dFdX1 = -diag[c1];
+                const double dx01 = xf[c0] - xf[c1];
+                diag[c0] += dFdX1;
+                b[c0] += dFdX1 * dx01;
+                off[p01[f]] = 0.0;
+                diag[c1] = -1.0;
+                off[p10[f]] = 1.0;
+                b[c1] = dx01;
+                is_bnd[c1] = 1;
+            }
+        }
+    eliminate(n_cells, nct, row, col, is_bnd, diag, off, b);
+    free(xf);
+    geom_free(&G);
+    return 0;
+}
+
 int ofv_phonon(int dim, int n_faces, const double *nodes, const int *fnptr, const int *face_nodes,
                const int *face_cells, int n_cells, int n_ghost, int n_groups, const int *gptr,
                const double *vg, double tau, const double *e0, const double *x, int *row, int *col,

@@ -20,6 +20,11 @@ int ofv_thermal(int dim, int n_faces, const double *nodes, const int *fnptr, con
                 const int *bc_type, const double *bc_value, const double *kcell, const double *src,
                 const double *x, const double *conv_flux, const double *cont_res, int central,
                 int *row, int *col, double *diag, double *off, double *b, unsigned char *is_bnd);
+int ofv_kinetic(int dim, int n_faces, const double *nodes, const int *fnptr, const int *face_nodes,
+                const int *face_cells, int n_cells, int n_ghost, int n_groups, const int *gptr,
+                const double *c, const double *nue, const double *feq, const double *x, const int *bc_kind,
+                const double *wall_u, double eps, int *row, int *col, double *diag, double *off, double *b,
+                unsigned char *is_bnd);
 int ofv_phonon(int dim, int n_faces, const double *nodes, const int *fnptr, const int *face_nodes,
                const int *face_cells, int n_cells, int n_ghost, int n_groups, const int *gptr,
                const double *vg, double tau, const double *e0, const double *x, int *row, int *col,

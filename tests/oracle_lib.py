@@ -57,6 +57,7 @@ def lib():
         L.ofv_gradient.argtypes = mesh_args + [_p, _p]
         L.ofv_thermal.argtypes = mesh_args + [_p, _p, _p, _p, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p]
         L.ofv_phonon.argtypes = mesh_args + [_p, _d, _p, _p, _p, _p, _p, _p, _p, _p]
+        L.ofv_kinetic.argtypes = mesh_args + [_p, _p, _p, _p, _p, _p, _d, _p, _p, _p, _p, _p, _p]
         _lib = L
     return _lib
 
@@ -224,6 +225,30 @@ def fv_thermal(mesh, k, src, x, bc_type, bc_value, conv_flux=None, cont_res=None
     cr = None if cont_res is None else np.ascontiguousarray(cont_res, np.float64)
     args, keep = _mesh_args(mesh)
     L.ofv_thermal(*args, ptr(bt), ptr(bv), ptr(k), ptr(src), ptr(x), ptr(cf), ptr(cr), int(bool(central)),
+                  ptr(row), ptr(col), ptr(diag), ptr(off), ptr(b), ptr(isb))
+    return dict(n_self=nc, n_total=nct, row=row, col=col[:nnz], diag=diag, off=off[:nnz], b=b,
+                is_boundary=isb)
+
+
+def fv_kinetic(mesh, c, nue, feq, x, bc_kind, wall_u, eps=1e-50):
+    """Oracle assembly of one ESBGK velocity direction (collision + upwind convection +
+    kinetic boundary faces) -> system dict. bc_kind[g]: 1 zero gradient, 0 wall-like;
+    wall_u: n_groups x 3 wall velocities."""
+    L = lib()
+    nf = mesh["face_cells"].size // 2
+    nc, ng = mesh["n_cells"], mesh["n_ghost"]
+    nct = nc + ng
+    nnz = L.ofv_thermal_nnz(nf, ptr(np.ascontiguousarray(mesh["face_cells"], np.int32)), nc, ng)
+    row = np.zeros(nct + 1, np.int32); col = np.zeros(max(nnz, 1), np.int32)
+    diag = np.zeros(nct); off = np.zeros(max(nnz, 1)); b = np.zeros(nct); isb = np.zeros(nct, np.uint8)
+    cv = np.ascontiguousarray(c, np.float64)
+    nue = np.ascontiguousarray(nue, np.float64)
+    feq = np.ascontiguousarray(feq, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    kind = np.ascontiguousarray(bc_kind, np.int32)
+    wu = np.ascontiguousarray(wall_u, np.float64).reshape(-1)
+    args, keep = _mesh_args(mesh)
+    L.ofv_kinetic(*args, ptr(cv), ptr(nue), ptr(feq), ptr(x), ptr(kind), ptr(wu), ctypes.c_double(eps),
                   ptr(row), ptr(col), ptr(diag), ptr(off), ptr(b), ptr(isb))
     return dict(n_self=nc, n_total=nct, row=row, col=col[:nnz], diag=diag, off=off[:nnz], b=b,
                 is_boundary=isb)
