@@ -43,6 +43,12 @@ class LinSolverParams(ctypes.Structure):
                 ("absoluteTolerance", _d)]
 
 
+class KineticOptions(ctypes.Structure):
+    _fields_ = [("rho_init", _d), ("T_init", _d), ("nonDimLt", _d), ("molecularWeight", _d), ("Tmuref", _d),
+                ("muref", _d), ("mu_w", _d), ("Prandtl", _d), ("fgamma", _i), ("ktrial", _i),
+                ("ToleranceX", _d), ("ToleranceF", _d), ("epsilon", _d)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "fvm_version": (_i, []),
@@ -133,6 +139,15 @@ SIGNATURES = {
     "fvm_phonon_advance": (_i, [_p, _i, _d, _d, _p, _p, _i, _p]),
     "fvm_phonon_get": (_i, [_p, _p, _p, _p]),
     "fvm_phonon_set_field": (_i, [_p, _p]),
+    "fvm_kinetic_assemble": (_i, [_p, _p, _p, _p, _p, _p, _p, _d, ctypes.POINTER(_p)]),
+    "fvm_quadrature_cartesian": (_i, [_i, _i, _i, _d, _d, _p, _p, _p, _p]),
+    "fvm_kinetic_default_options": (_i, [ctypes.POINTER(KineticOptions)]),
+    "fvm_kinetic_create": (_i, [ctypes.POINTER(_p), _p, _i, _p, _p, _p, _p, ctypes.POINTER(KineticOptions),
+                                _p, _p, _p]),
+    "fvm_kinetic_destroy": (_i, [_p]),
+    "fvm_kinetic_set_params": (_i, [_p, ctypes.POINTER(AmgParams)]),
+    "fvm_kinetic_advance": (_i, [_p, _i, _d, _d, _p, _p, _i, _p]),
+    "fvm_kinetic_get": (_i, [_p, _p, _p, _p, _p, _p, _p]),
 }
 
 _lib = None

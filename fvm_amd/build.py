@@ -11,7 +11,7 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libfvm_amd.so")
-SOURCES = ["setup.hip", "cycle.hip", "capi.hip", "assembly.hip", "mpnn.hip", "phonon.hip"]
+SOURCES = ["setup.hip", "cycle.hip", "capi.hip", "assembly.hip", "mpnn.hip", "phonon.hip", "kinetic.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

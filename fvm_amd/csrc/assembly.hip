@@ -726,6 +726,112 @@ __global__ void k_bc_phonon_ghost(int n_cells, int nct, const int *cf_row, const
     }
 }
 
+// ------------------------------------------------------------------ kinetic (ESBGK) directions
+// Collision term first (CollisionTermDiscretization.h:46-62: r -= V nu (f - feq),
+// diag -= V nu on self cells), then upwind convection with F = A . c over the
+// cell's faces in order (ConvectionDiscretization_Kmodel.h:171-197), as the
+// reference's Linearizer runs the two discretizations in that order.
+__global__ void k_cell_kinetic(int n_cells, int nct, const int *cf_row, const int *cf_col, const int *fcells,
+                               const int *pos01, const int *pos10, const int *row, const double *F,
+                               const double *vol, const double *nue, const double *feq, const double *x,
+                               double *diag, double *off, double *b) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nct; c += gridDim.x * blockDim.x) {
+        for (int p = row[c]; p < row[c + 1]; p++) off[p] = 0.0;
+        double d = 0.0, r = 0.0;
+        if (c < n_cells) {
+            r -= vol[c] * nue[c] * (x[c] - feq[c]);
+            d -= vol[c] * nue[c];
+        }
+        for (int q = cf_row[c]; q < cf_row[c + 1]; q++) {
+            const int f = cf_col[q];
+            const int c0 = fcells[2 * f], c1 = fcells[2 * f + 1];
+            const double Ff = F[f];
+            double flux;
+            if (Ff > 0.0) {
+                flux = Ff * x[c0];
+                if (c == c0) d -= Ff;
+                else off[pos10[f]] += Ff;
+            } else {
+                flux = Ff * x[c1];
+                if (c == c1) d += Ff;
+                else off[pos01[f]] -= Ff;
+            }
+            if (c == c0) r -= flux;
+            else r += flux;
+        }
+        diag[c] = d;
+        b[c] = r;
+    }
+}
+
+// incoming[f - first_bface] for every boundary face (KineticModel.h:1622-1661):
+// a zero-gradient group (kind 1) takes every direction as incoming; the wall-like
+// groups (kind 0) when c.en - uw.en < eps, en = A / |A|, uw the group's velocity.
+// grp = [gptr[0..ng], kind[0..ng-1]] as ints; wall_u: 3 per group.
+__global__ void k_kinetic_inflow(int first_bface, int nf, int ngroups, const int *grp, const double *wall_u,
+                                 const double *fA, const double *fAm, double c0, double c1, double c2, double eps,
+                                 uint8_t *incoming) {
+    for (int f = first_bface + blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
+        int g = 1;
+        while (g + 1 < ngroups && f >= grp[g + 1]) g++;
+        const int kind = grp[ngroups + 1 + g];
+        uint8_t in = 1;
+        if (kind != 1) {
+            const double e0 = fA[3 * f] / fAm[f], e1 = fA[3 * f + 1] / fAm[f], e2 = fA[3 * f + 2] / fAm[f];
+            const double cn = c0 * e0 + c1 * e1 + c2 * e2;
+            const double *uw = wall_u + 3 * g;
+            const double un = uw[0] * e0 + uw[1] * e1 + uw[2] * e2;
+            in = (cn - un < eps) ? 1 : 0;
+        }
+        incoming[f - first_bface] = in;
+    }
+}
+
+// boundary faces, interior side: Dirichlet with the ghost's own value for incoming
+// directions (GenericKineticBCS.h:76-116), extrapolation otherwise (:139-173);
+// faces in order per cell; reads the ghost diagonal before k_bc_kinetic_ghost resets it
+__global__ void k_bc_kinetic_interior(int n_cells, const int *cf_row, const int *cf_col, const int *fcells,
+                                      const int *pos01, int first_bface, const uint8_t *incoming, const double *x,
+                                      double *diag, double *off, double *b) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n_cells; c += gridDim.x * blockDim.x) {
+        for (int q = cf_row[c]; q < cf_row[c + 1]; q++) {
+            const int f = cf_col[q];
+            if (f < first_bface) continue;
+            const int c1 = fcells[2 * f + 1];
+            if (incoming[f - first_bface]) {
+                const double bval = x[c1];
+                const double dX = bval - x[c1];
+                b[c] += off[pos01[f]] * dX;
+                off[pos01[f]] = 0.0;
+            } else {
+                const double dFdX1 = -diag[c1];
+                const double dx01 = x[c] - x[c1];
+                diag[c] += dFdX1;
+                b[c] += dFdX1 * dx01;
+                off[pos01[f]] = 0.0;
+            }
+        }
+    }
+}
+
+__global__ void k_bc_kinetic_ghost(int n_cells, int nct, const int *cf_row, const int *cf_col, const int *fcells,
+                                   const int *pos10, int first_bface, const uint8_t *incoming, const double *x,
+                                   double *diag, double *off, double *b, uint8_t *isb) {
+    for (int gc = n_cells + blockIdx.x * blockDim.x + threadIdx.x; gc < nct; gc += gridDim.x * blockDim.x) {
+        const int f = cf_col[cf_row[gc]];
+        diag[gc] = -1.0;
+        if (incoming[f - first_bface]) {
+            off[pos10[f]] = 0.0;
+            b[gc] = 0.0;
+            isb[gc] = 0;
+        } else {
+            off[pos10[f]] = 1.0;
+            b[gc] = x[fcells[2 * f]] - x[gc];
+            isb[gc] = 1;
+        }
+    }
+}
+
 // ------------------------------------------------------------------ fields
 __global__ void k_lognormal(int N, int n_cells, int block, double sigma, unsigned long long seed,
                             double *k) {
@@ -1306,6 +1412,74 @@ int fvm_phonon_assemble(fvm_mesh *m, const double *vg, double tau, const double 
         fvm::k_eliminate<<<fvm::grid_for(nc, 256), 256, 0, s>>>(nc, m->cf_row.p, m->cf_col.p, m->fcells.p,
                                                                 m->pos01.p, m->pos10.p, S->isb.p, S->diag.p,
                                                                 S->off.p, S->b.p);
+        FVM_HIP(hipGetLastError());
+        FVM_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        delete S;
+        throw;
+    }
+    *out = S;
+    return FVM_OK;
+    FVM_API_END
+}
+
+int fvm_kinetic_assemble(fvm_mesh *m, const double *c, const double *d_nue, const double *d_feq, const double *d_x,
+                         const int32_t *bc_kind, const double *wall_u, double eps, fvm_system **out) {
+    FVM_API_BEGIN
+    FVM_CHECK(m && c && d_nue && d_feq && d_x && out, FVM_E_INVALID, "null argument");
+    FVM_CHECK(std::isfinite(c[0]) && std::isfinite(c[1]) && std::isfinite(c[2]), FVM_E_INVALID,
+              "the direction's velocity must be finite");
+    fvm::Dev d(m->device);
+    fvm::mesh_metrics(m);
+    hipStream_t s = m->stream;
+    const int nc = m->n_cells, nct = m->n_cells + m->n_ghost, nf = m->n_faces;
+    const int ng = m->n_groups;
+    FVM_CHECK(ng <= 1 || (bc_kind && wall_u), FVM_E_INVALID, "boundary groups need a kind and a wall velocity");
+    for (int g = 1; g < ng; g++)
+        FVM_CHECK(bc_kind[g] == 0 || bc_kind[g] == 1, FVM_E_INVALID, "kinetic boundary kind must be 0 or 1");
+    fvm_system *S = new fvm_system();
+    try {
+        S->device = m->device;
+        S->n_self = nc;
+        S->n_total = nct;
+        S->nnz = m->nnz;
+        S->row.alloc((size_t)nct + 1);
+        S->col.alloc(std::max<long long>(m->nnz, 1));
+        S->off.alloc(std::max<long long>(m->nnz, 1));
+        S->diag.alloc(nct); S->b.alloc(nct); S->isb.alloc(nct);
+        FVM_HIP(hipMemcpyAsync(S->row.p, m->cc_row.p, S->row.bytes(), hipMemcpyDeviceToDevice, s));
+        FVM_HIP(hipMemcpyAsync(S->col.p, m->cc_col.p, sizeof(int) * m->nnz, hipMemcpyDeviceToDevice, s));
+        DArr<double> F(std::max(nf, 1));
+        fvm::k_face_conv<<<fvm::grid_for(nf, 256), 256, 0, s>>>(nf, m->fA.p, c[0], c[1], c[2], F.p);
+        fvm::k_cell_kinetic<<<fvm::grid_for(nct, 256), 256, 0, s>>>(
+            nc, nct, m->cf_row.p, m->cf_col.p, m->fcells.p, m->pos01.p, m->pos10.p, m->cc_row.p, F.p, m->vol.p,
+            d_nue, d_feq, d_x, S->diag.p, S->off.p, S->b.p);
+        FVM_HIP(hipMemsetAsync(S->isb.p, 0, nct, s));
+        const int first_bface = m->gptr.size() > 1 ? m->gptr[1] : nf;
+        if (nf > first_bface) {
+            std::vector<int> grp((size_t)2 * ng + 1, 0);
+            for (int g = 0; g <= ng; g++) grp[g] = m->gptr[g];
+            for (int g = 1; g < ng; g++) grp[ng + 1 + g] = bc_kind[g];
+            DArr<int> d_grp(grp.size());
+            DArr<double> d_wall(3 * (size_t)ng);
+            DArr<uint8_t> inc((size_t)(nf - first_bface));
+            FVM_HIP(hipMemcpyAsync(d_grp.p, grp.data(), sizeof(int) * grp.size(), hipMemcpyHostToDevice, s));
+            FVM_HIP(hipMemcpyAsync(d_wall.p, wall_u, sizeof(double) * 3 * ng, hipMemcpyHostToDevice, s));
+            fvm::k_kinetic_inflow<<<fvm::grid_for(nf - first_bface, 256), 256, 0, s>>>(
+                first_bface, nf, ng, d_grp.p, d_wall.p, m->fA.p, m->fAm.p, c[0], c[1], c[2], eps, inc.p);
+            fvm::k_bc_kinetic_interior<<<fvm::grid_for(nc, 256), 256, 0, s>>>(
+                nc, m->cf_row.p, m->cf_col.p, m->fcells.p, m->pos01.p, first_bface, inc.p, d_x, S->diag.p,
+                S->off.p, S->b.p);
+            if (m->n_ghost)
+                fvm::k_bc_kinetic_ghost<<<fvm::grid_for(m->n_ghost, 256), 256, 0, s>>>(
+                    nc, nct, m->cf_row.p, m->cf_col.p, m->fcells.p, m->pos10.p, first_bface, inc.p, d_x,
+                    S->diag.p, S->off.p, S->b.p, S->isb.p);
+            fvm::k_eliminate<<<fvm::grid_for(nc, 256), 256, 0, s>>>(nc, m->cf_row.p, m->cf_col.p, m->fcells.p,
+                                                                    m->pos01.p, m->pos10.p, S->isb.p, S->diag.p,
+                                                                    S->off.p, S->b.p);
+            FVM_HIP(hipGetLastError());
+            FVM_HIP(hipStreamSynchronize(s));   // the small uploads above are freed on return
+        }
         FVM_HIP(hipGetLastError());
         FVM_HIP(hipStreamSynchronize(s));
     } catch (...) {

@@ -515,6 +515,72 @@ int fvm_phonon_advance(fvm_phonon *p, int niter, double rel_tol, double abs_tol,
 int fvm_phonon_get(const fvm_phonon *p, double *TL, double *e, double *e0);
 /* replace every direction's field (host, K blocks of n_cells + n_ghost) */
 int fvm_phonon_set_field(fvm_phonon *p, const double *e);
+/* One discrete velocity of the BGK kinetic model (KineticModel::
+ * linearizeKineticModel, esbgkbase/KineticModel.h:1505-1674): the collision term
+ * -V nu (f - feq) (CollisionTermDiscretization.h:46-62), then upwind convection
+ * with face flux F = A . c (ConvectionDiscretization_Kmodel.h:171-197), then per
+ * boundary face Dirichlet with the ghost's current value for incoming directions
+ * and extrapolation otherwise (GenericKineticBCS.h:76-173): bc_kind[g] = 1 (zero
+ * gradient) takes every direction as incoming, 0 (diffuse and specular walls)
+ * when c.en - uw.en < eps with uw = wall_u[3g..3g+2] (KineticModel.h:1622-1661);
+ * then initSolve's elimination of the extrapolated ghost rows. c: host double[3];
+ * d_nue: device, n_cells entries; d_feq: device, n_cells; d_x: device, n_cells +
+ * n_ghost (ghosts hold the boundary values); bc_kind, wall_u: host, per group
+ * (entry 0 unused). */
+int fvm_kinetic_assemble(fvm_mesh *m, const double *c, const double *d_nue, const double *d_feq,
+                         const double *d_x, const int32_t *bc_kind, const double *wall_u, double eps,
+                         fvm_system **out);
+
+/* ---------------------------------------------------------------------------
+ * BGK kinetic model (KineticModel, esbgkbase/KineticModel.h) on one mesh, on a
+ * discrete velocity quadrature.
+ * ------------------------------------------------------------------------- */
+/* Quadrature(N1, N2, N3, clim, T2), Cartesian type (Quadrature.h:83-181): N1 N2 N3
+ * velocities on the uniform grid [-clim sqrt(T2/2), clim sqrt(T2/2)]^3, x slowest,
+ * z fastest; w the product of the spacings. Host arrays of N1 N2 N3 entries. */
+int fvm_quadrature_cartesian(int N1, int N2, int N3, double clim, double T2, double *cx, double *cy,
+                             double *cz, double *w);
+/* KineticModelOptions (esbgkbase/KineticBC.h:61-113) used by the loop */
+typedef struct fvm_kinetic_options {
+    double rho_init, T_init, nonDimLt, molecularWeight, Tmuref, muref, mu_w, Prandtl;
+    int fgamma;          /* 0: Maxwellian of the stored coefficients, 1: BGK (Newton fit); 2 is refused */
+    int ktrial;          /* NewtonsMethod_ktrial */
+    double ToleranceX, ToleranceF;
+    double epsilon;      /* epsilon_ES: incoming test c.n - uw.n < epsilon */
+} fvm_kinetic_options;
+/* the reference's defaults, except fgamma = 1 (the reference defaults to 2) */
+int fvm_kinetic_default_options(fvm_kinetic_options *o);
+typedef struct fvm_kinetic fvm_kinetic;
+/* K velocities (cx, cy, cz, w: host); per face group g >= 1 (entry 0 unused):
+ * bc_kind[g] 0 diffuse wall at velocity wall_u[3g..3g+2] and temperature
+ * wall_T[g] ("wall", KineticBoundaryConditions.h:65-140), 1 zero gradient
+ * ("ZeroGradBC", :385-406), 2 specular symmetry ("symmetry", :287-330, reflection
+ * table from the group's first face, KineticModel.h:290-318). The constructor's
+ * state (KineticModel.h:81-120): density 1, velocity 0, temperature 1, f the
+ * Maxwellian of it, then its moments, collision frequency and equilibrium. The
+ * mesh is borrowed. */
+int fvm_kinetic_create(fvm_kinetic **out, fvm_mesh *mesh, int K, const double *cx, const double *cy,
+                       const double *cz, const double *w, const fvm_kinetic_options *opt,
+                       const int32_t *bc_kind, const double *wall_u, const double *wall_T);
+int fvm_kinetic_destroy(fvm_kinetic *p);
+/* parameters of the per-direction AMG (the kinetic linear solver); FVM_PREC_FP64 */
+int fvm_kinetic_set_params(fvm_kinetic *p, const fvm_amg_params *prm);
+/* KineticModel::advance (KineticModel.h:3360-3462): up to niter outer iterations;
+ * each assembles every direction (fvm_kinetic_assemble), solves them together with
+ * one batched AMG, sums the directions' initial 1-norms in direction order,
+ * postSolve + update, then the boundary values, moments, collision frequency and
+ * equilibrium, then stops if the sum is below abs_tol or its ratio to the initial
+ * norm (the largest of the first five iterations') below rel_tol. norms (may be
+ * NULL when cap is 0): the iterations' norms; *converged: 1 if the stop test ended
+ * the call. */
+int fvm_kinetic_advance(fvm_kinetic *p, int niter, double rel_tol, double abs_tol, int *iterations,
+                        double *norms, int cap, int *converged);
+/* host copies (any may be NULL): f and feq (K blocks of n_cells + n_ghost,
+ * direction-major), density, temperature, collision frequency (n_cells + n_ghost),
+ * velocity (3 blocks of n_cells + n_ghost: x, y, z) */
+int fvm_kinetic_get(const fvm_kinetic *p, double *f, double *feq, double *density, double *velocity,
+                    double *temperature, double *nu);
+
 /* fvm_amg_setup_device on an assembled system */
 int fvm_amg_setup_system(fvm_amg *h, const fvm_system *S);
 

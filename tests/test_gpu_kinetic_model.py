@@ -1,0 +1,118 @@
+"""BGK kinetic model on the device (fvm_kinetic_*: every velocity of an iteration
+assembled on the device and solved by one batched AMG; moments, collision
+frequency, Newton-fitted BGK equilibrium and wall values as device kernels)
+against the oracle's loop (oracle/kinetic_ref.py with the C oracle's assembly and
+AMG, the directions one after another).
+
+Bar: the per-direction assembly is the oracle's bit for bit on the same inputs.
+The loop agrees within 1e-9 relative: exp and pow of the device's math library
+and of the host's may differ in the last bit, so the equilibria, and with them
+every later system, differ at rounding level; the stop rule and iteration count
+are the same."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import hexmesh
+import oracle_lib as O
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import kinetic_ref as KR  # noqa: E402
+from test_kinetic_ref import kinetic_parts, reflections  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+BCS = {1: ("wall", np.array([0.0, -0.15, 0.0]), 1.0), 2: ("wall", np.array([0.0, 0.15, 0.05]), 1.2),
+       3: ("symmetry",), 4: ("symmetry",), 5: ("zerograd",), 6: ("zerograd",)}
+
+
+def _dev(a):
+    import torch
+    return torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda:0")
+
+
+def test_kinetic_direction_assembly_matches_oracle():
+    from fvm_amd.kinetic import assemble_direction
+    from fvm_amd.thermal import Mesh
+    N, jit, seed = 5, 0.2, 8
+    mesh = hexmesh.hex_mesh(N, jit, seed)
+    dm = Mesh.hex(N, jit, seed)
+    nc, ng = mesh["n_cells"], mesh["n_ghost"]
+    rng = np.random.default_rng(1)
+    nue = rng.uniform(0.2, 3.0, nc)
+    feq = rng.uniform(0.0, 1.0, nc)
+    x = rng.uniform(0.0, 1.0, nc + ng)
+    kind = np.zeros(7, np.int32); kind[5] = kind[6] = 1
+    wall_u = np.zeros((7, 3)); wall_u[1] = BCS[1][1]; wall_u[2] = BCS[2][1]
+    quad = KR.quadrature_cartesian(5, 5, 5, 4.0, 1.0)   # odd: some velocities lie in the faces' planes
+    for j in (0, 31, 62, 112, 124):
+        c = [quad[0][j], quad[1][j], quad[2][j]]
+        want = O.fv_kinetic(mesh, c, nue, feq, x, kind, wall_u)
+        S = assemble_direction(dm, c, _dev(nue), _dev(feq), _dev(x), kind, wall_u)
+        got = S.to_host()
+        for k in ("row", "col", "diag", "off", "b", "is_boundary"):
+            assert np.array_equal(got[k], want[k]), (j, k)
+
+
+def _run(fgamma, niter, N=3, jit=0.1, seed=6, q=(4, 4, 4), bcs=BCS, **kw):
+    from fvm_amd.kinetic import KineticModel, Quadrature
+    from fvm_amd.thermal import Mesh
+    mesh = hexmesh.hex_mesh(N, jit, seed)
+    nct = mesh["n_cells"] + mesh["n_ghost"]
+    g = O.fv_metrics(mesh)
+    quad = KR.quadrature_cartesian(*q, 4.0, 1.0)
+    o = dict(KR.DEFAULTS); o.update(fgamma=fgamma, rho_init=1e-4, **kw)
+    st = KR.init_state(nct, quad, o)
+    assemble, solve = kinetic_parts(mesh, quad, bcs)
+    norms_o, conv_o = KR.kinetic_advance(assemble, solve, mesh, g, quad, bcs, reflections(mesh, g, quad, bcs), o,
+                                         st, niter, 1e-30, 0.0)
+    model = KineticModel(Mesh.hex(N, jit, seed), Quadrature(*q, 4.0, 1.0), bcs, fgamma=fgamma, rho_init=1e-4, **kw)
+    model.set_solver(gsOrdering=0, relativeTolerance=1e-13, nMaxIterations=400)
+    conv = model.advance(niter, relTolerance=1e-30, absTolerance=0.0)
+    return mesh, st, norms_o, conv_o, model, conv
+
+
+@pytest.mark.parametrize("fgamma", [0, 1])
+def test_kinetic_model_matches_oracle_loop(fgamma):
+    mesh, st, norms_o, conv_o, model, conv = _run(fgamma, 4)
+    print("fgamma", fgamma, "norms", model.norms, flush=True)
+    assert conv == conv_o and len(model.norms) == len(norms_o) == 4
+    # the first iteration starts from the rest state, in equilibrium with itself
+    # (the walls act from the first update on): its norm is rounding noise
+    assert abs(model.norms[0] - norms_o[0]) <= 1e-12 * norms_o[1]
+    np.testing.assert_allclose(model.norms[1:], norms_o[1:], rtol=1e-9)
+    s = model.state()
+    scale = np.abs(st["f"]).max()
+    np.testing.assert_allclose(s["f"], st["f"], rtol=1e-9, atol=1e-12 * scale)
+    np.testing.assert_allclose(s["feq"], st["feq"], rtol=1e-9, atol=1e-12 * scale)
+    np.testing.assert_allclose(s["density"], st["density"], rtol=1e-10)
+    np.testing.assert_allclose(s["temperature"], st["T"], rtol=1e-10)
+    np.testing.assert_allclose(s["velocity"], st["v"], rtol=1e-8, atol=1e-12)
+    np.testing.assert_allclose(s["nu"], st["nu"], rtol=1e-10)
+
+
+def test_kinetic_rest_state_is_a_fixed_point_on_the_device():
+    from fvm_amd.kinetic import KineticModel, Quadrature
+    from fvm_amd.thermal import Mesh
+    bcs = {g: ("wall", np.zeros(3), 1.0) for g in range(1, 7)}
+    model = KineticModel(Mesh.hex(4, 0.15, 3), Quadrature(4, 4, 4, 4.0, 1.0), bcs)
+    s0 = model.state()
+    model.advance(2)
+    s1 = model.state()
+    assert max(model.norms) <= 1e-12 * np.abs(s0["f"]).sum() * s0["nu"].max()
+    np.testing.assert_allclose(s1["f"], s0["f"], rtol=1e-12, atol=1e-15)
+
+
+def test_kinetic_couette_on_the_device():
+    """Walls at x = 0, 1 moving in -y / +y, symmetry elsewhere: the half-turn about
+    the slab's axis reverses the flow; the device loop follows the oracle's."""
+    bcs = {1: ("wall", np.array([0.0, -0.2, 0.0]), 1.0), 2: ("wall", np.array([0.0, 0.2, 0.0]), 1.0),
+           3: ("symmetry",), 4: ("symmetry",), 5: ("symmetry",), 6: ("symmetry",)}
+    N = 4
+    mesh, st, norms_o, conv_o, model, conv = _run(1, 6, N=N, jit=0.0, seed=0, q=(5, 5, 5), bcs=bcs)
+    np.testing.assert_allclose(model.norms[1:], norms_o[1:], rtol=1e-9)
+    vy = model.state()["velocity"][: mesh["n_cells"], 1].reshape(N, N, N)
+    np.testing.assert_allclose(vy[:, ::-1, ::-1], -vy, atol=1e-11)
+    np.testing.assert_allclose(vy, st["v"][: mesh["n_cells"], 1].reshape(N, N, N), rtol=1e-8, atol=1e-13)
