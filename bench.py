@@ -350,6 +350,99 @@ def apply_knobs(amg, args):
         amg.set_tuning(key, getattr(args, key))
 
 
+def run_kinetic(args):
+    """BGK kinetic model (KineticModel::advance): Couette slab on an N^3 hex mesh,
+    walls at x = 0 and x = 1 moving in -y / +y at temperature 1, specular symmetry on
+    the other faces, Q^3 Cartesian velocities (clim 5.5, T2 1), fgamma 1, the
+    kinetic linear solver's defaults (AMG to 1e-15 relative, at most 100 cycles per
+    direction). A step is one outer iteration: every velocity assembled, all solved
+    by one batched AMG, then walls, moments, collision frequency and the BGK Newton
+    equilibrium. CPU baseline: the oracle's assembly + AMG for a sample of
+    directions, scaled to all of them, plus its per-cell update once."""
+    import torch
+    import fvm_amd  # noqa: F401
+    from fvm_amd.kinetic import KineticModel, Quadrature
+    from fvm_amd.thermal import Mesh
+    N, Q = args.kinetic, args.kinetic_q
+    mesh = Mesh.hex(N, 0.0, SEED)
+    quad = Quadrature(Q, Q, Q, 5.5, 1.0)
+    K = quad.getDirCount()
+    bcs = {1: ("wall", (0.0, -0.2, 0.0), 1.0), 2: ("wall", (0.0, 0.2, 0.0), 1.0),
+           3: ("symmetry",), 4: ("symmetry",), 5: ("symmetry",), 6: ("symmetry",)}
+    model = KineticModel(mesh, quad, bcs, fgamma=1)
+    log("kinetic: %d^3 cells, %d velocities (%d unknowns per iteration)" % (N, K, K * N ** 3))
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        model.advance(1, relTolerance=0.0, absTolerance=0.0)
+        log("warmup %d: %.3f s, norm %.4e" % (w, time.perf_counter() - t0, model.norms[-1]))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    model.advance(args.steps, relTolerance=0.0, absTolerance=0.0)
+    torch.cuda.synchronize()
+    per = (time.perf_counter() - t0) / args.steps
+    log("kinetic: %.3f s per outer iteration, norms %s" % (per, ["%.4e" % v for v in model.norms]))
+    cpu = None
+    if not args.skip_cpu:
+        cpu = kinetic_cpu_baseline(N, Q, K, bcs)
+        log("cpu: %.1f s per iteration (%s)" % (cpu["value"], cpu["sample"]))
+    out = {
+        "metric": "time per outer iteration, BGK kinetic model (all velocities assembled and solved)",
+        "value": round(per, 6), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * per, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (Couette slab, rest-state Maxwellian start)",
+        "config": {"workload": "%d^3 hex Couette slab, %d^3 Cartesian velocities (%d systems, %d unknowns), BGK, "
+                               "AMG per direction to 1e-15 / 100 cycles, batched" % (N, Q, K, K * N ** 3)},
+        "norms": model.norms, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def kinetic_cpu_baseline(N, Q, K, bcs, sample=3):
+    """The oracle on the same case: direction assembly + AMG (to 1e-15, 100 cycles)
+    for `sample` directions timed and scaled to K, plus one per-cell update (walls,
+    moments, collision frequency, Newton BGK over every cell) timed once."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import hexmesh
+    import kinetic_ref as KR
+    import oracle_lib as O
+    mesh = hexmesh.hex_mesh(N, 0.0, SEED)
+    nc = mesh["n_cells"]
+    nct = nc + mesh["n_ghost"]
+    g = O.fv_metrics(mesh)
+    quad = KR.quadrature_cartesian(Q, Q, Q, 5.5, 1.0)
+    o = dict(KR.DEFAULTS)
+    st = KR.init_state(nct, quad, o)
+    kind = np.zeros(7, np.int32)
+    wall_u = np.zeros((7, 3)); wall_u[1] = bcs[1][1]; wall_u[2] = bcs[2][1]
+    picks = np.linspace(0, K - 1, sample).astype(int)
+    t0 = time.perf_counter()
+    for j in picks:
+        S = O.fv_kinetic(mesh, [quad[0][j], quad[1][j], quad[2][j]], st["nu"][:nc], st["feq"][j][:nc], st["f"][j],
+                         kind, wall_u)
+        oa = O.OracleAMG(n_max_iterations=100, relative_tolerance=1e-15)
+        oa.setup(S["n_self"], S["n_total"], S["row"], S["col"], S["diag"], S["off"], S["is_boundary"])
+        oa.solve(S["b"])
+    per_dir = (time.perf_counter() - t0) / sample
+    refl = {}
+    gp = mesh["group_ptr"]
+    for gi in (3, 4, 5, 6):
+        f0 = gp[gi]
+        refl[gi] = KR.reflection_map(g["fA"][3 * f0:3 * f0 + 3] / g["fAm"][f0], *quad[:3])
+    kb = {gi: ("wall", np.array(bc[1]), bc[2]) if bc[0] == "wall" else bc for gi, bc in bcs.items()}
+    t1 = time.perf_counter()
+    macro = (st["density"], st["v"], st["T"])
+    KR.boundary_values(mesh, g["fA"], g["fAm"], kb, refl, st["f"], macro, *quad, o["epsilon"])
+    d, v, T, _ = KR.macroparameters(st["f"], *quad)
+    KR.collision_frequency(d, T, o)
+    KR.newton_bgk(d, v, T, st["coeff"], *quad, o)
+    upd = time.perf_counter() - t1
+    return {"value": per_dir * K + upd, "unit": "s", "cores": 1, "kind": "port",
+            "sample": "oracle assembly + AMG of %d of the %d directions (%.3f s each), scaled, + one oracle "
+                      "update of all cells (%.2f s)" % (sample, K, per_dir, upd)}
+
+
 def run_profile(args):
     """Short program for rocprofv3 (kernel trace and PMC passes): the 8M-cell
     system, `--steps` headline solves (setup, FP64 FGMRES(30) with the mixed
@@ -655,6 +748,9 @@ def main():
     ap.add_argument("--train-cycles", default="1,2",
                     help="plain AMG cycles before a training pair is taken (cycled through)")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--kinetic", type=int, default=0,
+                    help="instead of the 8M-cell solve: time outer iterations of the BGK kinetic model on an N^3 slab")
+    ap.add_argument("--kinetic-q", type=int, default=10, help="--kinetic: velocities per axis")
     ap.add_argument("--batch", type=int, default=0,
                     help="instead of the 8M-cell solve: time a sweep of this many phonon direction systems")
     args = ap.parse_args()
@@ -665,6 +761,9 @@ def main():
     if args.traffic:
         write_traffic_profile(args.traffic[0], args.traffic[1], args.n,
                               os.path.join(ROOT, "profiles", "hbm_traffic.json"))
+        return
+    if args.kinetic:
+        run_kinetic(args)
         return
     if args.batch:
         run_batch(args)
