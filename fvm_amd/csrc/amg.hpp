@@ -222,6 +222,12 @@ std::string autotune(fvm_amg *h, int reps);
 // device (metrics computed if needed)
 int mesh_device(const fvm_mesh *m);
 void mesh_device_metrics(fvm_mesh *m, const int **fcells, const double **fA, const double **fAm);
+hipStream_t mesh_stream(fvm_mesh *m);
+fvm_system *system_for_mesh(fvm_mesh *m);
+void kinetic_assemble_into(fvm_mesh *m, const double *c, const double *d_nue, const double *d_feq,
+                           const double *d_x, const int *grp, const double *wall, double eps, fvm_system *S,
+                           double *F, uint8_t *inc);
+void system_post_solve_async(const fvm_system *S, double *d_delta, hipStream_t s);
 // learned correction (mpnn.hip)
 void mpnn_bind(fvm_mpnn *m, int n, const int *row, const int *col, const double *off, const double *diag);
 void mpnn_correct(fvm_mpnn *m, hipStream_t st, const int *perm, const double *Az, const double *v, double *z,

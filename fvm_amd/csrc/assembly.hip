@@ -1091,6 +1091,77 @@ void mesh_device_metrics(fvm_mesh *m, const int **fcells, const double **fA, con
     *fA = m->fA.p;
     *fAm = m->fAm.p;
 }
+
+hipStream_t mesh_stream(fvm_mesh *m) { return m->stream; }
+
+// A system with the mesh's cell-cell pattern (row and column arrays copied on the
+// mesh's stream), values unset; the caller owns it (fvm_system_destroy).
+fvm_system *system_for_mesh(fvm_mesh *m) {
+    Dev d(m->device);
+    mesh_metrics(m);
+    const int nct = m->n_cells + m->n_ghost;
+    fvm_system *S = new fvm_system();
+    try {
+        S->device = m->device;
+        S->n_self = m->n_cells;
+        S->n_total = nct;
+        S->nnz = m->nnz;
+        S->row.alloc((size_t)nct + 1);
+        S->col.alloc(std::max<long long>(m->nnz, 1));
+        S->off.alloc(std::max<long long>(m->nnz, 1));
+        S->diag.alloc(std::max(nct, 1)); S->b.alloc(std::max(nct, 1)); S->isb.alloc(std::max(nct, 1));
+        FVM_HIP(hipMemcpyAsync(S->row.p, m->cc_row.p, S->row.bytes(), hipMemcpyDeviceToDevice, m->stream));
+        if (m->nnz)
+            FVM_HIP(hipMemcpyAsync(S->col.p, m->cc_col.p, sizeof(int) * m->nnz, hipMemcpyDeviceToDevice, m->stream));
+    } catch (...) {
+        delete S;
+        throw;
+    }
+    return S;
+}
+
+// One kinetic velocity's system into S (from system_for_mesh on this mesh), all on
+// the mesh's stream with no synchronisation: grp = [gptr (n_groups + 1), kind
+// (n_groups)] and wall (3 per group) on the device; F (n_faces) and inc (n_faces)
+// are scratch that the next call on the same stream may reuse.
+void kinetic_assemble_into(fvm_mesh *m, const double *c, const double *d_nue, const double *d_feq,
+                           const double *d_x, const int *grp, const double *wall, double eps, fvm_system *S,
+                           double *F, uint8_t *inc) {
+    hipStream_t s = m->stream;
+    const int nc = m->n_cells, nct = m->n_cells + m->n_ghost, nf = m->n_faces;
+    k_face_conv<<<grid_for(nf, 256), 256, 0, s>>>(nf, m->fA.p, c[0], c[1], c[2], F);
+    k_cell_kinetic<<<grid_for(nct, 256), 256, 0, s>>>(nc, nct, m->cf_row.p, m->cf_col.p, m->fcells.p, m->pos01.p,
+                                                      m->pos10.p, m->cc_row.p, F, m->vol.p, d_nue, d_feq, d_x,
+                                                      S->diag.p, S->off.p, S->b.p);
+    FVM_HIP(hipMemsetAsync(S->isb.p, 0, nct, s));
+    const int first_bface = m->gptr.size() > 1 ? m->gptr[1] : nf;
+    if (nf > first_bface) {
+        k_kinetic_inflow<<<grid_for(nf - first_bface, 256), 256, 0, s>>>(first_bface, nf, m->n_groups, grp, wall,
+                                                                         m->fA.p, m->fAm.p, c[0], c[1], c[2], eps,
+                                                                         inc);
+        k_bc_kinetic_interior<<<grid_for(nc, 256), 256, 0, s>>>(nc, m->cf_row.p, m->cf_col.p, m->fcells.p,
+                                                                m->pos01.p, first_bface, inc, d_x, S->diag.p,
+                                                                S->off.p, S->b.p);
+        if (m->n_ghost)
+            k_bc_kinetic_ghost<<<grid_for(m->n_ghost, 256), 256, 0, s>>>(nc, nct, m->cf_row.p, m->cf_col.p,
+                                                                         m->fcells.p, m->pos10.p, first_bface, inc,
+                                                                         d_x, S->diag.p, S->off.p, S->b.p, S->isb.p);
+        k_eliminate<<<grid_for(nc, 256), 256, 0, s>>>(nc, m->cf_row.p, m->cf_col.p, m->fcells.p, m->pos01.p,
+                                                      m->pos10.p, S->isb.p, S->diag.p, S->off.p, S->b.p);
+    }
+    FVM_HIP(hipGetLastError());
+}
+
+// postSolve of a system assembled on a mesh, on stream s without synchronising:
+// a mesh's ghost row holds only its face's interior cell, so no boundary row reads
+// another one and the rows are independent (CRMatrix.h:443-453 in any order)
+void system_post_solve_async(const fvm_system *S, double *d_delta, hipStream_t s) {
+    const int ng = S->n_total - S->n_self;
+    if (ng <= 0) return;
+    k_solve_boundary<false><<<grid_for(ng, 256), 256, 0, s>>>(S->n_self, S->n_total, S->row.p, S->col.p, S->diag.p,
+                                                              S->off.p, S->isb.p, S->b.p, d_delta);
+    FVM_HIP(hipGetLastError());
+}
 }  // namespace fvm
 
 #define FVM_API_BEGIN try {
@@ -1429,59 +1500,25 @@ int fvm_kinetic_assemble(fvm_mesh *m, const double *c, const double *d_nue, cons
     FVM_CHECK(m && c && d_nue && d_feq && d_x && out, FVM_E_INVALID, "null argument");
     FVM_CHECK(std::isfinite(c[0]) && std::isfinite(c[1]) && std::isfinite(c[2]), FVM_E_INVALID,
               "the direction's velocity must be finite");
-    fvm::Dev d(m->device);
-    fvm::mesh_metrics(m);
-    hipStream_t s = m->stream;
-    const int nc = m->n_cells, nct = m->n_cells + m->n_ghost, nf = m->n_faces;
     const int ng = m->n_groups;
     FVM_CHECK(ng <= 1 || (bc_kind && wall_u), FVM_E_INVALID, "boundary groups need a kind and a wall velocity");
     for (int g = 1; g < ng; g++)
         FVM_CHECK(bc_kind[g] == 0 || bc_kind[g] == 1, FVM_E_INVALID, "kinetic boundary kind must be 0 or 1");
-    fvm_system *S = new fvm_system();
+    fvm::Dev d(m->device);
+    fvm_system *S = fvm::system_for_mesh(m);
     try {
-        S->device = m->device;
-        S->n_self = nc;
-        S->n_total = nct;
-        S->nnz = m->nnz;
-        S->row.alloc((size_t)nct + 1);
-        S->col.alloc(std::max<long long>(m->nnz, 1));
-        S->off.alloc(std::max<long long>(m->nnz, 1));
-        S->diag.alloc(nct); S->b.alloc(nct); S->isb.alloc(nct);
-        FVM_HIP(hipMemcpyAsync(S->row.p, m->cc_row.p, S->row.bytes(), hipMemcpyDeviceToDevice, s));
-        FVM_HIP(hipMemcpyAsync(S->col.p, m->cc_col.p, sizeof(int) * m->nnz, hipMemcpyDeviceToDevice, s));
-        DArr<double> F(std::max(nf, 1));
-        fvm::k_face_conv<<<fvm::grid_for(nf, 256), 256, 0, s>>>(nf, m->fA.p, c[0], c[1], c[2], F.p);
-        fvm::k_cell_kinetic<<<fvm::grid_for(nct, 256), 256, 0, s>>>(
-            nc, nct, m->cf_row.p, m->cf_col.p, m->fcells.p, m->pos01.p, m->pos10.p, m->cc_row.p, F.p, m->vol.p,
-            d_nue, d_feq, d_x, S->diag.p, S->off.p, S->b.p);
-        FVM_HIP(hipMemsetAsync(S->isb.p, 0, nct, s));
-        const int first_bface = m->gptr.size() > 1 ? m->gptr[1] : nf;
-        if (nf > first_bface) {
-            std::vector<int> grp((size_t)2 * ng + 1, 0);
-            for (int g = 0; g <= ng; g++) grp[g] = m->gptr[g];
-            for (int g = 1; g < ng; g++) grp[ng + 1 + g] = bc_kind[g];
-            DArr<int> d_grp(grp.size());
-            DArr<double> d_wall(3 * (size_t)ng);
-            DArr<uint8_t> inc((size_t)(nf - first_bface));
-            FVM_HIP(hipMemcpyAsync(d_grp.p, grp.data(), sizeof(int) * grp.size(), hipMemcpyHostToDevice, s));
-            FVM_HIP(hipMemcpyAsync(d_wall.p, wall_u, sizeof(double) * 3 * ng, hipMemcpyHostToDevice, s));
-            fvm::k_kinetic_inflow<<<fvm::grid_for(nf - first_bface, 256), 256, 0, s>>>(
-                first_bface, nf, ng, d_grp.p, d_wall.p, m->fA.p, m->fAm.p, c[0], c[1], c[2], eps, inc.p);
-            fvm::k_bc_kinetic_interior<<<fvm::grid_for(nc, 256), 256, 0, s>>>(
-                nc, m->cf_row.p, m->cf_col.p, m->fcells.p, m->pos01.p, first_bface, inc.p, d_x, S->diag.p,
-                S->off.p, S->b.p);
-            if (m->n_ghost)
-                fvm::k_bc_kinetic_ghost<<<fvm::grid_for(m->n_ghost, 256), 256, 0, s>>>(
-                    nc, nct, m->cf_row.p, m->cf_col.p, m->fcells.p, m->pos10.p, first_bface, inc.p, d_x,
-                    S->diag.p, S->off.p, S->b.p, S->isb.p);
-            fvm::k_eliminate<<<fvm::grid_for(nc, 256), 256, 0, s>>>(nc, m->cf_row.p, m->cf_col.p, m->fcells.p,
-                                                                    m->pos01.p, m->pos10.p, S->isb.p, S->diag.p,
-                                                                    S->off.p, S->b.p);
-            FVM_HIP(hipGetLastError());
-            FVM_HIP(hipStreamSynchronize(s));   // the small uploads above are freed on return
-        }
-        FVM_HIP(hipGetLastError());
-        FVM_HIP(hipStreamSynchronize(s));
+        std::vector<int> grp((size_t)2 * ng + 1, 0);
+        for (int g = 0; g <= ng; g++) grp[g] = m->gptr[g];
+        for (int g = 1; g < ng; g++) grp[ng + 1 + g] = bc_kind[g];
+        DArr<int> d_grp(grp.size());
+        DArr<double> d_wall(3 * (size_t)std::max(ng, 1));
+        DArr<double> F(std::max(m->n_faces, 1));
+        DArr<uint8_t> inc(std::max(m->n_faces, 1));
+        FVM_HIP(hipMemcpyAsync(d_grp.p, grp.data(), sizeof(int) * grp.size(), hipMemcpyHostToDevice, m->stream));
+        if (ng > 1)
+            FVM_HIP(hipMemcpyAsync(d_wall.p, wall_u, sizeof(double) * 3 * ng, hipMemcpyHostToDevice, m->stream));
+        fvm::kinetic_assemble_into(m, c, d_nue, d_feq, d_x, d_grp.p, d_wall.p, eps, S, F.p, inc.p);
+        FVM_HIP(hipStreamSynchronize(m->stream));   // the uploads and scratch above are freed on return
     } catch (...) {
         delete S;
         throw;

@@ -22,7 +22,10 @@
 // operation order per cell and per face.
 #include "amg.hpp"
 
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 struct fvm_kinetic {
@@ -39,6 +42,11 @@ struct fvm_kinetic {
     fvm::DArr<double> rho, vel, T, nu, coeff;   // vel: 3 x nct; coeff: 5 x nct (component-major)
     fvm::DArr<int> d_grp, d_refl;      // gptr + kind; reflection tables (K per group, -1 unused)
     fvm::DArr<double> d_wall;          // uw (3) and Tw per group
+    fvm::DArr<int> d_agrp;             // the assembly's groups: gptr + kind (0 wall-like, 1 zero gradient)
+    fvm::DArr<double> d_awall;         // the assembly's wall velocities (3 per group; zero for symmetry)
+    fvm::DArr<double> d_F;             // assembly scratch: face fluxes
+    fvm::DArr<uint8_t> d_inc;          // assembly scratch: incoming flags
+    std::vector<fvm_system *> sys;     // one system per velocity, kept from iteration to iteration
     fvm_batch *batch = nullptr;
     fvm_amg_params prm{};
     int niters = 0;
@@ -470,6 +478,22 @@ int fvm_kinetic_create(fvm_kinetic **out, fvm_mesh *mesh, int K, const double *c
                 }
             }
         }
+        // the assembly's group table: symmetry groups take the wall-like incoming test
+        // with zero velocity; zero-gradient groups take every direction as incoming
+        std::vector<int> agrp(2 * (size_t)ngr + 1, 0);
+        for (int g = 0; g <= ngr; g++) agrp[g] = p->gptr[g];
+        std::vector<double> awall(3 * (size_t)ngr, 0.0);
+        for (int g = 1; g < ngr; g++) {
+            agrp[ngr + 1 + g] = p->kind[g] == 1 ? 1 : 0;
+            if (p->kind[g] == 0)
+                for (int a = 0; a < 3; a++) awall[3 * g + a] = p->wall_u[3 * g + a];
+        }
+        p->d_agrp.alloc(agrp.size());
+        p->d_awall.alloc(awall.size());
+        FVM_HIP(hipMemcpy(p->d_agrp.p, agrp.data(), sizeof(int) * agrp.size(), hipMemcpyHostToDevice));
+        FVM_HIP(hipMemcpy(p->d_awall.p, awall.data(), sizeof(double) * awall.size(), hipMemcpyHostToDevice));
+        p->d_F.alloc(std::max(nf, 1));
+        p->d_inc.alloc(std::max(nf, 1));
         p->d_refl.alloc(std::max<size_t>(refl.size(), 1));
         FVM_HIP(hipMemcpy(p->d_refl.p, refl.data(), sizeof(int) * refl.size(), hipMemcpyHostToDevice));
         // KineticModel constructor (KineticModel.h:81-120): rest Maxwellian, then its
@@ -512,6 +536,10 @@ int fvm_kinetic_destroy(fvm_kinetic *p) {
         KDevGuard d(p->device);
         (void)hipStreamSynchronize(p->stream);
         if (p->batch) fvm_batch_destroy(p->batch);
+        for (auto *S : p->sys)
+            if (S) fvm_system_destroy(S);
+        p->sys.clear();
+        p->d_agrp.release(); p->d_awall.release(); p->d_F.release(); p->d_inc.release();
         p->f.release(); p->feq.release(); p->delta.release(); p->rho.release(); p->vel.release();
         p->T.release(); p->nu.release(); p->coeff.release(); p->d_q.release(); p->d_grp.release();
         p->d_refl.release(); p->d_wall.release();
@@ -535,74 +563,82 @@ int fvm_kinetic_advance(fvm_kinetic *p, int niter, double rel_tol, double abs_to
     FVM_API_BEGIN
     FVM_CHECK(p && niter >= 0 && (cap == 0 || norms), FVM_E_INVALID, "bad argument");
     KDevGuard d(p->device);
-    const int K = p->K, nct = p->nct, nc = p->nc;
+    const int K = p->K, nct = p->nct;
     krc(fvm_batch_set_params(p->batch, &p->prm));
-    std::vector<fvm_system *> sys(K, nullptr);
-    auto free_systems = [&]() {
-        for (auto &S : sys)
-            if (S) { fvm_system_destroy(S); S = nullptr; }
-    };
-    // the assembly's wall-like groups (diffuse walls and symmetry: kind 0 there) and
-    // their velocities (symmetry groups keep the default zero velocity)
-    std::vector<int32_t> akind(p->ngroups, 0);
-    for (int g = 1; g < p->ngroups; g++) akind[g] = p->kind[g] == 1 ? 1 : 0;
-    std::vector<double> au(3 * (size_t)p->ngroups, 0.0);
-    for (int g = 1; g < p->ngroups; g++)
-        if (p->kind[g] == 0)
-            for (int a = 0; a < 3; a++) au[3 * g + a] = p->wall_u[3 * g + a];
-    int done = 0, conv = 0;
-    try {
-        std::vector<int32_t> ns(K), nt(K);
-        std::vector<const int32_t *> rows(K), cols(K);
-        std::vector<const double *> diags(K), offs(K), bs(K);
-        std::vector<const uint8_t *> bnds(K);
-        std::vector<double *> xs(K);
-        std::vector<double> r0(K), rf(K);
-        std::vector<int> cyc(K);
-        for (int n = 0; n < niter; n++) {
-            FVM_HIP(hipStreamSynchronize(p->stream));   // f, feq, nu of the previous iteration are final
-            for (int k = 0; k < K; k++) {
-                // initKineticModelLinearization + linearizeKineticModel + initSolve for direction k
-                const double c3[3] = {p->cx[k], p->cy[k], p->cz[k]};
-                krc(fvm_kinetic_assemble(p->mesh, c3, p->nu.p, p->feq.p + (size_t)k * nct, p->f.p + (size_t)k * nct,
-                                         akind.data(), au.data(), p->opt.epsilon, &sys[k]));
-                int s_n, s_t;
-                long long nz;
-                krc(fvm_system_info(sys[k], &s_n, &s_t, &nz));
-                ns[k] = s_n; nt[k] = s_t;
-                const double *bb;
-                krc(fvm_system_device(sys[k], &rows[k], &cols[k], &diags[k], &offs[k], &bb, &bnds[k]));
-                bs[k] = bb;
-                xs[k] = p->delta.p + (size_t)k * nct;
-            }
-            (void)nc;
-            FVM_HIP(hipMemsetAsync(p->delta.p, 0, sizeof(double) * (size_t)K * nct, p->stream));
-            FVM_HIP(hipStreamSynchronize(p->stream));
-            krc(fvm_batch_setup_device(p->batch, K, ns.data(), nt.data(), rows.data(), cols.data(), diags.data(),
-                                       offs.data(), bnds.data()));
-            krc(fvm_batch_solve_device(p->batch, bs.data(), xs.data(), r0.data(), rf.data(), cyc.data()));
-            double rnorm = r0[0];
-            for (int k = 1; k < K; k++) rnorm += r0[k];
-            for (int k = 0; k < K; k++) krc(fvm_system_post_solve(sys[k], xs[k]));
-            free_systems();
-            fvm::k_axpy_kin<<<fvm::grid_for((long long)K * nct, 256), 256, 0, p->stream>>>((long long)K * nct,
-                                                                                         p->delta.p, p->f.p);
-            FVM_HIP(hipGetLastError());
-            if (!p->have_initial) { p->initial_norm = rnorm; p->have_initial = true; }
-            if (p->niters < 5) p->initial_norm = std::max(p->initial_norm, rnorm);   // setMax
-            const double ratio = rnorm / p->initial_norm;
-            p->norms.push_back(rnorm);
-            if (done < cap) norms[done] = rnorm;
-            p->niters++;
-            done++;
-            kinetic_update(p);
-            if (rnorm < abs_tol || ratio < rel_tol) { conv = 1; break; }
-        }
-        FVM_HIP(hipStreamSynchronize(p->stream));
-    } catch (...) {
-        free_systems();
-        throw;
+    if (p->sys.empty()) {
+        // one system per velocity, with the mesh's pattern; re-assembled in place
+        p->sys.assign(K, nullptr);
+        for (int k = 0; k < K; k++) p->sys[k] = fvm::system_for_mesh(p->mesh);
     }
+    hipStream_t ms = fvm::mesh_stream(p->mesh);
+    int done = 0, conv = 0;
+    std::vector<int32_t> ns(K), nt(K);
+    std::vector<const int32_t *> rows(K), cols(K);
+    std::vector<const double *> diags(K), offs(K), bs(K);
+    std::vector<const uint8_t *> bnds(K);
+    std::vector<double *> xs(K);
+    std::vector<double> r0(K), rf(K);
+    std::vector<int> cyc(K);
+    for (int k = 0; k < K; k++) {
+        int s_n, s_t;
+        long long nz;
+        krc(fvm_system_info(p->sys[k], &s_n, &s_t, &nz));
+        ns[k] = s_n; nt[k] = s_t;
+        const double *bb;
+        krc(fvm_system_device(p->sys[k], &rows[k], &cols[k], &diags[k], &offs[k], &bb, &bnds[k]));
+        bs[k] = bb;
+        xs[k] = p->delta.p + (size_t)k * nct;
+    }
+    // phase timing to stderr when FVM_AMD_KINETIC_TIMING is set
+    const bool timing = getenv("FVM_AMD_KINETIC_TIMING") != nullptr;
+    auto now = [&]() {
+        FVM_HIP(hipDeviceSynchronize());
+        return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    };
+    for (int n = 0; n < niter; n++) {
+        FVM_HIP(hipStreamSynchronize(p->stream));   // f, feq, nu of the previous iteration are final
+        const double t_a = timing ? now() : 0.0;
+        for (int k = 0; k < K; k++) {
+            // initKineticModelLinearization + linearizeKineticModel + initSolve for velocity k
+            const double c3[3] = {p->cx[k], p->cy[k], p->cz[k]};
+            fvm::kinetic_assemble_into(p->mesh, c3, p->nu.p, p->feq.p + (size_t)k * nct, p->f.p + (size_t)k * nct,
+                                       p->d_agrp.p, p->d_awall.p, p->opt.epsilon, p->sys[k], p->d_F.p, p->d_inc.p);
+        }
+        FVM_HIP(hipMemsetAsync(p->delta.p, 0, sizeof(double) * (size_t)K * nct, p->stream));
+        FVM_HIP(hipStreamSynchronize(ms));
+        FVM_HIP(hipStreamSynchronize(p->stream));
+        const double t_b = timing ? now() : 0.0;
+        krc(fvm_batch_setup_device(p->batch, K, ns.data(), nt.data(), rows.data(), cols.data(), diags.data(),
+                                   offs.data(), bnds.data()));
+        const double t_c = timing ? now() : 0.0;
+        krc(fvm_batch_solve_device(p->batch, bs.data(), xs.data(), r0.data(), rf.data(), cyc.data()));
+        const double t_d = timing ? now() : 0.0;
+        double rnorm = r0[0];
+        for (int k = 1; k < K; k++) rnorm += r0[k];
+        // postSolve (every direction) and updateSolution
+        for (int k = 0; k < K; k++) fvm::system_post_solve_async(p->sys[k], xs[k], p->stream);
+        fvm::k_axpy_kin<<<fvm::grid_for((long long)K * nct, 256), 256, 0, p->stream>>>((long long)K * nct,
+                                                                                     p->delta.p, p->f.p);
+        FVM_HIP(hipGetLastError());
+        if (!p->have_initial) { p->initial_norm = rnorm; p->have_initial = true; }
+        if (p->niters < 5) p->initial_norm = std::max(p->initial_norm, rnorm);   // setMax
+        const double ratio = rnorm / p->initial_norm;
+        p->norms.push_back(rnorm);
+        if (done < cap) norms[done] = rnorm;
+        p->niters++;
+        done++;
+        kinetic_update(p);
+        if (timing) {
+            const double t_e = now();
+            int cmax = 0;
+            for (int k = 0; k < K; k++) cmax = std::max(cmax, cyc[k]);
+            fprintf(stderr, "[kinetic] iteration %d: assembly %.1f ms, setup %.1f ms, solve %.1f ms (cycles <= %d), "
+                            "post-solve + update %.1f ms\n", p->niters, 1e3 * (t_b - t_a), 1e3 * (t_c - t_b),
+                    1e3 * (t_d - t_c), cmax, 1e3 * (t_e - t_d));
+        }
+        if (rnorm < abs_tol || ratio < rel_tol) { conv = 1; break; }
+    }
+    FVM_HIP(hipStreamSynchronize(p->stream));
     if (iterations) *iterations = done;
     if (converged) *converged = conv;
     return FVM_OK;
