@@ -244,7 +244,10 @@ __global__ void k_kin_walls(int nct, int K, int first_bface, int nf, int ngroups
         const int kind = grp[ngroups + 1 + g];
         const int c0 = fcells[2 * fi], c1 = fcells[2 * fi + 1];
         const double e0 = fA[3 * fi] / fAm[fi], e1 = fA[3 * fi + 1] / fAm[fi], e2 = fA[3 * fi + 2] / fAm[fi];
-        if (kind == 1) {
+        if (kind == 3) {
+            // a group without a boundary-value rule ("CopyBC" in the reference's
+            // Couette script): callBoundaryConditions leaves its ghosts alone
+        } else if (kind == 1) {
             for (int j = 0; j < K; j++) f[(size_t)j * nct + c1] = f[(size_t)j * nct + c0];
         } else if (kind == 2) {
             const int *tab = refl + (size_t)g * K;
@@ -397,8 +400,9 @@ int fvm_kinetic_create(fvm_kinetic **out, fvm_mesh *mesh, int K, const double *c
     FVM_CHECK(ngr >= 1 && ngr <= 64, FVM_E_INVALID, "bad face groups");
     FVM_CHECK(ngr <= 1 || (bc_kind && wall_u && wall_T), FVM_E_INVALID, "boundary groups need their conditions");
     for (int g = 1; g < ngr; g++) {
-        FVM_CHECK(bc_kind[g] >= 0 && bc_kind[g] <= 2, FVM_E_INVALID,
-                  "kinetic boundary kind must be 0 (diffuse wall), 1 (zero gradient) or 2 (symmetry)");
+        FVM_CHECK(bc_kind[g] >= 0 && bc_kind[g] <= 3, FVM_E_INVALID,
+                  "kinetic boundary kind must be 0 (diffuse wall), 1 (zero gradient), 2 (symmetry) or 3 (no "
+                  "boundary-value rule)");
         FVM_CHECK(bc_kind[g] != 0 || (std::isfinite(wall_T[g]) && wall_T[g] > 0), FVM_E_INVALID,
                   "a diffuse wall needs a positive temperature");
     }

@@ -9,7 +9,7 @@ import numpy as np
 
 from ._lib import AmgParams, KineticOptions, check, lib
 
-_KINDS = {"wall": 0, "zerograd": 1, "symmetry": 2}
+_KINDS = {"wall": 0, "zerograd": 1, "symmetry": 2, "copy": 3}
 
 
 def _vp(a):
@@ -38,8 +38,9 @@ def default_options():
 
 
 class KineticModel:
-    """bcs: {face group: ("wall", (u, v, w), T) | ("zerograd",) | ("symmetry",)} for
-    every boundary group of the mesh; options by the reference's names (rho_init,
+    """bcs: {face group: ("wall", (u, v, w), T) | ("zerograd",) | ("symmetry",) |
+    ("copy",)} for every boundary group of the mesh ("copy": a group type without a
+    boundary-value rule, whose ghosts keep their values); options by the reference's names (rho_init,
     T_init, mu_w, Tmuref, muref, molecularWeight, nonDimLt, Prandtl, fgamma, ktrial,
     ToleranceX, ToleranceF, epsilon). The per-direction AMG's parameters follow the
     reference's AMG names (set_solver)."""

@@ -555,7 +555,10 @@ typedef struct fvm_kinetic fvm_kinetic;
  * bc_kind[g] 0 diffuse wall at velocity wall_u[3g..3g+2] and temperature
  * wall_T[g] ("wall", KineticBoundaryConditions.h:65-140), 1 zero gradient
  * ("ZeroGradBC", :385-406), 2 specular symmetry ("symmetry", :287-330, reflection
- * table from the group's first face, KineticModel.h:290-318). The constructor's
+ * table from the group's first face, KineticModel.h:290-318), 3 a group whose type
+ * has no boundary-value rule (the reference's Couette script sets "CopyBC",
+ * test/ESBGK-tests/Couette/Par_Couette.py:199-206): its ghosts keep their values and
+ * it is linearized like a wall at rest (KineticModel.h:1647-1661). The constructor's
  * state (KineticModel.h:81-120): density 1, velocity 0, temperature 1, f the
  * Maxwellian of it, then its moments, collision frequency and equilibrium. The
  * mesh is borrowed. */

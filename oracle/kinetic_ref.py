@@ -236,7 +236,9 @@ def boundary_values(mesh, fA, fAm, bcs, refl, f, macro, cx, cy, cz, w, eps):
     """callBoundaryConditions (KineticModel.h:3271-3330) for the diffuse wall
     (KineticBoundaryConditions.h:65-140), zero gradient (:385-406) and specular
     symmetry (:287-330) groups, face by face in group order. bcs[g] = ("wall", uw, Tw),
-    ("zerograd",) or ("symmetry",). f (K x ncells) and macro (density, v, T) are
+    ("zerograd",), ("symmetry",) or ("copy",): a group type callBoundaryConditions has
+    no branch for ("CopyBC" in test/ESBGK-tests/Couette/Par_Couette.py:199-206) keeps
+    its ghosts. f (K x ncells) and macro (density, v, T) are
     changed in place."""
     density, v, T = macro
     fcl = mesh["face_cells"].reshape(-1, 2)
@@ -247,6 +249,8 @@ def boundary_values(mesh, fA, fAm, bcs, refl, f, macro, cx, cy, cz, w, eps):
         for fi in range(gp[g], gp[g + 1]):
             c0, c1 = fcl[fi]
             en = fA[3 * fi:3 * fi + 3] / fAm[fi]
+            if kind == "copy":
+                continue     # a group type without a boundary-value rule: ghosts keep their values
             if kind == "zerograd":
                 f[:, c1] = f[:, c0]
             elif kind == "symmetry":

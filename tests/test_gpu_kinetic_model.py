@@ -116,3 +116,43 @@ def test_kinetic_couette_on_the_device():
     vy = model.state()["velocity"][: mesh["n_cells"], 1].reshape(N, N, N)
     np.testing.assert_allclose(vy[:, ::-1, ::-1], -vy, atol=1e-11)
     np.testing.assert_allclose(vy, st["v"][: mesh["n_cells"], 1].reshape(N, N, N), rtol=1e-8, atol=1e-13)
+
+
+def test_reference_couette_case():
+    """The reference's Couette case (test/ESBGK-tests/Couette_10.cas with the
+    boundary set-up of Couette/Par_Couette.py:188-206: zone 4 a wall moving at
+    0.0297 in x, zone 6 a wall at rest, both at temperature 1, zones 3 and 5
+    "CopyBC"; 10^3 velocities, clim 5.5; rho_init 9.28e-7, Newton tolerances 1e-14),
+    with the BGK equilibrium (the script's ES-BGK is not built): the device loop on
+    the 2-D mesh read by the Fluent reader follows the oracle's, and the gas starts
+    to move with the top wall."""
+    import fluent_golden as FG
+    from fvm_amd.kinetic import KineticModel, Quadrature
+    m = FG.read_mesh("couette_10.cas")
+    a = m.arrays()
+    g = O.fv_metrics(a)
+    bcs = {}
+    for gi, zid in enumerate(m.group_ids):
+        if gi == 0:
+            continue
+        bcs[gi] = {4: ("wall", np.array([0.0297, 0.0, 0.0]), 1.0),
+                   6: ("wall", np.zeros(3), 1.0)}.get(zid, ("copy",))
+    q = (10, 10, 10)
+    quad = KR.quadrature_cartesian(*q, 5.5, 1.0)
+    opts = dict(fgamma=1, rho_init=9.28e-7, ToleranceX=1e-14, ToleranceF=1e-14)
+    o = dict(KR.DEFAULTS); o.update(opts)
+    st = KR.init_state(a["n_cells"] + a["n_ghost"], quad, o)
+    assemble, solve = kinetic_parts(a, quad, bcs)
+    niter = 3
+    norms_o, _ = KR.kinetic_advance(assemble, solve, a, g, quad, bcs, reflections(a, g, quad, bcs), o, st, niter,
+                                    1e-30, 0.0)
+    model = KineticModel(m.to_device(), Quadrature(*q, 5.5, 1.0), bcs, **opts)
+    model.set_solver(gsOrdering=0, relativeTolerance=1e-13, nMaxIterations=400)
+    model.advance(niter, relTolerance=1e-30, absTolerance=0.0)
+    print("couette_10 norms", model.norms, flush=True)
+    np.testing.assert_allclose(model.norms[1:], norms_o[1:], rtol=1e-9)
+    s = model.state()
+    nc = a["n_cells"]
+    np.testing.assert_allclose(s["velocity"][:nc], st["v"][:nc], rtol=1e-8, atol=1e-14)
+    np.testing.assert_allclose(s["density"][:nc], st["density"][:nc], rtol=1e-11)
+    assert s["velocity"][:nc, 0].max() > 0

@@ -168,6 +168,11 @@ def test_specular_wall_mirrors_and_has_no_normal_flux():
         assert abs((w * (-cx) * face_f).sum()) <= 1e-13 * (w * np.abs(cx) * f0[:, c0]).sum()
 
 
+def test_group_without_rule_keeps_its_ghosts():
+    mesh, g, quad, bcs, refl, f0, f = _wall_case("copy")
+    np.testing.assert_array_equal(f, f0)
+
+
 def test_zero_gradient_copies_the_cell():
     mesh, g, quad, bcs, refl, f0, f = _wall_case("zerograd")
     fcl = mesh["face_cells"].reshape(-1, 2)
