@@ -12,7 +12,9 @@
 // gradient, specular symmetry), the moments (ComputeMacroparameters), the
 // collision frequency and the equilibrium (fgamma 0: the Maxwellian of the stored
 // coefficients around the new velocity; 1: the BGK equilibrium whose coefficients
-// a per-cell Newton solve fits to the cell's discrete moments), then the stop test.
+// a per-cell Newton solve fits to the cell's discrete moments; 2: that, then the
+// ES-BGK equilibrium fitted to the temperature tensor, which the collision term then
+// relaxes to), then the stop test.
 //
 // The directions are independent within an iteration: here they are assembled
 // one after another on the device and solved together by one batched hierarchy
@@ -40,6 +42,7 @@ struct fvm_kinetic {
     fvm::DArr<double> d_q;             // cx, cy, cz, w (K each)
     fvm::DArr<double> f, feq, delta;   // K x nct, direction-major
     fvm::DArr<double> rho, vel, T, nu, coeff;   // vel: 3 x nct; coeff: 5 x nct (component-major)
+    fvm::DArr<double> coeffg, T6, feqes;        // ES-BGK: 10 x nct, 6 x nct (Txx Tyy Tzz Txy Tyz Tzx), K x nct
     fvm::DArr<int> d_grp, d_refl;      // gptr + kind; reflection tables (K per group, -1 unused)
     fvm::DArr<double> d_wall;          // uw (3) and Tw per group
     fvm::DArr<int> d_agrp;             // the assembly's groups: gptr + kind (0 wall-like, 1 zero gradient)
@@ -121,46 +124,47 @@ __global__ void k_kin_collision(int nct, const double *rho, const double *T, dou
     }
 }
 
-// inverseGauss (fvmbase/MatrixOperation.h:200-290) on a 5 x 5 matrix held by one
+// inverseGauss (fvmbase/MatrixOperation.h:200-290) on an N x N matrix held by one
 // thread: scaled partial pivoting, multipliers kept below the diagonal, then the
 // unit right-hand sides eliminated and back-substituted
-__device__ void inverse_gauss5(double a[5][5], double x[5][5]) {
-    int indx[5];
-    double cs[5], b[5][5];
-    for (int i = 0; i < 5; i++) {
+template <int N>
+__device__ void inverse_gauss(double a[N][N], double x[N][N]) {
+    int indx[N];
+    double cs[N], b[N][N];
+    for (int i = 0; i < N; i++) {
         indx[i] = i;
-        for (int j = 0; j < 5; j++) { b[i][j] = i == j ? 1.0 : 0.0; x[i][j] = 0.0; }
+        for (int j = 0; j < N; j++) { b[i][j] = i == j ? 1.0 : 0.0; x[i][j] = 0.0; }
     }
-    for (int i = 0; i < 5; i++) {
+    for (int i = 0; i < N; i++) {
         double c1 = 0;
-        for (int j = 0; j < 5; j++)
+        for (int j = 0; j < N; j++)
             if (fabs(a[i][j]) > c1) c1 = fabs(a[i][j]);
         cs[i] = c1;
     }
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < N - 1; j++) {
         double pi1 = 0.0;
         int pk = j;
-        for (int i = j; i < 5; i++) {
+        for (int i = j; i < N; i++) {
             const double pi = fabs(a[indx[i]][j]) / cs[indx[i]];
             if (pi > pi1) { pi1 = pi; pk = i; }
         }
         const int itmp = indx[j];
         indx[j] = indx[pk];
         indx[pk] = itmp;
-        for (int i = j + 1; i < 5; i++) {
+        for (int i = j + 1; i < N; i++) {
             const double pj = a[indx[i]][j] / a[indx[j]][j];
             a[indx[i]][j] = pj;
-            for (int k = j + 1; k < 5; k++) a[indx[i]][k] = a[indx[i]][k] - pj * a[indx[j]][k];
+            for (int k = j + 1; k < N; k++) a[indx[i]][k] = a[indx[i]][k] - pj * a[indx[j]][k];
         }
     }
-    for (int i = 0; i < 4; i++)
-        for (int j = i + 1; j < 5; j++)
-            for (int k = 0; k < 5; k++) b[indx[j]][k] = b[indx[j]][k] - a[indx[j]][i] * b[indx[i]][k];
-    for (int col = 0; col < 5; col++) {
-        x[4][col] = b[indx[4]][col] / a[indx[4]][4];
-        for (int j = 3; j >= 0; j = j - 1) {
+    for (int i = 0; i < N - 1; i++)
+        for (int j = i + 1; j < N; j++)
+            for (int k = 0; k < N; k++) b[indx[j]][k] = b[indx[j]][k] - a[indx[j]][i] * b[indx[i]][k];
+    for (int col = 0; col < N; col++) {
+        x[N - 1][col] = b[indx[N - 1]][col] / a[indx[N - 1]][N - 1];
+        for (int j = N - 2; j >= 0; j = j - 1) {
             x[j][col] = b[indx[j]][col];
-            for (int k = j + 1; k < 5; k++) x[j][col] = x[j][col] - a[indx[j]][k] * x[k][col];
+            for (int k = j + 1; k < N; k++) x[j][col] = x[j][col] - a[indx[j]][k] * x[k][col];
             x[j][col] = x[j][col] / a[indx[j]][j];
         }
     }
@@ -197,7 +201,7 @@ __global__ void k_kin_newton_bgk(int nct, int K, const double *q, const double *
             for (int r = 0; r < 5; r++) errf += fabs(fvec[r]);
             if (errf <= tolf) break;
             double inv[5][5];
-            inverse_gauss5(fjac, inv);
+            inverse_gauss<5>(fjac, inv);
             double errx = 0.;
             for (int r = 0; r < 5; r++) {
                 double xr = 0.0;
@@ -221,6 +225,112 @@ __global__ void k_kin_equilibrium(int nct, int K, const double *q, const double 
         const double a0 = cx[j] - vel[c], a1 = cy[j] - vel[nct + c], a2 = cz[j] - vel[2 * (size_t)nct + c];
         feq[i] = coeff[c] * exp(-coeff[nct + c] * (a0 * a0 + a1 * a1 + a2 * a2) + coeff[2 * (size_t)nct + c] * a0 +
                                 coeff[3 * (size_t)nct + c] * a1 + coeff[4 * (size_t)nct + c] * a2);
+    }
+}
+
+// the ES-BGK equilibrium's exponent (KineticModel.h:1151-1161, 1196-1200)
+__device__ __forceinline__ double es_exponent(const double *xn, double cxj, double cyj, double czj, double c1,
+                                              double c2, double c3) {
+    return -xn[1] * (c1 * c1) + xn[2] * c1 - xn[3] * (c2 * c2) + xn[4] * c2 - xn[5] * (c3 * c3) + xn[6] * c3 +
+           xn[7] * cxj * cyj + xn[8] * cyj * czj + xn[9] * czj * cxj;
+}
+
+// ComputeMacroparametersESBGK (KineticModel.h:565-631): the temperature tensor of
+// (1 - 1/Pr) f + 1/Pr feq (feq the BGK equilibrium), per cell
+__global__ void k_kin_tensor(int nct, int K, const double *q, const double *f, const double *feq, const double *rho,
+                             const double *vel, double Pr, double *T6) {
+    const double *cx = q, *cy = q + K, *cz = q + 2 * K, *w = q + 3 * K;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nct; c += gridDim.x * blockDim.x) {
+        const double u0 = vel[c], u1 = vel[nct + c], u2 = vel[2 * (size_t)nct + c];
+        double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < K; j++) {
+            const double g = (1 - 1 / Pr) * f[(size_t)j * nct + c] + 1 / Pr * feq[(size_t)j * nct + c];
+            const double d0 = cx[j] - u0, d1 = cy[j] - u1, d2 = cz[j] - u2;
+            t[0] = t[0] + d0 * d0 * g * w[j];
+            t[1] = t[1] + d1 * d1 * g * w[j];
+            t[2] = t[2] + d2 * d2 * g * w[j];
+            t[3] = t[3] + d0 * d1 * g * w[j];
+            t[4] = t[4] + d1 * d2 * g * w[j];
+            t[5] = t[5] + d2 * d0 * g * w[j];
+        }
+        for (int a = 0; a < 6; a++) T6[(size_t)a * nct + c] = t[a] / rho[c];
+    }
+}
+
+// NewtonsMethodESBGK + setJacobianESBGK (KineticModel.h:1019-1108, 1166-1213), one cell per thread
+__global__ void k_kin_newton_esbgk(int nct, int K, const double *q, const double *rho, const double *vel,
+                                   const double *T6, int ktrial, double tolx, double tolf, double *coeffg) {
+    const double *cx = q, *cy = q + K, *cz = q + 2 * K, *w = q + 3 * K;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nct; c += gridDim.x * blockDim.x) {
+        const double d = rho[c], u0 = vel[c], u1 = vel[nct + c], u2 = vel[2 * (size_t)nct + c];
+        double Tt[6];
+        for (int a = 0; a < 6; a++) Tt[a] = T6[(size_t)a * nct + c];
+        double xn[10];
+        for (int r = 0; r < 10; r++) xn[r] = coeffg[(size_t)r * nct + c];
+        for (int trial = 0; trial < ktrial; trial++) {
+            double fjac[10][10], fvec[10];
+            for (int r = 0; r < 10; r++)
+                for (int k = 0; k < 10; k++) fjac[r][k] = 0.0;
+            fvec[0] = d;
+            fvec[1] = d * u0;
+            fvec[2] = d * u1;
+            fvec[3] = d * u2;
+            fvec[4] = d * (u0 * u0 + Tt[0]);
+            fvec[5] = d * (u1 * u1 + Tt[1]);
+            fvec[6] = d * (u2 * u2 + Tt[2]);
+            fvec[7] = d * (u0 * u1 + Tt[3]);
+            fvec[8] = d * (u1 * u2 + Tt[4]);
+            fvec[9] = d * (u2 * u0 + Tt[5]);
+            for (int j = 0; j < K; j++) {
+                const double c1 = cx[j] - u0, c2 = cy[j] - u1, c3 = cz[j] - u2;
+                const double E = xn[0] * exp(es_exponent(xn, cx[j], cy[j], cz[j], c1, c2, c3)) * w[j];
+                const double m[10] = {1.0, cx[j], cy[j], cz[j], cx[j] * cx[j], cy[j] * cy[j], cz[j] * cz[j],
+                                      cx[j] * cy[j], cy[j] * cz[j], cz[j] * cx[j]};
+                for (int r = 0; r < 10; r++) fvec[r] += -E * m[r];
+                const double me[10] = {-E / xn[0], E * (c1 * c1), -E * c1, E * (c2 * c2), -E * c2, E * (c3 * c3),
+                                       -E * c3, -E * cx[j] * cy[j], -E * cy[j] * cz[j], -E * cz[j] * cx[j]};
+                for (int r = 0; r < 10; r++)
+                    for (int k = 0; k < 10; k++) fjac[r][k] += m[r] * me[k];
+            }
+            double errf = 0.;
+            for (int r = 0; r < 10; r++) errf += fabs(fvec[r]);
+            if (errf <= tolf) break;
+            double inv[10][10];
+            inverse_gauss<10>(fjac, inv);
+            double errx = 0.;
+            for (int r = 0; r < 10; r++) {
+                double xr = 0.0;
+                for (int k = 0; k < 10; k++) xr += inv[r][k] * -fvec[k];
+                errx += fabs(xr);
+                xn[r] += xr;
+            }
+            if (errx <= tolx) break;
+        }
+        for (int r = 0; r < 10; r++) coeffg[(size_t)r * nct + c] = xn[r];
+    }
+}
+
+// the ES-BGK equilibrium (EquilibriumDistributionESBGK, KineticModel.h:1146-1162)
+__global__ void k_kin_equilibrium_es(int nct, int K, const double *q, const double *coeffg, const double *vel,
+                                     double *feqes) {
+    const double *cx = q, *cy = q + K, *cz = q + 2 * K;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < (long long)K * nct;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(i / nct), c = (int)(i % nct);
+        double xn[10];
+        for (int r = 0; r < 10; r++) xn[r] = coeffg[(size_t)r * nct + c];
+        const double c1 = cx[j] - vel[c], c2 = cy[j] - vel[nct + c], c3 = cz[j] - vel[2 * (size_t)nct + c];
+        feqes[i] = xn[0] * exp(es_exponent(xn, cx[j], cy[j], cz[j], c1, c2, c3));
+    }
+}
+
+// InitializeMacroparameters' ES-BGK coefficients (KineticModel.h:392-402)
+__global__ void k_kin_init_es(int nct, const double *coeff, double *coeffg) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nct; c += gridDim.x * blockDim.x) {
+        const double a0 = coeff[c], a1 = coeff[nct + c], a2 = coeff[2 * (size_t)nct + c],
+                     a3 = coeff[3 * (size_t)nct + c], a4 = coeff[4 * (size_t)nct + c];
+        const double g[10] = {a0, a1, a2, a1, a3, a1, a4, 0.0, 0.0, 0.0};
+        for (int r = 0; r < 10; r++) coeffg[(size_t)r * nct + c] = g[r];
     }
 }
 
@@ -334,13 +444,24 @@ void kinetic_update(fvm_kinetic *p) {
     fvm::k_kin_collision<<<fvm::grid_for(p->nct, 256), 256, 0, s>>>(p->nct, p->rho.p, p->T.p, o.T_init, o.Tmuref,
                                                                     o.muref, o.mu_w, mu0, o.Prandtl, o.fgamma,
                                                                     p->nu.p);
-    if (o.fgamma == 1)
+    if (o.fgamma >= 1)
         fvm::k_kin_newton_bgk<<<fvm::grid_for(p->nct, 64), 64, 0, s>>>(p->nct, p->K, p->d_q.p, p->rho.p, p->vel.p,
                                                                       p->T.p, o.ktrial, o.ToleranceX, o.ToleranceF,
                                                                       p->coeff.p);
     const long long tot = (long long)p->K * p->nct;
     fvm::k_kin_equilibrium<<<fvm::grid_for(tot, 256), 256, 0, s>>>(p->nct, p->K, p->d_q.p, p->coeff.p, p->vel.p,
                                                                    p->feq.p);
+    if (o.fgamma == 2) {
+        // EquilibriumDistributionESBGK (KineticModel.h:1112-1163): the tensor of the
+        // new f and BGK equilibrium, the ten-coefficient Newton fit, the equilibrium
+        fvm::k_kin_tensor<<<fvm::grid_for(p->nct, 256), 256, 0, s>>>(p->nct, p->K, p->d_q.p, p->f.p, p->feq.p,
+                                                                     p->rho.p, p->vel.p, o.Prandtl, p->T6.p);
+        fvm::k_kin_newton_esbgk<<<fvm::grid_for(p->nct, 64), 64, 0, s>>>(p->nct, p->K, p->d_q.p, p->rho.p,
+                                                                        p->vel.p, p->T6.p, o.ktrial, o.ToleranceX,
+                                                                        o.ToleranceF, p->coeffg.p);
+        fvm::k_kin_equilibrium_es<<<fvm::grid_for(tot, 256), 256, 0, s>>>(p->nct, p->K, p->d_q.p, p->coeffg.p,
+                                                                          p->vel.p, p->feqes.p);
+    }
     FVM_HIP(hipGetLastError());
 }
 }  // namespace
@@ -379,10 +500,10 @@ int fvm_quadrature_cartesian(int N1, int N2, int N3, double clim, double T2, dou
 
 int fvm_kinetic_default_options(fvm_kinetic_options *o) {
     if (!o) return FVM_E_INVALID;
-    // KineticModelOptions (esbgkbase/KineticBC.h:61-113); fgamma 1 (BGK) here
+    // KineticModelOptions (esbgkbase/KineticBC.h:61-113)
     o->rho_init = 9.28e-6; o->T_init = 273.15; o->nonDimLt = 1.0; o->molecularWeight = 40.0;
     o->Tmuref = 273.15; o->muref = 2.117e-5; o->mu_w = 0.81; o->Prandtl = 2.0 / 3.0;
-    o->fgamma = 1; o->ktrial = 50; o->ToleranceX = 1e-8; o->ToleranceF = 1e-16; o->epsilon = 1e-50;
+    o->fgamma = 2; o->ktrial = 50; o->ToleranceX = 1e-8; o->ToleranceF = 1e-16; o->epsilon = 1e-50;
     return FVM_OK;
 }
 
@@ -392,8 +513,10 @@ int fvm_kinetic_create(fvm_kinetic **out, fvm_mesh *mesh, int K, const double *c
     FVM_API_BEGIN
     FVM_CHECK(out && mesh && cx && cy && cz && w && opt, FVM_E_INVALID, "null argument");
     FVM_CHECK(K >= 1, FVM_E_INVALID, "need at least one direction");
-    FVM_CHECK(opt->fgamma == 0 || opt->fgamma == 1, FVM_E_INVALID,
-              "fgamma must be 0 (Maxwellian) or 1 (BGK); the ES-BGK equilibrium (2) is not built");
+    FVM_CHECK(opt->fgamma >= 0 && opt->fgamma <= 2, FVM_E_INVALID,
+              "fgamma must be 0 (Maxwellian), 1 (BGK) or 2 (ES-BGK)");
+    FVM_CHECK(opt->fgamma != 2 || (std::isfinite(opt->Prandtl) && opt->Prandtl != 0), FVM_E_INVALID,
+              "ES-BGK needs a finite non-zero Prandtl number");
     FVM_CHECK(opt->ktrial >= 0, FVM_E_INVALID, "ktrial must be >= 0");
     int nc, ng, nf, nn, ngr;
     krc(fvm_mesh_info(mesh, &nc, &ng, &nf, &nn, &ngr));
@@ -434,6 +557,11 @@ int fvm_kinetic_create(fvm_kinetic **out, fvm_mesh *mesh, int K, const double *c
         p->delta.alloc(std::max<size_t>(tot, 1));
         p->rho.alloc(std::max(p->nct, 1)); p->T.alloc(std::max(p->nct, 1)); p->nu.alloc(std::max(p->nct, 1));
         p->vel.alloc(3 * (size_t)std::max(p->nct, 1)); p->coeff.alloc(5 * (size_t)std::max(p->nct, 1));
+        if (opt->fgamma == 2) {
+            p->coeffg.alloc(10 * (size_t)std::max(p->nct, 1));
+            p->T6.alloc(6 * (size_t)std::max(p->nct, 1));
+            p->feqes.alloc(std::max<size_t>(tot, 1));
+        }
         std::vector<double> q(4 * (size_t)K);
         for (int j = 0; j < K; j++) { q[j] = cx[j]; q[K + j] = cy[j]; q[2 * K + j] = cz[j]; q[3 * K + j] = w[j]; }
         p->d_q.alloc(q.size());
@@ -517,6 +645,11 @@ int fvm_kinetic_create(fvm_kinetic **out, fvm_mesh *mesh, int K, const double *c
         fvm::k_kin_equilibrium<<<fvm::grid_for((long long)tot, 256), 256, 0, p->stream>>>(p->nct, K, p->d_q.p,
                                                                                           p->coeff.p, p->vel.p,
                                                                                           p->feq.p);
+        if (opt->fgamma == 2) {
+            fvm::k_kin_init_es<<<fvm::grid_for(p->nct, 256), 256, 0, p->stream>>>(p->nct, p->coeff.p, p->coeffg.p);
+            fvm::k_kin_equilibrium_es<<<fvm::grid_for((long long)tot, 256), 256, 0, p->stream>>>(
+                p->nct, K, p->d_q.p, p->coeffg.p, p->vel.p, p->feqes.p);
+        }
         FVM_HIP(hipGetLastError());
         FVM_HIP(hipStreamSynchronize(p->stream));
         krc(fvm_amg_default_params(&p->prm));
@@ -544,6 +677,7 @@ int fvm_kinetic_destroy(fvm_kinetic *p) {
             if (S) fvm_system_destroy(S);
         p->sys.clear();
         p->d_agrp.release(); p->d_awall.release(); p->d_F.release(); p->d_inc.release();
+        p->coeffg.release(); p->T6.release(); p->feqes.release();
         p->f.release(); p->feq.release(); p->delta.release(); p->rho.release(); p->vel.release();
         p->T.release(); p->nu.release(); p->coeff.release(); p->d_q.release(); p->d_grp.release();
         p->d_refl.release(); p->d_wall.release();
@@ -605,7 +739,8 @@ int fvm_kinetic_advance(fvm_kinetic *p, int niter, double rel_tol, double abs_to
         for (int k = 0; k < K; k++) {
             // initKineticModelLinearization + linearizeKineticModel + initSolve for velocity k
             const double c3[3] = {p->cx[k], p->cy[k], p->cz[k]};
-            fvm::kinetic_assemble_into(p->mesh, c3, p->nu.p, p->feq.p + (size_t)k * nct, p->f.p + (size_t)k * nct,
+            const double *feq_lin = p->opt.fgamma == 2 ? p->feqes.p : p->feq.p;
+            fvm::kinetic_assemble_into(p->mesh, c3, p->nu.p, feq_lin + (size_t)k * nct, p->f.p + (size_t)k * nct,
                                        p->d_agrp.p, p->d_awall.p, p->opt.epsilon, p->sys[k], p->d_F.p, p->d_inc.p);
         }
         FVM_HIP(hipMemsetAsync(p->delta.p, 0, sizeof(double) * (size_t)K * nct, p->stream));
@@ -657,7 +792,8 @@ int fvm_kinetic_get(const fvm_kinetic *p, double *f, double *feq, double *densit
     FVM_HIP(hipStreamSynchronize(p->stream));
     const size_t tot = (size_t)p->K * p->nct;
     if (f) FVM_HIP(hipMemcpy(f, p->f.p, sizeof(double) * tot, hipMemcpyDeviceToHost));
-    if (feq) FVM_HIP(hipMemcpy(feq, p->feq.p, sizeof(double) * tot, hipMemcpyDeviceToHost));
+    if (feq) FVM_HIP(hipMemcpy(feq, p->opt.fgamma == 2 ? p->feqes.p : p->feq.p, sizeof(double) * tot,
+                               hipMemcpyDeviceToHost));
     if (density) FVM_HIP(hipMemcpy(density, p->rho.p, sizeof(double) * p->nct, hipMemcpyDeviceToHost));
     if (velocity) FVM_HIP(hipMemcpy(velocity, p->vel.p, sizeof(double) * 3 * p->nct, hipMemcpyDeviceToHost));
     if (temperature) FVM_HIP(hipMemcpy(temperature, p->T.p, sizeof(double) * p->nct, hipMemcpyDeviceToHost));

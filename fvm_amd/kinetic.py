@@ -31,7 +31,7 @@ class Quadrature:
 
 
 def default_options():
-    """KineticModelOptions defaults (KineticBC.h:61-113), fgamma 1 (BGK)."""
+    """KineticModelOptions defaults (KineticBC.h:61-113): fgamma 2 (ES-BGK)."""
     o = KineticOptions()
     check(lib().fvm_kinetic_default_options(ctypes.byref(o)))
     return o

@@ -543,12 +543,12 @@ int fvm_quadrature_cartesian(int N1, int N2, int N3, double clim, double T2, dou
 /* KineticModelOptions (esbgkbase/KineticBC.h:61-113) used by the loop */
 typedef struct fvm_kinetic_options {
     double rho_init, T_init, nonDimLt, molecularWeight, Tmuref, muref, mu_w, Prandtl;
-    int fgamma;          /* 0: Maxwellian of the stored coefficients, 1: BGK (Newton fit); 2 is refused */
+    int fgamma;          /* 0: Maxwellian of the stored coefficients, 1: BGK, 2: ES-BGK (Newton fits) */
     int ktrial;          /* NewtonsMethod_ktrial */
     double ToleranceX, ToleranceF;
     double epsilon;      /* epsilon_ES: incoming test c.n - uw.n < epsilon */
 } fvm_kinetic_options;
-/* the reference's defaults, except fgamma = 1 (the reference defaults to 2) */
+/* the reference's defaults (fgamma 2) */
 int fvm_kinetic_default_options(fvm_kinetic_options *o);
 typedef struct fvm_kinetic fvm_kinetic;
 /* K velocities (cx, cy, cz, w: host); per face group g >= 1 (entry 0 unused):
@@ -578,7 +578,8 @@ int fvm_kinetic_set_params(fvm_kinetic *p, const fvm_amg_params *prm);
  * the call. */
 int fvm_kinetic_advance(fvm_kinetic *p, int niter, double rel_tol, double abs_tol, int *iterations,
                         double *norms, int cap, int *converged);
-/* host copies (any may be NULL): f and feq (K blocks of n_cells + n_ghost,
+/* host copies (any may be NULL): f and feq (the equilibrium the collision term
+ * relaxes to: ES-BGK's for fgamma 2) (K blocks of n_cells + n_ghost,
  * direction-major), density, temperature, collision frequency (n_cells + n_ghost),
  * velocity (3 blocks of n_cells + n_ghost: x, y, z) */
 int fvm_kinetic_get(const fvm_kinetic *p, double *f, double *feq, double *density, double *velocity,

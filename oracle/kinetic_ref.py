@@ -2,7 +2,8 @@
 
 Restatement of the ESBGK kinetic model loop around the per-direction solves
 (esbgkbase/KineticModel.h, KineticBoundaryConditions.h, Quadrature.h,
-fvmbase/MatrixOperation.h), BGK and Maxwellian equilibria (options fgamma 1 and 0),
+fvmbase/MatrixOperation.h), Maxwellian, BGK and ES-BGK equilibria (options fgamma 0,
+1 and 2),
 with the diffuse-wall, zero-gradient and specular (symmetry) boundaries. The
 per-direction assembly and the AMG solve are passed in (the C oracle's
 `ofv_kinetic` and AMG), so this file holds the quadrature, the moments, the
@@ -17,6 +18,8 @@ Pinned by tests/test_kinetic_ref.py, against what the mathematics fixes:
   values when recomputed from the definition);
 - the BGK Newton solve: the discrete moments of the equilibrium it returns equal
   the cell's moments (the conservation it is built for), reached quadratically;
+  the ES-BGK solve likewise for its ten moments, and at Prandtl 1 it gives the
+  BGK equilibrium back (the temperature tensor is then the BGK one's);
 - the scaled-pivoting Gauss inverse against numpy.linalg.inv;
 - the diffuse wall: zero net mass flux through the wall, ghost = wall Maxwellian
   for incoming directions; the specular wall: zero normal mass flux and mirrored
@@ -33,7 +36,7 @@ PI = np.arccos(-1.0)   # KineticBC.h:108 (pi = acos(-1))
 
 # KineticModelOptions defaults (KineticBC.h:61-113)
 DEFAULTS = dict(rho_init=9.28e-6, T_init=273.15, nonDimLt=1.0, molecularWeight=40.0, Tmuref=273.15,
-                muref=2.117e-5, mu_w=0.81, Prandtl=2.0 / 3.0, fgamma=1, ktrial=50, ToleranceX=1e-8,
+                muref=2.117e-5, mu_w=0.81, Prandtl=2.0 / 3.0, fgamma=2, ktrial=50, ToleranceX=1e-8,
                 ToleranceF=1e-16, epsilon=1e-50)
 
 
@@ -215,6 +218,94 @@ def newton_bgk(density, v, T, coeff, cx, cy, cz, w, o):
     return steps
 
 
+def macroparameters_es(f, feq, v, density, cx, cy, cz, w, Pr):
+    """ComputeMacroparametersESBGK (KineticModel.h:565-631): the temperature tensor
+    T_ab = sum_j (c_a - u_a)(c_b - u_b) ((1 - 1/Pr) f_j + 1/Pr feq_j) w_j / rho, with
+    feq the BGK equilibrium; returns (Txx, Tyy, Tzz, Txy, Tyz, Tzx) per cell."""
+    nct = f.shape[1]
+    T6 = [np.zeros(nct) for _ in range(6)]
+    for j in range(len(cx)):
+        g = (1 - 1 / Pr) * f[j] + 1 / Pr * feq[j]
+        d0, d1, d2 = cx[j] - v[:, 0], cy[j] - v[:, 1], cz[j] - v[:, 2]
+        T6[0] = T6[0] + d0 * d0 * g * w[j]
+        T6[1] = T6[1] + d1 * d1 * g * w[j]
+        T6[2] = T6[2] + d2 * d2 * g * w[j]
+        T6[3] = T6[3] + d0 * d1 * g * w[j]
+        T6[4] = T6[4] + d1 * d2 * g * w[j]
+        T6[5] = T6[5] + d2 * d0 * g * w[j]
+    return [t / density for t in T6]
+
+
+def es_exponent(xn, cx, cy, cz, u, j):
+    """The exponent of the ES-BGK equilibrium (KineticModel.h:1151-1161, 1196-1200)."""
+    c1, c2, c3 = cx[j] - u[:, 0], cy[j] - u[:, 1], cz[j] - u[:, 2]
+    return (-xn[:, 1] * (c1 * c1) + xn[:, 2] * c1 - xn[:, 3] * (c2 * c2) + xn[:, 4] * c2
+            - xn[:, 5] * (c3 * c3) + xn[:, 6] * c3
+            + xn[:, 7] * cx[j] * cy[j] + xn[:, 8] * cy[j] * cz[j] + xn[:, 9] * cz[j] * cx[j])
+
+
+def equilibrium_es(coeffg, v, cx, cy, cz):
+    """The ES-BGK equilibrium (EquilibriumDistributionESBGK, KineticModel.h:1146-1162)."""
+    K = len(cx)
+    out = np.empty((K, v.shape[0]))
+    for j in range(K):
+        out[j] = coeffg[:, 0] * np.exp(es_exponent(coeffg, cx, cy, cz, v, j))
+    return out
+
+
+def newton_esbgk(density, v, T6, coeffg, cx, cy, cz, w, o):
+    """NewtonsMethodESBGK + setJacobianESBGK (KineticModel.h:1019-1108, 1166-1213):
+    the ten coefficients fitted so that the equilibrium's discrete moments
+    [1, c, c_x^2, c_y^2, c_z^2, c_x c_y, c_y c_z, c_z c_x] equal
+    rho [1, u, u_a u_b + T_ab]; the same stop tests as the BGK fit."""
+    K = len(cx)
+    mab = np.stack([np.ones(K), cx, cy, cz, cx * cx, cy * cy, cz * cz, cx * cy, cy * cz, cz * cx], 1)
+    Txx, Tyy, Tzz, Txy, Tyz, Tzx = T6
+    n = len(density)
+    steps = np.zeros(n, int)
+    active = np.ones(n, bool)
+    for trial in range(o["ktrial"]):
+        idx = np.nonzero(active)[0]
+        if len(idx) == 0:
+            break
+        rho, u, a = density[idx], v[idx], coeffg[idx]
+        fvec = np.stack([rho, rho * u[:, 0], rho * u[:, 1], rho * u[:, 2],
+                         rho * (u[:, 0] * u[:, 0] + Txx[idx]), rho * (u[:, 1] * u[:, 1] + Tyy[idx]),
+                         rho * (u[:, 2] * u[:, 2] + Tzz[idx]), rho * (u[:, 0] * u[:, 1] + Txy[idx]),
+                         rho * (u[:, 1] * u[:, 2] + Tyz[idx]), rho * (u[:, 2] * u[:, 0] + Tzx[idx])], 1)
+        fjac = np.zeros((len(idx), 10, 10))
+        for j in range(K):
+            c1, c2, c3 = cx[j] - u[:, 0], cy[j] - u[:, 1], cz[j] - u[:, 2]
+            E = a[:, 0] * np.exp(es_exponent(a, cx, cy, cz, u, j)) * w[j]
+            for row in range(10):
+                fvec[:, row] += -E * mab[j, row]
+            mexp = (-E / a[:, 0], E * (c1 * c1), -E * c1, E * (c2 * c2), -E * c2, E * (c3 * c3), -E * c3,
+                    -E * cx[j] * cy[j], -E * cy[j] * cz[j], -E * cz[j] * cx[j])
+            for row in range(10):
+                for col in range(10):
+                    fjac[:, row, col] += mab[j, row] * mexp[col]
+        for q, c in enumerate(idx):
+            errf = 0.0
+            for row in range(10):
+                errf += abs(fvec[q, row])
+            if errf <= o["ToleranceF"]:
+                active[c] = False
+                continue
+            inv = inverse_gauss(fjac[q])
+            pvec = -fvec[q]
+            errx = 0.0
+            for row in range(10):
+                xr = 0.0
+                for col in range(10):
+                    xr += inv[row, col] * pvec[col]
+                errx += abs(xr)
+                coeffg[c, row] += xr
+            steps[c] += 1
+            if errx <= o["ToleranceX"]:
+                active[c] = False
+    return steps
+
+
 def reflection_map(en, cx, cy, cz):
     """Specular reflection table of a symmetry group (KineticModel.h:290-318): for
     every direction j the direction nearest to c_j - 2 (c_j.n) n (first minimum)."""
@@ -295,11 +386,18 @@ def init_state(nct, quad, o):
     coeff = np.zeros((nct, 5))
     coeff[:, 0] = density / np.power(PI * T, 1.5)
     coeff[:, 1] = 1 / T
+    coeffg = np.zeros((nct, 10))
+    if o["fgamma"] == 2:   # InitializeMacroparameters (KineticModel.h:392-402)
+        coeffg[:, 0] = coeff[:, 0]; coeffg[:, 1] = coeff[:, 1]; coeffg[:, 2] = coeff[:, 2]
+        coeffg[:, 3] = coeff[:, 1]; coeffg[:, 4] = coeff[:, 3]; coeffg[:, 5] = coeff[:, 1]
+        coeffg[:, 6] = coeff[:, 4]
     f = maxwellian(density, v, T, cx, cy, cz)
     density, v, T, p = macroparameters(f, cx, cy, cz, w)
     nu = collision_frequency(density, T, o)
     feq = equilibrium(coeff, v, cx, cy, cz)
-    return dict(f=f, feq=feq, density=density, v=v, T=T, nu=nu, coeff=coeff, niters=0, initial=None)
+    st = dict(f=f, feq=feq, density=density, v=v, T=T, nu=nu, coeff=coeff, coeffg=coeffg, niters=0, initial=None)
+    st["feq_lin"] = equilibrium_es(coeffg, v, cx, cy, cz) if o["fgamma"] == 2 else feq
+    return st
 
 
 def kinetic_advance(assemble, solve, mesh, metrics, quad, bcs, refl, o, st, niter, rel_tol, abs_tol):
@@ -309,8 +407,11 @@ def kinetic_advance(assemble, solve, mesh, metrics, quad, bcs, refl, o, st, nite
     iteration's), postSolve and updateSolution; the initial norm (first iteration,
     then the largest of the first five); then always the boundary values, moments,
     collision frequency and equilibrium (fgamma 0: the Maxwellian of the initial
-    coefficients around the new velocity; 1: the BGK Newton solve); then the stop
-    test. Returns (norms, converged)."""
+    coefficients around the new velocity; 1: the BGK Newton solve; 2: that, then the
+    temperature tensor and the ES-BGK Newton solve, KineticModel.h:3448-3452); then
+    the stop test. The collision term relaxes to the ES-BGK equilibrium for fgamma 2,
+    to the BGK / Maxwellian one otherwise (KineticModel.h:1520-1539). Returns
+    (norms, converged)."""
     cx, cy, cz, w = quad
     K = len(cx)
     nc = mesh["n_cells"]
@@ -318,7 +419,7 @@ def kinetic_advance(assemble, solve, mesh, metrics, quad, bcs, refl, o, st, nite
     for _ in range(niter):
         rnorm = None
         for j in range(K):
-            sysj = assemble(j, st["nu"][:nc], st["feq"][j][:nc], st["f"][j])
+            sysj = assemble(j, st["nu"][:nc], st["feq_lin"][j][:nc], st["f"][j])
             r0, delta = solve(sysj)
             rnorm = r0 if rnorm is None else rnorm + r0
             solve_boundary(sysj, delta)
@@ -335,9 +436,15 @@ def kinetic_advance(assemble, solve, mesh, metrics, quad, bcs, refl, o, st, nite
                         o["epsilon"])
         st["density"], st["v"], st["T"], _ = macroparameters(st["f"], cx, cy, cz, w)
         st["nu"] = collision_frequency(st["density"], st["T"], o)
-        if o["fgamma"] == 1:
+        if o["fgamma"] >= 1:
             newton_bgk(st["density"], st["v"], st["T"], st["coeff"], cx, cy, cz, w, o)
         st["feq"] = equilibrium(st["coeff"], st["v"], cx, cy, cz)
+        st["feq_lin"] = st["feq"]
+        if o["fgamma"] == 2:
+            T6 = macroparameters_es(st["f"], st["feq"], st["v"], st["density"], cx, cy, cz, w, o["Prandtl"])
+            st["T6"] = T6
+            newton_esbgk(st["density"], st["v"], T6, st["coeffg"], cx, cy, cz, w, o)
+            st["feq_lin"] = equilibrium_es(st["coeffg"], st["v"], cx, cy, cz)
         if rnorm < abs_tol or ratio < rel_tol:
             return norms, True
     return norms, False

@@ -1,6 +1,7 @@
 """BGK kinetic model on the device (fvm_kinetic_*: every velocity of an iteration
 assembled on the device and solved by one batched AMG; moments, collision
-frequency, Newton-fitted BGK equilibrium and wall values as device kernels)
+frequency, Newton-fitted BGK and ES-BGK equilibria and wall values as device
+kernels)
 against the oracle's loop (oracle/kinetic_ref.py with the C oracle's assembly and
 AMG, the directions one after another).
 
@@ -74,7 +75,7 @@ def _run(fgamma, niter, N=3, jit=0.1, seed=6, q=(4, 4, 4), bcs=BCS, **kw):
     return mesh, st, norms_o, conv_o, model, conv
 
 
-@pytest.mark.parametrize("fgamma", [0, 1])
+@pytest.mark.parametrize("fgamma", [0, 1, 2])
 def test_kinetic_model_matches_oracle_loop(fgamma):
     mesh, st, norms_o, conv_o, model, conv = _run(fgamma, 4)
     print("fgamma", fgamma, "norms", model.norms, flush=True)
@@ -122,10 +123,10 @@ def test_reference_couette_case():
     """The reference's Couette case (test/ESBGK-tests/Couette_10.cas with the
     boundary set-up of Couette/Par_Couette.py:188-206: zone 4 a wall moving at
     0.0297 in x, zone 6 a wall at rest, both at temperature 1, zones 3 and 5
-    "CopyBC"; 10^3 velocities, clim 5.5; rho_init 9.28e-7, Newton tolerances 1e-14),
-    with the BGK equilibrium (the script's ES-BGK is not built): the device loop on
-    the 2-D mesh read by the Fluent reader follows the oracle's, and the gas starts
-    to move with the top wall."""
+    "CopyBC"; 10^3 velocities, clim 5.5; rho_init 9.28e-7, Newton tolerances 1e-14,
+    the ES-BGK equilibrium, fgamma 2): the device loop on the 2-D mesh read by the
+    Fluent reader follows the oracle's, and the gas starts to move with the top
+    wall."""
     import fluent_golden as FG
     from fvm_amd.kinetic import KineticModel, Quadrature
     m = FG.read_mesh("couette_10.cas")
@@ -139,7 +140,7 @@ def test_reference_couette_case():
                    6: ("wall", np.zeros(3), 1.0)}.get(zid, ("copy",))
     q = (10, 10, 10)
     quad = KR.quadrature_cartesian(*q, 5.5, 1.0)
-    opts = dict(fgamma=1, rho_init=9.28e-7, ToleranceX=1e-14, ToleranceF=1e-14)
+    opts = dict(fgamma=2, rho_init=9.28e-7, ToleranceX=1e-14, ToleranceF=1e-14)
     o = dict(KR.DEFAULTS); o.update(opts)
     st = KR.init_state(a["n_cells"] + a["n_ghost"], quad, o)
     assemble, solve = kinetic_parts(a, quad, bcs)

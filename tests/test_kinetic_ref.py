@@ -92,8 +92,43 @@ def test_newton_bgk_conserves_the_discrete_moments():
     assert np.abs(coeff[:, 1] - 1 / T).max() > 1e-4
 
 
+def test_newton_esbgk_conserves_its_moments_and_reduces_to_bgk_at_prandtl_one():
+    """The ES-BGK equilibrium's ten discrete moments equal rho [1, u, u_a u_b + T_ab]
+    for the tensor of a non-equilibrium f; at Prandtl 1 the tensor is the BGK
+    equilibrium's and the ES-BGK equilibrium is the BGK one."""
+    cx, cy, cz, w = KR.quadrature_cartesian(8, 8, 8, 4.5, 1.0)
+    rng = np.random.default_rng(12)
+    n = 6
+    rho = rng.uniform(0.7, 1.5, n); u = rng.uniform(-0.2, 0.2, (n, 3)); T = rng.uniform(0.8, 1.3, n)
+    f = KR.maxwellian(rho, u, T, cx, cy, cz) * (1 + 0.1 * np.sin(np.arange(len(cx)))[:, None])
+    d, v, Tm, _ = KR.macroparameters(f, cx, cy, cz, w)
+    o = _opts(ToleranceF=1e-14, ToleranceX=1e-14)
+    coeff = np.zeros((n, 5)); coeff[:, 0] = d / np.power(KR.PI * Tm, 1.5); coeff[:, 1] = 1 / Tm
+    KR.newton_bgk(d, v, Tm, coeff, cx, cy, cz, w, o)
+    feq = KR.equilibrium(coeff, v, cx, cy, cz)
+    m = np.stack([np.ones_like(cx), cx, cy, cz, cx * cx, cy * cy, cz * cz, cx * cy, cy * cz, cz * cx], 1)
+    for Pr in (2.0 / 3.0, 1.0):
+        T6 = KR.macroparameters_es(f, feq, v, d, cx, cy, cz, w, Pr)
+        cg = np.zeros((n, 10))
+        cg[:, 0] = coeff[:, 0]; cg[:, 1] = coeff[:, 1]; cg[:, 2] = coeff[:, 2]; cg[:, 3] = coeff[:, 1]
+        cg[:, 4] = coeff[:, 3]; cg[:, 5] = coeff[:, 1]; cg[:, 6] = coeff[:, 4]
+        steps = KR.newton_esbgk(d, v, T6, cg, cx, cy, cz, w, o)
+        assert steps.max() <= 10
+        fes = KR.equilibrium_es(cg, v, cx, cy, cz)
+        got = (m * w[:, None]).T @ fes
+        Txx, Tyy, Tzz, Txy, Tyz, Tzx = T6
+        want = np.stack([d, d * v[:, 0], d * v[:, 1], d * v[:, 2], d * (v[:, 0] ** 2 + Txx),
+                         d * (v[:, 1] ** 2 + Tyy), d * (v[:, 2] ** 2 + Tzz), d * (v[:, 0] * v[:, 1] + Txy),
+                         d * (v[:, 1] * v[:, 2] + Tyz), d * (v[:, 2] * v[:, 0] + Tzx)])
+        np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-12)
+        if Pr == 1.0:
+            np.testing.assert_allclose(fes, feq, rtol=1e-9, atol=1e-13)
+        else:
+            assert np.abs(fes - feq).max() > 1e-6
+
+
 def test_collision_frequency_power_law():
-    o = _opts(T_init=300.0, Tmuref=300.0, mu_w=0.81)
+    o = _opts(T_init=300.0, Tmuref=300.0, mu_w=0.81, fgamma=1)
     rho = np.array([1.0, 2.0, 1.0]); T = np.array([1.0, 1.0, 2.0])
     nu = KR.collision_frequency(rho, T, o)
     R = 8314.0 / o["molecularWeight"]
