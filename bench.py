@@ -351,7 +351,7 @@ def apply_knobs(amg, args):
 
 
 def run_kinetic(args):
-    """BGK kinetic model (KineticModel::advance): Couette slab on an N^3 hex mesh,
+    """Kinetic model (KineticModel::advance): Couette slab on an N^3 hex mesh,
     walls at x = 0 and x = 1 moving in -y / +y at temperature 1, specular symmetry on
     the other faces, Q^3 Cartesian velocities (clim 5.5, T2 1), ES-BGK (fgamma 2), the
     kinetic linear solver's defaults (AMG to 1e-15 relative, at most 100 cycles per
@@ -387,7 +387,7 @@ def run_kinetic(args):
         cpu = kinetic_cpu_baseline(N, Q, K, bcs)
         log("cpu: %.1f s per iteration (%s)" % (cpu["value"], cpu["sample"]))
     out = {
-        "metric": "time per outer iteration, BGK kinetic model (all velocities assembled and solved)",
+        "metric": "time per outer iteration, ES-BGK kinetic model (all velocities assembled and solved)",
         "value": round(per, 6), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * per, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (Couette slab, rest-state Maxwellian start)",
