@@ -3,6 +3,8 @@
 // message for fvm_last_error().
 #include "amg.hpp"
 
+#include <climits>
+
 #include <algorithm>
 #include <chrono>
 
@@ -477,6 +479,10 @@ int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value) {
         FVM_CHECK(value == 0 || value == 8 || value == 16 || value == 32 || value == 64, FVM_E_INVALID,
                   "coarsen_lanes must be 0, 8, 16, 32 or 64");
         h->coarsen_lanes = (int)value;
+    } else if (k == "exact_wg_width") {
+        FVM_CHECK(value >= 0 && value <= INT_MAX, FVM_E_INVALID, "exact_wg_width must be in [0, INT_MAX]");
+        h->exact_wg_width = (int)value;
+        fvm::invalidate_cycle_graphs(h);
     } else if (k == "bipartite_levels") {
         FVM_CHECK(value >= 0 && value <= 64, FVM_E_INVALID, "bipartite_levels must be in [0, 64]");
         h->bipartite_levels = (int)value;
@@ -514,6 +520,7 @@ int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value) {
     else if (k == "coarsen_small_lanes") *value = h->coarsen_small_lanes;
     else if (k == "coarsen_grid") *value = h->coarsen_grid;
     else if (k == "bipartite_levels") *value = h->bipartite_levels;
+    else if (k == "exact_wg_width") *value = h->exact_wg_width;
     else return FVM_E_INVALID;
     return FVM_OK;
 }

@@ -106,6 +106,35 @@ __global__ void k_csr_gs_level(const int *__restrict__ ptr, int lvl, const int *
     }
 }
 
+// A whole reference-order sweep of a level in one workgroup: the dependency
+// levels one after another, separated by workgroup barriers instead of launches
+// (the rows of one dependency level never read each other, as in k_csr_gs_level).
+// The level offsets are staged through LDS in chunks.
+constexpr int GSWG_CHUNK = 2048;
+template <class V, class D, class X>
+__global__ void __launch_bounds__(1024)
+k_csr_gs_sweep_wg(int nlev, const int *__restrict__ ptr, const int *__restrict__ rows, const int *__restrict__ row,
+                  const int *__restrict__ col, const V *__restrict__ off, const D *__restrict__ diag, X *x,
+                  const X *__restrict__ b, const uint8_t *__restrict__ mask) {
+    __shared__ int s_ptr[GSWG_CHUNK + 1];
+    for (int c0 = 0; c0 < nlev; c0 += GSWG_CHUNK) {
+        const int cn = min(GSWG_CHUNK, nlev - c0);
+        for (int k = threadIdx.x; k <= cn; k += blockDim.x) s_ptr[k] = ptr[c0 + k];
+        __syncthreads();
+        for (int lv = 0; lv < cn; lv++) {
+            const int beg = s_ptr[lv], end = s_ptr[lv + 1];
+            for (int k = beg + threadIdx.x; k < end; k += blockDim.x) {
+                const int i = rows[k];
+                if (mask && !mask[i]) continue;   // row of a system that has stopped cycling
+                X s = b[i];
+                for (int p = row[i]; p < row[i + 1]; p++) s += (X)ldv(off, p) * x[col[p]];
+                x[i] = -s / (X)diag[i];
+            }
+            __syncthreads();
+        }
+    }
+}
+
 template <class V, class D, class X>
 __global__ void k_csr_jacobi(int n, const int *__restrict__ row, const int *__restrict__ col,
                              const V *__restrict__ off, const D *__restrict__ diag,
@@ -646,10 +675,17 @@ void gs_sweep(Run &R, int l, X *x, const X *b) {
             const int *ptr = dir ? L.bwd_ptr.p : L.fwd_ptr.p;
             const int *rows = dir ? L.bwd_rows.p : L.fwd_rows.p;
             const unsigned g = grid_for(maxw, BS);
-            for (int k = 0; k < nlev; k++) {
+            if (nlev > 0 && maxw <= R.h->exact_wg_width) {
+                // narrow dependency levels: the whole sweep in one workgroup
                 FVM_DISPATCH_VALS(R, l, L, X, (void)0,
-                    (k_csr_gs_level<<<g, BS, 0, s>>>(ptr, k, rows, L.row.p, L.col.p, val, dg, x, b,
-                                                      l == 0 ? R.mask0 : nullptr)));
+                    (k_csr_gs_sweep_wg<<<1, 1024, 0, s>>>(nlev, ptr, rows, L.row.p, L.col.p, val, dg, x, b,
+                                                          l == 0 ? R.mask0 : nullptr)));
+            } else {
+                for (int k = 0; k < nlev; k++) {
+                    FVM_DISPATCH_VALS(R, l, L, X, (void)0,
+                        (k_csr_gs_level<<<g, BS, 0, s>>>(ptr, k, rows, L.row.p, L.col.p, val, dg, x, b,
+                                                          l == 0 ? R.mask0 : nullptr)));
+                }
             }
         }
     }

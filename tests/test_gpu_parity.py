@@ -362,6 +362,25 @@ def test_autotune_keeps_results(prec):
         assert np.array_equal(ls0.delta, ls2.delta), bs
 
 
+@pytest.mark.parametrize("cycle", [0, 1, 2])
+def test_reference_order_sweep_in_one_workgroup_is_bit_identical(cycle):
+    """The reference-order smoother runs narrow sweeps in one workgroup
+    (exact_wg_width) and wide ones level by level; both give the oracle's bits,
+    so every width gives the same delta and history."""
+    m = matgen.grid3d(20, seed=29, contrast=1.5)
+    out = []
+    for width in (0, 64, 4096, 1 << 30):
+        ga = gpu_amg(gsOrdering=0, cycleType=cycle, nMaxIterations=12, relativeTolerance=1e-30)
+        ga.set_tuning("exact_wg_width", width)
+        ls = ls_from(m)
+        ga.solve(ls)
+        out.append((ls.delta.copy(), ga.history()))
+    want = oracle_for(m, n_max_iterations=12, relative_tolerance=1e-30, cycle_type=cycle).solve(m["b"])
+    for d, hist in out:
+        assert np.array_equal(d, want["x"])
+        assert np.array_equal(hist, out[0][1])
+
+
 def test_convergence_lines_match_the_reference_golden(tmp_path):
     """AMG.cpp:237-272 with MultiFieldReduction::print: the lines of the cavity
     solve equal AMG_MERGING_THERMAL/proc1/GOLDEN/convergence.dat (parallel-build
