@@ -141,7 +141,7 @@ struct fvm_amg {
     // tuning knobs (fvm_amg_set_tuning): rows below which a level runs in the
     // single-workgroup tail of the V-cycle / is coarsened by one workgroup
     long long tail_max_rows = 40000;
-    int exact_wg_width = 4096;         // reference-order sweeps whose widest dependency level has at most
+    int exact_wg_width = 1024;         // reference-order sweeps whose widest dependency level has at most
                                        // this many rows run in one workgroup (no result changes)
     long long coarsen_small_rows = 8192;
     int coarsen_lanes = 0;             // lanes per row of the agglomeration rounds (0: by row width)
