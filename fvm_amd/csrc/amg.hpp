@@ -40,6 +40,12 @@ struct Level {
     DArr<int> fwd_ptr, fwd_rows, bwd_ptr, bwd_rows;
     int fwd_nlev = 0, bwd_nlev = 0, fwd_maxw = 0, bwd_maxw = 0;
     std::vector<int> h_fwd_ptr, h_bwd_ptr;
+    // launch plan of a sweep: maximal runs of consecutive dependency levels of at most
+    // exact_wg_width rows share one single-workgroup launch, wider levels get a launch
+    // each (built for runs_width; -1: not built)
+    struct SweepRun { int lvl0, nlev, width; bool wg; };
+    std::vector<SweepRun> fwd_runs, bwd_runs;
+    int runs_width = -1;
     // coloured SELL layout
     bool fast_ready = false;
     int sell_u = 8;              // load depth of the SELL-64 row kernels (mean slice width, rounded up to 8/16/32)
@@ -141,8 +147,8 @@ struct fvm_amg {
     // tuning knobs (fvm_amg_set_tuning): rows below which a level runs in the
     // single-workgroup tail of the V-cycle / is coarsened by one workgroup
     long long tail_max_rows = 40000;
-    int exact_wg_width = 1024;         // reference-order sweeps whose widest dependency level has at most
-                                       // this many rows run in one workgroup (no result changes)
+    int exact_wg_width = 256;          // runs of consecutive dependency levels of at most this many rows
+                                       // sweep in one workgroup launch (no result changes)
     long long coarsen_small_rows = 8192;
     int coarsen_lanes = 0;             // lanes per row of the agglomeration rounds (0: by row width)
     int coarsen_small_lanes = 0;       // the same for the single-workgroup rounds (0: as coarsen_lanes)

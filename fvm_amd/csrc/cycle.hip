@@ -135,6 +135,24 @@ k_csr_gs_sweep_wg(int nlev, const int *__restrict__ ptr, const int *__restrict__
     }
 }
 
+// Launch plan of a reference-order sweep from its dependency-level offsets hp
+// (nlev levels): each maximal run of consecutive levels of at most W rows is one
+// single-workgroup launch; every wider level is a launch of its own.
+void sweep_runs(const std::vector<int> &hp, int nlev, int W, std::vector<Level::SweepRun> &out) {
+    out.clear();
+    for (int k = 0; k < nlev; k++) {
+        const int w = hp[k + 1] - hp[k];
+        if (w > W) {
+            out.push_back({k, 1, w, false});
+        } else if (!out.empty() && out.back().wg) {
+            out.back().nlev++;
+            out.back().width = std::max(out.back().width, w);
+        } else {
+            out.push_back({k, 1, w, true});
+        }
+    }
+}
+
 template <class V, class D, class X>
 __global__ void k_csr_jacobi(int n, const int *__restrict__ row, const int *__restrict__ col,
                              const V *__restrict__ off, const D *__restrict__ diag,
@@ -669,22 +687,26 @@ void gs_sweep(Run &R, int l, X *x, const X *b) {
             if (t0) timing_end(R.h, KT_SWEEP, t0, pass_bytes(value_type(R, l), sizeof(X), 3, rows, nnz));
         }
     } else {
+        if (L.runs_width != R.h->exact_wg_width) {
+            sweep_runs(L.h_fwd_ptr, L.fwd_nlev, R.h->exact_wg_width, L.fwd_runs);
+            sweep_runs(L.h_bwd_ptr, L.bwd_nlev, R.h->exact_wg_width, L.bwd_runs);
+            L.runs_width = R.h->exact_wg_width;
+        }
         for (int dir = 0; dir < 2; dir++) {
-            const int nlev = dir ? L.bwd_nlev : L.fwd_nlev;
-            const int maxw = dir ? L.bwd_maxw : L.fwd_maxw;
             const int *ptr = dir ? L.bwd_ptr.p : L.fwd_ptr.p;
             const int *rows = dir ? L.bwd_rows.p : L.fwd_rows.p;
-            const unsigned g = grid_for(maxw, BS);
-            if (nlev > 0 && maxw <= R.h->exact_wg_width) {
-                // narrow dependency levels: the whole sweep in one workgroup
-                FVM_DISPATCH_VALS(R, l, L, X, (void)0,
-                    (k_csr_gs_sweep_wg<<<1, 1024, 0, s>>>(nlev, ptr, rows, L.row.p, L.col.p, val, dg, x, b,
-                                                          l == 0 ? R.mask0 : nullptr)));
-            } else {
-                for (int k = 0; k < nlev; k++) {
+            for (const Level::SweepRun &run : dir ? L.bwd_runs : L.fwd_runs) {
+                if (run.wg) {
+                    // narrow dependency levels: the run in one workgroup, as wide as its widest level
+                    const int wb = std::min(1024, std::max(64, (run.width + 63) & ~63));
                     FVM_DISPATCH_VALS(R, l, L, X, (void)0,
-                        (k_csr_gs_level<<<g, BS, 0, s>>>(ptr, k, rows, L.row.p, L.col.p, val, dg, x, b,
-                                                          l == 0 ? R.mask0 : nullptr)));
+                        (k_csr_gs_sweep_wg<<<1, wb, 0, s>>>(run.nlev, ptr + run.lvl0, rows, L.row.p, L.col.p, val,
+                                                            dg, x, b, l == 0 ? R.mask0 : nullptr)));
+                } else {
+                    FVM_DISPATCH_VALS(R, l, L, X, (void)0,
+                        (k_csr_gs_level<<<grid_for(run.width, BS), BS, 0, s>>>(ptr, run.lvl0, rows, L.row.p, L.col.p,
+                                                                               val, dg, x, b,
+                                                                               l == 0 ? R.mask0 : nullptr)));
                 }
             }
         }

@@ -1690,6 +1690,7 @@ void prepare_exact_schedule(fvm_amg *h, Level &L) {
         if (dir) { L.bwd_nlev = nl; L.bwd_maxw = maxw; L.h_bwd_ptr = hp; }
         else { L.fwd_nlev = nl; L.fwd_maxw = maxw; L.h_fwd_ptr = hp; }
     }
+    L.runs_width = -1;
     L.sched_ready = true;
 }
 

@@ -209,9 +209,10 @@ int fvm_amg_kernel_bytes(const fvm_amg *h, int kind, double *total_bytes);
  * colours) and keep it when no edge joins equal colours, else the greedy
  * colouring (default 1; takes effect at the next setup). Only the colouring
  * keys change the coloured smoother's arithmetic (another Gauss-Seidel order);
- * the others never change a result; "exact_wg_width" -- reference-order sweeps
- * whose widest dependency level has at most this many rows run in one workgroup
- * (default 1024). */
+ * the others never change a result; "exact_wg_width" -- in reference-order
+ * sweeps, each run of consecutive dependency levels of at most this many rows is
+ * one single-workgroup launch, every wider level a launch of its own (default
+ * 256). */
 int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value);
 int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value);
 /* Autotuner (no counterpart in the reference): times every candidate launch
