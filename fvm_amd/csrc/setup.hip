@@ -120,6 +120,7 @@ __device__ __forceinline__ int atom_add(int *p, int v) {
 // Rows that became ready, buffered per lane and appended to the queue with one
 // atomic per wavefront (the queue tail is a single hot counter).
 constexpr int PB = 4;
+constexpr int FANB = 4;   // dependents per lane whose decrements are in flight together
 struct PushBuf {
     int v[PB];
     int n;
@@ -237,6 +238,98 @@ __device__ __forceinline__ void walk_ranges(int beg, int len, F &&f) {
         const int bj = __shfl(beg, lo, S);
         const int ej = __shfl(exc, lo, S);
         f(lo, k < total ? bj + (k - ej) : -1);
+    }
+}
+
+// Cross-lane steps of a decision without the LDS crossbar: DPP moves inside rows
+// of 16 lanes (gfx9 encodings: pairs within a quad, quads within a half-row and
+// half-rows within a row by mirroring; for a max/min every lane ends with its
+// group's result whatever the pairing), readlane across the rows of a wavefront.
+constexpr int DPP_QUAD_XOR1 = 0xB1;     // quad_perm [1,0,3,2]
+constexpr int DPP_QUAD_XOR2 = 0x4E;     // quad_perm [2,3,0,1]
+constexpr int DPP_HALF_MIRROR = 0x141;  // lane i <-> 7 - i within 8
+constexpr int DPP_MIRROR = 0x140;       // lane i <-> 15 - i within 16
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_mov(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int rl_int(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ double rl_dbl(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)rl_int((int)b, l), hi = (unsigned)rl_int((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Candidate neighbour of a decision: weight and CSR position (INT_MAX: none).
+struct Cand {
+    double w;
+    int p;
+};
+// b replaces a when b exists and a does not, or b is stronger, or as strong and
+// first in CSR order (the sequential loop's strict '>' keeps the first maximum)
+__device__ __forceinline__ Cand better(Cand a, Cand b) {
+    return (b.p != INT_MAX && (a.p == INT_MAX || b.w > a.w || (b.w == a.w && b.p < a.p))) ? b : a;
+}
+template <int CTRL>
+__device__ __forceinline__ Cand dpp_cand(Cand c) {
+    const long long b = __double_as_longlong(c.w);
+    const unsigned lo = (unsigned)dpp_mov<CTRL>((int)b), hi = (unsigned)dpp_mov<CTRL>((int)(b >> 32));
+    Cand o;
+    o.w = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    o.p = dpp_mov<CTRL>(c.p);
+    return o;
+}
+// The best candidate of the S lanes of each sub-group, on all of them.
+template <int S>
+__device__ __forceinline__ Cand reduce_cand(Cand c) {
+    if (S >= 2) c = better(c, dpp_cand<DPP_QUAD_XOR1>(c));
+    if (S >= 4) c = better(c, dpp_cand<DPP_QUAD_XOR2>(c));
+    if (S >= 8) c = better(c, dpp_cand<DPP_HALF_MIRROR>(c));
+    if (S >= 16) c = better(c, dpp_cand<DPP_MIRROR>(c));
+    if (S == 32) {
+        Cand o;
+        o.w = __shfl_xor(c.w, 16, 64);
+        o.p = __shfl_xor(c.p, 16, 64);
+        c = better(c, o);
+    }
+    if (S == 64) {   // every row holds its result: combine the four rows' results
+        Cand r0{rl_dbl(c.w, 0), rl_int(c.p, 0)}, r1{rl_dbl(c.w, 16), rl_int(c.p, 16)};
+        Cand r2{rl_dbl(c.w, 32), rl_int(c.p, 32)}, r3{rl_dbl(c.w, 48), rl_int(c.p, 48)};
+        c = better(better(r0, r1), better(r2, r3));
+    }
+    return c;
+}
+// v of lane l (0..S-1) of this lane's sub-group
+template <int S>
+__device__ __forceinline__ int sub_get(int v, int l) {
+    if (S == 64) return rl_int(v, l);
+    return __shfl(v, l, S);
+}
+
+// Visit the rows adjacent (through the transpose pattern) to the members of a
+// group, S at a time: lane j < MEMW of the sub-group holds member j's range
+// [tb, tb + tl) of colT (tl = 0: no member). The ranges are gathered to every
+// lane at once and each lane finds its element's owner by comparisons, so a
+// step costs one cross-lane latency instead of a scan and a search.
+template <int S, class F>
+__device__ __forceinline__ void walk_members(int tb, int tl, F &&f) {
+    const int lane = threadIdx.x & (S - 1);
+    int mb[MEMW], me[MEMW];
+    int total = 0;
+#pragma unroll
+    for (int j = 0; j < MEMW; j++) {
+        mb[j] = sub_get<S>(tb, j) - total;   // element k of member j is colT[mb[j] + k]
+        total += sub_get<S>(tl, j);
+        me[j] = total;
+    }
+    for (int k0 = 0; k0 < total; k0 += S) {
+        const int k = k0 + lane;
+        int idx = -1;
+#pragma unroll
+        for (int j = MEMW - 1; j >= 0; j--)
+            if (k < me[j]) idx = mb[j] + k;
+        f(k < total ? idx : -1);
     }
 }
 
@@ -374,8 +467,8 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
     if (dr < t) return;   // already paired by an earlier seed (its fan-out is done)
     const int pre = db + lane < de ? dep[db + lane] : 0;   // first S dependents of r
     // strongest ungrouped (u) and grouped (g) neighbours; ties: first in CSR order
-    double mu = -1.0, mg = -1.0;
-    int pu = INT_MAX, pg = INT_MAX, cu = -1, cg = -1, ub = 0, ue = 0, lg = -1;
+    Cand bu{-1.0, INT_MAX}, bg{-1.0, INT_MAX};
+    int cu = -1, cg = -1, ub = 0, ue = 0, lg = -1;
     for (int p = rb + lane; p < re; p += S) {
         const int c = col[p];
         const double o = off[p];
@@ -386,27 +479,27 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
         const int cb = rc.db, ce = rc.de, lc = rc.ldr;
         const double w = fabs(fabs(o) / (d0 < d1 ? d1 : d0));
         if (!(dc < t)) {
-            if (pu == INT_MAX || w > mu) { mu = w; pu = p; cu = c; ub = cb; ue = ce; }
+            if (bu.p == INT_MAX || w > bu.w) { bu.w = w; bu.p = p; cu = c; ub = cb; ue = ce; }
         } else {
-            if (pg == INT_MAX || w > mg) { mg = w; pg = p; cg = c; lg = lc; }
+            if (bg.p == INT_MAX || w > bg.w) { bg.w = w; bg.p = p; cg = c; lg = lc; }
         }
     }
-    for (int o = S / 2; o > 0; o >>= 1) {
-        const double mu2 = __shfl_xor(mu, o, 64);
-        const int pu2 = __shfl_xor(pu, o, 64);
-        const int cu2 = __shfl_xor(cu, o, 64);
-        const int ub2 = __shfl_xor(ub, o, 64);
-        const int ue2 = __shfl_xor(ue, o, 64);
-        if (pu2 != INT_MAX && (pu == INT_MAX || mu2 > mu || (mu2 == mu && pu2 < pu))) {
-            mu = mu2; pu = pu2; cu = cu2; ub = ub2; ue = ue2;
-        }
-        const double mg2 = __shfl_xor(mg, o, 64);
-        const int pg2 = __shfl_xor(pg, o, 64);
-        const int cg2 = __shfl_xor(cg, o, 64);
-        const int lg2 = __shfl_xor(lg, o, 64);
-        if (pg2 != INT_MAX && (pg == INT_MAX || mg2 > mg || (mg2 == mg && pg2 < pg))) {
-            mg = mg2; pg = pg2; cg = cg2; lg = lg2;
-        }
+    // the sub-group's best of each kind; the lane that holds it (position p is
+    // taken by lane (p - rb) mod S) hands out the neighbour's data
+    bu = reduce_cand<S>(bu);
+    bg = reduce_cand<S>(bg);
+    const double mu = bu.w, mg = bg.w;
+    if (bu.p != INT_MAX) {
+        const int l = (bu.p - rb) & (S - 1);
+        cu = sub_get<S>(cu, l); ub = sub_get<S>(ub, l); ue = sub_get<S>(ue, l);
+    } else {
+        cu = -1;
+    }
+    if (bg.p != INT_MAX) {
+        const int l = (bg.p - rb) & (S - 1);
+        cg = sub_get<S>(cg, l); lg = sub_get<S>(lg, l);
+    } else {
+        cg = -1;
     }
     int action;  // 0 pair, 1 new singleton group, 2 join cg's group, 3 retry next round
     int g = -1, slot = 0;
@@ -433,7 +526,7 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
         }
         const int count = __popcll(__ballot(is_mem) & sub_mask<S>());
         bool blocked = false;
-        walk_ranges<S>(tb, tl, [&](int, int idx) {
+        walk_members<S>(tb, tl, [&](int idx) {
             if (idx >= 0) {
                 const int s = colT[idx];
                 if (s < r && !isb(bnd, s) && !(rec[s].dround < t)) blocked = true;
@@ -470,19 +563,27 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
             rec[r].dround = t;
         }
     }
-    // fan-out: the dependents of r and, for a pair, of its partner
+    // fan-out: the dependents of r and, for a pair, of its partner. A lane issues
+    // the decrements of up to FANB dependents before it waits for any, so a long
+    // list costs one atomic round trip per FANB * S entries instead of per S.
     const int l0 = de - db, lt = l0 + (action == 0 ? ue - ub : 0);
-    for (int base = 0; base < lt; base += S) {
-        const int k = base + lane;
-        bool want = false;
-        int x = -1;
-        if (k < lt) {
-            const int e = k < l0 ? (base == 0 ? pre : dep[db + k]) : dep[ub + (k - l0)];
-            x = e & 0x7fffffff;
-            const int dec = 1 + (int)((unsigned)e >> 31);
-            want = (atom_add<WG>(&cnt[x], -dec) == dec);
+    for (int base = 0; base < lt; base += FANB * S) {
+        int x[FANB], dec[FANB], old[FANB];
+#pragma unroll
+        for (int u = 0; u < FANB; u++) {
+            const int k = base + u * S + lane;
+            x[u] = -1;
+            dec[u] = 0;
+            if (k < lt) {
+                const int e = k < l0 ? (k < S ? pre : dep[db + k]) : dep[ub + (k - l0)];
+                x[u] = e & 0x7fffffff;
+                dec[u] = 1 + (int)((unsigned)e >> 31);
+            }
         }
-        pb_push<WG>(pb, want, x, q, qcnt);
+#pragma unroll
+        for (int u = 0; u < FANB; u++) old[u] = x[u] >= 0 ? atom_add<WG>(&cnt[x[u]], -dec[u]) : 0;
+#pragma unroll
+        for (int u = 0; u < FANB; u++) pb_push<WG>(pb, x[u] >= 0 && old[u] == dec[u], x[u], q, qcnt);
     }
     pb_flush<WG>(pb, q, qcnt);
 }
