@@ -149,6 +149,10 @@ struct fvm_amg {
     long long tail_max_rows = 40000;
     int exact_wg_width = 256;          // runs of consecutive dependency levels of at most this many rows
                                        // sweep in one workgroup launch (no result changes)
+    int exact_flow = 1;                // reference-order sweeps as one dataflow launch per direction
+    fvm::DArr<int> flow_ctl;           // its block tickets, finished blocks and error flag
+    bool flow_used = false;            // a dataflow sweep ran since the last check
+    long long flow_ticks = 0;          // two seconds of the device's wall clock
     long long coarsen_small_rows = 8192;
     int coarsen_lanes = 0;             // lanes per row of the agglomeration rounds (0: by row width)
     int coarsen_small_lanes = 0;       // the same for the single-workgroup rounds (0: as coarsen_lanes)

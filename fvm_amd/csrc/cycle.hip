@@ -135,6 +135,180 @@ k_csr_gs_sweep_wg(int nlev, const int *__restrict__ ptr, const int *__restrict__
     }
 }
 
+// ---------------------------------------------------------- reference order as dataflow
+// A reference-order Gauss-Seidel sweep in one launch: thread k updates row
+// rows[k] of the dependency-level schedule (rows of earlier levels first) as soon
+// as every neighbour it reads the new value of has published its own. The new
+// values go to a separate array whose words are first set to a "pending" NaN
+// that the arithmetic never produces (a NaN result is stored as the canonical
+// quiet NaN), so one word carries both the value and its readiness: relaxed
+// device-scope atomic loads and stores of that word need no fence. The
+// neighbours whose old values a row reads are only read (from the input array).
+// Blocks take schedule chunks in the order they start (a ticket counter), so a
+// row waits only on rows of blocks already running or done: every wait ends.
+// A wave that sees no block finish for two seconds gives up and raises an error
+// flag (a broken schedule would otherwise spin forever); every wave that sees
+// the flag stops at once, and blocks that start after it skip their rows. The
+// pending words left behind are NaN, so the result is visibly invalid as well.
+template <class X> struct FlowWord;
+template <> struct FlowWord<double> {
+    using U = unsigned long long;
+    static constexpr U pend = 0x7FF4A5A5A5A5A5A5ull;
+    static constexpr U qnan = 0x7FF8000000000000ull;
+    static __device__ __forceinline__ U bits(double v) { return (U)__double_as_longlong(v); }
+    static __device__ __forceinline__ double val(U u) { return __longlong_as_double((long long)u); }
+};
+template <> struct FlowWord<float> {
+    using U = unsigned int;
+    static constexpr U pend = 0x7FA5A5A5u;
+    static constexpr U qnan = 0x7FC00000u;
+    static __device__ __forceinline__ U bits(float v) { return __float_as_uint(v); }
+    static __device__ __forceinline__ float val(U u) { return __uint_as_float(u); }
+};
+template <class X>
+__device__ __forceinline__ typename FlowWord<X>::U flow_load(const X *p) {
+    return __hip_atomic_load((const typename FlowWord<X>::U *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class X>
+__device__ __forceinline__ void flow_store(X *p, X v) {
+    using W = FlowWord<X>;
+    const typename W::U u = v != v ? W::qnan : W::bits(v);
+    __hip_atomic_store((typename W::U *)p, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// pending words for the sweep's output; ctl[0] (tickets) and ctl[1] (finished
+// blocks) to zero; ctl[2] (error) is kept until the host reads it
+template <class X>
+__global__ void k_flow_prep(int n, X *xn, int *ctl) {
+    if (blockIdx.x == 0 && threadIdx.x < 2) ctl[threadIdx.x] = 0;
+    using W = FlowWord<X>;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        ((typename W::U *)xn)[i] = W::pend;
+}
+
+constexpr int FLOW_BS = 256;
+constexpr int FLOW_RC = 16;   // entries of a row kept in registers while it waits
+// Waiting rows poll the one new value that usually comes last (the nearest row in
+// sweep order), then all of them; a wavefront with nothing to do naps between
+// polls. Measured on the 200^3 hierarchy (ms per reference-order V-cycle): 96.2
+// with up to 32 naps, 97.7 with 8, 103.4 without; polling every new value each
+// time: 107-110 (the polls' own traffic slows the rows that are ready).
+constexpr int FLOW_NAPS = 32;
+template <class V, class D, class X>
+__global__ void __launch_bounds__(FLOW_BS)
+k_csr_gs_flow(int n, int dir, const int *__restrict__ rows, const int *__restrict__ row, const int *__restrict__ col,
+              const V *__restrict__ off, const D *__restrict__ diag, const X *__restrict__ xo, X *xn,
+              const X *__restrict__ b, const uint8_t *__restrict__ mask, int *ctl, long long give_up) {
+    using W = FlowWord<X>;
+    __shared__ int s_chunk, s_abort;
+    if (threadIdx.x == 0) {
+        s_chunk = atomicAdd(&ctl[0], 1);
+        s_abort = __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const int k = s_abort ? n : s_chunk * FLOW_BS + threadIdx.x;
+    int i = 0, rb = 0, re = 0;
+    bool todo = false;
+    X bi = 0, di = 1;
+    int c[FLOW_RC];
+    X v[FLOW_RC], xv[FLOW_RC];
+    unsigned lowm = 0;   // entries (of the first FLOW_RC) that read a new value
+    int key = -1;        // the new value that usually comes last: polled alone until it is there
+    if (k < n) {
+        i = rows[k];
+        if (mask && !mask[i]) {
+            flow_store(xn + i, xo[i]);   // row of a system that has stopped cycling
+        } else {
+            todo = true;
+            rb = row[i];
+            re = row[i + 1];
+            bi = b[i];
+            di = (X)diag[i];
+#pragma unroll
+            for (int u = 0; u < FLOW_RC; u++) {
+                c[u] = 0;
+                v[u] = 0;
+                xv[u] = 0;
+                if (rb + u < re) {
+                    c[u] = col[rb + u];
+                    v[u] = (X)ldv(off, rb + u);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < FLOW_RC; u++)
+                if (rb + u < re) {
+                    if (dir ? c[u] > i : c[u] < i) {
+                        lowm |= 1u << u;
+                        // nearest in sweep order: the largest lower (forward) / smallest higher (backward) row
+                        if (key < 0 || (dir ? c[u] < key : c[u] > key)) key = c[u];
+                    } else {
+                        xv[u] = xo[c[u]];
+                    }
+                }
+        }
+    }
+    int spins = 0, seen = -1, nap = 0;
+    long long t_seen = 0;
+    while (__ballot(todo)) {
+        bool moved = false;
+        if (todo && key >= 0 && flow_load(xn + key) != W::pend) key = -1;
+        if (todo && key < 0) {
+            // the new values this row reads: all of them published?
+            bool ready = true;
+#pragma unroll
+            for (int u = 0; u < FLOW_RC; u++)
+                if (lowm >> u & 1u) {
+                    const typename W::U w = flow_load(xn + c[u]);
+                    ready = ready && w != W::pend;
+                    xv[u] = W::val(w);
+                }
+            for (int p = rb + FLOW_RC; p < re && ready; p++) {
+                const int j = col[p];
+                if (dir ? j > i : j < i) ready = flow_load(xn + j) != W::pend;
+            }
+            if (ready) {
+                // the reference's arithmetic: b, then the entries in CSR order, over -diag
+                X sacc = bi;
+#pragma unroll
+                for (int u = 0; u < FLOW_RC; u++)
+                    if (rb + u < re) sacc += v[u] * xv[u];
+                for (int p = rb + FLOW_RC; p < re; p++) {
+                    const int j = col[p];
+                    const X xj = (dir ? j > i : j < i) ? W::val(flow_load(xn + j)) : xo[j];
+                    sacc += (X)ldv(off, p) * xj;
+                }
+                flow_store(xn + i, -sacc / di);
+                todo = false;
+                moved = true;
+            }
+        }
+        if (__ballot(todo)) {
+            // back off while nothing in the wavefront moves (up to FLOW_NAPS naps of 128 cycles)
+            nap = __ballot(moved) ? 0 : (nap < FLOW_NAPS ? nap + 1 : FLOW_NAPS);
+            for (int z = 0; z < nap; z++) __builtin_amdgcn_s_sleep(2);
+            if (++spins == 4096) {   // every few milliseconds: has any block finished since?
+                spins = 0;
+                // one lane's view decides for the whole wavefront
+                const int pr = __builtin_amdgcn_readfirstlane(
+                    __hip_atomic_load(&ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                const int ab = __builtin_amdgcn_readfirstlane(
+                    __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                const long long now = (long long)wall_clock64();
+                if (ab) break;   // another wavefront gave up
+                if (pr != seen) {
+                    seen = pr;
+                    t_seen = now;
+                } else if (now - t_seen > give_up) {
+                    if (__lane_id() == 0) atomicOr(&ctl[2], 1);
+                    break;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&ctl[1], 1);
+}
+
 // Launch plan of a reference-order sweep from its dependency-level offsets hp
 // (nlev levels): each maximal run of consecutive levels of at most W rows is one
 // single-workgroup launch; every wider level is a launch of its own.
@@ -686,6 +860,23 @@ void gs_sweep(Run &R, int l, X *x, const X *b) {
             // b read, x read and written (the neighbours' x are counted once per updated row)
             if (t0) timing_end(R.h, KT_SWEEP, t0, pass_bytes(value_type(R, l), sizeof(X), 3, rows, nnz));
         }
+    } else if (R.h->exact_flow) {
+        fvm_amg *h = R.h;
+        FVM_CHECK(h->flow_ctl.p, FVM_E_INVALID, "reference-order sweep before its control words exist");
+        h->flow_used = true;
+        DArr<X> &xa = vx_arr<X>(L);
+        DArr<X> &ta = vt_arr<X>(L);
+        FVM_CHECK(x == xa.p, FVM_E_INVALID, "reference-order sweep on a vector other than the level's own");
+        const int n = L.n;
+        for (int dir = 0; dir < 2; dir++) {
+            const int *rows = dir ? L.bwd_rows.p : L.fwd_rows.p;
+            k_flow_prep<X><<<grid_for(n, BS, 4096), BS, 0, s>>>(n, ta.p, h->flow_ctl.p);
+            FVM_DISPATCH_VALS(R, l, L, X, (void)0,
+                (k_csr_gs_flow<<<(unsigned)((n + FLOW_BS - 1) / FLOW_BS), FLOW_BS, 0, s>>>(
+                    n, dir, rows, L.row.p, L.col.p, val, dg, xa.p, ta.p, b, l == 0 ? R.mask0 : nullptr,
+                    h->flow_ctl.p, h->flow_ticks)));
+            std::swap(xa, ta);   // the new values become x (two swaps per sweep: the arrays end where they began)
+        }
     } else {
         if (L.runs_width != R.h->exact_wg_width) {
             sweep_runs(L.h_fwd_ptr, L.fwd_nlev, R.h->exact_wg_width, L.fwd_runs);
@@ -920,6 +1111,13 @@ Run make_run(fvm_amg *h) {
     if (R.fast) prepare_fast(h);
     else if (h->prm.smootherType == FVM_GAUSS_SEIDEL)
         for (auto &L : h->lev) prepare_exact_schedule(h, L);
+    if (!R.fast && h->exact_flow && !h->flow_ctl.p) {   // before any graph capture of the cycle
+        h->flow_ctl.alloc(4);
+        FVM_HIP(hipMemsetAsync(h->flow_ctl.p, 0, h->flow_ctl.bytes(), h->stream));
+        int khz = 0;
+        FVM_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, h->device));
+        h->flow_ticks = 2000LL * (khz > 0 ? khz : 100000);
+    }
     if (R.mixed) {
         prepare_mixed(h, R.fast);
         if (h->direct_level < 0 && !h->direct_tried) prepare_direct(h);
@@ -990,6 +1188,19 @@ void store_finest(Run &R, double *d_x) {
 
 }  // namespace
 
+// After a solve's last synchronisation: a dataflow sweep that gave up (see
+// k_csr_gs_flow) makes the call fail.
+void flow_check(fvm_amg *h) {
+    if (!h->flow_used) return;
+    h->flow_used = false;
+    const int e = read_scalar(h->flow_ctl.p + 2, h->stream);
+    if (e) {
+        FVM_HIP(hipMemsetAsync(h->flow_ctl.p + 2, 0, sizeof(int), h->stream));
+        throw Error(FVM_E_NUMERIC, "a reference-order Gauss-Seidel sweep stopped waiting for a row that was never "
+                                   "updated (dependency schedule inconsistent with the matrix)");
+    }
+}
+
 void solve_device(fvm_amg *h, const double *d_b, double *d_x, double *rnorm0, double *rnorm_final,
                   int *cycles) {
     Run R = make_run(h);
@@ -1020,6 +1231,7 @@ void solve_device(fvm_amg *h, const double *d_b, double *d_x, double *rnorm0, do
     }
     store_finest(R, d_x);
     FVM_HIP(hipStreamSynchronize(h->stream));
+    flow_check(h);
     timing_harvest(h);
     if (rnorm0) *rnorm0 = r0;
     if (rnorm_final) *rnorm_final = rn;
@@ -1032,6 +1244,7 @@ void smooth_device(fvm_amg *h, const double *d_b, double *d_x) {
     one_cycle(R);
     store_finest(R, d_x);
     FVM_HIP(hipStreamSynchronize(h->stream));
+    flow_check(h);
 }
 
 // x = -A^{-1} b on the direct level with the cycle's own kernel, rows in the
@@ -1515,6 +1728,7 @@ void batch_solve(fvm_amg *h, BatchInfo &B, const double *const *d_b, double *con
     if (rnorm_final) FVM_HIP(hipMemcpyAsync(rnorm_final, B.rn.p, sizeof(double) * K, hipMemcpyDeviceToHost, s));
     if (cycles) FVM_HIP(hipMemcpyAsync(cycles, B.cyc.p, sizeof(int) * K, hipMemcpyDeviceToHost, s));
     FVM_HIP(hipStreamSynchronize(s));
+    flow_check(h);
 }
 
 
@@ -1926,6 +2140,7 @@ void fgmres_solve(fvm_amg *h, KrylovWork &W, const fvm_fgmres_params &p, const d
     }
     k_scatter<<<grid_for(n, BS), BS, 0, s>>>(n, perm, W.x.p, d_x);
     FVM_HIP(hipStreamSynchronize(s));
+    flow_check(h);
     timing_harvest(h);
     if (rnorm0) *rnorm0 = r0;
     if (rnorm_final) *rnorm_final = rn;
@@ -2043,6 +2258,7 @@ void bcgstab_solve(fvm_amg *h, const fvm_linsolver_params &p, const double *d_b,
     if (n > 0) k_scatter<<<gb, BS, 0, s>>>(n, perm, x.p, d_x);
     FVM_HIP(hipGetLastError());
     FVM_HIP(hipStreamSynchronize(s));
+    flow_check(h);
     if (rnorm0) *rnorm0 = r0;
     if (rnorm_final) *rnorm_final = rf;
     if (iterations) *iterations = it;
@@ -2117,6 +2333,7 @@ void cg_solve(fvm_amg *h, const fvm_linsolver_params &p, const double *d_b, doub
     if (n > 0) k_scatter<<<gb, BS, 0, s>>>(n, perm, x.p, d_x);
     FVM_HIP(hipGetLastError());
     FVM_HIP(hipStreamSynchronize(s));
+    flow_check(h);
     if (rnorm0) *rnorm0 = r0;
     if (rnorm_final) *rnorm_final = rf;
     if (iterations) *iterations = it;

@@ -57,7 +57,7 @@ enum { FVM_V_CYCLE = 0, FVM_W_CYCLE = 1, FVM_F_CYCLE = 2 };
 enum { FVM_GAUSS_SEIDEL = 0, FVM_JACOBI = 1 };
 /* Gauss-Seidel row order (no counterpart in the reference, which is sequential) */
 enum {
-    FVM_GS_EXACT = 0,      /* reference row order, level-scheduled: bit-identical to the reference */
+    FVM_GS_EXACT = 0,      /* reference row order (dependency-ordered dataflow): bit-identical to the reference */
     FVM_GS_MULTICOLOR = 1  /* graph-coloured order: a different (parallel) Gauss-Seidel */
 };
 /* arithmetic of the V-cycle (no counterpart in the reference, which is FP64 only) */
@@ -209,10 +209,11 @@ int fvm_amg_kernel_bytes(const fvm_amg *h, int kind, double *total_bytes);
  * colours) and keep it when no edge joins equal colours, else the greedy
  * colouring (default 1; takes effect at the next setup). Only the colouring
  * keys change the coloured smoother's arithmetic (another Gauss-Seidel order);
- * the others never change a result; "exact_wg_width" -- in reference-order
- * sweeps, each run of consecutive dependency levels of at most this many rows is
- * one single-workgroup launch, every wider level a launch of its own (default
- * 256). */
+ * the others never change a result; "exact_flow" -- reference-order sweeps as
+ * one dataflow launch per direction (default 1; 0: level by level);
+ * "exact_wg_width" -- in level-by-level reference-order sweeps, each run of
+ * consecutive dependency levels of at most this many rows is one
+ * single-workgroup launch, every wider level a launch of its own (default 256). */
 int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value);
 int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value);
 /* Autotuner (no counterpart in the reference): times every candidate launch

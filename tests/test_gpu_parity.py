@@ -364,14 +364,18 @@ def test_autotune_keeps_results(prec):
 
 @pytest.mark.parametrize("cycle", [0, 1, 2])
 def test_reference_order_sweep_in_one_workgroup_is_bit_identical(cycle):
-    """The reference-order smoother runs each stretch of narrow dependency levels
-    (at most exact_wg_width rows) in one workgroup and wider levels one launch
-    each; every split gives the oracle's bits, so every width gives the same
-    delta and history (8, 64, 200: sweeps that mix both launch kinds)."""
+    """The reference-order smoother runs as one dataflow launch per sweep
+    direction (exact_flow 1: each row updates once the rows it reads new values
+    of have published theirs), or level by level (exact_flow 0): each stretch of
+    narrow dependency levels (at most exact_wg_width rows) in one workgroup and
+    wider levels one launch each. Every variant gives the oracle's bits, so all
+    give the same delta and history (widths 8, 64, 200: sweeps that mix both
+    launch kinds)."""
     m = matgen.grid3d(20, seed=29, contrast=1.5)
     out = []
-    for width in (0, 8, 64, 200, 4096, 1 << 30):
+    for flow, width in ((1, 256), (0, 0), (0, 8), (0, 64), (0, 200), (0, 4096), (0, 1 << 30)):
         ga = gpu_amg(gsOrdering=0, cycleType=cycle, nMaxIterations=12, relativeTolerance=1e-30)
+        ga.set_tuning("exact_flow", flow)
         ga.set_tuning("exact_wg_width", width)
         ls = ls_from(m)
         ga.solve(ls)
