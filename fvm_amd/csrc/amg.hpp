@@ -153,6 +153,7 @@ struct fvm_amg {
     fvm::DArr<int> flow_ctl;           // its block tickets, finished blocks and error flag
     bool flow_used = false;            // a dataflow sweep ran since the last check
     long long flow_ticks = 0;          // two seconds of the device's wall clock
+    int flow_spin_limit = 0;           // waits per wavefront before a sweep gives up (0: no bound)
     long long coarsen_small_rows = 8192;
     int coarsen_lanes = 0;             // lanes per row of the agglomeration rounds (0: by row width)
     int coarsen_small_lanes = 0;       // the same for the single-workgroup rounds (0: as coarsen_lanes)

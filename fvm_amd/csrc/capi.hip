@@ -485,6 +485,9 @@ int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value) {
         fvm::invalidate_cycle_graphs(h);
     } else if (k == "exact_flow") {
         h->exact_flow = value != 0;
+    } else if (k == "exact_flow_spin_limit") {
+        FVM_CHECK(value >= 0 && value <= INT_MAX, FVM_E_INVALID, "exact_flow_spin_limit must be in [0, INT_MAX]");
+        h->flow_spin_limit = (int)value;
 
     } else if (k == "bipartite_levels") {
         FVM_CHECK(value >= 0 && value <= 64, FVM_E_INVALID, "bipartite_levels must be in [0, 64]");
@@ -525,6 +528,7 @@ int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value) {
     else if (k == "bipartite_levels") *value = h->bipartite_levels;
     else if (k == "exact_wg_width") *value = h->exact_wg_width;
     else if (k == "exact_flow") *value = h->exact_flow;
+    else if (k == "exact_flow_spin_limit") *value = h->flow_spin_limit;
     else return FVM_E_INVALID;
     return FVM_OK;
 }

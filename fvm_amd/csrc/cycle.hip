@@ -198,7 +198,8 @@ template <class V, class D, class X>
 __global__ void __launch_bounds__(FLOW_BS)
 k_csr_gs_flow(int n, int dir, const int *__restrict__ rows, const int *__restrict__ row, const int *__restrict__ col,
               const V *__restrict__ off, const D *__restrict__ diag, const X *__restrict__ xo, X *xn,
-              const X *__restrict__ b, const uint8_t *__restrict__ mask, int *ctl, long long give_up) {
+              const X *__restrict__ b, const uint8_t *__restrict__ mask, int *ctl, long long give_up,
+              int spin_limit) {
     using W = FlowWord<X>;
     __shared__ int s_chunk, s_abort;
     if (threadIdx.x == 0) {
@@ -247,7 +248,7 @@ k_csr_gs_flow(int n, int dir, const int *__restrict__ rows, const int *__restric
                 }
         }
     }
-    int spins = 0, seen = -1, nap = 0;
+    int spins = 0, seen = -1, nap = 0, waits = 0;
     long long t_seen = 0;
     while (__ballot(todo)) {
         bool moved = false;
@@ -285,6 +286,10 @@ k_csr_gs_flow(int n, int dir, const int *__restrict__ rows, const int *__restric
         if (__ballot(todo)) {
             // back off while nothing in the wavefront moves (up to FLOW_NAPS naps of 128 cycles)
             nap = __ballot(moved) ? 0 : (nap < FLOW_NAPS ? nap + 1 : FLOW_NAPS);
+            if (spin_limit > 0 && ++waits >= spin_limit) {   // a caller's bound on the waits (0: none)
+                if (__lane_id() == 0) atomicOr(&ctl[2], 1);
+                break;
+            }
             for (int z = 0; z < nap; z++) __builtin_amdgcn_s_sleep(2);
             if (++spins == 4096) {   // every few milliseconds: has any block finished since?
                 spins = 0;
@@ -874,7 +879,7 @@ void gs_sweep(Run &R, int l, X *x, const X *b) {
             FVM_DISPATCH_VALS(R, l, L, X, (void)0,
                 (k_csr_gs_flow<<<(unsigned)((n + FLOW_BS - 1) / FLOW_BS), FLOW_BS, 0, s>>>(
                     n, dir, rows, L.row.p, L.col.p, val, dg, xa.p, ta.p, b, l == 0 ? R.mask0 : nullptr,
-                    h->flow_ctl.p, h->flow_ticks)));
+                    h->flow_ctl.p, h->flow_ticks, h->flow_spin_limit)));
             std::swap(xa, ta);   // the new values become x (two swaps per sweep: the arrays end where they began)
         }
     } else {
@@ -1215,6 +1220,7 @@ void solve_device(fvm_amg *h, const double *d_b, double *d_x, double *rnorm0, do
             h->total_iterations++;
             one_cycle(R);
             rn = finest_residual_norm(R);
+            if (!std::isfinite(rn)) flow_check(h);   // a sweep that gave up is the more precise cause
             FVM_CHECK(std::isfinite(rn), FVM_E_NUMERIC,
                       "the cycle produced a non-finite residual (cycle " + std::to_string(i) +
                           "): the system holds non-finite or zero-diagonal entries, or the cycle diverged");

@@ -211,6 +211,9 @@ int fvm_amg_kernel_bytes(const fvm_amg *h, int kind, double *total_bytes);
  * keys change the coloured smoother's arithmetic (another Gauss-Seidel order);
  * the others never change a result; "exact_flow" -- reference-order sweeps as
  * one dataflow launch per direction (default 1; 0: level by level);
+ * "exact_flow_spin_limit" -- waits per wavefront after which such a sweep gives
+ * up and the call fails with FVM_E_NUMERIC (default 0: no bound; a sweep that
+ * sees no progress for two seconds gives up regardless);
  * "exact_wg_width" -- in level-by-level reference-order sweeps, each run of
  * consecutive dependency levels of at most this many rows is one
  * single-workgroup launch, every wider level a launch of its own (default 256). */

@@ -386,6 +386,26 @@ def test_reference_order_sweep_in_one_workgroup_is_bit_identical(cycle):
         assert np.array_equal(hist, out[0][1])
 
 
+def test_dataflow_sweep_bounded_wait_fails_loudly_then_recovers():
+    """A reference-order dataflow sweep whose wavefronts may wait only once
+    (exact_flow_spin_limit 1) gives up through its own bounded wait: the call
+    fails with the sweep's error, every kernel exits normally, and the next call
+    without the bound gives the oracle's bits again."""
+    from fvm_amd._lib import FVM_E_NUMERIC, FvmError
+    m = matgen.grid3d(10, seed=30)
+    ga = gpu_amg(gsOrdering=0, nMaxIterations=5, relativeTolerance=1e-30)
+    ga.set_tuning("exact_flow_spin_limit", 1)
+    for call in ("smooth", "solve"):
+        with pytest.raises(FvmError, match="stopped waiting") as e:
+            getattr(ga, call)(ls_from(m))
+        assert e.value.code == FVM_E_NUMERIC
+    ga.set_tuning("exact_flow_spin_limit", 0)
+    ls = ls_from(m)
+    ga.solve(ls)
+    want = oracle_for(m, n_max_iterations=5, relative_tolerance=1e-30).solve(m["b"])
+    assert np.array_equal(ls.delta, want["x"])
+
+
 def test_convergence_lines_match_the_reference_golden(tmp_path):
     """AMG.cpp:237-272 with MultiFieldReduction::print: the lines of the cavity
     solve equal AMG_MERGING_THERMAL/proc1/GOLDEN/convergence.dat (parallel-build
