@@ -308,11 +308,13 @@ __device__ __forceinline__ int sub_get(int v, int l) {
 }
 
 // Visit the rows adjacent (through the transpose pattern) to the members of a
-// group, S at a time: lane j < MEMW of the sub-group holds member j's range
+// group, WB * S at a time: lane j < MEMW of the sub-group holds member j's range
 // [tb, tb + tl) of colT (tl = 0: no member). The ranges are gathered to every
-// lane at once and each lane finds its element's owner by comparisons, so a
-// step costs one cross-lane latency instead of a scan and a search.
-template <int S, class F>
+// lane at once and each lane finds its elements' owners by comparisons, so a
+// step costs one cross-lane latency instead of a scan and a search; f gets WB
+// element indices per lane (-1: none), whose loads it can issue together.
+constexpr int JWB = 4;
+template <int S, int WB, class F>
 __device__ __forceinline__ void walk_members(int tb, int tl, F &&f) {
     const int lane = threadIdx.x & (S - 1);
     int mb[MEMW], me[MEMW];
@@ -323,13 +325,18 @@ __device__ __forceinline__ void walk_members(int tb, int tl, F &&f) {
         total += sub_get<S>(tl, j);
         me[j] = total;
     }
-    for (int k0 = 0; k0 < total; k0 += S) {
-        const int k = k0 + lane;
-        int idx = -1;
+    for (int k0 = 0; k0 < total; k0 += WB * S) {
+        int idx[WB];
 #pragma unroll
-        for (int j = MEMW - 1; j >= 0; j--)
-            if (k < me[j]) idx = mb[j] + k;
-        f(k < total ? idx : -1);
+        for (int w = 0; w < WB; w++) {
+            const int k = k0 + w * S + lane;
+            int id = -1;
+#pragma unroll
+            for (int j = MEMW - 1; j >= 0; j--)
+                if (k < me[j]) id = mb[j] + k;
+            idx[w] = k < total ? id : -1;
+        }
+        f(idx);
     }
 }
 
@@ -517,8 +524,8 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
         if (lane == 0) slot = mcnt[g];
         if (lane < MEMW) {
             const size_t e = (size_t)g * MEMW + lane;
-            if (memr[e] < t) {
-                const int m = mem[e];
+            const int mr = memr[e], m = mem[e];   // both at once; m is used only for a placed member
+            if (mr < t) {
                 is_mem = true;
                 tb = rowT[m];
                 tl = rowT[m + 1] - tb;
@@ -526,11 +533,16 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
         }
         const int count = __popcll(__ballot(is_mem) & sub_mask<S>());
         bool blocked = false;
-        walk_members<S>(tb, tl, [&](int idx) {
-            if (idx >= 0) {
-                const int s = colT[idx];
-                if (s < r && !isb(bnd, s) && !(rec[s].dround < t)) blocked = true;
-            }
+        // (a boundary row never decides and holds round -1, so it never blocks)
+        walk_members<S, JWB>(tb, tl, [&](const int (&idx)[JWB]) {
+            int sv[JWB], dv[JWB];
+#pragma unroll
+            for (int w = 0; w < JWB; w++) sv[w] = idx[w] >= 0 ? colT[idx[w]] : -1;
+#pragma unroll
+            for (int w = 0; w < JWB; w++) dv[w] = (sv[w] >= 0 && sv[w] < r) ? rec[sv[w]].dround : -1;
+#pragma unroll
+            for (int w = 0; w < JWB; w++)
+                if (!(dv[w] < t)) blocked = true;
         });
         // every lane of the sub-group takes part in the vote
         blocked = (__ballot(blocked) & sub_mask<S>()) != 0;
