@@ -149,6 +149,11 @@ struct fvm_amg {
     long long tail_max_rows = 40000;
     int exact_wg_width = 256;          // runs of consecutive dependency levels of at most this many rows
                                        // sweep in one workgroup launch (no result changes)
+    int reuse_setup = 0;               // opt-in: a setup of an unchanged matrix keeps the hierarchy
+    unsigned long long setup_sig = 0;  // signature of the hierarchy's inputs (when reuse_setup)
+    bool setup_sig_valid = false;
+    long long setup_reused = 0;        // setups skipped that way
+    bool parked = false;               // cleanup kept the hierarchy for such a setup
     int exact_flow = 1;                // reference-order sweeps as one dataflow launch per direction
     fvm::DArr<int> flow_ctl;           // its block tickets, finished blocks and error flag
     bool flow_used = false;            // a dataflow sweep ran since the last check

@@ -203,6 +203,8 @@ int fvm_amg_set_params(fvm_amg *h, const fvm_amg_params *p) {
         HandleScope g(h);
         h->lev.clear();
         h->ready = false;
+        h->parked = false;
+        h->setup_sig_valid = false;
     }
     return FVM_OK;
     FVM_API_END
@@ -330,11 +332,18 @@ int fvm_amg_cleanup(fvm_amg *h) {
     FVM_CHECK(h, FVM_E_INVALID, "null handle");
     HandleScope g(h);
     FVM_HIP(hipStreamSynchronize(h->stream));
+    h->ready = false;
+    if (h->reuse_setup && h->setup_sig_valid && !h->batch) {
+        // kept, but unusable until a setup of the same matrix takes it up again
+        h->parked = true;
+        return FVM_OK;
+    }
+    h->parked = false;
+    h->setup_sig_valid = false;
     h->lev.clear();
     h->dense_inv.release();
     h->direct_level = -1;
     h->direct_tried = false;
-    h->ready = false;
     return FVM_OK;
     FVM_API_END
 }
@@ -483,6 +492,16 @@ int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value) {
         FVM_CHECK(value >= 0 && value <= INT_MAX, FVM_E_INVALID, "exact_wg_width must be in [0, INT_MAX]");
         h->exact_wg_width = (int)value;
         fvm::invalidate_cycle_graphs(h);
+    } else if (k == "reuse_setup") {
+        h->reuse_setup = value != 0;
+        h->setup_sig_valid = false;   // the signature is taken from the next setup on
+        if (h->parked) {              // a hierarchy kept by cleanup is dropped
+            h->parked = false;
+            h->lev.clear();
+            h->dense_inv.release();
+            h->direct_level = -1;
+            h->direct_tried = false;
+        }
     } else if (k == "exact_flow") {
         h->exact_flow = value != 0;
     } else if (k == "exact_flow_spin_limit") {
@@ -528,6 +547,8 @@ int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value) {
     else if (k == "bipartite_levels") *value = h->bipartite_levels;
     else if (k == "exact_wg_width") *value = h->exact_wg_width;
     else if (k == "exact_flow") *value = h->exact_flow;
+    else if (k == "reuse_setup") *value = h->reuse_setup;
+    else if (k == "setups_reused") *value = h->setup_reused;
     else if (k == "exact_flow_spin_limit") *value = h->flow_spin_limit;
     else return FVM_E_INVALID;
     return FVM_OK;

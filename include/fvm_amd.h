@@ -209,7 +209,12 @@ int fvm_amg_kernel_bytes(const fvm_amg *h, int kind, double *total_bytes);
  * colours) and keep it when no edge joins equal colours, else the greedy
  * colouring (default 1; takes effect at the next setup). Only the colouring
  * keys change the coloured smoother's arithmetic (another Gauss-Seidel order);
- * the others never change a result; "exact_flow" -- reference-order sweeps as
+ * the others never change a result; "reuse_setup" -- opt-in: a setup whose
+ * matrix arrays and hierarchy parameters have the signature of the hierarchy
+ * in place (64-bit sums of mixed elements) keeps it, also across
+ * fvm_amg_cleanup, which then keeps the hierarchy for such a setup (default 0:
+ * every setup rebuilds, as the reference does after cleanup); "setups_reused"
+ * (read only) counts them; "exact_flow" -- reference-order sweeps as
  * one dataflow launch per direction (default 1; 0: level by level);
  * "exact_flow_spin_limit" -- waits per wavefront after which such a sweep gives
  * up and the call fails with FVM_E_NUMERIC (default 0: no bound; a sweep that

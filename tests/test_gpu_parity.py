@@ -386,6 +386,47 @@ def test_reference_order_sweep_in_one_workgroup_is_bit_identical(cycle):
         assert np.array_equal(hist, out[0][1])
 
 
+def test_opt_in_setup_reuse_gives_the_rebuilt_results():
+    """reuse_setup 1: a setup of the same matrix (a new system object, even after
+    cleanup, which the reference's models call after every solve) keeps the
+    hierarchy, recognised by a signature of the matrix and the hierarchy's
+    parameters; the solve equals a fresh handle's bit for bit. A changed value or
+    parameter rebuilds."""
+    m = matgen.grid3d(12, seed=31, contrast=1.5)
+    kw = dict(nMaxIterations=20, relativeTolerance=1e-30)
+    fresh = gpu_amg(**kw)
+    want = ls_from(m)
+    fresh.solve(want)
+    ga = gpu_amg(**kw)
+    ga.set_tuning("reuse_setup", 1)
+    ls = ls_from(m)
+    ga.solve(ls)
+    assert ga.get_tuning("setups_reused") == 0
+    ga.cleanup()
+    for k in range(2):
+        ls = ls_from(m)
+        ga.solve(ls)
+        assert ga.get_tuning("setups_reused") == k + 1
+        assert np.array_equal(ls.delta, want.delta)
+        np.testing.assert_array_equal(ga.history(), fresh.history())
+    # one changed coefficient: rebuilt, and equal to a fresh handle on that matrix
+    m2 = dict(m)
+    m2["off"] = m["off"].copy()
+    m2["off"][7] *= 1.5
+    ls2 = ls_from(m2)
+    ga.solve(ls2)
+    assert ga.get_tuning("setups_reused") == 2
+    f2 = gpu_amg(**kw)
+    want2 = ls_from(m2)
+    f2.solve(want2)
+    assert np.array_equal(ls2.delta, want2.delta)
+    # a hierarchy parameter: rebuilt as well
+    ga.maxCoarseLevels = 3
+    ls3 = ls_from(m2)
+    ga.solve(ls3)
+    assert ga.get_tuning("setups_reused") == 2 and ga.num_levels() == 4
+
+
 def test_dataflow_sweep_bounded_wait_fails_loudly_then_recovers():
     """A reference-order dataflow sweep whose wavefronts may wait only once
     (exact_flow_spin_limit 1) gives up through its own bounded wait: the call
