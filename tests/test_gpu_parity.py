@@ -447,6 +447,50 @@ def test_dataflow_sweep_bounded_wait_fails_loudly_then_recovers():
     assert np.array_equal(ls.delta, want["x"])
 
 
+def _continuity():
+    return O.load_mm(os.path.join(G, "cav32_continuity_matrix.mat"), os.path.join(G, "cav32_continuity_matrix.rhs"))
+
+
+@pytest.mark.parametrize("psr", [1, 0])
+def test_continuity_matrix_exact(psr):
+    """F:test/FLOW_CONTINUITY_MATRIX/GOLDEN/matrix.mat, the cavity's pressure-
+    correction matrix: negative diagonal, positive off-diagonals, two stored
+    zeros and one pinned row (diagonal -1). Maps, coarse matrices, delta and the
+    residual history equal the oracle's over 60 reference-order cycles, with
+    either stop rule (plain AMG converges slowly on it: the oracle needs about
+    500 cycles for a 1e-3 reduction)."""
+    m = _continuity()
+    kw = dict(nMaxIterations=60, relativeTolerance=1e-30, parallelStopRule=psr)
+    okw = dict(n_max_iterations=60, relative_tolerance=1e-30, parallel_stop_rule=psr)
+    ga, oa, ls, r0, ores = run_pair(m, kw, okw)
+    check_hierarchy(ga, oa, m)
+    assert ga.lastCycles == ores["cycles"] == 59
+    assert np.array_equal(ls.delta, ores["x"])
+    np.testing.assert_allclose(ga.history(), ores["history"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("prec,order", [(0, 0), (1, 1)])
+def test_continuity_matrix_fgmres_matches_dense_solve(prec, order):
+    """FGMRES(30) around the AMG reaches 1e-12 on the same matrix and agrees with
+    numpy.linalg.solve of A x = -b (the delta form r = b + A x) within the bound
+    that the condition number (1.4e6) puts on a 1e-12 residual reduction."""
+    f = _fvm()
+    m = _continuity()
+    n = m["n_self"]
+    A = np.diag(m["diag"][:n]).astype(np.float64)
+    for i in range(n):
+        for p in range(m["row"][i], m["row"][i + 1]):
+            A[i, m["col"][p]] += m["off"][p]
+    xs = np.linalg.solve(A, -m["b"][:n])
+    kr = f.FGMRES(gpu_amg(precision=prec, gsOrdering=order))
+    kr.relativeTolerance = 1e-12
+    kr.nMaxIterations = 400
+    ls = ls_from(m)
+    r0 = kr.solve(ls)
+    assert kr.history()[-1] / r0 < 1e-12
+    np.testing.assert_allclose(ls.delta[:n], xs, rtol=0, atol=1e-5 * np.abs(xs).max())
+
+
 def test_convergence_lines_match_the_reference_golden(tmp_path):
     """AMG.cpp:237-272 with MultiFieldReduction::print: the lines of the cavity
     solve equal AMG_MERGING_THERMAL/proc1/GOLDEN/convergence.dat (parallel-build
