@@ -32,6 +32,7 @@ struct Level {
     // is not structurally symmetric (sym == 0); sym == -1: not yet known
     int sym = -1;
     DArr<int> rowT, colT;
+    DArr<int> tposT;        // with rowT/colT: position of the entry (colT[q], c) in row colT[q]
     DArr<int> ci;           // to the next level (absent on the coarsest)
     int nc = 0;
     DArr<int> c2f_row, c2f_col;  // coarse->fine of ci (on this, the fine, level)
@@ -164,6 +165,7 @@ struct fvm_amg {
     int coarsen_small_lanes = 0;       // the same for the single-workgroup rounds (0: as coarsen_lanes)
     int coarsen_grid = 0;              // blocks per launched agglomeration round (0: 2048 / 1024)
     int bipartite_levels = 1;          // levels tried with a breadth-first two-colouring first (red-black)
+    int coarsen_pair_filter = 1;       // rows do not wait on seeds that can never pair their neighbour
     int sell_block = 256;              // threads per block of the SELL-64 smoother kernels
     int spmv_block = 256;              // threads per block of the SELL-64 residual / SpMV kernels
     int sell_xcd = 1, spmv_xcd = 0;    // XCD-contiguous block order for those kernel classes

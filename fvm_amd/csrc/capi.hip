@@ -502,6 +502,8 @@ int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value) {
             h->direct_level = -1;
             h->direct_tried = false;
         }
+    } else if (k == "coarsen_pair_filter") {
+        h->coarsen_pair_filter = value != 0;
     } else if (k == "exact_flow") {
         h->exact_flow = value != 0;
     } else if (k == "exact_flow_spin_limit") {
@@ -547,6 +549,7 @@ int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value) {
     else if (k == "bipartite_levels") *value = h->bipartite_levels;
     else if (k == "exact_wg_width") *value = h->exact_wg_width;
     else if (k == "exact_flow") *value = h->exact_flow;
+    else if (k == "coarsen_pair_filter") *value = h->coarsen_pair_filter;
     else if (k == "reuse_setup") *value = h->reuse_setup;
     else if (k == "setups_reused") *value = h->setup_reused;
     else if (k == "exact_flow_spin_limit") *value = h->flow_spin_limit;

@@ -192,9 +192,60 @@ __device__ __forceinline__ void pb_push(PushBuf &pb, bool want, int x, int *q, i
     pb.n += want ? 1 : 0;
 }
 
+// may_pair[p] for the entry p = (d, c) of row d: 0 when d can never pair with c.
+// d pairs its strongest ungrouped neighbour c only if w(d,c) > theta * (strongest
+// grouped weight). If some neighbour e != c has w(d,e) > w(d,c) and
+// theta * w(d,e) >= w(d,c), then whenever c is the strongest ungrouped
+// neighbour, e (stronger) is grouped, the strongest grouped weight is w(d,e) =
+// the row's maximum W1, and the test fails: d joins or starts a group instead.
+// That is, may_pair = !(W1 > w && theta * W1 >= w), with w computed exactly as
+// the decision computes it. Such an entry removes the "d could pair c first"
+// dependency of the rows adjacent to c (rows still wait for every lower
+// neighbour), which shortens the rounds' dependency chains.
+__global__ void k_may_pair(int n, const int *row, const int *col, const double *off, const double *diag,
+                           const uint8_t *bnd, double theta, uint8_t *mayp) {
+    for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < n; d += gridDim.x * blockDim.x) {
+        const int rb = row[d], re = row[d + 1];
+        if (isb(bnd, d)) {   // never decides
+            for (int p = rb; p < re; p++) mayp[p] = 0;
+            continue;
+        }
+        const double d0 = fabs(diag[d]);
+        double w1 = -1.0;
+        for (int p = rb; p < re; p++) {
+            const int c = col[p];
+            if (isb(bnd, c)) continue;
+            const double d1 = fabs(diag[c]);
+            const double w = fabs(fabs(off[p]) / (d0 < d1 ? d1 : d0));
+            if (w > w1) w1 = w;
+        }
+        for (int p = rb; p < re; p++) {
+            const int c = col[p];
+            if (isb(bnd, c)) { mayp[p] = 0; continue; }
+            const double d1 = fabs(diag[c]);
+            const double w = fabs(fabs(off[p]) / (d0 < d1 ? d1 : d0));
+            mayp[p] = (w1 > w && theta * w1 >= w) ? 0 : 1;
+        }
+    }
+}
+
+// For a structurally symmetric pattern (the transpose pattern is the pattern
+// itself): tpos[q] = position of the entry (col[q], c) in row col[q], for every
+// entry q of row c (-1 if absent).
+__global__ void k_tpos_sym(int n, const int *row, const int *col, int *tpos) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+        for (int q = row[c]; q < row[c + 1]; q++) {
+            const int s = col[q];
+            int pos = -1;
+            for (int p = row[s]; p < row[s + 1]; p++)
+                if (col[p] == c) { pos = p; break; }
+            tpos[q] = pos;
+        }
+}
+
 __global__ void k_cnt_init(int n, const int *row, const int *col, const int *rowT, const int *colT,
-                           int sym, const uint8_t *bnd, int *cnt, int *dround, int *ldr, int *memr,
-                           int *mcnt, int *q, int *qcnt) {
+                           int sym, const uint8_t *bnd, const uint8_t *mayp, const int *tpos, int *cnt,
+                           int *dround, int *ldr, int *memr, int *mcnt, int *q, int *qcnt) {
     const long long gt = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const int r = (int)(gt / INIT_SUB);
     const int lane = threadIdx.x & (INIT_SUB - 1);
@@ -215,9 +266,10 @@ __global__ void k_cnt_init(int n, const int *row, const int *col, const int *row
                 if (c < r) {
                     k += sym ? 2 : 1;   // symmetric: the rowT part is the same set
                 } else {
+                    // s could pair c first (unless s can never pair c: may_pair of its entry)
                     for (int qq = rowT[c]; qq < rowT[c + 1]; qq++) {
                         const int s = colT[qq];
-                        k += (s < r && !isb(bnd, s));
+                        k += (s < r && !isb(bnd, s) && (!mayp || tpos[qq] < 0 || mayp[tpos[qq]]));
                     }
                 }
             }
@@ -390,7 +442,7 @@ __device__ __forceinline__ void walk_members(int tb, int tl, F &&f) {
 // writes one run.
 template <int S, bool FILL>
 __global__ void k_dep_fill(int n, const int *row, const int *col, const int *rowT, const int *colT, int sym,
-                           const uint8_t *bnd, const int *dptr, int *out) {
+                           const uint8_t *bnd, const uint8_t *mayp, const int *dptr, int *out) {
     const int lane = threadIdx.x & (S - 1);
     const int sub = (blockIdx.x * blockDim.x + threadIdx.x) / S;
     const int nsub = (gridDim.x * blockDim.x) / S;
@@ -413,7 +465,7 @@ __global__ void k_dep_fill(int n, const int *row, const int *col, const int *row
                 if (c > d && !isb(bnd, c)) {
                     a_ok = true;
                     tb = rowT[c];
-                    tl = rowT[c + 1] - tb;
+                    tl = (!mayp || mayp[p]) ? rowT[c + 1] - tb : 0;   // C part only if d may pair c
                 }
             }
             // A: x in row(d), x > d
@@ -716,11 +768,14 @@ __global__ void k_tr_count(int n, const int *row, const int *col, int *cnt) {
         for (int p = row[i]; p < row[i + 1]; p++) atomicAdd(&cnt[col[p]], 1);
 }
 
-__global__ void k_tr_fill(int n, const int *row, const int *col, const int *rowT, int *fill, int *colT) {
+__global__ void k_tr_fill(int n, const int *row, const int *col, const int *rowT, int *fill, int *colT,
+                          int *tpos) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         for (int p = row[i]; p < row[i + 1]; p++) {
             const int c = col[p];
-            colT[rowT[c] + atomicAdd(&fill[c], 1)] = i;
+            const int q = rowT[c] + atomicAdd(&fill[c], 1);
+            colT[q] = i;
+            tpos[q] = p;   // the entry (i, c) this transposed entry stands for
         }
 }
 
@@ -1386,7 +1441,8 @@ void ensure_transpose(fvm_amg *h, Level &L) {
     L.rowT.alloc((size_t)n + 1);
     h->ws.scan.exclusive(cnt.p, L.rowT.p, n + 1, s);
     L.colT.alloc(std::max<long long>(L.nnz, 1));
-    k_tr_fill<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.row.p, L.col.p, L.rowT.p, fill.p, L.colT.p);
+    L.tposT.alloc(std::max<long long>(L.nnz, 1));
+    k_tr_fill<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.row.p, L.col.p, L.rowT.p, fill.p, L.colT.p, L.tposT.p);
     FVM_HIP(hipGetLastError());
 }
 
@@ -1405,12 +1461,30 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
     int *rounds_out = ctr.p + 4;
     const uint8_t *bnd = L.bnd.p;
     int *q0 = lists.p, *q1 = lists.p + n;
+    // entries (d, c) along which d can never pair c, and for the counts the position
+    // of the entry each transposed entry stands for
+    DArr<uint8_t> mayp;
+    DArr<int> tpos_sym;
+    const int *tpos = nullptr;
+    if (h->coarsen_pair_filter && L.nnz > 0) {
+        mayp.alloc(L.nnz);
+        k_may_pair<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.row.p, L.col.p, L.off.p, L.diag.p, bnd, theta, mayp.p);
+        if (sym) {
+            tpos_sym.alloc(L.nnz);
+            k_tpos_sym<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.row.p, L.col.p, tpos_sym.p);
+            tpos = tpos_sym.p;
+        } else {
+            tpos = L.tposT.p;
+        }
+        FVM_HIP(hipGetLastError());
+    }
+    const uint8_t *mp = tpos ? mayp.p : nullptr;
     // initial counts; the frontier of round 1 goes to q1 (round t reads list t&1)
     {
         const long long threads = (long long)n * INIT_SUB;
         const unsigned g = (unsigned)((threads + 255) / 256);
-        k_cnt_init<<<g, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, cnt.p, dround.p, ldr.p,
-                                     memr.p, mcnt.p, q1, qc + 1);
+        k_cnt_init<<<g, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, mp, tpos, cnt.p, dround.p,
+                                     ldr.p, memr.p, mcnt.p, q1, qc + 1);
         FVM_HIP(hipGetLastError());
     }
     // dependents lists (count, scan, fill)
@@ -1419,7 +1493,7 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
         DArr<int> dcnt((size_t)n + 1);
         FVM_HIP(hipMemsetAsync(dcnt.p + n, 0, sizeof(int), s));
         const unsigned gc = (unsigned)std::max<long long>(1, std::min<long long>(((long long)n * 16 + 255) / 256, 65536));
-        k_dep_fill<16, false><<<gc, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, nullptr, dcnt.p);
+        k_dep_fill<16, false><<<gc, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, mp, nullptr, dcnt.p);
         h->ws.scan.exclusive(dcnt.p, dptr.p, n + 1, s);
     }
     const int ndep = read_scalar(dptr.p + n, s);
@@ -1427,7 +1501,7 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
     DArr<int> dep((size_t)std::max(ndep, 1));
     {
         const unsigned gf = (unsigned)std::max<long long>(1, std::min<long long>(((long long)n * 16 + 255) / 256, 65536));
-        k_dep_fill<16, true><<<gf, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, dptr.p, dep.p);
+        k_dep_fill<16, true><<<gf, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, mp, dptr.p, dep.p);
         FVM_HIP(hipGetLastError());
     }
     DArr<RowRec> rec(std::max(n, 1));
