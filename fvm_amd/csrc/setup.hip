@@ -202,26 +202,31 @@ __device__ __forceinline__ void pb_push(PushBuf &pb, bool want, int x, int *q, i
 // the decision computes it. Such an entry removes the "d could pair c first"
 // dependency of the rows adjacent to c (rows still wait for every lower
 // neighbour), which shortens the rounds' dependency chains.
+// MP lanes per row (a row's entries spread over them).
+constexpr int MP = 8;
 __global__ void k_may_pair(int n, const int *row, const int *col, const double *off, const double *diag,
                            const uint8_t *bnd, double theta, uint8_t *mayp) {
-    for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < n; d += gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x & (MP - 1);
+    const int sub = (blockIdx.x * blockDim.x + threadIdx.x) / MP, nsub = (gridDim.x * blockDim.x) / MP;
+    for (int d = sub; d < n; d += nsub) {   // uniform over each sub-group
         const int rb = row[d], re = row[d + 1];
-        if (isb(bnd, d)) {   // never decides
-            for (int p = rb; p < re; p++) mayp[p] = 0;
-            continue;
-        }
+        const bool bd = isb(bnd, d);        // never decides: no pair at all
         const double d0 = fabs(diag[d]);
         double w1 = -1.0;
-        for (int p = rb; p < re; p++) {
+        for (int p = rb + lane; p < re; p += MP) {
             const int c = col[p];
             if (isb(bnd, c)) continue;
             const double d1 = fabs(diag[c]);
             const double w = fabs(fabs(off[p]) / (d0 < d1 ? d1 : d0));
             if (w > w1) w1 = w;
         }
-        for (int p = rb; p < re; p++) {
+        for (int o = MP / 2; o > 0; o >>= 1) {
+            const double w2 = __shfl_xor(w1, o, MP);
+            if (w2 > w1) w1 = w2;
+        }
+        for (int p = rb + lane; p < re; p += MP) {
             const int c = col[p];
-            if (isb(bnd, c)) { mayp[p] = 0; continue; }
+            if (bd || isb(bnd, c)) { mayp[p] = 0; continue; }
             const double d1 = fabs(diag[c]);
             const double w = fabs(fabs(off[p]) / (d0 < d1 ? d1 : d0));
             mayp[p] = (w1 > w && theta * w1 >= w) ? 0 : 1;
@@ -229,22 +234,31 @@ __global__ void k_may_pair(int n, const int *row, const int *col, const double *
     }
 }
 
-// For a structurally symmetric pattern (the transpose pattern is the pattern
-// itself): tpos[q] = position of the entry (col[q], c) in row col[q], for every
-// entry q of row c (-1 if absent).
-__global__ void k_tpos_sym(int n, const int *row, const int *col, int *tpos) {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
-        for (int q = row[c]; q < row[c + 1]; q++) {
-            const int s = col[q];
+// The flags of the transpose pattern: mayT[q] = may_pair of the entry (colT[q], c)
+// of row colT[q], for every entry q of row c of the transpose pattern. Its
+// position comes from tposT (non-symmetric patterns) or, for a structurally
+// symmetric pattern (the transpose pattern is the pattern), from a scan of that
+// row (1 if absent, which keeps the dependency).
+__global__ void k_may_pair_t(int n, const int *row, const int *col, const int *rowT, const int *colT,
+                             const int *tposT, const uint8_t *mayp, uint8_t *mayT) {
+    const int lane = threadIdx.x & (MP - 1);
+    const int sub = (blockIdx.x * blockDim.x + threadIdx.x) / MP, nsub = (gridDim.x * blockDim.x) / MP;
+    for (int c = sub; c < n; c += nsub)
+        for (int q = rowT[c] + lane; q < rowT[c + 1]; q += MP) {
             int pos = -1;
-            for (int p = row[s]; p < row[s + 1]; p++)
-                if (col[p] == c) { pos = p; break; }
-            tpos[q] = pos;
+            if (tposT) {
+                pos = tposT[q];
+            } else {
+                const int s = colT[q];
+                for (int p = row[s]; p < row[s + 1]; p++)
+                    if (col[p] == c) { pos = p; break; }
+            }
+            mayT[q] = pos < 0 ? 1 : mayp[pos];
         }
 }
 
 __global__ void k_cnt_init(int n, const int *row, const int *col, const int *rowT, const int *colT,
-                           int sym, const uint8_t *bnd, const uint8_t *mayp, const int *tpos, int *cnt,
+                           int sym, const uint8_t *bnd, const uint8_t *mayT, int *cnt,
                            int *dround, int *ldr, int *memr, int *mcnt, int *q, int *qcnt) {
     const long long gt = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const int r = (int)(gt / INIT_SUB);
@@ -269,7 +283,7 @@ __global__ void k_cnt_init(int n, const int *row, const int *col, const int *row
                     // s could pair c first (unless s can never pair c: may_pair of its entry)
                     for (int qq = rowT[c]; qq < rowT[c + 1]; qq++) {
                         const int s = colT[qq];
-                        k += (s < r && !isb(bnd, s) && (!mayp || tpos[qq] < 0 || mayp[tpos[qq]]));
+                        k += (s < r && !isb(bnd, s) && (!mayT || mayT[qq]));
                     }
                 }
             }
@@ -1463,27 +1477,24 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
     int *q0 = lists.p, *q1 = lists.p + n;
     // entries (d, c) along which d can never pair c, and for the counts the position
     // of the entry each transposed entry stands for
-    DArr<uint8_t> mayp;
-    DArr<int> tpos_sym;
-    const int *tpos = nullptr;
+    DArr<uint8_t> mayp, mayT;
+    const uint8_t *mp = nullptr, *mt = nullptr;
     if (h->coarsen_pair_filter && L.nnz > 0) {
         mayp.alloc(L.nnz);
-        k_may_pair<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.row.p, L.col.p, L.off.p, L.diag.p, bnd, theta, mayp.p);
-        if (sym) {
-            tpos_sym.alloc(L.nnz);
-            k_tpos_sym<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.row.p, L.col.p, tpos_sym.p);
-            tpos = tpos_sym.p;
-        } else {
-            tpos = L.tposT.p;
-        }
+        mayT.alloc(L.nnz);
+        const unsigned gm = grid_for((long long)n * MP, BS, 65536);
+        k_may_pair<<<gm, BS, 0, s>>>(n, L.row.p, L.col.p, L.off.p, L.diag.p, bnd, theta, mayp.p);
+        k_may_pair_t<<<gm, BS, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym ? nullptr : L.tposT.p, mayp.p,
+                                       mayT.p);
         FVM_HIP(hipGetLastError());
+        mp = mayp.p;
+        mt = mayT.p;
     }
-    const uint8_t *mp = tpos ? mayp.p : nullptr;
     // initial counts; the frontier of round 1 goes to q1 (round t reads list t&1)
     {
         const long long threads = (long long)n * INIT_SUB;
         const unsigned g = (unsigned)((threads + 255) / 256);
-        k_cnt_init<<<g, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, mp, tpos, cnt.p, dround.p,
+        k_cnt_init<<<g, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, mt, cnt.p, dround.p,
                                      ldr.p, memr.p, mcnt.p, q1, qc + 1);
         FVM_HIP(hipGetLastError());
     }
