@@ -192,16 +192,25 @@ __device__ __forceinline__ void pb_push(PushBuf &pb, bool want, int x, int *q, i
     pb.n += want ? 1 : 0;
 }
 
-// may_pair[p] for the entry p = (d, c) of row d: 0 when d can never pair with c.
+// Flags of the entry p = (d, c) of row d (one byte each):
+// MAYPAIR: d may pair with c. It is 0 when d can never pair with c.
 // d pairs its strongest ungrouped neighbour c only if w(d,c) > theta * (strongest
 // grouped weight). If some neighbour e != c has w(d,e) > w(d,c) and
 // theta * w(d,e) >= w(d,c), then whenever c is the strongest ungrouped
 // neighbour, e (stronger) is grouped, the strongest grouped weight is w(d,e) =
 // the row's maximum W1, and the test fails: d joins or starts a group instead.
-// That is, may_pair = !(W1 > w && theta * W1 >= w), with w computed exactly as
-// the decision computes it. Such an entry removes the "d could pair c first"
-// dependency of the rows adjacent to c (rows still wait for every lower
-// neighbour), which shortens the rounds' dependency chains.
+// That is, MAYPAIR = !(W1 > w && theta * W1 >= w), with w computed exactly as
+// the decision computes it.
+// IGNORES: d's decision is the same whether c is grouped or not. That holds when
+// d's strongest lower neighbour (grouped by the time d decides, as every lower
+// row is) has weight L1 > w(d,c) and theta * L1 >= w(d,c): then c is never the
+// strongest grouped neighbour, and were c the strongest ungrouped one the pair
+// test would fail, so d joins or starts a group either way, with the same target.
+// A row waits on a lower seed s because s could pair a higher neighbour c of the
+// row first; that dependency is dropped when s may not pair c or the row ignores
+// c. Rows still wait for every lower neighbour. This shortens the rounds'
+// dependency chains; the decisions do not change.
+constexpr uint8_t MAYPAIR = 1, IGNORES = 2;
 // MP lanes per row (a row's entries spread over them).
 constexpr int MP = 8;
 __global__ void k_may_pair(int n, const int *row, const int *col, const double *off, const double *diag,
@@ -212,24 +221,26 @@ __global__ void k_may_pair(int n, const int *row, const int *col, const double *
         const int rb = row[d], re = row[d + 1];
         const bool bd = isb(bnd, d);        // never decides: no pair at all
         const double d0 = fabs(diag[d]);
-        double w1 = -1.0;
+        double w1 = -1.0, l1 = -1.0;        // strongest neighbour, strongest lower neighbour
         for (int p = rb + lane; p < re; p += MP) {
             const int c = col[p];
             if (isb(bnd, c)) continue;
             const double d1 = fabs(diag[c]);
             const double w = fabs(fabs(off[p]) / (d0 < d1 ? d1 : d0));
             if (w > w1) w1 = w;
+            if (c < d && w > l1) l1 = w;
         }
         for (int o = MP / 2; o > 0; o >>= 1) {
-            const double w2 = __shfl_xor(w1, o, MP);
+            const double w2 = __shfl_xor(w1, o, MP), l2 = __shfl_xor(l1, o, MP);
             if (w2 > w1) w1 = w2;
+            if (l2 > l1) l1 = l2;
         }
         for (int p = rb + lane; p < re; p += MP) {
             const int c = col[p];
             if (bd || isb(bnd, c)) { mayp[p] = 0; continue; }
             const double d1 = fabs(diag[c]);
             const double w = fabs(fabs(off[p]) / (d0 < d1 ? d1 : d0));
-            mayp[p] = (w1 > w && theta * w1 >= w) ? 0 : 1;
+            mayp[p] = ((w1 > w && theta * w1 >= w) ? 0 : MAYPAIR) | ((l1 > w && theta * l1 >= w) ? IGNORES : 0);
         }
     }
 }
@@ -253,12 +264,12 @@ __global__ void k_may_pair_t(int n, const int *row, const int *col, const int *r
                 for (int p = row[s]; p < row[s + 1]; p++)
                     if (col[p] == c) { pos = p; break; }
             }
-            mayT[q] = pos < 0 ? 1 : mayp[pos];
+            mayT[q] = pos < 0 ? MAYPAIR : mayp[pos];   // absent: keep the dependency
         }
 }
 
 __global__ void k_cnt_init(int n, const int *row, const int *col, const int *rowT, const int *colT,
-                           int sym, const uint8_t *bnd, const uint8_t *mayT, int *cnt,
+                           int sym, const uint8_t *bnd, const uint8_t *mayp, const uint8_t *mayT, int *cnt,
                            int *dround, int *ldr, int *memr, int *mcnt, int *q, int *qcnt) {
     const long long gt = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const int r = (int)(gt / INIT_SUB);
@@ -279,11 +290,11 @@ __global__ void k_cnt_init(int n, const int *row, const int *col, const int *row
                 if (isb(bnd, c)) continue;
                 if (c < r) {
                     k += sym ? 2 : 1;   // symmetric: the rowT part is the same set
-                } else {
-                    // s could pair c first (unless s can never pair c: may_pair of its entry)
+                } else if (!mayp || !(mayp[p] & IGNORES)) {
+                    // s could pair c first (unless s may not pair c)
                     for (int qq = rowT[c]; qq < rowT[c + 1]; qq++) {
                         const int s = colT[qq];
-                        k += (s < r && !isb(bnd, s) && (!mayT || mayT[qq]));
+                        k += (s < r && !isb(bnd, s) && (!mayT || (mayT[qq] & MAYPAIR)));
                     }
                 }
             }
@@ -456,7 +467,8 @@ __device__ __forceinline__ void walk_members(int tb, int tl, F &&f) {
 // writes one run.
 template <int S, bool FILL>
 __global__ void k_dep_fill(int n, const int *row, const int *col, const int *rowT, const int *colT, int sym,
-                           const uint8_t *bnd, const uint8_t *mayp, const int *dptr, int *out) {
+                           const uint8_t *bnd, const uint8_t *mayp, const uint8_t *mayT, const int *dptr,
+                           int *out) {
     const int lane = threadIdx.x & (S - 1);
     const int sub = (blockIdx.x * blockDim.x + threadIdx.x) / S;
     const int nsub = (gridDim.x * blockDim.x) / S;
@@ -479,7 +491,7 @@ __global__ void k_dep_fill(int n, const int *row, const int *col, const int *row
                 if (c > d && !isb(bnd, c)) {
                     a_ok = true;
                     tb = rowT[c];
-                    tl = (!mayp || mayp[p]) ? rowT[c + 1] - tb : 0;   // C part only if d may pair c
+                    tl = (!mayp || (mayp[p] & MAYPAIR)) ? rowT[c + 1] - tb : 0;   // C part only if d may pair c
                 }
             }
             // A: x in row(d), x > d
@@ -493,7 +505,8 @@ __global__ void k_dep_fill(int n, const int *row, const int *col, const int *row
                 int x = -1;
                 if (idx >= 0) {
                     x = colT[idx];
-                    w = x > d && x < cj && !isb(bnd, x);
+                    // (and unless x's decision ignores c: flag of the entry (x, c))
+                    w = x > d && x < cj && !isb(bnd, x) && !(mayT && (mayT[idx] & IGNORES));
                 }
                 const unsigned long long mc = __ballot(w) & smask;
                 if (FILL && w) out[pos + __popcll(mc & below)] = x;
@@ -1494,7 +1507,7 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
     {
         const long long threads = (long long)n * INIT_SUB;
         const unsigned g = (unsigned)((threads + 255) / 256);
-        k_cnt_init<<<g, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, mt, cnt.p, dround.p,
+        k_cnt_init<<<g, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, mp, mt, cnt.p, dround.p,
                                      ldr.p, memr.p, mcnt.p, q1, qc + 1);
         FVM_HIP(hipGetLastError());
     }
@@ -1504,7 +1517,8 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
         DArr<int> dcnt((size_t)n + 1);
         FVM_HIP(hipMemsetAsync(dcnt.p + n, 0, sizeof(int), s));
         const unsigned gc = (unsigned)std::max<long long>(1, std::min<long long>(((long long)n * 16 + 255) / 256, 65536));
-        k_dep_fill<16, false><<<gc, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, mp, nullptr, dcnt.p);
+        k_dep_fill<16, false><<<gc, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, mp, mt, nullptr,
+                                                 dcnt.p);
         h->ws.scan.exclusive(dcnt.p, dptr.p, n + 1, s);
     }
     const int ndep = read_scalar(dptr.p + n, s);
@@ -1512,7 +1526,7 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
     DArr<int> dep((size_t)std::max(ndep, 1));
     {
         const unsigned gf = (unsigned)std::max<long long>(1, std::min<long long>(((long long)n * 16 + 255) / 256, 65536));
-        k_dep_fill<16, true><<<gf, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, mp, dptr.p, dep.p);
+        k_dep_fill<16, true><<<gf, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, mp, mt, dptr.p, dep.p);
         FVM_HIP(hipGetLastError());
     }
     DArr<RowRec> rec(std::max(n, 1));
