@@ -3,7 +3,10 @@ decide_row in fvm_amd/csrc/setup.hip), emulated on the CPU with the kernels'
 snapshot semantics, dependency counters and frontier. It pins the algorithm itself: on
 symmetric and non-symmetric patterns, with boundary rows, the parallel rounds
 give exactly the sequential greedy map of the oracle (CRMatrix.h:468-586), and
-no two decisions of one round write the same state.
+no two decisions of one round write the same state. With filt, the dependencies
+that k_may_pair proves idle are dropped (a seed that can never pair the
+neighbour, a row whose decision ignores the neighbour's grouping): the map must
+not change, on matrices with tied weights too.
 """
 import os
 
@@ -18,7 +21,11 @@ G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 INF = 1 << 30
 
 
-def emulate(m, bnd=None, theta=0.65):
+def weight(off, d0, d1):
+    return abs(abs(off) / (d1 if d0 < d1 else d0))
+
+
+def emulate(m, bnd=None, theta=0.65, filt=False):
     """Count-based rounds as in k_cnt_init / k_coarsen_round / decide_row."""
     n = m["n_self"]
     row0, col0, off0 = m["row"], m["col"], m["off"]
@@ -35,6 +42,27 @@ def emulate(m, bnd=None, theta=0.65):
         for p in range(row[i], row[i + 1]):
             rowT[col[p]].append(i)
     sym = all(sorted(rowT[i]) == sorted(col[row[i]:row[i + 1]]) for i in range(n))
+    # k_may_pair: per entry (d, c) of row d, may d pair c / does d's decision ignore c
+    pos = {}
+    may = [True] * len(col)
+    ign = [False] * len(col)
+    for d in range(n):
+        for p in range(row[d], row[d + 1]):
+            pos[(d, col[p])] = p
+        ws = [(col[p], weight(off[p], abs(diag[d]), abs(diag[col[p]]))) for p in range(row[d], row[d + 1])
+              if not isb(col[p])]
+        w1 = max([w for _, w in ws], default=-1.0)
+        l1 = max([w for c, w in ws if c < d], default=-1.0)
+        for p in range(row[d], row[d + 1]):
+            c = col[p]
+            if isb(d) or isb(c):
+                may[p] = False
+                continue
+            w = weight(off[p], abs(diag[d]), abs(diag[c]))
+            may[p] = not (w1 > w and theta * w1 >= w)
+            ign[p] = l1 > w and theta * l1 >= w
+    may_sc = lambda s, c: (not filt) or may[pos[(s, c)]]
+    ign_xc = lambda x, c: filt and ign[pos[(x, c)]]
     cnt = [0] * n
     dr = [(-1 if isb(i) else INF) for i in range(n)]
     ldr = [-1] * n
@@ -54,8 +82,8 @@ def emulate(m, bnd=None, theta=0.65):
                 continue
             if c < r:
                 k += 2 if sym else 1
-            else:
-                k += sum(1 for s in rowT[c] if s < r and not isb(s))
+            elif not ign_xc(r, c):
+                k += sum(1 for s in rowT[c] if s < r and not isb(s) and may_sc(s, c))
         cnt[r] = k
         if k == 0:
             frontier.append(r)
@@ -78,10 +106,10 @@ def emulate(m, bnd=None, theta=0.65):
                         nxt.append(x)
         for p in range(row[d], row[d + 1]):
             c = col[p]
-            if c <= d or isb(c):
+            if c <= d or isb(c) or not may_sc(d, c):
                 continue
             for x in rowT[c]:
-                if d < x < c and not isb(x):
+                if d < x < c and not isb(x) and not ign_xc(x, c):
                     cnt[x] -= 1
                     if cnt[x] == 0:
                         nxt.append(x)
@@ -159,10 +187,13 @@ def emulate(m, bnd=None, theta=0.65):
 
 def _check(m, bnd=None):
     nc, ci = O.coarsen(m, is_bnd=bnd)
-    g, ci2, t, ev = emulate(m, bnd)
-    assert g == nc
-    assert np.array_equal(ci[: m["n_self"]], ci2)
-    return t
+    rounds = []
+    for filt in (False, True):
+        g, ci2, t, ev = emulate(m, bnd, filt=filt)
+        assert g == nc, filt
+        assert np.array_equal(ci[: m["n_self"]], ci2), filt
+        rounds.append(t)
+    return rounds
 
 
 def test_rounds_model_kat226_all_levels():
@@ -195,3 +226,42 @@ def test_rounds_model_random_nonsymmetric(trial):
 
 def test_rounds_model_grid_with_ghosts():
     _check(matgen.grid3d(7, seed=3))
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_rounds_model_filters_shorten_coarse_levels(seed):
+    """A lognormal-coefficient grid and its coarse levels: the filtered
+    dependencies give the oracle's maps in no more rounds, and fewer on the
+    coarse levels, where chains of 'could pair first' dependencies dominate."""
+    m = matgen.grid3d(9, seed=40 + seed, contrast=2.5, ghosts=False)
+    a = O.OracleAMG()
+    a.setup(m["n_self"], m["n_total"], m["row"], m["col"], m["diag"], m["off"])
+    fewer = 0
+    for l in range(min(4, a.num_levels())):
+        full, filtered = _check(a.level(l))
+        assert filtered <= full
+        fewer += filtered < full
+    assert fewer >= 1
+
+
+@pytest.mark.parametrize("trial", range(8))
+def test_rounds_model_filters_with_tied_symmetric_weights(trial):
+    """Symmetric values drawn from {0.5, 1, 2} (many exactly tied weights, where
+    the first-in-CSR-order rule decides) on a random symmetric pattern."""
+    rng = np.random.default_rng(300 + trial)
+    n = int(rng.integers(20, 160))
+    edges = {}
+    for _ in range(int(n * rng.uniform(1.5, 4))):
+        i, j = (int(v) for v in rng.choice(n, 2, replace=False))
+        edges[(min(i, j), max(i, j))] = float(rng.choice([0.5, 1.0, 2.0]))
+    rows = [[] for _ in range(n)]
+    for (i, j), v in edges.items():
+        rows[i].append((j, -v)); rows[j].append((i, -v))
+    row = np.zeros(n + 1, np.int32)
+    row[1:] = np.cumsum([len(r) for r in rows])
+    col = np.array([c for r in rows for c, _ in r], np.int32)
+    off = np.array([v for r in rows for _, v in r], np.float64)
+    diag = np.array([sum(-v for _, v in r) + float(rng.choice([0.0, 0.5])) + 1e-3 for r in rows])
+    m = dict(n_self=n, n_total=n, row=row, col=col, diag=diag, off=off)
+    bnd = (rng.uniform(size=n) < 0.1).astype(np.uint8) if trial % 2 else None
+    _check(m, bnd)
