@@ -160,12 +160,15 @@ struct fvm_amg {
     bool flow_used = false;            // a dataflow sweep ran since the last check
     long long flow_ticks = 0;          // two seconds of the device's wall clock
     int flow_spin_limit = 0;           // waits per wavefront before a sweep gives up (0: no bound)
-    long long coarsen_small_rows = 8192;
+    long long coarsen_small_rows = 2048;
     int coarsen_lanes = 0;             // lanes per row of the agglomeration rounds (0: by row width)
-    int coarsen_small_lanes = 0;       // the same for the single-workgroup rounds (0: as coarsen_lanes)
+    int coarsen_small_lanes = 16;      // the same for the single-workgroup rounds (0: as coarsen_lanes)
     int coarsen_grid = 0;              // blocks per launched agglomeration round (0: 2048 / 1024)
     int bipartite_levels = 1;          // levels tried with a breadth-first two-colouring first (red-black)
     int coarsen_pair_filter = 1;       // rows do not wait on seeds that can never pair their neighbour
+    int coarsen_lower_filter = 1;      // ... nor on lower neighbours other than the strongest ones
+    int coarsen_wait_lists = 2;        // blocked joins wait in their blocker's list (1: only those that
+                                       // wait for the group they join; 0: they retry every round)
     int sell_block = 256;              // threads per block of the SELL-64 smoother kernels
     int spmv_block = 256;              // threads per block of the SELL-64 residual / SpMV kernels
     int sell_xcd = 1, spmv_xcd = 0;    // XCD-contiguous block order for those kernel classes

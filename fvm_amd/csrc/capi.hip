@@ -504,6 +504,11 @@ int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value) {
         }
     } else if (k == "coarsen_pair_filter") {
         h->coarsen_pair_filter = value != 0;
+    } else if (k == "coarsen_lower_filter") {
+        h->coarsen_lower_filter = value != 0;
+    } else if (k == "coarsen_wait_lists") {
+        FVM_CHECK(value >= 0 && value <= 2, FVM_E_INVALID, "coarsen_wait_lists must be 0, 1 or 2");
+        h->coarsen_wait_lists = (int)value;
     } else if (k == "exact_flow") {
         h->exact_flow = value != 0;
     } else if (k == "exact_flow_spin_limit") {
@@ -550,6 +555,8 @@ int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value) {
     else if (k == "exact_wg_width") *value = h->exact_wg_width;
     else if (k == "exact_flow") *value = h->exact_flow;
     else if (k == "coarsen_pair_filter") *value = h->coarsen_pair_filter;
+    else if (k == "coarsen_lower_filter") *value = h->coarsen_lower_filter;
+    else if (k == "coarsen_wait_lists") *value = h->coarsen_wait_lists;
     else if (k == "reuse_setup") *value = h->reuse_setup;
     else if (k == "setups_reused") *value = h->setup_reused;
     else if (k == "exact_flow_spin_limit") *value = h->flow_spin_limit;

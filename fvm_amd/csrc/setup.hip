@@ -192,6 +192,44 @@ __device__ __forceinline__ void pb_push(PushBuf &pb, bool want, int x, int *q, i
     pb.n += want ? 1 : 0;
 }
 
+// Waiter lists of the agglomeration rounds: a blocked join waits in the list of
+// the row it waits for (RowRec::wh, linked through wnext) instead of returning to
+// the frontier every round; the row closes its list when it decides and puts the
+// waiters on the next frontier. A closed list (the row decided in this round)
+// sends the waiter to the next frontier itself, so no wake-up is lost.
+// Every access to a head or a link is an atomic read-modify-write, and a waiter
+// awaits its link's exchange before it moves the head. A waker that finds the
+// waiter in the list therefore reads the link with an exchange of its own after
+// the waiter's was performed, with no release or acquire fence (which would write
+// back or invalidate a whole L2 cache).
+constexpr int WCLOSED = -2;
+constexpr int WTAKEN = -3;   // a link already read by its waker
+
+template <bool WG>
+__device__ __forceinline__ int atom_xchg(int *p, int v) {
+    if (WG) return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Row r joins the waiters of row b (one lane). False: b's list is closed (b has
+// decided in this round) or there are no lists, and r must go to the next frontier.
+template <bool WG>
+__device__ __forceinline__ bool park(int r, int *wh, int *wnext) {
+    if (!wnext) return false;
+    int h = WG ? __hip_atomic_load(wh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+               : __hip_atomic_load(wh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (h != WCLOSED) {
+        const int o = atom_xchg<WG>(&wnext[r], h);
+        asm volatile("" ::"v"(o) : "memory");   // the link is performed before the head moves
+        const bool ok = WG ? __hip_atomic_compare_exchange_strong(wh, &h, r, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                                 __HIP_MEMORY_SCOPE_WORKGROUP)
+                           : __hip_atomic_compare_exchange_strong(wh, &h, r, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                                 __HIP_MEMORY_SCOPE_AGENT);
+        if (ok) return true;
+    }
+    return false;
+}
+
 // Flags of the entry p = (d, c) of row d (one byte each):
 // MAYPAIR: d may pair with c. It is 0 when d can never pair with c.
 // d pairs its strongest ungrouped neighbour c only if w(d,c) > theta * (strongest
@@ -210,11 +248,18 @@ __device__ __forceinline__ void pb_push(PushBuf &pb, bool want, int x, int *q, i
 // row first; that dependency is dropped when s may not pair c or the row ignores
 // c. Rows still wait for every lower neighbour. This shortens the rounds'
 // dependency chains; the decisions do not change.
-constexpr uint8_t MAYPAIR = 1, IGNORES = 2;
+// STRONG (entries (d, c) with c < d): c is one of d's strongest lower neighbours.
+// Every lower row is grouped by the time d decides (the sequential loop has
+// processed it), so d reads of a lower neighbour c only its group, and only when
+// c is d's strongest grouped neighbour, which a lower neighbour weaker than the
+// strongest lower one never is. So d waits for a lower neighbour c only if c may
+// pair d or c is STRONG; the decision counts every lower neighbour as grouped, and
+// a join whose target is still undecided retries in the next round.
+constexpr uint8_t MAYPAIR = 1, IGNORES = 2, STRONG = 4;
 // MP lanes per row (a row's entries spread over them).
 constexpr int MP = 8;
 __global__ void k_may_pair(int n, const int *row, const int *col, const double *off, const double *diag,
-                           const uint8_t *bnd, double theta, uint8_t *mayp) {
+                           const uint8_t *bnd, double theta, int lower, uint8_t *mayp) {
     const int lane = threadIdx.x & (MP - 1);
     const int sub = (blockIdx.x * blockDim.x + threadIdx.x) / MP, nsub = (gridDim.x * blockDim.x) / MP;
     for (int d = sub; d < n; d += nsub) {   // uniform over each sub-group
@@ -240,7 +285,8 @@ __global__ void k_may_pair(int n, const int *row, const int *col, const double *
             if (bd || isb(bnd, c)) { mayp[p] = 0; continue; }
             const double d1 = fabs(diag[c]);
             const double w = fabs(fabs(off[p]) / (d0 < d1 ? d1 : d0));
-            mayp[p] = ((w1 > w && theta * w1 >= w) ? 0 : MAYPAIR) | ((l1 > w && theta * l1 >= w) ? IGNORES : 0);
+            mayp[p] = ((w1 > w && theta * w1 >= w) ? 0 : MAYPAIR) | ((l1 > w && theta * l1 >= w) ? IGNORES : 0) |
+                      ((c < d && (!lower || !(l1 > w))) ? STRONG : 0);
         }
     }
 }
@@ -264,7 +310,7 @@ __global__ void k_may_pair_t(int n, const int *row, const int *col, const int *r
                 for (int p = row[s]; p < row[s + 1]; p++)
                     if (col[p] == c) { pos = p; break; }
             }
-            mayT[q] = pos < 0 ? MAYPAIR : mayp[pos];   // absent: keep the dependency
+            mayT[q] = pos < 0 ? (MAYPAIR | STRONG) : mayp[pos];   // absent: keep the dependencies
         }
 }
 
@@ -280,16 +326,20 @@ __global__ void k_cnt_init(int n, const int *row, const int *col, const int *row
     if (valid) {
         b = isb(bnd, r);
         if (!b) {
+            // A: s < r may pair r (entry (s, r), the transposed flag)
             if (!sym)
                 for (int qq = rowT[r] + lane; qq < rowT[r + 1]; qq += INIT_SUB) {
                     const int s = colT[qq];
-                    k += (s < r && !isb(bnd, s));
+                    k += (s < r && !isb(bnd, s) && (!mayT || (mayT[qq] & MAYPAIR)));
                 }
             for (int p = row[r] + lane; p < row[r + 1]; p += INIT_SUB) {
                 const int c = col[p];
                 if (isb(bnd, c)) continue;
                 if (c < r) {
-                    k += sym ? 2 : 1;   // symmetric: the rowT part is the same set
+                    // B: r may need c's group (c STRONG); symmetric patterns: A as well (c may
+                    // pair r: entry (c, r), the transposed flag of the same index)
+                    if (!mayp) k += sym ? 2 : 1;
+                    else k += ((mayp[p] & STRONG) ? 1 : 0) + (sym && (mayT[p] & MAYPAIR) ? 1 : 0);
                 } else if (!mayp || !(mayp[p] & IGNORES)) {
                     // s could pair c first (unless s may not pair c)
                     for (int qq = rowT[c]; qq < rowT[c + 1]; qq++) {
@@ -486,17 +536,27 @@ __global__ void k_dep_fill(int n, const int *row, const int *col, const int *row
             const int p = b0 + lane;
             int c = -1, tb = 0, tl = 0;
             bool a_ok = false;
+            unsigned a_two = two;
             if (p < re) {
                 c = col[p];
                 if (c > d && !isb(bnd, c)) {
-                    a_ok = true;
                     tb = rowT[c];
                     tl = (!mayp || (mayp[p] & MAYPAIR)) ? rowT[c + 1] - tb : 0;   // C part only if d may pair c
+                    if (!mayp) {
+                        a_ok = true;
+                    } else {
+                        // A: d may pair c; symmetric patterns: B as well (c needs d's group:
+                        // entry (c, d), the transposed flag of the same index)
+                        const int a = (mayp[p] & MAYPAIR) ? 1 : 0;
+                        const int b = (sym && (mayT[p] & STRONG)) ? 1 : 0;
+                        a_ok = a + b > 0;
+                        a_two = a + b == 2 ? 0x80000000u : 0u;
+                    }
                 }
             }
             // A: x in row(d), x > d
             const unsigned long long ma = __ballot(a_ok) & smask;
-            if (FILL && a_ok) out[pos + __popcll(ma & below)] = (int)((unsigned)c | two);
+            if (FILL && a_ok) out[pos + __popcll(ma & below)] = (int)((unsigned)c | a_two);
             pos += __popcll(ma);
             // C: x in rowT(c) with d < x < c
             walk_ranges<S>(tb, tl, [&](int owner, int idx) {
@@ -522,7 +582,8 @@ __global__ void k_dep_fill(int n, const int *row, const int *col, const int *row
                 int x = -1;
                 if (p < te) {
                     x = colT[p];
-                    w = x > d && !isb(bnd, x);
+                    // x needs d's group only if d is one of x's strongest lower neighbours
+                    w = x > d && !isb(bnd, x) && (!mayT || (mayT[p] & STRONG));
                 }
                 const unsigned long long mb = __ballot(w) & smask;
                 if (FILL && w) out[pos + __popcll(mb & below)] = x;
@@ -542,7 +603,10 @@ struct alignas(32) RowRec {
     int dround;     // round the row was grouped in (INT_MAX: not yet, -1: boundary)
     int ldr;        // leader (seed) of its group, -1 while ungrouped
     int db, de;     // range of its dependents list
-    int pad0, pad1;
+    int blk;        // the undecided row b a blocked join of this row waits for: b for the
+                    // row whose group it joins, -2 - b for a row that could still join it
+                    // (-1: none)
+    int wh;         // first row waiting for this one to decide (-1: none, WCLOSED: decided)
 };
 
 __global__ void k_rec_pack(int n, const double *diag, const int *dptr, const int *dround, const int *ldr,
@@ -554,7 +618,8 @@ __global__ void k_rec_pack(int n, const double *diag, const int *dptr, const int
         r.ldr = ldr[i];
         r.db = dptr[i];
         r.de = dptr[i + 1];
-        r.pad0 = r.pad1 = 0;
+        r.blk = -1;
+        r.wh = -1;
         rec[i] = r;
     }
 }
@@ -566,6 +631,14 @@ __global__ void k_rec_unpack(int n, const RowRec *rec, int *dround, int *ldr) {
     }
 }
 
+// The next waiter after x (read by lane 0 of the sub-group, given to all its lanes).
+template <int S, bool WG>
+__device__ __forceinline__ int next_waiter(int x, int *wnext) {
+    int nx = 0;
+    if ((threadIdx.x & (S - 1)) == 0) nx = atom_xchg<WG>(&wnext[x], WTAKEN);
+    return sub_get<S>(nx, 0);
+}
+
 // Decide one frontier row with the S lanes of its sub-group. Returns via the
 // queue (fan-out) and the shared state arrays. Loads that a later step may need
 // (the first dependents of r, the dependents range of every neighbour, the group
@@ -575,7 +648,7 @@ template <int S, bool WG>
 __device__ void decide_row(int r, int t, double theta, const int *row, const int *col, const int *rowT,
                            const int *colT, const double *off, const uint8_t *bnd, RowRec *rec,
                            const int *dep, int *mem, int *memr, int *mcnt, int *cnt, int *q, int *qcnt,
-                           int *nretry) {
+                           int *nretry, int *wnext, bool wall) {
     const int lane = threadIdx.x & (S - 1);
     // state, row range, dependents range and diagonal of r in one step
     const RowRec rr = rec[r];
@@ -583,10 +656,30 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
     const int dr = rr.dround, db = rr.db, de = rr.de;
     const double d0 = rr.ad;
     if (dr < t) return;   // already paired by an earlier seed (its fan-out is done)
+    const int rb0 = rr.blk >= 0 ? rr.blk : -2 - rr.blk;
+    if (rr.blk != -1 && !(rec[rb0].dround < t)) {
+        // a blocked join whose blocker has not decided yet: waits again (its list,
+        // or the next round), without re-evaluating the decision
+        bool parked = false;
+        if (lane == 0) {
+            atom_add<WG>(nretry, 1);
+            parked = (wall || rr.blk >= 0) && park<WG>(r, &rec[rb0].wh, wnext);
+        }
+        PushBuf pb;
+        pb.n = 0;
+        pb_push<WG>(pb, lane == 0 && !parked, r, q, qcnt);
+        pb_flush<WG>(pb, q, qcnt);
+        return;
+    }
     const int pre = db + lane < de ? dep[db + lane] : 0;   // first S dependents of r
+    // r's waiter list is closed now, while the neighbours load, and opened again
+    // with the same waiters if r turns out blocked; rows that find it closed in
+    // between go to the next frontier themselves
+    int w0 = -1, w1 = -1;
+    if (wnext && lane == 0) w0 = atom_xchg<WG>(&rec[r].wh, WCLOSED);
     // strongest ungrouped (u) and grouped (g) neighbours; ties: first in CSR order
     Cand bu{-1.0, INT_MAX}, bg{-1.0, INT_MAX};
-    int cu = -1, cg = -1, ub = 0, ue = 0, lg = -1;
+    int cu = -1, cg = -1, ub = 0, ue = 0, lg = -1, rg = -1;
     for (int p = rb + lane; p < re; p += S) {
         const int c = col[p];
         const double o = off[p];
@@ -596,10 +689,12 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
         const int dc = rc.dround;
         const int cb = rc.db, ce = rc.de, lc = rc.ldr;
         const double w = fabs(fabs(o) / (d0 < d1 ? d1 : d0));
-        if (!(dc < t)) {
+        // grouped: every lower row (the sequential loop has processed it), and a
+        // higher one paired in an earlier round
+        if (!(c < r) && !(dc < t)) {
             if (bu.p == INT_MAX || w > bu.w) { bu.w = w; bu.p = p; cu = c; ub = cb; ue = ce; }
         } else {
-            if (bg.p == INT_MAX || w > bg.w) { bg.w = w; bg.p = p; cg = c; lg = lc; }
+            if (bg.p == INT_MAX || w > bg.w) { bg.w = w; bg.p = p; cg = c; lg = lc; rg = dc; }
         }
     }
     // the sub-group's best of each kind; the lane that holds it (position p is
@@ -615,16 +710,21 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
     }
     if (bg.p != INT_MAX) {
         const int l = (bg.p - rb) & (S - 1);
-        cg = sub_get<S>(cg, l); lg = sub_get<S>(lg, l);
+        cg = sub_get<S>(cg, l); lg = sub_get<S>(lg, l); rg = sub_get<S>(rg, l);
     } else {
         cg = -1;
     }
     int action;  // 0 pair, 1 new singleton group, 2 join cg's group, 3 retry next round
-    int g = -1, slot = 0;
+    int g = -1, slot = 0, blocker = -1;
+    bool longw = false;   // blocked by the row whose group r joins (not one that could join it)
     if (cu != -1 && (cg == -1 || mu > theta * mg)) {
         action = 0;
     } else if (cg == -1) {
         action = 1;
+    } else if (!(rg < t)) {
+        action = 3;   // cg (a lower row) has not decided yet: its group is not known
+        blocker = cg;
+        longw = true;
     } else {
         g = lg;   // ldr[cg], final since cg was grouped in an earlier round
         // members of g placed before round t (one lane each), and every s < r
@@ -644,6 +744,7 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
         }
         const int count = __popcll(__ballot(is_mem) & sub_mask<S>());
         bool blocked = false;
+        int bs = -1;
         // (a boundary row never decides and holds round -1, so it never blocks)
         walk_members<S, JWB>(tb, tl, [&](const int (&idx)[JWB]) {
             int sv[JWB], dv[JWB];
@@ -653,20 +754,36 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
             for (int w = 0; w < JWB; w++) dv[w] = (sv[w] >= 0 && sv[w] < r) ? rec[sv[w]].dround : -1;
 #pragma unroll
             for (int w = 0; w < JWB; w++)
-                if (!(dv[w] < t)) blocked = true;
+                if (!(dv[w] < t)) {
+                    blocked = true;
+                    bs = sv[w];
+                }
         });
-        // every lane of the sub-group takes part in the vote
-        blocked = (__ballot(blocked) & sub_mask<S>()) != 0;
+        // every lane of the sub-group takes part in the vote; the first blocked lane
+        // names the row to wait for
+        const unsigned long long bm = __ballot(blocked) & sub_mask<S>();
+        if (bm) blocker = sub_get<S>(bs, (__ffsll((long long)bm) - 1) & (S - 1));
+        blocked = bm != 0;
         action = blocked ? 3 : ((count > 2 + 2) ? 1 : 2);   // coarseCount > groupSize+2
     }
     PushBuf pb;
     pb.n = 0;
     if (action == 3) {
-        if (lane == 0) atom_add<WG>(nretry, 1);   // statistics: blocked joins
-        pb_push<WG>(pb, lane == 0, r, q, qcnt);
+        bool parked = false;
+        if (lane == 0) {
+            atom_add<WG>(nretry, 1);   // statistics: blocked joins
+            rec[r].blk = longw ? blocker : -2 - blocker;
+            if (wnext) atom_xchg<WG>(&rec[r].wh, w0);   // open r's list again
+            // a row that could still join the group mostly decides within a round or
+            // two: waiting in its list pays off only with wall (retries are cheaper)
+            parked = (wall || longw) && park<WG>(r, &rec[blocker].wh, wnext);
+        }
+        pb_push<WG>(pb, lane == 0 && !parked, r, q, qcnt);
         pb_flush<WG>(pb, q, qcnt);
         return;
     }
+    // the partner's list is closed before its state is written
+    if (wnext && lane == 0 && action == 0) w1 = atom_xchg<WG>(&rec[cu].wh, WCLOSED);
     if (lane == 0) {
         if (action == 0) {
             rec[r].ldr = r; rec[cu].ldr = r;
@@ -686,6 +803,7 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
             rec[r].dround = t;
         }
     }
+    // the waiters of r and of its partner are walked after the fan-out
     // fan-out: the dependents of r and, for a pair, of its partner. A lane issues
     // the decrements of up to FANB dependents before it waits for any, so a long
     // list costs one atomic round trip per FANB * S entries instead of per S.
@@ -708,6 +826,12 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
 #pragma unroll
         for (int u = 0; u < FANB; u++) pb_push<WG>(pb, x[u] >= 0 && old[u] == dec[u], x[u], q, qcnt);
     }
+    if (wnext) {
+        w0 = sub_get<S>(w0, 0);
+        w1 = sub_get<S>(w1, 0);
+        for (int x = w0; x >= 0; x = next_waiter<S, WG>(x, wnext)) pb_push<WG>(pb, lane == 0, x, q, qcnt);
+        for (int x = w1; x >= 0; x = next_waiter<S, WG>(x, wnext)) pb_push<WG>(pb, lane == 0, x, q, qcnt);
+    }
     pb_flush<WG>(pb, q, qcnt);
 }
 
@@ -718,7 +842,7 @@ __global__ void __launch_bounds__(256)
 k_coarsen_round(int t, int n, const int *row, const int *col, const int *rowT, const int *colT,
                 const double *off, const uint8_t *bnd, RowRec *rec, const int *dep, double theta, int *mem,
                 int *memr, int *mcnt, int *cnt, const int *cur, const int *cur_cnt, int *nxt, int *nxt_cnt,
-                int *zero_cnt, int *nretry) {
+                int *zero_cnt, int *nretry, int *wnext, int wall) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *zero_cnt = 0;
     // one sub-group of S lanes per frontier row
     const int sub = (blockIdx.x * blockDim.x + threadIdx.x) / S;
@@ -733,7 +857,7 @@ k_coarsen_round(int t, int n, const int *row, const int *col, const int *rowT, c
         const int i = base + (sub % (64 / S));
         if (i < nf)
             decide_row<S, false>(i == sub ? r0 : cur[i], t, theta, row, col, rowT, colT, off, bnd, rec, dep,
-                                 mem, memr, mcnt, cnt, nxt, nxt_cnt, nretry);
+                                 mem, memr, mcnt, cnt, nxt, nxt_cnt, nretry, wnext, wall != 0);
     }
 }
 
@@ -748,7 +872,7 @@ __global__ void __launch_bounds__(1024)
 k_coarsen_small(int n, int max_rounds, const int *row, const int *col, const int *rowT, const int *colT,
                 const double *off, const uint8_t *bnd, RowRec *rec, const int *dep, double theta, int *mem,
                 int *memr, int *mcnt, int *cnt, int *qa, int *qb, const int *qcnt0, int *rounds_out,
-                int *nretry) {
+                int *nretry, int *wnext, int wall) {
     __shared__ int s_q[2];   // s_q[t & 1]: size of the frontier of round t
     int *cur = qa, *nxt = qb;
     if (threadIdx.x == 0) {
@@ -769,7 +893,7 @@ k_coarsen_small(int n, int max_rounds, const int *row, const int *col, const int
             const int i = base + (__lane_id() / S);
             if (i < nf)
                 decide_row<S, true>(cur[i], t, theta, row, col, rowT, colT, off, bnd, rec, dep, mem, memr, mcnt,
-                                    cnt, nxt, &s_q[(t + 1) & 1], nretry);
+                                    cnt, nxt, &s_q[(t + 1) & 1], nretry, wnext, wall != 0);
         }
         __syncthreads();
         if (threadIdx.x == 0) s_q[t & 1] = 0;   // every thread has read it: the counter of round t + 2
@@ -1496,7 +1620,8 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
         mayp.alloc(L.nnz);
         mayT.alloc(L.nnz);
         const unsigned gm = grid_for((long long)n * MP, BS, 65536);
-        k_may_pair<<<gm, BS, 0, s>>>(n, L.row.p, L.col.p, L.off.p, L.diag.p, bnd, theta, mayp.p);
+        k_may_pair<<<gm, BS, 0, s>>>(n, L.row.p, L.col.p, L.off.p, L.diag.p, bnd, theta, h->coarsen_lower_filter,
+                                     mayp.p);
         k_may_pair_t<<<gm, BS, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym ? nullptr : L.tposT.p, mayp.p,
                                        mayT.p);
         FVM_HIP(hipGetLastError());
@@ -1532,6 +1657,11 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
     DArr<RowRec> rec(std::max(n, 1));
     k_rec_pack<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.diag.p, dptr.p, dround.p, ldr.p, rec.p);
     FVM_HIP(hipGetLastError());
+    // links of the waiter lists (nullptr: blocked joins retry every round)
+    DArr<int> wlink;
+    if (h->coarsen_wait_lists) wlink.alloc(std::max(n, 1));
+    int *wn = h->coarsen_wait_lists ? wlink.p : nullptr;
+    const int wall = h->coarsen_wait_lists >= 2;   // also joins blocked by a row that could join
     int t = 1;
     const long long cap = 4LL * n + 64;
     const char *path;
@@ -1548,7 +1678,7 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
         auto launch_small = [&](auto kern) {
             kern<<<1, 1024, 0, s>>>(n, (int)std::min<long long>(cap, INT_MAX - 1), L.row.p, L.col.p, rowT, colT,
                                     L.off.p, bnd, rec.p, dep.p, theta, mem.p, memr.p, mcnt.p, cnt.p, q1, q0, qc + 1,
-                                    rounds_out, ctr.p + 6);
+                                    rounds_out, ctr.p + 6, wn, wall);
         };
         const int ssub = h->coarsen_small_lanes ? h->coarsen_small_lanes : sub;
         if (ssub == 8) launch_small(k_coarsen_small<8>);
@@ -1574,7 +1704,7 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
                     kern<<<g_round, 256, 0, s>>>(
                         t, n, L.row.p, L.col.p, rowT, colT, L.off.p, bnd, rec.p, dep.p, theta, mem.p, memr.p,
                         mcnt.p, cnt.p, (t & 1) ? q1 : q0, qc + (t % 3), (t & 1) ? q0 : q1, qc + ((t + 1) % 3),
-                        qc + ((t + 2) % 3), ctr.p + 6);
+                        qc + ((t + 2) % 3), ctr.p + 6, wn, wall);
                 };
                 if (sub == 8) launch(k_coarsen_round<8>);
                 else if (sub == THIN_SUB) launch(k_coarsen_round<THIN_SUB>);

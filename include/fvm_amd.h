@@ -201,9 +201,10 @@ int fvm_amg_kernel_bytes(const fvm_amg *h, int kind, double *total_bytes);
  * this many finest levels are read with non-temporal loads (default 2);
  * "coarsen_lanes" -- lanes per row in the agglomeration rounds (8/16/32/64; 0,
  * the default: chosen per level from its row width); "coarsen_small_rows" --
- * levels with at most this many rows are agglomerated by one workgroup;
- * "coarsen_small_lanes" -- lanes per row in those single-workgroup rounds (0: as
- * coarsen_lanes); "coarsen_grid" -- blocks per launched agglomeration round (0:
+ * levels with at most this many rows are agglomerated by one workgroup
+ * (default 2048);
+ * "coarsen_small_lanes" -- lanes per row in those single-workgroup rounds
+ * (default 16; 0: as coarsen_lanes); "coarsen_grid" -- blocks per launched agglomeration round (0:
  * 2048 for narrow rows, 1024 otherwise); "bipartite_levels" -- this many finest
  * levels are first coloured by breadth-first parity (red-black order, two
  * colours) and keep it when no edge joins equal colours, else the greedy
@@ -221,7 +222,16 @@ int fvm_amg_kernel_bytes(const fvm_amg *h, int kind, double *total_bytes);
  * sees no progress for two seconds gives up regardless);
  * "exact_wg_width" -- in level-by-level reference-order sweeps, each run of
  * consecutive dependency levels of at most this many rows is one
- * single-workgroup launch, every wider level a launch of its own (default 256). */
+ * single-workgroup launch, every wider level a launch of its own (default 256);
+ * "coarsen_pair_filter" -- 1 (default): an agglomeration row does not wait for
+ * a seed that can never pair its neighbour, nor for a neighbour whose decision
+ * ignores it (its strongest lower neighbour dominates); 0: every dependency of
+ * the reference order is waited for; "coarsen_lower_filter" -- 1 (default): a
+ * row waits only for its strongest lower neighbours and those that may pair it;
+ * "coarsen_wait_lists" -- 2 (default): a blocked join waits in a list of the row
+ * it waits for and returns to the frontier when that row decides; 1: only joins
+ * waiting for the group they join do; 0: blocked joins retry every round. These
+ * three leave the aggregates unchanged. */
 int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value);
 int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value);
 /* Autotuner (no counterpart in the reference): times every candidate launch

@@ -306,6 +306,28 @@ def test_coarsening_paths_agree(small, kat, lanes):
     check_hierarchy(ga, oa, m)
 
 
+@pytest.mark.parametrize("pair,lower,waits", [(0, 0, 2), (1, 0, 0), (1, 0, 2), (1, 1, 0), (1, 1, 1), (1, 1, 2)])
+@pytest.mark.parametrize("case", ["grid", "kat", "grid_small", "kat_small", "grid32"])
+def test_coarsening_filters_and_wait_lists_agree(pair, lower, waits, case):
+    # the dependency filters and the blocked joins' waiter lists change the rounds,
+    # never the maps: every combination gives the oracle's hierarchy
+    if case.startswith("kat"):
+        m = O.load_mm(os.path.join(G, "MatrixMarket226.dat"), os.path.join(G, "rhs226.dat"))
+    elif case == "grid32":
+        m = matgen.grid3d(32, seed=23, contrast=2.5)
+    else:
+        m = matgen.grid3d(14, seed=21, contrast=2.0)
+    oa = oracle_for(m)
+    ga = gpu_amg()
+    ga.set_tuning("coarsen_small_rows", 1 << 30 if case.endswith("small") else 0)
+    ga.set_tuning("coarsen_pair_filter", pair)
+    ga.set_tuning("coarsen_lower_filter", lower)
+    ga.set_tuning("coarsen_wait_lists", waits)
+    ls = ls_from(m)
+    ga.setup(ls)
+    check_hierarchy(ga, oa, m)
+
+
 @pytest.mark.parametrize("prec,smoother,npre", [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)])
 def test_tail_kernel_is_bit_identical(prec, smoother, npre):
     # neither the single-workgroup tail nor replaying levels >= 1 from a captured
