@@ -163,7 +163,9 @@ struct fvm_amg {
     long long coarsen_small_rows = 2048;
     int coarsen_lanes = 0;             // lanes per row of the agglomeration rounds (0: by row width)
     int coarsen_small_lanes = 16;      // the same for the single-workgroup rounds (0: as coarsen_lanes)
-    int coarsen_grid = 0;              // blocks per launched agglomeration round (0: 2048 / 1024)
+    int coarsen_grid = 0;              // blocks per launched agglomeration round (0: 2048 / 1024 per 256 threads)
+    int coarsen_block = 512;           // threads per block of a launched agglomeration round
+    int coarsen_lds_entries = 8;       // entries per thread of such a block's next-frontier queue in LDS
     int bipartite_levels = 1;          // levels tried with a breadth-first two-colouring first (red-black)
     int coarsen_pair_filter = 1;       // rows do not wait on seeds that can never pair their neighbour
     int coarsen_lower_filter = 1;      // ... nor on lower neighbours other than the strongest ones

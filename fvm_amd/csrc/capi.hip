@@ -506,6 +506,12 @@ int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value) {
         h->coarsen_pair_filter = value != 0;
     } else if (k == "coarsen_lower_filter") {
         h->coarsen_lower_filter = value != 0;
+    } else if (k == "coarsen_block") {
+        FVM_CHECK(value == 256 || value == 512 || value == 1024, FVM_E_INVALID, "coarsen_block must be 256, 512 or 1024");
+        h->coarsen_block = (int)value;
+    } else if (k == "coarsen_lds_entries") {
+        FVM_CHECK(value >= 0 && value <= 16, FVM_E_INVALID, "coarsen_lds_entries must be in [0, 16]");
+        h->coarsen_lds_entries = (int)value;
     } else if (k == "coarsen_wait_lists") {
         FVM_CHECK(value >= 0 && value <= 2, FVM_E_INVALID, "coarsen_wait_lists must be 0, 1 or 2");
         h->coarsen_wait_lists = (int)value;
@@ -557,6 +563,8 @@ int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value) {
     else if (k == "coarsen_pair_filter") *value = h->coarsen_pair_filter;
     else if (k == "coarsen_lower_filter") *value = h->coarsen_lower_filter;
     else if (k == "coarsen_wait_lists") *value = h->coarsen_wait_lists;
+    else if (k == "coarsen_block") *value = h->coarsen_block;
+    else if (k == "coarsen_lds_entries") *value = h->coarsen_lds_entries;
     else if (k == "reuse_setup") *value = h->reuse_setup;
     else if (k == "setups_reused") *value = h->setup_reused;
     else if (k == "exact_flow_spin_limit") *value = h->flow_spin_limit;

@@ -153,9 +153,12 @@ __device__ __forceinline__ int atom_add(int *p, int v) {
 // atomic per wavefront (the queue tail is a single hot counter).
 constexpr int PB = 4;
 constexpr int FANB = 4;   // dependents per lane whose decrements are in flight together
+
 struct PushBuf {
     int v[PB];
     int n;
+    int *lb, *ln;   // the block's queue in LDS and its size (nullptr: append to q directly)
+    int lcap;       // entries the LDS queue holds
 };
 
 template <bool WG>
@@ -173,6 +176,29 @@ __device__ __forceinline__ void pb_flush(PushBuf &pb, int *q, int *qcnt) {
     const int lane = __lane_id();
     const int leader = __ffsll((long long)__ballot(1)) - 1;
     int base = 0;
+    if (pb.lb) {
+        // the block's LDS queue, appended to q once per block; what does not fit
+        // goes to q at once
+        if (lane == leader) base = __hip_atomic_fetch_add(pb.ln, tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        base = __shfl(base, leader, 64);
+        const int lo = max(base, pb.lcap);   // first slot past the LDS queue
+        const int spill = base + tot - lo;
+        int gb = 0;
+        if (spill > 0) {
+            if (lane == leader) gb = atom_add<WG>(qcnt, spill);
+            gb = __shfl(gb, leader, 64);
+        }
+        const unsigned long long lt = (1ull << lane) - 1ull;
+#pragma unroll
+        for (int k = 0; k < PB; k++)
+            if (pb.n > k) {
+                const int idx = base + before[k] + __popcll(m[k] & lt);
+                if (idx < pb.lcap) pb.lb[idx] = pb.v[k];
+                else q[gb + idx - lo] = pb.v[k];
+            }
+        pb.n = 0;
+        return;
+    }
     if (lane == leader) base = atom_add<WG>(qcnt, tot);
     base = __shfl(base, leader, 64);
     const unsigned long long lt = (1ull << lane) - 1ull;
@@ -648,7 +674,7 @@ template <int S, bool WG>
 __device__ void decide_row(int r, int t, double theta, const int *row, const int *col, const int *rowT,
                            const int *colT, const double *off, const uint8_t *bnd, RowRec *rec,
                            const int *dep, int *mem, int *memr, int *mcnt, int *cnt, int *q, int *qcnt,
-                           int *nretry, int *wnext, bool wall) {
+                           int *nretry, int *wnext, bool wall, int *lb, int *ln, int lcap) {
     const int lane = threadIdx.x & (S - 1);
     // state, row range, dependents range and diagonal of r in one step
     const RowRec rr = rec[r];
@@ -662,11 +688,14 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
         // or the next round), without re-evaluating the decision
         bool parked = false;
         if (lane == 0) {
-            atom_add<WG>(nretry, 1);
+            if (nretry) atom_add<WG>(nretry, 1);   // statistics, when asked for
             parked = (wall || rr.blk >= 0) && park<WG>(r, &rec[rb0].wh, wnext);
         }
         PushBuf pb;
         pb.n = 0;
+        pb.lb = lb;
+        pb.ln = ln;
+        pb.lcap = lcap;
         pb_push<WG>(pb, lane == 0 && !parked, r, q, qcnt);
         pb_flush<WG>(pb, q, qcnt);
         return;
@@ -768,10 +797,13 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
     }
     PushBuf pb;
     pb.n = 0;
+    pb.lb = lb;
+    pb.ln = ln;
+    pb.lcap = lcap;
     if (action == 3) {
         bool parked = false;
         if (lane == 0) {
-            atom_add<WG>(nretry, 1);   // statistics: blocked joins
+            if (nretry) atom_add<WG>(nretry, 1);   // statistics: blocked joins
             rec[r].blk = longw ? blocker : -2 - blocker;
             if (wnext) atom_xchg<WG>(&rec[r].wh, w0);   // open r's list again
             // a row that could still join the group mostly decides within a round or
@@ -838,11 +870,15 @@ __device__ void decide_row(int r, int t, double theta, const int *row, const int
 // One round over the frontier (multi-block), one wavefront per frontier row.
 // Reads frontier t from cur, appends frontier t+1 to nxt; zeroes the counter of round t+2.
 template <int S>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 k_coarsen_round(int t, int n, const int *row, const int *col, const int *rowT, const int *colT,
                 const double *off, const uint8_t *bnd, RowRec *rec, const int *dep, double theta, int *mem,
                 int *memr, int *mcnt, int *cnt, const int *cur, const int *cur_cnt, int *nxt, int *nxt_cnt,
-                int *zero_cnt, int *nretry, int *wnext, int wall) {
+                int *zero_cnt, int *nretry, int *wnext, int wall, int lcap) {
+    extern __shared__ int s_lq[];   // lcap entries
+    __shared__ int s_ln, s_base;
+    if (threadIdx.x == 0) s_ln = 0;
+    __syncthreads();
     if (blockIdx.x == 0 && threadIdx.x == 0) *zero_cnt = 0;
     // one sub-group of S lanes per frontier row
     const int sub = (blockIdx.x * blockDim.x + threadIdx.x) / S;
@@ -857,8 +893,14 @@ k_coarsen_round(int t, int n, const int *row, const int *col, const int *rowT, c
         const int i = base + (sub % (64 / S));
         if (i < nf)
             decide_row<S, false>(i == sub ? r0 : cur[i], t, theta, row, col, rowT, colT, off, bnd, rec, dep,
-                                 mem, memr, mcnt, cnt, nxt, nxt_cnt, nretry, wnext, wall != 0);
+                                 mem, memr, mcnt, cnt, nxt, nxt_cnt, nretry, wnext, wall != 0, s_lq, &s_ln, lcap);
     }
+    // the block's queue entries go to the next frontier with one atomic
+    __syncthreads();
+    const int c = min(s_ln, lcap);
+    if (threadIdx.x == 0) s_base = c > 0 ? atomicAdd(nxt_cnt, c) : 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < c; i += blockDim.x) nxt[s_base + i] = s_lq[i];
 }
 
 // All rounds of a level in one workgroup: rounds are separated by workgroup
@@ -893,7 +935,7 @@ k_coarsen_small(int n, int max_rounds, const int *row, const int *col, const int
             const int i = base + (__lane_id() / S);
             if (i < nf)
                 decide_row<S, true>(cur[i], t, theta, row, col, rowT, colT, off, bnd, rec, dep, mem, memr, mcnt,
-                                    cnt, nxt, &s_q[(t + 1) & 1], nretry, wnext, wall != 0);
+                                    cnt, nxt, &s_q[(t + 1) & 1], nretry, wnext, wall != 0, nullptr, nullptr, 0);
         }
         __syncthreads();
         if (threadIdx.x == 0) s_q[t & 1] = 0;   // every thread has read it: the counter of round t + 2
@@ -1006,6 +1048,8 @@ __global__ void k_color_key(int n, const int *color, const int *row, int by_len,
 // (CRMatrix.h:616-640); each value is summed in flattened order by broadcasting the
 // entries one by one (CRMatrix.h:722-755), so every sum adds the same terms in the
 // same order as the sequential loop.
+constexpr int GCH = 4;   // chunks of a coarse row's entries held in registers by k_gal
+
 struct GalEntry {
     int J;        // coarse column of an off-diagonal entry (-1: diagonal entry or dropped)
     int isd;      // 1: diagonal entry of a fine row, or an entry inside the aggregate (J == I)
@@ -1080,9 +1124,58 @@ __global__ void k_gal(int nc, const int *c2f_row, const int *c2f_col, const int 
             fo[t] = m;
             if (t < nf) m += 1 + ln;
         }
-        // pass over the chunks of S entries: distinct columns in first-appearance order
         int ndist = 0;   // distinct columns in the chunks before this one
         double d = 0.0;
+        if (m <= GCH * S) {
+            // up to GCH chunks of S entries held in registers: one pass over all the
+            // entries, broadcast one by one in flattened order, finds every first
+            // appearance and sums every column
+            GalEntry me[GCH];
+            bool first[GCH];
+            double acc[GCH];
+#pragma unroll
+            for (int c = 0; c < GCH; c++) {
+                me[c] = gal_entry(c * S + lane, m, nf, fi, fo, fb, I, col, diag, off, ci);
+                first[c] = me[c].J >= 0;
+                acc[c] = 0.0;
+            }
+#pragma unroll
+            for (int ce = 0; ce < GCH; ce++) {
+                if (ce * S >= m) break;   // uniform over the sub-group
+                for (int k = 0; k < S; k++) {
+                    // every lane takes part in each shuffle
+                    const int Jo = __shfl(me[ce].J, k, S);
+                    const double vo = __shfl(me[ce].v, k, S);
+                    const int isdo = __shfl(me[ce].isd, k, S);
+#pragma unroll
+                    for (int c = 0; c < GCH; c++)
+                        if (me[c].J >= 0 && Jo == me[c].J) {
+                            if (ce * S + k < c * S + lane) first[c] = false;   // an earlier entry
+                            acc[c] += vo;
+                        }
+                    if (lane == 0 && isdo) d += vo;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < GCH; c++) {
+                if (c * S >= m) break;
+                const unsigned long long bal = __ballot(first[c]) & sub_mask<S>();
+                const int below = __popcll(bal & ((1ull << __lane_id()) - 1ull));
+                if (first[c]) {
+                    const int pos = toff[I] + ndist + below;
+                    tcol[pos] = me[c].J;
+                    tval[pos] = acc[c];
+                }
+                ndist += __popcll(bal);
+            }
+            if (lane == 0) {
+                cdiag[I] = d;
+                cnt[I] = ndist;
+            }
+            continue;
+        }
+        // longer rows: pass over the chunks of S entries, distinct columns in
+        // first-appearance order
         for (int c0 = 0; c0 < m; c0 += S) {
             const GalEntry me = gal_entry(c0 + lane, m, nf, fi, fo, fb, I, col, diag, off, ci);
             // first appearance: no earlier entry (earlier chunks, then earlier lanes) with the same J
@@ -1597,6 +1690,8 @@ void ensure_transpose(fvm_amg *h, Level &L) {
     FVM_HIP(hipGetLastError());
 }
 
+constexpr int CTR_STRIDE = 64;   // ints between the agglomeration's counters (256 bytes)
+
 // Agglomerate level L (reference order); fills L.ci, L.nc and returns the rounds used.
 int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
     hipStream_t s = h->stream;
@@ -1606,10 +1701,13 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
     const int *rowT = sym ? L.row.p : L.rowT.p;
     const int *colT = sym ? L.col.p : L.colT.p;
     DArr<int> ldr(n), dround(n), mem((size_t)n * MEMW), memr((size_t)n * MEMW), mcnt(n), cnt(n),
-        lists(2 * (size_t)n + 2), ctr(8);
+        lists(2 * (size_t)n + 2), ctr(6 * CTR_STRIDE);
     FVM_HIP(hipMemsetAsync(ctr.p, 0, ctr.bytes(), s));
-    int *qc = ctr.p;             // qc[0..2] rotating frontier counters
-    int *rounds_out = ctr.p + 4;
+    // counters one cache line apart: every queue append of a round is an atomic on
+    // one of them, and atomics on a shared line wait for each other
+    int *qc = ctr.p;             // qc[0], qc[CTR_STRIDE], qc[2 * CTR_STRIDE]: rotating frontier counters
+    int *rounds_out = ctr.p + 3 * CTR_STRIDE;
+    int *nretry = retries ? ctr.p + 4 * CTR_STRIDE : nullptr;
     const uint8_t *bnd = L.bnd.p;
     int *q0 = lists.p, *q1 = lists.p + n;
     // entries (d, c) along which d can never pair c, and for the counts the position
@@ -1633,7 +1731,7 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
         const long long threads = (long long)n * INIT_SUB;
         const unsigned g = (unsigned)((threads + 255) / 256);
         k_cnt_init<<<g, 256, 0, s>>>(n, L.row.p, L.col.p, rowT, colT, sym, bnd, mp, mt, cnt.p, dround.p,
-                                     ldr.p, memr.p, mcnt.p, q1, qc + 1);
+                                     ldr.p, memr.p, mcnt.p, q1, qc + CTR_STRIDE);
         FVM_HIP(hipGetLastError());
     }
     // dependents lists (count, scan, fill)
@@ -1677,8 +1775,8 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
         path = "single-workgroup";
         auto launch_small = [&](auto kern) {
             kern<<<1, 1024, 0, s>>>(n, (int)std::min<long long>(cap, INT_MAX - 1), L.row.p, L.col.p, rowT, colT,
-                                    L.off.p, bnd, rec.p, dep.p, theta, mem.p, memr.p, mcnt.p, cnt.p, q1, q0, qc + 1,
-                                    rounds_out, ctr.p + 6, wn, wall);
+                                    L.off.p, bnd, rec.p, dep.p, theta, mem.p, memr.p, mcnt.p, cnt.p, q1, q0,
+                                    qc + CTR_STRIDE, rounds_out, nretry, wn, wall);
         };
         const int ssub = h->coarsen_small_lanes ? h->coarsen_small_lanes : sub;
         if (ssub == 8) launch_small(k_coarsen_small<8>);
@@ -1694,17 +1792,23 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
         path = "multi-block";
         // blocks per round launch: grid-stride loops over the frontier, so any
         // size is correct; "coarsen_grid" overrides the default
-        const unsigned g_round = h->coarsen_grid ? (unsigned)h->coarsen_grid : (sub < ROUND_SUB ? 2048u : 1024u);
+        // threads per block ("coarsen_block"): each block appends its part of the
+        // next frontier with one atomic
+        const int bt = h->coarsen_block;
+        const unsigned g_round = h->coarsen_grid ? (unsigned)h->coarsen_grid
+                                                 : (sub < ROUND_SUB ? 2048u : 1024u) * 256u / (unsigned)bt;
+        const int lcap = h->coarsen_lds_entries * bt;   // entries of each block's LDS queue
+        const size_t lds = std::max<size_t>((size_t)lcap * sizeof(int), sizeof(int));
         const int batch = 32;
         h->probe.init();
         int t_prev = t;
         for (int b = 0;; b++) {
             for (int k = 0; k < batch; k++, t++) {
                 auto launch = [&](auto kern) {
-                    kern<<<g_round, 256, 0, s>>>(
+                    kern<<<g_round, bt, lds, s>>>(
                         t, n, L.row.p, L.col.p, rowT, colT, L.off.p, bnd, rec.p, dep.p, theta, mem.p, memr.p,
-                        mcnt.p, cnt.p, (t & 1) ? q1 : q0, qc + (t % 3), (t & 1) ? q0 : q1, qc + ((t + 1) % 3),
-                        qc + ((t + 2) % 3), ctr.p + 6, wn, wall);
+                        mcnt.p, cnt.p, (t & 1) ? q1 : q0, qc + (t % 3) * CTR_STRIDE, (t & 1) ? q0 : q1,
+                        qc + ((t + 1) % 3) * CTR_STRIDE, qc + ((t + 2) % 3) * CTR_STRIDE, nretry, wn, wall, lcap);
                 };
                 if (sub == 8) launch(k_coarsen_round<8>);
                 else if (sub == THIN_SUB) launch(k_coarsen_round<THIN_SUB>);
@@ -1712,7 +1816,7 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
                 else launch(k_coarsen_round<ROUND_SUB>);
             }
             FVM_HIP(hipGetLastError());
-            h->probe.post(b, qc + (t % 3), s);   // frontier of round t (the next one)
+            h->probe.post(b, qc + (t % 3) * CTR_STRIDE, s);   // frontier of round t (the next one)
             if (b > 0 && h->probe.wait(b - 1) == 0) { t = t_prev; break; }
             t_prev = t;
             if (t >= cap) break;
@@ -1736,7 +1840,7 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
     k_assign_ci<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, ldr.p, bnd, gid.p, L.ci.p);
     FVM_HIP(hipGetLastError());
     L.nc = read_scalar(gid.p + n, s);
-    if (retries) *retries = read_scalar(ctr.p + 6, s);
+    if (retries) *retries = read_scalar(nretry, s);
     return t - 1;
 }
 

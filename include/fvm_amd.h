@@ -204,8 +204,12 @@ int fvm_amg_kernel_bytes(const fvm_amg *h, int kind, double *total_bytes);
  * levels with at most this many rows are agglomerated by one workgroup
  * (default 2048);
  * "coarsen_small_lanes" -- lanes per row in those single-workgroup rounds
- * (default 16; 0: as coarsen_lanes); "coarsen_grid" -- blocks per launched agglomeration round (0:
- * 2048 for narrow rows, 1024 otherwise); "bipartite_levels" -- this many finest
+ * (default 16; 0: as coarsen_lanes); "coarsen_grid" -- blocks per launched
+ * agglomeration round (0: 2048 for narrow rows, 1024 otherwise, per 256 threads
+ * of a block); "coarsen_block" -- threads per block of those rounds (256, 512 or
+ * 1024; default 512: each block appends its part of the next frontier with one
+ * atomic); "coarsen_lds_entries" -- that part is staged in LDS, this many
+ * entries per thread (0..16, default 8), and what does not fit is appended at once; "bipartite_levels" -- this many finest
  * levels are first coloured by breadth-first parity (red-black order, two
  * colours) and keep it when no edge joins equal colours, else the greedy
  * colouring (default 1; takes effect at the next setup). Only the colouring

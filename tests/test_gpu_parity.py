@@ -287,12 +287,14 @@ def test_device_entry_points_match_host():
     assert np.array_equal(x.cpu().numpy(), ls.delta)
 
 
-@pytest.mark.parametrize("small,kat,lanes", [(0, 0, 0), (1 << 30, 0, 0), (0, 1, 0), (1 << 30, 1, 0),
-                                              (1 << 30, 0, 8), (1 << 30, 0, 16), (1 << 30, 0, 32),
-                                              (1 << 30, 1, 8), (0, 0, 8), (0, 1, 16)])
-def test_coarsening_paths_agree(small, kat, lanes):
-    # launched rounds and single-workgroup rounds, with every lane width per row,
-    # give the oracle's maps (symmetric grid and the non-symmetric KAT-226)
+@pytest.mark.parametrize("small,kat,lanes,block", [(0, 0, 0, 512), (1 << 30, 0, 0, 512), (0, 1, 0, 512),
+                                                    (1 << 30, 1, 0, 512), (1 << 30, 0, 8, 512),
+                                                    (1 << 30, 0, 16, 512), (1 << 30, 0, 32, 512),
+                                                    (1 << 30, 1, 8, 512), (0, 0, 8, 256), (0, 1, 16, 1024),
+                                                    (0, 0, 0, 1024), (0, 1, 0, 256)])
+def test_coarsening_paths_agree(small, kat, lanes, block):
+    # launched rounds (every block size) and single-workgroup rounds, with every lane
+    # width per row, give the oracle's maps (symmetric grid and the non-symmetric KAT-226)
     if kat:
         m = O.load_mm(os.path.join(G, "MatrixMarket226.dat"), os.path.join(G, "rhs226.dat"))
     else:
@@ -301,6 +303,9 @@ def test_coarsening_paths_agree(small, kat, lanes):
     ga = gpu_amg()
     ga.set_tuning("coarsen_small_rows", small)
     ga.set_tuning("coarsen_lanes", lanes)
+    ga.set_tuning("coarsen_block", block)
+    # 1 or 0 queue entries per thread in LDS: most or all appends spill to the global queue
+    ga.set_tuning("coarsen_lds_entries", {256: 1, 1024: 0}.get(block, 8))
     ls = ls_from(m)
     ga.setup(ls)
     check_hierarchy(ga, oa, m)
