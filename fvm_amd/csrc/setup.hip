@@ -876,17 +876,22 @@ k_coarsen_round(int t, int n, const int *row, const int *col, const int *rowT, c
                 int *memr, int *mcnt, int *cnt, const int *cur, const int *cur_cnt, int *nxt, int *nxt_cnt,
                 int *zero_cnt, int *nretry, int *wnext, int wall, int lcap) {
     extern __shared__ int s_lq[];   // lcap entries
-    __shared__ int s_ln, s_base;
-    if (threadIdx.x == 0) s_ln = 0;
-    __syncthreads();
-    if (blockIdx.x == 0 && threadIdx.x == 0) *zero_cnt = 0;
+    __shared__ int s_ln, s_base, s_nf;
     // one sub-group of S lanes per frontier row
     const int sub = (blockIdx.x * blockDim.x + threadIdx.x) / S;
     const int nsub = (gridDim.x * blockDim.x) / S;
-    // the frontier size and this sub-group's first entry in one step (the list
-    // holds n entries; an entry past the frontier is not used)
-    const int nf = *cur_cnt;
+    // this sub-group's first entry, loaded with the frontier size (the list holds n
+    // entries; an entry past the frontier is not used)
     const int r0 = sub < n ? cur[sub] : 0;
+    if (threadIdx.x == 0) {
+        s_ln = 0;
+        // the frontier size, loaded once per block: every wavefront loading the
+        // same word queues thousands of requests on one L2 channel
+        s_nf = *cur_cnt;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) *zero_cnt = 0;
+    const int nf = s_nf;
     // the loop bound is uniform over each wavefront's sub-groups only up to the
     // last pass: sub-groups past the frontier simply skip the row
     for (int base = sub - (sub % (64 / S)); base < nf; base += nsub) {
@@ -1333,6 +1338,9 @@ __global__ void k_color_prio(int n, const int *row, const int *seg_off, int nseg
 __global__ void k_color_round(int t, int n, const int *row, const int *col, const int *rowT,
                               const int *colT, const unsigned long long *prio, int *color, int *cround,
                               int *left) {
+    __shared__ int s_done;   // rows coloured by the block: one atomic per block on left
+    if (threadIdx.x == 0) s_done = 0;
+    __syncthreads();
     int done = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (cround[i] < t) continue;
@@ -1361,7 +1369,9 @@ __global__ void k_color_round(int t, int n, const int *row, const int *col, cons
         cround[i] = t;
         done++;
     }
-    if (done) atomicSub(left, done);
+    if (done) atomicAdd(&s_done, done);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_done) atomicSub(left, s_done);
 }
 
 // Two-colouring by breadth-first parity (a graph without odd cycles, e.g. the
@@ -1382,10 +1392,22 @@ __global__ void k_bfs_seed(int n, const int *seg_off, int nseg, int *color, int 
     }
 }
 
+constexpr int BFS_LQ = 2048;   // entries of a breadth-first round's per-block LDS queue
+
 __global__ void k_bfs_round(const int *row, const int *col, const int *rowT, const int *colT, int *color,
                             const int *cur, const int *curcnt, int *nxt, int *nxtcnt, int *zero_cnt) {
+    // the frontier size, loaded once per block; the rows reached go to an LDS queue
+    // that the block appends to the next frontier with one atomic (what does not
+    // fit is appended at once)
+    __shared__ int s_nf, s_ln, s_base;
+    __shared__ int s_lq[BFS_LQ];
+    if (threadIdx.x == 0) {
+        s_nf = *curcnt;
+        s_ln = 0;
+    }
+    __syncthreads();
     if (blockIdx.x == 0 && threadIdx.x == 0) *zero_cnt = 0;
-    const int nf = *curcnt;
+    const int nf = s_nf;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nf; t += gridDim.x * blockDim.x) {
         const int i = cur[t];
         const int want = 1 - color[i];
@@ -1395,10 +1417,19 @@ __global__ void k_bfs_round(const int *row, const int *col, const int *rowT, con
             if (!rp) break;
             for (int p = rp[i]; p < rp[i + 1]; p++) {
                 const int j = cp[p];
-                if (atomicCAS(&color[j], -1, want) == -1) nxt[atomicAdd(nxtcnt, 1)] = j;
+                if (atomicCAS(&color[j], -1, want) == -1) {
+                    const int k = atomicAdd(&s_ln, 1);
+                    if (k < BFS_LQ) s_lq[k] = j;
+                    else nxt[atomicAdd(nxtcnt, 1)] = j;
+                }
             }
         }
     }
+    __syncthreads();
+    const int c = min(s_ln, BFS_LQ);
+    if (threadIdx.x == 0) s_base = c > 0 ? atomicAdd(nxtcnt, c) : 0;
+    __syncthreads();
+    for (int k = threadIdx.x; k < c; k += blockDim.x) nxt[s_base + k] = s_lq[k];
 }
 
 // bad[s] != 0: segment s is not two-coloured (each system of a batch decides alone)
