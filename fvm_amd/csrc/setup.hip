@@ -1828,7 +1828,8 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
         const int bt = h->coarsen_block;
         const unsigned g_round = h->coarsen_grid ? (unsigned)h->coarsen_grid
                                                  : (sub < ROUND_SUB ? 2048u : 1024u) * 256u / (unsigned)bt;
-        const int lcap = h->coarsen_lds_entries * bt;   // entries of each block's LDS queue
+        // entries of each block's LDS queue (at most 32 KiB of LDS)
+        const int lcap = std::min(h->coarsen_lds_entries * bt, 8192);
         const size_t lds = std::max<size_t>((size_t)lcap * sizeof(int), sizeof(int));
         const int batch = 32;
         h->probe.init();

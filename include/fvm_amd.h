@@ -209,7 +209,8 @@ int fvm_amg_kernel_bytes(const fvm_amg *h, int kind, double *total_bytes);
  * of a block); "coarsen_block" -- threads per block of those rounds (256, 512 or
  * 1024; default 512: each block appends its part of the next frontier with one
  * atomic); "coarsen_lds_entries" -- that part is staged in LDS, this many
- * entries per thread (0..16, default 8), and what does not fit is appended at once; "bipartite_levels" -- this many finest
+ * entries per thread (0..16, default 8; at most 8192 per block), and what does not
+ * fit is appended at once; "bipartite_levels" -- this many finest
  * levels are first coloured by breadth-first parity (red-black order, two
  * colours) and keep it when no edge joins equal colours, else the greedy
  * colouring (default 1; takes effect at the next setup). Only the colouring
