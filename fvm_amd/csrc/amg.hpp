@@ -184,6 +184,9 @@ struct fvm_amg {
     // single-workgroup tail of the V-cycle: levels >= tail_level run in one launch
     int tail_level = -1;
     int tail_prec = -1;                // precision the descriptors were built for
+    int tail_lds = 1;                  // tail levels that fit sweep from LDS
+    int tail_lds_lim = -1;             // LDS bytes per workgroup of the device (-1: not queried)
+    std::vector<int> tail_lds_bytes[2];   // LDS bytes per level (FP64 / mixed descriptors)
     fvm::DArr<unsigned char> tail_desc;
     // live timing of finest-level kernels with HIP events on the handle's stream
     bool ktime = false;

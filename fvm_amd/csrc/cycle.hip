@@ -635,7 +635,10 @@ struct TailLv {
     const int *pci;                 // fine -> coarse (level + 1), coloured indices
     const int *c2f_row, *c2f_col;   // rows of level + 1 -> rows of this level
     void *b, *x, *t;
+    int lds;                        // bytes of its sweep data staged in LDS (upper bound)
 };
+
+__host__ __device__ constexpr int al16(long long b) { return (int)((b + 15) & ~15LL); }
 
 template <class V, class D, class X>
 __device__ __forceinline__ void tail_rows(const TailLv &L, int beg, int end, const X *xin, X *xout,
@@ -679,6 +682,47 @@ __device__ void tail_sweeps(const TailLv &L, int ns, int jacobi) {
     }
 }
 
+// The sweeps of a level whose matrix and vectors fit in the workgroup's LDS run
+// from there: its SELL columns and values, slice offsets, diagonal, b and x are
+// copied in, swept with the same row code (same arithmetic, same order), and x
+// is copied back. A colour step then waits on LDS instead of the L2.
+template <class V, class D, class X>
+__device__ void tail_sweeps_lds(const TailLv &L, int ns, int jacobi, unsigned char *sm, int cap) {
+    if (jacobi || ns <= 0 || L.lds == 0 || L.lds > cap) {
+        tail_sweeps<V, D, X>(L, ns, jacobi);
+        return;
+    }
+    const int n = L.n, nsl = (n + 63) >> 6;
+    const int E = L.soff[nsl];
+    int o = 0;
+    int *scol = (int *)(sm + o); o += al16(4LL * E);
+    V *val = (V *)(sm + o); o += al16((long long)sizeof(V) * E);
+    int *soff = (int *)(sm + o); o += al16(4LL * (nsl + 1));
+    D *dg = (D *)(sm + o); o += al16((long long)sizeof(D) * n);
+    X *b = (X *)(sm + o); o += al16((long long)sizeof(X) * n);
+    X *x = (X *)(sm + o);
+    const V *gv = (const V *)L.val;
+    for (int i = threadIdx.x; i < E; i += blockDim.x) {
+        scol[i] = L.scol[i];
+        val[i] = gv[i];
+    }
+    for (int i = threadIdx.x; i <= nsl; i += blockDim.x) soff[i] = L.soff[i];
+    const D *gd = (const D *)L.diag;
+    const X *gb = (const X *)L.b;
+    X *gx = (X *)L.x;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        dg[i] = gd[i];
+        b[i] = gb[i];
+        x[i] = gx[i];
+    }
+    __syncthreads();
+    TailLv Ls = L;
+    Ls.scol = scol; Ls.val = val; Ls.soff = soff; Ls.diag = dg; Ls.b = b; Ls.x = x;
+    tail_sweeps<V, D, X>(Ls, ns, jacobi);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) gx[i] = x[i];
+    __syncthreads();
+}
+
 template <class V, class D, class X>
 __device__ void tail_fill0(const TailLv &L) {
     X *x = (X *)L.x;
@@ -687,7 +731,9 @@ __device__ void tail_fill0(const TailLv &L) {
 
 template <class V, class D, class X>
 __global__ void __launch_bounds__(1024)
-k_tail(const TailLv *lv, int l0, int last, int npre, int npost, int jacobi, const double *inv, int direct) {
+k_tail(const TailLv *lv, int l0, int last, int npre, int npost, int jacobi, const double *inv, int direct,
+       int lds_cap) {
+    extern __shared__ __align__(16) unsigned char s_tail[];   // lds_cap bytes
     // descend: level l0 holds its b, its x is zero
     for (int l = l0; l < last; l++) {
         const TailLv &L = lv[l];
@@ -695,7 +741,7 @@ k_tail(const TailLv *lv, int l0, int last, int npre, int npost, int jacobi, cons
         if (npre > 0) {
             tail_fill0<V, D, X>(L);
             __syncthreads();
-            tail_sweeps<V, D, X>(L, npre, jacobi);
+            tail_sweeps_lds<V, D, X>(L, npre, jacobi, s_tail, lds_cap);
             // residual r = b + A x into t
             const V *val = (const V *)L.val;
             const D *dg = (const D *)L.diag;
@@ -736,7 +782,7 @@ k_tail(const TailLv *lv, int l0, int last, int npre, int npost, int jacobi, cons
         } else {
             tail_fill0<V, D, X>(L);
             __syncthreads();
-            tail_sweeps<V, D, X>(L, npre + npost, jacobi);
+            tail_sweeps_lds<V, D, X>(L, npre + npost, jacobi, s_tail, lds_cap);
         }
         __syncthreads();
     }
@@ -754,7 +800,7 @@ k_tail(const TailLv *lv, int l0, int last, int npre, int npost, int jacobi, cons
             }
         }
         __syncthreads();
-        tail_sweeps<V, D, X>(L, npost, jacobi);
+        tail_sweeps_lds<V, D, X>(L, npost, jacobi, s_tail, lds_cap);
     }
 }
 
@@ -990,6 +1036,27 @@ void build_tail_desc(Run &R) {
         t.diag = vt == V64 ? (const void *)L.pdiag.p : (const void *)L.pdiag32.p;
         t.pci = L.pci.p; t.c2f_row = L.pc2f_row.p; t.c2f_col = L.pc2f_col.p;
         t.b = vb<X>(L); t.x = vx<X>(L); t.t = vt_arr<X>(L).p;
+        const long long E = (long long)L.sell_col.n, ns = ((long long)L.n + 63) / 64;
+        const int vs = vt == V64 ? 8 : vt == V32 ? 4 : 2, ds = vt == V64 ? 8 : 4, xs = (int)sizeof(X);
+        const long long need = (long long)al16(4 * E) + al16(vs * E) + al16(4 * (ns + 1)) + al16((long long)ds * L.n) +
+                               2LL * al16((long long)xs * L.n);
+        t.lds = (L.n > 0 && need <= (1 << 30)) ? (int)need : 0;
+    }
+    auto &lb = h->tail_lds_bytes[prec];
+    lb.assign(d.size(), 0);
+    for (size_t l = 0; l < d.size(); l++) lb[l] = d[l].lds;
+    // the device's LDS per workgroup, granted to the tail kernel once (outside any capture)
+    if (h->tail_lds_lim < 0) {
+        int dev = 0, lim = 0;
+        FVM_HIP(hipGetDevice(&dev));
+        FVM_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        h->tail_lds_lim = std::max(lim, 0);
+        if (lim > 0) {
+            FVM_HIP(hipFuncSetAttribute((const void *)k_tail<uint16_t, float, float>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+            FVM_HIP(hipFuncSetAttribute((const void *)k_tail<double, double, double>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+        }
     }
     h->tail_desc.alloc(sizeof(TailLv) * d.size());
     FVM_HIP(hipMemcpyAsync(h->tail_desc.p, d.data(), sizeof(TailLv) * d.size(), hipMemcpyHostToDevice, h->stream));
@@ -1047,12 +1114,19 @@ void cycle(Run &R, int type, int l, bool x_zero) {
         const TailLv *d = (const TailLv *)h->tail_desc.p;
         const int direct = (R.mixed && R.last == h->direct_level) ? 1 : 0;
         const int jac = h->prm.smootherType == FVM_JACOBI ? 1 : 0;
-        if (R.mixed)
-            k_tail<uint16_t, float, float><<<1, 1024, 0, s>>>(d, l, R.last, h->prm.nPreSweeps, h->prm.nPostSweeps,
-                                                               jac, h->dense_inv.p, direct);
-        else
-            k_tail<double, double, double><<<1, 1024, 0, s>>>(d, l, R.last, h->prm.nPreSweeps,
-                                                                h->prm.nPostSweeps, jac, h->dense_inv.p, direct);
+        // LDS for the largest tail level within the limit set up with the descriptors
+        int cap = 0;
+        if (h->tail_lds)
+            for (int k = l; k <= R.last; k++) {
+                const int need = h->tail_lds_bytes[R.mixed ? 1 : 0][k];
+                if (need > cap && need <= h->tail_lds_lim) cap = need;
+            }
+        auto launch = [&](auto kern) {
+            kern<<<1, 1024, cap, s>>>(d, l, R.last, h->prm.nPreSweeps, h->prm.nPostSweeps, jac, h->dense_inv.p,
+                                      direct, cap);
+        };
+        if (R.mixed) launch(k_tail<uint16_t, float, float>);
+        else launch(k_tail<double, double, double>);
         FVM_HIP(hipGetLastError());
         return;
     }
@@ -1309,7 +1383,7 @@ std::string autotune(fvm_amg *h, int reps) {
     h->ktime = false;
     std::vector<int> blocks = {128, 256, 512};
     const int keep_sx = h->sell_xcd, keep_px = h->spmv_xcd, keep_cg = h->cycle_graph;
-    std::vector<long long> tails = {0, 2500, 10000, 40000};
+    std::vector<long long> tails = {0, 1000, 2500, 10000, 40000};
     hipEvent_t e0, e1;
     FVM_HIP(hipEventCreate(&e0));
     FVM_HIP(hipEventCreate(&e1));

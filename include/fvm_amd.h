@@ -190,7 +190,9 @@ int fvm_amg_kernel_timing(const fvm_amg *h, int kind, long long *launches, doubl
 int fvm_amg_kernel_bytes(const fvm_amg *h, int kind, double *total_bytes);
 /* Performance knobs that never change results (no reference counterpart; set
  * by the autotuner): "tail_max_rows" -- V-cycle levels with at most this many
- * rows run in one single-workgroup launch (0 disables); "sell_block" -- threads
+ * rows run in one single-workgroup launch (0 disables); "tail_lds" -- 1
+ * (default): there, a level whose matrix and vectors fit in the workgroup's LDS
+ * is swept from LDS; "sell_block" -- threads
  * per block of the SELL-64 smoother kernels, "spmv_block" -- of the SELL-64
  * residual and SpMV kernels (64/128/256/512 each); "sell_xcd" / "spmv_xcd" --
  * 1: blocks of those kernels are renumbered so that each of the chip's 8 XCDs
@@ -242,7 +244,7 @@ int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value);
 /* Autotuner (no counterpart in the reference): times every candidate launch
  * shape that cannot change a result -- "sell_block" (threads per block of the
  * coloured SELL-64 smoother kernels: 128/256/512) x "tail_max_rows"
- * (0/2500/1e4/4e4) x "sell_xcd" (0/1) x "cycle_graph" (0/1) -- with one warm-up and `reps` timed
+ * (0/1000/2500/1e4/4e4) x "sell_xcd" (0/1) x "cycle_graph" (0/1) -- with one warm-up and `reps` timed
  * cycles each on the handle's current hierarchy and parameters, keeps the
  * fastest, then "nt_levels" (0..3) with those shapes, then times the
  * finest-level residual alone for "spmv_block"

@@ -339,11 +339,14 @@ def test_tail_kernel_is_bit_identical(prec, smoother, npre):
     # graph may change a single bit of the coloured cycle
     m = matgen.grid3d(18, seed=22, contrast=1.5)
     res = []
-    for tail, graph in ((0, 0), (1 << 30, 0), (0, 1), (1 << 30, 1), (2500, 1)):
+    # tail_lds: the tail's levels that fit in LDS are swept from there (1) or not (0)
+    for tail, graph, lds in ((0, 0, 1), (1 << 30, 0, 1), (0, 1, 1), (1 << 30, 1, 1), (2500, 1, 1), (1000, 0, 1),
+                             (1 << 30, 0, 0)):
         ga = gpu_amg(gsOrdering=1, precision=prec, smootherType=smoother, nPreSweeps=npre, nMaxIterations=12,
                      relativeTolerance=1e-30)
         ga.set_tuning("tail_max_rows", tail)
         ga.set_tuning("cycle_graph", graph)
+        ga.set_tuning("tail_lds", lds)
         ls = ls_from(m)
         ga.solve(ls)
         res.append((ls.delta.copy(), ga.history()))
@@ -367,7 +370,7 @@ def test_autotune_keeps_results(prec):
     ls1 = ls_from(m)
     ga.setup(ls1)
     rep = ga.autotune(reps=2)
-    assert len(rep["candidates"]) == 48 and len(rep["spmv_candidates"]) == 6
+    assert len(rep["candidates"]) == 60 and len(rep["spmv_candidates"]) == 6
     ch = rep["chosen"]
     assert any(c["sell_block"] == ch["sell_block"] and c["tail_max_rows"] == ch["tail_max_rows"]
                for c in rep["candidates"])
