@@ -702,9 +702,28 @@ __device__ void tail_sweeps_lds(const TailLv &L, int ns, int jacobi, unsigned ch
     X *b = (X *)(sm + o); o += al16((long long)sizeof(X) * n);
     X *x = (X *)(sm + o);
     const V *gv = (const V *)L.val;
-    for (int i = threadIdx.x; i < E; i += blockDim.x) {
-        scol[i] = L.scol[i];
-        val[i] = gv[i];
+    // eight entries per thread in flight at a time: the copy is a few round trips
+    // to memory instead of one per entry
+    constexpr int SB = 8;
+    for (int i0 = threadIdx.x; i0 < E; i0 += SB * blockDim.x) {
+        int c[SB];
+        V v[SB];
+#pragma unroll
+        for (int u = 0; u < SB; u++) {
+            const int i = i0 + u * blockDim.x;
+            if (i < E) {
+                c[u] = L.scol[i];
+                v[u] = gv[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SB; u++) {
+            const int i = i0 + u * blockDim.x;
+            if (i < E) {
+                scol[i] = c[u];
+                val[i] = v[u];
+            }
+        }
     }
     for (int i = threadIdx.x; i <= nsl; i += blockDim.x) soff[i] = L.soff[i];
     const D *gd = (const D *)L.diag;
