@@ -1895,8 +1895,10 @@ void coarse_operator(fvm_amg *h, Level &F, Level &C) {
     hipStream_t s = h->stream;
     const int nc = F.nc;
     // lanes per coarse row from the mean number of flattened entries per coarse row
+    // (16 lanes hold up to four chunks of a row in registers: measured fastest on
+    // the 200^3 hierarchy, whose rows average 14-36 entries)
     const double mavg = nc ? (double)(F.nnz + F.n) / nc : 0.0;
-    const int S = mavg <= 24 ? 16 : (mavg <= 40 ? 32 : 64);
+    const int S = h->gal_lanes ? h->gal_lanes : (mavg <= 40 ? 16 : (mavg <= 56 ? 32 : 64));
     const unsigned g = (unsigned)std::min<long long>(((long long)nc * S + 255) / 256, 65536);
     // slots of the one-pass product: row I gets as many as it has flattened entries
     // (their total is at most nnz + n of the fine level)
