@@ -477,6 +477,9 @@ int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value) {
         FVM_CHECK(value == 0 || value == 16 || value == 32 || value == 64, FVM_E_INVALID,
                   "gal_lanes must be 0, 16, 32 or 64");
         h->gal_lanes = (int)value;
+    } else if (k == "colour_grid") {
+        FVM_CHECK(value >= 16 && value <= 4096, FVM_E_INVALID, "colour_grid must be in [16, 4096]");
+        h->colour_grid = (int)value;
     } else if (k == "tail_lds") {
         h->tail_lds = value != 0;
     } else if (k == "coarsen_small_rows") {
@@ -572,6 +575,7 @@ int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value) {
     else if (k == "coarsen_block") *value = h->coarsen_block;
     else if (k == "tail_lds") *value = h->tail_lds;
     else if (k == "gal_lanes") *value = h->gal_lanes;
+    else if (k == "colour_grid") *value = h->colour_grid;
     else if (k == "coarsen_lds_entries") *value = h->coarsen_lds_entries;
     else if (k == "reuse_setup") *value = h->reuse_setup;
     else if (k == "setups_reused") *value = h->setup_reused;

@@ -2268,6 +2268,9 @@ void prepare_fast_level(fvm_amg *h, Level &L, int l, hipStream_t s, Workspace &w
     FVM_CHECK(L.sym >= 0, FVM_E_INVALID, "level pattern not analysed before colouring");
     const int *rT = L.sym ? nullptr : L.rowT.p;
     const int *cT = L.sym ? nullptr : L.colT.p;
+    // blocks of a colouring round (grid-stride loops): this stream runs beside the
+    // agglomeration rounds, which are latency-bound and need few CUs but prompt ones
+    const unsigned cgrid = (unsigned)h->colour_grid;
     DArr<int> color(n), cround(n), left(1);
     // segments (systems) of the level: with every one two-coloured the greedy
     // colouring is skipped; with some, it runs and the others keep their red-black
@@ -2287,7 +2290,7 @@ void prepare_fast_level(fvm_amg *h, Level &L, int l, hipStream_t s, Workspace &w
             for (int k = 0; k < 16; k++, t++) {
                 const int *cur = (t & 1) ? qb.p : qa.p;
                 int *nxt = (t & 1) ? qa.p : qb.p;
-                k_bfs_round<<<1024, BS, 0, s>>>(L.row.p, L.col.p, rT, cT, bfs_color.p, cur, cnt.p + (t % 3), nxt,
+                k_bfs_round<<<cgrid, BS, 0, s>>>(L.row.p, L.col.p, rT, cT, bfs_color.p, cur, cnt.p + (t % 3), nxt,
                                                 cnt.p + ((t + 1) % 3), cnt.p + ((t + 2) % 3));
             }
             FVM_HIP(hipGetLastError());
@@ -2313,7 +2316,7 @@ void prepare_fast_level(fvm_amg *h, Level &L, int l, hipStream_t s, Workspace &w
         int t = 1;
         for (;;) {
             for (int k = 0; k < 8; k++, t++)
-                k_color_round<<<grid_for(n, BS, 4096), BS, 0, s>>>(t, n, L.row.p, L.col.p, rT, cT, prio.p,
+                k_color_round<<<std::min(grid_for(n, BS, 4096), cgrid), BS, 0, s>>>(t, n, L.row.p, L.col.p, rT, cT, prio.p,
                                                                    color.p, cround.p, left.p);
             FVM_HIP(hipGetLastError());
             if (read_scalar(left.p, s) == 0) break;
