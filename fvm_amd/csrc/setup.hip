@@ -1333,17 +1333,28 @@ __global__ void k_color_prio(int n, const int *row, const int *seg_off, int nseg
     }
 }
 
-// Jones-Plassmann round: an uncoloured row whose priority beats every uncoloured
-// neighbour's takes the first colour its coloured neighbours leave free.
-__global__ void k_color_round(int t, int n, const int *row, const int *col, const int *rowT,
-                              const int *colT, const unsigned long long *prio, int *color, int *cround,
-                              int *left) {
-    __shared__ int s_done;   // rows coloured by the block: one atomic per block on left
-    if (threadIdx.x == 0) s_done = 0;
+// Jones-Plassmann round over the rows still uncoloured (list cur): a row whose
+// priority beats every uncoloured neighbour takes the first colour its coloured
+// neighbours leave free; the others go to the next list, through a per-block LDS
+// queue appended with one atomic (what does not fit is appended at once). The
+// decisions of a round read only colours of earlier rounds, so the order of the
+// list does not matter.
+constexpr int COL_LQ = 2048;
+
+__global__ void k_color_round(int t, const int *row, const int *col, const int *rowT, const int *colT,
+                              const unsigned long long *prio, int *color, int *cround, const int *cur,
+                              const int *cur_cnt, int *nxt, int *nxt_cnt, int *zero_cnt) {
+    __shared__ int s_nf, s_ln, s_base;
+    __shared__ int s_lq[COL_LQ];
+    if (threadIdx.x == 0) {
+        s_nf = *cur_cnt;   // the list size, loaded once per block
+        s_ln = 0;
+    }
     __syncthreads();
-    int done = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (cround[i] < t) continue;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *zero_cnt = 0;
+    const int nf = s_nf;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nf; k += gridDim.x * blockDim.x) {
+        const int i = cur[k];
         const unsigned long long pi = prio[i];
         bool best = true;
         unsigned long long used[4] = {0, 0, 0, 0};
@@ -1362,16 +1373,22 @@ __global__ void k_color_round(int t, int n, const int *row, const int *col, cons
                 }
             }
         }
-        if (!best) continue;
+        if (!best) {
+            const int q = atomicAdd(&s_ln, 1);
+            if (q < COL_LQ) s_lq[q] = i;
+            else nxt[atomicAdd(nxt_cnt, 1)] = i;
+            continue;
+        }
         int c = 0;
         while (c < 256 && ((used[c >> 6] >> (c & 63)) & 1ull)) c++;
         color[i] = c;
         cround[i] = t;
-        done++;
     }
-    if (done) atomicAdd(&s_done, done);
     __syncthreads();
-    if (threadIdx.x == 0 && s_done) atomicSub(left, s_done);
+    const int c = min(s_ln, COL_LQ);
+    if (threadIdx.x == 0) s_base = c > 0 ? atomicAdd(nxt_cnt, c) : 0;
+    __syncthreads();
+    for (int k = threadIdx.x; k < c; k += blockDim.x) nxt[s_base + k] = s_lq[k];
 }
 
 // Two-colouring by breadth-first parity (a graph without odd cycles, e.g. the
@@ -2271,7 +2288,7 @@ void prepare_fast_level(fvm_amg *h, Level &L, int l, hipStream_t s, Workspace &w
     // blocks of a colouring round (grid-stride loops): this stream runs beside the
     // agglomeration rounds, which are latency-bound and need few CUs but prompt ones
     const unsigned cgrid = (unsigned)h->colour_grid;
-    DArr<int> color(n), cround(n), left(1);
+    DArr<int> color(n), cround(n);
     // segments (systems) of the level: with every one two-coloured the greedy
     // colouring is skipped; with some, it runs and the others keep their red-black
     const int nsys = seg_off ? nseg : 1;
@@ -2312,14 +2329,19 @@ void prepare_fast_level(fvm_amg *h, Level &L, int l, hipStream_t s, Workspace &w
         k_color_prio<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, L.row.p, seg_off, nseg, prio.p);
         FVM_HIP(hipMemsetAsync(color.p, 0, color.bytes(), s));
         k_fill_int<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, cround.p, INT_MAX);
-        k_fill_int<<<1, 1, 0, s>>>(1, left.p, n);
+        // lists of the uncoloured rows (round t reads list t & 1), sizes in rotating counters
+        DArr<int> la(n), lb(n), lcnt(3);
+        k_iota<<<grid_for(n, BS, 4096), BS, 0, s>>>(n, la.p);
+        FVM_HIP(hipMemsetAsync(lcnt.p, 0, lcnt.bytes(), s));
+        k_fill_int<<<1, 1, 0, s>>>(1, lcnt.p + 1, n);
         int t = 1;
         for (;;) {
             for (int k = 0; k < 8; k++, t++)
-                k_color_round<<<std::min(grid_for(n, BS, 4096), cgrid), BS, 0, s>>>(t, n, L.row.p, L.col.p, rT, cT, prio.p,
-                                                                   color.p, cround.p, left.p);
+                k_color_round<<<std::min(grid_for(n, BS, 4096), cgrid), BS, 0, s>>>(
+                    t, L.row.p, L.col.p, rT, cT, prio.p, color.p, cround.p, (t & 1) ? la.p : lb.p, lcnt.p + (t % 3),
+                    (t & 1) ? lb.p : la.p, lcnt.p + ((t + 1) % 3), lcnt.p + ((t + 2) % 3));
             FVM_HIP(hipGetLastError());
-            if (read_scalar(left.p, s) == 0) break;
+            if (read_scalar(lcnt.p + (t % 3), s) == 0) break;   // the list of round t (the next one)
             FVM_CHECK(t < 4 * n + 64, FVM_E_NUMERIC, "colouring did not terminate");
         }
         if (n_two > 0)

@@ -218,7 +218,7 @@ int fvm_amg_kernel_bytes(const fvm_amg *h, int kind, double *total_bytes);
  * colouring (default 1; takes effect at the next setup); "gal_lanes" -- lanes
  * per coarse row of the coarse operator (16/32/64; 0, the default: 16 up to 40
  * flattened entries per coarse row on average, more above); "colour_grid" --
- * blocks per colouring round (16..4096, default 512): the colouring runs on a
+ * blocks per colouring round (16..4096, default 256): the colouring runs on a
  * second stream beside the agglomeration rounds, and fewer blocks leave those
  * latency-bound rounds more of the chip. Only the colouring
  * keys change the coloured smoother's arithmetic (another Gauss-Seidel order);
