@@ -167,6 +167,7 @@ struct fvm_amg {
     int coarsen_block = 512;           // threads per block of a launched agglomeration round
     int gal_lanes = 0;                 // lanes per coarse row of the coarse operator (0: by row length)
     int colour_grid = 256;             // blocks per colouring round (second setup stream)
+    int coarsen_batch = 32;            // launched agglomeration rounds queued between frontier checks
     int coarsen_lds_entries = 8;       // entries per thread of such a block's next-frontier queue in LDS
     int bipartite_levels = 1;          // levels tried with a breadth-first two-colouring first (red-black)
     int coarsen_pair_filter = 1;       // rows do not wait on seeds that can never pair their neighbour

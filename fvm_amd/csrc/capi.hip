@@ -480,6 +480,9 @@ int fvm_amg_set_tuning(fvm_amg *h, const char *key, long long value) {
     } else if (k == "colour_grid") {
         FVM_CHECK(value >= 16 && value <= 4096, FVM_E_INVALID, "colour_grid must be in [16, 4096]");
         h->colour_grid = (int)value;
+    } else if (k == "coarsen_batch") {
+        FVM_CHECK(value >= 4 && value <= 256, FVM_E_INVALID, "coarsen_batch must be in [4, 256]");
+        h->coarsen_batch = (int)value;
     } else if (k == "tail_lds") {
         h->tail_lds = value != 0;
     } else if (k == "coarsen_small_rows") {
@@ -576,6 +579,7 @@ int fvm_amg_get_tuning(const fvm_amg *h, const char *key, long long *value) {
     else if (k == "tail_lds") *value = h->tail_lds;
     else if (k == "gal_lanes") *value = h->gal_lanes;
     else if (k == "colour_grid") *value = h->colour_grid;
+    else if (k == "coarsen_batch") *value = h->coarsen_batch;
     else if (k == "coarsen_lds_entries") *value = h->coarsen_lds_entries;
     else if (k == "reuse_setup") *value = h->reuse_setup;
     else if (k == "setups_reused") *value = h->setup_reused;

@@ -1848,7 +1848,7 @@ int coarsen_level(fvm_amg *h, Level &L, double theta, int *retries = nullptr) {
         // entries of each block's LDS queue (at most 32 KiB of LDS)
         const int lcap = std::min(h->coarsen_lds_entries * bt, 8192);
         const size_t lds = std::max<size_t>((size_t)lcap * sizeof(int), sizeof(int));
-        const int batch = 32;
+        const int batch = h->coarsen_batch;
         h->probe.init();
         int t_prev = t;
         for (int b = 0;; b++) {

@@ -220,7 +220,9 @@ int fvm_amg_kernel_bytes(const fvm_amg *h, int kind, double *total_bytes);
  * flattened entries per coarse row on average, more above); "colour_grid" --
  * blocks per colouring round (16..4096, default 256): the colouring runs on a
  * second stream beside the agglomeration rounds, and fewer blocks leave those
- * latency-bound rounds more of the chip. Only the colouring
+ * latency-bound rounds more of the chip; "coarsen_batch" -- launched agglomeration
+ * rounds queued between two reads of the frontier size (4..256, default 32). Only
+ * the colouring
  * keys change the coloured smoother's arithmetic (another Gauss-Seidel order);
  * the others never change a result; "reuse_setup" -- opt-in: a setup whose
  * matrix arrays and hierarchy parameters have the signature of the hierarchy
