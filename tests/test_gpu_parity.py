@@ -416,6 +416,26 @@ def test_reference_order_sweep_in_one_workgroup_is_bit_identical(cycle):
         assert np.array_equal(hist, out[0][1])
 
 
+@pytest.mark.parametrize("key,good,bad", [
+    ("coarsen_pair_filter", [0, 1], []), ("coarsen_lower_filter", [0, 1], []),
+    ("coarsen_wait_lists", [0, 1, 2], [3, -1]), ("coarsen_block", [256, 512, 1024], [128, 300]),
+    ("coarsen_lds_entries", [0, 8, 16], [17, -1]), ("coarsen_batch", [4, 32, 256], [2, 257]),
+    ("tail_lds", [0, 1], []), ("gal_lanes", [0, 16, 32, 64], [8, 48]), ("colour_grid", [16, 256, 4096], [8, 5000])])
+def test_setup_tuning_keys_round_trip_and_reject_bad_values(key, good, bad):
+    # the agglomeration, coarse-operator, colouring and tail knobs: a value set is the
+    # value read, and one out of range is refused with the previous value kept
+    from fvm_amd._lib import FvmError
+    ga = gpu_amg()
+    for v in good:
+        ga.set_tuning(key, v)
+        assert ga.get_tuning(key) == v
+    keep = ga.get_tuning(key)
+    for v in bad:
+        with pytest.raises(FvmError):
+            ga.set_tuning(key, v)
+        assert ga.get_tuning(key) == keep
+
+
 def test_opt_in_setup_reuse_gives_the_rebuilt_results():
     """reuse_setup 1: a setup of the same matrix (a new system object, even after
     cleanup, which the reference's models call after every solve) keeps the
