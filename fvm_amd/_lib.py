@@ -70,6 +70,7 @@ SIGNATURES = {
     "fvm_amg_num_levels": (_i, [_p]),
     "fvm_amg_level_info": (_i, [_p, _i, _p, _p]),
     "fvm_amg_level_get": (_i, [_p, _i, _p, _p, _p, _p, _p]),
+    "fvm_amg_level_colours": (_i, [_p, _i, _p, _p]),
     "fvm_amg_setup_stats": (_i, [_p, _p, _p]),
     "fvm_amg_direct_level": (_i, [_p, _p, _p, _p, _p]),
     "fvm_amg_set_output": (_i, [_p, ctypes.c_char_p, ctypes.c_char_p, _i]),

@@ -180,6 +180,16 @@ class AMG:
         return dict(n=n.value, nnz=nnz.value, row=row, col=col[:nnz.value], diag=diag[:n.value],
                     off=off[:nnz.value], ci=ci[:n.value])
 
+    def level_colours(self, l):
+        """Colour of every row of level l (reference order; -1 without colouring) and
+        the number of colours (fvm_amg_level_colours)."""
+        n = ctypes.c_int()
+        check(lib().fvm_amg_level_info(self._h, l, ctypes.byref(n), None))
+        col = np.zeros(max(n.value, 1), np.int32)
+        k = ctypes.c_int()
+        check(lib().fvm_amg_level_colours(self._h, l, _ptr(col), ctypes.byref(k)))
+        return col[:n.value], k.value
+
     def direct_level(self, with_inverse=False):
         """Coarsest dense solve of the mixed cycle: dict(level, n, attempts[, inv])."""
         lv, n, att = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -388,6 +388,33 @@ int fvm_amg_level_get(const fvm_amg *h, int level, int32_t *row, int32_t *col, d
     FVM_API_END
 }
 
+int fvm_amg_level_colours(const fvm_amg *h, int level, int32_t *colour, int *ncolour) {
+    FVM_API_BEGIN
+    FVM_CHECK(h && h->ready && level >= 0 && level < (int)h->lev.size(), FVM_E_INVALID, "bad level");
+    DeviceGuard g(h->device);
+    const fvm::Level &L = h->lev[level];
+    const bool have = L.ncolor > 0 && L.iperm.p && (int)L.color_ptr.size() == L.ncolor + 1;
+    if (ncolour) *ncolour = have ? L.ncolor : 0;
+    if (colour && L.n > 0) {
+        if (!have) {
+            for (int i = 0; i < L.n; i++) colour[i] = -1;
+        } else {
+            // colour of row i: the colour range holding its position iperm[i] in the
+            // coloured order
+            std::vector<int> ip(L.n);
+            FVM_HIP(hipMemcpyAsync(ip.data(), L.iperm.p, sizeof(int) * L.n, hipMemcpyDeviceToHost, h->stream));
+            FVM_HIP(hipStreamSynchronize(h->stream));
+            for (int i = 0; i < L.n; i++) {
+                const int p = ip[i];
+                int c = (int)(std::upper_bound(L.color_ptr.begin(), L.color_ptr.end(), p) - L.color_ptr.begin()) - 1;
+                colour[i] = (p >= 0 && p < L.n && c >= 0 && c < L.ncolor) ? c : -1;
+            }
+        }
+    }
+    return FVM_OK;
+    FVM_API_END
+}
+
 int fvm_amg_set_output(fvm_amg *h, const char *field_name, const char *path, int style) {
     FVM_API_BEGIN
     FVM_CHECK(h, FVM_E_INVALID, "null handle");

@@ -164,6 +164,11 @@ int fvm_amg_level_info(const fvm_amg *h, int level, int *n_rows, int *nnz);
  * coarsest level). */
 int fvm_amg_level_get(const fvm_amg *h, int level, int32_t *row, int32_t *col, double *diag,
                       double *off, int32_t *ci);
+/* Colours of a level's coloured smoother (no reference counterpart; for tests and
+ * diagnostics): colour[i] of every row in reference order (may be NULL; -1 on a
+ * level without colouring), *ncolour the number of colours (0 without). Rows of
+ * one colour share no matrix entry. */
+int fvm_amg_level_colours(const fvm_amg *h, int level, int32_t *colour, int *ncolour);
 /* Coarsest dense solve of the mixed-precision cycle (no reference counterpart):
  * the first coarse level with at most 256 rows is inverted in FP64 by Gauss-Jordan
  * with partial pivoting; a singular level keeps the smoother instead (level -1)

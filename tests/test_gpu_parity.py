@@ -436,6 +436,37 @@ def test_setup_tuning_keys_round_trip_and_reject_bad_values(key, good, bad):
         assert ga.get_tuning(key) == keep
 
 
+@pytest.mark.parametrize("case", ["grid", "kat", "grid32"])
+def test_every_level_colouring_is_proper(case):
+    # the coloured smoother's colours (red-black on a bipartite finest level, the
+    # greedy rounds over the uncoloured rows elsewhere): no entry joins two rows of
+    # one colour, every row has a colour, and the colours are 0..k-1
+    if case == "kat":
+        m = O.load_mm(os.path.join(G, "MatrixMarket226.dat"), os.path.join(G, "rhs226.dat"))
+    elif case == "grid32":
+        m = matgen.grid3d(32, seed=24, contrast=2.5)
+    else:
+        m = matgen.grid3d(14, seed=21, contrast=2.0)
+    ga = gpu_amg(gsOrdering=1)
+    ls = ls_from(m)
+    ga.setup(ls)
+    checked = 0
+    for l in range(ga.num_levels()):
+        colour, k = ga.level_colours(l)
+        if k == 0:
+            continue
+        g = ga.level(l)
+        assert colour.min() == 0 and colour.max() == k - 1
+        row, col = g["row"], g["col"]
+        rows = np.repeat(np.arange(g["n"]), np.diff(row))
+        keep = col < g["n"]   # entries to rows of this level (ghost columns have no colour)
+        assert not np.any(colour[rows[keep]] == colour[col[keep]]), "entry inside one colour on level %d" % l
+        checked += 1
+    assert checked > 0
+    if case.startswith("grid"):
+        assert ga.level_colours(0)[1] == 2   # red-black on the hexahedral finest level
+
+
 def test_opt_in_setup_reuse_gives_the_rebuilt_results():
     """reuse_setup 1: a setup of the same matrix (a new system object, even after
     cleanup, which the reference's models call after every solve) keeps the
